@@ -1,0 +1,257 @@
+/*
+ * thunderegg_amd.h -- C ABI of the MI355X (gfx950) patch-parallel GMG hot path.
+ *
+ * Drop-in boundary for ThunderEgg's plugin/operator API on this path.  The
+ * reference exposes C++ virtual interfaces only (no FFI); each entry point
+ * below names the reference member function it replaces (paths relative to
+ * the reference's src/ThunderEgg/).  The C++ wrappers in
+ * thunderegg_amd/include/ThunderEgg/ restore the reference's class names and
+ * method shapes on top of these functions.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative status; te_last_error()
+ *    returns a thread-local message for the last failure.  Status -3 means a
+ *    breakdown of an iterative solver (Iterative::BreakdownError).
+ *  - Vectors are device pointers in the reference layout (Vector.h:125-154):
+ *    patch-major, then component, then z, y, x, each axis padded by one ghost
+ *    cell on both sides; patches in Domain local-index order.
+ *  - dtype selects the element type: TE_F64 (double) or TE_F32 (float).
+ *  - All work is enqueued on `stream` (a hipStream_t; NULL = default stream)
+ *    and is asynchronous unless stated otherwise.  No entry point allocates
+ *    device memory except the *_create functions and te_device_alloc.
+ *  - One host thread per stream; objects are immutable after creation.
+ */
+#ifndef THUNDEREGG_AMD_H
+#define THUNDEREGG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TE_OK 0
+#define TE_ERR_ARG (-1)
+#define TE_ERR_HIP (-2)
+#define TE_ERR_BREAKDOWN (-3)
+#define TE_ERR_UNSUPPORTED (-4)
+
+enum te_dtype { TE_F64 = 0, TE_F32 = 1 };
+/* GhostFillingType.h: Faces, Edges, Corners */
+enum te_fill_type { TE_FACES = 0, TE_EDGES = 1, TE_CORNERS = 2 };
+enum te_filler_kind { TE_BILINEAR = 0, TE_BIQUADRATIC = 1, TE_TRILINEAR = 2 };
+/* NbrType.h:33-48 (-1 = no neighbour) */
+enum te_nbr_type { TE_NBR_NONE = -1, TE_NBR_NORMAL = 0, TE_NBR_COARSE = 1, TE_NBR_FINE = 2 };
+
+typedef void* te_stream; /* hipStream_t */
+
+/* One patch of a Domain: PatchInfo<D> (PatchInfo.h:52-463) with its
+ * Normal/Coarse/FineNbrInfo slots flattened.  Slot = Face<D,M>::sum_of_faces +
+ * face value: 2D corners 0-3, sides 4-7; 3D corners 0-7, edges 8-19, sides 20-25.
+ * Fine neighbours list 2^M ids in Orthant<M> order on the shared face. */
+typedef struct te_patch_info {
+  int32_t id, rank, parent_id, parent_rank;
+  int32_t orth_on_parent; /* -1 == Orthant<D>::null() */
+  int32_t refine_level;
+  int32_t child_ids[8];
+  int32_t child_ranks[8];
+  double starts[3];
+  double spacings[3];
+  int32_t nbr_type[26];
+  int32_t nbr_orth[26]; /* CoarseNbrInfo::orth_on_coarse, -1 otherwise */
+  int32_t nbr_ids[26][4];
+  int32_t nbr_ranks[26][4];
+} te_patch_info;
+
+typedef struct te_domain te_domain;
+typedef struct te_ghost_filler te_ghost_filler;
+typedef struct te_level_link te_level_link;
+typedef struct te_dft_plan te_dft_plan;
+typedef struct te_gmg te_gmg;
+typedef struct te_forest te_forest;
+
+/* ------------------------------------------------------------ runtime -- */
+const char* te_last_error(void);
+int te_device_count(int* out);
+int te_device_alloc(size_t bytes, void** out);
+int te_device_free(void* p);
+int te_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int te_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int te_memcpy_d2d(void* dst, const void* src, size_t bytes, te_stream stream);
+int te_memset(void* dst, int value, size_t bytes, te_stream stream);
+int te_stream_synchronize(te_stream stream);
+/* Create/destroy a stream at the default priority. */
+int te_stream_create(te_stream* out);
+int te_stream_destroy(te_stream s);
+
+/* ------------------------------------------------------ domain / tables --
+ * Domain<D>(comm, id, ns, num_ghost_cells, first, last) (Domain.h:223-248):
+ * local_index = position, neighbour local indexes resolved by id (Domain.h:94-109).
+ * One rank: every neighbour must be in the table.  num_ghost must be 1. */
+int te_domain_create(int dim, const int* ns, int num_ghost, int num_patches, const te_patch_info* patches,
+                     te_domain** out);
+int te_domain_destroy(te_domain* d);
+int te_domain_num_patches(const te_domain* d);
+/* Number of elements of a vector with ncomp components (padded layout). */
+int64_t te_domain_vector_length(const te_domain* d, int ncomp);
+/* Neighbour local-index table, [num_patches][26][4], -1 where empty. */
+int te_domain_get_nbr_local(const te_domain* d, int32_t* out);
+
+/* GMG::InterLevelComm (GMG/InterLevelComm.h:125-224), one rank: every fine patch
+ * has a local parent.  parent local index per fine patch. */
+int te_level_link_create(const te_domain* fine, const te_domain* coarse, te_level_link** out);
+int te_level_link_destroy(te_level_link* l);
+int te_level_link_get_parents(const te_level_link* l, int32_t* out);
+
+/* ------------------------------------------------------- ghost filling --
+ * BiLinearGhostFiller / BiQuadraticGhostFiller (2D), TriLinearGhostFiller (3D):
+ * constructors (BiLinearGhostFiller.cpp:24-27, TriLinearGhostFiller.cpp:573-585). */
+int te_ghost_filler_create(const te_domain* d, int kind, int fill_type, te_ghost_filler** out);
+int te_ghost_filler_destroy(te_ghost_filler* g);
+/* GhostFiller::fillGhost(const Vector&) (GhostFiller.h:53; MPIGhostFiller.h:1055-1092):
+ * ghost cells facing a neighbour are overwritten; interior cells are only read. */
+int te_ghost_fill(const te_ghost_filler* g, int dtype, void* u, int ncomp, te_stream stream);
+
+/* ----------------------------------------------------------- operators --
+ * PatchOperator::apply (PatchOperator.h:126-143) for Poisson::StarPatchOperator
+ * (Poisson/StarPatchOperator.h:75-101): fills ghosts of u, then f = A u.
+ * Physical-boundary ghost values are applied inline (-u Dirichlet, +u Neumann)
+ * and not stored back into u. */
+int te_star_apply(const te_ghost_filler* g, int dtype, int neumann, void* u, void* f, int ncomp,
+                  te_stream stream);
+/* VarPoisson::StarPatchOperator::apply (VarPoisson/StarPatchOperator.h:61-109); coeffs
+ * (same dtype, ncomp 1) must be ghost-filled once beforehand, as the reference's
+ * constructor does (:70). */
+int te_varstar_apply(const te_ghost_filler* g, int dtype, const void* coeffs, void* u, void* f, int ncomp,
+                     te_stream stream);
+/* Cycle::restrict residual (GMG/Cycle.h:57-65): fills ghosts of u, then r = f - A u.
+ * op_kind 0 Poisson (neumann flag), 1 VarPoisson (coeffs). */
+int te_residual(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs, const void* f,
+                void* u, void* r, int ncomp, te_stream stream);
+
+/* ------------------------------------------------------------ smoothers --
+ * Poisson::DFTPatchSolver (Poisson/DFTPatchSolver.h:369-450).  neumann_mask is the
+ * reference's std::bitset<Side<D>::number_of> (bit s = side s).  op_kind selects the
+ * modifyRHSForInternalBoundaryConditions of the wrapped operator: 0 Poisson, 1
+ * VarPoisson (coeffs: ghost-filled beta, float64, ncomp 1). */
+int te_dft_plan_create(const te_ghost_filler* g, unsigned neumann_mask, int op_kind, const double* coeffs,
+                       te_dft_plan** out);
+int te_dft_plan_destroy(te_dft_plan* p);
+/* PatchSolver::smooth (PatchSolver.h:142-170): fill ghosts of u, then per patch
+ * u = DFT solve of (f - interface terms from u's ghosts).  ncomp 1. */
+int te_dft_smooth(const te_dft_plan* p, int dtype, const void* f, void* u, te_stream stream);
+/* PatchSolver::apply (PatchSolver.h:98-134): u := 0, then per-patch solves. */
+int te_dft_apply(const te_dft_plan* p, int dtype, const void* f, void* u, te_stream stream);
+
+/* Iterative::PatchSolver<D>(BiCGStab, op) smooth (Iterative/PatchSolver.h:134-175):
+ * per patch, an inner BiCGStab (Iterative/BiCGStab.h:111-203) on the patch operator
+ * with internal boundary conditions, started from u.  iters (device int32[P], may be
+ * NULL) receives per-patch iteration counts.  With continue_on_breakdown == 0 a
+ * breakdown in any patch returns TE_ERR_BREAKDOWN (checked synchronously). */
+int te_patch_bicgstab_smooth(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs,
+                             int max_it, double tol, int continue_on_breakdown, const void* f, void* u,
+                             int32_t* iters, te_stream stream);
+
+/* --------------------------------------------------- inter-level transfer --
+ * MPIRestrictor::restrict + LinearRestrictor::restrictPatches (GMG/MPIRestrictor.h:55-83,
+ * GMG/LinearRestrictor.h:101-191), extrapolate_boundary_ghosts = false:
+ * coarse := 0 (with ghosts), then 2^D-cell averages / copies. */
+int te_restrict(const te_level_link* l, int dtype, const void* fine, void* coarse, int ncomp, te_stream stream);
+/* MPIInterpolator::interpolate + DirectInterpolator (GMG/MPIInterpolator.h:79-106,
+ * GMG/DirectInterpolator.h:60-91): fine += injection of coarse. */
+int te_interpolate(const te_level_link* l, int dtype, const void* coarse, void* fine, int ncomp,
+                   te_stream stream);
+
+/* -------------------------------------------------------- vector algebra --
+ * Vector<D> BLAS-1 (Vector.h:391-611), interior cells only. */
+int te_vec_set(const te_domain* d, int dtype, void* x, double alpha, int ncomp, te_stream s);
+int te_vec_set_with_ghost(const te_domain* d, int dtype, void* x, double alpha, int ncomp, te_stream s);
+int te_vec_scale(const te_domain* d, int dtype, void* x, double alpha, int ncomp, te_stream s);
+int te_vec_shift(const te_domain* d, int dtype, void* x, double delta, int ncomp, te_stream s);
+int te_vec_copy(const te_domain* d, int dtype, void* x, const void* b, int ncomp, te_stream s);
+int te_vec_add(const te_domain* d, int dtype, void* x, const void* b, int ncomp, te_stream s);
+/* x += alpha a */
+int te_vec_add_scaled(const te_domain* d, int dtype, void* x, double alpha, const void* a, int ncomp,
+                      te_stream s);
+/* x += alpha a + beta b */
+int te_vec_add_scaled2(const te_domain* d, int dtype, void* x, double alpha, const void* a, double beta,
+                       const void* b, int ncomp, te_stream s);
+/* x = alpha x + b */
+int te_vec_scale_then_add(const te_domain* d, int dtype, void* x, double alpha, const void* b, int ncomp,
+                          te_stream s);
+/* x = alpha x + beta b */
+int te_vec_scale_then_add_scaled(const te_domain* d, int dtype, void* x, double alpha, double beta,
+                                 const void* b, int ncomp, te_stream s);
+/* x = alpha x + beta b + gamma c */
+int te_vec_scale_then_add_scaled2(const te_domain* d, int dtype, void* x, double alpha, double beta,
+                                  const void* b, double gamma, const void* c, int ncomp, te_stream s);
+/* Reductions: synchronous, result in *out (Vector::dot/twoNorm/infNorm; infNorm is
+ * max(0, max v), Vector.h:568-579). */
+int te_vec_dot(const te_domain* d, int dtype, const void* a, const void* b, int ncomp, double* out,
+               te_stream s);
+int te_vec_two_norm(const te_domain* d, int dtype, const void* a, int ncomp, double* out, te_stream s);
+int te_vec_inf_norm(const te_domain* d, int dtype, const void* a, int ncomp, double* out, te_stream s);
+/* Element-type conversion of a whole padded vector (dst_dtype <- src_dtype). */
+int te_vec_convert(const te_domain* d, int dst_dtype, void* dst, int src_dtype, const void* src, int ncomp,
+                   te_stream s);
+
+/* ------------------------------------------------ multigrid + Krylov --
+ * GMG::CycleBuilder + VCycle (GMG/CycleBuilder.h:97-192, GMG/VCycle.h:46-70) over a
+ * chain of domains (finest first), with per-level device workspaces allocated here,
+ * once.  precision: TE_F64 = FP64 everywhere; TE_F32 = FP32 V-cycle (ghost fill,
+ * stencil, smoother, restrict/interpolate) with the coarsest-level solve in FP64. */
+typedef struct te_gmg_options {
+  int filler_kind;    /* te_filler_kind */
+  int fill_type;      /* te_fill_type */
+  int op_kind;        /* 0 Poisson star, 1 VarPoisson star */
+  int neumann;        /* Poisson star: Neumann physical boundaries */
+  int smoother;       /* 0 DFTPatchSolver, 1 Iterative::PatchSolver(BiCGStab) */
+  unsigned neumann_mask;
+  int inner_max_it;   /* inner BiCGStab (smoother 1) */
+  double inner_tol;
+  int continue_on_breakdown;
+  int pre_sweeps, post_sweeps, coarse_sweeps; /* CycleOpts (GMG/CycleOpts.h:33-55) */
+  int precision;      /* TE_F64 or TE_F32 */
+  int use_graph;      /* capture the V-cycle as a hipGraph */
+} te_gmg_options;
+
+/* coeffs: VarPoisson beta per level (device float64 pointers, ghost-filled), or NULL. */
+int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_options* opts,
+                  const double* const* coeffs, te_gmg** out);
+int te_gmg_destroy(te_gmg* g);
+/* Cycle::apply(f, u) (GMG/Cycle.h:94-98): u := V-cycle(f); float64 vectors of level 0. */
+int te_gmg_vcycle(te_gmg* g, const double* f, double* u, te_stream stream);
+/* Iterative::BiCGStab::solve(A, x, b, Mr) (Iterative/BiCGStab.h:111-203), A = level-0
+ * operator, Mr = this V-cycle (use_precond != 0) or none.  x holds the initial guess.
+ * history (host, may be NULL) receives the recursive relative residual per iteration
+ * (entry 0 = initial), up to history_cap entries.  *iterations = the reference's
+ * return value.  Returns TE_ERR_BREAKDOWN when rho == 0. */
+int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int max_it, double tol,
+                    int* iterations, double* history, int history_cap, te_stream stream);
+/* Seconds spent in the last te_gmg_bicgstab, from the first A x to convergence. */
+double te_gmg_last_solve_seconds(const te_gmg* g);
+
+/* ----------------------------------------------- mesh generation (host) --
+ * Quadtree / octree forest on the unit square / cube, replacing
+ * P4estDomainGenerator / P8estDomainGenerator (P4estDomainGenerator.cpp:205-738):
+ * Morton-ordered leaves, ids in leaf order, one-level-at-a-time coarsening of
+ * complete families, neighbour linking of faces, edges and corners. */
+int te_forest_create(int dim, int base_level, te_forest** out);
+int te_forest_destroy(te_forest* f);
+int te_forest_num_leaves(const te_forest* f);
+/* levels[i], coords[i*dim + a] (integer coordinates in units of the leaf size). */
+int te_forest_get_leaves(const te_forest* f, int32_t* levels, int32_t* coords);
+/* Refine every leaf with flags[i] != 0 into its 2^dim children. */
+int te_forest_refine(te_forest* f, const uint8_t* flags);
+/* 2:1 balance across faces, edges and corners (p4est_balance with CONNECT_FULL). */
+int te_forest_balance(te_forest* f);
+/* Domains finest (index 0) to coarsest: number of levels, and patch counts. */
+int te_forest_build_domains(te_forest* f, const int* ns, int* num_levels);
+int te_forest_domain_size(const te_forest* f, int level, int* num_patches);
+int te_forest_domain_patches(const te_forest* f, int level, te_patch_info* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

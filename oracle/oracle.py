@@ -1,0 +1,224 @@
+"""ctypes binding of the C oracle (oracle/te_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker / reported CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.ora_domain_create.restype = vp
+        L.ora_domain_create.argtypes = [C.c_int, ip, C.c_int, C.c_int, vp]
+        L.ora_domain_destroy.argtypes = [vp]
+        L.ora_domain_nbr_local.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.ora_domain_link_parents.argtypes = [vp, vp]
+        L.ora_domain_parent_local.argtypes = [vp, C.c_int]
+        L.ora_domain_vec_len.restype = C.c_long
+        L.ora_domain_vec_len.argtypes = [vp, C.c_int]
+        L.ora_enumerate_calls.argtypes = [vp, C.c_int, ip, C.c_int]
+        L.ora_fill_ghost.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int]
+        L.ora_poisson_apply.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int]
+        L.ora_varpoisson_apply.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int]
+        L.ora_dft_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, C.c_uint, dp, dp]
+        L.ora_bicgstab_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double,
+                                          C.c_int, dp, dp, C.c_int, ip]
+        L.ora_restrict.argtypes = [vp, vp, dp, dp, C.c_int]
+        L.ora_interpolate.argtypes = [vp, vp, dp, dp, C.c_int]
+        L.ora_vcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.ora_bicgstab_solve.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int,
+                                         C.c_double, dp, C.c_int]
+        for f in ("ora_vec_dot",):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [vp, dp, dp, C.c_int]
+        for f in ("ora_vec_two_norm", "ora_vec_inf_norm"):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [vp, dp, C.c_int]
+        L.ora_dft_matrix.argtypes = [C.c_int, C.c_int, dp]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class _Level(C.Structure):
+    _fields_ = [
+        ("dom", C.c_void_p),
+        ("filler_kind", C.c_int),
+        ("fill_type", C.c_int),
+        ("op_kind", C.c_int),
+        ("neumann", C.c_int),
+        ("coeffs", C.POINTER(C.c_double)),
+        ("smoother_kind", C.c_int),
+        ("neumann_mask", C.c_uint),
+        ("inner_max_it", C.c_int),
+        ("inner_tol", C.c_double),
+        ("continue_on_breakdown", C.c_int),
+    ]
+
+
+class Domain:
+    """One level of patches (reference ``Domain<D>``), oracle side."""
+
+    def __init__(self, dim, ns, patches: np.ndarray, num_ghost: int = 1):
+        self.dim = dim
+        self.ns = tuple(int(v) for v in ns)
+        self.patches = np.ascontiguousarray(patches)
+        self.np = len(patches)
+        nsa = (C.c_int * 3)(*(list(self.ns) + [1] * (3 - dim)))
+        self.h = lib().ora_domain_create(dim, nsa, num_ghost, self.np, self.patches.ctypes.data)
+        if not self.h:
+            raise ValueError("bad domain")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_domain_destroy(self.h)
+            self.h = None
+
+    def shape(self, ncomp=1):
+        return (self.np, ncomp) + tuple(self.ns[a] + 2 for a in reversed(range(self.dim)))
+
+    def zeros(self, ncomp=1):
+        return np.zeros(self.shape(ncomp))
+
+    def nbr_local(self, p, slot, k=0):
+        return lib().ora_domain_nbr_local(self.h, p, slot, k)
+
+    def link_parents(self, coarse: "Domain"):
+        rc = lib().ora_domain_link_parents(self.h, coarse.h)
+        if rc:
+            raise ValueError("fine patch without parent in coarse domain")
+
+    def parent_local(self, p):
+        return lib().ora_domain_parent_local(self.h, p)
+
+    def calls(self, fill_type=0):
+        n = lib().ora_enumerate_calls(self.h, fill_type, None, 0)
+        out = np.zeros((max(n, 1), 6), dtype=np.int32)
+        lib().ora_enumerate_calls(self.h, fill_type, out.ctypes.data_as(C.POINTER(C.c_int)), n)
+        return out[:n]
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed rc={rc}")
+
+
+def fill_ghost(dom: Domain, u: np.ndarray, kind=0, fill_type=0):
+    ncomp = u.shape[1]
+    _check(lib().ora_fill_ghost(dom.h, kind, fill_type, _dp(u), ncomp), "fill_ghost")
+
+
+def poisson_apply(dom: Domain, u, kind=0, fill_type=0, neumann=False):
+    f = np.zeros_like(u)
+    _check(lib().ora_poisson_apply(dom.h, kind, fill_type, int(neumann), _dp(u), _dp(f), u.shape[1]), "apply")
+    return f
+
+
+def varpoisson_apply(dom: Domain, coeffs, u, kind=0, fill_type=0):
+    f = np.zeros_like(u)
+    _check(lib().ora_varpoisson_apply(dom.h, kind, fill_type, _dp(coeffs), _dp(u), _dp(f), u.shape[1]),
+           "varpoisson apply")
+    return f
+
+
+def dft_smooth(dom: Domain, f, u, kind=0, fill_type=0, op_kind=0, coeffs=None, neumann_mask=0):
+    cp = _dp(coeffs) if coeffs is not None else None
+    _check(lib().ora_dft_smooth(dom.h, kind, fill_type, op_kind, cp, neumann_mask, _dp(f), _dp(u)), "dft_smooth")
+
+
+def bicgstab_smooth(dom: Domain, f, u, kind=0, fill_type=0, op_kind=0, coeffs=None, neumann=False, max_it=1000,
+                    tol=1e-12, continue_on_breakdown=False):
+    cp = _dp(coeffs) if coeffs is not None else None
+    iters = np.zeros(dom.np, dtype=np.int32)
+    _check(lib().ora_bicgstab_smooth(dom.h, kind, fill_type, op_kind, cp, int(neumann), max_it, tol,
+                                     int(continue_on_breakdown), _dp(f), _dp(u), u.shape[1],
+                                     iters.ctypes.data_as(C.POINTER(C.c_int))), "bicgstab_smooth")
+    return iters
+
+
+def restrict(fine: Domain, coarse: Domain, fv):
+    cv = coarse.zeros(fv.shape[1])
+    _check(lib().ora_restrict(fine.h, coarse.h, _dp(fv), _dp(cv), fv.shape[1]), "restrict")
+    return cv
+
+
+def interpolate(coarse: Domain, fine: Domain, cv, fv):
+    _check(lib().ora_interpolate(coarse.h, fine.h, _dp(cv), _dp(fv), fv.shape[1]), "interpolate")
+
+
+class Levels:
+    """GMG level chain (reference ``GMG::Level``), oracle side."""
+
+    def __init__(self, domains, filler_kind=0, fill_type=0, op_kind=0, neumann=False, coeffs=None,
+                 smoother_kind=0, neumann_mask=0, inner_max_it=1000, inner_tol=1e-12,
+                 continue_on_breakdown=False):
+        self.domains = domains
+        for a, b in zip(domains[:-1], domains[1:]):
+            a.link_parents(b)
+        self.coeffs = coeffs  # keep alive
+        arr = (_Level * len(domains))()
+        for i, d in enumerate(domains):
+            L = arr[i]
+            L.dom = d.h
+            L.filler_kind = filler_kind
+            L.fill_type = fill_type
+            L.op_kind = op_kind
+            L.neumann = int(neumann)
+            L.coeffs = _dp(coeffs[i]) if coeffs is not None else None
+            L.smoother_kind = smoother_kind
+            L.neumann_mask = neumann_mask
+            L.inner_max_it = inner_max_it
+            L.inner_tol = inner_tol
+            L.continue_on_breakdown = int(continue_on_breakdown)
+        self.arr = arr
+
+    def vcycle(self, f, pre=1, post=1, coarse=1):
+        u = np.zeros_like(f)
+        _check(lib().ora_vcycle_apply(C.cast(self.arr, C.c_void_p), len(self.domains), pre, post, coarse, _dp(f),
+                                      _dp(u)), "vcycle")
+        return u
+
+    def bicgstab(self, b, x=None, precond=True, pre=1, post=1, coarse=1, max_it=1000, tol=1e-10):
+        x = np.zeros_like(b) if x is None else x
+        hist = np.full(max_it + 2, np.nan)
+        its = lib().ora_bicgstab_solve(C.cast(self.arr, C.c_void_p), len(self.domains), int(precond), pre, post,
+                                       coarse, _dp(b), _dp(x), max_it, tol, _dp(hist), len(hist))
+        if its < 0:
+            raise RuntimeError(f"oracle BiCGStab failed ({its})")
+        return x, its, hist[: its + 1]
+
+
+def dot(dom, a, b):
+    return lib().ora_vec_dot(dom.h, _dp(a), _dp(b), a.shape[1])
+
+
+def two_norm(dom, a):
+    return lib().ora_vec_two_norm(dom.h, _dp(a), a.shape[1])
+
+
+def inf_norm(dom, a):
+    return lib().ora_vec_inf_norm(dom.h, _dp(a), a.shape[1])
+
+
+def dft_matrix(kind: int, n: int):
+    m = np.zeros(n * n)
+    lib().ora_dft_matrix(kind, n, _dp(m))
+    return m.reshape(n, n)

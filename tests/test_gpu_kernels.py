@@ -1,0 +1,220 @@
+"""GPU parity of the single-level kernels against the oracle, through the C ABI.
+
+FP64: max-norm relative difference <= 1e-12 (the hot path's FP64 parity bar).
+FP32: the oracle runs on the float-rounded inputs in FP64; max-norm relative
+difference <= 2e-6 (a few float ulps accumulated over the stencil / transforms).
+"""
+import numpy as np
+import pytest
+
+from helpers import MESH_2D, MESH_3D, face_ghost_mask, load, random_field, relerr
+from oracle import oracle as O
+from thunderegg_amd import mesh
+from thunderegg_amd import runtime as R
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-6
+
+
+def _cases_2d():
+    for name in MESH_2D:
+        for ns in [(8, 8), (16, 16), (10, 6)]:
+            yield name, ns
+
+
+def _dev(dim, ns, lv):
+    return R.Domain(dim, ns, lv)
+
+
+def _print_case(*a):
+    print("case:", *a, flush=True)
+
+
+@pytest.mark.parametrize("name,ns", list(_cases_2d()))
+@pytest.mark.parametrize("kind", [R.BILINEAR, R.BIQUADRATIC])
+@pytest.mark.parametrize("fill_type", [R.FACES, R.CORNERS])
+def test_ghost_fill_2d(name, ns, kind, fill_type):
+    _print_case("ghost2d", name, ns, kind, fill_type)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    od = O.Domain(dim, ns, lv)
+    dd = _dev(dim, ns, lv)
+    g = R.GhostFiller(dd, kind, fill_type)
+    for dtype, tol in [(R.F64, TOL64), (R.F32, TOL32)]:
+        for ncomp in (1, 2):
+            u0 = random_field(od.shape(ncomp), 11 + ncomp, R.np_dtype(dtype))
+            ref = u0.astype(np.float64)
+            O.fill_ghost(od, ref, kind=kind, fill_type=fill_type)
+            du = R.DeviceArray.from_host(u0)
+            g.fill_ghost(du)
+            got = du.get()
+            assert relerr(got, ref) <= tol, (dtype, ncomp)
+            # interior cells are untouched
+            assert np.array_equal(mesh.interior(got, dim), mesh.interior(u0, dim))
+
+
+@pytest.mark.parametrize("name", MESH_3D)
+@pytest.mark.parametrize("ns", [(4, 4, 4), (6, 4, 8)])
+@pytest.mark.parametrize("fill_type", [R.FACES, R.EDGES, R.CORNERS])
+def test_ghost_fill_3d(name, ns, fill_type):
+    _print_case("ghost3d", name, ns, fill_type)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    od = O.Domain(dim, ns, lv)
+    dd = _dev(dim, ns, lv)
+    g = R.GhostFiller(dd, R.TRILINEAR, fill_type)
+    for dtype, tol in [(R.F64, TOL64), (R.F32, TOL32)]:
+        u0 = random_field(od.shape(1), 5, R.np_dtype(dtype))
+        ref = u0.astype(np.float64)
+        O.fill_ghost(od, ref, kind=2, fill_type=fill_type)
+        du = R.DeviceArray.from_host(u0)
+        g.fill_ghost(du)
+        got = du.get()
+        assert relerr(got, ref) <= tol
+        assert face_ghost_mask(lv, dim, ns, fill_type).sum() > 0
+
+
+@pytest.mark.parametrize("name,ns", [(n, s) for n, s in _cases_2d() if s != (10, 6)] +
+                         [(n, (4, 4, 4)) for n in MESH_3D])
+@pytest.mark.parametrize("neumann", [False, True])
+def test_star_apply(name, ns, neumann):
+    _print_case("star", name, ns, neumann)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    kind = R.TRILINEAR if dim == 3 else R.BILINEAR
+    od = O.Domain(dim, ns, lv)
+    dd = _dev(dim, ns, lv)
+    g = R.GhostFiller(dd, kind, R.FACES)
+    for dtype, tol in [(R.F64, TOL64), (R.F32, 5e-6)]:
+        u0 = random_field(od.shape(1), 3, R.np_dtype(dtype))
+        ref = O.poisson_apply(od, u0.astype(np.float64), kind=kind, neumann=neumann)
+        du = R.DeviceArray.from_host(u0)
+        df = dd.vector(1, dtype)
+        g.star_apply(du, df, neumann=neumann)
+        got = df.get()
+        assert relerr(got, ref) <= tol, dtype
+        # residual r = f - A u
+        f0 = random_field(od.shape(1), 4, R.np_dtype(dtype))
+        dfin = R.DeviceArray.from_host(f0)
+        dr = dd.vector(1, dtype)
+        g.residual(dfin, du, dr, op_kind=0, neumann=neumann)
+        rref = np.zeros_like(ref)
+        mesh.interior(rref, dim)[...] = mesh.interior(f0.astype(np.float64) - ref, dim)
+        assert relerr(dr.get(), rref) <= tol
+
+
+@pytest.mark.parametrize("name,ns", [("2d_uniform_2x2_refined_nw_mpi1.json", (8, 8)),
+                                     ("2d_uniform_8x8_refined_cross_mpi1.json", (16, 16)),
+                                     ("3d_refined_bnw_2x2x2_mpi1.json", (4, 4, 4))])
+def test_varpoisson_apply(name, ns):
+    _print_case("varpoisson", name, ns)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    kind = R.TRILINEAR if dim == 3 else R.BILINEAR
+    od = O.Domain(dim, ns, lv)
+    dd = _dev(dim, ns, lv)
+    g = R.GhostFiller(dd, kind, R.FACES)
+    X = mesh.cell_centers(lv, dim, ns)
+    beta = np.exp(np.sin(3 * X[..., 0]) * np.cos(2 * X[..., 1]))[:, None]
+    O.fill_ghost(od, beta, kind=kind)
+    for dtype, tol in [(R.F64, TOL64), (R.F32, 5e-6)]:
+        u0 = random_field(od.shape(1), 8, R.np_dtype(dtype))
+        b = beta.astype(R.np_dtype(dtype))
+        ref = O.varpoisson_apply(od, b.astype(np.float64), u0.astype(np.float64), kind=kind)
+        du = R.DeviceArray.from_host(u0)
+        dc = R.DeviceArray.from_host(b)
+        df = dd.vector(1, dtype)
+        g.varstar_apply(dc, du, df)
+        assert relerr(df.get(), ref) <= tol
+
+
+@pytest.mark.parametrize("name,ns", [("2d_uniform_2x2_refined_nw_mpi1.json", (8, 8)),
+                                     ("2d_uniform_8x8_refined_cross_mpi1.json", (16, 16)),
+                                     ("2d_uniform_8x8_refined_cross_mpi1.json", (6, 10)),
+                                     ("3d_mid_refine_4x4x4_mpi1.json", (4, 4, 4)),
+                                     ("3d_refined_bnw_2x2x2_mpi1.json", (6, 4, 4))])
+def test_restrict_interpolate(name, ns):
+    _print_case("transfer", name, ns)
+    dim, levels = load(name, ns)
+    of, oc = O.Domain(dim, ns, levels[0]), O.Domain(dim, ns, levels[1])
+    of.link_parents(oc)
+    df, dc = _dev(dim, ns, levels[0]), _dev(dim, ns, levels[1])
+    link = R.LevelLink(df, dc)
+    assert np.array_equal(link.parents(), [of.parent_local(p) for p in range(of.np)])
+    for dtype, tol in [(R.F64, TOL64), (R.F32, TOL32)]:
+        for ncomp in (1, 2):
+            fv = random_field(of.shape(ncomp), 21, R.np_dtype(dtype))
+            cref = O.restrict(of, oc, fv.astype(np.float64))
+            dfv = R.DeviceArray.from_host(fv)
+            dcv = R.DeviceArray.from_host(random_field(oc.shape(ncomp), 1, R.np_dtype(dtype)))
+            link.restrict(dfv, dcv)
+            assert relerr(dcv.get(), cref) <= tol
+            cv = random_field(oc.shape(ncomp), 22, R.np_dtype(dtype))
+            fref = fv.astype(np.float64).copy()
+            O.interpolate(oc, of, cv.astype(np.float64), fref)
+            dcv.upload(cv)
+            link.interpolate(dcv, dfv)
+            assert relerr(dfv.get(), fref) <= tol
+
+
+@pytest.mark.parametrize("name,ns", [("2d_uniform_2x2_mpi1.json", (8, 8)),
+                                     ("2d_uniform_2x2_refined_nw_mpi1.json", (16, 16)),
+                                     ("2d_uniform_8x8_refined_cross_mpi1.json", (32, 32)),
+                                     ("2d_uniform_8x8_refined_cross_mpi1.json", (10, 6)),
+                                     ("3d_uniform_2x2x2_mpi1.json", (8, 8, 8)),
+                                     ("3d_refined_bnw_2x2x2_mpi1.json", (16, 16, 16))])
+@pytest.mark.parametrize("neumann_mask", [0, 0b0101])
+def test_dft_smooth(name, ns, neumann_mask):
+    _print_case("dft", name, ns, neumann_mask)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    kind = R.TRILINEAR if dim == 3 else R.BIQUADRATIC
+    od = O.Domain(dim, ns, lv)
+    dd = _dev(dim, ns, lv)
+    g = R.GhostFiller(dd, kind, R.FACES)
+    plan = R.DftPlan(g, neumann_mask=neumann_mask)
+    for dtype, tol in [(R.F64, TOL64), (R.F32, 2e-5)]:
+        f0 = random_field(od.shape(1), 31, R.np_dtype(dtype))
+        u0 = random_field(od.shape(1), 32, R.np_dtype(dtype))
+        uref = u0.astype(np.float64)
+        O.dft_smooth(od, f0.astype(np.float64), uref, kind=kind, neumann_mask=neumann_mask)
+        du = R.DeviceArray.from_host(u0)
+        plan.smooth(R.DeviceArray.from_host(f0), du)
+        got = du.get()
+        assert relerr(mesh.interior(got, dim), mesh.interior(uref, dim)) <= tol, dtype
+
+
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_vector_algebra(dtype):
+    _print_case("vector", dtype)
+    dim, levels = load("2d_uniform_8x8_refined_cross_mpi1.json", (16, 16))
+    lv = levels[0]
+    od = O.Domain(dim, (16, 16), lv)
+    dd = _dev(dim, (16, 16), lv)
+    nt = R.np_dtype(dtype)
+    tol = TOL64 if dtype == R.F64 else 1e-6
+    a = random_field(od.shape(2), 1, nt)
+    b = random_field(od.shape(2), 2, nt)
+    da, db = R.DeviceArray.from_host(a), R.DeviceArray.from_host(b)
+    ia, ib = mesh.interior(a, 2).astype(np.float64), mesh.interior(b, 2).astype(np.float64)
+    assert abs(dd.dot(da, db) - (ia * ib).sum()) <= tol * np.abs(ia * ib).sum()
+    assert abs(dd.two_norm(da) - np.sqrt((ia * ia).sum())) <= tol * np.sqrt((ia * ia).sum())
+    assert dd.inf_norm(da) == pytest.approx(max(ia.max(), 0.0), rel=1e-15)
+    neg = R.DeviceArray.from_host(-np.abs(a))
+    assert dd.inf_norm(neg) == 0.0  # Vector::infNorm is max(0, max v)
+    L = R.lib()
+    x = da
+    R.check(L.te_vec_add_scaled2(dd.h, dtype, x.ptr, 0.5, db.ptr, -2.0, db.ptr, 2, None))
+    exp = a.astype(np.float64).copy()
+    mesh.interior(exp, 2)[...] += ib * 0.5 + ib * -2.0
+    assert relerr(x.get(), exp) <= tol
+    R.check(L.te_vec_scale_then_add(dd.h, dtype, x.ptr, 3.0, db.ptr, 2, None))
+    mesh.interior(exp, 2)[...] = mesh.interior(exp, 2) * 3.0 + ib
+    assert relerr(x.get(), exp) <= tol
+    R.check(L.te_vec_set_with_ghost(dd.h, dtype, x.ptr, 1.5, 2, None))
+    assert np.all(x.get() == 1.5)
+    R.check(L.te_vec_set(dd.h, dtype, x.ptr, 0.0, 2, None))
+    got = x.get()
+    assert np.all(mesh.interior(got, 2) == 0.0) and got.max() == 1.5

@@ -1,0 +1,27 @@
+// DFT patch-solver plan shared by the host code and the kernels.
+#pragma once
+
+#include "../te_internal.h"
+
+namespace te {
+constexpr int kDftMatDim = 64;                      // largest supported n
+constexpr int kDftMatStride = kDftMatDim * kDftMatDim;
+}  // namespace te
+
+struct te_dft_plan {
+  const te_ghost_filler* g = nullptr;
+  unsigned mask = 0;
+  int op_kind = 0;
+  int* d_ptype = nullptr;       // per patch: fwd type per axis [0..2], eig type [3..5], zero mode [6]
+  double* d_mats64 = nullptr;   // [6 types][3 axes][kDftMatStride]
+  float* d_mats32 = nullptr;
+  double* d_s2_64 = nullptr;    // [3 eig types][3 axes][kDftMatDim]
+  float* d_s2_32 = nullptr;
+  const double* d_coeffs64 = nullptr;  // VarPoisson beta (caller-owned)
+  float* d_coeffs32 = nullptr;         // plan-owned float copy
+};
+
+namespace te {
+int launch_dft(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out, bool uzero,
+               hipStream_t s);
+}  // namespace te

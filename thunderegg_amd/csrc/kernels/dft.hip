@@ -1,0 +1,188 @@
+// Poisson::DFTPatchSolver::solveSinglePatch (Poisson/DFTPatchSolver.h:420-450) for
+// every patch of a level in one launch, one workgroup per patch, generic
+// (VALU) form for any n.  The patch is staged in LDS; each axis transform
+// y = M x (executePlan, :205-265) is a dense n x n product over all lines of
+// that axis; eigenvalues (:320-361) come from a per-axis sin^2 table.
+//
+// Template parameters: T = arithmetic / u type, TF = type of f, TO = type of
+// the output.  UZERO: u is known to be identically zero (the first smoothing
+// sweep of a freshly zeroed level), so the interface terms of
+// modifyRHSForInternalBoundaryConditions vanish and u is not read.
+#include "../te_internal.h"
+#include "dft.h"
+
+namespace te {
+namespace {
+
+template <typename T, typename TF, typename TO, int D, int OP, bool UZERO>
+__global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ dp, const int* __restrict__ ptype,
+                                                  const T* __restrict__ mats, const T* __restrict__ s2tab,
+                                                  const TF* __restrict__ f, const T* __restrict__ u,
+                                                  const T* __restrict__ coeffs, TO* __restrict__ out, Layout L)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int p = blockIdx.x;
+  const int n0 = L.n[0], n1 = L.n[1], n2 = D == 3 ? L.n[2] : 1;
+  const int N = n0 * n1 * n2;
+  const int nmax = n0 > n1 ? (n0 > n2 ? n0 : n2) : (n1 > n2 ? n1 : n2);
+  T* A = sm;
+  T* B = sm + N;
+  T* MF = sm + 2 * N;                 // forward matrices, D x nmax^2
+  T* MI = MF + D * nmax * nmax;       // inverse matrices
+  const int* pt = ptype + p * 8;      // fwd type per axis [0..2], eig type per axis [3..5], zero mode [6]
+  const DevPatch& P = dp[p];
+  const int nn[3] = { n0, n1, n2 };
+
+  for (int a = 0; a < D; a++) {
+    int na = nn[a];
+    int tf = pt[a];
+    int ti = tf == 0 ? 1 : (tf == 3 ? 4 : tf);  // DCT_II->DCT_III, DST_II->DST_III, IV types self-inverse
+    for (int k = threadIdx.x; k < na * na; k += blockDim.x) {
+      MF[a * nmax * nmax + k] = mats[(tf * 3 + a) * kDftMatStride + k];
+      MI[a * nmax * nmax + k] = mats[(ti * 3 + a) * kDftMatStride + k];
+    }
+  }
+  // f_copy = f, then modifyRHSForInternalBoundaryConditions (Poisson :165-180, VarPoisson :168-186)
+  for (int o = threadIdx.x; o < N; o += blockDim.x) {
+    int x = o % n0;
+    int y = (o / n0) % n1;
+    int z = D == 3 ? o / (n0 * n1) : 0;
+    int gi = lin(L, p, 0, x, y, z);
+    T v = T(f[gi]);
+    if (!UZERO) {
+      const int c3[3] = { x, y, z };
+      const int st[3] = { 1, L.sx, L.sy };
+      const int sbase = D == 2 ? 4 : 20;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+#pragma unroll
+        for (int up = 0; up < 2; up++) {
+          bool edge = up ? (c3[a] == nn[a] - 1) : (c3[a] == 0);
+          if (!edge || P.type[sbase + 2 * a + up] < 0) continue;
+          int gidx = up ? gi + st[a] : gi - st[a];
+          T ih2 = T(P.inv_h2[a]);
+          if (OP == 0) {
+            v -= (u[gidx] + u[gi]) * ih2;
+          } else {
+            int cq = gi - p * L.ps;
+            int cg = up ? cq + st[a] : cq - st[a];
+            T cb = coeffs[p * L.cs + cq], cgv = coeffs[p * L.cs + cg];
+            v -= (u[gidx] + u[gi]) * (cb + cgv) * (ih2 * T(0.5));
+          }
+        }
+      }
+    }
+    A[o] = v;
+  }
+  __syncthreads();
+  // forward transform along each axis (DST-II / DCT-II / DCT-IV / DST-IV)
+  T* src = A;
+  T* dst = B;
+  for (int a = 0; a < D; a++) {
+    int na = nn[a];
+    int sta = a == 0 ? 1 : (a == 1 ? n0 : n0 * n1);
+    const T* M = MF + a * nmax * nmax;
+    for (int o = threadIdx.x; o < N; o += blockDim.x) {
+      int k = (o / sta) % na;
+      int base = o - k * sta;
+      T acc = T(0);
+      for (int j = 0; j < na; j++) acc += M[k * na + j] * src[base + j * sta];
+      dst[o] = acc;
+    }
+    __syncthreads();
+    T* t = src;
+    src = dst;
+    dst = t;
+  }
+  // divide by the eigenvalues
+  for (int o = threadIdx.x; o < N; o += blockDim.x) {
+    int k3[3] = { o % n0, (o / n0) % n1, D == 3 ? o / (n0 * n1) : 0 };
+    T eig = T(0);
+    for (int a = 0; a < D; a++) {
+      double h = P.h[a];
+      eig -= T(4.0 / (h * h)) * s2tab[(pt[3 + a] * 3 + a) * kDftMatDim + k3[a]];
+    }
+    src[o] /= eig;
+  }
+  __syncthreads();
+  if (pt[6] && threadIdx.x == 0) src[0] = T(0);  // all-Neumann patch without neighbours (:438-440)
+  __syncthreads();
+  for (int a = 0; a < D; a++) {
+    int na = nn[a];
+    int sta = a == 0 ? 1 : (a == 1 ? n0 : n0 * n1);
+    const T* M = MI + a * nmax * nmax;
+    for (int o = threadIdx.x; o < N; o += blockDim.x) {
+      int k = (o / sta) % na;
+      int base = o - k * sta;
+      T acc = T(0);
+      for (int j = 0; j < na; j++) acc += M[k * na + j] * src[base + j * sta];
+      dst[o] = acc;
+    }
+    __syncthreads();
+    T* t = src;
+    src = dst;
+    dst = t;
+  }
+  T scale = T(1);
+  for (int a = 0; a < D; a++) scale *= T(2.0) / T(nn[a]);
+  for (int o = threadIdx.x; o < N; o += blockDim.x) {
+    int x = o % n0;
+    int y = (o / n0) % n1;
+    int z = D == 3 ? o / (n0 * n1) : 0;
+    out[lin(L, p, 0, x, y, z)] = TO(src[o] * scale);
+  }
+}
+
+template <typename T, typename TF, typename TO, int D>
+void launch_d(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzero, hipStream_t s)
+{
+  const te_domain* d = pl->g->d;
+  Layout L = make_layout(d->dim, d->ns, 1, d->np);
+  int nmax = 0;
+  int N = 1;
+  for (int a = 0; a < D; a++) {
+    nmax = d->ns[a] > nmax ? d->ns[a] : nmax;
+    N *= d->ns[a];
+  }
+  size_t shmem = sizeof(T) * (2 * (size_t)N + 2 * (size_t)D * nmax * nmax);
+  const T* mats = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32);
+  const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
+  const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);
+#define TE_DL(OP_, UZ_)                                                                                      \
+  hipLaunchKernelGGL((dft_kernel<T, TF, TO, D, OP_, UZ_>), dim3(d->np), dim3(256), shmem, s, d->d_patches, \
+                     pl->d_ptype, mats, s2, f, u, cf, out, L)
+  if (pl->op_kind == 0) {
+    if (uzero) TE_DL(0, true); else TE_DL(0, false);
+  } else {
+    if (uzero) TE_DL(1, true); else TE_DL(1, false);
+  }
+#undef TE_DL
+}
+
+template <typename T, typename TF, typename TO>
+void launch_t(const te_dft_plan* pl, const void* f, const void* u, void* out, bool uzero, hipStream_t s)
+{
+  if (pl->g->d->dim == 2) launch_d<T, TF, TO, 2>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, s);
+  else launch_d<T, TF, TO, 3>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, s);
+}
+
+}  // namespace
+
+int launch_dft(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out, bool uzero,
+               hipStream_t s)
+{
+  if (pl->g->d->np == 0) return TE_OK;
+  if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64) launch_t<double, double, double>(pl, f, u, out, uzero, s);
+  else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32) launch_t<float, float, float>(pl, f, u, out, uzero, s);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32) launch_t<float, double, float>(pl, f, u, out, uzero, s);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F64) launch_t<float, double, double>(pl, f, u, out, uzero, s);
+  else {
+    set_error("unsupported dtype combination for the DFT patch solver");
+    return TE_ERR_UNSUPPORTED;
+  }
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+}  // namespace te
