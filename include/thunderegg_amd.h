@@ -223,7 +223,11 @@ int te_gmg_destroy(te_gmg* g);
 /* Cycle::apply(f, u) (GMG/Cycle.h:94-98): u := V-cycle(f); float64 vectors of level 0. */
 int te_gmg_vcycle(te_gmg* g, const double* f, double* u, te_stream stream);
 /* Iterative::BiCGStab::solve(A, x, b, Mr) (Iterative/BiCGStab.h:111-203), A = level-0
- * operator, Mr = this V-cycle (use_precond != 0) or none.  x holds the initial guess.
+ * operator, Mr = this V-cycle (use_precond 1) or none (0).  x holds the initial guess.
+ * The reference's early exit (||s|| small) updates x += alpha p even with a
+ * preconditioner (BiCGStab.h:162-167); use_precond 2 runs the V-cycle-preconditioned
+ * solve with that exit corrected to x += alpha M p, which is what reaches the
+ * tolerance in the true residual.
  * history (host, may be NULL) receives the recursive relative residual per iteration
  * (entry 0 = initial), up to history_cap entries.  *iterations = the reference's
  * return value.  Returns TE_ERR_BREAKDOWN when rho == 0. */

@@ -197,6 +197,7 @@ class Levels:
         return u
 
     def bicgstab(self, b, x=None, precond=True, pre=1, post=1, coarse=1, max_it=1000, tol=1e-10):
+        """precond: False, True (the reference's solve) or 2 (early exit with alpha M p)."""
         x = np.zeros_like(b) if x is None else x
         hist = np.full(max_it + 2, np.nan)
         its = lib().ora_bicgstab_solve(C.cast(self.arr, C.c_void_p), len(self.domains), int(precond), pre, post,

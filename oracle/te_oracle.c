@@ -897,7 +897,7 @@ double ora_vec_inf_norm(const ora_domain* d, const double* a, int ncomp)
 typedef int (*apply_fn)(void* ctx, double* in, double* out);
 
 static int bicgstab(const vspace* V, apply_fn A, void* actx, double* x, const double* b, apply_fn Mr,
-                    void* mctx, int max_it, double tol, double* hist, int hist_cap, int* breakdown)
+                    void* mctx, int max_it, double tol, double* hist, int hist_cap, int* breakdown, int exit_mp)
 {
   long len = vs_len(V);
   double* resid = (double*)calloc(len, sizeof(double));
@@ -934,7 +934,9 @@ static int bicgstab(const vspace* V, apply_fn A, void* actx, double* x, const do
     FOR_INTERIOR(V, i) s[i] = resid[i];
     FOR_INTERIOR(V, i) s[i] += ap[i] * -alpha;
     if (vs_norm(V, s) / r0_norm <= tol) {
-      FOR_INTERIOR(V, i) x[i] += p[i] * alpha; /* quirk: p, not M p (BiCGStab.h:162-167) */
+      /* the reference adds alpha p, not alpha M p (BiCGStab.h:162-167); exit_mp selects M p */
+      const double* dirp = (exit_mp && Mr) ? mp : p;
+      FOR_INTERIOR(V, i) x[i] += dirp[i] * alpha;
       break;
     }
     if (Mr) {
@@ -1158,7 +1160,7 @@ static int bicgstab_smooth(const opctx* o, int max_it, double tol, int cont, con
     modify_rhs(o, p, u, fcopy, nc, p, 0);
     spctx sc = { o, p, nc };
     int bd = 0;
-    int its = bicgstab(&V1, single_patch_apply, &sc, u + p * plen, fcopy, NULL, NULL, max_it, tol, NULL, 0, &bd);
+    int its = bicgstab(&V1, single_patch_apply, &sc, u + p * plen, fcopy, NULL, NULL, max_it, tol, NULL, 0, &bd, 0);
     if (bd && !cont) {
       free(fcopy);
       return -10;
@@ -1314,7 +1316,7 @@ int ora_bicgstab_solve(const ora_level* levels, int nlevels, int use_precond, in
   vcctx c = { levels, nlevels, pre, post, coarse, 0 };
   int bd = 0;
   int its = bicgstab(&V, level_apply_cb, (void*)&levels[0], x, b, use_precond ? vcycle_cb : NULL, &c, max_it, tol,
-                     hist, hist_cap, &bd);
+                     hist, hist_cap, &bd, use_precond == 2);
   if (bd) return -1;
   if (c.err) return -2;
   return its;

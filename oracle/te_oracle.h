@@ -105,7 +105,8 @@ int ora_vcycle_apply(const ora_level* levels, int nlevels, int pre, int post, in
                      const double* f, double* u);
 
 /* Iterative::BiCGStab::solve (Iterative/BiCGStab.h:111-203) with A = level 0's operator
- * and Mr = the V-cycle over `levels` (or none when use_precond == 0).
+ * and Mr = the V-cycle over `levels` (use_precond 1), or none (0).  use_precond 2 is the
+ * V-cycle with the early exit updating x by alpha M p instead of the reference's alpha p.
  * Residual history (recursive ||r||/||b||, entry 0 = initial) written to hist.
  * Returns iterations, or -1 on breakdown (rho == 0). */
 int ora_bicgstab_solve(const ora_level* levels, int nlevels, int use_precond, int pre, int post,

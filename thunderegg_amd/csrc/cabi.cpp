@@ -11,6 +11,7 @@
 
 #include "kernels/blas.h"
 #include "kernels/dft.h"
+#include "kernels/krylov.h"
 #include "te_internal.h"
 
 namespace te {
@@ -169,6 +170,7 @@ int te_domain_create(int dim, const int* ns, int num_ghost, int num_patches, con
   }
   if (e == hipSuccess) e = hipMalloc(&d->d_scratch, sizeof(double) * kMaxPartials);
   if (e == hipSuccess) e = hipMalloc(&d->d_result, sizeof(double) * 16);
+  if (e == hipSuccess) e = hipMalloc(&d->d_flags, sizeof(int32_t) * 4);
   if (e != hipSuccess) {
     set_error(std::string("te_domain_create: ") + hipGetErrorString(e));
     te_domain_destroy(d);
@@ -184,6 +186,7 @@ int te_domain_destroy(te_domain* d)
   if (d->d_patches) (void)hipFree(d->d_patches);
   if (d->d_scratch) (void)hipFree(d->d_scratch);
   if (d->d_result) (void)hipFree(d->d_result);
+  if (d->d_flags) (void)hipFree(d->d_flags);
   delete d;
   return TE_OK;
 }
@@ -309,7 +312,7 @@ int te_star_apply(const te_ghost_filler* g, int dtype, int neumann, void* u, voi
   TE_ARG(u != f, "u and f must be distinct vectors");
   int rc = launch_ghost_fill(g, dtype, u, ncomp, S(s));
   if (rc) return rc;
-  return launch_star(g, dtype, 0, neumann, nullptr, nullptr, u, f, ncomp, S(s));
+  return launch_star(g, dtype, dtype, dtype, 0, neumann, nullptr, nullptr, u, f, ncomp, S(s));
 }
 
 int te_varstar_apply(const te_ghost_filler* g, int dtype, const void* coeffs, void* u, void* f, int ncomp,
@@ -319,7 +322,7 @@ int te_varstar_apply(const te_ghost_filler* g, int dtype, const void* coeffs, vo
   TE_ARG(u != f, "u and f must be distinct vectors");
   int rc = launch_ghost_fill(g, dtype, u, ncomp, S(s));
   if (rc) return rc;
-  return launch_star(g, dtype, 1, 0, coeffs, nullptr, u, f, ncomp, S(s));
+  return launch_star(g, dtype, dtype, dtype, 1, 0, coeffs, nullptr, u, f, ncomp, S(s));
 }
 
 int te_residual(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs, const void* f,
@@ -330,7 +333,7 @@ int te_residual(const te_ghost_filler* g, int dtype, int op_kind, int neumann, c
   TE_ARG(r != u, "r and u must be distinct vectors");
   int rc = launch_ghost_fill(g, dtype, u, ncomp, S(s));
   if (rc) return rc;
-  return launch_star(g, dtype, op_kind, neumann, coeffs, f, u, r, ncomp, S(s));
+  return launch_star(g, dtype, dtype, dtype, op_kind, neumann, coeffs, f, u, r, ncomp, S(s));
 }
 
 // ------------------------------------------------------------------- DFT ----
@@ -471,6 +474,30 @@ int te_dft_apply(const te_dft_plan* p, int dtype, const void* f, void* u, te_str
   int rc = launch_vec_op(p->g->d, dtype, VOP_SET_WITH_GHOST, u, nullptr, nullptr, 0.0, 0, 0, 1, S(s));
   if (rc) return rc;
   return launch_dft(p, dtype, dtype, dtype, f, u, u, true, S(s));
+}
+
+int te_patch_bicgstab_smooth(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs,
+                             int max_it, double tol, int continue_on_breakdown, const void* f, void* u,
+                             int32_t* iters, te_stream s)
+{
+  TE_ARG(g && f && u, "null argument");
+  TE_ARG(dtype == TE_F64 || dtype == TE_F32, "bad dtype");
+  TE_ARG(op_kind == 0 || coeffs, "VarPoisson needs coefficients");
+  TE_ARG(max_it >= 0, "bad iteration limit");
+  int rc = launch_ghost_fill(g, dtype, u, 1, S(s));
+  if (rc) return rc;
+  int32_t* flag = g->d->d_flags;
+  TE_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), S(s)));
+  rc = launch_patch_bicgstab(g->d, dtype, op_kind, neumann, coeffs, f, u, max_it, tol, iters, flag, S(s));
+  if (rc) return rc;
+  int32_t h = 0;
+  TE_HIP(hipMemcpyAsync(&h, flag, sizeof(int32_t), hipMemcpyDeviceToHost, S(s)));
+  TE_HIP(hipStreamSynchronize(S(s)));
+  if (h && !continue_on_breakdown) {
+    set_error("BiCGStab broke down in a patch solve (rho was 0)");
+    return TE_ERR_BREAKDOWN;
+  }
+  return TE_OK;
 }
 
 // ---------------------------------------------------------------- transfer --

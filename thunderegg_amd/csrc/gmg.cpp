@@ -1,0 +1,472 @@
+// Geometric multigrid V-cycle and the outer BiCGStab on the device.
+//
+//  - GMG::CycleBuilder / Level / VCycle (GMG/CycleBuilder.h:97-192, GMG/Level.h,
+//    GMG/VCycle.h:46-70) with Cycle::restrict (GMG/Cycle.h:57-65): every level owns
+//    its ghost filler, smoother plan, link to the coarser level and preallocated
+//    f / u / r vectors, so applying the cycle allocates nothing and the whole
+//    cycle can be captured as one hipGraph.
+//  - Mixed precision: the V-cycle runs in FP32 (ghost fill, stencil, smoother,
+//    restrict, interpolate) except the coarsest level, which stays FP64; the
+//    finest level reads the FP64 right-hand side and its last smoothing sweep
+//    writes the FP64 result directly.
+//  - Iterative::BiCGStab::solve (Iterative/BiCGStab.h:111-203) in FP64 with the
+//    scalars on the device; the host reads ||s|| and ||r|| once each per iteration.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels/blas.h"
+#include "kernels/dft.h"
+#include "kernels/krylov.h"
+#include "te_internal.h"
+
+using namespace te;
+
+namespace {
+
+struct GLevel {
+  const te_domain* d = nullptr;
+  te_ghost_filler* gf = nullptr;
+  te_dft_plan* plan = nullptr;
+  te_level_link* link = nullptr;  // this level -> next coarser level
+  int dt = TE_F64;
+  void* f = nullptr;
+  void* u = nullptr;
+  void* r = nullptr;
+  const void* coeffs = nullptr;      // in this level's dtype
+  const double* coeffs64 = nullptr;  // caller's FP64 coefficients
+  float* coeffs32 = nullptr;         // owned when dt == F32 and VarPoisson
+};
+
+size_t esize(int dt) { return dt == TE_F64 ? 8 : 4; }
+
+}  // namespace
+
+struct te_gmg {
+  te_gmg_options o{};
+  std::vector<GLevel> L;
+  hipStream_t s = nullptr;
+  double *resid = nullptr, *rhat = nullptr, *p = nullptr, *ap = nullptr, *as = nullptr, *sv = nullptr,
+         *ms = nullptr, *mp = nullptr;
+  double *scal = nullptr, *pa = nullptr, *pb = nullptr;
+  double* host_scal = nullptr;  // pinned
+  int nblk = 0;
+  int32_t* d_flags = nullptr;
+  std::map<std::pair<const void*, const void*>, hipGraphExec_t> vgraphs;
+  std::map<std::pair<const void*, const void*>, std::pair<hipGraphExec_t, hipGraphExec_t>> bgraphs;
+  double last_seconds = 0.0;
+};
+
+namespace {
+
+#define TE_RC(x)             \
+  do {                       \
+    int rc_ = (x);           \
+    if (rc_) return rc_;     \
+  } while (0)
+
+int zero_with_ghost(te_gmg* g, int li, void* v, int dt)
+{
+  return launch_vec_op(g->L[li].d, dt, VOP_SET_WITH_GHOST, v, nullptr, nullptr, 0.0, 0, 0, 1, g->s);
+}
+
+// One smoothing sweep on level li: PatchSolver::smooth (PatchSolver.h:142-170).
+// uzero: u is known to be zero (fresh coarse-level vector).  out/dto: where the
+// new u goes (== u except for the finest level's last mixed-precision sweep).
+int smooth(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void* out, int dto)
+{
+  GLevel& Lv = g->L[li];
+  if (g->o.smoother == 0) {
+    if (!uzero) TE_RC(launch_ghost_fill(Lv.gf, Lv.dt, u, 1, g->s));
+    return launch_dft(Lv.plan, Lv.dt, dtf, dto, f, u, out, uzero, g->s);
+  }
+  // Iterative::PatchSolver(BiCGStab): f and u share the level dtype here
+  if (uzero) TE_RC(zero_with_ghost(g, li, u, Lv.dt));
+  else TE_RC(launch_ghost_fill(Lv.gf, Lv.dt, u, 1, g->s));
+  double tol = g->o.inner_tol;
+  if (Lv.dt == TE_F32 && tol < 1e-6) tol = 1e-6;  // an FP32 inner solve cannot reach tighter
+  TE_RC(launch_patch_bicgstab(Lv.d, Lv.dt, g->o.op_kind, g->o.neumann, Lv.coeffs, f, u, g->o.inner_max_it, tol,
+                              nullptr, g->d_flags, g->s));
+  if (out != u) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), dto, out, Lv.dt, u, g->s));
+  return TE_OK;
+}
+
+// VCycle::visit (GMG/VCycle.h:46-70)
+int visit(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
+{
+  GLevel& Lv = g->L[li];
+  const int nl = (int)g->L.size();
+  const bool finest = li == 0;
+  const bool mixed_out = finest && Lv.dt != TE_F64;  // result must be written to u_ext in FP64
+  void* u = (finest && !mixed_out) ? (void*)u_ext : Lv.u;
+  if (li == nl - 1) {
+    int n = g->o.coarse_sweeps;
+    if (n == 0) {
+      TE_RC(zero_with_ghost(g, li, u, Lv.dt));
+      if (mixed_out) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), TE_F64, u_ext, Lv.dt, u, g->s));
+    }
+    for (int k = 0; k < n; k++) {
+      bool last = k == n - 1;
+      TE_RC(smooth(g, li, f, dtf, u, k == 0, (mixed_out && last) ? (void*)u_ext : u,
+                   (mixed_out && last) ? TE_F64 : Lv.dt));
+    }
+    return TE_OK;
+  }
+  for (int k = 0; k < g->o.pre_sweeps; k++) TE_RC(smooth(g, li, f, dtf, u, k == 0, u, Lv.dt));
+  if (g->o.pre_sweeps == 0) TE_RC(zero_with_ghost(g, li, u, Lv.dt));
+  // Cycle::restrict: r = f - A u, then restrict into the coarser level's f
+  TE_RC(launch_ghost_fill(Lv.gf, Lv.dt, u, 1, g->s));
+  TE_RC(launch_star(Lv.gf, Lv.dt, dtf, Lv.dt, g->o.op_kind, g->o.neumann, Lv.coeffs, f, u, Lv.r, 1, g->s));
+  GLevel& Cv = g->L[li + 1];
+  TE_RC(launch_restrict(Lv.link, Lv.dt, Lv.r, Cv.dt, Cv.f, 1, g->s));
+  TE_RC(visit(g, li + 1, Cv.f, Cv.dt, u_ext));
+  TE_RC(launch_interpolate(Lv.link, Cv.dt, Cv.u, Lv.dt, u, 1, g->s));
+  int n = g->o.post_sweeps;
+  for (int k = 0; k < n; k++) {
+    bool last = k == n - 1;
+    TE_RC(smooth(g, li, f, dtf, u, false, (mixed_out && last) ? (void*)u_ext : u,
+                 (mixed_out && last) ? TE_F64 : Lv.dt));
+  }
+  if (mixed_out && n == 0) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), TE_F64, u_ext, Lv.dt, u, g->s));
+  return TE_OK;
+}
+
+// Cycle::apply (GMG/Cycle.h:94-98) enqueued on g->s.
+int vcycle_enqueue(te_gmg* g, const double* f, double* u)
+{
+  GLevel& L0 = g->L[0];
+  if (L0.dt != TE_F64 && g->o.smoother != 0) {
+    // the inner-Krylov smoother works on same-dtype f and u: convert f once
+    TE_RC(launch_convert(te_domain_vector_length(L0.d, 1), L0.dt, L0.f, TE_F64, f, g->s));
+    return visit(g, 0, L0.f, L0.dt, u);
+  }
+  return visit(g, 0, f, TE_F64, u);
+}
+
+int vcycle_launch(te_gmg* g, const double* f, double* u)
+{
+  if (!g->o.use_graph) return vcycle_enqueue(g, f, u);
+  auto key = std::make_pair((const void*)f, (const void*)u);
+  auto it = g->vgraphs.find(key);
+  if (it == g->vgraphs.end()) {
+    hipGraph_t graph;
+    TE_HIP(hipStreamBeginCapture(g->s, hipStreamCaptureModeThreadLocal));
+    int rc = vcycle_enqueue(g, f, u);
+    hipError_t e = hipStreamEndCapture(g->s, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+      set_error(std::string("V-cycle graph capture: ") + hipGetErrorString(e));
+      return TE_ERR_HIP;
+    }
+    hipGraphExec_t exec;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      set_error(std::string("V-cycle graph instantiate: ") + hipGetErrorString(e));
+      return TE_ERR_HIP;
+    }
+    it = g->vgraphs.emplace(key, exec).first;
+  }
+  TE_HIP(hipGraphLaunch(it->second, g->s));
+  return TE_OK;
+}
+
+int check_flags(te_gmg* g)
+{
+  int32_t flag = 0;
+  TE_HIP(hipMemcpyAsync(&flag, g->d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, g->s));
+  TE_HIP(hipStreamSynchronize(g->s));
+  if (flag && !g->o.continue_on_breakdown) {
+    TE_HIP(hipMemsetAsync(g->d_flags, 0, sizeof(int32_t), g->s));
+    set_error("BiCGStab broke down in a patch smoother (rho was 0)");
+    return TE_ERR_BREAKDOWN;
+  }
+  return TE_OK;
+}
+
+void free_gmg(te_gmg* g)
+{
+  if (!g) return;
+  for (auto& kv : g->vgraphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : g->bgraphs) {
+    (void)hipGraphExecDestroy(kv.second.first);
+    (void)hipGraphExecDestroy(kv.second.second);
+  }
+  for (auto& Lv : g->L) {
+    if (Lv.gf) te_ghost_filler_destroy(Lv.gf);
+    if (Lv.plan) te_dft_plan_destroy(Lv.plan);
+    if (Lv.link) te_level_link_destroy(Lv.link);
+    (void)hipFree(Lv.f);
+    (void)hipFree(Lv.u);
+    (void)hipFree(Lv.r);
+    (void)hipFree(Lv.coeffs32);
+  }
+  double* vs[] = { g->resid, g->rhat, g->p, g->ap, g->as, g->sv, g->ms, g->mp, g->scal, g->pa, g->pb };
+  for (double* v : vs) (void)hipFree(v);
+  (void)hipFree(g->d_flags);
+  if (g->host_scal) (void)hipHostFree(g->host_scal);
+  if (g->s) (void)hipStreamDestroy(g->s);
+  delete g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_options* opts,
+                  const double* const* coeffs, te_gmg** out)
+{
+  if (!levels || num_levels < 1 || !opts || !out) {
+    set_error("te_gmg_create: bad argument");
+    return TE_ERR_ARG;
+  }
+  if (opts->precision != TE_F64 && opts->precision != TE_F32) {
+    set_error("te_gmg_create: precision must be TE_F64 or TE_F32");
+    return TE_ERR_ARG;
+  }
+  if (opts->op_kind == 1 && !coeffs) {
+    set_error("te_gmg_create: VarPoisson needs per-level coefficients");
+    return TE_ERR_ARG;
+  }
+  if (opts->smoother != 0 && opts->smoother != 1) {
+    set_error("te_gmg_create: smoother must be 0 (DFT) or 1 (BiCGStab patch solver)");
+    return TE_ERR_ARG;
+  }
+  auto g = new te_gmg();
+  g->o = *opts;
+  hipError_t e = hipStreamCreate(&g->s);
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    delete g;
+    return TE_ERR_HIP;
+  }
+  g->L.resize(num_levels);
+  int rc = TE_OK;
+  for (int i = 0; i < num_levels && !rc; i++) {
+    GLevel& Lv = g->L[i];
+    Lv.d = levels[i];
+    bool coarsest = i == num_levels - 1;
+    Lv.dt = (opts->precision == TE_F32 && !coarsest) ? TE_F32 : TE_F64;
+    rc = te_ghost_filler_create(Lv.d, opts->filler_kind, opts->fill_type, &Lv.gf);
+    if (rc) break;
+    long n = te_domain_vector_length(Lv.d, 1);
+    size_t bytes = esize(Lv.dt) * (size_t)(n > 0 ? n : 1);
+    if ((e = hipMalloc(&Lv.f, bytes)) != hipSuccess || (e = hipMalloc(&Lv.u, bytes)) != hipSuccess ||
+        (e = hipMalloc(&Lv.r, bytes)) != hipSuccess) {
+      set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
+      rc = TE_ERR_HIP;
+      break;
+    }
+    (void)hipMemset(Lv.f, 0, bytes);
+    (void)hipMemset(Lv.u, 0, bytes);
+    (void)hipMemset(Lv.r, 0, bytes);
+    if (opts->op_kind == 1) {
+      if (!coeffs[i]) {
+        set_error("te_gmg_create: missing coefficients for a level");
+        rc = TE_ERR_ARG;
+        break;
+      }
+      Lv.coeffs64 = coeffs[i];
+      if (Lv.dt == TE_F64) {
+        Lv.coeffs = coeffs[i];
+      } else {
+        if ((e = hipMalloc(&Lv.coeffs32, sizeof(float) * (n > 0 ? n : 1))) != hipSuccess) {
+          set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
+          rc = TE_ERR_HIP;
+          break;
+        }
+        rc = launch_convert(n, TE_F32, Lv.coeffs32, TE_F64, coeffs[i], g->s);
+        Lv.coeffs = Lv.coeffs32;
+      }
+    }
+    if (opts->smoother == 0 && !rc)
+      rc = te_dft_plan_create(Lv.gf, opts->neumann_mask, opts->op_kind, opts->op_kind == 1 ? coeffs[i] : nullptr,
+                              &Lv.plan);
+    if (i > 0 && !rc) rc = te_level_link_create(g->L[i - 1].d, Lv.d, &g->L[i - 1].link);
+  }
+  if (!rc) {
+    const te_domain* d0 = levels[0];
+    long n = te_domain_vector_length(d0, 1);
+    size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 1);
+    double** vs[] = { &g->resid, &g->rhat, &g->p, &g->ap, &g->as, &g->sv, &g->ms, &g->mp };
+    for (double** v : vs) {
+      if ((e = hipMalloc(v, bytes)) != hipSuccess) break;
+      (void)hipMemset(*v, 0, bytes);
+    }
+    g->nblk = krylov_blocks(d0);
+    if (e == hipSuccess) e = hipMalloc(&g->scal, sizeof(double) * 16);
+    if (e == hipSuccess) e = hipMalloc(&g->pa, sizeof(double) * (g->nblk + 1));
+    if (e == hipSuccess) e = hipMalloc(&g->pb, sizeof(double) * (g->nblk + 1));
+    if (e == hipSuccess) e = hipMalloc(&g->d_flags, sizeof(int32_t) * 4);
+    if (e == hipSuccess) e = hipMemset(g->d_flags, 0, sizeof(int32_t) * 4);
+    if (e == hipSuccess) e = hipMemset(g->scal, 0, sizeof(double) * 16);
+    if (e == hipSuccess) e = hipHostMalloc(&g->host_scal, sizeof(double) * 16, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->s);
+    if (e != hipSuccess) {
+      set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
+      rc = TE_ERR_HIP;
+    }
+  }
+  if (rc) {
+    free_gmg(g);
+    return rc;
+  }
+  *out = g;
+  return TE_OK;
+}
+
+int te_gmg_destroy(te_gmg* g)
+{
+  free_gmg(g);
+  return TE_OK;
+}
+
+int te_gmg_vcycle(te_gmg* g, const double* f, double* u, te_stream stream)
+{
+  if (!g || !f || !u) {
+    set_error("te_gmg_vcycle: bad argument");
+    return TE_ERR_ARG;
+  }
+  TE_HIP(hipStreamSynchronize((hipStream_t)stream));
+  TE_RC(vcycle_launch(g, f, u));
+  TE_HIP(hipStreamSynchronize(g->s));
+  return check_flags(g);
+}
+
+// Iterative::BiCGStab::solve (Iterative/BiCGStab.h:111-203)
+int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int max_it, double tol, int* iterations,
+                    double* history, int history_cap, te_stream stream)
+{
+  if (!g || !b || !x) {
+    set_error("te_gmg_bicgstab: bad argument");
+    return TE_ERR_ARG;
+  }
+  TE_HIP(hipStreamSynchronize((hipStream_t)stream));
+  GLevel& L0 = g->L[0];
+  const te_domain* d = L0.d;
+  te_ghost_filler* gf = L0.gf;
+  const double* cf = g->o.op_kind == 1 ? L0.coeffs64 : nullptr;  // the outer operator is FP64
+  const int nb = g->nblk;
+  hipStream_t s = g->s;
+  double* h = g->host_scal;
+  auto read_scal = [&]() -> int {
+    TE_HIP(hipMemcpyAsync(h, g->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s));
+    TE_HIP(hipStreamSynchronize(s));
+    return TE_OK;
+  };
+  double* mp = use_precond ? g->mp : g->p;
+  double* ms = use_precond ? g->ms : g->sv;
+  // phase 1: mp = M p; ap = A mp (+ rhat.ap); alpha; s = r - alpha ap; ||s||
+  auto phase1 = [&]() -> int {
+    if (use_precond) TE_RC(vcycle_enqueue(g, g->p, g->mp));
+    TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
+    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, g->ap, g->rhat, g->pa, nullptr, s));
+    TE_RC(launch_bicg_finalize(FIN_ALPHA, g->pa, nullptr, nb, g->scal, s));
+    TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pa, nullptr,
+                      s));
+    return launch_bicg_finalize(FIN_SNORM, g->pa, nullptr, nb, g->scal, s);
+  };
+  // phase 2: ms = M s; as = A ms (+ as.s, as.as); omega; x, r updates; rho, beta, ||r||; p update
+  auto phase2 = [&]() -> int {
+    if (use_precond) TE_RC(vcycle_enqueue(g, g->sv, g->ms));
+    TE_RC(launch_ghost_fill(gf, TE_F64, ms, 1, s));
+    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, g->as, g->sv, g->pa, g->pb, s));
+    TE_RC(launch_bicg_finalize(FIN_OMEGA, g->pa, g->pb, nb, g->scal, s));
+    TE_RC(launch_bicg(BICG_XR, d, g->scal, x, g->resid, mp, ms, g->ap, g->as, g->rhat, g->pa, g->pb, s));
+    TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s));
+    return launch_bicg(BICG_P, d, g->scal, g->p, nullptr, g->ap, g->resid, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, s);
+  };
+  hipGraphExec_t gx1 = nullptr, gx2 = nullptr;
+  if (g->o.use_graph) {
+    auto key = std::make_pair((const void*)b, (const void*)x);
+    auto it = g->bgraphs.find(key);
+    if (it == g->bgraphs.end() || use_precond == 0) {
+      hipGraphExec_t ex[2];
+      for (int ph = 0; ph < 2; ph++) {
+        hipGraph_t graph;
+        TE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = ph == 0 ? phase1() : phase2();
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc) return rc;
+        if (e != hipSuccess) {
+          set_error(std::string("BiCGStab graph capture: ") + hipGetErrorString(e));
+          return TE_ERR_HIP;
+        }
+        e = hipGraphInstantiate(&ex[ph], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+          set_error(std::string("BiCGStab graph instantiate: ") + hipGetErrorString(e));
+          return TE_ERR_HIP;
+        }
+      }
+      if (use_precond) {
+        it = g->bgraphs.emplace(key, std::make_pair(ex[0], ex[1])).first;
+        gx1 = it->second.first;
+        gx2 = it->second.second;
+      } else {
+        gx1 = ex[0];
+        gx2 = ex[1];
+      }
+    } else {
+      gx1 = it->second.first;
+      gx2 = it->second.second;
+    }
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = TE_OK;
+  int num_its = 0;
+  // resid = b - A x ; r0_norm = ||b|| ; rhat = p = resid ; rho = rhat.resid
+  TE_RC(launch_ghost_fill(gf, TE_F64, x, 1, s));
+  TE_RC(launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, b, x, g->resid, 1, s));
+  TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s));
+  TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s));
+  TE_RC(read_scal());
+  double r0 = h[SC_R0];
+  double residual = h[SC_RNORM] / r0;
+  double rho = h[SC_RHO];
+  if (history && history_cap > 0) history[0] = residual;
+  if (r0 != 0.0) {
+    while (residual > tol && num_its < max_it) {
+      if (rho == 0.0) {
+        set_error("BiCGStab broke down, rho was 0 on iteration " + std::to_string(num_its));
+        rc = TE_ERR_BREAKDOWN;
+        break;
+      }
+      if (gx1) TE_HIP(hipGraphLaunch(gx1, s));
+      else TE_RC(phase1());
+      TE_RC(read_scal());
+      if (h[SC_SNORM] / r0 <= tol) {
+        const double* dirp = use_precond == 2 ? mp : g->p;  // reference: alpha p (BiCGStab.h:162-167)
+        TE_RC(launch_bicg(BICG_XP, d, g->scal, x, nullptr, dirp, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                          s));
+        TE_HIP(hipStreamSynchronize(s));
+        break;
+      }
+      if (gx2) TE_HIP(hipGraphLaunch(gx2, s));
+      else TE_RC(phase2());
+      TE_RC(read_scal());
+      num_its++;
+      rho = h[SC_RHO];
+      residual = h[SC_RNORM] / r0;
+      if (history && num_its < history_cap) history[num_its] = residual;
+    }
+  }
+  TE_HIP(hipStreamSynchronize(s));
+  auto t1 = std::chrono::steady_clock::now();
+  g->last_seconds = std::chrono::duration<double>(t1 - t0).count();
+  if (!use_precond && gx1) {
+    (void)hipGraphExecDestroy(gx1);
+    (void)hipGraphExecDestroy(gx2);
+  }
+  if (iterations) *iterations = num_its;
+  if (rc) return rc;
+  return check_flags(g);
+}
+
+double te_gmg_last_solve_seconds(const te_gmg* g) { return g ? g->last_seconds : -1.0; }
+
+}  // extern "C"

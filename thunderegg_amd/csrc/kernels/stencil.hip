@@ -96,19 +96,27 @@ void launch_dim(const te_domain* d, int op_kind, bool resid, const T* u, const T
 
 }  // namespace
 
-// fin != nullptr: out = fin - A u (residual); else out = A u.  u, coeffs, fin, out share dtype.
-int launch_star(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs, const void* fin,
-                const void* u, void* out, int ncomp, hipStream_t s)
+// fin != nullptr: out = fin - A u (residual); else out = A u.  dt: type of u and coeffs,
+// dtf: type of fin, dto: type of out.  Supported: (f64,f64,f64), (f32,f32,f32), (f32,f64,f32).
+int launch_star(const te_ghost_filler* g, int dt, int dtf, int dto, int op_kind, int neumann, const void* coeffs,
+                const void* fin, const void* u, void* out, int ncomp, hipStream_t s)
 {
   const te_domain* d = g->d;
   if (d->np == 0) return TE_OK;
   bool resid = fin != nullptr;
-  if (dtype == TE_F64)
+  if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64)
     launch_dim<double, double, double>(d, op_kind, resid, (const double*)u, (const double*)coeffs,
                                        (const double*)fin, (double*)out, ncomp, neumann, s);
-  else
+  else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32)
     launch_dim<float, float, float>(d, op_kind, resid, (const float*)u, (const float*)coeffs, (const float*)fin,
                                     (float*)out, ncomp, neumann, s);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32)
+    launch_dim<float, double, float>(d, op_kind, resid, (const float*)u, (const float*)coeffs,
+                                     (const double*)fin, (float*)out, ncomp, neumann, s);
+  else {
+    set_error("unsupported dtype combination for the star stencil");
+    return TE_ERR_UNSUPPORTED;
+  }
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

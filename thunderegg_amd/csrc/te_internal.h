@@ -80,6 +80,7 @@ struct te_domain {
   double* d_scratch = nullptr;     // reduction partials
   unsigned* d_counter = nullptr;   // last-block counter for reductions
   double* d_result = nullptr;      // reduction results
+  int32_t* d_flags = nullptr;      // breakdown flags
   int scratch_len = 0;
 };
 
@@ -104,7 +105,7 @@ namespace te {
 
 // ---- launchers (defined in kernels/*.hip) ----
 int launch_ghost_fill(const te_ghost_filler* g, int dtype, void* u, int ncomp, hipStream_t s);
-int launch_star(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs,
+int launch_star(const te_ghost_filler* g, int dt, int dtf, int dto, int op_kind, int neumann, const void* coeffs,
                 const void* fin, const void* u, void* out, int ncomp, hipStream_t s);
 int launch_restrict(const te_level_link* l, int dtype_in, const void* fine, int dtype_out, void* coarse, int ncomp,
                     hipStream_t s);

@@ -1,0 +1,150 @@
+"""Host-side driver objects over the C ABI: the forest (mesh generator), the
+multigrid hierarchy and the outer BiCGStab.
+
+These mirror the reference's setup sequence (SURVEY call stack F): a domain
+generator produces the finest and successively coarser domains; every level gets
+a ghost filler, operator and smoother; CycleBuilder assembles the V-cycle;
+BiCGStab solves with the cycle as right preconditioner.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import mesh
+from . import runtime as R
+
+
+class Forest:
+    """Quadtree (2D) / octree (3D) forest on the unit square / cube (te_forest),
+    the replacement of P4estDomainGenerator / P8estDomainGenerator."""
+
+    def __init__(self, dim: int, base_level: int):
+        self.dim = dim
+        h = C.c_void_p()
+        R.check(R.lib().te_forest_create(dim, base_level, C.byref(h)))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            R.lib().te_forest_destroy(self.h)
+            self.h = None
+
+    def leaves(self):
+        n = R.lib().te_forest_num_leaves(self.h)
+        lv = np.zeros(n, dtype=np.int32)
+        co = np.zeros((n, self.dim), dtype=np.int32)
+        R.check(R.lib().te_forest_get_leaves(self.h, lv.ctypes.data, co.ctypes.data))
+        return lv, co
+
+    def leaf_boxes(self):
+        """(lower corner, size) of every leaf in unit coordinates."""
+        lv, co = self.leaves()
+        size = 1.0 / (1 << lv.astype(np.int64))
+        return co * size[:, None], size
+
+    def refine(self, flags):
+        flags = np.ascontiguousarray(flags, dtype=np.uint8)
+        R.check(R.lib().te_forest_refine(self.h, flags.ctypes.data))
+
+    def balance(self):
+        R.check(R.lib().te_forest_balance(self.h))
+
+    def build_domains(self, ns):
+        """Patch tables of every level, finest first (getFinestDomain, getCoarserDomain...)."""
+        nsa = (C.c_int * 3)(*(list(ns) + [1] * (3 - len(ns))))
+        nl = C.c_int()
+        R.check(R.lib().te_forest_build_domains(self.h, nsa, C.byref(nl)))
+        out = []
+        for k in range(nl.value):
+            n = C.c_int()
+            R.check(R.lib().te_forest_domain_size(self.h, k, C.byref(n)))
+            arr = np.zeros(n.value, dtype=mesh.PATCH_DTYPE)
+            R.check(R.lib().te_forest_domain_patches(self.h, k, arr.ctypes.data))
+            out.append(arr)
+        return out
+
+
+def uniform_levels(dim: int, level: int, ns):
+    f = Forest(dim, level)
+    return f.build_domains(ns)
+
+
+class _Opts(C.Structure):
+    _fields_ = [
+        ("filler_kind", C.c_int),
+        ("fill_type", C.c_int),
+        ("op_kind", C.c_int),
+        ("neumann", C.c_int),
+        ("smoother", C.c_int),
+        ("neumann_mask", C.c_uint),
+        ("inner_max_it", C.c_int),
+        ("inner_tol", C.c_double),
+        ("continue_on_breakdown", C.c_int),
+        ("pre_sweeps", C.c_int),
+        ("post_sweeps", C.c_int),
+        ("coarse_sweeps", C.c_int),
+        ("precision", C.c_int),
+        ("use_graph", C.c_int),
+    ]
+
+
+class GMG:
+    """V-cycle hierarchy + outer BiCGStab on the device (te_gmg).
+
+    levels: patch tables finest first.  smoother 0 = DFTPatchSolver,
+    1 = Iterative::PatchSolver(BiCGStab).  precision F64 or F32 (FP32 V-cycle,
+    FP64 coarsest solve, FP64 outer iteration)."""
+
+    def __init__(self, dim, ns, levels, filler_kind=R.BILINEAR, fill_type=R.FACES, op_kind=0, neumann=False,
+                 smoother=0, neumann_mask=0, inner_max_it=1000, inner_tol=1e-12, continue_on_breakdown=False,
+                 pre_sweeps=1, post_sweeps=1, coarse_sweeps=1, precision=R.F64, use_graph=True, coeffs=None):
+        self.dim, self.ns = dim, tuple(ns)
+        self.domains = [R.Domain(dim, ns, lv) for lv in levels]
+        self.coeffs = None
+        cptrs = None
+        if coeffs is not None:
+            self.coeffs = [R.DeviceArray.from_host(np.ascontiguousarray(c, dtype=np.float64)) for c in coeffs]
+            cptrs = (C.c_void_p * len(levels))(*[c.ptr for c in self.coeffs])
+        o = _Opts(filler_kind, fill_type, op_kind, int(neumann), smoother, neumann_mask, inner_max_it, inner_tol,
+                  int(continue_on_breakdown), pre_sweeps, post_sweeps, coarse_sweeps, precision, int(use_graph))
+        hs = (C.c_void_p * len(self.domains))(*[d.h for d in self.domains])
+        h = C.c_void_p()
+        R.check(R.lib().te_gmg_create(hs, len(self.domains), C.byref(o), cptrs, C.byref(h)))
+        self.h = h.value
+        self.shape = self.domains[0].shape(1)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            R.lib().te_gmg_destroy(self.h)
+            self.h = None
+
+    def vcycle_device(self, f: R.DeviceArray, u: R.DeviceArray):
+        R.check(R.lib().te_gmg_vcycle(self.h, f.ptr, u.ptr, None))
+
+    def vcycle(self, f: np.ndarray) -> np.ndarray:
+        df = R.DeviceArray.from_host(np.ascontiguousarray(f, dtype=np.float64))
+        du = R.DeviceArray(self.shape, np.float64)
+        du.zero()
+        self.vcycle_device(df, du)
+        return du.get()
+
+    def solve_device(self, b: R.DeviceArray, x: R.DeviceArray, tol=1e-10, max_it=1000, precond=True,
+                     history_cap=0):
+        """precond: False (none), True (the reference's solve), 2 (V-cycle with the
+        early exit corrected to x += alpha M p)."""
+        its = C.c_int(0)
+        hist = np.full(max(history_cap, 1), np.nan)
+        R.check(R.lib().te_gmg_bicgstab(self.h, b.ptr, x.ptr, int(precond), max_it, tol, C.byref(its),
+                                        hist.ctypes.data_as(C.POINTER(C.c_double)), history_cap, None))
+        return its.value, hist[: its.value + 1] if history_cap else None
+
+    def solve(self, b: np.ndarray, x0: np.ndarray | None = None, tol=1e-10, max_it=1000, precond=True):
+        db = R.DeviceArray.from_host(np.ascontiguousarray(b, dtype=np.float64))
+        dx = R.DeviceArray.from_host(np.zeros(self.shape) if x0 is None else np.ascontiguousarray(x0, np.float64))
+        its, hist = self.solve_device(db, dx, tol=tol, max_it=max_it, precond=precond, history_cap=max_it + 2)
+        return dx.get(), its, hist
+
+    def last_solve_seconds(self):
+        return R.lib().te_gmg_last_solve_seconds(self.h)
