@@ -1,0 +1,275 @@
+#!/usr/bin/env python3
+"""Benchmark: DOF/s to a 1e-10 relative residual on one MI355X.
+
+Workload (default --config 2): 2D adaptive Poisson, quadtree levels 4-9 refined
+within 0.05 of 8 seeded random points, 2:1 balanced, 19,930 patches of 16x16
+cells (5.1M DOF), BiQuadratic ghost filling, 5-point star operator, DFT patch
+smoother (1 pre / 1 post / 1 coarse sweep) in a 10-level V-cycle preconditioning
+an FP64 BiCGStab to 1e-10.
+
+A "step" is one complete solve from x = 0 to ||r||/||b|| <= 1e-10 with the
+V-cycle in FP32 (coarsest level FP64) under FP64 BiCGStab refinement; `value`
+is finest-level DOF x steps / wall time of the timed steps, inputs resident in
+HBM.  The plain FP64 GPU solve is timed the same way and reported beside it.
+The BiCGStab early exit uses x += alpha M p (the reference adds alpha p, which
+leaves the true residual above the tolerance whenever that exit fires).
+
+Prints one JSON line.  --gpus N > 1 exits: this build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFS = 157.3
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def true_residual(R, filler, dom, b, x, r):
+    """||b - A x|| / ||b|| in FP64 on the device."""
+    filler.residual(b, x, r)
+    return dom.two_norm(r) / dom.two_norm(b)
+
+
+def time_launches(R, fn, n, stream):
+    e0, e1 = R.Event(), R.Event()
+    fn()  # warm
+    stream.sync()
+    e0.record(stream)
+    for _ in range(n):
+        fn()
+    e1.record(stream)
+    return R.Event.elapsed_ms(e0, e1) / n
+
+
+def kernel_rooflines(R, P, stream, reps=50):
+    """Average launch time of the finest-level kernels (HIP events on their stream)
+    and their algorithmic bytes / flops per launch."""
+    import numpy as np
+
+    from thunderegg_amd import mesh
+
+    dim, ns = P.dim, P.ns
+    dom = R.Domain(dim, ns, P.levels[0])
+    npch = len(P.levels[0])
+    N = int(np.prod(ns))
+    if dim == 2:  # face ghost cells per patch
+        G = 2 * (ns[0] + ns[1])
+    else:
+        G = 2 * (ns[0] * ns[1] + ns[1] * ns[2] + ns[0] * ns[2])
+    # faces with a neighbour, per patch (ghost cells actually filled / read)
+    sb = 4 if dim == 2 else 20
+    has = P.levels[0]["nbr_type"][:, sb:sb + 2 * dim] >= 0
+    face_cells = np.array([ns[1], ns[1], ns[0], ns[0]] if dim == 2 else
+                          [ns[1] * ns[2]] * 2 + [ns[0] * ns[2]] * 2 + [ns[0] * ns[1]] * 2)
+    Gn = float((has * face_cells[None, :]).sum()) / npch  # neighbour-facing ghost cells per patch
+    out = {}
+    filler = R.GhostFiller(dom, P.filler, R.FACES)
+    for dt, s in ((R.F32, 4), (R.F64, 8)):
+        u = R.DeviceArray.from_host(np.random.default_rng(1).standard_normal(dom.shape(1)), R.np_dtype(dt))
+        f = R.DeviceArray.from_host(np.random.default_rng(2).standard_normal(dom.shape(1)), R.np_dtype(dt))
+        filler.fill_ghost(u)
+        tag = "f32" if dt == R.F32 else "f64"
+        # star stencil: s (N + G) read + s N write per patch (ghosts already filled)
+        t = time_launches(R, lambda: R.check(R.lib().te_star_apply_patches(filler.h, dt, 0, u.ptr, f.ptr, 1,
+                                                                           stream.h)), reps, stream)
+        byt = npch * s * (2 * N + G)
+        out[f"star_apply_{tag}"] = {"ms": t, "bytes": byt, "GBs": byt / t / 1e6}
+        # ghost fill: s (sources) + s (ghosts written), neighbour-facing faces
+        t = time_launches(R, lambda: filler.fill_ghost(u, stream=stream.h), reps, stream)
+        byt = npch * s * 2 * Gn
+        out[f"ghost_fill_{tag}"] = {"ms": t, "bytes": byt, "GBs": byt / t / 1e6}
+        if P.smoother == 0 and P.op_kind == 0:
+            plan = R.DftPlan(filler)
+            t = time_launches(R, lambda: R.check(R.lib().te_dft_patch_solve(plan.h, dt, f.ptr, u.ptr, stream.h)),
+                              reps, stream)
+            # f (N) + u ring for the interface terms (2 Gn) read, u (N) written; flops of the
+            # separable transforms: 2D 8 n^3 (+ n^2 divides) per patch
+            byt = npch * s * (2 * N + 2 * Gn)
+            fl = npch * (8 * ns[0] ** 3 + N) if dim == 2 else npch * (12 * ns[0] ** 4 + N)
+            out[f"dft_patch_solve_{tag}"] = {"ms": t, "bytes": byt, "GBs": byt / t / 1e6, "flops": fl,
+                                            "TFs": fl / t / 1e9}
+        del u, f
+    return out
+
+
+def cpu_baseline(P, gpu_iters, sample_iters=3):
+    """The oracle (FP64 C restatement of the reference algorithm, one core) on the same
+    problem: a bounded sample of `sample_iters` outer BiCGStab iterations, extrapolated to
+    the GPU FP64 solve's iteration count."""
+    from oracle import oracle as O
+
+    od = [O.Domain(P.dim, P.ns, lv) for lv in P.levels]
+    kw = {}
+    if P.op_kind == 1:
+        coeffs = []
+        for lv, d in zip(P.levels, od):
+            c = P.coeffs[len(coeffs)].copy()
+            O.fill_ghost(d, c, kind=P.filler)
+            coeffs.append(c)
+        kw = {"op_kind": 1, "coeffs": coeffs}
+    L = O.Levels(od, filler_kind=P.filler, smoother_kind=P.smoother, **kw)
+    t0 = time.perf_counter()
+    _, its, _ = L.bicgstab(P.rhs.copy(), tol=1e-30, precond=2, pre=P.pre, post=P.post, coarse=P.coarse,
+                           max_it=sample_iters)
+    dt = time.perf_counter() - t0
+    per_it = dt / max(its, 1)
+    est = per_it * max(gpu_iters, 1)
+    return {
+        "value": P.dof / est,
+        "unit": "DOF/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle FP64 BiCGStab+V-cycle on the full {P.dof}-DOF problem, {its} outer iterations "
+                   f"in {dt:.2f} s ({per_it:.3f} s/iteration), extrapolated to the {gpu_iters} iterations "
+                   f"the FP64 GPU solve needs"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
+                          "n_gpus_requested": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+
+    from thunderegg_amd import configs
+    from thunderegg_amd import runtime as R
+    from thunderegg_amd import solver
+
+    R.lib()  # fails loudly when the HIP library is missing
+    t0 = time.perf_counter()
+    P = configs.CONFIGS[args.config]()
+    t_mesh = time.perf_counter() - t0
+    log(f"config {args.config}: {P.name}; levels {[len(l) for l in P.levels]}; DOF {P.dof}; mesh {t_mesh:.1f}s")
+
+    def make(precision):
+        return solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, op_kind=P.op_kind,
+                          coeffs=P.coeffs, pre_sweeps=P.pre, post_sweeps=P.post, coarse_sweeps=P.coarse,
+                          precision=precision, use_graph=True)
+
+    t0 = time.perf_counter()
+    g_mixed = make(R.F32)
+    g_fp64 = None if args.no_fp64 else make(R.F64)
+    t_setup = time.perf_counter() - t0
+    dom = g_mixed.domains[0]
+    b = R.DeviceArray.from_host(P.rhs)
+    xm = R.DeviceArray(dom.shape(1), np.float64)
+    x64 = R.DeviceArray(dom.shape(1), np.float64)
+    r = R.DeviceArray(dom.shape(1), np.float64)
+    filler = R.GhostFiller(dom, P.filler, R.FACES)
+
+    def run(g, x, k):
+        its = []
+        R.sync()
+        ts = []
+        t_start = time.perf_counter()
+        for _ in range(k):
+            t1 = time.perf_counter()
+            x.zero()
+            it, _ = g.solve_device(b, x, tol=1e-10, max_it=500, precond=2)
+            R.sync()
+            ts.append(time.perf_counter() - t1)
+            its.append(it)
+        R.sync()
+        return time.perf_counter() - t_start, its, ts
+
+    # warmup (includes hipGraph capture of both BiCGStab phases)
+    run(g_mixed, xm, args.warmup)
+    if g_fp64:
+        run(g_fp64, x64, args.warmup)
+    log("warmup done")
+    T_mixed, its_mixed, ts_mixed = run(g_mixed, xm, args.steps)
+    log(f"mixed: {args.steps} solves in {T_mixed:.3f}s, iterations {its_mixed}")
+    res = {"mixed_true_residual": true_residual(R, filler, dom, b, xm, r)}
+    if g_fp64:
+        T_64, its_64, ts_64 = run(g_fp64, x64, args.steps)
+        log(f"fp64: {args.steps} solves in {T_64:.3f}s, iterations {its_64}")
+        res["fp64_true_residual"] = true_residual(R, filler, dom, b, x64, r)
+        a, c = xm.get(), x64.get()
+        from thunderegg_amd import mesh
+
+        ia, ic = mesh.interior(a, P.dim), mesh.interior(c, P.dim)
+        res["mixed_vs_fp64_rel_l2"] = float(np.sqrt(((ia - ic) ** 2).sum() / (ic ** 2).sum()))
+    ms = T_mixed / args.steps * 1e3
+    value = P.dof * args.steps / T_mixed
+    out = {
+        "metric": "DOF/s to 1e-10 rel residual, FP32-precond+FP64 refine vs FP64 GPU; HBM GB/s",
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle / f64 BiCGStab",
+        "data": "synthetic (seeded mesh and right-hand side)",
+        "config": {
+            "workload": P.name,
+            "config_id": args.config,
+            "patches_finest": len(P.levels[0]),
+            "levels": [len(l) for l in P.levels],
+            "dof": P.dof,
+            "cells_per_patch": list(P.ns),
+            "tolerance": 1e-10,
+            "smoothing": [P.pre, P.post, P.coarse],
+        },
+        "iterations_mixed": its_mixed,
+        "solve_ms_median_mixed": statistics.median(ts_mixed) * 1e3,
+        "setup_seconds": {"mesh": t_mesh, "plans": t_setup},
+    }
+    out.update(res)
+    if g_fp64:
+        out["fp64_dof_per_s"] = P.dof * args.steps / T_64
+        out["iterations_fp64"] = its_64
+        out["solve_ms_median_fp64"] = statistics.median(ts_64) * 1e3
+        out["mixed_over_fp64"] = out["value"] / out["fp64_dof_per_s"]
+    if not args.no_roofline:
+        st = R.Stream()
+        kr = kernel_rooflines(R, P, st)
+        out["kernels"] = kr
+        dom_k = "dft_patch_solve_f32" if "dft_patch_solve_f32" in kr else "star_apply_f32"
+        k = kr[dom_k]
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", f"{dom_k}_cfg{args.config}_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as fh:
+                traffic = json.load(fh).get("bytes_per_launch")
+        if "flops" in k and False:
+            out["roofline"] = {"bound": "mfma", "achieved": k["TFs"], "peak": MFMA_FP32_PEAK_TFS, "unit": "TFLOP/s",
+                               "frac": k["TFs"] / MFMA_FP32_PEAK_TFS, "traffic": traffic, "kernel": dom_k}
+        else:
+            out["roofline"] = {"bound": "hbm", "achieved": k["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom_k,
+                               "bytes_per_launch": k["bytes"], "ms_per_launch": k["ms"]}
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(P, out.get("iterations_fp64", its_mixed)[0])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

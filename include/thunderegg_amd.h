@@ -84,6 +84,12 @@ int te_stream_synchronize(te_stream stream);
 /* Create/destroy a stream at the default priority. */
 int te_stream_create(te_stream* out);
 int te_stream_destroy(te_stream s);
+/* HIP events for timing work on a stream (hipEventRecord / hipEventElapsedTime). */
+typedef void* te_event;
+int te_event_create(te_event* out);
+int te_event_destroy(te_event e);
+int te_event_record(te_event e, te_stream s);
+int te_event_elapsed_ms(te_event start, te_event stop, float* ms); /* waits for stop */
 
 /* ------------------------------------------------------ domain / tables --
  * Domain<D>(comm, id, ns, num_ghost_cells, first, last) (Domain.h:223-248):
@@ -120,6 +126,11 @@ int te_ghost_fill(const te_ghost_filler* g, int dtype, void* u, int ncomp, te_st
  * and not stored back into u. */
 int te_star_apply(const te_ghost_filler* g, int dtype, int neumann, void* u, void* f, int ncomp,
                   te_stream stream);
+/* The per-patch loop of PatchOperator::apply alone (PatchOperator.h:136-142: f := 0 with
+ * ghosts, then applySinglePatch on every patch): u's neighbour ghosts must already be
+ * filled.  One launch of the batched stencil. */
+int te_star_apply_patches(const te_ghost_filler* g, int dtype, int neumann, const void* u, void* f, int ncomp,
+                          te_stream stream);
 /* VarPoisson::StarPatchOperator::apply (VarPoisson/StarPatchOperator.h:61-109); coeffs
  * (same dtype, ncomp 1) must be ghost-filled once beforehand, as the reference's
  * constructor does (:70). */
@@ -143,6 +154,10 @@ int te_dft_plan_destroy(te_dft_plan* p);
 int te_dft_smooth(const te_dft_plan* p, int dtype, const void* f, void* u, te_stream stream);
 /* PatchSolver::apply (PatchSolver.h:98-134): u := 0, then per-patch solves. */
 int te_dft_apply(const te_dft_plan* p, int dtype, const void* f, void* u, te_stream stream);
+/* The per-patch loop of PatchSolver::smooth alone (PatchSolver.h:156-166, each patch
+ * DFTPatchSolver::solveSinglePatch, Poisson/DFTPatchSolver.h:420-450): u's ghosts
+ * must already be filled.  One launch of the batched patch solve. */
+int te_dft_patch_solve(const te_dft_plan* p, int dtype, const void* f, void* u, te_stream stream);
 
 /* Iterative::PatchSolver<D>(BiCGStab, op) smooth (Iterative/PatchSolver.h:134-175):
  * per patch, an inner BiCGStab (Iterative/BiCGStab.h:111-203) on the patch operator

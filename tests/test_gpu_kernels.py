@@ -186,6 +186,46 @@ def test_dft_smooth(name, ns, neumann_mask):
         assert relerr(mesh.interior(got, dim), mesh.interior(uref, dim)) <= tol, dtype
 
 
+@pytest.mark.parametrize("ns", [(16, 16), (32, 32)])
+@pytest.mark.parametrize("op_kind", [0, 1])
+def test_dft_mfma_matches_generic(ns, op_kind, monkeypatch):
+    """The matrix-core patch solve (2D, n = 16/32, Dirichlet) against the generic kernel
+    and against the oracle."""
+    _print_case("dft-mfma", ns, op_kind)
+    dim, levels = load("2d_uniform_8x8_refined_cross_mpi1.json", ns)
+    lv = levels[0]
+    od = O.Domain(dim, ns, lv)
+    dd = _dev(dim, ns, lv)
+    g = R.GhostFiller(dd, R.BIQUADRATIC, R.FACES)
+    coeffs = None
+    if op_kind == 1:
+        X = mesh.cell_centers(lv, dim, ns)
+        coeffs = np.exp(np.sin(3 * X[..., 0]) * np.cos(2 * X[..., 1]))[:, None].copy()
+        O.fill_ghost(od, coeffs, kind=R.BIQUADRATIC)
+    dc = R.DeviceArray.from_host(coeffs) if coeffs is not None else None
+    plan_mfma = R.DftPlan(g, op_kind=op_kind, coeffs=dc)
+    monkeypatch.setenv("TE_DFT_VALU", "1")
+    plan_valu = R.DftPlan(g, op_kind=op_kind, coeffs=dc)
+    for dtype, tol in [(R.F64, 1e-12), (R.F32, 2e-5)]:
+        f0 = random_field(od.shape(1), 51, R.np_dtype(dtype))
+        u0 = random_field(od.shape(1), 52, R.np_dtype(dtype))
+        uref = u0.astype(np.float64)
+        O.dft_smooth(od, f0.astype(np.float64), uref, kind=R.BIQUADRATIC, op_kind=op_kind, coeffs=coeffs)
+        outs = []
+        for plan in (plan_mfma, plan_valu):
+            du = R.DeviceArray.from_host(u0)
+            plan.smooth(R.DeviceArray.from_host(f0), du)
+            outs.append(mesh.interior(du.get(), dim))
+        assert relerr(outs[0], mesh.interior(uref, dim)) <= tol, dtype
+        assert relerr(outs[0], outs[1]) <= tol, dtype
+        # apply (u := 0 first): the zero-u variant of the kernel
+        du = R.DeviceArray.from_host(u0)
+        plan_mfma.apply(R.DeviceArray.from_host(f0), du)
+        uz = np.zeros_like(uref)
+        O.dft_smooth(od, f0.astype(np.float64), uz, kind=R.BIQUADRATIC, op_kind=op_kind, coeffs=coeffs)
+        assert relerr(mesh.interior(du.get(), dim), mesh.interior(uz, dim)) <= tol
+
+
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 def test_vector_algebra(dtype):
     _print_case("vector", dtype)

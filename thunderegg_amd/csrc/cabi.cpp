@@ -3,6 +3,7 @@
 // its arguments on the host before anything is enqueued.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -86,6 +87,31 @@ int te_stream_create(te_stream* out)
 int te_stream_destroy(te_stream s)
 {
   TE_HIP(hipStreamDestroy(S(s)));
+  return TE_OK;
+}
+int te_event_create(te_event* out)
+{
+  TE_ARG(out, "null output");
+  hipEvent_t e;
+  TE_HIP(hipEventCreate(&e));
+  *out = (te_event)e;
+  return TE_OK;
+}
+int te_event_destroy(te_event e)
+{
+  TE_HIP(hipEventDestroy((hipEvent_t)e));
+  return TE_OK;
+}
+int te_event_record(te_event e, te_stream s)
+{
+  TE_HIP(hipEventRecord((hipEvent_t)e, S(s)));
+  return TE_OK;
+}
+int te_event_elapsed_ms(te_event a, te_event b, float* ms)
+{
+  TE_ARG(ms, "null output");
+  TE_HIP(hipEventSynchronize((hipEvent_t)b));
+  TE_HIP(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
   return TE_OK;
 }
 
@@ -315,6 +341,14 @@ int te_star_apply(const te_ghost_filler* g, int dtype, int neumann, void* u, voi
   return launch_star(g, dtype, dtype, dtype, 0, neumann, nullptr, nullptr, u, f, ncomp, S(s));
 }
 
+int te_star_apply_patches(const te_ghost_filler* g, int dtype, int neumann, const void* u, void* f, int ncomp,
+                          te_stream s)
+{
+  TE_ARG(g && u && f && ncomp >= 1 && ncomp <= 8, "bad argument");
+  TE_ARG(u != f, "u and f must be distinct vectors");
+  return launch_star(g, dtype, dtype, dtype, 0, neumann, nullptr, nullptr, u, f, ncomp, S(s));
+}
+
 int te_varstar_apply(const te_ghost_filler* g, int dtype, const void* coeffs, void* u, void* f, int ncomp,
                      te_stream s)
 {
@@ -390,6 +424,8 @@ int te_dft_plan_create(const te_ghost_filler* g, unsigned neumann_mask, int op_k
   pl->g = g;
   pl->mask = neumann_mask;
   pl->op_kind = op_kind;
+  const char* valu = std::getenv("TE_DFT_VALU");  // force the generic kernel (testing)
+  pl->use_mfma = (valu && valu[0] == '1') ? 0 : 1;
   std::vector<double> mats((size_t)6 * 3 * kDftMatStride, 0.0), s2((size_t)3 * 3 * kDftMatDim, 0.0);
   for (int t = 0; t < 6; t++)
     for (int a = 0; a < D; a++) dft_matrix(t, d->ns[a], &mats[((size_t)t * 3 + a) * kDftMatStride]);
@@ -465,6 +501,13 @@ int te_dft_smooth(const te_dft_plan* p, int dtype, const void* f, void* u, te_st
   TE_ARG(dtype == TE_F64 || dtype == TE_F32, "bad dtype");
   int rc = launch_ghost_fill(p->g, dtype, u, 1, S(s));
   if (rc) return rc;
+  return launch_dft(p, dtype, dtype, dtype, f, u, u, false, S(s));
+}
+
+int te_dft_patch_solve(const te_dft_plan* p, int dtype, const void* f, void* u, te_stream s)
+{
+  TE_ARG(p && f && u, "null argument");
+  TE_ARG(dtype == TE_F64 || dtype == TE_F32, "bad dtype");
   return launch_dft(p, dtype, dtype, dtype, f, u, u, false, S(s));
 }
 

@@ -37,8 +37,9 @@ struct GLevel {
   void* u = nullptr;
   void* r = nullptr;
   const void* coeffs = nullptr;      // in this level's dtype
-  const double* coeffs64 = nullptr;  // caller's FP64 coefficients
+  const double* coeffs64 = nullptr;  // FP64 coefficients (ghost-filled)
   float* coeffs32 = nullptr;         // owned when dt == F32 and VarPoisson
+  double* coeffs64_own = nullptr;    // ghost-filled copy of the caller's coefficients
 };
 
 size_t esize(int dt) { return dt == TE_F64 ? 8 : 4; }
@@ -203,6 +204,7 @@ void free_gmg(te_gmg* g)
     (void)hipFree(Lv.u);
     (void)hipFree(Lv.r);
     (void)hipFree(Lv.coeffs32);
+    (void)hipFree(Lv.coeffs64_own);
   }
   double* vs[] = { g->resid, g->rhat, g->p, g->ap, g->as, g->sv, g->ms, g->mp, g->scal, g->pa, g->pb };
   for (double* v : vs) (void)hipFree(v);
@@ -269,21 +271,34 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
         rc = TE_ERR_ARG;
         break;
       }
-      Lv.coeffs64 = coeffs[i];
+      // VarPoisson::StarPatchOperator fills the coefficient ghosts once (VarPoisson/StarPatchOperator.h:70)
+      if ((e = hipMalloc(&Lv.coeffs64_own, sizeof(double) * (n > 0 ? n : 1))) != hipSuccess ||
+          (e = hipMemcpyAsync(Lv.coeffs64_own, coeffs[i], sizeof(double) * n, hipMemcpyDeviceToDevice, g->s)) !=
+            hipSuccess) {
+        set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
+        rc = TE_ERR_HIP;
+        break;
+      }
+      if ((rc = launch_ghost_fill(Lv.gf, TE_F64, Lv.coeffs64_own, 1, g->s))) break;
+      Lv.coeffs64 = Lv.coeffs64_own;
       if (Lv.dt == TE_F64) {
-        Lv.coeffs = coeffs[i];
+        Lv.coeffs = Lv.coeffs64;
       } else {
         if ((e = hipMalloc(&Lv.coeffs32, sizeof(float) * (n > 0 ? n : 1))) != hipSuccess) {
           set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
           rc = TE_ERR_HIP;
           break;
         }
-        rc = launch_convert(n, TE_F32, Lv.coeffs32, TE_F64, coeffs[i], g->s);
+        rc = launch_convert(n, TE_F32, Lv.coeffs32, TE_F64, Lv.coeffs64, g->s);
         Lv.coeffs = Lv.coeffs32;
+      }
+      if (!rc && (e = hipStreamSynchronize(g->s)) != hipSuccess) {
+        set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
+        rc = TE_ERR_HIP;
       }
     }
     if (opts->smoother == 0 && !rc)
-      rc = te_dft_plan_create(Lv.gf, opts->neumann_mask, opts->op_kind, opts->op_kind == 1 ? coeffs[i] : nullptr,
+      rc = te_dft_plan_create(Lv.gf, opts->neumann_mask, opts->op_kind, opts->op_kind == 1 ? Lv.coeffs64 : nullptr,
                               &Lv.plan);
     if (i > 0 && !rc) rc = te_level_link_create(g->L[i - 1].d, Lv.d, &g->L[i - 1].link);
   }

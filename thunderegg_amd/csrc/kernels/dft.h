@@ -19,9 +19,13 @@ struct te_dft_plan {
   float* d_s2_32 = nullptr;
   const double* d_coeffs64 = nullptr;  // VarPoisson beta (caller-owned)
   float* d_coeffs32 = nullptr;         // plan-owned float copy
+  int use_mfma = 1;                    // matrix-core kernel where supported (2D, n 16/32, Dirichlet)
 };
 
 namespace te {
 int launch_dft(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out, bool uzero,
                hipStream_t s);
+bool dft_mfma_supported(const te_dft_plan* pl);
+int launch_dft_mfma(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
+                    bool uzero, hipStream_t s);
 }  // namespace te

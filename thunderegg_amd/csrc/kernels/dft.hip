@@ -173,6 +173,7 @@ int launch_dft(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, c
                hipStream_t s)
 {
   if (pl->g->d->np == 0) return TE_OK;
+  if (pl->use_mfma && dft_mfma_supported(pl)) return launch_dft_mfma(pl, dt, dtf, dto, f, u, out, uzero, s);
   if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64) launch_t<double, double, double>(pl, f, u, out, uzero, s);
   else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32) launch_t<float, float, float>(pl, f, u, out, uzero, s);
   else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32) launch_t<float, double, float>(pl, f, u, out, uzero, s);

@@ -1,0 +1,266 @@
+// DFTPatchSolver::solveSinglePatch (Poisson/DFTPatchSolver.h:420-450) on the
+// matrix cores, 2D, all-Dirichlet patches, n = 16 or 32.
+//
+// Each wavefront solves one patch: U = S3 . ((S . F . S^T) / lambda) . S3^T * (2/n)^2,
+// with S the DST-II and S3 the DST-III matrix (the reference's per-line dgemv along
+// x, then along y, :205-265).  Every product is a sequence of 16x16x4 MFMAs
+// (v_mfma_f32_16x16x4_f32 / v_mfma_f64_16x16x4_f64) over 16x16 tiles:
+//   Y = F S^T   F from LDS in the A-operand layout, S^T fragments from LDS;
+//   Z = S Y     Y stays in registers: the accumulator layout is used directly as
+//               the B operand by permuting the K index to match it;
+//   V = S3 Z'   (Z' = Z / lambda) the same register-resident trick;
+//   U = V S3^T  V goes through LDS once to reach the A-operand layout.
+// Lane layouts (checked on gfx950): A[i][k] at lane i + 16k, B[k][j] at lane j + 16k;
+// accumulators: f32 lane l, reg r holds D[4(l/16) + r][l%16], f64 holds D[4r + l/16][l%16].
+// The transform matrices are staged in LDS once per workgroup; a workgroup of 4
+// wavefronts walks groups of 4 patches.
+#include "../te_internal.h"
+#include "dft.h"
+
+namespace te {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mf;
+template <>
+struct Mf<float> {
+  typedef floatx4 V;
+  static __device__ __forceinline__ V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  // row of accumulator register r in lane group g
+  static __device__ __forceinline__ int row(int r, int g) { return 4 * g + r; }
+};
+template <>
+struct Mf<double> {
+  typedef doublex4 V;
+  static __device__ __forceinline__ V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int r, int g) { return 4 * r + g; }
+};
+
+constexpr int kWaves = 4;
+
+template <typename T, typename TF, typename TO, int NT, int OP, bool UZERO>
+__global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* __restrict__ dp,
+                                                               const T* __restrict__ mats, const T* __restrict__ s2tab,
+                                                               const TF* __restrict__ f, const T* __restrict__ u,
+                                                               const T* __restrict__ coeffs, TO* __restrict__ out,
+                                                               Layout L)
+{
+  constexpr int n = 16 * NT;
+  constexpr int ld = n + 1;  // padded LDS row
+  typedef typename Mf<T>::V V;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Sm = reinterpret_cast<T*>(smem_raw);  // DST-II, row-major n x n
+  T* S3 = Sm + n * n;                      // DST-III
+  T* s2 = S3 + n * n;                      // sin^2((k+1) pi / 2n), k < n
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15;
+  const int g = lane >> 4;
+  T* X = s2 + n + wave * n * ld;  // per-wave scratch n x ld
+
+  for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+    Sm[k] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
+    S3[k] = mats[(4 * 3 + 0) * kDftMatStride + k];  // DST_III, axis 0
+  }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) s2[k] = s2tab[(2 * 3 + 0) * kDftMatDim + k];
+  __syncthreads();
+
+  const int ngroups = (L.np + kWaves - 1) / kWaves;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int p = grp * kWaves + wave;
+    const bool active = p < L.np;
+    const int base = p * L.ps;
+    // ---- f_copy = f (interior) into LDS
+    if (active) {
+      for (int q = lane; q < n * n; q += 64) {
+        int y = q / n, x = q - y * n;
+        X[y * ld + x] = T(f[base + (y + 1) * L.sx + x + 1]);
+      }
+    }
+    __syncthreads();
+    // ---- modifyRHSForInternalBoundaryConditions: west/east first, then south/north
+    //      (sides in increasing order, Poisson/StarPatchOperator.h:165-180)
+    if (active && !UZERO) {
+      const DevPatch& P = dp[p];
+      for (int ph = 0; ph < 2; ph++) {
+        for (int q = lane; q < 2 * n; q += 64) {
+          int up = q / n, t = q - up * n;
+          int s = 2 * ph + up;
+          if (P.type[4 + s] < 0) continue;
+          int gi, bi;  // padded index of the ghost and of the boundary cell
+          int xi;      // LDS index of the boundary cell
+          if (ph == 0) {
+            gi = base + (t + 1) * L.sx + (up ? n + 1 : 0);
+            bi = base + (t + 1) * L.sx + (up ? n : 1);
+            xi = t * ld + (up ? n - 1 : 0);
+          } else {
+            gi = base + (up ? n + 1 : 0) * L.sx + t + 1;
+            bi = base + (up ? n : 1) * L.sx + t + 1;
+            xi = (up ? n - 1 : 0) * ld + t;
+          }
+          T ih2 = T(P.inv_h2[ph]);
+          if (OP == 0) {
+            X[xi] -= (u[gi] + u[bi]) * ih2;
+          } else {
+            int cg = gi - p * L.ps + p * L.cs, cb = bi - p * L.ps + p * L.cs;
+            X[xi] -= (u[gi] + u[bi]) * (coeffs[cb] + coeffs[cg]) * (ih2 * T(0.5));
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // ---- Y = F S^T : Y[i][j] = sum_k F[i][k] S[j][k]
+    V Y[NT][NT];
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++) {
+        V acc = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int K = 0; K < NT; K++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            int k = 16 * K + 4 * s + g;
+            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], Sm[(16 * J + c) * n + k], acc);
+          }
+        Y[I][J] = acc;
+      }
+    // ---- Z = S Y : the B operand comes from Y's registers (K index = row of register s)
+    V Z[NT][NT];
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++) {
+        V acc = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int K = 0; K < NT; K++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            int k = 16 * K + Mf<T>::row(s, g);
+            acc = Mf<T>::mma(Sm[(16 * I + c) * n + k], Y[K][J][s], acc);
+          }
+        Z[I][J] = acc;
+      }
+    // ---- divide by the eigenvalues -4/hx^2 s2[kx] - 4/hy^2 s2[ky] (getEigenValues, :320-361)
+    {
+      const DevPatch& P = dp[active ? p : 0];
+      T lx = T(4.0 / (P.h[0] * P.h[0])), ly = T(4.0 / (P.h[1] * P.h[1]));
+#pragma unroll
+      for (int I = 0; I < NT; I++)
+#pragma unroll
+        for (int J = 0; J < NT; J++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            int ky = 16 * I + Mf<T>::row(r, g), kx = 16 * J + c;
+            T eig = T(0);
+            eig -= lx * s2[kx];
+            eig -= ly * s2[ky];
+            Z[I][J][r] = Z[I][J][r] / eig;
+          }
+    }
+    // ---- V = S3 Z'
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++) {
+        V acc = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int K = 0; K < NT; K++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            int k = 16 * K + Mf<T>::row(s, g);
+            acc = Mf<T>::mma(S3[(16 * I + c) * n + k], Z[K][J][s], acc);
+          }
+        Y[I][J] = acc;  // reuse Y for V
+      }
+    // V -> LDS (accumulator layout), then U = V S3^T with V in the A layout
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) X[(16 * I + Mf<T>::row(r, g)) * ld + 16 * J + c] = Y[I][J][r];
+    __syncthreads();
+    const T scale = T(4.0 / double(n * n));  // prod over axes of 2/n (:442-449)
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++) {
+        V acc = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int K = 0; K < NT; K++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            int k = 16 * K + 4 * s + g;
+            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], S3[(16 * J + c) * n + k], acc);
+          }
+        if (active) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            int y = 16 * I + Mf<T>::row(r, g), x = 16 * J + c;
+            out[base + (y + 1) * L.sx + x + 1] = TO(acc[r] * scale);
+          }
+        }
+      }
+    __syncthreads();
+  }
+}
+
+template <typename T, typename TF, typename TO, int NT>
+void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzero, hipStream_t s)
+{
+  const te_domain* d = pl->g->d;
+  Layout L = make_layout(2, d->ns, 1, d->np);
+  constexpr int n = 16 * NT;
+  size_t shmem = sizeof(T) * (2 * n * n + n + kWaves * n * (n + 1));
+  const T* mats = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32);
+  const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
+  const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);
+  int ngroups = (d->np + kWaves - 1) / kWaves;
+  int grid = ngroups < 4096 ? ngroups : 4096;
+#define TE_MK(OP_, UZ_)                                                                                    \
+  hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, NT, OP_, UZ_>), dim3(grid), dim3(64 * kWaves), shmem, s, \
+                     d->d_patches, mats, s2, f, u, cf, out, L)
+  if (pl->op_kind == 0) {
+    if (uzero) TE_MK(0, true); else TE_MK(0, false);
+  } else {
+    if (uzero) TE_MK(1, true); else TE_MK(1, false);
+  }
+#undef TE_MK
+}
+
+template <typename T, typename TF, typename TO>
+void launch_t(const te_dft_plan* pl, const void* f, const void* u, void* out, bool uzero, hipStream_t s)
+{
+  if (pl->g->d->ns[0] == 16) launch_nt<T, TF, TO, 1>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, s);
+  else launch_nt<T, TF, TO, 2>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, s);
+}
+
+}  // namespace
+
+bool dft_mfma_supported(const te_dft_plan* pl)
+{
+  const te_domain* d = pl->g->d;
+  return d->dim == 2 && d->ns[0] == d->ns[1] && (d->ns[0] == 16 || d->ns[0] == 32) && pl->mask == 0;
+}
+
+int launch_dft_mfma(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
+                    bool uzero, hipStream_t s)
+{
+  if (pl->g->d->np == 0) return TE_OK;
+  if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64) launch_t<double, double, double>(pl, f, u, out, uzero, s);
+  else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32) launch_t<float, float, float>(pl, f, u, out, uzero, s);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32) launch_t<float, double, float>(pl, f, u, out, uzero, s);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F64) launch_t<float, double, double>(pl, f, u, out, uzero, s);
+  else {
+    set_error("unsupported dtype combination for the DFT patch solver");
+    return TE_ERR_UNSUPPORTED;
+  }
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+}  // namespace te

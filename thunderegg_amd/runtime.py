@@ -37,6 +37,10 @@ SIGNATURES = {
     "te_stream_synchronize": (C.c_int, [_vp]),
     "te_stream_create": (C.c_int, [C.POINTER(_vp)]),
     "te_stream_destroy": (C.c_int, [_vp]),
+    "te_event_create": (C.c_int, [C.POINTER(_vp)]),
+    "te_event_destroy": (C.c_int, [_vp]),
+    "te_event_record": (C.c_int, [_vp, _vp]),
+    "te_event_elapsed_ms": (C.c_int, [_vp, _vp, C.POINTER(C.c_float)]),
     "te_domain_create": (C.c_int, [C.c_int, _ip, C.c_int, C.c_int, _vp, C.POINTER(_vp)]),
     "te_domain_destroy": (C.c_int, [_vp]),
     "te_domain_num_patches": (C.c_int, [_vp]),
@@ -49,12 +53,14 @@ SIGNATURES = {
     "te_ghost_filler_destroy": (C.c_int, [_vp]),
     "te_ghost_fill": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp]),
     "te_star_apply": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp]),
+    "te_star_apply_patches": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp]),
     "te_varstar_apply": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp]),
     "te_residual": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, C.c_int, _vp]),
     "te_dft_plan_create": (C.c_int, [_vp, C.c_uint, C.c_int, _vp, C.POINTER(_vp)]),
     "te_dft_plan_destroy": (C.c_int, [_vp]),
     "te_dft_smooth": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
     "te_dft_apply": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+    "te_dft_patch_solve": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
     "te_patch_bicgstab_smooth": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_double, C.c_int,
                                            _vp, _vp, _vp, _vp]),
     "te_restrict": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp]),
@@ -314,6 +320,46 @@ class DftPlan:
 
     def apply(self, f: DeviceArray, u: DeviceArray, stream=None):
         check(lib().te_dft_apply(self.h, _dt(u), f.ptr, u.ptr, stream))
+
+
+class Stream:
+    """A HIP stream at the default priority."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().te_stream_create(C.byref(h)))
+        self.h = h.value
+
+    def sync(self):
+        check(lib().te_stream_synchronize(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().te_stream_destroy(self.h)
+            self.h = None
+
+
+class Event:
+    """A HIP event; elapsed_ms(a, b) waits for b."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().te_event_create(C.byref(h)))
+        self.h = h.value
+
+    def record(self, stream=None):
+        check(lib().te_event_record(self.h, stream.h if isinstance(stream, Stream) else stream))
+
+    @staticmethod
+    def elapsed_ms(a: "Event", b: "Event") -> float:
+        ms = C.c_float()
+        check(lib().te_event_elapsed_ms(a.h, b.h, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().te_event_destroy(self.h)
+            self.h = None
 
 
 def device_count():
