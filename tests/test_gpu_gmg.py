@@ -38,6 +38,8 @@ CASES = [
     ("uniform2d_bicg", 2, (8, 8), lambda ns: solver.uniform_levels(2, 2, ns), R.BILINEAR, 1),
     ("cross2d_biquad_dft", 2, (8, 8), lambda ns: _fixture_levels("2d_uniform_8x8_refined_cross_mpi1.json", ns)[1],
      R.BIQUADRATIC, 0),
+    ("cross2d_biquad_dft16", 2, (16, 16), lambda ns: _fixture_levels("2d_uniform_8x8_refined_cross_mpi1.json", ns)[1],
+     R.BIQUADRATIC, 0),
     ("cross2d_bilin_bicg", 2, (8, 8), lambda ns: _fixture_levels("2d_uniform_8x8_refined_cross_mpi1.json", ns)[1],
      R.BILINEAR, 1),
     ("mid3d_dft", 3, (4, 4, 4), lambda ns: _fixture_levels("3d_mid_refine_4x4x4_mpi1.json", ns)[1], R.TRILINEAR, 0),
@@ -198,3 +200,32 @@ def test_varpoisson_gmg():
     ax = O.varpoisson_apply(ods[0], coeffs[0], xm.copy(), kind=R.BILINEAR)
     r = mesh.interior(f - ax, dim)
     assert np.sqrt((r * r).sum()) / np.sqrt((mesh.interior(f, dim) ** 2).sum()) <= 1.5e-10
+
+
+@pytest.mark.parametrize("filler", [R.BILINEAR, R.BIQUADRATIC])
+@pytest.mark.parametrize("prec", [R.F64, R.F32])
+@pytest.mark.parametrize("op_kind", [0, 1])
+def test_fused_vcycle_matches_unfused(monkeypatch, filler, prec, op_kind):
+    """The fused 2D launches (ghosts computed inside the smoother / residual-restriction /
+    outer-operator kernels) give the same V-cycle and solve as one kernel per step."""
+    _print_case("fused-vs-unfused", filler, prec, op_kind)
+    ns = (16, 16)
+    dim, levels = load("2d_uniform_8x8_refined_cross_mpi1.json", ns)
+    coeffs = None
+    if op_kind == 1:
+        coeffs = []
+        for lv in levels:
+            X = mesh.cell_centers(lv, dim, ns)
+            coeffs.append(np.exp(np.sin(3 * X[..., 0]) * np.cos(4 * X[..., 1]))[:, None].copy())
+    kw = dict(filler_kind=filler, smoother=0, op_kind=op_kind, coeffs=coeffs, precision=prec)
+    gf = solver.GMG(dim, ns, levels, **kw)
+    monkeypatch.setenv("TE_NO_FUSE", "1")
+    gu = solver.GMG(dim, ns, levels, **kw)
+    r0 = random_field(mesh.padded_shape(len(levels[0]), dim, ns), 23)
+    tol = 1e-12 if prec == R.F64 else 2e-5
+    assert relerr(mesh.interior(gf.vcycle(r0), dim), mesh.interior(gu.vcycle(r0), dim)) <= tol
+    f = _rhs_sin(levels, dim, ns)
+    xf, itf, _ = gf.solve(f, tol=1e-10, precond=2)
+    xu, itu, _ = gu.solve(f, tol=1e-10, precond=2)
+    assert abs(itf - itu) <= (0 if prec == R.F64 and op_kind == 0 else 2)
+    assert relerr(mesh.interior(xf, dim), mesh.interior(xu, dim)) <= 1e-8

@@ -24,6 +24,7 @@ SOURCES = [
     "kernels/blas.hip",
     "kernels/dft.hip",
     "kernels/dft_mfma.hip",
+    "kernels/fused2d.hip",
     "kernels/patch_bicgstab.hip",
     "kernels/krylov.hip",
 ]

@@ -13,6 +13,7 @@
 //    scalars on the device; the host reads ||s|| and ||r|| once each per iteration.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -36,6 +37,8 @@ struct GLevel {
   void* f = nullptr;
   void* u = nullptr;
   void* r = nullptr;
+  void* u2 = nullptr;  // second solution buffer (fused path ping-pongs between u and u2)
+  void* cur = nullptr; // buffer holding this level's solution after its visit
   const void* coeffs = nullptr;      // in this level's dtype
   const double* coeffs64 = nullptr;  // FP64 coefficients (ghost-filled)
   float* coeffs32 = nullptr;         // owned when dt == F32 and VarPoisson
@@ -59,6 +62,8 @@ struct te_gmg {
   std::map<std::pair<const void*, const void*>, hipGraphExec_t> vgraphs;
   std::map<std::pair<const void*, const void*>, std::pair<hipGraphExec_t, hipGraphExec_t>> bgraphs;
   double last_seconds = 0.0;
+  bool fused = false;  // 2D matrix-core path: ghost fills fused into the stencil / smoother launches
+  bool fly = false;    // outer operator computes ghosts on the fly
 };
 
 namespace {
@@ -135,10 +140,77 @@ int visit(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
   return TE_OK;
 }
 
+// VCycle::visit on the fused path: per level, one launch per smoothing sweep and one
+// for Cycle::restrict (ghost fill + residual + restriction); interpolation and the
+// ghost fill of the first post-smoothing sweep happen inside that sweep's launch.
+// Sweeps after the first pre-smoothing one read the solution from one buffer and
+// write the other; the finest level's last sweep writes the caller's FP64 vector.
+int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
+{
+  GLevel& Lv = g->L[li];
+  const int nl = (int)g->L.size();
+  const bool finest = li == 0;
+  void* buf[2] = { Lv.u, Lv.u2 };
+  int cur = 0;
+  // one sweep from buf[cur]: gm 0 = u known zero, 1 = ghosts on the fly, 2 = with interpolation
+  auto sweep = [&](int mode, bool last_of_level) -> int {
+    const bool to_ext = finest && last_of_level;
+    void* out = to_ext ? (void*)u_ext : buf[cur ^ 1];
+    int dto = to_ext ? TE_F64 : Lv.dt;
+    const void* uc = nullptr;
+    int dtc = TE_F64;
+    if (mode == 2) {
+      uc = g->L[li + 1].cur;
+      dtc = g->L[li + 1].dt;
+    }
+    int rc = launch_dft_mfma_fused(Lv.plan, Lv.dt, dtf, dto, f, buf[cur], out, mode == 0, mode == 0 ? 0 : mode,
+                                   dtc, uc, mode == 2 ? Lv.link : nullptr, g->s);
+    if (rc) return rc;
+    if (to_ext) {
+      Lv.cur = u_ext;
+    } else {
+      cur ^= 1;
+      Lv.cur = buf[cur];
+    }
+    return TE_OK;
+  };
+  if (li == nl - 1) {
+    int n = g->o.coarse_sweeps;
+    if (n == 0) {
+      TE_RC(zero_with_ghost(g, li, buf[0], Lv.dt));
+      Lv.cur = buf[0];
+      if (finest) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), TE_F64, u_ext, Lv.dt, buf[0], g->s));
+      return TE_OK;
+    }
+    for (int k = 0; k < n; k++) TE_RC(sweep(k == 0 ? 0 : 1, k == n - 1));
+    return TE_OK;
+  }
+  for (int k = 0; k < g->o.pre_sweeps; k++) TE_RC(sweep(k == 0 ? 0 : 1, false));
+  if (g->o.pre_sweeps == 0) {
+    TE_RC(zero_with_ghost(g, li, buf[0], Lv.dt));
+    Lv.cur = buf[0];
+  }
+  GLevel& Cv = g->L[li + 1];
+  TE_RC(launch_resid_restrict(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur], Cv.dt,
+                              Cv.f, g->s));
+  TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext));
+  int n = g->o.post_sweeps;
+  if (n == 0) {
+    // DirectInterpolator alone (GMG/DirectInterpolator.h:60-91)
+    TE_RC(launch_interpolate(Lv.link, Cv.dt, Cv.cur, Lv.dt, buf[cur], 1, g->s));
+    Lv.cur = buf[cur];
+    if (finest) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), TE_F64, u_ext, Lv.dt, buf[cur], g->s));
+    return TE_OK;
+  }
+  for (int k = 0; k < n; k++) TE_RC(sweep(k == 0 ? 2 : 1, k == n - 1));
+  return TE_OK;
+}
+
 // Cycle::apply (GMG/Cycle.h:94-98) enqueued on g->s.
 int vcycle_enqueue(te_gmg* g, const double* f, double* u)
 {
   GLevel& L0 = g->L[0];
+  if (g->fused) return visit_fused(g, 0, f, TE_F64, u);
   if (L0.dt != TE_F64 && g->o.smoother != 0) {
     // the inner-Krylov smoother works on same-dtype f and u: convert f once
     TE_RC(launch_convert(te_domain_vector_length(L0.d, 1), L0.dt, L0.f, TE_F64, f, g->s));
@@ -203,6 +275,7 @@ void free_gmg(te_gmg* g)
     (void)hipFree(Lv.f);
     (void)hipFree(Lv.u);
     (void)hipFree(Lv.r);
+    (void)hipFree(Lv.u2);
     (void)hipFree(Lv.coeffs32);
     (void)hipFree(Lv.coeffs64_own);
   }
@@ -257,7 +330,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     long n = te_domain_vector_length(Lv.d, 1);
     size_t bytes = esize(Lv.dt) * (size_t)(n > 0 ? n : 1);
     if ((e = hipMalloc(&Lv.f, bytes)) != hipSuccess || (e = hipMalloc(&Lv.u, bytes)) != hipSuccess ||
-        (e = hipMalloc(&Lv.r, bytes)) != hipSuccess) {
+        (e = hipMalloc(&Lv.r, bytes)) != hipSuccess || (e = hipMalloc(&Lv.u2, bytes)) != hipSuccess) {
       set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
       rc = TE_ERR_HIP;
       break;
@@ -265,6 +338,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     (void)hipMemset(Lv.f, 0, bytes);
     (void)hipMemset(Lv.u, 0, bytes);
     (void)hipMemset(Lv.r, 0, bytes);
+    (void)hipMemset(Lv.u2, 0, bytes);
     if (opts->op_kind == 1) {
       if (!coeffs[i]) {
         set_error("te_gmg_create: missing coefficients for a level");
@@ -301,6 +375,20 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
       rc = te_dft_plan_create(Lv.gf, opts->neumann_mask, opts->op_kind, opts->op_kind == 1 ? Lv.coeffs64 : nullptr,
                               &Lv.plan);
     if (i > 0 && !rc) rc = te_level_link_create(g->L[i - 1].d, Lv.d, &g->L[i - 1].link);
+  }
+  if (!rc) {
+    // Fused launches where every level supports them; TE_NO_FUSE=1 keeps the
+    // one-kernel-per-step path (same arithmetic, for comparison and timing).
+    const char* nf = std::getenv("TE_NO_FUSE");
+    const bool allow = !(nf && nf[0] == '1');
+    bool fused = allow && opts->smoother == 0;
+    for (int i = 0; i < num_levels && fused; i++) {
+      const GLevel& Lv = g->L[i];
+      fused = Lv.plan && Lv.plan->use_mfma && dft_mfma_supported(Lv.plan) &&
+              (i == num_levels - 1 || resid_restrict_fused_supported(Lv.gf));
+    }
+    g->fused = fused;
+    g->fly = allow && star_dot_fly_supported(g->L[0].gf);
   }
   if (!rc) {
     const te_domain* d0 = levels[0];
@@ -377,8 +465,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // phase 1: mp = M p; ap = A mp (+ rhat.ap); alpha; s = r - alpha ap; ||s||
   auto phase1 = [&]() -> int {
     if (use_precond) TE_RC(vcycle_enqueue(g, g->p, g->mp));
-    TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
-    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, g->ap, g->rhat, g->pa, nullptr, s));
+    if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
+    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
     TE_RC(launch_bicg_finalize(FIN_ALPHA, g->pa, nullptr, nb, g->scal, s));
     TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pa, nullptr,
                       s));
@@ -387,8 +475,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // phase 2: ms = M s; as = A ms (+ as.s, as.as); omega; x, r updates; rho, beta, ||r||; p update
   auto phase2 = [&]() -> int {
     if (use_precond) TE_RC(vcycle_enqueue(g, g->sv, g->ms));
-    TE_RC(launch_ghost_fill(gf, TE_F64, ms, 1, s));
-    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, g->as, g->sv, g->pa, g->pb, s));
+    if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, ms, 1, s));
+    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, g->as, g->sv, g->pa, g->pb, g->fly, s));
     TE_RC(launch_bicg_finalize(FIN_OMEGA, g->pa, g->pb, nb, g->scal, s));
     TE_RC(launch_bicg(BICG_XR, d, g->scal, x, g->resid, mp, ms, g->ap, g->as, g->rhat, g->pa, g->pb, s));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s));

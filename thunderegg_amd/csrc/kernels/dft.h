@@ -28,4 +28,12 @@ int launch_dft(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, c
 bool dft_mfma_supported(const te_dft_plan* pl);
 int launch_dft_mfma(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
                     bool uzero, hipStream_t s);
+// gm: 0 ghosts read from u, 1 ghosts computed on the fly, 2 on the fly from u + P uc
+// (uc: the coarser level's solution of dtype dtc, link: fine -> coarse).
+int launch_dft_mfma_fused(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
+                          bool uzero, int gm, int dtc, const void* uc, const te_level_link* link, hipStream_t s);
+bool resid_restrict_fused_supported(const te_ghost_filler* g);
+int launch_resid_restrict(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
+                          const void* coeffs, int dt, int dtf, const void* f, const void* u, int dtc, void* fc,
+                          hipStream_t s);
 }  // namespace te

@@ -12,10 +12,18 @@
 //   U = V S3^T  V goes through LDS once to reach the A-operand layout.
 // Lane layouts (checked on gfx950): A[i][k] at lane i + 16k, B[k][j] at lane j + 16k;
 // accumulators: f32 lane l, reg r holds D[4(l/16) + r][l%16], f64 holds D[4r + l/16][l%16].
-// The transform matrices are staged in LDS once per workgroup; a workgroup of 4
-// wavefronts walks groups of 4 patches.
+//
+// The interface terms of modifyRHSForInternalBoundaryConditions need u's ghost
+// values on sides with neighbours.  GM selects where they come from:
+//   GM_MEM    read from u's ghost cells (filled by a preceding ghost fill);
+//   GM_FLY    computed here from the neighbours' interior values (fused ghost fill);
+//   GM_INTERP as GM_FLY on u + P uc, the DirectInterpolator correction from the
+//             coarser level (GMG/DirectInterpolator.h:60-91) applied on the fly, so
+//             interpolation, ghost fill and the post-smoothing sweep are one launch
+//             (the result must then go to a buffer other than u).
 #include "../te_internal.h"
 #include "dft.h"
+#include "ghost2d.h"
 
 namespace te {
 namespace {
@@ -41,12 +49,43 @@ struct Mf<double> {
 
 constexpr int kWaves = 4;
 
-template <typename T, typename TF, typename TO, int NT, int OP, bool UZERO>
+// u (+ injection of the coarser level's uc) at interior cell (x, y) of patch q
+template <typename T, typename TC, bool INTERP>
+struct UAcc {
+  const T* u;
+  const TC* uc;
+  const int32_t* parent;
+  const int32_t* porth;
+  Layout L;
+  __device__ __forceinline__ T operator()(int q, int x, int y) const
+  {
+    T v = u[q * L.ps + (y + 1) * L.sx + x + 1];
+    if (INTERP) {
+      int o = porth[q];
+      int cx = x, cy = y;
+      if (o >= 0) {
+        cx = (x + ((o & 1) ? L.n[0] : 0)) / 2;
+        cy = (y + ((o & 2) ? L.n[1] : 0)) / 2;
+      }
+      v = T(v + T(uc[parent[q] * L.ps + (cy + 1) * L.sx + cx + 1]));
+    }
+    return v;
+  }
+};
+
+template <typename T, typename TC>
+struct FusedArgs {
+  const TC* uc;
+  const int32_t* parent;
+  const int32_t* porth;
+};
+
+template <typename T, typename TF, typename TO, typename TC, int NT, int OP, bool UZERO, int GM, int KIND>
 __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* __restrict__ dp,
                                                                const T* __restrict__ mats, const T* __restrict__ s2tab,
                                                                const TF* __restrict__ f, const T* __restrict__ u,
                                                                const T* __restrict__ coeffs, TO* __restrict__ out,
-                                                               Layout L)
+                                                               Layout L, FusedArgs<T, TC> fa)
 {
   constexpr int n = 16 * NT;
   constexpr int ld = n + 1;  // padded LDS row
@@ -60,6 +99,7 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
   const int c = lane & 15;
   const int g = lane >> 4;
   T* X = s2 + n + wave * n * ld;  // per-wave scratch n x ld
+  UAcc<T, TC, GM == 2> U{ u, fa.uc, fa.parent, fa.porth, L };
 
   for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
     Sm[k] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
@@ -92,21 +132,34 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
           if (P.type[4 + s] < 0) continue;
           int gi, bi;  // padded index of the ghost and of the boundary cell
           int xi;      // LDS index of the boundary cell
+          int bx, by;  // boundary cell
           if (ph == 0) {
             gi = base + (t + 1) * L.sx + (up ? n + 1 : 0);
             bi = base + (t + 1) * L.sx + (up ? n : 1);
             xi = t * ld + (up ? n - 1 : 0);
+            bx = up ? n - 1 : 0;
+            by = t;
           } else {
             gi = base + (up ? n + 1 : 0) * L.sx + t + 1;
             bi = base + (up ? n : 1) * L.sx + t + 1;
             xi = (up ? n - 1 : 0) * ld + t;
+            bx = t;
+            by = up ? n - 1 : 0;
+          }
+          T ug, ub;
+          if (GM == 0) {
+            ug = u[gi];
+            ub = u[bi];
+          } else {
+            ug = ghost2d<T, KIND>(P, p, s, t, n, n, U);
+            ub = U(p, bx, by);
           }
           T ih2 = T(P.inv_h2[ph]);
           if (OP == 0) {
-            X[xi] -= (u[gi] + u[bi]) * ih2;
+            X[xi] -= (ug + ub) * ih2;
           } else {
             int cg = gi - p * L.ps + p * L.cs, cb = bi - p * L.ps + p * L.cs;
-            X[xi] -= (u[gi] + u[bi]) * (coeffs[cb] + coeffs[cg]) * (ih2 * T(0.5));
+            X[xi] -= (ug + ub) * (coeffs[cb] + coeffs[cg]) * (ih2 * T(0.5));
           }
         }
         __syncthreads();
@@ -209,8 +262,9 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
   }
 }
 
-template <typename T, typename TF, typename TO, int NT>
-void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzero, hipStream_t s)
+template <typename T, typename TF, typename TO, typename TC, int NT>
+void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzero, int gm, const TC* uc,
+               const te_level_link* link, hipStream_t s)
 {
   const te_domain* d = pl->g->d;
   Layout L = make_layout(2, d->ns, 1, d->np);
@@ -219,24 +273,37 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   const T* mats = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32);
   const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
   const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);
+  FusedArgs<T, TC> fa{ uc, link ? link->d_parent : nullptr, link ? link->d_orth : nullptr };
   int ngroups = (d->np + kWaves - 1) / kWaves;
   int grid = ngroups < 4096 ? ngroups : 4096;
-#define TE_MK(OP_, UZ_)                                                                                    \
-  hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, NT, OP_, UZ_>), dim3(grid), dim3(64 * kWaves), shmem, s, \
-                     d->d_patches, mats, s2, f, u, cf, out, L)
+  const int kind = pl->g->kind;
+#define TE_MK(OP_, UZ_, GM_, K_)                                                                            \
+  hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_>), dim3(grid), dim3(64 * kWaves), \
+                     shmem, s, d->d_patches, mats, s2, f, u, cf, out, L, fa)
+#define TE_MKG(OP_)                                                         \
+  if (uzero) TE_MK(OP_, true, 0, 0);                                        \
+  else if (gm == 0) TE_MK(OP_, false, 0, 0);                                \
+  else if (gm == 1 && kind == TE_BILINEAR) TE_MK(OP_, false, 1, TE_BILINEAR); \
+  else if (gm == 1) TE_MK(OP_, false, 1, TE_BIQUADRATIC);                   \
+  else if (kind == TE_BILINEAR) TE_MK(OP_, false, 2, TE_BILINEAR);          \
+  else TE_MK(OP_, false, 2, TE_BIQUADRATIC);
   if (pl->op_kind == 0) {
-    if (uzero) TE_MK(0, true); else TE_MK(0, false);
+    TE_MKG(0)
   } else {
-    if (uzero) TE_MK(1, true); else TE_MK(1, false);
+    TE_MKG(1)
   }
+#undef TE_MKG
 #undef TE_MK
 }
 
-template <typename T, typename TF, typename TO>
-void launch_t(const te_dft_plan* pl, const void* f, const void* u, void* out, bool uzero, hipStream_t s)
+template <typename T, typename TF, typename TO, typename TC>
+void launch_t(const te_dft_plan* pl, const void* f, const void* u, void* out, bool uzero, int gm, const void* uc,
+              const te_level_link* link, hipStream_t s)
 {
-  if (pl->g->d->ns[0] == 16) launch_nt<T, TF, TO, 1>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, s);
-  else launch_nt<T, TF, TO, 2>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, s);
+  if (pl->g->d->ns[0] == 16)
+    launch_nt<T, TF, TO, TC, 1>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, gm, (const TC*)uc, link, s);
+  else
+    launch_nt<T, TF, TO, TC, 2>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, gm, (const TC*)uc, link, s);
 }
 
 }  // namespace
@@ -250,15 +317,40 @@ bool dft_mfma_supported(const te_dft_plan* pl)
 int launch_dft_mfma(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
                     bool uzero, hipStream_t s)
 {
+  return launch_dft_mfma_fused(pl, dt, dtf, dto, f, u, out, uzero, 0, TE_F64, nullptr, nullptr, s);
+}
+
+int launch_dft_mfma_fused(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
+                          bool uzero, int gm, int dtc, const void* uc, const te_level_link* link, hipStream_t s)
+{
   if (pl->g->d->np == 0) return TE_OK;
-  if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64) launch_t<double, double, double>(pl, f, u, out, uzero, s);
-  else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32) launch_t<float, float, float>(pl, f, u, out, uzero, s);
-  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32) launch_t<float, double, float>(pl, f, u, out, uzero, s);
-  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F64) launch_t<float, double, double>(pl, f, u, out, uzero, s);
-  else {
+  if (gm == 2 && (!uc || !link)) {
+    set_error("fused interpolation needs the coarser level's u and the level link");
+    return TE_ERR_ARG;
+  }
+  const bool c64 = gm != 2 || dtc == TE_F64;
+#define TE_LT(T_, TF_, TO_)                                                                      \
+  do {                                                                                           \
+    if (c64) launch_t<T_, TF_, TO_, double>(pl, f, u, out, uzero, gm, uc, link, s);              \
+    else launch_t<T_, TF_, TO_, float>(pl, f, u, out, uzero, gm, uc, link, s);                   \
+  } while (0)
+  if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64) {
+    if (!c64) {
+      set_error("FP64 level with an FP32 coarser level is not supported");
+      return TE_ERR_UNSUPPORTED;
+    }
+    launch_t<double, double, double, double>(pl, f, u, out, uzero, gm, uc, link, s);
+  } else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32) {
+    TE_LT(float, float, float);
+  } else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32) {
+    TE_LT(float, double, float);
+  } else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F64) {
+    TE_LT(float, double, double);
+  } else {
     set_error("unsupported dtype combination for the DFT patch solver");
     return TE_ERR_UNSUPPORTED;
   }
+#undef TE_LT
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

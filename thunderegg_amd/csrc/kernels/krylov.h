@@ -10,8 +10,9 @@ enum BicgMode { BICG_INIT = 0, BICG_S, BICG_XP, BICG_XR, BICG_P };
 enum BicgFinalize { FIN_INIT = 0, FIN_ALPHA, FIN_SNORM, FIN_OMEGA, FIN_RHO };
 
 int krylov_blocks(const te_domain* d);
+bool star_dot_fly_supported(const te_ghost_filler* g);
 int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
-                    double* out, const double* w, double* pa, double* pb, hipStream_t s);
+                    double* out, const double* w, double* pa, double* pb, bool fly, hipStream_t s);
 int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const double* a0,
                 const double* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
                 hipStream_t s);

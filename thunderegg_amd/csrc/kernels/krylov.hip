@@ -1,79 +1,140 @@
 // Outer FP64 BiCGStab (Iterative/BiCGStab.h:111-203) as fused device kernels.
 // The scalars (rho, alpha, omega, beta and the norms) live in a small device
-// array; each vector kernel also emits one partial sum per block for the next
-// inner product, and a single-block finalize kernel folds the partials in a
-// fixed order and updates the scalars.  The host only reads the two norms
-// that decide termination.
+// array; each vector kernel also emits one partial sum per workgroup for the
+// next inner product, and a single-workgroup finalize kernel folds the
+// partials in a fixed order and updates the scalars.  The host only reads the
+// two norms that decide termination.
+//
+// The vector kernels are patch-blocked: a workgroup walks whole patches, and
+// each thread precomputes the padded offsets of the cells it owns in a patch,
+// so the per-element work is a load, the arithmetic and a store.
 #include "../te_internal.h"
 #include "blas.h"
+#include "ghost2d.h"
 #include "krylov.h"
 
 namespace te {
 namespace {
 
-__device__ __forceinline__ bool cell_of(const Layout& L, int idx, int* p_out, int* q_out, int* c3)
+constexpr int kThreads = 256;
+constexpr int kMaxCells = 24;  // largest per-thread cell count instantiated (3D n = 16: 18^3 / 256 -> 23)
+
+// The cells a thread owns in every patch (at most CPT; unrolled so they stay in registers).
+template <int CPT>
+struct CellSet {
+  int n;              // cells of this thread
+  int off[CPT];       // padded offset in the patch
+  unsigned bnd[CPT];  // bit 2a: lower boundary on axis a, bit 2a+1: upper; bit 7: ghost cell
+};
+
+template <int CPT>
+__device__ __forceinline__ void make_cells(const Layout& L, bool with_ghosts, CellSet<CPT>& cs)
 {
-  int p = idx / L.ps;
-  int rem = idx - p * L.ps;
-  int c = rem / L.cs;
-  int q = rem - c * L.cs;
-  *p_out = p;
-  *q_out = q;
-  int x, y, z = 0;
-  if (L.dim == 2) {
-    y = q / L.sx;
-    x = q - y * L.sx;
-  } else {
-    z = q / L.sy;
-    int r2 = q - z * L.sy;
-    y = r2 / L.sx;
-    x = r2 - y * L.sx;
-    z -= 1;
+  cs.n = 0;
+  const int n0 = L.n[0], n1 = L.n[1], n2 = L.dim == 3 ? L.n[2] : 1;
+  const int p0 = n0 + 2, p1 = n1 + 2, p2 = L.dim == 3 ? n2 + 2 : 1;
+  const int total = with_ghosts ? p0 * p1 * p2 : n0 * n1 * n2;
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    const int o = threadIdx.x + k * blockDim.x;
+    cs.off[k] = 0;
+    cs.bnd[k] = 0;
+    if (o >= total) continue;
+    int x, y, z;
+    if (with_ghosts) {
+      x = o % p0 - 1;
+      y = (o / p0) % p1 - 1;
+      z = L.dim == 3 ? o / (p0 * p1) - 1 : 0;
+    } else {
+      x = o % n0;
+      y = (o / n0) % n1;
+      z = L.dim == 3 ? o / (n0 * n1) : 0;
+    }
+    bool ghost = x < 0 || x >= n0 || y < 0 || y >= n1 || (L.dim == 3 && (z < 0 || z >= n2));
+    unsigned char b = ghost ? 0x80 : 0;
+    if (!ghost) {
+      b |= (x == 0 ? 1 : 0) | (x == n0 - 1 ? 2 : 0) | (y == 0 ? 4 : 0) | (y == n1 - 1 ? 8 : 0);
+      if (L.dim == 3) b |= (z == 0 ? 16 : 0) | (z == n2 - 1 ? 32 : 0);
+    }
+    cs.off[k] = (x + 1) + (y + 1) * L.sx + (L.dim == 3 ? (z + 1) * L.sy : 0);
+    cs.bnd[k] = b;
+    cs.n = k + 1;
   }
-  x -= 1;
-  y -= 1;
-  c3[0] = x;
-  c3[1] = y;
-  c3[2] = z;
-  return x >= 0 && x < L.n[0] && y >= 0 && y < L.n[1] && (L.dim == 2 || (z >= 0 && z < L.n[2]));
 }
 
-// out = A u (ghosts 0), with optional fused sums  pa = sum out*w,  pb = sum out*out.
-template <int D, int OP, bool WDOT, bool SQ>
-__global__ void __launch_bounds__(256) star_dot_kernel(const DevPatch* __restrict__ dp, const double* __restrict__ u,
-                                                       const double* __restrict__ cf, double* __restrict__ out,
-                                                       const double* __restrict__ w, double* __restrict__ pa,
-                                                       double* __restrict__ pb, Layout L, int neumann)
+struct PlainAcc64 {
+  const double* u;
+  Layout L;
+  __device__ __forceinline__ double operator()(int q, int x, int y) const
+  {
+    return u[q * L.ps + (y + 1) * L.sx + x + 1];
+  }
+};
+
+// out = A u (ghosts of out written 0), with optional fused sums pa = sum w*out, pb = sum out*out.
+// FK >= 0 (2D): neighbour ghost values are computed on the fly with ghost filler kind FK
+// instead of being read from u (so no separate ghost fill precedes the apply).
+template <int CPT, int D, int OP, bool WDOT, bool SQ, int FK>
+__global__ void __launch_bounds__(kThreads) star_dot_kernel(const DevPatch* __restrict__ dp,
+                                                            const double* __restrict__ u,
+                                                            const double* __restrict__ cf, double* __restrict__ out,
+                                                            const double* __restrict__ w, double* __restrict__ pa,
+                                                            double* __restrict__ pb, Layout L, int neumann)
 {
-  long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long total = (long)L.np * L.ps;
+  CellSet<CPT> cs;
+  make_cells<CPT>(L, true, cs);
+  const int st[3] = { 1, L.sx, L.sy };
+  const int sb = D == 2 ? 4 : 20;
+  PlainAcc64 U{ u, L };
   double sa = 0.0, sq = 0.0;
-  if (gid < total) {
-    int idx = (int)gid, p, q, c3[3];
-    if (!cell_of(L, idx, &p, &q, c3)) {
-      out[idx] = 0.0;
-    } else {
-      const DevPatch& P = dp[p];
-      double mid = u[idx];
+  for (int p = blockIdx.x; p < L.np; p += gridDim.x) {
+    const DevPatch& P = dp[p];
+    unsigned phys = 0;  // bit s: side s has no neighbour
+#pragma unroll
+    for (int s = 0; s < 2 * D; s++) phys |= (P.type[sb + s] < 0 ? 1u : 0u) << s;
+    double ih2[3] = { P.inv_h2[0], P.inv_h2[1], P.inv_h2[2] };
+    const int base = p * L.ps;
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+      if (k >= cs.n) continue;
+      const int i = base + cs.off[k];
+      const unsigned b = cs.bnd[k];
+      if (b & 0x80) {
+        out[i] = 0.0;
+        continue;
+      }
+      double mid = u[i];
       double bc = (OP == 0 && neumann) ? mid : -mid;
-      const int st[3] = { 1, L.sx, L.sy };
       double acc = 0.0;
 #pragma unroll
       for (int a = 0; a < D; a++) {
-        bool lo_phys = c3[a] == 0 && P.type[(D == 2 ? 4 : 20) + 2 * a] < 0;
-        bool hi_phys = c3[a] == L.n[a] - 1 && P.type[(D == 2 ? 4 : 20) + 2 * a + 1] < 0;
-        double lower = lo_phys ? bc : u[idx - st[a]];
-        double upper = hi_phys ? bc : u[idx + st[a]];
-        if (OP == 0) {
-          acc += (upper - 2.0 * mid + lower) * P.inv_h2[a];
+        double lower, upper;
+        if (FK >= 0 && D == 2 && (b & (3u << (2 * a)))) {
+          const int yy = cs.off[k] / L.sx - 1, xx = cs.off[k] % L.sx - 1;
+          const int na = L.n[a], nt = L.n[1 - a], t = a == 0 ? yy : xx;
+          if ((b >> (2 * a)) & 1)
+            lower = ((phys >> (2 * a)) & 1) ? bc : ghost2d<double, FK < 0 ? 0 : FK>(P, p, 2 * a, t, na, nt, U);
+          else
+            lower = u[i - st[a]];
+          if ((b >> (2 * a + 1)) & 1)
+            upper = ((phys >> (2 * a + 1)) & 1) ? bc : ghost2d<double, FK < 0 ? 0 : FK>(P, p, 2 * a + 1, t, na, nt, U);
+          else
+            upper = u[i + st[a]];
         } else {
-          double cm = cf[p * L.cs + q], cl = cf[p * L.cs + q - st[a]], cu = cf[p * L.cs + q + st[a]];
-          acc += ((cu + cm) * (upper - mid) - (cl + cm) * (mid - lower)) * (P.inv_h2[a] * 0.5);
+          lower = ((b >> (2 * a)) & 1) && ((phys >> (2 * a)) & 1) ? bc : u[i - st[a]];
+          upper = ((b >> (2 * a + 1)) & 1) && ((phys >> (2 * a + 1)) & 1) ? bc : u[i + st[a]];
+        }
+        if (OP == 0) {
+          acc += (upper - 2.0 * mid + lower) * ih2[a];
+        } else {
+          const int ci = p * L.cs + cs.off[k];
+          double cm = cf[ci], cl = cf[ci - st[a]], cu = cf[ci + st[a]];
+          acc += ((cu + cm) * (upper - mid) - (cl + cm) * (mid - lower)) * (ih2[a] * 0.5);
         }
       }
-      out[idx] = acc;
-      if (WDOT) sa = w[idx] * acc;  // rhat.dot(ap) / as.dot(s): w first, as the reference
-      if (SQ) sq = acc * acc;
+      out[i] = acc;
+      if (WDOT) sa += w[i] * acc;  // rhat.dot(ap) / as.dot(s): w first, as the reference
+      if (SQ) sq += acc * acc;
     }
   }
   if (WDOT) {
@@ -87,48 +148,55 @@ __global__ void __launch_bounds__(256) star_dot_kernel(const DevPatch* __restric
 }
 
 // Vector updates of one BiCGStab iteration (interior cells only).
-template <int MODE>
-__global__ void __launch_bounds__(256) bicg_kernel(Layout L, const double* __restrict__ scal, double* __restrict__ v0,
-                                                   double* __restrict__ v1, const double* __restrict__ a0,
-                                                   const double* __restrict__ a1, const double* __restrict__ a2,
-                                                   const double* __restrict__ a3, const double* __restrict__ a4,
-                                                   double* __restrict__ pa, double* __restrict__ pb)
+template <int CPT, int MODE>
+__global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* __restrict__ scal,
+                                                        double* __restrict__ v0, double* __restrict__ v1,
+                                                        const double* __restrict__ a0,
+                                                        const double* __restrict__ a1,
+                                                        const double* __restrict__ a2,
+                                                        const double* __restrict__ a3,
+                                                        const double* __restrict__ a4, double* __restrict__ pa,
+                                                        double* __restrict__ pb)
 {
-  long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long total = (long)L.np * L.ps;
+  CellSet<CPT> cs;
+  make_cells<CPT>(L, false, cs);
   double sa = 0.0, sb = 0.0;
-  int p, q, c3[3];
-  if (gid < total && cell_of(L, (int)gid, &p, &q, c3)) {
-    int i = (int)gid;
-    if (MODE == BICG_INIT) {
-      // v0 = rhat, v1 = p, a0 = r, a1 = b: rhat = r; p = r; sums b.b and rhat.r
-      double r = a0[i];
-      v0[i] = r;
-      v1[i] = r;
-      sa = a1[i] * a1[i];
-      sb = r * r;
-    } else if (MODE == BICG_S) {
-      // v0 = s, a0 = r, a1 = ap: s = r; s += ap * -alpha; sum s.s
-      double s = a0[i];
-      s += a1[i] * -scal[SC_ALPHA];
-      v0[i] = s;
-      sa = s * s;
-    } else if (MODE == BICG_XP) {
-      // v0 = x, a0 = p: x += p * alpha (early exit, BiCGStab.h:162-167)
-      v0[i] += a0[i] * scal[SC_ALPHA];
-    } else if (MODE == BICG_XR) {
-      // v0 = x, v1 = r, a0 = mp, a1 = ms, a2 = ap, a3 = as, a4 = rhat
-      double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA];
-      v0[i] += a0[i] * alpha + a1[i] * omega;
-      double r = v1[i];
-      r += a2[i] * -alpha + a3[i] * -omega;
-      v1[i] = r;
-      sa = r * a4[i];
-      sb = r * r;
-    } else {  // BICG_P: v0 = p, a0 = ap, a1 = r
-      double pv = v0[i];
-      pv += a0[i] * -scal[SC_OMEGA];
-      v0[i] = pv * scal[SC_BETA] + a1[i];
+  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], beta = scal[SC_BETA];
+  for (int p = blockIdx.x; p < L.np; p += gridDim.x) {
+    const int base = p * L.ps;
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+      if (k >= cs.n) continue;
+      const int i = base + cs.off[k];
+      if (MODE == BICG_INIT) {
+        // v0 = rhat, v1 = p, a0 = r, a1 = b: rhat = r; p = r; sums b.b and rhat.r
+        double r = a0[i];
+        v0[i] = r;
+        v1[i] = r;
+        sa += a1[i] * a1[i];
+        sb += r * r;
+      } else if (MODE == BICG_S) {
+        // v0 = s, a0 = r, a1 = ap: s = r; s += ap * -alpha; sum s.s
+        double s = a0[i];
+        s += a1[i] * -alpha;
+        v0[i] = s;
+        sa += s * s;
+      } else if (MODE == BICG_XP) {
+        // v0 = x, a0 = p: x += p * alpha (early exit, BiCGStab.h:162-167)
+        v0[i] += a0[i] * alpha;
+      } else if (MODE == BICG_XR) {
+        // v0 = x, v1 = r, a0 = mp, a1 = ms, a2 = ap, a3 = as, a4 = rhat
+        v0[i] += a0[i] * alpha + a1[i] * omega;
+        double r = v1[i];
+        r += a2[i] * -alpha + a3[i] * -omega;
+        v1[i] = r;
+        sa += r * a4[i];
+        sb += r * r;
+      } else {  // BICG_P: v0 = p, a0 = ap, a1 = r
+        double pv = v0[i];
+        pv += a0[i] * -omega;
+        v0[i] = pv * beta + a1[i];
+      }
     }
   }
   if (MODE == BICG_INIT || MODE == BICG_S || MODE == BICG_XR) {
@@ -141,17 +209,17 @@ __global__ void __launch_bounds__(256) bicg_kernel(Layout L, const double* __res
   }
 }
 
+// fixed-order fold of n partials by one workgroup (strided, coalesced reads)
 __device__ double fold(const double* __restrict__ part, int n)
 {
   double acc = 0.0;
-  int chunk = (n + blockDim.x - 1) / blockDim.x;
-  int b0 = threadIdx.x * chunk;
-  for (int k = b0; k < b0 + chunk && k < n; k++) acc += part[k];
+  for (int k = threadIdx.x; k < n; k += blockDim.x) acc += part[k];
   return block_reduce<RED_SUM>(acc);
 }
 
-__global__ void __launch_bounds__(256) bicg_finalize(int mode, const double* __restrict__ pa,
-                                                     const double* __restrict__ pb, int n, double* __restrict__ scal)
+__global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double* __restrict__ pa,
+                                                          const double* __restrict__ pb, int n,
+                                                          double* __restrict__ scal)
 {
   double A = fold(pa, n);
   double B = (mode == FIN_INIT || mode == FIN_OMEGA || mode == FIN_RHO) ? fold(pb, n) : 0.0;
@@ -176,35 +244,77 @@ __global__ void __launch_bounds__(256) bicg_finalize(int mode, const double* __r
 
 }  // namespace
 
-int krylov_blocks(const te_domain* d)
+// smallest instantiated per-thread cell count covering `cells`
+static int pick_cpt(int cells)
 {
-  Layout L = make_layout(d->dim, d->ns, 1, d->np);
-  long total = (long)L.np * L.ps;
-  return (int)((total + 255) / 256);
+  int need = (cells + kThreads - 1) / kThreads;
+  const int opts[] = { 2, 5, 8, 16, kMaxCells };
+  for (int c : opts)
+    if (c >= need) return c;
+  return -1;
 }
 
+int krylov_blocks(const te_domain* d)
+{
+  int g = d->np < 2048 ? d->np : 2048;
+  return g < 1 ? 1 : g;
+}
+
+bool star_dot_fly_supported(const te_ghost_filler* g)
+{
+  return g->d->dim == 2 && (g->kind == TE_BILINEAR || g->kind == TE_BIQUADRATIC);
+}
+
+// out = A u with the dot w.out (w required) and optionally out.out (pb != null).
+// fly: compute neighbour ghosts on the fly (2D) instead of reading them from u.
 int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
-                    double* out, const double* w, double* pa, double* pb, hipStream_t s)
+                    double* out, const double* w, double* pa, double* pb, bool fly, hipStream_t s)
 {
   const te_domain* d = g->d;
   Layout L = make_layout(d->dim, d->ns, 1, d->np);
-  int grid = krylov_blocks(d);
-  bool wd = w != nullptr, sq = pb != nullptr;
-#define TE_SD(D_, OP_, W_, Q_)                                                                               \
-  hipLaunchKernelGGL((star_dot_kernel<D_, OP_, W_, Q_>), dim3(grid), dim3(256), 0, s, d->d_patches, u, coeffs, \
-                     out, w, pa, pb, L, neumann)
-#define TE_SDW(D_, OP_)                                  \
-  if (wd && sq) TE_SD(D_, OP_, true, true);              \
-  else if (wd) TE_SD(D_, OP_, true, false);              \
-  else if (sq) TE_SD(D_, OP_, false, true);              \
-  else TE_SD(D_, OP_, false, false);
-  if (d->dim == 2) {
-    if (op_kind == 0) { TE_SDW(2, 0) } else { TE_SDW(2, 1) }
-  } else {
-    if (op_kind == 0) { TE_SDW(3, 0) } else { TE_SDW(3, 1) }
+  int cpt = pick_cpt(L.cs);
+  if (cpt < 0 || !w) {
+    set_error("fused Krylov apply: patch too large or missing dot operand");
+    return TE_ERR_UNSUPPORTED;
   }
-#undef TE_SDW
+  const int fk = (fly && star_dot_fly_supported(g)) ? g->kind : -1;
+  int grid = krylov_blocks(d);
+  bool sq = pb != nullptr;
+#define TE_SDC(C_, D_, OP_, Q_, F_)                                                                              \
+  hipLaunchKernelGGL((star_dot_kernel<C_, D_, OP_, true, Q_, F_>), dim3(grid), dim3(kThreads), 0, s, d->d_patches, \
+                     u, coeffs, out, w, pa, pb, L, neumann)
+#define TE_SD(D_, OP_, Q_, F_)                      \
+  do {                                              \
+    switch (cpt) {                                  \
+      case 2: TE_SDC(2, D_, OP_, Q_, F_); break;    \
+      case 5: TE_SDC(5, D_, OP_, Q_, F_); break;    \
+      case 8: TE_SDC(8, D_, OP_, Q_, F_); break;    \
+      case 16: TE_SDC(16, D_, OP_, Q_, F_); break;  \
+      default: TE_SDC(kMaxCells, D_, OP_, Q_, F_);  \
+    }                                               \
+  } while (0)
+#define TE_SDQ(D_, OP_, F_)                \
+  do {                                     \
+    if (sq) TE_SD(D_, OP_, true, F_);      \
+    else TE_SD(D_, OP_, false, F_);        \
+  } while (0)
+#define TE_SDF(OP_)                                   \
+  do {                                                \
+    if (fk == TE_BILINEAR) TE_SDQ(2, OP_, TE_BILINEAR);  \
+    else if (fk == TE_BIQUADRATIC) TE_SDQ(2, OP_, TE_BIQUADRATIC); \
+    else TE_SDQ(2, OP_, -1);                           \
+  } while (0)
+  if (d->dim == 2) {
+    if (op_kind == 0) TE_SDF(0);
+    else TE_SDF(1);
+  } else {
+    if (op_kind == 0) TE_SDQ(3, 0, -1);
+    else TE_SDQ(3, 1, -1);
+  }
+#undef TE_SDF
+#undef TE_SDQ
 #undef TE_SD
+#undef TE_SDC
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
@@ -214,9 +324,25 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
                 hipStream_t s)
 {
   Layout L = make_layout(d->dim, d->ns, 1, d->np);
+  int cpt = pick_cpt(d->ns[0] * d->ns[1] * (d->dim == 3 ? d->ns[2] : 1));
+  if (cpt < 0) {
+    set_error("patch too large for the fused Krylov kernels");
+    return TE_ERR_UNSUPPORTED;
+  }
   int grid = krylov_blocks(d);
-#define TE_BK(M_) \
-  hipLaunchKernelGGL((bicg_kernel<M_>), dim3(grid), dim3(256), 0, s, L, scal, v0, v1, a0, a1, a2, a3, a4, pa, pb)
+#define TE_BKC(C_, M_)                                                                                        \
+  hipLaunchKernelGGL((bicg_kernel<C_, M_>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0, a1, a2, a3, \
+                     a4, pa, pb)
+#define TE_BK(M_)                      \
+  do {                                 \
+    switch (cpt) {                     \
+      case 2: TE_BKC(2, M_); break;    \
+      case 5: TE_BKC(5, M_); break;    \
+      case 8: TE_BKC(8, M_); break;    \
+      case 16: TE_BKC(16, M_); break;  \
+      default: TE_BKC(kMaxCells, M_);  \
+    }                                  \
+  } while (0)
   switch (mode) {
     case BICG_INIT: TE_BK(BICG_INIT); break;
     case BICG_S: TE_BK(BICG_S); break;
@@ -225,13 +351,14 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
     default: TE_BK(BICG_P); break;
   }
 #undef TE_BK
+#undef TE_BKC
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
 
 int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s)
 {
-  hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(256), 0, s, mode, pa, pb, n, scal);
+  hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, n, scal);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
