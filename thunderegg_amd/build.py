@@ -47,7 +47,7 @@ FLAGS = [
 
 def _compile(src: str) -> str:
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
+    obj = _obj(src)
     cmd = [HIPCC] + FLAGS + (["-x", "hip"] if src.endswith(".hip") else []) + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -58,12 +58,31 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None) -> str:
+def _obj(src: str) -> str:
+    return os.path.join(BUILD, src.replace("/", "_") + ".o")
+
+
+def _up_to_date(src: str) -> bool:
+    obj = _obj(src)
+    if not os.path.exists(obj):
+        return False
+    deps = [os.path.join(CSRC, src)]
+    for root in (CSRC, os.path.join(HERE, "..", "include")):
+        for dp, _, files in os.walk(root):
+            deps += [os.path.join(dp, f) for f in files if f.endswith(".h")]
+    return os.path.getmtime(obj) > max(os.path.getmtime(d) for d in deps)
+
+
+def build(jobs: int | None = None, incremental: bool = False) -> str:
+    """Compile every source (incremental=True: only those older than their object
+    or any header) and link the shared library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    todo = [s for s in srcs if not (incremental and _up_to_date(s))]
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [_obj(s) for s in srcs]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -74,4 +93,4 @@ def build(jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    print(build())
+    print(build(incremental="--incremental" in sys.argv))

@@ -99,6 +99,7 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
   const int c = lane & 15;
   const int g = lane >> 4;
   T* X = s2 + n + wave * n * ld;  // per-wave scratch n x ld
+  T* Bt = s2 + n + kWaves * n * ld + wave * 4 * n;  // per-wave interface terms of the 4 sides
   UAcc<T, TC, GM == 2> U{ u, fa.uc, fa.parent, fa.porth, L };
 
   for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
@@ -121,45 +122,61 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
       }
     }
     __syncthreads();
-    // ---- modifyRHSForInternalBoundaryConditions: west/east first, then south/north
-    //      (sides in increasing order, Poisson/StarPatchOperator.h:165-180)
-    if (active && !UZERO) {
-      const DevPatch& P = dp[p];
+    // ---- modifyRHSForInternalBoundaryConditions: the interface term of every side cell
+    //      is formed first (a lane per cell, so all its loads are in flight together),
+    //      then applied west/east first, then south/north (sides in increasing order,
+    //      Poisson/StarPatchOperator.h:165-180)
+    if (!UZERO) {
+      if (active) {
+        const DevPatch& P = dp[p];
+        for (int q = lane; q < 4 * n; q += 64) {
+          const int s = q / n, t = q - s * n;
+          const int ph = s >> 1, up = s & 1;
+          T term = T(0);
+          if (P.type[4 + s] >= 0) {
+            int gi, bi;  // padded index of the ghost and of the boundary cell
+            int bx, by;  // boundary cell
+            if (ph == 0) {
+              gi = base + (t + 1) * L.sx + (up ? n + 1 : 0);
+              bi = base + (t + 1) * L.sx + (up ? n : 1);
+              bx = up ? n - 1 : 0;
+              by = t;
+            } else {
+              gi = base + (up ? n + 1 : 0) * L.sx + t + 1;
+              bi = base + (up ? n : 1) * L.sx + t + 1;
+              bx = t;
+              by = up ? n - 1 : 0;
+            }
+            T ug, ub;
+            if (GM == 0) {
+              ug = u[gi];
+              ub = u[bi];
+            } else {
+              ug = ghost2d<T, KIND>(P, p, s, t, n, n, U);
+              ub = U(p, bx, by);
+            }
+            const T ih2 = T(P.inv_h2[ph]);
+            if (OP == 0) {
+              term = (ug + ub) * ih2;
+            } else {
+              const int cg = gi - p * L.ps + p * L.cs, cb = bi - p * L.ps + p * L.cs;
+              term = (ug + ub) * (coeffs[cb] + coeffs[cg]) * (ih2 * T(0.5));
+            }
+          }
+          Bt[q] = term;
+        }
+      }
+      __syncthreads();
+#pragma unroll
       for (int ph = 0; ph < 2; ph++) {
-        for (int q = lane; q < 2 * n; q += 64) {
-          int up = q / n, t = q - up * n;
-          int s = 2 * ph + up;
-          if (P.type[4 + s] < 0) continue;
-          int gi, bi;  // padded index of the ghost and of the boundary cell
-          int xi;      // LDS index of the boundary cell
-          int bx, by;  // boundary cell
-          if (ph == 0) {
-            gi = base + (t + 1) * L.sx + (up ? n + 1 : 0);
-            bi = base + (t + 1) * L.sx + (up ? n : 1);
-            xi = t * ld + (up ? n - 1 : 0);
-            bx = up ? n - 1 : 0;
-            by = t;
-          } else {
-            gi = base + (up ? n + 1 : 0) * L.sx + t + 1;
-            bi = base + (up ? n : 1) * L.sx + t + 1;
-            xi = (up ? n - 1 : 0) * ld + t;
-            bx = t;
-            by = up ? n - 1 : 0;
-          }
-          T ug, ub;
-          if (GM == 0) {
-            ug = u[gi];
-            ub = u[bi];
-          } else {
-            ug = ghost2d<T, KIND>(P, p, s, t, n, n, U);
-            ub = U(p, bx, by);
-          }
-          T ih2 = T(P.inv_h2[ph]);
-          if (OP == 0) {
-            X[xi] -= (ug + ub) * ih2;
-          } else {
-            int cg = gi - p * L.ps + p * L.cs, cb = bi - p * L.ps + p * L.cs;
-            X[xi] -= (ug + ub) * (coeffs[cb] + coeffs[cg]) * (ih2 * T(0.5));
+        if (active) {
+          const DevPatch& P = dp[p];
+          for (int q = lane; q < 2 * n; q += 64) {
+            const int up = q / n, t = q - up * n;
+            const int s = 2 * ph + up;
+            if (P.type[4 + s] < 0) continue;
+            const int xi = ph == 0 ? t * ld + (up ? n - 1 : 0) : (up ? n - 1 : 0) * ld + t;
+            X[xi] -= Bt[s * n + t];
           }
         }
         __syncthreads();
@@ -269,7 +286,7 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   const te_domain* d = pl->g->d;
   Layout L = make_layout(2, d->ns, 1, d->np);
   constexpr int n = 16 * NT;
-  size_t shmem = sizeof(T) * (2 * n * n + n + kWaves * n * (n + 1));
+  size_t shmem = sizeof(T) * (2 * n * n + n + kWaves * n * (n + 1) + kWaves * 4 * n);
   const T* mats = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32);
   const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
   const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);

@@ -1,15 +1,23 @@
-// Cycle::restrict (GMG/Cycle.h:57-65) in one launch, 2D, even n: for every fine
-// patch, its ghost values are computed on the fly from its neighbours (the fill
-// of MPIGhostFiller::fillGhost, as a gather), r = f - A u is evaluated with the
-// star stencil (Poisson/StarPatchOperator.h:75-101, VarPoisson :61-109), and r is
-// restricted straight into the coarser level's right-hand side
-// (LinearRestrictor::restrictPatches, GMG/LinearRestrictor.h:101-155): a child
-// patch writes its quadrant of the parent (2x2 averages), an uncoarsened patch
-// copies.  The residual vector itself is never stored.  Each coarse interior
-// cell is written by exactly one fine patch; the coarse ghost cells are left
-// as they are (zero from allocation; nothing reads the ghosts of a right-hand side).
+// Fused 2D launches that compute ghost values inside the consumer instead of a
+// separate ghost fill (MPIGhostFiller::fillGhost as a gather, GhostFiller.h):
+//
+//  - resid_restrict_2d: Cycle::restrict (GMG/Cycle.h:57-65) in one launch: r = f - A u
+//    with the star stencil (Poisson/StarPatchOperator.h:75-101, VarPoisson :61-109),
+//    restricted straight into the coarser level's right-hand side
+//    (LinearRestrictor::restrictPatches, GMG/LinearRestrictor.h:101-155).  The
+//    residual vector itself is never stored.
+//  - star_dot_2d: the outer BiCGStab's A p / A s (Iterative/BiCGStab.h:150-170) with
+//    the inner products that follow them folded in, FP64.
+//
+// A wavefront owns one patch at a time and stages it in a padded LDS tile: the
+// interior with coalesced loads, then the side ghosts with one lane per ghost cell
+// (tile_ghosts_2d), so the divergent ghost formulas and their dependent neighbour
+// loads run once per patch instead of once per boundary cell.  The stencil then
+// reads only the tile.
 #include "../te_internal.h"
+#include "blas.h"
 #include "ghost2d.h"
+#include "krylov.h"
 
 namespace te {
 namespace {
@@ -23,6 +31,29 @@ struct PlainAcc {
   __device__ __forceinline__ T operator()(int q, int x, int y) const { return u[q * L.ps + (y + 1) * L.sx + x + 1]; }
 };
 
+// (A u)(x, y) from the staged tile; ci: padded index of the cell in the coefficient vector
+template <typename T, int OP>
+__device__ __forceinline__ T star_tile(const T* tile, int w, int x, int y, const T* ih2, const T* cf, int ci,
+                                       int sx)
+{
+  const int k = (y + 1) * w + x + 1;
+  const T mid = tile[k];
+  T acc = T(0);
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    const int st = a == 0 ? 1 : w;
+    const T lower = tile[k - st], upper = tile[k + st];
+    if (OP == 0) {
+      acc += (upper - T(2) * mid + lower) * ih2[a];
+    } else {
+      const int cst = a == 0 ? 1 : sx;
+      const T cm = cf[ci], cl = cf[ci - cst], cu = cf[ci + cst];
+      acc += ((cu + cm) * (upper - mid) - (cl + cm) * (mid - lower)) * (ih2[a] * T(0.5));
+    }
+  }
+  return acc;
+}
+
 template <typename T, typename TF, typename TC, int OP, int KIND>
 __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch* __restrict__ dp,
                                                                  const T* __restrict__ u, const T* __restrict__ cf,
@@ -32,68 +63,115 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
                                                                  int neumann)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int n0 = L.n[0], n1 = L.n[1];
-  const int N = n0 * n1;
+  const int n0 = L.n[0], n1 = L.n[1], w = n0 + 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  T* R = reinterpret_cast<T*>(smem_raw) + wave * N;
+  T* tile = reinterpret_cast<T*>(smem_raw) + wave * w * (n1 + 2);
   PlainAcc<T> U{ u, L };
+  const T phys = (OP == 0 && neumann) ? T(1) : T(-1);
   const int ngroups = (L.np + kWaves - 1) / kWaves;
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int p = grp * kWaves + wave;
     const bool active = p < L.np;
-    const int o = active ? porth[p] : -1;
-    const int cp = active ? parent[p] : 0;
+    if (active) tile_interior_2d<T>(p, n0, n1, U, tile, lane);
+    __syncthreads();
+    if (active) tile_ghosts_2d<T, KIND>(dp[p], p, n0, n1, U, tile, phys, lane);
+    __syncthreads();
     if (active) {
       const DevPatch& P = dp[p];
       const T ih2[2] = { T(P.inv_h2[0]), T(P.inv_h2[1]) };
-      for (int q = lane; q < N; q += 64) {
-        const int x = q % n0, y = q / n0;
-        const int i = p * L.ps + (y + 1) * L.sx + x + 1;
-        const T mid = u[i];
-        const T bc = (OP == 0 && neumann) ? mid : -mid;
-        T acc = T(0);
-#pragma unroll
-        for (int a = 0; a < 2; a++) {
-          const int ca = a == 0 ? x : y, na = a == 0 ? n0 : n1, t = a == 0 ? y : x;
-          const int st = a == 0 ? 1 : L.sx;
-          T lower, upper;
-          if (ca == 0) lower = P.type[4 + 2 * a] < 0 ? bc : ghost2d<T, KIND>(P, p, 2 * a, t, na, a == 0 ? n1 : n0, U);
-          else lower = u[i - st];
-          if (ca == na - 1)
-            upper = P.type[5 + 2 * a] < 0 ? bc : ghost2d<T, KIND>(P, p, 2 * a + 1, t, na, a == 0 ? n1 : n0, U);
-          else upper = u[i + st];
-          if (OP == 0) {
-            acc += (upper - T(2) * mid + lower) * ih2[a];
-          } else {
-            const int ci = p * L.cs + (y + 1) * L.sx + x + 1;
-            const T cm = cf[ci], cl = cf[ci - st], cu = cf[ci + st];
-            acc += ((cu + cm) * (upper - mid) - (cl + cm) * (mid - lower)) * (ih2[a] * T(0.5));
-          }
-        }
-        const T r = T(f[i]) - acc;
-        if (o < 0) fc[cp * L.ps + (y + 1) * L.sx + x + 1] = TC(r);  // copyToParent
-        else R[q] = r;
-      }
-    }
-    __syncthreads();
-    if (active && o >= 0) {
-      // restrictToCoarserParent: coarse[(i + s) / 2] += fine[i] / 4, fine cells in x-fastest order
+      const int o = porth[p], cp = parent[p];
       const int h0 = n0 / 2, h1 = n1 / 2;
       const int ox = (o & 1) ? h0 : 0, oy = (o & 2) ? h1 : 0;
-      const T w = T(0.25);
-      for (int q = lane; q < h0 * h1; q += 64) {
-        const int X = q % h0, Y = q / h0;
-        T acc = T(0);
-        acc += R[(2 * Y) * n0 + 2 * X] * w;
-        acc += R[(2 * Y) * n0 + 2 * X + 1] * w;
-        acc += R[(2 * Y + 1) * n0 + 2 * X] * w;
-        acc += R[(2 * Y + 1) * n0 + 2 * X + 1] * w;
-        fc[cp * L.ps + (Y + oy + 1) * L.sx + X + ox + 1] = TC(acc);
+      // a lane per 2x2 block of fine cells = one coarse cell
+      for (int Q = lane; Q < h0 * h1; Q += 64) {
+        const int X = Q % h0, Y = Q / h0;
+        T r[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int x = 2 * X + (j & 1), y = 2 * Y + (j >> 1);
+          const int c = (y + 1) * L.sx + x + 1;
+          r[j] = T(f[p * L.ps + c]) - star_tile<T, OP>(tile, w, x, y, ih2, cf, p * L.cs + c, L.sx);
+        }
+        if (o < 0) {  // copyToParent
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int x = 2 * X + (j & 1), y = 2 * Y + (j >> 1);
+            fc[cp * L.ps + (y + 1) * L.sx + x + 1] = TC(r[j]);
+          }
+        } else {
+          // restrictToCoarserParent: coarse[(i + s) / 2] += fine[i] / 4 over the fine
+          // cells in x-fastest order
+          T acc = T(0);
+#pragma unroll
+          for (int j = 0; j < 4; j++) acc += r[j] * T(0.25);
+          fc[cp * L.ps + (Y + oy + 1) * L.sx + X + ox + 1] = TC(acc);
+        }
       }
     }
     __syncthreads();
   }
 }
+
+// out = A u (ghost cells of out written 0); pa[block] = sum w * out, pb[block] = sum out * out
+template <int OP, int KIND, bool SQ>
+__global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __restrict__ dp,
+                                                           const double* __restrict__ u,
+                                                           const double* __restrict__ cf, double* __restrict__ out,
+                                                           const double* __restrict__ wv, double* __restrict__ pa,
+                                                           double* __restrict__ pb, Layout L, int neumann)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n0 = L.n[0], n1 = L.n[1], w = n0 + 2;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* tile = reinterpret_cast<double*>(smem_raw) + wave * w * (n1 + 2);
+  PlainAcc<double> U{ u, L };
+  const double phys = (OP == 0 && neumann) ? 1.0 : -1.0;
+  double sa = 0.0, sq = 0.0;
+  const int ngroups = (L.np + kWaves - 1) / kWaves;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int p = grp * kWaves + wave;
+    const bool active = p < L.np;
+    if (active) tile_interior_2d<double>(p, n0, n1, U, tile, lane);
+    __syncthreads();
+    if (active) tile_ghosts_2d<double, KIND>(dp[p], p, n0, n1, U, tile, phys, lane);
+    __syncthreads();
+    if (active) {
+      const DevPatch& P = dp[p];
+      const double ih2[2] = { P.inv_h2[0], P.inv_h2[1] };
+      const int base = p * L.ps;
+      for (int q = lane; q < n0 * n1; q += 64) {
+        const int y = q / n0, x = q - y * n0;
+        const int c = (y + 1) * L.sx + x + 1;
+        const double acc = star_tile<double, OP>(tile, w, x, y, ih2, cf, p * L.cs + c, L.sx);
+        out[base + c] = acc;
+        sa += wv[base + c] * acc;  // rhat.dot(ap) / as.dot(s): w first, as the reference
+        if (SQ) sq += acc * acc;
+      }
+      // ghost frame of out: rows -1 and n1 (with corners), then columns -1 and n0
+      for (int g = lane; g < 2 * w + 2 * n1; g += 64) {
+        int x, y;
+        if (g < 2 * w) {
+          y = g < w ? -1 : n1;
+          x = (g < w ? g : g - w) - 1;
+        } else {
+          const int h = g - 2 * w;
+          x = h < n1 ? -1 : n0;
+          y = h < n1 ? h : h - n1;
+        }
+        out[base + (y + 1) * L.sx + x + 1] = 0.0;
+      }
+    }
+    __syncthreads();
+  }
+  double r = block_reduce<RED_SUM>(sa);
+  if (threadIdx.x == 0) pa[blockIdx.x] = r;
+  if (SQ) {
+    r = block_reduce<RED_SUM>(sq);
+    if (threadIdx.x == 0) pb[blockIdx.x] = r;
+  }
+}
+
+size_t tile_bytes(const te_domain* d, size_t esz) { return esz * kWaves * (d->ns[0] + 2) * (d->ns[1] + 2); }
 
 template <typename T, typename TF, typename TC>
 void launch_rr(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann, const void* coeffs,
@@ -101,7 +179,7 @@ void launch_rr(const te_ghost_filler* g, const te_level_link* link, int op_kind,
 {
   const te_domain* d = g->d;
   Layout L = make_layout(2, d->ns, 1, d->np);
-  size_t shmem = sizeof(T) * kWaves * d->ns[0] * d->ns[1];
+  size_t shmem = tile_bytes(d, sizeof(T));
   int ngroups = (d->np + kWaves - 1) / kWaves;
   int grid = ngroups < 8192 ? ngroups : 8192;
 #define TE_RR(OP_, K_)                                                                                      \
@@ -120,10 +198,13 @@ void launch_rr(const te_ghost_filler* g, const te_level_link* link, int op_kind,
 
 }  // namespace
 
+// a tile of doubles for each of the kWaves waves must fit the 64 KiB a workgroup may use
+static bool tile_fits(const te_domain* d) { return tile_bytes(d, sizeof(double)) <= 65536; }
+
 bool resid_restrict_fused_supported(const te_ghost_filler* g)
 {
   const te_domain* d = g->d;
-  return d->dim == 2 && d->ns[0] % 2 == 0 && d->ns[1] % 2 == 0 && d->ns[0] * d->ns[1] <= 4096 &&
+  return d->dim == 2 && d->ns[0] % 2 == 0 && d->ns[1] % 2 == 0 && tile_fits(d) &&
          (g->kind == TE_BILINEAR || g->kind == TE_BIQUADRATIC);
 }
 
@@ -133,6 +214,10 @@ int launch_resid_restrict(const te_ghost_filler* g, const te_level_link* link, i
                           hipStream_t s)
 {
   if (g->d->np == 0) return TE_OK;
+  if (!resid_restrict_fused_supported(g)) {
+    set_error("fused residual / restriction: needs 2D, even n, BiLinear or BiQuadratic ghosts");
+    return TE_ERR_UNSUPPORTED;
+  }
   if (dt == TE_F64 && dtf == TE_F64 && dtc == TE_F64) launch_rr<double, double, double>(g, link, op_kind, neumann, coeffs, f, u, fc, s);
   else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F32) launch_rr<float, float, float>(g, link, op_kind, neumann, coeffs, f, u, fc, s);
   else if (dt == TE_F32 && dtf == TE_F64 && dtc == TE_F32) launch_rr<float, double, float>(g, link, op_kind, neumann, coeffs, f, u, fc, s);
@@ -142,6 +227,42 @@ int launch_resid_restrict(const te_ghost_filler* g, const te_level_link* link, i
     set_error("unsupported dtype combination for the fused residual / restriction");
     return TE_ERR_UNSUPPORTED;
   }
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+bool star_dot_fly_supported(const te_ghost_filler* g)
+{
+  return g->d->dim == 2 && tile_fits(g->d) && (g->kind == TE_BILINEAR || g->kind == TE_BIQUADRATIC);
+}
+
+int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
+                       double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s)
+{
+  const te_domain* d = g->d;
+  if (!star_dot_fly_supported(g)) {
+    set_error("fused 2D operator: needs 2D, BiLinear or BiQuadratic ghosts and a tile within 64 KiB");
+    return TE_ERR_UNSUPPORTED;
+  }
+  Layout L = make_layout(2, d->ns, 1, d->np);
+  size_t shmem = tile_bytes(d, sizeof(double));
+#define TE_SD(OP_, K_, Q_)                                                                                    \
+  hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches, u, \
+                     coeffs, out, w, pa, pb, L, neumann)
+#define TE_SDQ(OP_, K_)          \
+  do {                           \
+    if (pb) TE_SD(OP_, K_, true);  \
+    else TE_SD(OP_, K_, false);    \
+  } while (0)
+  if (op_kind == 0) {
+    if (g->kind == TE_BILINEAR) TE_SDQ(0, TE_BILINEAR);
+    else TE_SDQ(0, TE_BIQUADRATIC);
+  } else {
+    if (g->kind == TE_BILINEAR) TE_SDQ(1, TE_BILINEAR);
+    else TE_SDQ(1, TE_BIQUADRATIC);
+  }
+#undef TE_SDQ
+#undef TE_SD
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

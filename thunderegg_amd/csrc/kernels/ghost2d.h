@@ -95,4 +95,69 @@ __device__ __forceinline__ T ghost2d(const DevPatch& P, int p, int s, int t, int
   return G;
 }
 
+// ---- per-wave staging of one patch into a padded LDS tile ----
+// tile[(y + 1) * (n0 + 2) + x + 1] holds cell (x, y), x in [-1, n0], y in [-1, n1].
+
+// Side ring cell g in [0, 2 (n0 + n1)): sides 0, 1 (x faces, n1 cells each), then 2, 3.
+__device__ __forceinline__ void ring_cell(int g, int n0, int n1, int* s, int* t)
+{
+  if (g < 2 * n1) {
+    *s = g >= n1 ? 1 : 0;
+    *t = g - *s * n1;
+  } else {
+    g -= 2 * n1;
+    *s = g >= n0 ? 3 : 2;
+    *t = g - (*s - 2) * n0;
+  }
+}
+
+// Interior reads of the staged patch p come from the tile, other patches from G.
+template <typename T, class Acc>
+struct TileAcc {
+  const T* tile;
+  int p;
+  int w;
+  Acc G;
+  __device__ __forceinline__ T operator()(int q, int x, int y) const
+  {
+    return q == p ? tile[(y + 1) * w + x + 1] : G(q, x, y);
+  }
+};
+
+// Interior of patch p, coalesced along x; one wave (64 lanes) per tile.
+template <typename T, class Acc>
+__device__ __forceinline__ void tile_interior_2d(int p, int n0, int n1, Acc U, T* tile, int lane)
+{
+  const int w = n0 + 2;
+  for (int q = lane; q < n0 * n1; q += 64) {
+    const int y = q / n0, x = q - y * n0;
+    tile[(y + 1) * w + x + 1] = U(p, x, y);
+  }
+}
+
+// Side ghosts of patch p (after its interior is in the tile and visible to the wave):
+// a lane per ghost cell, neighbour sides by ghost2d, physical sides as
+// phys * (adjacent interior value) (the Dirichlet / Neumann image, phys = -1 / +1).
+template <typename T, int KIND, class Acc>
+__device__ __forceinline__ void tile_ghosts_2d(const DevPatch& P, int p, int n0, int n1, Acc U, T* tile, T phys,
+                                               int lane)
+{
+  const int w = n0 + 2;
+  TileAcc<T, Acc> A{ tile, p, w, U };
+  for (int g = lane; g < 2 * (n0 + n1); g += 64) {
+    int s, t;
+    ring_cell(g, n0, n1, &s, &t);
+    const int a = s >> 1, up = s & 1;
+    const int na = a == 0 ? n0 : n1, nt = a == 0 ? n1 : n0;
+    int bx, by;
+    side_xy(s, 0, t, na, &bx, &by);
+    const int d = up ? 1 : -1;
+    const int gx = a == 0 ? bx + d : bx, gy = a == 0 ? by : by + d;
+    T v;
+    if (P.type[4 + s] < 0) v = phys * tile[(by + 1) * w + bx + 1];
+    else v = ghost2d<T, KIND>(P, p, s, t, na, nt, A);
+    tile[(gy + 1) * w + gx + 1] = v;
+  }
+}
+
 }  // namespace te

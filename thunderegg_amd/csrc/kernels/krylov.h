@@ -13,6 +13,9 @@ int krylov_blocks(const te_domain* d);
 bool star_dot_fly_supported(const te_ghost_filler* g);
 int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
                     double* out, const double* w, double* pa, double* pb, bool fly, hipStream_t s);
+// 2D, ghosts computed in the launch; writes `grid` partials (fused2d.hip)
+int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
+                       double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s);
 int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const double* a0,
                 const double* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
                 hipStream_t s);

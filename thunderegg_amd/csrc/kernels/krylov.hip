@@ -10,7 +10,6 @@
 // so the per-element work is a load, the arithmetic and a store.
 #include "../te_internal.h"
 #include "blas.h"
-#include "ghost2d.h"
 #include "krylov.h"
 
 namespace te {
@@ -62,19 +61,9 @@ __device__ __forceinline__ void make_cells(const Layout& L, bool with_ghosts, Ce
   }
 }
 
-struct PlainAcc64 {
-  const double* u;
-  Layout L;
-  __device__ __forceinline__ double operator()(int q, int x, int y) const
-  {
-    return u[q * L.ps + (y + 1) * L.sx + x + 1];
-  }
-};
-
 // out = A u (ghosts of out written 0), with optional fused sums pa = sum w*out, pb = sum out*out.
-// FK >= 0 (2D): neighbour ghost values are computed on the fly with ghost filler kind FK
-// instead of being read from u (so no separate ghost fill precedes the apply).
-template <int CPT, int D, int OP, bool WDOT, bool SQ, int FK>
+// The ghost cells of u must be filled.
+template <int CPT, int D, int OP, bool WDOT, bool SQ>
 __global__ void __launch_bounds__(kThreads) star_dot_kernel(const DevPatch* __restrict__ dp,
                                                             const double* __restrict__ u,
                                                             const double* __restrict__ cf, double* __restrict__ out,
@@ -85,7 +74,6 @@ __global__ void __launch_bounds__(kThreads) star_dot_kernel(const DevPatch* __re
   make_cells<CPT>(L, true, cs);
   const int st[3] = { 1, L.sx, L.sy };
   const int sb = D == 2 ? 4 : 20;
-  PlainAcc64 U{ u, L };
   double sa = 0.0, sq = 0.0;
   for (int p = blockIdx.x; p < L.np; p += gridDim.x) {
     const DevPatch& P = dp[p];
@@ -108,22 +96,8 @@ __global__ void __launch_bounds__(kThreads) star_dot_kernel(const DevPatch* __re
       double acc = 0.0;
 #pragma unroll
       for (int a = 0; a < D; a++) {
-        double lower, upper;
-        if (FK >= 0 && D == 2 && (b & (3u << (2 * a)))) {
-          const int yy = cs.off[k] / L.sx - 1, xx = cs.off[k] % L.sx - 1;
-          const int na = L.n[a], nt = L.n[1 - a], t = a == 0 ? yy : xx;
-          if ((b >> (2 * a)) & 1)
-            lower = ((phys >> (2 * a)) & 1) ? bc : ghost2d<double, FK < 0 ? 0 : FK>(P, p, 2 * a, t, na, nt, U);
-          else
-            lower = u[i - st[a]];
-          if ((b >> (2 * a + 1)) & 1)
-            upper = ((phys >> (2 * a + 1)) & 1) ? bc : ghost2d<double, FK < 0 ? 0 : FK>(P, p, 2 * a + 1, t, na, nt, U);
-          else
-            upper = u[i + st[a]];
-        } else {
-          lower = ((b >> (2 * a)) & 1) && ((phys >> (2 * a)) & 1) ? bc : u[i - st[a]];
-          upper = ((b >> (2 * a + 1)) & 1) && ((phys >> (2 * a + 1)) & 1) ? bc : u[i + st[a]];
-        }
+        const double lower = ((b >> (2 * a)) & 1) && ((phys >> (2 * a)) & 1) ? bc : u[i - st[a]];
+        const double upper = ((b >> (2 * a + 1)) & 1) && ((phys >> (2 * a + 1)) & 1) ? bc : u[i + st[a]];
         if (OP == 0) {
           acc += (upper - 2.0 * mid + lower) * ih2[a];
         } else {
@@ -260,13 +234,8 @@ int krylov_blocks(const te_domain* d)
   return g < 1 ? 1 : g;
 }
 
-bool star_dot_fly_supported(const te_ghost_filler* g)
-{
-  return g->d->dim == 2 && (g->kind == TE_BILINEAR || g->kind == TE_BIQUADRATIC);
-}
-
 // out = A u with the dot w.out (w required) and optionally out.out (pb != null).
-// fly: compute neighbour ghosts on the fly (2D) instead of reading them from u.
+// fly: compute neighbour ghosts on the fly (2D, launch_star_dot_2d) instead of reading them from u.
 int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
                     double* out, const double* w, double* pa, double* pb, bool fly, hipStream_t s)
 {
@@ -277,41 +246,35 @@ int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const do
     set_error("fused Krylov apply: patch too large or missing dot operand");
     return TE_ERR_UNSUPPORTED;
   }
-  const int fk = (fly && star_dot_fly_supported(g)) ? g->kind : -1;
   int grid = krylov_blocks(d);
+  // the partial sums cover krylov_blocks(d) workgroups whichever kernel runs
+  if (fly) return launch_star_dot_2d(g, op_kind, neumann, coeffs, u, out, w, pa, pb, grid, s);
   bool sq = pb != nullptr;
-#define TE_SDC(C_, D_, OP_, Q_, F_)                                                                              \
-  hipLaunchKernelGGL((star_dot_kernel<C_, D_, OP_, true, Q_, F_>), dim3(grid), dim3(kThreads), 0, s, d->d_patches, \
+#define TE_SDC(C_, D_, OP_, Q_)                                                                               \
+  hipLaunchKernelGGL((star_dot_kernel<C_, D_, OP_, true, Q_>), dim3(grid), dim3(kThreads), 0, s, d->d_patches, \
                      u, coeffs, out, w, pa, pb, L, neumann)
-#define TE_SD(D_, OP_, Q_, F_)                      \
-  do {                                              \
-    switch (cpt) {                                  \
-      case 2: TE_SDC(2, D_, OP_, Q_, F_); break;    \
-      case 5: TE_SDC(5, D_, OP_, Q_, F_); break;    \
-      case 8: TE_SDC(8, D_, OP_, Q_, F_); break;    \
-      case 16: TE_SDC(16, D_, OP_, Q_, F_); break;  \
-      default: TE_SDC(kMaxCells, D_, OP_, Q_, F_);  \
-    }                                               \
+#define TE_SD(D_, OP_, Q_)                      \
+  do {                                          \
+    switch (cpt) {                              \
+      case 2: TE_SDC(2, D_, OP_, Q_); break;    \
+      case 5: TE_SDC(5, D_, OP_, Q_); break;    \
+      case 8: TE_SDC(8, D_, OP_, Q_); break;    \
+      case 16: TE_SDC(16, D_, OP_, Q_); break;  \
+      default: TE_SDC(kMaxCells, D_, OP_, Q_);  \
+    }                                           \
   } while (0)
-#define TE_SDQ(D_, OP_, F_)                \
-  do {                                     \
-    if (sq) TE_SD(D_, OP_, true, F_);      \
-    else TE_SD(D_, OP_, false, F_);        \
-  } while (0)
-#define TE_SDF(OP_)                                   \
-  do {                                                \
-    if (fk == TE_BILINEAR) TE_SDQ(2, OP_, TE_BILINEAR);  \
-    else if (fk == TE_BIQUADRATIC) TE_SDQ(2, OP_, TE_BIQUADRATIC); \
-    else TE_SDQ(2, OP_, -1);                           \
+#define TE_SDQ(D_, OP_)                \
+  do {                                 \
+    if (sq) TE_SD(D_, OP_, true);      \
+    else TE_SD(D_, OP_, false);        \
   } while (0)
   if (d->dim == 2) {
-    if (op_kind == 0) TE_SDF(0);
-    else TE_SDF(1);
+    if (op_kind == 0) TE_SDQ(2, 0);
+    else TE_SDQ(2, 1);
   } else {
-    if (op_kind == 0) TE_SDQ(3, 0, -1);
-    else TE_SDQ(3, 1, -1);
+    if (op_kind == 0) TE_SDQ(3, 0);
+    else TE_SDQ(3, 1);
   }
-#undef TE_SDF
 #undef TE_SDQ
 #undef TE_SD
 #undef TE_SDC
