@@ -81,7 +81,7 @@ struct FusedArgs {
 };
 
 template <typename T, typename TF, typename TO, typename TC, int NT, int OP, bool UZERO, int GM, int KIND>
-__global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* __restrict__ dp,
+__global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 : 1) dft_mfma_kernel(const DevPatch* __restrict__ dp,
                                                                const T* __restrict__ mats, const T* __restrict__ s2tab,
                                                                const TF* __restrict__ f, const T* __restrict__ u,
                                                                const T* __restrict__ coeffs, TO* __restrict__ out,
@@ -100,6 +100,10 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
   const int g = lane >> 4;
   T* X = s2 + n + wave * n * ld;  // per-wave scratch n x ld
   T* Bt = s2 + n + kWaves * n * ld + wave * 4 * n;  // per-wave interface terms of the 4 sides
+  // n = 16 with ghosts computed here: the patch's own values (u, or u + P uc) are staged
+  // in a padded tile first, so the ghost formulas read their self terms from LDS
+  constexpr bool TILE = GM != 0 && NT == 1;
+  T* Tl = s2 + n + kWaves * n * ld + kWaves * 4 * n + wave * (n + 2) * (n + 2);
   UAcc<T, TC, GM == 2> U{ u, fa.uc, fa.parent, fa.porth, L };
 
   for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
@@ -107,19 +111,25 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
     S3[k] = mats[(4 * 3 + 0) * kDftMatStride + k];  // DST_III, axis 0
   }
   for (int k = threadIdx.x; k < n; k += blockDim.x) s2[k] = s2tab[(2 * 3 + 0) * kDftMatDim + k];
-  __syncthreads();
+  // no barrier here: the first patch's loads go out together with the matrices', and the
+  // barrier after its right-hand-side copy (every workgroup runs at least one group,
+  // grid <= groups) orders both before any read
 
   const int ngroups = (L.np + kWaves - 1) / kWaves;
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int p = grp * kWaves + wave;
     const bool active = p < L.np;
     const int base = p * L.ps;
-    // ---- f_copy = f (interior) into LDS
+    // ---- f_copy = f (interior) into LDS; the side information of the ghost pass is
+    //      loaded at the same time
+    SideInfo si[NT];  // ring cells lane + 64 j: side j * 64 / n + lane / n
     if (active) {
       for (int q = lane; q < n * n; q += 64) {
         int y = q / n, x = q - y * n;
         X[y * ld + x] = T(f[base + (y + 1) * L.sx + x + 1]);
       }
+      if (TILE) tile_interior_2d<T>(p, n, n, U, Tl, lane);
+      if (!UZERO) ring_prefetch<NT>(dp[p], n, n, lane, si);
     }
     __syncthreads();
     // ---- modifyRHSForInternalBoundaryConditions: the interface term of every side cell
@@ -129,11 +139,13 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
     if (!UZERO) {
       if (active) {
         const DevPatch& P = dp[p];
-        for (int q = lane; q < 4 * n; q += 64) {
+#pragma unroll
+        for (int j = 0; j < NT; j++) {  // 4 n = 64 NT ring cells
+          const int q = lane + 64 * j;
           const int s = q / n, t = q - s * n;
           const int ph = s >> 1, up = s & 1;
           T term = T(0);
-          if (P.type[4 + s] >= 0) {
+          if (si[j].type >= 0) {
             int gi, bi;  // padded index of the ghost and of the boundary cell
             int bx, by;  // boundary cell
             if (ph == 0) {
@@ -151,8 +163,12 @@ __global__ void __launch_bounds__(64 * kWaves) dft_mfma_kernel(const DevPatch* _
             if (GM == 0) {
               ug = u[gi];
               ub = u[bi];
+            } else if (TILE) {
+              TileAcc<T, UAcc<T, TC, GM == 2>> A{ Tl, p, n + 2, U };
+              ug = ghost2d<T, KIND>(si[j], p, s, t, n, n, A);
+              ub = Tl[(by + 1) * (n + 2) + bx + 1];
             } else {
-              ug = ghost2d<T, KIND>(P, p, s, t, n, n, U);
+              ug = ghost2d<T, KIND>(si[j], p, s, t, n, n, U);
               ub = U(p, bx, by);
             }
             const T ih2 = T(P.inv_h2[ph]);
@@ -286,7 +302,9 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   const te_domain* d = pl->g->d;
   Layout L = make_layout(2, d->ns, 1, d->np);
   constexpr int n = 16 * NT;
-  size_t shmem = sizeof(T) * (2 * n * n + n + kWaves * n * (n + 1) + kWaves * 4 * n);
+  const bool tile = gm != 0 && NT == 1;
+  size_t shmem = sizeof(T) * (2 * n * n + n + kWaves * n * (n + 1) + kWaves * 4 * n +
+                              (tile ? kWaves * (n + 2) * (n + 2) : 0));
   const T* mats = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32);
   const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
   const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);

@@ -64,6 +64,7 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n0 = L.n[0], n1 = L.n[1], w = n0 + 2;
+  const int h0 = n0 / 2, h1 = n1 / 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   T* tile = reinterpret_cast<T*>(smem_raw) + wave * w * (n1 + 2);
   PlainAcc<T> U{ u, L };
@@ -72,15 +73,21 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int p = grp * kWaves + wave;
     const bool active = p < L.np;
-    if (active) tile_interior_2d<T>(p, n0, n1, U, tile, lane);
+    // everything that does not depend on the tile is loaded together with its interior
+    SideInfo si[1];
+    int o = -1, cp = 0;
+    if (active) {
+      tile_interior_2d<T>(p, n0, n1, U, tile, lane);
+      ring_prefetch<1>(dp[p], n0, n1, lane, si);
+      o = porth[p];
+      cp = parent[p];
+    }
     __syncthreads();
-    if (active) tile_ghosts_2d<T, KIND>(dp[p], p, n0, n1, U, tile, phys, lane);
+    if (active) tile_ghosts_2d<T, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
     __syncthreads();
     if (active) {
       const DevPatch& P = dp[p];
       const T ih2[2] = { T(P.inv_h2[0]), T(P.inv_h2[1]) };
-      const int o = porth[p], cp = parent[p];
-      const int h0 = n0 / 2, h1 = n1 / 2;
       const int ox = (o & 1) ? h0 : 0, oy = (o & 2) ? h1 : 0;
       // a lane per 2x2 block of fine cells = one coarse cell
       for (int Q = lane; Q < h0 * h1; Q += 64) {
@@ -131,14 +138,18 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int p = grp * kWaves + wave;
     const bool active = p < L.np;
-    if (active) tile_interior_2d<double>(p, n0, n1, U, tile, lane);
+    const int base = p * L.ps;
+    SideInfo si[1];
+    if (active) {
+      tile_interior_2d<double>(p, n0, n1, U, tile, lane);
+      ring_prefetch<1>(dp[p], n0, n1, lane, si);
+    }
     __syncthreads();
-    if (active) tile_ghosts_2d<double, KIND>(dp[p], p, n0, n1, U, tile, phys, lane);
+    if (active) tile_ghosts_2d<double, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
     __syncthreads();
     if (active) {
       const DevPatch& P = dp[p];
       const double ih2[2] = { P.inv_h2[0], P.inv_h2[1] };
-      const int base = p * L.ps;
       for (int q = lane; q < n0 * n1; q += 64) {
         const int y = q / n0, x = q - y * n0;
         const int c = (y + 1) * L.sx + x + 1;
@@ -233,7 +244,9 @@ int launch_resid_restrict(const te_ghost_filler* g, const te_level_link* link, i
 
 bool star_dot_fly_supported(const te_ghost_filler* g)
 {
-  return g->d->dim == 2 && tile_fits(g->d) && (g->kind == TE_BILINEAR || g->kind == TE_BIQUADRATIC);
+  const te_domain* d = g->d;
+  return d->dim == 2 && d->ns[0] % 2 == 0 && d->ns[1] % 2 == 0 && tile_fits(d) &&
+         (g->kind == TE_BILINEAR || g->kind == TE_BIQUADRATIC);
 }
 
 int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
@@ -241,7 +254,7 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
 {
   const te_domain* d = g->d;
   if (!star_dot_fly_supported(g)) {
-    set_error("fused 2D operator: needs 2D, BiLinear or BiQuadratic ghosts and a tile within 64 KiB");
+    set_error("fused 2D operator: needs 2D, even n, BiLinear or BiQuadratic ghosts and a tile within 64 KiB");
     return TE_ERR_UNSUPPORTED;
   }
   Layout L = make_layout(2, d->ns, 1, d->np);

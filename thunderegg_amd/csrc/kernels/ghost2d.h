@@ -24,21 +24,33 @@ __device__ __forceinline__ void side_xy(int s, int k, int t, int n, int* x, int*
   }
 }
 
-// n: cells along the side's axis (the depth direction); nt: cells along the side
-template <typename T, int KIND, class Acc>
-__device__ __forceinline__ T ghost2d(const DevPatch& P, int p, int s, int t, int n, int nt, Acc U)
+// What a ghost formula needs to know about one side of a patch (loaded ahead of use).
+struct SideInfo {
+  int type;  // -1 physical boundary, 0 same level, 1 coarser, 2 finer neighbours
+  int orth;  // for a coarser neighbour: which half of its side this patch touches
+  int nbr0, nbr1;
+};
+
+__device__ __forceinline__ SideInfo side_info(const DevPatch& P, int s)
 {
   const int slot = 4 + s;
-  const int type = P.type[slot];
+  return SideInfo{ P.type[slot], P.orth[slot], P.nbr[slot][0], P.nbr[slot][1] };
+}
+
+// n: cells along the side's axis (the depth direction); nt: cells along the side (even)
+template <typename T, int KIND, class Acc>
+__device__ __forceinline__ T ghost2d(const SideInfo& si, int p, int s, int t, int n, int nt, Acc U)
+{
+  const int type = si.type;
   int x, y;
   auto S = [&](int q, int side, int k, int tt) -> T {
     side_xy(side, k, tt, n, &x, &y);
     return U(q, x, y);
   };
-  if (type == 0) return S(P.nbr[slot][0], s ^ 1, 0, t);
+  if (type == 0) return S(si.nbr0, s ^ 1, 0, t);
   if (type == 1) {
-    const int q = P.nbr[slot][0];
-    const int o = P.orth[slot];
+    const int q = si.nbr0;
+    const int o = si.orth;
     if (KIND == TE_BILINEAR) {
       const int off = o ? nt : 0;
       T G = T(0);
@@ -55,42 +67,49 @@ __device__ __forceinline__ T ghost2d(const DevPatch& P, int p, int s, int t, int
       return G + T(2.0 / 3.0) * S(q, s ^ 1, 0, (t + off) / 2);
     }
     T G = T(2) * S(p, s, 0, t) / T(3) - S(p, s, 1, t) / T(5);
-    T add;
-#define TE_CC(k) S(q, s ^ 1, 0, (k))
-    if (o == 0) {
-      if (t == 0) add = T(3) * TE_CC(0) / T(4) - T(3) * TE_CC(1) / T(10) + TE_CC(2) / T(12);
-      else if (t == 1) add = T(7) * TE_CC(0) / T(20) + T(7) * TE_CC(1) / T(30) - TE_CC(2) / T(20);
-      else if (t % 2 == 0) add = TE_CC(t / 2 - 1) / T(12) + TE_CC(t / 2) / T(2) - TE_CC(t / 2 + 1) / T(20);
-      else add = -TE_CC(t / 2 - 1) / T(20) + TE_CC(t / 2) / T(2) + TE_CC(t / 2 + 1) / T(12);
+    // three cells of the coarse neighbour's side, i0, i0 + di, i0 + 2 di, loaded
+    // unconditionally; the reference's four weight patterns are then applied to them
+    // (so the compiler does not keep the loads of every case live at once)
+    const int e = o == 0 ? t : nt - 1 - t;  // distance from the end of the coarse side
+    int i0, di, cs;
+    if (e <= 1) {
+      i0 = o == 0 ? 0 : nt - 1;
+      di = o == 0 ? 1 : -1;
+      cs = e;
     } else {
-      if (t == nt - 1) add = T(3) * TE_CC(nt - 1) / T(4) - T(3) * TE_CC(nt - 2) / T(10) + TE_CC(nt - 3) / T(12);
-      else if (t == nt - 2) add = T(7) * TE_CC(nt - 1) / T(20) + T(7) * TE_CC(nt - 2) / T(30) - TE_CC(nt - 3) / T(20);
-      else {
-        int m = (t + nt) / 2;
-        if ((t + nt) % 2 == 0) add = TE_CC(m - 1) / T(12) + TE_CC(m) / T(2) - TE_CC(m + 1) / T(20);
-        else add = -TE_CC(m - 1) / T(20) + TE_CC(m) / T(2) + TE_CC(m + 1) / T(12);
-      }
+      const int tt = o == 0 ? t : t + nt;
+      i0 = tt / 2 - 1;
+      di = 1;
+      cs = 2 + (tt & 1);
     }
-#undef TE_CC
+    const T c0 = S(q, s ^ 1, 0, i0), c1 = S(q, s ^ 1, 0, i0 + di), c2 = S(q, s ^ 1, 0, i0 + 2 * di);
+    T add;
+    if (cs == 0) add = T(3) * c0 / T(4) - T(3) * c1 / T(10) + c2 / T(12);
+    else if (cs == 1) add = T(7) * c0 / T(20) + T(7) * c1 / T(30) - c2 / T(20);
+    else if (cs == 2) add = c0 / T(12) + c1 / T(2) - c2 / T(20);
+    else add = -c0 / T(20) + c1 / T(2) + c2 / T(12);
     return G + add;
   }
-  // fine neighbours
+  // fine neighbours (patch sides of even length: ghost t lies against fine cells 2t, 2t + 1
+  // of neighbour 0 when 2t < nt, else of neighbour 1)
   T G;
   if (KIND == TE_BILINEAR) {
     G = T(-1.0 / 3.0) * S(p, s, 0, t);
   } else {
-    if (t == 0) G = -S(p, s, 0, 0) / T(10) + S(p, s, 0, 1) / T(15) - S(p, s, 0, 2) / T(30);
-    else if (t == nt - 1) G = -S(p, s, 0, nt - 1) / T(10) + S(p, s, 0, nt - 2) / T(15) - S(p, s, 0, nt - 3) / T(30);
-    else G = -S(p, s, 0, t - 1) / T(30) - S(p, s, 0, t + 1) / T(30);
-  }
-  for (int i = 0; i < 2; i++) {
-    const int q = P.nbr[slot][i];
-    const int off = i * nt;
-    for (int tf = 2 * t - off; tf <= 2 * t - off + 1; tf++) {
-      if (tf < 0 || tf >= nt) continue;
-      if (KIND == TE_BILINEAR) G += T(2.0 / 3.0) * S(q, s ^ 1, 0, tf);
-      else G += S(q, s ^ 1, 0, tf) / T(3) + S(q, s ^ 1, 1, tf) / T(5);
+    if (t == 0 || t == nt - 1) {
+      const int i0 = t == 0 ? 0 : nt - 1, di = t == 0 ? 1 : -1;
+      G = -S(p, s, 0, i0) / T(10) + S(p, s, 0, i0 + di) / T(15) - S(p, s, 0, i0 + 2 * di) / T(30);
+    } else {
+      G = -S(p, s, 0, t - 1) / T(30) - S(p, s, 0, t + 1) / T(30);
     }
+  }
+  const int i = 2 * t >= nt ? 1 : 0;
+  const int q = i ? si.nbr1 : si.nbr0;
+  const int tf = 2 * t - i * nt;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    if (KIND == TE_BILINEAR) G += T(2.0 / 3.0) * S(q, s ^ 1, 0, tf + j);
+    else G += S(q, s ^ 1, 0, tf + j) / T(3) + S(q, s ^ 1, 1, tf + j) / T(5);
   }
   return G;
 }
@@ -135,18 +154,36 @@ __device__ __forceinline__ void tile_interior_2d(int p, int n0, int n1, Acc U, T
   }
 }
 
+// Side information for ring cell lane + 64 j of a patch (j < GMAX), loaded ahead of the
+// values it is combined with.
+template <int GMAX>
+__device__ __forceinline__ void ring_prefetch(const DevPatch& P, int n0, int n1, int lane, SideInfo (&si)[GMAX])
+{
+#pragma unroll
+  for (int j = 0; j < GMAX; j++) {
+    const int g = lane + 64 * j;
+    int s = 0, t;
+    if (g < 2 * (n0 + n1)) ring_cell(g, n0, n1, &s, &t);
+    si[j] = side_info(P, s);
+  }
+}
+
 // Side ghosts of patch p (after its interior is in the tile and visible to the wave):
 // a lane per ghost cell, neighbour sides by ghost2d, physical sides as
 // phys * (adjacent interior value) (the Dirichlet / Neumann image, phys = -1 / +1).
+// si0: side information of the lane's first ring cell (ring_prefetch<1>); later ones are
+// loaded here.  The loop is kept rolled so that one copy of the ghost formulas is live.
 template <typename T, int KIND, class Acc>
-__device__ __forceinline__ void tile_ghosts_2d(const DevPatch& P, int p, int n0, int n1, Acc U, T* tile, T phys,
-                                               int lane)
+__device__ __forceinline__ void tile_ghosts_2d(const DevPatch& P, const SideInfo& si0, int p, int n0, int n1, Acc U,
+                                               T* tile, T phys, int lane)
 {
   const int w = n0 + 2;
   TileAcc<T, Acc> A{ tile, p, w, U };
+#pragma unroll 1
   for (int g = lane; g < 2 * (n0 + n1); g += 64) {
     int s, t;
     ring_cell(g, n0, n1, &s, &t);
+    const SideInfo si = g == lane ? si0 : side_info(P, s);
     const int a = s >> 1, up = s & 1;
     const int na = a == 0 ? n0 : n1, nt = a == 0 ? n1 : n0;
     int bx, by;
@@ -154,8 +191,8 @@ __device__ __forceinline__ void tile_ghosts_2d(const DevPatch& P, int p, int n0,
     const int d = up ? 1 : -1;
     const int gx = a == 0 ? bx + d : bx, gy = a == 0 ? by : by + d;
     T v;
-    if (P.type[4 + s] < 0) v = phys * tile[(by + 1) * w + bx + 1];
-    else v = ghost2d<T, KIND>(P, p, s, t, na, nt, A);
+    if (si.type < 0) v = phys * tile[(by + 1) * w + bx + 1];
+    else v = ghost2d<T, KIND>(si, p, s, t, na, nt, A);
     tile[(gy + 1) * w + gx + 1] = v;
   }
 }
