@@ -10,28 +10,38 @@
  * HIP kernels instead gather ("pull") every ghost value, so the two
  * formulations are independent and only agree if both encode the same weights.
  *
- * How each part is pinned (the checks live in tests/test_oracle*.py):
- *  - tables (Domain, enumerateCalls): the reference's mesh fixtures
- *    (the JSON meshes under test/mesh_inputs) and the call-set invariants of
- *    test/MPIGhostFiller_MPI1.cpp; closed-form call counts.
+ * How each part is pinned (tests/test_oracle_pins.py; the forest and the mesh
+ * tables are checked in tests/test_host.py):
+ *  - tables (Domain, enumerateCalls): the reference's mesh fixtures (the JSON
+ *    meshes of test/mesh_inputs); the local face calls are exactly one per
+ *    (patch, side) neighbour, in patch / face / orthant order, with the table's
+ *    neighbour indexes (test/MPIGhostFiller_MPI1.cpp).
  *  - ghost fillers: exactness on linear (BiLinear, TriLinear) and quadratic
  *    (BiQuadratic) functions at ghost-cell centres on the reference's refined
- *    meshes (test/BiLinearGhostFiller_MPI1.cpp:254-300,
- *    test/BiQuadraticGhostFiller_MPI1.cpp:254-300,
- *    test/TriLinearGhostFiller_MPI1.cpp:261-310); weights of every ghost sum to 1.
- *  - star operators: exactness on quadratics with a known Laplacian, 2nd-order
- *    convergence (test/Poisson/StarPatchOperator_MPI1.cpp:176-271,
- *    test/VarPoisson/StarPatchOperator_MPI1.cpp:133-228); symmetric-negative
- *    definiteness of the assembled dense matrix.
- *  - DFT patch solver: its matrices against scipy.fft dst/dct (types II/III/IV);
- *    the solve is the exact inverse of the single-patch operator with internal
- *    boundary conditions (dense brute force).
- *  - restrict / interpolate: known answers of test/GMG/LinearRestrictor_MPI1.cpp
- *    and test/GMG/DirectInterpolator_MPI1.cpp, exactness on constants/linears.
- *  - V-cycle: the two-grid / multi-grid error-propagation identity built from the
- *    dense matrices of the pinned components (S, R, P, A) in numpy.
- *  - BiCGStab: converges to the dense direct solve; the 2I, zero-RHS and
- *    converged-initial-guess known answers of test/Iterative/BiCGStab_MPI1.cpp.
+ *    meshes, with zeroed and with garbage ghosts, even and odd n
+ *    (test/BiLinearGhostFiller_MPI1.cpp, test/BiQuadraticGhostFiller_MPI1.cpp,
+ *    test/TriLinearGhostFiller_MPI1.cpp); a cubic is not reproduced (the test
+ *    has teeth).
+ *  - star operators: ghost fill + stencil give the exact Laplacian of quadratics
+ *    away from the physical boundary on refined meshes; 2nd-order convergence on
+ *    uniform meshes (test/Poisson/StarPatchOperator_MPI1.cpp,
+ *    test/VarPoisson/StarPatchOperator_MPI1.cpp); VarPoisson with constant beta is
+ *    beta x Poisson; the uniform-mesh matrix is symmetric negative definite.
+ *  - DFT patch solver: every transform matrix against scipy.fft dst/dct
+ *    (types II/III/IV); the solve is the exact inverse of the single-patch
+ *    operator (2D and 3D); 2nd-order convergence of one block-Jacobi sweep
+ *    (test/Poisson/DFTPatchSolver_MPI1.cpp).
+ *  - restrict / interpolate: exactness on linears (test/GMG/LinearRestrictor_MPI1.cpp)
+ *    and the known answer of test/GMG/DirectInterpolator_MPI1.cpp.
+ *  - V-cycle: one level and one patch is the exact inverse; two levels equal the
+ *    VCycle::visit sequence composed from the separately pinned smoother,
+ *    operator, restrictor and interpolator.
+ *  - BiCGStab: converges to the dense direct solve of the assembled operator
+ *    (plain, preconditioned, corrected early exit); zero right-hand side and a
+ *    converged initial guess take 0 iterations (test/Iterative/BiCGStab_MPI1.cpp).
+ *  Parity unpinned: the Neumann variants (DCT transforms inside the solve, Neumann
+ *  physical ghosts) are checked only through the matrices above, and the inner
+ *  BiCGStab patch smoother only through its convergence to the patch solve.
  */
 #include "te_oracle.h"
 

@@ -1,0 +1,476 @@
+"""CPU tests that pin the oracle (oracle/te_oracle.c) before it is trusted as the
+checker of the HIP path.  Each check is one the reference's own tests make, or one
+the mathematics fixes, chosen so that a dropped term, a wrong sign or index, or a
+transposed operand fails it:
+
+  ghost fillers   polynomial exactness on the reference's fixture meshes
+                  (test/BiLinearGhostFiller_MPI1.cpp, test/BiQuadraticGhostFiller_MPI1.cpp,
+                  test/TriLinearGhostFiller_MPI1.cpp), also with ghosts already set;
+  operators       exactness of ghost fill + star stencil on polynomials at every cell
+                  away from the physical boundary, second-order convergence
+                  (test/Poisson/StarPatchOperator_MPI1.cpp), VarPoisson reducing to
+                  Poisson for constant beta;
+  DFT solver      DST matrices against scipy, exact inverse of the single-patch
+                  operator with internal boundary conditions, second-order convergence
+                  (test/Poisson/DFTPatchSolver_MPI1.cpp);
+  transfers       LinearRestrictor exact on linear functions
+                  (test/GMG/LinearRestrictor_MPI1.cpp), DirectInterpolator known answer
+                  (test/GMG/DirectInterpolator_MPI1.cpp);
+  BiCGStab        against a dense direct solve of the assembled operator (brute force
+                  on a 64-DOF mesh), zero right-hand side, converged initial guess
+                  (test/Iterative/BiCGStab_MPI1.cpp);
+  V-cycle         one patch and one level: the cycle is the exact inverse; two levels:
+                  the cycle equals its definition (VCycle.h:46-70) composed from the
+                  separately pinned smoother, operator, restrictor and interpolator.
+"""
+import numpy as np
+import pytest
+import scipy.fft
+
+from helpers import MESH_2D, MESH_3D, load, random_field, relerr
+from oracle import oracle as O
+from thunderegg_amd import mesh
+
+BIL, BIQ, TRI = 0, 1, 2
+FACES, EDGES, CORNERS = 0, 1, 2
+
+MESH_2D_REF = ["2d_uniform_2x2_mpi1.json", "2d_uniform_2x2_refined_nw_mpi1.json",
+               "2d_uniform_8x8_refined_cross_mpi1.json"]
+
+
+def _fun2(kind):
+    if kind == BIL:
+        return lambda x, y: 1 + ((x * 0.3) + y)
+    return lambda x, y: 1 + ((x * x + 0.3 * x + 0.3) + (0.5 * y * y))
+
+
+def _check_filled_2d(lv, dim, ns, u, expected, fill_type):
+    err = 0.0
+    for p in range(len(lv)):
+        t = lv["nbr_type"][p]
+        for s in range(4):
+            if t[mesh.side_slot(2, s)] < 0:
+                continue
+            sl = [(slice(1, -1), 0), (slice(1, -1), -1), (0, slice(1, -1)), (-1, slice(1, -1))][s]
+            err = max(err, np.abs(u[p, 0][sl] - expected[p, 0][sl]).max())
+        if fill_type == CORNERS:
+            for c in range(4):
+                if t[c] < 0:
+                    continue
+                iy, ix = (-1 if c & 2 else 0), (-1 if c & 1 else 0)
+                err = max(err, abs(u[p, 0, iy, ix] - expected[p, 0, iy, ix]))
+    return err
+
+
+@pytest.mark.parametrize("name", MESH_2D_REF)
+@pytest.mark.parametrize("kind,nss", [(BIL, [(2, 2), (10, 10), (10, 2)]), (BIQ, [(10, 10), (13, 13), (10, 13)])])
+@pytest.mark.parametrize("fill_type", [FACES, CORNERS])
+def test_ghost_filler_2d_exact(name, kind, nss, fill_type):
+    fun = _fun2(kind)
+    for ns in nss:
+        dim, levels = load(name, ns)
+        lv = levels[0]
+        d = O.Domain(dim, ns, lv)
+        X = mesh.cell_centers(lv, dim, ns)
+        expected = fun(X[..., 0], X[..., 1])[:, None]
+        for ghosts_preset in (False, True):
+            u = expected.copy()
+            if not ghosts_preset:  # ghosts start at zero
+                u[..., 0, :] = 0
+                u[..., -1, :] = 0
+                u[..., :, 0] = 0
+                u[..., :, -1] = 0
+            else:  # "ghost already set": garbage in the ghosts must not leak into the result
+                g = random_field(u.shape, 5)
+                for sl in [(Ellipsis, 0, slice(None)), (Ellipsis, -1, slice(None)), (Ellipsis, slice(None), 0),
+                           (Ellipsis, slice(None), -1)]:
+                    u[sl] = g[sl]
+            O.fill_ghost(d, u, kind=kind, fill_type=fill_type)
+            assert np.array_equal(mesh.interior(u, dim), mesh.interior(expected, dim))
+            err = _check_filled_2d(lv, dim, ns, u, expected, fill_type)
+            assert err <= 1e-12 * np.abs(expected).max(), (ns, ghosts_preset, err)
+
+
+def test_biquadratic_is_not_exact_for_cubics():
+    """The exactness test has teeth: a cubic breaks BiQuadratic at coarse-fine faces."""
+    ns = (10, 10)
+    dim, levels = load("2d_uniform_2x2_refined_nw_mpi1.json", ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    X = mesh.cell_centers(lv, dim, ns)
+    expected = (X[..., 0] ** 3 + X[..., 1] ** 3)[:, None]
+    u = expected.copy()
+    O.fill_ghost(d, u, kind=BIQ, fill_type=FACES)
+    assert _check_filled_2d(lv, dim, ns, u, expected, FACES) > 1e-6
+
+
+@pytest.mark.parametrize("name", MESH_3D)
+@pytest.mark.parametrize("fill_type", [FACES, EDGES, CORNERS])
+def test_trilinear_exact_3d(name, fill_type):
+    ns = (4, 6, 8)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    X = mesh.cell_centers(lv, dim, ns)
+    expected = (1 + 0.3 * X[..., 0] + X[..., 1] - 0.7 * X[..., 2])[:, None]
+    u = expected.copy()
+    mask = np.zeros(u.shape, dtype=bool)
+    mask[..., 1:-1, 1:-1, 1:-1] = True
+    u[~mask] = 0.0
+    O.fill_ghost(d, u, kind=TRI, fill_type=fill_type)
+    from helpers import face_ghost_mask
+    gm = face_ghost_mask(lv, dim, ns, fill_type)
+    assert gm.any()
+    assert np.abs(u[gm] - expected[gm]).max() <= 1e-12 * np.abs(expected).max()
+
+
+def _away_from_boundary(lv, dim, ns):
+    """Mask of interior cells whose stencil does not touch a physical boundary."""
+    shp = mesh.padded_shape(len(lv), dim, ns)
+    m = np.zeros(shp, dtype=bool)
+    m[(Ellipsis,) + (slice(1, -1),) * dim] = True
+    for p in range(len(lv)):
+        for s in range(2 * dim):
+            if lv["nbr_type"][p][mesh.side_slot(dim, s)] >= 0:
+                continue
+            a, up = s >> 1, s & 1
+            idx = [slice(None)] * dim
+            idx[dim - 1 - a] = -2 if up else 1
+            m[(p, 0) + tuple(idx)] = False
+    return m
+
+
+@pytest.mark.parametrize("name", MESH_2D)
+def test_poisson_apply_exact_on_quadratics(name):
+    """Ghost fill (BiQuadratic) + 5-point stencil reproduce the Laplacian of a quadratic at
+    every cell not next to the physical boundary, including coarse-fine interfaces."""
+    ns = (10, 10)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    X = mesh.cell_centers(lv, dim, ns)
+    u = (1 + X[..., 0] * X[..., 0] + 0.3 * X[..., 0] + 0.5 * X[..., 1] * X[..., 1] - 0.2 * X[..., 0] * X[..., 1])[:, None]
+    f = O.poisson_apply(d, u.copy(), kind=BIQ)
+    m = _away_from_boundary(lv, dim, ns)
+    assert np.abs(f[m] - 3.0).max() <= 1e-8  # 2 + 1
+    # a linear function is harmonic: BiLinear suffices
+    ul = (1 + 5 * X[..., 0] + 13 * X[..., 1])[:, None]
+    fl = O.poisson_apply(d, ul.copy(), kind=BIL)
+    assert np.abs(fl[m]).max() <= 1e-8
+
+
+def _sin_problem(lv, dim, ns):
+    X = mesh.cell_centers(lv, dim, ns)
+    x, y = X[..., 0], X[..., 1]
+    g = np.sin(np.pi * x) * np.sin(2 * np.pi * y)
+    return g[:, None], (-5 * np.pi ** 2 * g)[:, None]
+
+
+@pytest.mark.parametrize("name", ["2d_uniform_2x2_mpi1.json", "2d_uniform_4x4_mpi1.json"])
+def test_poisson_second_order(name):
+    """Uniform meshes (test/Poisson/StarPatchOperator_MPI1.cpp sizes 32 -> 64).  On refined
+    meshes the linear interface interpolation leaves an O(1) truncation error on a layer
+    of O(1/h) cells, so the global order there is 1/2 (BiLinear) or about 3/2 (BiQuadratic);
+    those meshes are pinned by the polynomial exactness tests instead."""
+    errs = []
+    for i in (1, 2):
+        ns = (32 * i, 32 * i)
+        dim, levels = load(name, ns)
+        lv = levels[0]
+        d = O.Domain(dim, ns, lv)
+        g, fexact = _sin_problem(lv, dim, ns)
+        f = O.poisson_apply(d, g.copy(), kind=BIL)
+        e = mesh.interior(f - fexact, dim)
+        errs.append(np.sqrt((e * e).sum() / (mesh.interior(fexact, dim) ** 2).sum()))
+    assert np.log(errs[0] / errs[1]) / np.log(2) > 1.8
+
+
+def test_varpoisson_constant_beta_is_poisson():
+    """beta = 1: (c+ + c)(u+ - u) - (c- + c)(u - u-) over 2h^2 is the 5-point Laplacian."""
+    ns = (8, 8)
+    dim, levels = load("2d_uniform_2x2_refined_nw_mpi1.json", ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    u = random_field(d.shape(), 4)
+    c = np.ones_like(u)
+    a = O.poisson_apply(d, u.copy(), kind=BIL)
+    b = O.varpoisson_apply(d, c, u.copy(), kind=BIL)
+    assert relerr(mesh.interior(b, dim), mesh.interior(a, dim)) <= 1e-13
+    # beta = 3 scales the operator
+    b3 = O.varpoisson_apply(d, 3 * c, u.copy(), kind=BIL)
+    assert relerr(mesh.interior(b3, dim), 3 * mesh.interior(a, dim)) <= 1e-13
+
+
+@pytest.mark.parametrize("n", [4, 10, 13, 16])
+def test_dft_matrices_match_scipy(n):
+    """DST-II (kind 3 in the plan's table) and DST-III (kind 4) against scipy's
+    unnormalised transforms: the reference's matrices are scipy's / 2."""
+    e = np.eye(n)
+    s2 = O.dft_matrix(3, n)
+    s3 = O.dft_matrix(4, n)
+    ref2 = scipy.fft.dst(e, type=2, axis=0) / 2
+    ref3 = scipy.fft.dst(e, type=3, axis=0) / 2
+    assert np.allclose(s2, ref2, atol=1e-13)
+    assert np.allclose(s3, ref3, atol=1e-13)
+    # S3 S2 = (n / 2) I: the transform pair inverts up to the scale prod(2/n)
+    assert np.allclose(s3 @ s2, (n / 2) * np.eye(n), atol=1e-11)
+
+
+def test_dft_solve_is_exact_inverse_single_patch():
+    """One patch, homogeneous Dirichlet: the DFT smooth solves A u = f exactly."""
+    ns = (12, 12)
+    dim, levels = load("2d_uniform_2x2_mpi1.json", ns)
+    lv = levels[-1]  # coarsest level: one patch
+    assert len(lv) == 1
+    d = O.Domain(dim, ns, lv)
+    f = random_field(d.shape(), 8)
+    u = np.zeros_like(f)
+    O.dft_smooth(d, f, u, kind=BIL)
+    back = O.poisson_apply(d, u.copy(), kind=BIL)
+    assert relerr(mesh.interior(back, dim), mesh.interior(f, dim)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["2d_uniform_2x2_mpi1.json", "2d_uniform_8x8_refined_cross_mpi1.json"])
+@pytest.mark.parametrize("nx", [10, 13])
+def test_dft_second_order(name, nx):
+    """test/Poisson/DFTPatchSolver_MPI1.cpp: one block-Jacobi sweep from the exact solution
+    with interface ghosts filled from it converges at second order (homogeneous
+    Dirichlet variant of the reference problem)."""
+    errs = []
+    for i in (1, 2):
+        ns = (nx * i, nx * i)
+        dim, levels = load(name, ns)
+        lv = levels[0]
+        d = O.Domain(dim, ns, lv)
+        g, f = _sin_problem(lv, dim, ns)
+        u = g.copy()
+        O.dft_smooth(d, f, u, kind=BIL)
+        e = mesh.interior(u - g, dim)
+        errs.append(np.sqrt((e * e).sum() / (mesh.interior(g, dim) ** 2).sum()))
+    assert np.log(errs[0] / errs[1]) / np.log(2) > 1.8
+
+
+@pytest.mark.parametrize("name", ["2d_uniform_4x4_mpi1.json", "2d_uniform_2x2_refined_nw_mpi1.json"])
+@pytest.mark.parametrize("ns", [(2, 2), (10, 10), (2, 10)])
+def test_restrict_exact_on_linear(name, ns):
+    dim, levels = load(name, ns)
+    fine, coarse = O.Domain(dim, ns, levels[0]), O.Domain(dim, ns, levels[1])
+    fine.link_parents(coarse)
+    fun = _fun2(BIL)
+    Xf = mesh.cell_centers(levels[0], dim, ns)
+    Xc = mesh.cell_centers(levels[1], dim, ns)
+    fv = fun(Xf[..., 0], Xf[..., 1])[:, None]
+    cexp = fun(Xc[..., 0], Xc[..., 1])[:, None]
+    cv = O.restrict(fine, coarse, fv)
+    assert relerr(mesh.interior(cv, dim), mesh.interior(cexp, dim)) <= 1e-13
+
+
+@pytest.mark.parametrize("name", ["2d_uniform_4x4_mpi1.json", "2d_uniform_2x2_refined_nw_mpi1.json"])
+@pytest.mark.parametrize("nx,ny", [(2, 2), (10, 10), (2, 10)])
+def test_interpolate_known_answer(name, nx, ny):
+    """test/GMG/DirectInterpolator_MPI1.cpp: coarse cell value 1 + id nx ny + i + j nx lands
+    on its fine cells (piecewise-constant prolongation), added to what is there."""
+    ns = (nx, ny)
+    dim, levels = load(name, ns)
+    fl, cl = levels[0], levels[1]
+    fine, coarse = O.Domain(dim, ns, fl), O.Domain(dim, ns, cl)
+    fine.link_parents(coarse)
+    cv = coarse.zeros()
+    for p in range(len(cl)):
+        for j in range(ny):
+            for i in range(nx):
+                cv[p, 0, j + 1, i + 1] = 1 + cl["id"][p] * nx * ny + i + j * nx
+    base = random_field(fine.shape(), 2)
+    fv = base.copy()
+    O.interpolate(coarse, fine, cv, fv)
+    exp = base.copy()
+    for p in range(len(fl)):
+        o = fl["orth_on_parent"][p]
+        pid = fl["parent_id"][p]
+        sx = 0 if o < 0 or not (o & 1) else nx
+        sy = 0 if o < 0 or not (o & 2) else ny
+        if o < 0:
+            sx = sy = 0
+        for j in range(ny):
+            for i in range(nx):
+                if o < 0:
+                    ci, cj = i, j
+                else:
+                    ci, cj = (i + sx) // 2, (j + sy) // 2
+                exp[p, 0, j + 1, i + 1] += 1 + pid * nx * ny + ci + cj * nx
+    assert relerr(mesh.interior(fv, dim), mesh.interior(exp, dim)) <= 1e-15
+
+
+def _dense_operator(d, dim, ns, kind=BIL):
+    """Assemble A column by column from the oracle's apply (interior DOF only)."""
+    shp = d.shape()
+    inner = np.zeros(shp, dtype=bool)
+    inner[(Ellipsis,) + (slice(1, -1),) * dim] = True
+    pos = np.flatnonzero(inner.ravel())
+    A = np.zeros((len(pos), len(pos)))
+    for k, q in enumerate(pos):
+        e = np.zeros(shp)
+        e.ravel()[q] = 1.0
+        A[:, k] = O.poisson_apply(d, e, kind=kind).ravel()[pos]
+    return A, pos
+
+
+def test_bicgstab_against_dense_solve():
+    ns = (4, 4)
+    dim, levels = load("2d_uniform_2x2_refined_nw_mpi1.json", ns)
+    ods = [O.Domain(dim, ns, lv) for lv in levels]
+    A, pos = _dense_operator(ods[0], dim, ns)
+    assert A.shape[0] == len(levels[0]) * 16
+    b = random_field(ods[0].shape(), 12)
+    xd = np.linalg.solve(A, b.ravel()[pos])
+    L = O.Levels(ods, filler_kind=BIL, smoother_kind=0)
+    for precond in (False, True, 2):
+        x, its, hist = L.bicgstab(b.copy(), precond=precond, tol=1e-13)
+        assert its > 0 and hist[-1] <= 1e-13
+        assert relerr(x.ravel()[pos], xd) <= 1e-9, precond
+
+
+def test_bicgstab_zero_rhs_and_converged_guess():
+    ns = (8, 8)
+    dim, levels = load("2d_uniform_2x2_mpi1.json", ns)
+    ods = [O.Domain(dim, ns, lv) for lv in levels]
+    L = O.Levels(ods, filler_kind=BIL, smoother_kind=0)
+    z = np.zeros(ods[0].shape())
+    x0 = random_field(z.shape, 3)
+    x, its, _ = L.bicgstab(z, x=x0.copy())
+    # zero iterations and x untouched (its ghost cells are workspace of the operator)
+    assert its == 0 and np.array_equal(mesh.interior(x, dim), mesh.interior(x0, dim))
+    g, f = _sin_problem(levels[0], dim, ns)
+    # (the corrected early exit, so that xs is as good as its recursive residual says)
+    xs, _, _ = L.bicgstab(f.copy(), tol=1e-12, precond=2)
+    _, its2, _ = L.bicgstab(f.copy(), x=xs.copy(), tol=1e-6)
+    assert its2 == 0
+
+
+def test_vcycle_one_patch_is_exact_inverse():
+    ns = (8, 8)
+    dim, levels = load("2d_uniform_2x2_mpi1.json", ns)
+    d = O.Domain(dim, ns, levels[-1])
+    L = O.Levels([d], filler_kind=BIL, smoother_kind=0)
+    f = random_field(d.shape(), 6)
+    u = L.vcycle(f.copy())
+    back = O.poisson_apply(d, u.copy(), kind=BIL)
+    assert relerr(mesh.interior(back, dim), mesh.interior(f, dim)) <= 1e-12
+
+
+@pytest.mark.parametrize("name,kind", [("2d_uniform_2x2_refined_nw_mpi1.json", BIL),
+                                       ("2d_uniform_8x8_refined_cross_mpi1.json", BIQ)])
+def test_vcycle_two_levels_matches_definition(name, kind):
+    """VCycle::visit (GMG/VCycle.h:46-70) with one pre, one post and one coarse sweep:
+    u1 = S(f, 0); r = f - A u1; fc = R r; uc = S_c(fc, 0); u2 = u1 + P uc; u = S(f, u2)."""
+    ns = (8, 8)
+    dim, levels = load(name, ns)
+    levels = levels[-2:]
+    fd, cd = O.Domain(dim, ns, levels[0]), O.Domain(dim, ns, levels[1])
+    L = O.Levels([fd, cd], filler_kind=kind, smoother_kind=0)
+    f = random_field(fd.shape(), 9)
+    got = L.vcycle(f.copy())
+    u1 = np.zeros_like(f)
+    O.dft_smooth(fd, f, u1, kind=kind)
+    r = f - O.poisson_apply(fd, u1.copy(), kind=kind)
+    fc = O.restrict(fd, cd, r)
+    uc = np.zeros_like(fc)
+    O.dft_smooth(cd, fc, uc, kind=kind)
+    u2 = u1.copy()
+    O.interpolate(cd, fd, uc, u2)
+    O.dft_smooth(fd, f, u2, kind=kind)
+    assert relerr(mesh.interior(got, dim), mesh.interior(u2, dim)) <= 1e-13
+
+
+@pytest.mark.parametrize("name", MESH_2D + MESH_3D)
+def test_enumerated_calls_match_tables(name):
+    """test/MPIGhostFiller_MPI1.cpp: the local face calls are one per (patch, side) with a
+    normal or coarse neighbour and one per fine neighbour (2^(D-1) per side), ordered by
+    patch, then face value, then orthant."""
+    dim = 2 if name.startswith("2d") else 3
+    ns = (4,) * dim
+    _, levels = load(name, ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    calls = d.calls(FACES)
+    sides = calls[calls[:, 0] == dim - 1]
+    expect = []
+    for p in range(len(lv)):
+        for s in range(2 * dim):
+            t = int(lv["nbr_type"][p][mesh.side_slot(dim, s)])
+            if t < 0:
+                continue
+            for o in range(1 << (dim - 1) if t == 2 else 1):
+                expect.append((p, s, t))
+    got = [(int(c[1]), int(c[2]), int(c[3])) for c in sides]
+    assert got == expect
+    # neighbour local indexes are those of the tables
+    for c in sides:
+        p, s, t, o, nl = int(c[1]), int(c[2]), int(c[3]), int(c[4]), int(c[5])
+        k = o if t == 2 else 0
+        assert d.nbr_local(p, mesh.side_slot(dim, s), k) == nl
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_uniform_operator_is_symmetric_negative_definite(dim):
+    ns = (4,) * dim
+    name = "2d_uniform_2x2_mpi1.json" if dim == 2 else "3d_uniform_2x2x2_mpi1.json"
+    _, levels = load(name, ns)
+    d = O.Domain(dim, ns, levels[0])
+    shp = d.shape()
+    inner = np.zeros(shp, dtype=bool)
+    inner[(Ellipsis,) + (slice(1, -1),) * dim] = True
+    pos = np.flatnonzero(inner.ravel())
+    A = np.zeros((len(pos), len(pos)))
+    for k, q in enumerate(pos):
+        e = np.zeros(shp)
+        e.ravel()[q] = 1.0
+        A[:, k] = O.poisson_apply(d, e, kind=BIL if dim == 2 else TRI).ravel()[pos]
+    assert np.allclose(A, A.T, atol=1e-9 * np.abs(A).max())
+    assert np.linalg.eigvalsh(A).max() < 0
+
+
+@pytest.mark.parametrize("n", [4, 9, 16])
+def test_dct_and_dst4_matrices_match_scipy(n):
+    """The Neumann-side transforms of the DFT plan (kinds 0-2, 5: DCT-II, DCT-III, DCT-IV,
+    DST-IV) against scipy's unnormalised transforms (= 2 x the plan's matrices)."""
+    e = np.eye(n)
+    for kind, typ, fn in [(0, 2, scipy.fft.dct), (1, 3, scipy.fft.dct), (2, 4, scipy.fft.dct), (5, 4, scipy.fft.dst)]:
+        assert np.allclose(O.dft_matrix(kind, n), fn(e, type=typ, axis=0) / 2, atol=1e-13), kind
+
+
+def test_dft_solve_is_exact_inverse_single_patch_3d():
+    ns = (4, 6, 8)
+    dim, levels = load("3d_uniform_2x2x2_mpi1.json", ns)
+    lv = levels[-1]
+    assert len(lv) == 1
+    d = O.Domain(dim, ns, lv)
+    f = random_field(d.shape(), 18)
+    u = np.zeros_like(f)
+    O.dft_smooth(d, f, u, kind=TRI)
+    back = O.poisson_apply(d, u.copy(), kind=TRI)
+    assert relerr(mesh.interior(back, dim), mesh.interior(f, dim)) <= 1e-12
+
+
+def test_varpoisson_second_order():
+    """test/VarPoisson/StarPatchOperator_MPI1.cpp: beta = 1 + x y, u = sin(pi x) sin(2 pi y),
+    f = div(beta grad u), uniform mesh, sizes 32 -> 64."""
+    errs = []
+    for n in (32, 64):
+        ns = (n, n)
+        dim, levels = load("2d_uniform_2x2_mpi1.json", ns)
+        lv = levels[0]
+        d = O.Domain(dim, ns, lv)
+        X = mesh.cell_centers(lv, dim, ns)
+        x, y = X[..., 0], X[..., 1]
+        beta = (1 + x * y)[:, None]
+        O.fill_ghost(d, beta, kind=BIL)
+        u = (np.sin(np.pi * x) * np.sin(2 * np.pi * y))[:, None]
+        ux = np.pi * np.cos(np.pi * x) * np.sin(2 * np.pi * y)
+        uy = 2 * np.pi * np.sin(np.pi * x) * np.cos(2 * np.pi * y)
+        lap = -5 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(2 * np.pi * y)
+        fe = ((1 + x * y) * lap + y * ux + x * uy)[:, None]
+        f = O.varpoisson_apply(d, beta, u.copy(), kind=BIL)
+        e = mesh.interior(f - fe, dim)
+        errs.append(np.sqrt((e * e).sum() / (mesh.interior(fe, dim) ** 2).sum()))
+    assert np.log(errs[0] / errs[1]) / np.log(2) > 1.8
