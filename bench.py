@@ -29,7 +29,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-MFMA_FP32_PEAK_TFS = 157.3
 
 
 def log(*a):
@@ -54,53 +53,51 @@ def time_launches(R, fn, n, stream):
 
 
 def kernel_rooflines(R, P, stream, reps=50):
-    """Average launch time of the finest-level kernels (HIP events on their stream)
-    and their algorithmic bytes / flops per launch."""
+    """Average launch time (HIP events on the launch stream) of the finest-level kernels
+    of the mixed-precision V-cycle, launched standalone through the C ABI with the same
+    dtype combination as inside the solve, and their algorithmic bytes per launch (every
+    input read once, every output written once; ghost values computed from neighbours
+    are cache traffic, not algorithmic)."""
     import numpy as np
 
-    from thunderegg_amd import mesh
-
     dim, ns = P.dim, P.ns
-    dom = R.Domain(dim, ns, P.levels[0])
-    npch = len(P.levels[0])
+    lv0, lv1 = P.levels[0], P.levels[1]
+    d0, d1 = R.Domain(dim, ns, lv0), R.Domain(dim, ns, lv1)
+    n0, n1 = len(lv0), len(lv1)
     N = int(np.prod(ns))
-    if dim == 2:  # face ghost cells per patch
-        G = 2 * (ns[0] + ns[1])
-    else:
-        G = 2 * (ns[0] * ns[1] + ns[1] * ns[2] + ns[0] * ns[2])
-    # faces with a neighbour, per patch (ghost cells actually filled / read)
-    sb = 4 if dim == 2 else 20
-    has = P.levels[0]["nbr_type"][:, sb:sb + 2 * dim] >= 0
-    face_cells = np.array([ns[1], ns[1], ns[0], ns[0]] if dim == 2 else
-                          [ns[1] * ns[2]] * 2 + [ns[0] * ns[2]] * 2 + [ns[0] * ns[1]] * 2)
-    Gn = float((has * face_cells[None, :]).sum()) / npch  # neighbour-facing ghost cells per patch
+    rng = np.random.default_rng(1)
     out = {}
-    filler = R.GhostFiller(dom, P.filler, R.FACES)
-    for dt, s in ((R.F32, 4), (R.F64, 8)):
-        u = R.DeviceArray.from_host(np.random.default_rng(1).standard_normal(dom.shape(1)), R.np_dtype(dt))
-        f = R.DeviceArray.from_host(np.random.default_rng(2).standard_normal(dom.shape(1)), R.np_dtype(dt))
-        filler.fill_ghost(u)
-        tag = "f32" if dt == R.F32 else "f64"
-        # star stencil: s (N + G) read + s N write per patch (ghosts already filled)
-        t = time_launches(R, lambda: R.check(R.lib().te_star_apply_patches(filler.h, dt, 0, u.ptr, f.ptr, 1,
-                                                                           stream.h)), reps, stream)
-        byt = npch * s * (2 * N + G)
-        out[f"star_apply_{tag}"] = {"ms": t, "bytes": byt, "GBs": byt / t / 1e6}
-        # ghost fill: s (sources) + s (ghosts written), neighbour-facing faces
-        t = time_launches(R, lambda: filler.fill_ghost(u, stream=stream.h), reps, stream)
-        byt = npch * s * 2 * Gn
-        out[f"ghost_fill_{tag}"] = {"ms": t, "bytes": byt, "GBs": byt / t / 1e6}
-        if P.smoother == 0 and P.op_kind == 0:
-            plan = R.DftPlan(filler)
-            t = time_launches(R, lambda: R.check(R.lib().te_dft_patch_solve(plan.h, dt, f.ptr, u.ptr, stream.h)),
-                              reps, stream)
-            # f (N) + u ring for the interface terms (2 Gn) read, u (N) written; flops of the
-            # separable transforms: 2D 8 n^3 (+ n^2 divides) per patch
-            byt = npch * s * (2 * N + 2 * Gn)
-            fl = npch * (8 * ns[0] ** 3 + N) if dim == 2 else npch * (12 * ns[0] ** 4 + N)
-            out[f"dft_patch_solve_{tag}"] = {"ms": t, "bytes": byt, "GBs": byt / t / 1e6, "flops": fl,
-                                            "TFs": fl / t / 1e9}
-        del u, f
+    g = R.GhostFiller(d0, P.filler, R.FACES)
+    link = R.LevelLink(d0, d1)
+    f64 = R.DeviceArray.from_host(rng.standard_normal(d0.shape(1)))
+    u32 = R.DeviceArray.from_host(rng.standard_normal(d0.shape(1)), np.float32)
+    o32 = R.DeviceArray(d0.shape(1), np.float32)
+    o64 = R.DeviceArray(d0.shape(1), np.float64)
+    uc32 = R.DeviceArray.from_host(rng.standard_normal(d1.shape(1)), np.float32)
+    fc32 = R.DeviceArray(d1.shape(1), np.float32)
+
+    def rec(name, fn, byt, flops=None):
+        t = time_launches(R, fn, reps, stream)
+        out[name] = {"ms": t, "bytes": byt, "GBs": byt / t / 1e6}
+        if flops:
+            out[name].update({"flops": flops, "TFs": flops / t / 1e9})
+
+    if dim == 2 and P.smoother == 0:
+        plan = R.DftPlan(g, op_kind=P.op_kind)
+        fl = n0 * (8 * ns[0] ** 3 + N)  # four n^3 products of 2 n^3 flops, n^2 divides
+        # pre-smoothing from u = 0: f (f64) read, u (f32) written
+        rec("dft_apply_f32_in64", lambda: plan.apply_mixed(R.F32, f64, o32, stream=stream.h), n0 * N * (8 + 4), fl)
+        # Cycle::restrict: u (f32), f (f64) read; coarse f (f32) written
+        rec("residual_restrict_f32", lambda: link.residual_restrict(g, f64, u32, fc32, stream=stream.h),
+            n0 * N * (4 + 8) + n1 * N * 4)
+        # post-smoothing with the interpolation: f (f64), u (f32), coarse u (f32) read; u (f64) written
+        rec("dft_smooth_interp_f32_out64",
+            lambda: plan.smooth_interp(link, f64, u32, uc32, o64, stream=stream.h), n0 * N * (8 + 4 + 8) + n1 * N * 4,
+            fl)
+    # copy of a vector of known size (calibrates the HBM byte counters)
+    c64 = R.DeviceArray(d0.shape(1), np.float64)
+    rec("vec_copy_f64", lambda: R.check(R.lib().te_vec_copy(d0.h, R.F64, c64.ptr, f64.ptr, 1, stream.h)),
+        2 * n0 * N * 8)
     return out
 
 
@@ -146,6 +143,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the standalone kernel timings (for counter runs under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
@@ -163,6 +162,10 @@ def main():
     P = configs.CONFIGS[args.config]()
     t_mesh = time.perf_counter() - t0
     log(f"config {args.config}: {P.name}; levels {[len(l) for l in P.levels]}; DOF {P.dof}; mesh {t_mesh:.1f}s")
+    if args.kernels_only:
+        kr = kernel_rooflines(R, P, R.Stream(), reps=args.steps)
+        print(json.dumps({"kernels": kr}), flush=True)
+        return
 
     def make(precision):
         return solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, op_kind=P.op_kind,
@@ -251,20 +254,18 @@ def main():
         st = R.Stream()
         kr = kernel_rooflines(R, P, st)
         out["kernels"] = kr
-        dom_k = "dft_patch_solve_f32" if "dft_patch_solve_f32" in kr else "star_apply_f32"
+        # the dominant kernel: the longest of the finest-level V-cycle launches
+        cands = [k for k in kr if k != "vec_copy_f64"]
+        dom_k = max(cands, key=lambda k: kr[k]["ms"])
         k = kr[dom_k]
         traffic = None
-        tf = os.path.join(ROOT, "profiles", f"{dom_k}_cfg{args.config}_traffic.json")
+        tf = os.path.join(ROOT, "profiles", f"traffic_config{args.config}.json")
         if os.path.exists(tf):
             with open(tf) as fh:
-                traffic = json.load(fh).get("bytes_per_launch")
-        if "flops" in k and False:
-            out["roofline"] = {"bound": "mfma", "achieved": k["TFs"], "peak": MFMA_FP32_PEAK_TFS, "unit": "TFLOP/s",
-                               "frac": k["TFs"] / MFMA_FP32_PEAK_TFS, "traffic": traffic, "kernel": dom_k}
-        else:
-            out["roofline"] = {"bound": "hbm", "achieved": k["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom_k,
-                               "bytes_per_launch": k["bytes"], "ms_per_launch": k["ms"]}
+                traffic = json.load(fh).get(dom_k, {}).get("bytes_per_launch")
+        out["roofline"] = {"bound": "hbm", "achieved": k["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom_k,
+                           "bytes_per_launch": k["bytes"], "ms_per_launch": k["ms"]}
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(P, out.get("iterations_fp64", its_mixed)[0])

@@ -159,6 +159,30 @@ int te_dft_apply(const te_dft_plan* p, int dtype, const void* f, void* u, te_str
  * must already be filled.  One launch of the batched patch solve. */
 int te_dft_patch_solve(const te_dft_plan* p, int dtype, const void* f, void* u, te_stream stream);
 
+/* ---- fused launches of the V-cycle (2D, BiLinear / BiQuadratic ghosts; the DFT ones
+ * need the matrix-core plan: n = 16 or 32, Dirichlet).  Each replaces a sequence of the
+ * calls above by one launch that computes ghost values from the neighbours' interior
+ * values instead of reading ghost cells; none of them reads or writes ghost cells.
+ * dtype: arithmetic and u type; dtype_f / dtype_out / dtype_c: f, output and
+ * coarse-level types (TE_F32 / TE_F64, the mixed-precision cycle's conversions). */
+/* PatchSolver::apply (PatchSolver.h:98-134) from u = 0: out = per-patch solve of f. */
+int te_dft_apply_mixed(const te_dft_plan* p, int dtype, int dtype_f, int dtype_out, const void* f, void* out,
+                       te_stream stream);
+/* The up-leg of VCycle::visit (GMG/VCycle.h:64-67): Interpolator::interpolate
+ * (DirectInterpolator, GMG/DirectInterpolator.h:60-91) of the coarser level's uc into u
+ * followed by PatchSolver::smooth (PatchSolver.h:142-170), as one launch writing the
+ * smoothed u + P uc to out (out must not alias u; u and uc are not modified).  link:
+ * this level -> the coarser level. */
+int te_dft_smooth_interp(const te_dft_plan* p, const te_level_link* link, int dtype, int dtype_f, int dtype_out,
+                         const void* f, const void* u, int dtype_c, const void* uc, void* out, te_stream stream);
+/* Cycle::restrict (GMG/Cycle.h:57-65): r = f - A u (ghost fill, star operator) and
+ * MPIRestrictor::restrict of r (GMG/LinearRestrictor.h:101-155) into the coarser
+ * level's right-hand side fc, one launch; r is not stored and fc's ghost cells are not
+ * written.  Even n. */
+int te_residual_restrict(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
+                         const void* coeffs, int dtype, int dtype_f, const void* f, const void* u, int dtype_c,
+                         void* fc, te_stream stream);
+
 /* Iterative::PatchSolver<D>(BiCGStab, op) smooth (Iterative/PatchSolver.h:134-175):
  * per patch, an inner BiCGStab (Iterative/BiCGStab.h:111-203) on the patch operator
  * with internal boundary conditions, started from u.  iters (device int32[P], may be

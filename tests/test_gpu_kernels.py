@@ -226,6 +226,65 @@ def test_dft_mfma_matches_generic(ns, op_kind, monkeypatch):
         assert relerr(mesh.interior(du.get(), dim), mesh.interior(uz, dim)) <= tol
 
 
+@pytest.mark.parametrize("name,ns", [("2d_uniform_8x8_refined_cross_mpi1.json", (16, 16)),
+                                     ("2d_uniform_2x2_refined_nw_mpi1.json", (32, 32))])
+@pytest.mark.parametrize("kind", [R.BILINEAR, R.BIQUADRATIC])
+@pytest.mark.parametrize("op_kind", [0, 1])
+def test_fused_vcycle_launches(name, ns, kind, op_kind):
+    """te_dft_apply_mixed, te_dft_smooth_interp and te_residual_restrict against the oracle's
+    unfused sequences (PatchSolver::apply; interpolate then smooth; residual then restrict),
+    FP64 and FP32 arithmetic, including the mixed-precision input / output types."""
+    _print_case("fused", name, ns, kind, op_kind)
+    dim, levels = load(name, ns)
+    fl, cl = levels[0], levels[1]
+    fo, co = O.Domain(dim, ns, fl), O.Domain(dim, ns, cl)
+    fo.link_parents(co)
+    fd, cd = _dev(dim, ns, fl), _dev(dim, ns, cl)
+    link = R.LevelLink(fd, cd)
+    g = R.GhostFiller(fd, kind, R.FACES)
+    coeffs = None
+    if op_kind == 1:
+        X = mesh.cell_centers(fl, dim, ns)
+        coeffs = np.exp(np.sin(3 * X[..., 0]) * np.cos(2 * X[..., 1]))[:, None].copy()
+        O.fill_ghost(fo, coeffs, kind=kind)
+    f0 = random_field(fo.shape(1), 61)
+    u0 = random_field(fo.shape(1), 62)
+    uc0 = random_field(co.shape(1), 63)
+    # oracle references, FP64
+    ua = np.zeros_like(f0)
+    O.dft_smooth(fo, f0.copy(), ua, kind=kind, op_kind=op_kind, coeffs=coeffs)
+    ui = u0.copy()
+    O.interpolate(co, fo, uc0, ui)
+    O.dft_smooth(fo, f0.copy(), ui, kind=kind, op_kind=op_kind, coeffs=coeffs)
+    if op_kind == 0:
+        r = f0 - O.poisson_apply(fo, u0.copy(), kind=kind)
+    else:
+        r = f0 - O.varpoisson_apply(fo, coeffs, u0.copy(), kind=kind)
+    fcr = O.restrict(fo, co, r)
+    for dt, tol in [(R.F64, TOL64), (R.F32, 2e-5)]:
+        npd = R.np_dtype(dt)
+        dc = R.DeviceArray.from_host(coeffs, npd) if coeffs is not None else None
+        dc64 = R.DeviceArray.from_host(coeffs) if coeffs is not None else None
+        plan = R.DftPlan(g, op_kind=op_kind, coeffs=dc64)
+        for tf, to in [(dt, dt), (R.F64, dt), (R.F64, R.F64)]:
+            f = R.DeviceArray.from_host(f0, R.np_dtype(tf))
+            out = R.DeviceArray(fo.shape(1), R.np_dtype(to))
+            plan.apply_mixed(dt, f, out)
+            assert relerr(mesh.interior(out.get(), dim), mesh.interior(ua, dim)) <= tol, (dt, tf, to)
+            u = R.DeviceArray.from_host(u0, npd)
+            for tc in ([R.F64, R.F32] if dt == R.F32 else [R.F64]):
+                uc = R.DeviceArray.from_host(uc0, R.np_dtype(tc))
+                out = R.DeviceArray(fo.shape(1), R.np_dtype(to))
+                plan.smooth_interp(link, f, u, uc, out)
+                assert relerr(mesh.interior(out.get(), dim), mesh.interior(ui, dim)) <= tol, (dt, tf, to, tc)
+                fc = R.DeviceArray(co.shape(1), R.np_dtype(tc))
+                fc.upload(np.zeros(co.shape(1)))
+                link.residual_restrict(g, f, u, fc, op_kind=op_kind, coeffs=dc)
+                assert relerr(mesh.interior(fc.get(), dim), mesh.interior(fcr, dim)) <= tol, (dt, tf, tc)
+            # u and uc are inputs only
+            assert np.array_equal(u.get(), u0.astype(npd))
+
+
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 def test_vector_algebra(dtype):
     _print_case("vector", dtype)

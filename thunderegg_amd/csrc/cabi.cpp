@@ -519,6 +519,41 @@ int te_dft_apply(const te_dft_plan* p, int dtype, const void* f, void* u, te_str
   return launch_dft(p, dtype, dtype, dtype, f, u, u, true, S(s));
 }
 
+static bool dtype_ok(int t) { return t == TE_F64 || t == TE_F32; }
+
+int te_dft_apply_mixed(const te_dft_plan* p, int dtype, int dtype_f, int dtype_out, const void* f, void* out,
+                       te_stream s)
+{
+  TE_ARG(p && f && out, "null argument");
+  TE_ARG(dtype_ok(dtype) && dtype_ok(dtype_f) && dtype_ok(dtype_out), "bad dtype");
+  TE_ARG(p->use_mfma && dft_mfma_supported(p), "fused DFT launches need the matrix-core plan (2D, n 16/32, Dirichlet)");
+  return launch_dft_mfma_fused(p, dtype, dtype_f, dtype_out, f, out, out, true, 0, TE_F64, nullptr, nullptr, S(s));
+}
+
+int te_dft_smooth_interp(const te_dft_plan* p, const te_level_link* l, int dtype, int dtype_f, int dtype_out,
+                         const void* f, const void* u, int dtype_c, const void* uc, void* out, te_stream s)
+{
+  TE_ARG(p && l && f && u && uc && out, "null argument");
+  TE_ARG(out != u, "out must not alias u");
+  TE_ARG(dtype_ok(dtype) && dtype_ok(dtype_f) && dtype_ok(dtype_out) && dtype_ok(dtype_c), "bad dtype");
+  TE_ARG(l->fine == p->g->d, "the link's fine level is not the plan's domain");
+  TE_ARG(p->use_mfma && dft_mfma_supported(p), "fused DFT launches need the matrix-core plan (2D, n 16/32, Dirichlet)");
+  TE_ARG(p->g->kind == TE_BILINEAR || p->g->kind == TE_BIQUADRATIC, "fused ghosts: BiLinear or BiQuadratic");
+  return launch_dft_mfma_fused(p, dtype, dtype_f, dtype_out, f, u, out, false, 2, dtype_c, uc, l, S(s));
+}
+
+int te_residual_restrict(const te_ghost_filler* g, const te_level_link* l, int op_kind, int neumann,
+                         const void* coeffs, int dtype, int dtype_f, const void* f, const void* u, int dtype_c,
+                         void* fc, te_stream s)
+{
+  TE_ARG(g && l && f && u && fc, "null argument");
+  TE_ARG(op_kind == 0 || coeffs, "VarPoisson needs coefficients");
+  TE_ARG(dtype_ok(dtype) && dtype_ok(dtype_f) && dtype_ok(dtype_c), "bad dtype");
+  TE_ARG(l->fine == g->d, "the link's fine level is not the filler's domain");
+  TE_ARG(resid_restrict_fused_supported(g), "fused residual / restriction: 2D, even n, BiLinear or BiQuadratic");
+  return launch_resid_restrict(g, l, op_kind, neumann, coeffs, dtype, dtype_f, f, u, dtype_c, fc, S(s));
+}
+
 int te_patch_bicgstab_smooth(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs,
                              int max_it, double tol, int continue_on_breakdown, const void* f, void* u,
                              int32_t* iters, te_stream s)

@@ -61,6 +61,10 @@ SIGNATURES = {
     "te_dft_smooth": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
     "te_dft_apply": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
     "te_dft_patch_solve": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+    "te_dft_apply_mixed": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "te_dft_smooth_interp": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp, _vp, _vp]),
+    "te_residual_restrict": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp,
+                                       _vp]),
     "te_patch_bicgstab_smooth": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_double, C.c_int,
                                            _vp, _vp, _vp, _vp]),
     "te_restrict": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp]),
@@ -259,6 +263,12 @@ class LevelLink:
     def restrict(self, fine: DeviceArray, coarse: DeviceArray, stream=None):
         check(lib().te_restrict(self.h, _dt(fine), fine.ptr, coarse.ptr, fine.shape[1], stream))
 
+    def residual_restrict(self, filler: "GhostFiller", f: DeviceArray, u: DeviceArray, fc: DeviceArray,
+                          op_kind=0, neumann=False, coeffs: DeviceArray | None = None, stream=None):
+        """fc = restriction of f - A u, one launch (u's dtype is the arithmetic type)."""
+        check(lib().te_residual_restrict(filler.h, self.h, op_kind, int(neumann), coeffs.ptr if coeffs else None,
+                                         _dt(u), _dt(f), f.ptr, u.ptr, _dt(fc), fc.ptr, stream))
+
     def interpolate(self, coarse: DeviceArray, fine: DeviceArray, stream=None):
         check(lib().te_interpolate(self.h, _dt(fine), coarse.ptr, fine.ptr, fine.shape[1], stream))
 
@@ -320,6 +330,16 @@ class DftPlan:
 
     def apply(self, f: DeviceArray, u: DeviceArray, stream=None):
         check(lib().te_dft_apply(self.h, _dt(u), f.ptr, u.ptr, stream))
+
+    def apply_mixed(self, dtype, f: DeviceArray, out: DeviceArray, stream=None):
+        """From u = 0, computing in `dtype`; f and out keep their own dtypes."""
+        check(lib().te_dft_apply_mixed(self.h, dtype, _dt(f), _dt(out), f.ptr, out.ptr, stream))
+
+    def smooth_interp(self, link: "LevelLink", f: DeviceArray, u: DeviceArray, uc: DeviceArray, out: DeviceArray,
+                      stream=None):
+        """out = smooth(f, u + P uc), one launch (u's dtype is the arithmetic type)."""
+        check(lib().te_dft_smooth_interp(self.h, link.h, _dt(u), _dt(f), _dt(out), f.ptr, u.ptr, _dt(uc), uc.ptr,
+                                         out.ptr, stream))
 
 
 class Stream:
