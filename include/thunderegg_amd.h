@@ -4,9 +4,9 @@
  * Drop-in boundary for ThunderEgg's plugin/operator API on this path.  The
  * reference exposes C++ virtual interfaces only (no FFI); each entry point
  * below names the reference member function it replaces (paths relative to
- * the reference's src/ThunderEgg/).  The C++ wrappers in
- * thunderegg_amd/include/ThunderEgg/ restore the reference's class names and
- * method shapes on top of these functions.
+ * the reference's src/ThunderEgg/).  The C++ API in include/ThunderEgg/
+ * (ThunderEgg/amd.h) restores the reference's class names and method shapes on
+ * top of these functions.
  *
  * Conventions
  *  - Every function returns 0 on success or a negative status; te_last_error()

@@ -1,0 +1,158 @@
+// Drives the C++ API (include/ThunderEgg/) the way a program written against the
+// reference's hot-path classes does: domains from a forest, per-level fillers,
+// operators, DFT smoothers, restrictors and interpolators, a CycleBuilder V-cycle, and
+// Iterative::BiCGStab.  Checks: the device cycle and the generic (virtual-call) cycle
+// give the same V-cycle and the same solve; mixed precision reaches the tolerance; the
+// reference's log format; the builder's error behaviour.  Exit status 0 = pass.
+#include <ThunderEgg/GMG/CycleBuilder.h>
+#include <ThunderEgg/Iterative/BiCGStab.h>
+#include <ThunderEgg/Poisson/DFTPatchSolver.h>
+
+#include <cmath>
+#include <sstream>
+
+using namespace ThunderEgg;
+
+static int failures = 0;
+#define CHECK(c)                                                        \
+  do {                                                                  \
+    if (!(c)) {                                                         \
+      std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #c); \
+      failures++;                                                       \
+    }                                                                   \
+  } while (0)
+
+// a smoother the library does not know: forces the generic cycle
+template <int D>
+class WrappedSmoother : public GMG::Smoother<D> {
+ public:
+  explicit WrappedSmoother(const GMG::Smoother<D>& s) : s_(s.clone()) {}
+  WrappedSmoother* clone() const override { return new WrappedSmoother(*this); }
+  void smooth(const Vector<D>& f, Vector<D>& u) const override { s_->smooth(f, u); }
+
+ private:
+  std::shared_ptr<GMG::Smoother<D>> s_;
+};
+
+static std::shared_ptr<GMG::Cycle<2>> build(const std::vector<Domain<2>>& doms, const GMG::CycleOpts& opts,
+                                           bool wrap)
+{
+  GMG::CycleBuilder<2> builder(opts);
+  for (size_t i = 0; i < doms.size(); i++) {
+    BiQuadraticGhostFiller gf(doms[i], GhostFillingType::Faces);
+    Poisson::StarPatchOperator<2> op(doms[i], gf);
+    Poisson::DFTPatchSolver<2> dft(op, std::bitset<4>());
+    WrappedSmoother<2> wrapped(dft);
+    const GMG::Smoother<2>& sm = wrap ? (const GMG::Smoother<2>&)wrapped : (const GMG::Smoother<2>&)dft;
+    if (i == 0) {
+      builder.addFinestLevel(op, sm, GMG::LinearRestrictor<2>(doms[0], doms[1]));
+    } else if (i + 1 < doms.size()) {
+      builder.addIntermediateLevel(op, sm, GMG::LinearRestrictor<2>(doms[i], doms[i + 1]),
+                                   GMG::DirectInterpolator<2>(doms[i], doms[i - 1]));
+    } else {
+      builder.addCoarsestLevel(op, sm, GMG::DirectInterpolator<2>(doms[i], doms[i - 1]));
+    }
+  }
+  return builder.getCycle();
+}
+
+static double rel_diff(const Vector<2>& a, const Vector<2>& b)
+{
+  Vector<2> d(a);
+  d.addScaled(-1.0, b);
+  return d.twoNorm() / b.twoNorm();
+}
+
+int main()
+{
+  // uniform level-2 forest refined in one corner, 16x16 patches
+  ForestDomainGenerator<2> gen(2, { 16, 16 });
+  gen.refineWhere([](int level, std::array<int, 2> c) { return level == 2 && c[0] < 2 && c[1] < 2; });
+  std::vector<Domain<2>> doms{ gen.getFinestDomain() };
+  while (gen.hasCoarserDomain()) doms.push_back(gen.getCoarserDomain());
+  std::printf("levels:");
+  for (auto& d : doms) std::printf(" %d", d.getNumLocalPatches());
+  std::printf("\n");
+  CHECK(doms.size() >= 3);
+
+  // right-hand side: f = -8 pi^2 sin(2 pi x) sin(2 pi y) at the cell centres
+  Vector<2> f(doms[0], 1);
+  {
+    std::vector<double> h = f.copyToHost();
+    const auto& P = doms[0].getPatchInfoVector();
+    const int n = 16, w = n + 2;
+    for (size_t p = 0; p < P.size(); p++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          double x = P[p].starts[0] + (i + 0.5) * P[p].spacings[0];
+          double y = P[p].starts[1] + (j + 0.5) * P[p].spacings[1];
+          h[p * w * w + (size_t)(j + 1) * w + i + 1] =
+            -8 * M_PI * M_PI * std::sin(2 * M_PI * x) * std::sin(2 * M_PI * y);
+        }
+    f.copyFromHost(h);
+  }
+  BiQuadraticGhostFiller gf0(doms[0], GhostFillingType::Faces);
+  Poisson::StarPatchOperator<2> A(doms[0], gf0);
+
+  GMG::CycleOpts opts;
+  auto dev = build(doms, opts, false);
+  auto gen_cycle = build(doms, opts, true);
+  CHECK(dev->device() != nullptr);
+  CHECK(gen_cycle->device() == nullptr);
+
+  // one V-cycle: device == generic
+  Vector<2> u1(doms[0], 1), u2(doms[0], 1);
+  dev->apply(f, u1);
+  gen_cycle->apply(f, u2);
+  double dv = rel_diff(u1, u2);
+  std::printf("vcycle device vs generic: %.3e\n", dv);
+  CHECK(dv <= 1e-12);
+
+  // BiCGStab to 1e-10: same iterations, same solution, reference log format
+  Iterative::BiCGStab<2> solver;
+  solver.setTolerance(1e-10);
+  CHECK(solver.getMaxIterations() == 1000);
+  Vector<2> x1(doms[0], 1), x2(doms[0], 1);
+  std::ostringstream log1, log2;
+  int it1 = solver.solve(A, x1, f, dev.get(), true, log1);
+  int it2 = solver.solve(A, x2, f, gen_cycle.get(), true, log2);
+  double dx = rel_diff(x1, x2);
+  std::printf("bicgstab iterations device %d generic %d, solution diff %.3e\n", it1, it2, dx);
+  CHECK(it1 == it2 && it1 > 0);
+  CHECK(dx <= 1e-9);
+  CHECK(log1.str().substr(0, 6) == "    0 ");
+  std::printf("%s", log1.str().c_str());
+
+  // mixed precision cycle reaches the tolerance in the true residual
+  GMG::CycleOpts mopts;
+  mopts.precision = "mixed";
+  auto mixed = build(doms, mopts, false);
+  Vector<2> x3(doms[0], 1);
+  int it3 = solver.solve(A, x3, f, mixed.get());
+  Vector<2> r(doms[0], 1);
+  A.apply(x3, r);
+  r.scaleThenAdd(-1.0, f);
+  double tr = r.twoNorm() / f.twoNorm();
+  std::printf("mixed: %d iterations, true residual %.3e (the reference's early exit adds alpha p)\n", it3, tr);
+  CHECK(it3 <= it1 + 2);
+  CHECK(tr <= 1e-6);
+
+  // zero right-hand side: 0 iterations, x untouched
+  Vector<2> z(doms[0], 1), x4(doms[0], 1);
+  x4.set(1.0);
+  CHECK(solver.solve(A, x4, z, dev.get()) == 0);
+  CHECK(std::fabs(x4.infNorm() - 1.0) < 1e-15);
+
+  // builder errors are RuntimeErrors, as in the reference
+  bool threw = false;
+  try {
+    GMG::CycleBuilder<2> b(opts);
+    b.getCycle();
+  } catch (const RuntimeError&) {
+    threw = true;
+  }
+  CHECK(threw);
+
+  std::printf(failures ? "api_test: %d failures\n" : "api_test: ok\n", failures);
+  return failures ? 1 : 0;
+}

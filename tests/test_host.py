@@ -176,3 +176,30 @@ def test_uniform_levels_shape():
     p = lv[0]
     assert np.allclose(p["spacings"][:, :2], 1 / 256)
     assert (p["nbr_type"][:, 4:8] >= -1).all()
+
+
+def _build_api_test(tmp):
+    """g++ build of tests/cpp/api_test.cpp against include/ThunderEgg (the reference-shaped
+    C++ API) and the C ABI library; host compiler only."""
+    import subprocess
+    exe = os.path.join(str(tmp), "api_test")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "api_test.cpp"), "-L" + os.path.join(ROOT, "thunderegg_amd"),
+           "-lthunderegg_amd", "-Wl,-rpath," + os.path.join(ROOT, "thunderegg_amd"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_cpp_api_compiles(tmp_path):
+    _build_api_test(tmp_path)
+    # every forwarding header compiles on its own
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    hdrs = sorted(os.path.relpath(os.path.join(dp, f), inc) for dp, _, fs in os.walk(os.path.join(inc, "ThunderEgg"))
+                  for f in fs if f.endswith(".h"))
+    src = tmp_path / "all.cpp"
+    src.write_text("".join(f"#include <{h}>\n" for h in hdrs) + "int main() { return 0; }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I" + inc, str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
