@@ -63,6 +63,7 @@ struct te_gmg {
   std::map<std::pair<const void*, const void*>, std::pair<hipGraphExec_t, hipGraphExec_t>> bgraphs;
   double last_seconds = 0.0;
   bool fused = false;  // 2D matrix-core path: ghost fills fused into the stencil / smoother launches
+  std::vector<char> rr_pre;  // per level: restriction fused with the coarser level's first sweep
   bool fly = false;    // outer operator computes ghosts on the fly
 };
 
@@ -145,7 +146,9 @@ int visit(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
 // ghost fill of the first post-smoothing sweep happen inside that sweep's launch.
 // Sweeps after the first pre-smoothing one read the solution from one buffer and
 // write the other; the finest level's last sweep writes the caller's FP64 vector.
-int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
+// pre_done: this level's first sweep (from zero) was already done by the finer level's
+// launch (launch_rr_pre), its result in Lv.u.
+int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext, bool pre_done)
 {
   GLevel& Lv = g->L[li];
   const int nl = (int)g->L.size();
@@ -182,18 +185,27 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
       if (finest) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), TE_F64, u_ext, Lv.dt, buf[0], g->s));
       return TE_OK;
     }
-    for (int k = 0; k < n; k++) TE_RC(sweep(k == 0 ? 0 : 1, k == n - 1));
+    for (int k = pre_done ? 1 : 0; k < n; k++) TE_RC(sweep(k == 0 ? 0 : 1, k == n - 1));
     return TE_OK;
   }
-  for (int k = 0; k < g->o.pre_sweeps; k++) TE_RC(sweep(k == 0 ? 0 : 1, false));
+  for (int k = pre_done ? 1 : 0; k < g->o.pre_sweeps; k++) TE_RC(sweep(k == 0 ? 0 : 1, false));
   if (g->o.pre_sweeps == 0) {
     TE_RC(zero_with_ghost(g, li, buf[0], Lv.dt));
     Lv.cur = buf[0];
   }
   GLevel& Cv = g->L[li + 1];
-  TE_RC(launch_resid_restrict(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur], Cv.dt,
-                              Cv.f, g->s));
-  TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext));
+  const int next_first = li + 1 == nl - 1 ? g->o.coarse_sweeps : g->o.pre_sweeps;
+  if (next_first > 0 && g->rr_pre[li]) {
+    // Cycle::restrict and the coarser level's first sweep (from zero) in one launch
+    TE_RC(launch_rr_pre(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur], Cv.plan,
+                        Cv.dt, Cv.f, Cv.u, g->s));
+    Cv.cur = Cv.u;
+    TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext, true));
+  } else {
+    TE_RC(launch_resid_restrict(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur],
+                                Cv.dt, Cv.f, g->s));
+    TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext, false));
+  }
   int n = g->o.post_sweeps;
   if (n == 0) {
     // DirectInterpolator alone (GMG/DirectInterpolator.h:60-91)
@@ -210,7 +222,7 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
 int vcycle_enqueue(te_gmg* g, const double* f, double* u)
 {
   GLevel& L0 = g->L[0];
-  if (g->fused) return visit_fused(g, 0, f, TE_F64, u);
+  if (g->fused) return visit_fused(g, 0, f, TE_F64, u, false);
   if (L0.dt != TE_F64 && g->o.smoother != 0) {
     // the inner-Krylov smoother works on same-dtype f and u: convert f once
     TE_RC(launch_convert(te_domain_vector_length(L0.d, 1), L0.dt, L0.f, TE_F64, f, g->s));
@@ -389,6 +401,18 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     }
     g->fused = fused;
     g->fly = allow && star_dot_fly_supported(g->L[0].gf);
+    const char* nr = std::getenv("TE_NO_RRPRE");
+    const bool allow_rp = fused && !(nr && nr[0] == '1');
+    g->rr_pre.assign(num_levels, 0);
+    // Only where the coarser level is small: there the launch and its dependent loads
+    // dominate; on large levels the separate launches keep all four waves of a workgroup
+    // busy in the patch solves (one wave per coarse patch solves in the fused launch).
+    constexpr int kRrPreMaxCoarse = 1024;
+    for (int i = 0; i + 1 < num_levels && allow_rp; i++)
+      g->rr_pre[i] = (rr_pre_supported(g->L[i].gf, g->L[i + 1].plan) &&
+                      te_domain_num_patches(g->L[i + 1].d) <= kRrPreMaxCoarse)
+                       ? 1
+                       : 0;
   }
   if (!rc) {
     const te_domain* d0 = levels[0];

@@ -6,6 +6,12 @@
 namespace te {
 constexpr int kDftMatDim = 64;                      // largest supported n
 constexpr int kDftMatStride = kDftMatDim * kDftMatDim;
+// Cycle::restrict of the level of g fused with the zero-guess sweep of the next coarser
+// level (plan pc): writes the coarse f (fc) and the coarse u (uc); 2D, 16x16 patches.
+bool rr_pre_supported(const te_ghost_filler* g, const te_dft_plan* pc);
+int launch_rr_pre(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
+                  const void* coeffs, int dt, int dtf, const void* f, const void* u, const te_dft_plan* pc, int dtc,
+                  void* fc, void* uc, hipStream_t s);
 }  // namespace te
 
 struct te_dft_plan {
@@ -36,4 +42,10 @@ bool resid_restrict_fused_supported(const te_ghost_filler* g);
 int launch_resid_restrict(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
                           const void* coeffs, int dt, int dtf, const void* f, const void* u, int dtc, void* fc,
                           hipStream_t s);
+// Cycle::restrict of the level of g fused with the zero-guess sweep of the next coarser
+// level (plan pc): writes the coarse f (fc) and the coarse u (uc); 2D, 16x16 patches.
+bool rr_pre_supported(const te_ghost_filler* g, const te_dft_plan* pc);
+int launch_rr_pre(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
+                  const void* coeffs, int dt, int dtf, const void* f, const void* u, const te_dft_plan* pc, int dtc,
+                  void* fc, void* uc, hipStream_t s);
 }  // namespace te

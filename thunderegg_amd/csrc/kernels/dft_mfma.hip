@@ -80,6 +80,116 @@ struct FusedArgs {
   const int32_t* porth;
 };
 
+// The patch solve of one wavefront: X (LDS, n x ld, the modified right-hand side) ->
+// out[base + padded (x, y)] (interior only), lx = 4 / hx^2, ly = 4 / hy^2.  X is
+// overwritten.  Contains a workgroup barrier: every wave of the workgroup must call it
+// (waves without a patch with active = false: they only take part in the barrier).
+template <typename T, typename TO, int NT>
+__device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, const T* __restrict__ S3,
+                                               const T* __restrict__ s2, T lx, T ly, TO* __restrict__ out,
+                                               int base, int sx, bool active)
+{
+  constexpr int n = 16 * NT;
+  constexpr int ld = n + 1;
+  typedef typename Mf<T>::V V;
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15;
+  const int g = lane >> 4;
+  if (active) {  // inactive waves only take part in the barrier
+    // ---- Y = F S^T : Y[i][j] = sum_k F[i][k] S[j][k]
+    V Y[NT][NT];
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++) {
+        V acc = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int K = 0; K < NT; K++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            int k = 16 * K + 4 * s + g;
+            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], Sm[(16 * J + c) * n + k], acc);
+          }
+        Y[I][J] = acc;
+      }
+    // ---- Z = S Y : the B operand comes from Y's registers (K index = row of register s)
+    V Z[NT][NT];
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++) {
+        V acc = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int K = 0; K < NT; K++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            int k = 16 * K + Mf<T>::row(s, g);
+            acc = Mf<T>::mma(Sm[(16 * I + c) * n + k], Y[K][J][s], acc);
+          }
+        Z[I][J] = acc;
+      }
+    // ---- divide by the eigenvalues -4/hx^2 s2[kx] - 4/hy^2 s2[ky] (getEigenValues, :320-361)
+    {
+#pragma unroll
+      for (int I = 0; I < NT; I++)
+#pragma unroll
+        for (int J = 0; J < NT; J++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            int ky = 16 * I + Mf<T>::row(r, g), kx = 16 * J + c;
+            T eig = T(0);
+            eig -= lx * s2[kx];
+            eig -= ly * s2[ky];
+            Z[I][J][r] = Z[I][J][r] / eig;
+          }
+    }
+    // ---- V = S3 Z'
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++) {
+        V acc = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int K = 0; K < NT; K++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            int k = 16 * K + Mf<T>::row(s, g);
+            acc = Mf<T>::mma(S3[(16 * I + c) * n + k], Z[K][J][s], acc);
+          }
+        Y[I][J] = acc;  // reuse Y for V
+      }
+    // V -> LDS (accumulator layout), then U = V S3^T with V in the A layout
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) X[(16 * I + Mf<T>::row(r, g)) * ld + 16 * J + c] = Y[I][J][r];
+  }
+  __syncthreads();
+  if (active) {
+    const T scale = T(4.0 / double(n * n));  // prod over axes of 2/n (:442-449)
+#pragma unroll
+    for (int I = 0; I < NT; I++)
+#pragma unroll
+      for (int J = 0; J < NT; J++) {
+        V acc = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int K = 0; K < NT; K++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            int k = 16 * K + 4 * s + g;
+            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], S3[(16 * J + c) * n + k], acc);
+          }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int y = 16 * I + Mf<T>::row(r, g), x = 16 * J + c;
+          out[base + (y + 1) * sx + x + 1] = TO(acc[r] * scale);
+        }
+      }
+  }
+}
+
 template <typename T, typename TF, typename TO, typename TC, int NT, int OP, bool UZERO, int GM, int KIND>
 __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 : 1) dft_mfma_kernel(const DevPatch* __restrict__ dp,
                                                                const T* __restrict__ mats, const T* __restrict__ s2tab,
@@ -89,15 +199,12 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 :
 {
   constexpr int n = 16 * NT;
   constexpr int ld = n + 1;  // padded LDS row
-  typedef typename Mf<T>::V V;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* Sm = reinterpret_cast<T*>(smem_raw);  // DST-II, row-major n x n
   T* S3 = Sm + n * n;                      // DST-III
   T* s2 = S3 + n * n;                      // sin^2((k+1) pi / 2n), k < n
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int c = lane & 15;
-  const int g = lane >> 4;
   T* X = s2 + n + wave * n * ld;  // per-wave scratch n x ld
   T* Bt = s2 + n + kWaves * n * ld + wave * 4 * n;  // per-wave interface terms of the 4 sides
   // n = 16 with ghosts computed here: the patch's own values (u, or u + P uc) are staged
@@ -198,99 +305,11 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 :
         __syncthreads();
       }
     }
-    // ---- Y = F S^T : Y[i][j] = sum_k F[i][k] S[j][k]
-    V Y[NT][NT];
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++) {
-        V acc = { 0, 0, 0, 0 };
-#pragma unroll
-        for (int K = 0; K < NT; K++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            int k = 16 * K + 4 * s + g;
-            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], Sm[(16 * J + c) * n + k], acc);
-          }
-        Y[I][J] = acc;
-      }
-    // ---- Z = S Y : the B operand comes from Y's registers (K index = row of register s)
-    V Z[NT][NT];
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++) {
-        V acc = { 0, 0, 0, 0 };
-#pragma unroll
-        for (int K = 0; K < NT; K++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            int k = 16 * K + Mf<T>::row(s, g);
-            acc = Mf<T>::mma(Sm[(16 * I + c) * n + k], Y[K][J][s], acc);
-          }
-        Z[I][J] = acc;
-      }
-    // ---- divide by the eigenvalues -4/hx^2 s2[kx] - 4/hy^2 s2[ky] (getEigenValues, :320-361)
     {
       const DevPatch& P = dp[active ? p : 0];
-      T lx = T(4.0 / (P.h[0] * P.h[0])), ly = T(4.0 / (P.h[1] * P.h[1]));
-#pragma unroll
-      for (int I = 0; I < NT; I++)
-#pragma unroll
-        for (int J = 0; J < NT; J++)
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            int ky = 16 * I + Mf<T>::row(r, g), kx = 16 * J + c;
-            T eig = T(0);
-            eig -= lx * s2[kx];
-            eig -= ly * s2[ky];
-            Z[I][J][r] = Z[I][J][r] / eig;
-          }
+      const T lx = T(4.0 / (P.h[0] * P.h[0])), ly = T(4.0 / (P.h[1] * P.h[1]));
+      dft_solve_wave<T, TO, NT>(X, Sm, S3, s2, lx, ly, out, base, L.sx, active);
     }
-    // ---- V = S3 Z'
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++) {
-        V acc = { 0, 0, 0, 0 };
-#pragma unroll
-        for (int K = 0; K < NT; K++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            int k = 16 * K + Mf<T>::row(s, g);
-            acc = Mf<T>::mma(S3[(16 * I + c) * n + k], Z[K][J][s], acc);
-          }
-        Y[I][J] = acc;  // reuse Y for V
-      }
-    // V -> LDS (accumulator layout), then U = V S3^T with V in the A layout
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) X[(16 * I + Mf<T>::row(r, g)) * ld + 16 * J + c] = Y[I][J][r];
-    __syncthreads();
-    const T scale = T(4.0 / double(n * n));  // prod over axes of 2/n (:442-449)
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++) {
-        V acc = { 0, 0, 0, 0 };
-#pragma unroll
-        for (int K = 0; K < NT; K++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            int k = 16 * K + 4 * s + g;
-            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], S3[(16 * J + c) * n + k], acc);
-          }
-        if (active) {
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            int y = 16 * I + Mf<T>::row(r, g), x = 16 * J + c;
-            out[base + (y + 1) * L.sx + x + 1] = TO(acc[r] * scale);
-          }
-        }
-      }
     __syncthreads();
   }
 }
@@ -341,6 +360,127 @@ void launch_t(const te_dft_plan* pl, const void* f, const void* u, void* out, bo
     launch_nt<T, TF, TO, TC, 2>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, gm, (const TC*)uc, link, s);
 }
 
+
+// Cycle::restrict of a level fused with the first smoothing sweep of the next coarser
+// level, which starts from u = 0 and so needs no ghost values (VCycle::visit,
+// GMG/VCycle.h:46-70: the coarse level's pre-smoothing follows the restriction
+// directly), 16 x 16 cells:
+//   one workgroup per coarse patch P, one wave per child c of P (2^D children, or P
+//   itself when it was not coarsened): c's u is staged in a tile with on-the-fly ghosts,
+//   r = f - A u is formed on c's cells and restricted (2x2 averages, or a copy) into X,
+//   the coarse right-hand side in LDS; then fc[P] = X (kept for the rest of the cycle)
+//   and uc[P] = DFT solve of X by one wave.
+// T: fine arithmetic, TF: fine f, TC: coarse level type (its f, u and DFT arithmetic).
+template <typename T, typename TF, typename TC, int OP, int KIND>
+__global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __restrict__ dpf,
+                                                             const T* __restrict__ u, const T* __restrict__ cf,
+                                                             const TF* __restrict__ f, Layout Lf, int neumann,
+                                                             const int32_t* __restrict__ child,
+                                                             const int32_t* __restrict__ nchild,
+                                                             const int32_t* __restrict__ porth,
+                                                             const DevPatch* __restrict__ dpc,
+                                                             const TC* __restrict__ mats,
+                                                             const TC* __restrict__ s2tab, TC* __restrict__ fc,
+                                                             TC* __restrict__ uc, Layout Lc)
+{
+  constexpr int n = 16;
+  constexpr int ld = n + 1;
+  constexpr int w = n + 2;
+  constexpr int h = n / 2;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  TC* Sm = reinterpret_cast<TC*>(smem_raw);
+  TC* S3 = Sm + n * n;
+  TC* s2 = S3 + n * n;
+  TC* X = s2 + n;  // the coarse patch's right-hand side, shared by the workgroup
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  T* tile = reinterpret_cast<T*>(X + n * ld) + wave * w * w;
+  PlainAcc<T> U{ u, Lf };
+  const T phys = (OP == 0 && neumann) ? T(1) : T(-1);
+  for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+    Sm[k] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
+    S3[k] = mats[(4 * 3 + 0) * kDftMatStride + k];  // DST_III, axis 0
+  }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) s2[k] = s2tab[(2 * 3 + 0) * kDftMatDim + k];
+  // (the first patch's barriers order the matrix loads before their use)
+  for (int P = blockIdx.x; P < Lc.np; P += gridDim.x) {
+    // wave j handles child j of P (P itself when it was not coarsened: one child)
+    const bool act = wave < nchild[P];
+    const int c = act ? child[P * 8 + wave] : 0;
+    SideInfo si[1];
+    int o = -1;
+    if (act) {
+      tile_interior_2d<T>(c, n, n, U, tile, lane);
+      ring_prefetch<1>(dpf[c], n, n, lane, si);
+      o = porth[c];
+    }
+    __syncthreads();
+    if (act) tile_ghosts_2d<T, KIND>(dpf[c], si[0], c, n, n, U, tile, phys, lane);
+    __syncthreads();
+    if (act) {
+      const DevPatch& Pf = dpf[c];
+      const T ih2[2] = { T(Pf.inv_h2[0]), T(Pf.inv_h2[1]) };
+      const int ox = (o & 1) ? h : 0, oy = (o & 2) ? h : 0;
+      const int Xb = lane % h, Yb = lane / h;  // a lane per 2x2 block (h * h = 64)
+      T r[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int x = 2 * Xb + (q & 1), y = 2 * Yb + (q >> 1);
+        const int ci = (y + 1) * Lf.sx + x + 1;
+        r[q] = T(f[c * Lf.ps + ci]) - star_tile<T, OP>(tile, w, x, y, ih2, cf, c * Lf.cs + ci, Lf.sx);
+      }
+      if (o < 0) {  // copyToParent
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int x = 2 * Xb + (q & 1), y = 2 * Yb + (q >> 1);
+          const TC v = TC(r[q]);
+          X[y * ld + x] = v;
+          fc[P * Lc.ps + (y + 1) * Lc.sx + x + 1] = v;
+        }
+      } else {  // restrictToCoarserParent, fine cells in x-fastest order
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc += r[q] * T(0.25);
+        const TC v = TC(acc);
+        X[(Yb + oy) * ld + Xb + ox] = v;
+        fc[P * Lc.ps + (Yb + oy + 1) * Lc.sx + Xb + ox + 1] = v;
+      }
+    }
+    __syncthreads();
+    // the coarse patch solve from u = 0 by wave 0 (the others take part in its barrier)
+    const DevPatch& Pc = dpc[P];
+    const TC lx = TC(4.0 / (Pc.h[0] * Pc.h[0])), ly = TC(4.0 / (Pc.h[1] * Pc.h[1]));
+    dft_solve_wave<TC, TC, 1>(X, Sm, S3, s2, lx, ly, uc, P * Lc.ps, Lc.sx, wave == 0);
+    __syncthreads();  // X is rewritten by the next patch
+  }
+}
+
+template <typename T, typename TF, typename TC>
+void launch_rp(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann, const void* coeffs,
+               const void* f, const void* u, const te_dft_plan* pc, void* fc, void* uc, hipStream_t s)
+{
+  const te_domain* df = g->d;
+  const te_domain* dc = pc->g->d;
+  Layout Lf = make_layout(2, df->ns, 1, df->np);
+  Layout Lc = make_layout(2, dc->ns, 1, dc->np);
+  constexpr int n = 16;
+  size_t shmem = sizeof(TC) * (2 * n * n + n + n * (n + 1)) + sizeof(T) * kWaves * (n + 2) * (n + 2);
+  const TC* mats = (const TC*)(sizeof(TC) == 8 ? (const void*)pc->d_mats64 : (const void*)pc->d_mats32);
+  const TC* s2 = (const TC*)(sizeof(TC) == 8 ? (const void*)pc->d_s2_64 : (const void*)pc->d_s2_32);
+  int grid = dc->np < 16384 ? dc->np : 16384;  // a coarse patch per workgroup
+#define TE_RP(OP_, K_)                                                                                          \
+  hipLaunchKernelGGL((rr_pre_kernel<T, TF, TC, OP_, K_>), dim3(grid), dim3(64 * kWaves), shmem, s, df->d_patches, \
+                     (const T*)u, (const T*)coeffs, (const TF*)f, Lf, neumann, link->d_child, link->d_nchild,      \
+                     link->d_orth, dc->d_patches, mats, s2, (TC*)fc, (TC*)uc, Lc)
+  if (op_kind == 0) {
+    if (g->kind == TE_BILINEAR) TE_RP(0, TE_BILINEAR);
+    else TE_RP(0, TE_BIQUADRATIC);
+  } else {
+    if (g->kind == TE_BILINEAR) TE_RP(1, TE_BILINEAR);
+    else TE_RP(1, TE_BIQUADRATIC);
+  }
+#undef TE_RP
+}
+
 }  // namespace
 
 bool dft_mfma_supported(const te_dft_plan* pl)
@@ -386,6 +526,38 @@ int launch_dft_mfma_fused(const te_dft_plan* pl, int dt, int dtf, int dto, const
     return TE_ERR_UNSUPPORTED;
   }
 #undef TE_LT
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+
+bool rr_pre_supported(const te_ghost_filler* gf, const te_dft_plan* pc)
+{
+  const te_domain* df = gf->d;
+  const te_domain* dc = pc->g->d;
+  return df->dim == 2 && df->ns[0] == 16 && df->ns[1] == 16 && dc->ns[0] == 16 && dc->ns[1] == 16 &&
+         (gf->kind == TE_BILINEAR || gf->kind == TE_BIQUADRATIC) && pc->use_mfma && dft_mfma_supported(pc);
+}
+
+// dt: fine u / coeffs type; dtf: fine f type; dtc: coarse level type
+int launch_rr_pre(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
+                  const void* coeffs, int dt, int dtf, const void* f, const void* u, const te_dft_plan* pc, int dtc,
+                  void* fc, void* uc, hipStream_t s)
+{
+  if (pc->g->d->np == 0) return TE_OK;
+  if (!rr_pre_supported(g, pc) || link->fine != g->d || link->coarse != pc->g->d) {
+    set_error("fused restriction + coarse smoothing: 2D 16x16 patches, matrix-core plan, matching link");
+    return TE_ERR_UNSUPPORTED;
+  }
+  if (dt == TE_F64 && dtf == TE_F64 && dtc == TE_F64) launch_rp<double, double, double>(g, link, op_kind, neumann, coeffs, f, u, pc, fc, uc, s);
+  else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F32) launch_rp<float, float, float>(g, link, op_kind, neumann, coeffs, f, u, pc, fc, uc, s);
+  else if (dt == TE_F32 && dtf == TE_F64 && dtc == TE_F32) launch_rp<float, double, float>(g, link, op_kind, neumann, coeffs, f, u, pc, fc, uc, s);
+  else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F64) launch_rp<float, float, double>(g, link, op_kind, neumann, coeffs, f, u, pc, fc, uc, s);
+  else if (dt == TE_F32 && dtf == TE_F64 && dtc == TE_F64) launch_rp<float, double, double>(g, link, op_kind, neumann, coeffs, f, u, pc, fc, uc, s);
+  else {
+    set_error("unsupported dtype combination for the fused restriction + coarse smoothing");
+    return TE_ERR_UNSUPPORTED;
+  }
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

@@ -24,36 +24,6 @@ namespace {
 
 constexpr int kWaves = 4;
 
-template <typename T>
-struct PlainAcc {
-  const T* u;
-  Layout L;
-  __device__ __forceinline__ T operator()(int q, int x, int y) const { return u[q * L.ps + (y + 1) * L.sx + x + 1]; }
-};
-
-// (A u)(x, y) from the staged tile; ci: padded index of the cell in the coefficient vector
-template <typename T, int OP>
-__device__ __forceinline__ T star_tile(const T* tile, int w, int x, int y, const T* ih2, const T* cf, int ci,
-                                       int sx)
-{
-  const int k = (y + 1) * w + x + 1;
-  const T mid = tile[k];
-  T acc = T(0);
-#pragma unroll
-  for (int a = 0; a < 2; a++) {
-    const int st = a == 0 ? 1 : w;
-    const T lower = tile[k - st], upper = tile[k + st];
-    if (OP == 0) {
-      acc += (upper - T(2) * mid + lower) * ih2[a];
-    } else {
-      const int cst = a == 0 ? 1 : sx;
-      const T cm = cf[ci], cl = cf[ci - cst], cu = cf[ci + cst];
-      acc += ((cu + cm) * (upper - mid) - (cl + cm) * (mid - lower)) * (ih2[a] * T(0.5));
-    }
-  }
-  return acc;
-}
-
 template <typename T, typename TF, typename TC, int OP, int KIND>
 __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch* __restrict__ dp,
                                                                  const T* __restrict__ u, const T* __restrict__ cf,

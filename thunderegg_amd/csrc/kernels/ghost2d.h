@@ -114,6 +114,36 @@ __device__ __forceinline__ T ghost2d(const SideInfo& si, int p, int s, int t, in
   return G;
 }
 
+template <typename T>
+struct PlainAcc {  // interior value of patch q at (x, y)
+  const T* u;
+  Layout L;
+  __device__ __forceinline__ T operator()(int q, int x, int y) const { return u[q * L.ps + (y + 1) * L.sx + x + 1]; }
+};
+
+// (A u)(x, y) from the staged tile; ci: padded index of the cell in the coefficient vector
+template <typename T, int OP>
+__device__ __forceinline__ T star_tile(const T* tile, int w, int x, int y, const T* ih2, const T* cf, int ci,
+                                       int sx)
+{
+  const int k = (y + 1) * w + x + 1;
+  const T mid = tile[k];
+  T acc = T(0);
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    const int st = a == 0 ? 1 : w;
+    const T lower = tile[k - st], upper = tile[k + st];
+    if (OP == 0) {
+      acc += (upper - T(2) * mid + lower) * ih2[a];
+    } else {
+      const int cst = a == 0 ? 1 : sx;
+      const T cm = cf[ci], cl = cf[ci - cst], cu = cf[ci + cst];
+      acc += ((cu + cm) * (upper - mid) - (cl + cm) * (mid - lower)) * (ih2[a] * T(0.5));
+    }
+  }
+  return acc;
+}
+
 // ---- per-wave staging of one patch into a padded LDS tile ----
 // tile[(y + 1) * (n0 + 2) + x + 1] holds cell (x, y), x in [-1, n0], y in [-1, n1].
 
