@@ -82,8 +82,8 @@ struct FusedArgs {
 
 // The patch solve of one wavefront: X (LDS, n x ld, the modified right-hand side) ->
 // out[base + padded (x, y)] (interior only), lx = 4 / hx^2, ly = 4 / hy^2.  X is
-// overwritten.  Contains a workgroup barrier: every wave of the workgroup must call it
-// (waves without a patch with active = false: they only take part in the barrier).
+// overwritten (the wave's own LDS; no workgroup barrier).  active = false: the wave
+// skips the arithmetic.
 template <typename T, typename TO, int NT>
 __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, const T* __restrict__ S3,
                                                const T* __restrict__ s2, T lx, T ly, TO* __restrict__ out,
@@ -95,7 +95,7 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
   const int lane = threadIdx.x & 63;
   const int c = lane & 15;
   const int g = lane >> 4;
-  if (active) {  // inactive waves only take part in the barrier
+  if (active) {
     // ---- Y = F S^T : Y[i][j] = sum_k F[i][k] S[j][k]
     V Y[NT][NT];
 #pragma unroll
@@ -108,7 +108,7 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
 #pragma unroll
           for (int s = 0; s < 4; s++) {
             int k = 16 * K + 4 * s + g;
-            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], Sm[(16 * J + c) * n + k], acc);
+            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], Sm[(16 * J + c) * ld + k], acc);
           }
         Y[I][J] = acc;
       }
@@ -124,7 +124,7 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
 #pragma unroll
           for (int s = 0; s < 4; s++) {
             int k = 16 * K + Mf<T>::row(s, g);
-            acc = Mf<T>::mma(Sm[(16 * I + c) * n + k], Y[K][J][s], acc);
+            acc = Mf<T>::mma(Sm[(16 * I + c) * ld + k], Y[K][J][s], acc);
           }
         Z[I][J] = acc;
       }
@@ -154,7 +154,7 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
 #pragma unroll
           for (int s = 0; s < 4; s++) {
             int k = 16 * K + Mf<T>::row(s, g);
-            acc = Mf<T>::mma(S3[(16 * I + c) * n + k], Z[K][J][s], acc);
+            acc = Mf<T>::mma(S3[(16 * I + c) * ld + k], Z[K][J][s], acc);
           }
         Y[I][J] = acc;  // reuse Y for V
       }
@@ -166,7 +166,7 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
 #pragma unroll
         for (int r = 0; r < 4; r++) X[(16 * I + Mf<T>::row(r, g)) * ld + 16 * J + c] = Y[I][J][r];
   }
-  __syncthreads();
+  wave_sync();  // V in LDS, read back in the A-operand layout
   if (active) {
     const T scale = T(4.0 / double(n * n));  // prod over axes of 2/n (:442-449)
 #pragma unroll
@@ -179,7 +179,7 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
 #pragma unroll
           for (int s = 0; s < 4; s++) {
             int k = 16 * K + 4 * s + g;
-            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], S3[(16 * J + c) * n + k], acc);
+            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], S3[(16 * J + c) * ld + k], acc);
           }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -200,9 +200,11 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 :
   constexpr int n = 16 * NT;
   constexpr int ld = n + 1;  // padded LDS row
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Sm = reinterpret_cast<T*>(smem_raw);  // DST-II, row-major n x n
-  T* S3 = Sm + n * n;                      // DST-III
-  T* s2 = S3 + n * n;                      // sin^2((k+1) pi / 2n), k < n
+  // the transform matrices with rows padded to n + 1 (the MFMA operand loads of a lane
+  // group walk down a column: an unpadded pitch of n puts 8 lanes on one LDS bank)
+  T* Sm = reinterpret_cast<T*>(smem_raw);  // DST-II, row-major n x ld
+  T* S3 = Sm + n * ld;                     // DST-III
+  T* s2 = S3 + n * ld;                     // sin^2((k+1) pi / 2n), k < n
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   T* X = s2 + n + wave * n * ld;  // per-wave scratch n x ld
@@ -214,103 +216,101 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 :
   UAcc<T, TC, GM == 2> U{ u, fa.uc, fa.parent, fa.porth, L };
 
   for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
-    Sm[k] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
-    S3[k] = mats[(4 * 3 + 0) * kDftMatStride + k];  // DST_III, axis 0
+    const int row = k / n, col = k - row * n;
+    Sm[row * ld + col] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
+    S3[row * ld + col] = mats[(4 * 3 + 0) * kDftMatStride + k];  // DST_III, axis 0
   }
   for (int k = threadIdx.x; k < n; k += blockDim.x) s2[k] = s2tab[(2 * 3 + 0) * kDftMatDim + k];
-  // no barrier here: the first patch's loads go out together with the matrices', and the
-  // barrier after its right-hand-side copy (every workgroup runs at least one group,
-  // grid <= groups) orders both before any read
 
-  const int ngroups = (L.np + kWaves - 1) / kWaves;
-  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    const int p = grp * kWaves + wave;
-    const bool active = p < L.np;
+  // Each wave walks its own patches with private LDS (X, Bt, Tl): wave-level ordering
+  // suffices inside the loop.  The one workgroup barrier orders the shared matrices; the
+  // wave's first patch is staged before it so that its loads overlap the matrices'.
+  SideInfo si[NT];  // ring cells lane + 64 j: side j * 64 / n + lane / n
+  // ---- f_copy = f (interior) into LDS; the side information of the ghost pass is
+  //      loaded at the same time
+  auto stage = [&](int p) {
     const int base = p * L.ps;
-    // ---- f_copy = f (interior) into LDS; the side information of the ghost pass is
-    //      loaded at the same time
-    SideInfo si[NT];  // ring cells lane + 64 j: side j * 64 / n + lane / n
-    if (active) {
-      for (int q = lane; q < n * n; q += 64) {
-        int y = q / n, x = q - y * n;
-        X[y * ld + x] = T(f[base + (y + 1) * L.sx + x + 1]);
-      }
-      if (TILE) tile_interior_2d<T>(p, n, n, U, Tl, lane);
-      if (!UZERO) ring_prefetch<NT>(dp[p], n, n, lane, si);
+    for (int q = lane; q < n * n; q += 64) {
+      int y = q / n, x = q - y * n;
+      X[y * ld + x] = T(f[base + (y + 1) * L.sx + x + 1]);
     }
-    __syncthreads();
+    if (TILE) tile_interior_2d<T>(p, n, n, U, Tl, lane);
+    if (!UZERO) ring_prefetch<NT>(dp[p], n, n, lane, si);
+  };
+  const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
+  int p = xr.first + wave;
+  if (p < xr.end) stage(p);
+  __syncthreads();
+  for (bool first = true; p < xr.end; p += xr.step, first = false) {
+    if (!first) {
+      stage(p);
+      wave_sync();
+    }
+    const int base = p * L.ps;
+    const DevPatch& P = dp[p];
     // ---- modifyRHSForInternalBoundaryConditions: the interface term of every side cell
     //      is formed first (a lane per cell, so all its loads are in flight together),
     //      then applied west/east first, then south/north (sides in increasing order,
     //      Poisson/StarPatchOperator.h:165-180)
     if (!UZERO) {
-      if (active) {
-        const DevPatch& P = dp[p];
 #pragma unroll
-        for (int j = 0; j < NT; j++) {  // 4 n = 64 NT ring cells
-          const int q = lane + 64 * j;
-          const int s = q / n, t = q - s * n;
-          const int ph = s >> 1, up = s & 1;
-          T term = T(0);
-          if (si[j].type >= 0) {
-            int gi, bi;  // padded index of the ghost and of the boundary cell
-            int bx, by;  // boundary cell
-            if (ph == 0) {
-              gi = base + (t + 1) * L.sx + (up ? n + 1 : 0);
-              bi = base + (t + 1) * L.sx + (up ? n : 1);
-              bx = up ? n - 1 : 0;
-              by = t;
-            } else {
-              gi = base + (up ? n + 1 : 0) * L.sx + t + 1;
-              bi = base + (up ? n : 1) * L.sx + t + 1;
-              bx = t;
-              by = up ? n - 1 : 0;
-            }
-            T ug, ub;
-            if (GM == 0) {
-              ug = u[gi];
-              ub = u[bi];
-            } else if (TILE) {
-              TileAcc<T, UAcc<T, TC, GM == 2>> A{ Tl, p, n + 2, U };
-              ug = ghost2d<T, KIND>(si[j], p, s, t, n, n, A);
-              ub = Tl[(by + 1) * (n + 2) + bx + 1];
-            } else {
-              ug = ghost2d<T, KIND>(si[j], p, s, t, n, n, U);
-              ub = U(p, bx, by);
-            }
-            const T ih2 = T(P.inv_h2[ph]);
-            if (OP == 0) {
-              term = (ug + ub) * ih2;
-            } else {
-              const int cg = gi - p * L.ps + p * L.cs, cb = bi - p * L.ps + p * L.cs;
-              term = (ug + ub) * (coeffs[cb] + coeffs[cg]) * (ih2 * T(0.5));
-            }
+      for (int j = 0; j < NT; j++) {  // 4 n = 64 NT ring cells
+        const int q = lane + 64 * j;
+        const int s = q / n, t = q - s * n;
+        const int ph = s >> 1, up = s & 1;
+        T term = T(0);
+        if (si[j].type >= 0) {
+          int gi, bi;  // padded index of the ghost and of the boundary cell
+          int bx, by;  // boundary cell
+          if (ph == 0) {
+            gi = base + (t + 1) * L.sx + (up ? n + 1 : 0);
+            bi = base + (t + 1) * L.sx + (up ? n : 1);
+            bx = up ? n - 1 : 0;
+            by = t;
+          } else {
+            gi = base + (up ? n + 1 : 0) * L.sx + t + 1;
+            bi = base + (up ? n : 1) * L.sx + t + 1;
+            bx = t;
+            by = up ? n - 1 : 0;
           }
-          Bt[q] = term;
+          T ug, ub;
+          if (GM == 0) {
+            ug = u[gi];
+            ub = u[bi];
+          } else if (TILE) {
+            TileAcc<T, UAcc<T, TC, GM == 2>> A{ Tl, p, n + 2, U };
+            ug = ghost2d<T, KIND>(si[j], p, s, t, n, n, A);
+            ub = Tl[(by + 1) * (n + 2) + bx + 1];
+          } else {
+            ug = ghost2d<T, KIND>(si[j], p, s, t, n, n, U);
+            ub = U(p, bx, by);
+          }
+          const T ih2 = T(P.inv_h2[ph]);
+          if (OP == 0) {
+            term = (ug + ub) * ih2;
+          } else {
+            const int cg = gi - p * L.ps + p * L.cs, cb = bi - p * L.ps + p * L.cs;
+            term = (ug + ub) * (coeffs[cb] + coeffs[cg]) * (ih2 * T(0.5));
+          }
         }
+        Bt[q] = term;
       }
-      __syncthreads();
+      wave_sync();
 #pragma unroll
       for (int ph = 0; ph < 2; ph++) {
-        if (active) {
-          const DevPatch& P = dp[p];
-          for (int q = lane; q < 2 * n; q += 64) {
-            const int up = q / n, t = q - up * n;
-            const int s = 2 * ph + up;
-            if (P.type[4 + s] < 0) continue;
-            const int xi = ph == 0 ? t * ld + (up ? n - 1 : 0) : (up ? n - 1 : 0) * ld + t;
-            X[xi] -= Bt[s * n + t];
-          }
+        for (int q = lane; q < 2 * n; q += 64) {
+          const int up = q / n, t = q - up * n;
+          const int s = 2 * ph + up;
+          if (P.type[4 + s] < 0) continue;
+          const int xi = ph == 0 ? t * ld + (up ? n - 1 : 0) : (up ? n - 1 : 0) * ld + t;
+          X[xi] -= Bt[s * n + t];
         }
-        __syncthreads();
+        wave_sync();
       }
     }
-    {
-      const DevPatch& P = dp[active ? p : 0];
-      const T lx = T(4.0 / (P.h[0] * P.h[0])), ly = T(4.0 / (P.h[1] * P.h[1]));
-      dft_solve_wave<T, TO, NT>(X, Sm, S3, s2, lx, ly, out, base, L.sx, active);
-    }
-    __syncthreads();
+    const T lx = T(4.0 / (P.h[0] * P.h[0])), ly = T(4.0 / (P.h[1] * P.h[1]));
+    dft_solve_wave<T, TO, NT>(X, Sm, S3, s2, lx, ly, out, base, L.sx, true);
+    wave_sync();  // X and the tile are rewritten by the next patch
   }
 }
 
@@ -322,7 +322,7 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   Layout L = make_layout(2, d->ns, 1, d->np);
   constexpr int n = 16 * NT;
   const bool tile = gm != 0 && NT == 1;
-  size_t shmem = sizeof(T) * (2 * n * n + n + kWaves * n * (n + 1) + kWaves * 4 * n +
+  size_t shmem = sizeof(T) * (2 * n * (n + 1) + n + kWaves * n * (n + 1) + kWaves * 4 * n +
                               (tile ? kWaves * (n + 2) * (n + 2) : 0));
   const T* mats = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32);
   const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
@@ -388,21 +388,23 @@ __global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __r
   constexpr int w = n + 2;
   constexpr int h = n / 2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  TC* Sm = reinterpret_cast<TC*>(smem_raw);
-  TC* S3 = Sm + n * n;
-  TC* s2 = S3 + n * n;
+  TC* Sm = reinterpret_cast<TC*>(smem_raw);  // rows padded to ld, as in dft_mfma_kernel
+  TC* S3 = Sm + n * ld;
+  TC* s2 = S3 + n * ld;
   TC* X = s2 + n;  // the coarse patch's right-hand side, shared by the workgroup
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   T* tile = reinterpret_cast<T*>(X + n * ld) + wave * w * w;
   PlainAcc<T> U{ u, Lf };
   const T phys = (OP == 0 && neumann) ? T(1) : T(-1);
   for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
-    Sm[k] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
-    S3[k] = mats[(4 * 3 + 0) * kDftMatStride + k];  // DST_III, axis 0
+    const int row = k / n, col = k - row * n;
+    Sm[row * ld + col] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
+    S3[row * ld + col] = mats[(4 * 3 + 0) * kDftMatStride + k];  // DST_III, axis 0
   }
   for (int k = threadIdx.x; k < n; k += blockDim.x) s2[k] = s2tab[(2 * 3 + 0) * kDftMatDim + k];
   // (the first patch's barriers order the matrix loads before their use)
-  for (int P = blockIdx.x; P < Lc.np; P += gridDim.x) {
+  const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, Lc.np, 1);
+  for (int P = xr.first; P < xr.end; P += xr.step) {
     // wave j handles child j of P (P itself when it was not coarsened: one child)
     const bool act = wave < nchild[P];
     const int c = act ? child[P * 8 + wave] : 0;
@@ -463,7 +465,7 @@ void launch_rp(const te_ghost_filler* g, const te_level_link* link, int op_kind,
   Layout Lf = make_layout(2, df->ns, 1, df->np);
   Layout Lc = make_layout(2, dc->ns, 1, dc->np);
   constexpr int n = 16;
-  size_t shmem = sizeof(TC) * (2 * n * n + n + n * (n + 1)) + sizeof(T) * kWaves * (n + 2) * (n + 2);
+  size_t shmem = sizeof(TC) * (2 * n * (n + 1) + n + n * (n + 1)) + sizeof(T) * kWaves * (n + 2) * (n + 2);
   const TC* mats = (const TC*)(sizeof(TC) == 8 ? (const void*)pc->d_mats64 : (const void*)pc->d_mats32);
   const TC* s2 = (const TC*)(sizeof(TC) == 8 ? (const void*)pc->d_s2_64 : (const void*)pc->d_s2_32);
   int grid = dc->np < 16384 ? dc->np : 16384;  // a coarse patch per workgroup

@@ -39,23 +39,18 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
   T* tile = reinterpret_cast<T*>(smem_raw) + wave * w * (n1 + 2);
   PlainAcc<T> U{ u, L };
   const T phys = (OP == 0 && neumann) ? T(1) : T(-1);
-  const int ngroups = (L.np + kWaves - 1) / kWaves;
-  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    const int p = grp * kWaves + wave;
-    const bool active = p < L.np;
+  // each wave walks its own patches; its tile is private, so wave-level ordering suffices
+  const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
+  for (int p = xr.first + wave; p < xr.end; p += xr.step) {
     // everything that does not depend on the tile is loaded together with its interior
     SideInfo si[1];
-    int o = -1, cp = 0;
-    if (active) {
-      tile_interior_2d<T>(p, n0, n1, U, tile, lane);
-      ring_prefetch<1>(dp[p], n0, n1, lane, si);
-      o = porth[p];
-      cp = parent[p];
-    }
-    __syncthreads();
-    if (active) tile_ghosts_2d<T, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
-    __syncthreads();
-    if (active) {
+    tile_interior_2d<T>(p, n0, n1, U, tile, lane);
+    ring_prefetch<1>(dp[p], n0, n1, lane, si);
+    const int o = porth[p], cp = parent[p];
+    wave_sync();
+    tile_ghosts_2d<T, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
+    wave_sync();
+    {
       const DevPatch& P = dp[p];
       const T ih2[2] = { T(P.inv_h2[0]), T(P.inv_h2[1]) };
       const int ox = (o & 1) ? h0 : 0, oy = (o & 2) ? h1 : 0;
@@ -85,7 +80,7 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
         }
       }
     }
-    __syncthreads();
+    wave_sync();  // the tile is rewritten for the next patch
   }
 }
 
@@ -104,20 +99,16 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
   PlainAcc<double> U{ u, L };
   const double phys = (OP == 0 && neumann) ? 1.0 : -1.0;
   double sa = 0.0, sq = 0.0;
-  const int ngroups = (L.np + kWaves - 1) / kWaves;
-  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    const int p = grp * kWaves + wave;
-    const bool active = p < L.np;
+  const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
+  for (int p = xr.first + wave; p < xr.end; p += xr.step) {
     const int base = p * L.ps;
     SideInfo si[1];
-    if (active) {
-      tile_interior_2d<double>(p, n0, n1, U, tile, lane);
-      ring_prefetch<1>(dp[p], n0, n1, lane, si);
-    }
-    __syncthreads();
-    if (active) tile_ghosts_2d<double, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
-    __syncthreads();
-    if (active) {
+    tile_interior_2d<double>(p, n0, n1, U, tile, lane);
+    ring_prefetch<1>(dp[p], n0, n1, lane, si);
+    wave_sync();
+    tile_ghosts_2d<double, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
+    wave_sync();
+    {
       const DevPatch& P = dp[p];
       const double ih2[2] = { P.inv_h2[0], P.inv_h2[1] };
       for (int q = lane; q < n0 * n1; q += 64) {
@@ -142,7 +133,7 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
         out[base + (y + 1) * L.sx + x + 1] = 0.0;
       }
     }
-    __syncthreads();
+    wave_sync();
   }
   double r = block_reduce<RED_SUM>(sa);
   if (threadIdx.x == 0) pa[blockIdx.x] = r;

@@ -144,6 +144,36 @@ __device__ __forceinline__ T star_tile(const T* tile, int w, int x, int y, const
   return acc;
 }
 
+// XCD-aware work split.  Workgroups are dispatched round-robin over the 8 XCDs (block
+// b runs on XCD b % 8), each with its own L2.  Giving XCD x the contiguous items
+// [x N/8, (x+1) N/8) (patches in Morton order: mostly each other's neighbours), walked
+// by that XCD's blocks, keeps the neighbour reads of the ghost formulas in one L2 and
+// the XCDs equally loaded.  Block b covers items first, first + step, ... < end in
+// units of `per` consecutive items (per = waves per block when each wave takes one).
+// Grids smaller than 8 blocks fall back to plain striding.
+struct XcdRange {
+  int first, step, end;
+};
+__device__ __forceinline__ XcdRange xcd_range(int b, int G, int n, int per)
+{
+  if (G < 8) return XcdRange{ b * per, G * per, n };
+  const int x = b & 7, i = b >> 3;
+  const int gx = (G >> 3) + (x < (G & 7) ? 1 : 0);  // blocks on XCD x
+  const int lo = (int)((long)x * n / 8), hi = (int)((long)(x + 1) * n / 8);
+  return XcdRange{ lo + i * per, gx * per, hi };
+}
+
+// Orders the LDS accesses of one wavefront (every lane of the wave executes it): a
+// wave's LDS operations are performed in issue order, so only the compiler has to be
+// kept from moving memory operations across this point.  Replaces a workgroup barrier
+// where each wave works on LDS of its own.
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---- per-wave staging of one patch into a padded LDS tile ----
 // tile[(y + 1) * (n0 + 2) + x + 1] holds cell (x, y), x in [-1, n0], y in [-1, n1].
 
