@@ -191,7 +191,7 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
 }
 
 template <typename T, typename TF, typename TO, typename TC, int NT, int OP, bool UZERO, int GM, int KIND>
-__global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 : 1) dft_mfma_kernel(const DevPatch* __restrict__ dp,
+__global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM == 0 ? 5 : 4) : 1) dft_mfma_kernel(const DevPatch* __restrict__ dp,
                                                                const T* __restrict__ mats, const T* __restrict__ s2tab,
                                                                const TF* __restrict__ f, const T* __restrict__ u,
                                                                const T* __restrict__ coeffs, TO* __restrict__ out,
@@ -228,24 +228,105 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 :
   SideInfo si[NT];  // ring cells lane + 64 j: side j * 64 / n + lane / n
   // ---- f_copy = f (interior) into LDS; the side information of the ghost pass is
   //      loaded at the same time
-  auto stage = [&](int p) {
+  // n = 16: the next patch's right-hand side is loaded into registers while this one is
+  // solved (FPL values per lane), so a wave's loads overlap its own arithmetic
+  constexpr int FPL = n * n / 64;
+  constexpr bool PREF = NT == 1;
+  TF fn[PREF ? FPL : 1];
+  auto fload = [&](int p) {
     const int base = p * L.ps;
-    for (int q = lane; q < n * n; q += 64) {
-      int y = q / n, x = q - y * n;
-      X[y * ld + x] = T(f[base + (y + 1) * L.sx + x + 1]);
+#pragma unroll
+    for (int k = 0; k < (PREF ? FPL : 1); k++) {
+      const int q = lane + 64 * k, y = q / n, x = q - y * n;
+      fn[k] = f[base + (y + 1) * L.sx + x + 1];
+    }
+  };
+  auto fput = [&]() {
+#pragma unroll
+    for (int k = 0; k < (PREF ? FPL : 1); k++) {
+      const int q = lane + 64 * k, y = q / n, x = q - y * n;
+      X[y * ld + x] = T(fn[k]);
+    }
+  };
+  // with ghosts computed here (TILE): the next patch's own values and side information
+  // too; its coarse-level values once its parent index has arrived (mid-iteration)
+  constexpr bool PREF_T = TILE && PREF;
+  T un[PREF_T ? FPL : 1];
+  TC ucn[PREF_T ? FPL : 1];
+  SideInfo sin[NT];
+  int on = -1, parn = 0;
+  auto tload_a = [&](int p) {
+#pragma unroll
+    for (int k = 0; k < (PREF_T ? FPL : 1); k++) {
+      const int q = lane + 64 * k, y = q / n, x = q - y * n;
+      un[k] = u[p * L.ps + (y + 1) * L.sx + x + 1];
+    }
+    if (GM == 2) {
+      on = fa.porth[p];
+      parn = fa.parent[p];
+    }
+    ring_prefetch<NT>(dp[p], n, n, lane, sin);
+  };
+  auto tload_b = [&]() {
+    if (GM == 2) {
+#pragma unroll
+      for (int k = 0; k < (PREF_T ? FPL : 1); k++) {
+        const int q = lane + 64 * k, y = q / n, x = q - y * n;
+        int cx = x, cy = y;
+        if (on >= 0) {
+          cx = (x + ((on & 1) ? L.n[0] : 0)) / 2;
+          cy = (y + ((on & 2) ? L.n[1] : 0)) / 2;
+        }
+        ucn[k] = fa.uc[parn * L.ps + (cy + 1) * L.sx + cx + 1];
+      }
+    }
+  };
+  auto tput = [&]() {  // the values UAcc would return, staged from the registers
+#pragma unroll
+    for (int k = 0; k < (PREF_T ? FPL : 1); k++) {
+      const int q = lane + 64 * k, y = q / n, x = q - y * n;
+      Tl[(y + 1) * (n + 2) + x + 1] = GM == 2 ? T(un[k] + T(ucn[k])) : un[k];
+    }
+#pragma unroll
+    for (int j = 0; j < NT; j++) si[j] = sin[j];
+  };
+  auto stage = [&](int p) {
+    if (PREF_T) {
+      fput();
+      tput();
+      return;
+    }
+    if (PREF) {
+      fput();
+    } else {
+      const int base = p * L.ps;
+      for (int q = lane; q < n * n; q += 64) {
+        int y = q / n, x = q - y * n;
+        X[y * ld + x] = T(f[base + (y + 1) * L.sx + x + 1]);
+      }
     }
     if (TILE) tile_interior_2d<T>(p, n, n, U, Tl, lane);
     if (!UZERO) ring_prefetch<NT>(dp[p], n, n, lane, si);
   };
   const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
   int p = xr.first + wave;
-  if (p < xr.end) stage(p);
+  if (p < xr.end) {
+    if (PREF) fload(p);
+    if (PREF_T) {
+      tload_a(p);
+      tload_b();
+    }
+    stage(p);
+  }
   __syncthreads();
   for (bool first = true; p < xr.end; p += xr.step, first = false) {
     if (!first) {
       stage(p);
       wave_sync();
     }
+    const bool has_next = p + xr.step < xr.end;
+    if (PREF && has_next) fload(p + xr.step);
+    if (PREF_T && has_next) tload_a(p + xr.step);
     const int base = p * L.ps;
     const DevPatch& P = dp[p];
     // ---- modifyRHSForInternalBoundaryConditions: the interface term of every side cell
@@ -295,6 +376,7 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? 5 :
         }
         Bt[q] = term;
       }
+      if (PREF_T && has_next) tload_b();
       wave_sync();
 #pragma unroll
       for (int ph = 0; ph < 2; ph++) {
@@ -329,7 +411,10 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);
   FusedArgs<T, TC> fa{ uc, link ? link->d_parent : nullptr, link ? link->d_orth : nullptr };
   int ngroups = (d->np + kWaves - 1) / kWaves;
-  int grid = ngroups < 4096 ? ngroups : 4096;
+  // n = 16: about as many workgroups as are resident at once, so each wave solves
+  // several patches and its prefetch of the next one overlaps the current solve
+  const int cap = NT == 1 ? 2048 : 4096;
+  int grid = ngroups < cap ? ngroups : cap;
   const int kind = pl->g->kind;
 #define TE_MK(OP_, UZ_, GM_, K_)                                                                            \
   hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_>), dim3(grid), dim3(64 * kWaves), \
