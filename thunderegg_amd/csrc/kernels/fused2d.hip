@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
 }
 
 // out = A u (ghost cells of out written 0); pa[block] = sum w * out, pb[block] = sum out * out
-template <int OP, int KIND, bool SQ>
+template <int OP, int KIND, bool SQ, bool P16>
 __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __restrict__ dp,
                                                            const double* __restrict__ u,
                                                            const double* __restrict__ cf, double* __restrict__ out,
@@ -100,23 +100,49 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
   const double phys = (OP == 0 && neumann) ? 1.0 : -1.0;
   double sa = 0.0, sq = 0.0;
   const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
+  // P16: the next patch's u, w and side information are loaded during the current one
+  double un[P16 ? 4 : 1], wn[P16 ? 4 : 1], wcur[P16 ? 4 : 1];
+  SideInfo sin[1];
+  auto load = [&](int p) {
+#pragma unroll
+    for (int k = 0; k < (P16 ? 4 : 1); k++) {
+      const int q = lane + 64 * k;
+      const int c = p * L.ps + ((q >> 4) + 1) * L.sx + (q & 15) + 1;
+      un[k] = u[c];
+      wn[k] = wv[c];
+    }
+    ring_prefetch<1>(dp[p], n0, n1, lane, sin);
+  };
+  if (P16 && xr.first + wave < xr.end) load(xr.first + wave);
   for (int p = xr.first + wave; p < xr.end; p += xr.step) {
     const int base = p * L.ps;
     SideInfo si[1];
-    tile_interior_2d<double>(p, n0, n1, U, tile, lane);
-    ring_prefetch<1>(dp[p], n0, n1, lane, si);
+    if (P16) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int q = lane + 64 * k;
+        tile[((q >> 4) + 1) * w + (q & 15) + 1] = un[k];
+        wcur[k] = wn[k];
+      }
+      si[0] = sin[0];
+    } else {
+      tile_interior_2d<double>(p, n0, n1, U, tile, lane);
+      ring_prefetch<1>(dp[p], n0, n1, lane, si);
+    }
     wave_sync();
+    if (P16 && p + xr.step < xr.end) load(p + xr.step);
     tile_ghosts_2d<double, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
     wave_sync();
     {
       const DevPatch& P = dp[p];
       const double ih2[2] = { P.inv_h2[0], P.inv_h2[1] };
-      for (int q = lane; q < n0 * n1; q += 64) {
+      for (int q = lane, k = 0; q < n0 * n1; q += 64, k++) {
         const int y = q / n0, x = q - y * n0;
         const int c = (y + 1) * L.sx + x + 1;
         const double acc = star_tile<double, OP>(tile, w, x, y, ih2, cf, p * L.cs + c, L.sx);
         out[base + c] = acc;
-        sa += wv[base + c] * acc;  // rhat.dot(ap) / as.dot(s): w first, as the reference
+        const double wq = P16 ? wcur[k & 3] : wv[base + c];
+        sa += wq * acc;  // rhat.dot(ap) / as.dot(s): w first, as the reference
         if (SQ) sq += acc * acc;
       }
       // ghost frame of out: rows -1 and n1 (with corners), then columns -1 and n0
@@ -220,13 +246,16 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
   }
   Layout L = make_layout(2, d->ns, 1, d->np);
   size_t shmem = tile_bytes(d, sizeof(double));
-#define TE_SD(OP_, K_, Q_)                                                                                    \
-  hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches, u, \
+  const bool p16 = d->ns[0] == 16 && d->ns[1] == 16;
+#define TE_SD(OP_, K_, Q_, P_)                                                                                    \
+  hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches, u, \
                      coeffs, out, w, pa, pb, L, neumann)
-#define TE_SDQ(OP_, K_)          \
-  do {                           \
-    if (pb) TE_SD(OP_, K_, true);  \
-    else TE_SD(OP_, K_, false);    \
+#define TE_SDQ(OP_, K_)                      \
+  do {                                       \
+    if (pb && p16) TE_SD(OP_, K_, true, true);   \
+    else if (pb) TE_SD(OP_, K_, true, false);    \
+    else if (p16) TE_SD(OP_, K_, false, true);   \
+    else TE_SD(OP_, K_, false, false);           \
   } while (0)
   if (op_kind == 0) {
     if (g->kind == TE_BILINEAR) TE_SDQ(0, TE_BILINEAR);
