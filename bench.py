@@ -170,7 +170,7 @@ def main():
     def make(precision):
         return solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, op_kind=P.op_kind,
                           coeffs=P.coeffs, pre_sweeps=P.pre, post_sweeps=P.post, coarse_sweeps=P.coarse,
-                          precision=precision, use_graph=True)
+                          inner_max_it=P.inner_max_it, inner_tol=P.inner_tol, precision=precision, use_graph=True)
 
     t0 = time.perf_counter()
     g_mixed = make(R.F32)
