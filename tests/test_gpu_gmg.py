@@ -115,6 +115,30 @@ def test_bicgstab_unpreconditioned():
     assert relerr(mesh.interior(x, 2), mesh.interior(xref, 2)) <= 1e-6
 
 
+def test_bicgstab_early_exit():
+    """The s-norm test (BiCGStab.h:162-167) decided on the device: with a right-hand side
+    that is an eigenvector of A, unpreconditioned, s = r - alpha A r vanishes on the first
+    iteration, x += alpha p and the solve stops with 0 iterations counted.  A later solve
+    on the same (cached) iteration graph starts from the running state again."""
+    _print_case("bicgstab-early-exit")
+    ns = (8, 8)
+    levels = solver.uniform_levels(2, 1, ns)
+    f = _rhs_sin(levels, 2, ns)
+    ol = _oracle_levels(2, ns, levels, R.BILINEAR, 0)
+    xref, its_ref, _ = ol.bicgstab(f.copy(), tol=1e-9, precond=False)
+    assert its_ref == 0
+    for graph in (False, True):
+        g = solver.GMG(2, ns, levels, filler_kind=R.BILINEAR, smoother=0, precision=R.F64, use_graph=graph)
+        x, its, _ = g.solve(f, tol=1e-9, precond=False)
+        assert its == 0, graph
+        assert relerr(mesh.interior(x, 2), mesh.interior(xref, 2)) <= 1e-12, graph
+        f2 = random_field(f.shape, 17)
+        xr2, its_r2, _ = ol.bicgstab(f2.copy(), tol=1e-9, precond=False)
+        x2, its2, _ = g.solve(f2, tol=1e-9, precond=False)
+        assert abs(its2 - its_r2) <= 1, graph
+        assert relerr(mesh.interior(x2, 2), mesh.interior(xr2, 2)) <= 1e-6, graph
+
+
 def test_bicgstab_zero_rhs_and_converged_guess():
     """test/Iterative/BiCGStab_MPI1.cpp:132-154 (zero RHS -> 0 iterations, x untouched)
     and :202-240 (converged initial guess -> 0 iterations)."""

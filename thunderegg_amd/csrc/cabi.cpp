@@ -32,6 +32,33 @@ using namespace te;
 
 static inline hipStream_t S(te_stream s) { return (hipStream_t)s; }
 
+std::vector<int32_t> te::ring_table(const te_domain* d, const int32_t* parent, const int32_t* orth_on_parent)
+{
+  std::vector<int32_t> r((size_t)d->np * 4 * 2 * 4, 0);
+  for (int p = 0; p < d->np; p++)
+    for (int s = 0; s < 4; s++) {
+      const int sl = 4 + s;  // 2D face slots follow the four corner slots
+      const te_patch_info& P = d->patches[p];
+      const int type = P.nbr_type[sl];
+      for (int h = 0; h < 2; h++) {
+        int32_t* e = &r[(((size_t)p * 4 + s) * 2 + h) * 4];
+        const int q = type < 0 ? -1 : d->nbr_local[((size_t)p * 26 + sl) * 4 + (type == TE_NBR_FINE ? h : 0)];
+        e[0] = type;
+        e[1] = P.nbr_orth[sl];
+        e[2] = q;
+        e[3] = (parent && q >= 0) ? parent[q] * 8 + orth_on_parent[q] + 1 : 0;
+      }
+    }
+  return r;
+}
+
+static hipError_t upload_ring(const std::vector<int32_t>& r, int32_t** out)
+{
+  hipError_t e = hipMalloc(out, sizeof(int32_t) * std::max<size_t>(r.size(), 4));
+  if (e == hipSuccess && !r.empty()) e = hipMemcpy(*out, r.data(), sizeof(int32_t) * r.size(), hipMemcpyHostToDevice);
+  return e;
+}
+
 extern "C" {
 
 const char* te_last_error(void) { return te::g_err.c_str(); }
@@ -118,6 +145,7 @@ int te_event_elapsed_ms(te_event a, te_event b, float* ms)
 // ------------------------------------------------------------------ domain --
 static int face_count(int dim) { return dim == 2 ? 8 : 26; }
 
+
 int te_domain_create(int dim, const int* ns, int num_ghost, int num_patches, const te_patch_info* patches,
                      te_domain** out)
 {
@@ -197,6 +225,7 @@ int te_domain_create(int dim, const int* ns, int num_ghost, int num_patches, con
   if (e == hipSuccess) e = hipMalloc(&d->d_scratch, sizeof(double) * kMaxPartials);
   if (e == hipSuccess) e = hipMalloc(&d->d_result, sizeof(double) * 16);
   if (e == hipSuccess) e = hipMalloc(&d->d_flags, sizeof(int32_t) * 4);
+  if (e == hipSuccess && dim == 2) e = upload_ring(ring_table(d, nullptr, nullptr), &d->d_ring);
   if (e != hipSuccess) {
     set_error(std::string("te_domain_create: ") + hipGetErrorString(e));
     te_domain_destroy(d);
@@ -213,6 +242,7 @@ int te_domain_destroy(te_domain* d)
   if (d->d_scratch) (void)hipFree(d->d_scratch);
   if (d->d_result) (void)hipFree(d->d_result);
   if (d->d_flags) (void)hipFree(d->d_flags);
+  if (d->d_ring) (void)hipFree(d->d_ring);
   delete d;
   return TE_OK;
 }
@@ -272,6 +302,7 @@ int te_level_link_create(const te_domain* fine, const te_domain* coarse, te_leve
   if (e == hipSuccess && fine->np) e = hipMemcpy(l->d_orth, orth.data(), sizeof(int32_t) * fine->np, hipMemcpyHostToDevice);
   if (e == hipSuccess && coarse->np) e = hipMemcpy(l->d_child, child.data(), sizeof(int32_t) * coarse->np * 8, hipMemcpyHostToDevice);
   if (e == hipSuccess && coarse->np) e = hipMemcpy(l->d_nchild, nchild.data(), sizeof(int32_t) * coarse->np, hipMemcpyHostToDevice);
+  if (e == hipSuccess && fine->dim == 2) e = upload_ring(ring_table(fine, l->parent.data(), orth.data()), &l->d_ring);
   if (e != hipSuccess) {
     set_error(std::string("te_level_link_create: ") + hipGetErrorString(e));
     te_level_link_destroy(l);
@@ -288,6 +319,7 @@ int te_level_link_destroy(te_level_link* l)
   (void)hipFree(l->d_orth);
   (void)hipFree(l->d_child);
   (void)hipFree(l->d_nchild);
+  (void)hipFree(l->d_ring);
   delete l;
   return TE_OK;
 }

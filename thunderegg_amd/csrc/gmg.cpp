@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -60,7 +61,8 @@ struct te_gmg {
   int nblk = 0;
   int32_t* d_flags = nullptr;
   std::map<std::pair<const void*, const void*>, hipGraphExec_t> vgraphs;
-  std::map<std::pair<const void*, const void*>, std::pair<hipGraphExec_t, hipGraphExec_t>> bgraphs;
+  // one BiCGStab iteration per (b, x, use_precond)
+  std::map<std::tuple<const void*, const void*, int>, hipGraphExec_t> bgraphs;
   double last_seconds = 0.0;
   bool fused = false;  // 2D matrix-core path: ghost fills fused into the stencil / smoother launches
   std::vector<char> rr_pre;  // per level: restriction fused with the coarser level's first sweep
@@ -276,10 +278,7 @@ void free_gmg(te_gmg* g)
 {
   if (!g) return;
   for (auto& kv : g->vgraphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto& kv : g->bgraphs) {
-    (void)hipGraphExecDestroy(kv.second.first);
-    (void)hipGraphExecDestroy(kv.second.second);
-  }
+  for (auto& kv : g->bgraphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& Lv : g->L) {
     if (Lv.gf) te_ghost_filler_destroy(Lv.gf);
     if (Lv.plan) te_dft_plan_destroy(Lv.plan);
@@ -486,18 +485,23 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   };
   double* mp = use_precond ? g->mp : g->p;
   double* ms = use_precond ? g->ms : g->sv;
-  // phase 1: mp = M p; ap = A mp (+ rhat.ap); alpha; s = r - alpha ap; ||s||
-  auto phase1 = [&]() -> int {
+  // One iteration, enqueued as a whole: the host reads the scalars once per iteration.
+  //   mp = M p; ap = A mp (+ rhat.ap); alpha; s = r - alpha ap; ||s|| and the s-norm test;
+  //   if it fired: x += alpha p (mp with use_precond = 2), and the updates after it are
+  //   skipped on the device (the kernels read the state word); else
+  //   ms = M s; as = A ms (+ as.s, as.as); omega; x, r updates; rho, beta, ||r||; p update.
+  // The preconditioner and operator launches of the second half run even after the test
+  // fired (their results are then unused): that happens at most once per solve.
+  auto iteration = [&]() -> int {
     if (use_precond) TE_RC(vcycle_enqueue(g, g->p, g->mp));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
     TE_RC(launch_bicg_finalize(FIN_ALPHA, g->pa, nullptr, nb, g->scal, s));
     TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pa, nullptr,
                       s));
-    return launch_bicg_finalize(FIN_SNORM, g->pa, nullptr, nb, g->scal, s);
-  };
-  // phase 2: ms = M s; as = A ms (+ as.s, as.as); omega; x, r updates; rho, beta, ||r||; p update
-  auto phase2 = [&]() -> int {
+    TE_RC(launch_bicg_finalize(FIN_SNORM, g->pa, nullptr, nb, g->scal, s));
+    const double* dirp = use_precond == 2 ? mp : g->p;  // reference: alpha p (BiCGStab.h:162-167)
+    TE_RC(launch_bicg(BICG_XP, d, g->scal, x, nullptr, dirp, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, s));
     if (use_precond) TE_RC(vcycle_enqueue(g, g->sv, g->ms));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, ms, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, g->as, g->sv, g->pa, g->pb, g->fly, s));
@@ -507,41 +511,30 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     return launch_bicg(BICG_P, d, g->scal, g->p, nullptr, g->ap, g->resid, nullptr, nullptr, nullptr, nullptr,
                        nullptr, s);
   };
-  hipGraphExec_t gx1 = nullptr, gx2 = nullptr;
+  hipGraphExec_t gx = nullptr;
   if (g->o.use_graph) {
-    auto key = std::make_pair((const void*)b, (const void*)x);
+    auto key = std::make_tuple((const void*)b, (const void*)x, use_precond);
     auto it = g->bgraphs.find(key);
-    if (it == g->bgraphs.end() || use_precond == 0) {
-      hipGraphExec_t ex[2];
-      for (int ph = 0; ph < 2; ph++) {
-        hipGraph_t graph;
-        TE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = ph == 0 ? phase1() : phase2();
-        hipError_t e = hipStreamEndCapture(s, &graph);
-        if (rc) return rc;
-        if (e != hipSuccess) {
-          set_error(std::string("BiCGStab graph capture: ") + hipGetErrorString(e));
-          return TE_ERR_HIP;
-        }
-        e = hipGraphInstantiate(&ex[ph], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-          set_error(std::string("BiCGStab graph instantiate: ") + hipGetErrorString(e));
-          return TE_ERR_HIP;
-        }
+    if (it == g->bgraphs.end()) {
+      hipGraph_t graph;
+      TE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc = iteration();
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (rc) return rc;
+      if (e != hipSuccess) {
+        set_error(std::string("BiCGStab graph capture: ") + hipGetErrorString(e));
+        return TE_ERR_HIP;
       }
-      if (use_precond) {
-        it = g->bgraphs.emplace(key, std::make_pair(ex[0], ex[1])).first;
-        gx1 = it->second.first;
-        gx2 = it->second.second;
-      } else {
-        gx1 = ex[0];
-        gx2 = ex[1];
+      hipGraphExec_t ex;
+      e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        set_error(std::string("BiCGStab graph instantiate: ") + hipGetErrorString(e));
+        return TE_ERR_HIP;
       }
-    } else {
-      gx1 = it->second.first;
-      gx2 = it->second.second;
+      it = g->bgraphs.emplace(key, ex).first;
     }
+    gx = it->second;
   }
   auto t0 = std::chrono::steady_clock::now();
   int rc = TE_OK;
@@ -550,7 +543,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   TE_RC(launch_ghost_fill(gf, TE_F64, x, 1, s));
   TE_RC(launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, b, x, g->resid, 1, s));
   TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s));
-  TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s));
+  TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s, tol));
   TE_RC(read_scal());
   double r0 = h[SC_R0];
   double residual = h[SC_RNORM] / r0;
@@ -563,19 +556,10 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
         rc = TE_ERR_BREAKDOWN;
         break;
       }
-      if (gx1) TE_HIP(hipGraphLaunch(gx1, s));
-      else TE_RC(phase1());
+      if (gx) TE_HIP(hipGraphLaunch(gx, s));
+      else TE_RC(iteration());
       TE_RC(read_scal());
-      if (h[SC_SNORM] / r0 <= tol) {
-        const double* dirp = use_precond == 2 ? mp : g->p;  // reference: alpha p (BiCGStab.h:162-167)
-        TE_RC(launch_bicg(BICG_XP, d, g->scal, x, nullptr, dirp, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                          s));
-        TE_HIP(hipStreamSynchronize(s));
-        break;
-      }
-      if (gx2) TE_HIP(hipGraphLaunch(gx2, s));
-      else TE_RC(phase2());
-      TE_RC(read_scal());
+      if (h[SC_STATE] == ST_EARLY) break;  // x was updated on the device
       num_its++;
       rho = h[SC_RHO];
       residual = h[SC_RNORM] / r0;
@@ -585,10 +569,6 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   TE_HIP(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
   g->last_seconds = std::chrono::duration<double>(t1 - t0).count();
-  if (!use_precond && gx1) {
-    (void)hipGraphExecDestroy(gx1);
-    (void)hipGraphExecDestroy(gx2);
-  }
   if (iterations) *iterations = num_its;
   if (rc) return rc;
   return check_flags(g);

@@ -195,7 +195,8 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
                                                                const T* __restrict__ mats, const T* __restrict__ s2tab,
                                                                const TF* __restrict__ f, const T* __restrict__ u,
                                                                const T* __restrict__ coeffs, TO* __restrict__ out,
-                                                               Layout L, FusedArgs<T, TC> fa)
+                                                               Layout L, FusedArgs<T, TC> fa,
+                                                               const int4* __restrict__ ring)
 {
   constexpr int n = 16 * NT;
   constexpr int ld = n + 1;  // padded LDS row
@@ -248,13 +249,19 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
       X[y * ld + x] = T(fn[k]);
     }
   };
-  // with ghosts computed here (TILE): the next patch's own values and side information
-  // too; its coarse-level values once its parent index has arrived (mid-iteration)
+  // with ghosts computed here (TILE): the next patch's own values and its ring records
+  // (ring table: lane = ring cell, side lane / n, cell lane % n), then, once those have
+  // arrived (mid-iteration), its coarse-level values and the neighbour values of every
+  // lane's ghost cell (ghost2d_gather; with GM_INTERP each as u and coarse u separately)
   constexpr bool PREF_T = TILE && PREF;
   T un[PREF_T ? FPL : 1];
   TC ucn[PREF_T ? FPL : 1];
-  SideInfo sin[NT];
   int on = -1, parn = 0;
+  int4 rn = make_int4(-1, 0, -1, 0), rc = rn;  // ring records of the next / current patch
+  T gu[4];
+  TC gc[4];
+  T gv[4];  // the current patch's neighbour values (u, or u + P uc)
+  const int rs = lane / n, rt = lane - (lane / n) * n;  // the lane's ring cell (PREF_T: 4 n = 64)
   auto tload_a = [&](int p) {
 #pragma unroll
     for (int k = 0; k < (PREF_T ? FPL : 1); k++) {
@@ -265,7 +272,7 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
       on = fa.porth[p];
       parn = fa.parent[p];
     }
-    ring_prefetch<NT>(dp[p], n, n, lane, sin);
+    rn = ring[(p * 4 + rs) * 2 + (2 * rt >= n ? 1 : 0)];
   };
   auto tload_b = [&]() {
     if (GM == 2) {
@@ -280,6 +287,24 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
         ucn[k] = fa.uc[parn * L.ps + (cy + 1) * L.sx + cx + 1];
       }
     }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      gu[k] = T(0);
+      gc[k] = TC(0);
+    }
+    const int pq = rn.w;
+    ghost2d_gather<KIND>(rn.x, rn.y, rn.z, rs, rt, n, n, [&](int k, int q, int x, int y) {
+      gu[k] = u[q * L.ps + (y + 1) * L.sx + x + 1];
+      if (GM == 2) {
+        const int o = (pq & 7) - 1;
+        int cx = x, cy = y;
+        if (o >= 0) {
+          cx = (x + ((o & 1) ? L.n[0] : 0)) / 2;
+          cy = (y + ((o & 2) ? L.n[1] : 0)) / 2;
+        }
+        gc[k] = fa.uc[(pq >> 3) * L.ps + (cy + 1) * L.sx + cx + 1];
+      }
+    });
   };
   auto tput = [&]() {  // the values UAcc would return, staged from the registers
 #pragma unroll
@@ -287,8 +312,9 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
       const int q = lane + 64 * k, y = q / n, x = q - y * n;
       Tl[(y + 1) * (n + 2) + x + 1] = GM == 2 ? T(un[k] + T(ucn[k])) : un[k];
     }
+    rc = rn;
 #pragma unroll
-    for (int j = 0; j < NT; j++) si[j] = sin[j];
+    for (int k = 0; k < 4; k++) gv[k] = GM == 2 ? T(gu[k] + T(gc[k])) : gu[k];
   };
   auto stage = [&](int p) {
     if (PREF_T) {
@@ -340,7 +366,7 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
         const int s = q / n, t = q - s * n;
         const int ph = s >> 1, up = s & 1;
         T term = T(0);
-        if (si[j].type >= 0) {
+        if ((PREF_T ? rc.x : si[j].type) >= 0) {
           int gi, bi;  // padded index of the ghost and of the boundary cell
           int bx, by;  // boundary cell
           if (ph == 0) {
@@ -358,6 +384,16 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
           if (GM == 0) {
             ug = u[gi];
             ub = u[bi];
+          } else if (PREF_T) {
+            // self terms from the tile, neighbour terms gathered during the previous patch
+            ug = ghost2d_combine<T, KIND>(rc.x, rc.y, t, n,
+                                          [&](int k, int tt) {
+                                            int x, y;
+                                            side_xy(s, k, tt, n, &x, &y);
+                                            return Tl[(y + 1) * (n + 2) + x + 1];
+                                          },
+                                          gv);
+            ub = Tl[(by + 1) * (n + 2) + bx + 1];
           } else if (TILE) {
             TileAcc<T, UAcc<T, TC, GM == 2>> A{ Tl, p, n + 2, U };
             ug = ghost2d<T, KIND>(si[j], p, s, t, n, n, A);
@@ -410,6 +446,8 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
   const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);
   FusedArgs<T, TC> fa{ uc, link ? link->d_parent : nullptr, link ? link->d_orth : nullptr };
+  // ghosts from the ring table: the link's (with the coarse parents) when interpolating
+  const int4* ring = (const int4*)(gm == 2 ? link->d_ring : d->d_ring);
   int ngroups = (d->np + kWaves - 1) / kWaves;
   // n = 16: about as many workgroups as are resident at once, so each wave solves
   // several patches and its prefetch of the next one overlaps the current solve
@@ -418,7 +456,7 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   const int kind = pl->g->kind;
 #define TE_MK(OP_, UZ_, GM_, K_)                                                                            \
   hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_>), dim3(grid), dim3(64 * kWaves), \
-                     shmem, s, d->d_patches, mats, s2, f, u, cf, out, L, fa)
+                     shmem, s, d->d_patches, mats, s2, f, u, cf, out, L, fa, ring)
 #define TE_MKG(OP_)                                                         \
   if (uzero) TE_MK(OP_, true, 0, 0);                                        \
   else if (gm == 0) TE_MK(OP_, false, 0, 0);                                \

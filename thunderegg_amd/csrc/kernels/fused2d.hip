@@ -24,13 +24,13 @@ namespace {
 
 constexpr int kWaves = 4;
 
-template <typename T, typename TF, typename TC, int OP, int KIND>
+template <typename T, typename TF, typename TC, int OP, int KIND, bool P16>
 __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch* __restrict__ dp,
                                                                  const T* __restrict__ u, const T* __restrict__ cf,
                                                                  const TF* __restrict__ f, TC* __restrict__ fc,
                                                                  const int32_t* __restrict__ parent,
                                                                  const int32_t* __restrict__ porth, Layout L,
-                                                                 int neumann)
+                                                                 int neumann, const int4* __restrict__ ring)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n0 = L.n[0], n1 = L.n[1], w = n0 + 2;
@@ -41,14 +41,58 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
   const T phys = (OP == 0 && neumann) ? T(1) : T(-1);
   // each wave walks its own patches; its tile is private, so wave-level ordering suffices
   const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
+  // P16 (16 x 16 cells: ring cell = lane, 2x2 block = lane): the next patch's ring
+  // records, parent, orth, u and f (the lane's 2x2 block) are loaded during the current
+  // one, and the neighbour values of its ghosts as soon as the records are in
+  T un[P16 ? 4 : 1], gu[4];
+  TF fn[P16 ? 4 : 1];
+  int on = -1, cpn = 0;
+  int4 rn = make_int4(-1, 0, -1, 0);
+  auto load_a = [&](int p) {
+    int rs, rt;
+    rn = ring[ring_index(p, lane, n0, n1, &rs, &rt)];
+    on = porth[p];
+    cpn = parent[p];
+#pragma unroll
+    for (int k = 0; k < (P16 ? 4 : 1); k++) {
+      const int q = lane + 64 * k;
+      un[k] = u[p * L.ps + ((q >> 4) + 1) * L.sx + (q & 15) + 1];
+      const int x = 2 * (lane & 7) + (k & 1), y = 2 * (lane >> 3) + (k >> 1);
+      fn[k] = f[p * L.ps + (y + 1) * L.sx + x + 1];
+    }
+  };
+  auto load_b = [&]() { ring_gather_plain<T, KIND>(rn, lane, n0, n1, u, L, gu); };
+  if (P16 && xr.first + wave < xr.end) {
+    load_a(xr.first + wave);
+    load_b();
+  }
   for (int p = xr.first + wave; p < xr.end; p += xr.step) {
     // everything that does not depend on the tile is loaded together with its interior
     SideInfo si[1];
-    tile_interior_2d<T>(p, n0, n1, U, tile, lane);
-    ring_prefetch<1>(dp[p], n0, n1, lane, si);
-    const int o = porth[p], cp = parent[p];
+    int o, cp;
+    TF fcur[P16 ? 4 : 1];
+    const int4 rc = rn;
+    if (P16) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int q = lane + 64 * k;
+        tile[((q >> 4) + 1) * w + (q & 15) + 1] = un[k];
+        fcur[k] = fn[k];
+      }
+      o = on;
+      cp = cpn;
+    } else {
+      tile_interior_2d<T>(p, n0, n1, U, tile, lane);
+      ring_prefetch<1>(dp[p], n0, n1, lane, si);
+      o = porth[p];
+      cp = parent[p];
+    }
     wave_sync();
-    tile_ghosts_2d<T, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
+    const bool has_next = P16 && p + xr.step < xr.end;
+    if (has_next) load_a(p + xr.step);
+    if (P16) tile_ghost_gathered<T, KIND>(rc, lane, n0, n1, tile, phys, gu);
+    else tile_ghosts_2d<T, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
+    if (has_next) load_b();
     wave_sync();
     {
       const DevPatch& P = dp[p];
@@ -62,7 +106,8 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
         for (int j = 0; j < 4; j++) {
           const int x = 2 * X + (j & 1), y = 2 * Y + (j >> 1);
           const int c = (y + 1) * L.sx + x + 1;
-          r[j] = T(f[p * L.ps + c]) - star_tile<T, OP>(tile, w, x, y, ih2, cf, p * L.cs + c, L.sx);
+          const TF fv = P16 ? fcur[j] : f[p * L.ps + c];
+          r[j] = T(fv) - star_tile<T, OP>(tile, w, x, y, ih2, cf, p * L.cs + c, L.sx);
         }
         if (o < 0) {  // copyToParent
 #pragma unroll
@@ -90,7 +135,8 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
                                                            const double* __restrict__ u,
                                                            const double* __restrict__ cf, double* __restrict__ out,
                                                            const double* __restrict__ wv, double* __restrict__ pa,
-                                                           double* __restrict__ pb, Layout L, int neumann)
+                                                           double* __restrict__ pb, Layout L, int neumann,
+                                                           const int4* __restrict__ ring)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n0 = L.n[0], n1 = L.n[1], w = n0 + 2;
@@ -100,10 +146,15 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
   const double phys = (OP == 0 && neumann) ? 1.0 : -1.0;
   double sa = 0.0, sq = 0.0;
   const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
-  // P16: the next patch's u, w and side information are loaded during the current one
+  // P16 (16 x 16 cells: ring cell = lane): the next patch's u, w and ring records are
+  // loaded during the current one, and the neighbour values of its ghosts as soon as
+  // the records are in
   double un[P16 ? 4 : 1], wn[P16 ? 4 : 1], wcur[P16 ? 4 : 1];
-  SideInfo sin[1];
-  auto load = [&](int p) {
+  int4 rn = make_int4(-1, 0, -1, 0);
+  double gu[4];
+  auto load_a = [&](int p) {
+    int rs, rt;
+    rn = ring[ring_index(p, lane, n0, n1, &rs, &rt)];
 #pragma unroll
     for (int k = 0; k < (P16 ? 4 : 1); k++) {
       const int q = lane + 64 * k;
@@ -111,12 +162,16 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
       un[k] = u[c];
       wn[k] = wv[c];
     }
-    ring_prefetch<1>(dp[p], n0, n1, lane, sin);
   };
-  if (P16 && xr.first + wave < xr.end) load(xr.first + wave);
+  auto load_b = [&]() { ring_gather_plain<double, KIND>(rn, lane, n0, n1, u, L, gu); };
+  if (P16 && xr.first + wave < xr.end) {
+    load_a(xr.first + wave);
+    load_b();
+  }
   for (int p = xr.first + wave; p < xr.end; p += xr.step) {
     const int base = p * L.ps;
     SideInfo si[1];
+    int4 rc = rn;
     if (P16) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -124,14 +179,16 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
         tile[((q >> 4) + 1) * w + (q & 15) + 1] = un[k];
         wcur[k] = wn[k];
       }
-      si[0] = sin[0];
     } else {
       tile_interior_2d<double>(p, n0, n1, U, tile, lane);
       ring_prefetch<1>(dp[p], n0, n1, lane, si);
     }
     wave_sync();
-    if (P16 && p + xr.step < xr.end) load(p + xr.step);
-    tile_ghosts_2d<double, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
+    const bool has_next = P16 && p + xr.step < xr.end;
+    if (has_next) load_a(p + xr.step);
+    if (P16) tile_ghost_gathered<double, KIND>(rc, lane, n0, n1, tile, phys, gu);
+    else tile_ghosts_2d<double, KIND>(dp[p], si[0], p, n0, n1, U, tile, phys, lane);
+    if (has_next) load_b();
     wave_sync();
     {
       const DevPatch& P = dp[p];
@@ -178,19 +235,29 @@ void launch_rr(const te_ghost_filler* g, const te_level_link* link, int op_kind,
   const te_domain* d = g->d;
   Layout L = make_layout(2, d->ns, 1, d->np);
   size_t shmem = tile_bytes(d, sizeof(T));
+  const bool p16 = d->ns[0] == 16 && d->ns[1] == 16;
   int ngroups = (d->np + kWaves - 1) / kWaves;
-  int grid = ngroups < 8192 ? ngroups : 8192;
-#define TE_RR(OP_, K_)                                                                                      \
-  hipLaunchKernelGGL((resid_restrict_2d<T, TF, TC, OP_, K_>), dim3(grid), dim3(64 * kWaves), shmem, s,     \
-                     d->d_patches, (const T*)u, (const T*)coeffs, (const TF*)f, (TC*)fc, link->d_parent,   \
-                     link->d_orth, L, neumann)
+  // 16 x 16: about as many workgroups as are resident at once, so that each wave takes
+  // several patches and its loads of the next one overlap the current one
+  const int cap = p16 ? 2048 : 8192;
+  int grid = ngroups < cap ? ngroups : cap;
+#define TE_RR(OP_, K_, P_)                                                                                    \
+  hipLaunchKernelGGL((resid_restrict_2d<T, TF, TC, OP_, K_, P_>), dim3(grid), dim3(64 * kWaves), shmem, s,   \
+                     d->d_patches, (const T*)u, (const T*)coeffs, (const TF*)f, (TC*)fc, link->d_parent,     \
+                     link->d_orth, L, neumann, (const int4*)d->d_ring)
+#define TE_RRP(OP_, K_)             \
+  do {                              \
+    if (p16) TE_RR(OP_, K_, true);  \
+    else TE_RR(OP_, K_, false);     \
+  } while (0)
   if (op_kind == 0) {
-    if (g->kind == TE_BILINEAR) TE_RR(0, TE_BILINEAR);
-    else TE_RR(0, TE_BIQUADRATIC);
+    if (g->kind == TE_BILINEAR) TE_RRP(0, TE_BILINEAR);
+    else TE_RRP(0, TE_BIQUADRATIC);
   } else {
-    if (g->kind == TE_BILINEAR) TE_RR(1, TE_BILINEAR);
-    else TE_RR(1, TE_BIQUADRATIC);
+    if (g->kind == TE_BILINEAR) TE_RRP(1, TE_BILINEAR);
+    else TE_RRP(1, TE_BIQUADRATIC);
   }
+#undef TE_RRP
 #undef TE_RR
 }
 
@@ -249,7 +316,7 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
   const bool p16 = d->ns[0] == 16 && d->ns[1] == 16;
 #define TE_SD(OP_, K_, Q_, P_)                                                                                    \
   hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches, u, \
-                     coeffs, out, w, pa, pb, L, neumann)
+                     coeffs, out, w, pa, pb, L, neumann, (const int4*)d->d_ring)
 #define TE_SDQ(OP_, K_)                      \
   do {                                       \
     if (pb && p16) TE_SD(OP_, K_, true, true);   \

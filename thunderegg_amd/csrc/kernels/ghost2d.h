@@ -37,26 +37,80 @@ __device__ __forceinline__ SideInfo side_info(const DevPatch& P, int s)
   return SideInfo{ P.type[slot], P.orth[slot], P.nbr[slot][0], P.nbr[slot][1] };
 }
 
-// n: cells along the side's axis (the depth direction); nt: cells along the side (even)
-template <typename T, int KIND, class Acc>
-__device__ __forceinline__ T ghost2d(const SideInfo& si, int p, int s, int t, int n, int nt, Acc U)
+// One ghost value is a sum of self terms (interior cells of the patch itself) and
+// neighbour terms.  ghost2d_gather names the neighbour cells, in a fixed order, through
+// ld(k, q, x, y) (k: position in the order, q: neighbour patch, (x, y): its cell);
+// ghost2d_combine forms the value from the self cells S(k, tt) (depth k from side s,
+// tangential index tt) and the gathered neighbour values v[k].  Split this way, a caller
+// can issue the neighbour loads of a later patch early and combine them when it gets there.
+// q: the neighbour the ghost reads (for finer neighbours: nbr1 when 2 t >= nt, else nbr0).
+// n: cells along the side's axis (the depth direction); nt: cells along the side (even).
+// The coarser-neighbour BiQuadratic stencil: three cells of the coarse side i0, i0 + di,
+// i0 + 2 di and which of the reference's four weight patterns applies.
+__device__ __forceinline__ void coarse_quad_cells(int o, int t, int nt, int* i0, int* di, int* cs)
 {
-  const int type = si.type;
+  const int e = o == 0 ? t : nt - 1 - t;  // distance from the end of the coarse side
+  if (e <= 1) {
+    *i0 = o == 0 ? 0 : nt - 1;
+    *di = o == 0 ? 1 : -1;
+    *cs = e;
+  } else {
+    const int tt = o == 0 ? t : t + nt;
+    *i0 = tt / 2 - 1;
+    *di = 1;
+    *cs = 2 + (tt & 1);
+  }
+}
+
+template <int KIND, class Ld>
+__device__ __forceinline__ void ghost2d_gather(int type, int orth, int q, int s, int t, int n, int nt, Ld ld)
+{
   int x, y;
-  auto S = [&](int q, int side, int k, int tt) -> T {
-    side_xy(side, k, tt, n, &x, &y);
-    return U(q, x, y);
-  };
-  if (type == 0) return S(si.nbr0, s ^ 1, 0, t);
-  if (type == 1) {
-    const int q = si.nbr0;
-    const int o = si.orth;
+  const int so = s ^ 1;
+  if (type == 0) {
+    side_xy(so, 0, t, n, &x, &y);
+    ld(0, q, x, y);
+  } else if (type == 1) {
     if (KIND == TE_BILINEAR) {
-      const int off = o ? nt : 0;
+      side_xy(so, 0, (t + (orth ? nt : 0)) / 2, n, &x, &y);
+      ld(0, q, x, y);
+    } else {
+      int i0, di, cs;
+      coarse_quad_cells(orth, t, nt, &i0, &di, &cs);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        side_xy(so, 0, i0 + k * di, n, &x, &y);
+        ld(k, q, x, y);
+      }
+    }
+  } else if (type == 2) {
+    const int tf = 2 * t - (2 * t >= nt ? nt : 0);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      if (KIND == TE_BILINEAR) {
+        side_xy(so, 0, tf + j, n, &x, &y);
+        ld(j, q, x, y);
+      } else {
+        side_xy(so, 0, tf + j, n, &x, &y);
+        ld(2 * j, q, x, y);
+        side_xy(so, 1, tf + j, n, &x, &y);
+        ld(2 * j + 1, q, x, y);
+      }
+    }
+  }
+}
+
+template <typename T, int KIND, class Self>
+__device__ __forceinline__ T ghost2d_combine(int type, int orth, int t, int nt, Self S, const T (&v)[4])
+{
+  if (type == 0) return v[0];
+  if (type == 1) {
+    if (KIND == TE_BILINEAR) {
+      const int off = orth ? nt : 0;
       T G = T(0);
       for (int tp = t - 1; tp <= t + 1; tp++) {
         if (tp < 0 || tp >= nt) continue;
-        T sv = S(p, s, 0, tp);
+        T sv = S(0, tp);
         if (tp == t) {
           G += T(2.0 / 3.0) * sv;
         } else {
@@ -64,25 +118,12 @@ __device__ __forceinline__ T ghost2d(const SideInfo& si, int p, int s, int t, in
           if (partner == t) G += T(-1.0 / 3.0) * sv;
         }
       }
-      return G + T(2.0 / 3.0) * S(q, s ^ 1, 0, (t + off) / 2);
+      return G + T(2.0 / 3.0) * v[0];
     }
-    T G = T(2) * S(p, s, 0, t) / T(3) - S(p, s, 1, t) / T(5);
-    // three cells of the coarse neighbour's side, i0, i0 + di, i0 + 2 di, loaded
-    // unconditionally; the reference's four weight patterns are then applied to them
-    // (so the compiler does not keep the loads of every case live at once)
-    const int e = o == 0 ? t : nt - 1 - t;  // distance from the end of the coarse side
+    T G = T(2) * S(0, t) / T(3) - S(1, t) / T(5);
     int i0, di, cs;
-    if (e <= 1) {
-      i0 = o == 0 ? 0 : nt - 1;
-      di = o == 0 ? 1 : -1;
-      cs = e;
-    } else {
-      const int tt = o == 0 ? t : t + nt;
-      i0 = tt / 2 - 1;
-      di = 1;
-      cs = 2 + (tt & 1);
-    }
-    const T c0 = S(q, s ^ 1, 0, i0), c1 = S(q, s ^ 1, 0, i0 + di), c2 = S(q, s ^ 1, 0, i0 + 2 * di);
+    coarse_quad_cells(orth, t, nt, &i0, &di, &cs);
+    const T c0 = v[0], c1 = v[1], c2 = v[2];
     T add;
     if (cs == 0) add = T(3) * c0 / T(4) - T(3) * c1 / T(10) + c2 / T(12);
     else if (cs == 1) add = T(7) * c0 / T(20) + T(7) * c1 / T(30) - c2 / T(20);
@@ -90,28 +131,47 @@ __device__ __forceinline__ T ghost2d(const SideInfo& si, int p, int s, int t, in
     else add = -c0 / T(20) + c1 / T(2) + c2 / T(12);
     return G + add;
   }
-  // fine neighbours (patch sides of even length: ghost t lies against fine cells 2t, 2t + 1
-  // of neighbour 0 when 2t < nt, else of neighbour 1)
+  // finer neighbours (ghost t lies against fine cells 2t, 2t + 1 of neighbour 0 when
+  // 2t < nt, else of neighbour 1)
   T G;
   if (KIND == TE_BILINEAR) {
-    G = T(-1.0 / 3.0) * S(p, s, 0, t);
+    G = T(-1.0 / 3.0) * S(0, t);
   } else {
     if (t == 0 || t == nt - 1) {
       const int i0 = t == 0 ? 0 : nt - 1, di = t == 0 ? 1 : -1;
-      G = -S(p, s, 0, i0) / T(10) + S(p, s, 0, i0 + di) / T(15) - S(p, s, 0, i0 + 2 * di) / T(30);
+      G = -S(0, i0) / T(10) + S(0, i0 + di) / T(15) - S(0, i0 + 2 * di) / T(30);
     } else {
-      G = -S(p, s, 0, t - 1) / T(30) - S(p, s, 0, t + 1) / T(30);
+      G = -S(0, t - 1) / T(30) - S(0, t + 1) / T(30);
     }
   }
-  const int i = 2 * t >= nt ? 1 : 0;
-  const int q = i ? si.nbr1 : si.nbr0;
-  const int tf = 2 * t - i * nt;
 #pragma unroll
   for (int j = 0; j < 2; j++) {
-    if (KIND == TE_BILINEAR) G += T(2.0 / 3.0) * S(q, s ^ 1, 0, tf + j);
-    else G += S(q, s ^ 1, 0, tf + j) / T(3) + S(q, s ^ 1, 1, tf + j) / T(5);
+    if (KIND == TE_BILINEAR) G += T(2.0 / 3.0) * v[j];
+    else G += v[2 * j] / T(3) + v[2 * j + 1] / T(5);
   }
   return G;
+}
+
+// the neighbour a ghost cell reads
+__device__ __forceinline__ int ghost_nbr(const SideInfo& si, int t, int nt)
+{
+  return si.type == 2 && 2 * t >= nt ? si.nbr1 : si.nbr0;
+}
+
+// The value of ghost cell t on side s of patch p; `U(q, x, y)` reads every cell.
+template <typename T, int KIND, class Acc>
+__device__ __forceinline__ T ghost2d(const SideInfo& si, int p, int s, int t, int n, int nt, Acc U)
+{
+  T v[4] = { T(0), T(0), T(0), T(0) };
+  ghost2d_gather<KIND>(si.type, si.orth, ghost_nbr(si, t, nt), s, t, n, nt,
+                       [&](int k, int q, int x, int y) { v[k] = U(q, x, y); });
+  return ghost2d_combine<T, KIND>(si.type, si.orth, t, nt,
+                                  [&](int k, int tt) {
+                                    int x, y;
+                                    side_xy(s, k, tt, n, &x, &y);
+                                    return U(p, x, y);
+                                  },
+                                  v);
 }
 
 template <typename T>
@@ -255,6 +315,61 @@ __device__ __forceinline__ void tile_ghosts_2d(const DevPatch& P, const SideInfo
     else v = ghost2d<T, KIND>(si, p, s, t, na, nt, A);
     tile[(gy + 1) * w + gx + 1] = v;
   }
+}
+
+// ---- ghosts from the ring table, gathered ahead ----
+// Ring cell g of an n0 x n1 patch: its record in the ring table (the int4 at the
+// returned index) and the side / tangential index.
+__device__ __forceinline__ int ring_index(int p, int g, int n0, int n1, int* s, int* t)
+{
+  ring_cell(g, n0, n1, s, t);
+  const int nt = (*s >> 1) == 0 ? n1 : n0;
+  return (p * 4 + *s) * 2 + (2 * *t >= nt ? 1 : 0);
+}
+
+// The neighbour values ring cell g of a patch needs (record r), read from u.
+template <typename T, int KIND>
+__device__ __forceinline__ void ring_gather_plain(const int4& r, int g, int n0, int n1, const T* __restrict__ u,
+                                                  const Layout& L, T (&v)[4])
+{
+  int s, t;
+  ring_cell(g, n0, n1, &s, &t);
+  const int na = (s >> 1) == 0 ? n0 : n1, nt = (s >> 1) == 0 ? n1 : n0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = T(0);
+  ghost2d_gather<KIND>(r.x, r.y, r.z, s, t, na, nt,
+                       [&](int k, int q, int x, int y) { v[k] = u[q * L.ps + (y + 1) * L.sx + x + 1]; });
+}
+
+// The ghost of ring cell g (record r, gathered neighbour values v) written into the tile
+// of the patch, whose interior is in place: ghost2d_combine with the self terms from the
+// tile, or phys * (adjacent interior value) on a physical side.
+template <typename T, int KIND>
+__device__ __forceinline__ void tile_ghost_gathered(const int4& r, int g, int n0, int n1, T* tile, T phys,
+                                                    const T (&v)[4])
+{
+  const int w = n0 + 2;
+  int s, t;
+  ring_cell(g, n0, n1, &s, &t);
+  const int a = s >> 1, up = s & 1;
+  const int na = a == 0 ? n0 : n1, nt = a == 0 ? n1 : n0;
+  int bx, by;
+  side_xy(s, 0, t, na, &bx, &by);
+  const int d = up ? 1 : -1;
+  const int gx = a == 0 ? bx + d : bx, gy = a == 0 ? by : by + d;
+  T val;
+  if (r.x < 0) {
+    val = phys * tile[(by + 1) * w + bx + 1];
+  } else {
+    val = ghost2d_combine<T, KIND>(r.x, r.y, t, nt,
+                                   [&](int k, int tt) {
+                                     int x, y;
+                                     side_xy(s, k, tt, na, &x, &y);
+                                     return tile[(y + 1) * w + x + 1];
+                                   },
+                                   v);
+  }
+  tile[(gy + 1) * w + gx + 1] = val;
 }
 
 }  // namespace te

@@ -121,7 +121,8 @@ __global__ void __launch_bounds__(kThreads) star_dot_kernel(const DevPatch* __re
   }
 }
 
-// Vector updates of one BiCGStab iteration (interior cells only).
+// Vector updates of one BiCGStab iteration (interior cells only).  After the s-norm test
+// has fired (state ST_EARLY) only BICG_XP runs, and BICG_XP runs only then.
 template <int CPT, int MODE>
 __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* __restrict__ scal,
                                                         double* __restrict__ v0, double* __restrict__ v1,
@@ -132,6 +133,10 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
                                                         const double* __restrict__ a4, double* __restrict__ pa,
                                                         double* __restrict__ pb)
 {
+  if (MODE != BICG_INIT) {
+    const double st = scal[SC_STATE];
+    if (MODE == BICG_XP ? st != ST_EARLY : st != ST_RUN) return;
+  }
   CellSet<CPT> cs;
   make_cells<CPT>(L, false, cs);
   double sa = 0.0, sb = 0.0;
@@ -193,8 +198,9 @@ __device__ double fold(const double* __restrict__ part, int n)
 
 __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double* __restrict__ pa,
                                                           const double* __restrict__ pb, int n,
-                                                          double* __restrict__ scal)
+                                                          double* __restrict__ scal, double tol)
 {
+  if (mode != FIN_INIT && scal[SC_STATE] != ST_RUN) return;
   double A = fold(pa, n);
   double B = (mode == FIN_INIT || mode == FIN_OMEGA || mode == FIN_RHO) ? fold(pb, n) : 0.0;
   if (threadIdx.x != 0) return;
@@ -203,9 +209,17 @@ __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double
       scal[SC_R0] = sqrt(A);
       scal[SC_RHO] = B;
       scal[SC_RNORM] = sqrt(B);
+      scal[SC_TOL] = tol;
+      scal[SC_STATE] = ST_RUN;
       break;
     case FIN_ALPHA: scal[SC_ALPHA] = scal[SC_RHO] / A; break;
-    case FIN_SNORM: scal[SC_SNORM] = sqrt(A); break;
+    case FIN_SNORM: {
+      // if (s.twoNorm() / r0_norm <= tolerance): x += alpha p and stop (BiCGStab.h:162-167)
+      const double sn = sqrt(A);
+      scal[SC_SNORM] = sn;
+      if (sn / scal[SC_R0] <= scal[SC_TOL]) scal[SC_STATE] = ST_EARLY;
+      break;
+    }
     case FIN_OMEGA: scal[SC_OMEGA] = A / B; break;
     default: {  // FIN_RHO
       double rho_new = A;
@@ -319,9 +333,9 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
   return TE_OK;
 }
 
-int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s)
+int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s, double tol)
 {
-  hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, n, scal);
+  hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, n, scal, tol);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

@@ -82,6 +82,7 @@ struct te_domain {
   double* d_result = nullptr;      // reduction results
   int32_t* d_flags = nullptr;      // breakdown flags
   int scratch_len = 0;
+  int32_t* d_ring = nullptr;       // 2D: ring table (te::ring_table), pq fields 0
 };
 
 struct te_ghost_filler {
@@ -99,9 +100,19 @@ struct te_level_link {
   // per coarse patch: children (fine local indexes) in increasing local-index order
   int32_t* d_child = nullptr;       // [np_coarse][8]
   int32_t* d_nchild = nullptr;      // [np_coarse]
+  int32_t* d_ring = nullptr;        // 2D: ring table of the fine level with the pq fields set
 };
 
 namespace te {
+
+// 2D ring table: for patch p, side s and half h of that side (h = 1 for ghost cells
+// t >= n / 2), the int4 record at [(p * 4 + s) * 2 + h] = {type, orth, q, pq} of what a
+// ghost cell there reads: the side's neighbour type and orth_on_coarse, the neighbour
+// patch q (for finer neighbours the one facing that half; -1 on a physical side) and,
+// when the table belongs to a level link, q's parent * 8 + orth_on_parent(q) + 1 in the
+// coarser level (0 otherwise).  One 16-byte load gives a lane everything it needs to
+// issue the neighbour reads of its ghost cell.
+std::vector<int32_t> ring_table(const te_domain* d, const int32_t* parent, const int32_t* orth_on_parent);
 
 // ---- launchers (defined in kernels/*.hip) ----
 int launch_ghost_fill(const te_ghost_filler* g, int dtype, void* u, int ncomp, hipStream_t s);
