@@ -11,6 +11,7 @@
 //    writes the FP64 result directly.
 //  - Iterative::BiCGStab::solve (Iterative/BiCGStab.h:111-203) in FP64 with the
 //    scalars on the device; the host reads ||s|| and ||r|| once each per iteration.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -58,6 +59,10 @@ struct te_gmg {
          *ms = nullptr, *mp = nullptr;
   double *scal = nullptr, *pa = nullptr, *pb = nullptr;
   double* host_scal = nullptr;  // pinned
+  // host-coherent copy of the scalars written by the device at the end of each BiCGStab
+  // iteration (host and device addresses)
+  double* mirror_h = nullptr;
+  double* mirror_d = nullptr;
   int nblk = 0;
   int32_t* d_flags = nullptr;
   std::map<std::pair<const void*, const void*>, hipGraphExec_t> vgraphs;
@@ -294,6 +299,7 @@ void free_gmg(te_gmg* g)
   for (double* v : vs) (void)hipFree(v);
   (void)hipFree(g->d_flags);
   if (g->host_scal) (void)hipHostFree(g->host_scal);
+  if (g->mirror_h) (void)hipHostFree(g->mirror_h);
   if (g->s) (void)hipStreamDestroy(g->s);
   delete g;
 }
@@ -430,6 +436,12 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     if (e == hipSuccess) e = hipMemset(g->d_flags, 0, sizeof(int32_t) * 4);
     if (e == hipSuccess) e = hipMemset(g->scal, 0, sizeof(double) * 16);
     if (e == hipSuccess) e = hipHostMalloc(&g->host_scal, sizeof(double) * 16, hipHostMallocDefault);
+    if (e == hipSuccess)
+      e = hipHostMalloc(&g->mirror_h, sizeof(double) * 16, hipHostMallocCoherent | hipHostMallocMapped);
+    if (e == hipSuccess) {
+      std::memset(g->mirror_h, 0, sizeof(double) * 16);
+      e = hipHostGetDevicePointer((void**)&g->mirror_d, g->mirror_h, 0);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(g->s);
     if (e != hipSuccess) {
       set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
@@ -485,7 +497,9 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   };
   double* mp = use_precond ? g->mp : g->p;
   double* ms = use_precond ? g->ms : g->sv;
-  // One iteration, enqueued as a whole: the host reads the scalars once per iteration.
+  // One iteration, enqueued as a whole: the host reads the scalars once per iteration,
+  // from the copy FIN_RHO publishes (so the next iteration is enqueued while the p update
+  // still runs).
   //   mp = M p; ap = A mp (+ rhat.ap); alpha; s = r - alpha ap; ||s|| and the s-norm test;
   //   if it fired: x += alpha p (mp with use_precond = 2), and the updates after it are
   //   skipped on the device (the kernels read the state word); else
@@ -507,7 +521,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, g->as, g->sv, g->pa, g->pb, g->fly, s));
     TE_RC(launch_bicg_finalize(FIN_OMEGA, g->pa, g->pb, nb, g->scal, s));
     TE_RC(launch_bicg(BICG_XR, d, g->scal, x, g->resid, mp, ms, g->ap, g->as, g->rhat, g->pa, g->pb, s));
-    TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s));
+    TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
     return launch_bicg(BICG_P, d, g->scal, g->p, nullptr, g->ap, g->resid, nullptr, nullptr, nullptr, nullptr,
                        nullptr, s);
   };
@@ -545,6 +559,29 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s));
   TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s, tol));
   TE_RC(read_scal());
+  double seq = h[SC_SEQ];
+  // wait for the iteration with sequence number `want` to publish its scalars into h
+  auto wait_published = [&](double want) -> int {
+    volatile double* m = g->mirror_h;
+    for (unsigned spin = 1;; spin++) {
+      if (m[SC_SEQ] == want) break;
+      if ((spin & 255) == 0) {
+        hipError_t q = hipStreamQuery(s);
+        if (q == hipSuccess && m[SC_SEQ] == want) break;
+        if (q == hipSuccess) {
+          set_error("BiCGStab: the iteration finished without publishing its scalars");
+          return TE_ERR_HIP;
+        }
+        if (q != hipErrorNotReady) {
+          set_error(std::string("BiCGStab iteration: ") + hipGetErrorString(q));
+          return TE_ERR_HIP;
+        }
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    for (int k = 0; k < SC_COUNT; k++) h[k] = m[k];
+    return TE_OK;
+  };
   double r0 = h[SC_R0];
   double residual = h[SC_RNORM] / r0;
   double rho = h[SC_RHO];
@@ -558,7 +595,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
       }
       if (gx) TE_HIP(hipGraphLaunch(gx, s));
       else TE_RC(iteration());
-      TE_RC(read_scal());
+      seq += 1.0;
+      TE_RC(wait_published(seq));
       if (h[SC_STATE] == ST_EARLY) break;  // x was updated on the device
       num_its++;
       rho = h[SC_RHO];

@@ -5,7 +5,11 @@
 
 namespace te {
 
-enum BicgScalar { SC_RHO = 0, SC_ALPHA, SC_OMEGA, SC_BETA, SC_R0, SC_SNORM, SC_RNORM, SC_TOL, SC_STATE, SC_COUNT };
+enum BicgScalar {
+  SC_RHO = 0, SC_ALPHA, SC_OMEGA, SC_BETA, SC_R0, SC_SNORM, SC_RNORM, SC_TOL, SC_STATE,
+  SC_SEQ,  // iterations published by FIN_RHO (never reset)
+  SC_COUNT
+};
 // SC_STATE: the iteration runs (ST_RUN) until the s-norm test of BiCGStab.h:162 fires
 // (ST_EARLY); from then on only the early-exit update x += alpha p runs.
 enum BicgState { ST_RUN = 0, ST_EARLY = 1 };
@@ -22,9 +26,12 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
 int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const double* a0,
                 const double* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
                 hipStream_t s);
-// tol: the relative tolerance, stored by FIN_INIT for the s-norm test of FIN_SNORM
+// tol: the relative tolerance, stored by FIN_INIT for the s-norm test of FIN_SNORM.
+// mirror (FIN_RHO): device pointer to host-coherent memory of SC_COUNT doubles; FIN_RHO
+// copies the scalars there and then bumps SC_SEQ, so the host can read an iteration's
+// outcome while the rest of the iteration is still running.
 int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s,
-                         double tol = 0.0);
+                         double tol = 0.0, double* mirror = nullptr);
 int launch_patch_bicgstab(const te_domain* d, int dtype, int op_kind, int neumann, const void* coeffs, const void* f,
                           void* u, int max_it, double tol, int32_t* iters, int32_t* bd, hipStream_t s);
 

@@ -196,13 +196,19 @@ __device__ double fold(const double* __restrict__ part, int n)
   return block_reduce<RED_SUM>(acc);
 }
 
+// After the s-norm test has fired only FIN_RHO runs, to publish the state.
 __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double* __restrict__ pa,
                                                           const double* __restrict__ pb, int n,
-                                                          double* __restrict__ scal, double tol)
+                                                          double* __restrict__ scal, double tol,
+                                                          double* __restrict__ mirror)
 {
-  if (mode != FIN_INIT && scal[SC_STATE] != ST_RUN) return;
-  double A = fold(pa, n);
-  double B = (mode == FIN_INIT || mode == FIN_OMEGA || mode == FIN_RHO) ? fold(pb, n) : 0.0;
+  const bool run = mode == FIN_INIT || scal[SC_STATE] == ST_RUN;
+  if (!run && mode != FIN_RHO) return;
+  double A = 0.0, B = 0.0;
+  if (run) {
+    A = fold(pa, n);
+    B = (mode == FIN_INIT || mode == FIN_OMEGA || mode == FIN_RHO) ? fold(pb, n) : 0.0;
+  }
   if (threadIdx.x != 0) return;
   switch (mode) {
     case FIN_INIT:
@@ -222,10 +228,19 @@ __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double
     }
     case FIN_OMEGA: scal[SC_OMEGA] = A / B; break;
     default: {  // FIN_RHO
-      double rho_new = A;
-      scal[SC_BETA] = rho_new * scal[SC_ALPHA] / (scal[SC_RHO] * scal[SC_OMEGA]);
-      scal[SC_RHO] = rho_new;
-      scal[SC_RNORM] = sqrt(B);
+      if (run) {
+        double rho_new = A;
+        scal[SC_BETA] = rho_new * scal[SC_ALPHA] / (scal[SC_RHO] * scal[SC_OMEGA]);
+        scal[SC_RHO] = rho_new;
+        scal[SC_RNORM] = sqrt(B);
+      }
+      const double seq = scal[SC_SEQ] + 1.0;
+      scal[SC_SEQ] = seq;
+      if (mirror) {
+        for (int k = 0; k < SC_SEQ; k++) mirror[k] = scal[k];
+        __threadfence_system();  // the scalars reach the host before the sequence number
+        *(volatile double*)&mirror[SC_SEQ] = seq;
+      }
     }
   }
 }
@@ -333,9 +348,10 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
   return TE_OK;
 }
 
-int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s, double tol)
+int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s, double tol,
+                         double* mirror)
 {
-  hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, n, scal, tol);
+  hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, n, scal, tol, mirror);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
