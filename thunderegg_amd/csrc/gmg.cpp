@@ -72,6 +72,11 @@ struct te_gmg {
   bool fused = false;  // 2D matrix-core path: ghost fills fused into the stencil / smoother launches
   std::vector<char> rr_pre;  // per level: restriction fused with the coarser level's first sweep
   bool fly = false;    // outer operator computes ghosts on the fly
+  // mixed precision on the fused path: M p and M s are FP32 values (the finest level's last
+  // sweep computes in FP32), so the BiCGStab keeps them in FP32 buffers
+  bool m32 = false;
+  float* mp32 = nullptr;
+  float* ms32 = nullptr;
 };
 
 namespace {
@@ -152,10 +157,12 @@ int visit(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
 // for Cycle::restrict (ghost fill + residual + restriction); interpolation and the
 // ghost fill of the first post-smoothing sweep happen inside that sweep's launch.
 // Sweeps after the first pre-smoothing one read the solution from one buffer and
-// write the other; the finest level's last sweep writes the caller's FP64 vector.
+// write the other; the finest level's last sweep writes the caller's vector u_ext
+// (type ext_dt: FP64, or FP32 for the mixed-precision BiCGStab's own buffers, which then
+// hold the same values: that sweep computes in FP32).
 // pre_done: this level's first sweep (from zero) was already done by the finer level's
 // launch (launch_rr_pre), its result in Lv.u.
-int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext, bool pre_done)
+int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_dt, bool pre_done)
 {
   GLevel& Lv = g->L[li];
   const int nl = (int)g->L.size();
@@ -165,8 +172,8 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext, bool p
   // one sweep from buf[cur]: gm 0 = u known zero, 1 = ghosts on the fly, 2 = with interpolation
   auto sweep = [&](int mode, bool last_of_level) -> int {
     const bool to_ext = finest && last_of_level;
-    void* out = to_ext ? (void*)u_ext : buf[cur ^ 1];
-    int dto = to_ext ? TE_F64 : Lv.dt;
+    void* out = to_ext ? u_ext : buf[cur ^ 1];
+    int dto = to_ext ? ext_dt : Lv.dt;
     const void* uc = nullptr;
     int dtc = TE_F64;
     if (mode == 2) {
@@ -189,7 +196,7 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext, bool p
     if (n == 0) {
       TE_RC(zero_with_ghost(g, li, buf[0], Lv.dt));
       Lv.cur = buf[0];
-      if (finest) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), TE_F64, u_ext, Lv.dt, buf[0], g->s));
+      if (finest) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), ext_dt, u_ext, Lv.dt, buf[0], g->s));
       return TE_OK;
     }
     for (int k = pre_done ? 1 : 0; k < n; k++) TE_RC(sweep(k == 0 ? 0 : 1, k == n - 1));
@@ -207,35 +214,41 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, double* u_ext, bool p
     TE_RC(launch_rr_pre(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur], Cv.plan,
                         Cv.dt, Cv.f, Cv.u, g->s));
     Cv.cur = Cv.u;
-    TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext, true));
+    TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext, ext_dt, true));
   } else {
     TE_RC(launch_resid_restrict(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur],
                                 Cv.dt, Cv.f, g->s));
-    TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext, false));
+    TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext, ext_dt, false));
   }
   int n = g->o.post_sweeps;
   if (n == 0) {
     // DirectInterpolator alone (GMG/DirectInterpolator.h:60-91)
     TE_RC(launch_interpolate(Lv.link, Cv.dt, Cv.cur, Lv.dt, buf[cur], 1, g->s));
     Lv.cur = buf[cur];
-    if (finest) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), TE_F64, u_ext, Lv.dt, buf[cur], g->s));
+    if (finest) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), ext_dt, u_ext, Lv.dt, buf[cur], g->s));
     return TE_OK;
   }
   for (int k = 0; k < n; k++) TE_RC(sweep(k == 0 ? 2 : 1, k == n - 1));
   return TE_OK;
 }
 
-// Cycle::apply (GMG/Cycle.h:94-98) enqueued on g->s.
-int vcycle_enqueue(te_gmg* g, const double* f, double* u)
+// Cycle::apply (GMG/Cycle.h:94-98) enqueued on g->s.  udt: type of u (FP32 only on the
+// fused mixed-precision path, see visit_fused).
+int vcycle_enqueue(te_gmg* g, const double* f, void* u, int udt = TE_F64)
 {
   GLevel& L0 = g->L[0];
-  if (g->fused) return visit_fused(g, 0, f, TE_F64, u, false);
+  if (g->fused) return visit_fused(g, 0, f, TE_F64, u, udt, false);
+  if (udt != TE_F64) {
+    set_error("internal: an FP32 V-cycle result needs the fused path");
+    return TE_ERR_UNSUPPORTED;
+  }
+  double* ud = (double*)u;
   if (L0.dt != TE_F64 && g->o.smoother != 0) {
     // the inner-Krylov smoother works on same-dtype f and u: convert f once
     TE_RC(launch_convert(te_domain_vector_length(L0.d, 1), L0.dt, L0.f, TE_F64, f, g->s));
-    return visit(g, 0, L0.f, L0.dt, u);
+    return visit(g, 0, L0.f, L0.dt, ud);
   }
-  return visit(g, 0, f, TE_F64, u);
+  return visit(g, 0, f, TE_F64, ud);
 }
 
 int vcycle_launch(te_gmg* g, const double* f, double* u)
@@ -297,6 +310,8 @@ void free_gmg(te_gmg* g)
   }
   double* vs[] = { g->resid, g->rhat, g->p, g->ap, g->as, g->sv, g->ms, g->mp, g->scal, g->pa, g->pb };
   for (double* v : vs) (void)hipFree(v);
+  (void)hipFree(g->mp32);
+  (void)hipFree(g->ms32);
   (void)hipFree(g->d_flags);
   if (g->host_scal) (void)hipHostFree(g->host_scal);
   if (g->mirror_h) (void)hipHostFree(g->mirror_h);
@@ -406,6 +421,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     }
     g->fused = fused;
     g->fly = allow && star_dot_fly_supported(g->L[0].gf);
+    g->m32 = g->fused && g->fly && g->L[0].dt == TE_F32;
     const char* nr = std::getenv("TE_NO_RRPRE");
     const bool allow_rp = fused && !(nr && nr[0] == '1');
     g->rr_pre.assign(num_levels, 0);
@@ -427,6 +443,12 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     for (double** v : vs) {
       if ((e = hipMalloc(v, bytes)) != hipSuccess) break;
       (void)hipMemset(*v, 0, bytes);
+    }
+    if (g->m32) {
+      for (float** v : { &g->mp32, &g->ms32 }) {
+        if (e == hipSuccess) e = hipMalloc(v, sizeof(float) * (size_t)(n > 0 ? n : 1));
+        if (e == hipSuccess) e = hipMemset(*v, 0, sizeof(float) * (size_t)(n > 0 ? n : 1));
+      }
     }
     g->nblk = krylov_blocks(d0);
     if (e == hipSuccess) e = hipMalloc(&g->scal, sizeof(double) * 16);
@@ -495,8 +517,11 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     TE_HIP(hipStreamSynchronize(s));
     return TE_OK;
   };
-  double* mp = use_precond ? g->mp : g->p;
-  double* ms = use_precond ? g->ms : g->sv;
+  // M p and M s: FP64, or FP32 on the fused mixed-precision path (the same values)
+  const bool m32 = use_precond && g->m32;
+  const int dtm = m32 ? TE_F32 : TE_F64;
+  void* mp = use_precond ? (m32 ? (void*)g->mp32 : (void*)g->mp) : (void*)g->p;
+  void* ms = use_precond ? (m32 ? (void*)g->ms32 : (void*)g->ms) : (void*)g->sv;
   // One iteration, enqueued as a whole: the host reads the scalars once per iteration,
   // from the copy FIN_RHO publishes (so the next iteration is enqueued while the p update
   // still runs).
@@ -507,20 +532,22 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // The preconditioner and operator launches of the second half run even after the test
   // fired (their results are then unused): that happens at most once per solve.
   auto iteration = [&]() -> int {
-    if (use_precond) TE_RC(vcycle_enqueue(g, g->p, g->mp));
+    if (use_precond) TE_RC(vcycle_enqueue(g, g->p, mp, dtm));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
-    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
+    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, dtm, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
     TE_RC(launch_bicg_finalize(FIN_ALPHA, g->pa, nullptr, nb, g->scal, s));
     TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pa, nullptr,
                       s));
     TE_RC(launch_bicg_finalize(FIN_SNORM, g->pa, nullptr, nb, g->scal, s));
-    const double* dirp = use_precond == 2 ? mp : g->p;  // reference: alpha p (BiCGStab.h:162-167)
-    TE_RC(launch_bicg(BICG_XP, d, g->scal, x, nullptr, dirp, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, s));
-    if (use_precond) TE_RC(vcycle_enqueue(g, g->sv, g->ms));
+    // reference: alpha p (BiCGStab.h:162-167); use_precond = 2: alpha M p
+    const bool xp_m = use_precond == 2;
+    TE_RC(launch_bicg(BICG_XP, d, g->scal, x, nullptr, xp_m ? mp : (const void*)g->p, nullptr, nullptr, nullptr,
+                      nullptr, nullptr, nullptr, s, xp_m ? dtm : TE_F64));
+    if (use_precond) TE_RC(vcycle_enqueue(g, g->sv, ms, dtm));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, ms, 1, s));
-    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, g->as, g->sv, g->pa, g->pb, g->fly, s));
+    TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, dtm, g->as, g->sv, g->pa, g->pb, g->fly, s));
     TE_RC(launch_bicg_finalize(FIN_OMEGA, g->pa, g->pb, nb, g->scal, s));
-    TE_RC(launch_bicg(BICG_XR, d, g->scal, x, g->resid, mp, ms, g->ap, g->as, g->rhat, g->pa, g->pb, s));
+    TE_RC(launch_bicg(BICG_XR, d, g->scal, x, g->resid, mp, ms, g->ap, g->as, g->rhat, g->pa, g->pb, s, dtm));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
     return launch_bicg(BICG_P, d, g->scal, g->p, nullptr, g->ap, g->resid, nullptr, nullptr, nullptr, nullptr,
                        nullptr, s);

@@ -129,10 +129,19 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
   }
 }
 
-// out = A u (ghost cells of out written 0); pa[block] = sum w * out, pb[block] = sum out * out
-template <int OP, int KIND, bool SQ, bool P16>
+// interior value of patch q at (x, y), widened to T
+template <typename T, typename TU>
+struct WideAcc {
+  const TU* u;
+  Layout L;
+  __device__ __forceinline__ T operator()(int q, int x, int y) const { return T(u[q * L.ps + (y + 1) * L.sx + x + 1]); }
+};
+
+// out = A u (ghost cells of out written 0); pa[block] = sum w * out, pb[block] = sum out * out.
+// TU: type of u (FP32 values are widened to FP64 as they are read: the arithmetic is FP64).
+template <int OP, int KIND, bool SQ, bool P16, typename TU>
 __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __restrict__ dp,
-                                                           const double* __restrict__ u,
+                                                           const TU* __restrict__ u,
                                                            const double* __restrict__ cf, double* __restrict__ out,
                                                            const double* __restrict__ wv, double* __restrict__ pa,
                                                            double* __restrict__ pb, Layout L, int neumann,
@@ -142,7 +151,7 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
   const int n0 = L.n[0], n1 = L.n[1], w = n0 + 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double* tile = reinterpret_cast<double*>(smem_raw) + wave * w * (n1 + 2);
-  PlainAcc<double> U{ u, L };
+  WideAcc<double, TU> U{ u, L };
   const double phys = (OP == 0 && neumann) ? 1.0 : -1.0;
   double sa = 0.0, sq = 0.0;
   const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
@@ -159,7 +168,7 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
     for (int k = 0; k < (P16 ? 4 : 1); k++) {
       const int q = lane + 64 * k;
       const int c = p * L.ps + ((q >> 4) + 1) * L.sx + (q & 15) + 1;
-      un[k] = u[c];
+      un[k] = double(u[c]);
       wn[k] = wv[c];
     }
   };
@@ -303,8 +312,8 @@ bool star_dot_fly_supported(const te_ghost_filler* g)
          (g->kind == TE_BILINEAR || g->kind == TE_BIQUADRATIC);
 }
 
-int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
-                       double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s)
+int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const void* u,
+                       int udt, double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s)
 {
   const te_domain* d = g->d;
   if (!star_dot_fly_supported(g)) {
@@ -314,9 +323,16 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
   Layout L = make_layout(2, d->ns, 1, d->np);
   size_t shmem = tile_bytes(d, sizeof(double));
   const bool p16 = d->ns[0] == 16 && d->ns[1] == 16;
-#define TE_SD(OP_, K_, Q_, P_)                                                                                    \
-  hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches, u, \
-                     coeffs, out, w, pa, pb, L, neumann, (const int4*)d->d_ring)
+#define TE_SD(OP_, K_, Q_, P_)                                                                                      \
+  do {                                                                                                             \
+    if (udt == TE_F32)                                                                                             \
+      hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_, float>), dim3(grid), dim3(64 * kWaves), shmem, s,           \
+                         d->d_patches, (const float*)u, coeffs, out, w, pa, pb, L, neumann, (const int4*)d->d_ring); \
+    else                                                                                                           \
+      hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_, double>), dim3(grid), dim3(64 * kWaves), shmem, s,          \
+                         d->d_patches, (const double*)u, coeffs, out, w, pa, pb, L, neumann,                       \
+                         (const int4*)d->d_ring);                                                                  \
+  } while (0)
 #define TE_SDQ(OP_, K_)                      \
   do {                                       \
     if (pb && p16) TE_SD(OP_, K_, true, true);   \

@@ -327,9 +327,9 @@ __device__ __forceinline__ int ring_index(int p, int g, int n0, int n1, int* s, 
   return (p * 4 + *s) * 2 + (2 * *t >= nt ? 1 : 0);
 }
 
-// The neighbour values ring cell g of a patch needs (record r), read from u.
-template <typename T, int KIND>
-__device__ __forceinline__ void ring_gather_plain(const int4& r, int g, int n0, int n1, const T* __restrict__ u,
+// The neighbour values ring cell g of a patch needs (record r), read from u (widened to T).
+template <typename T, int KIND, typename TU>
+__device__ __forceinline__ void ring_gather_plain(const int4& r, int g, int n0, int n1, const TU* __restrict__ u,
                                                   const Layout& L, T (&v)[4])
 {
   int s, t;
@@ -338,7 +338,7 @@ __device__ __forceinline__ void ring_gather_plain(const int4& r, int g, int n0, 
 #pragma unroll
   for (int k = 0; k < 4; k++) v[k] = T(0);
   ghost2d_gather<KIND>(r.x, r.y, r.z, s, t, na, nt,
-                       [&](int k, int q, int x, int y) { v[k] = u[q * L.ps + (y + 1) * L.sx + x + 1]; });
+                       [&](int k, int q, int x, int y) { v[k] = T(u[q * L.ps + (y + 1) * L.sx + x + 1]); });
 }
 
 // The ghost of ring cell g (record r, gathered neighbour values v) written into the tile

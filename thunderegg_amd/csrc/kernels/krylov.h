@@ -18,14 +18,16 @@ enum BicgFinalize { FIN_INIT = 0, FIN_ALPHA, FIN_SNORM, FIN_OMEGA, FIN_RHO };
 
 int krylov_blocks(const te_domain* d);
 bool star_dot_fly_supported(const te_ghost_filler* g);
-int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
+// udt: type of u (FP32 only with fly: the values are widened to FP64 as they are read)
+int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const void* u, int udt,
                     double* out, const double* w, double* pa, double* pb, bool fly, hipStream_t s);
 // 2D, ghosts computed in the launch; writes `grid` partials (fused2d.hip)
-int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
-                       double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s);
-int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const double* a0,
-                const double* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
-                hipStream_t s);
+int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const void* u,
+                       int udt, double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s);
+// dtm: type of a0 and a1 for BICG_XR (M p, M s) and of a0 for BICG_XP; FP64 for the other modes
+int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const void* a0,
+                const void* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
+                hipStream_t s, int dtm = TE_F64);
 // tol: the relative tolerance, stored by FIN_INIT for the s-norm test of FIN_SNORM.
 // mirror (FIN_RHO): device pointer to host-coherent memory of SC_COUNT doubles; FIN_RHO
 // copies the scalars there and then bumps SC_SEQ, so the host can read an iteration's

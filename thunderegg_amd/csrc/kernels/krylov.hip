@@ -8,6 +8,8 @@
 // The vector kernels are patch-blocked: a workgroup walks whole patches, and
 // each thread precomputes the padded offsets of the cells it owns in a patch,
 // so the per-element work is a load, the arithmetic and a store.
+#include <type_traits>
+
 #include "../te_internal.h"
 #include "blas.h"
 #include "krylov.h"
@@ -123,11 +125,12 @@ __global__ void __launch_bounds__(kThreads) star_dot_kernel(const DevPatch* __re
 
 // Vector updates of one BiCGStab iteration (interior cells only).  After the s-norm test
 // has fired (state ST_EARLY) only BICG_XP runs, and BICG_XP runs only then.
-template <int CPT, int MODE>
+// TM: type of M p and M s (a0, a1 of BICG_XR; a0 of BICG_XP), widened to FP64 as read.
+template <int CPT, int MODE, typename TM>
 __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* __restrict__ scal,
                                                         double* __restrict__ v0, double* __restrict__ v1,
-                                                        const double* __restrict__ a0,
-                                                        const double* __restrict__ a1,
+                                                        const void* __restrict__ a0v,
+                                                        const void* __restrict__ a1v,
                                                         const double* __restrict__ a2,
                                                         const double* __restrict__ a3,
                                                         const double* __restrict__ a4, double* __restrict__ pa,
@@ -137,6 +140,9 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
     const double st = scal[SC_STATE];
     if (MODE == BICG_XP ? st != ST_EARLY : st != ST_RUN) return;
   }
+  typedef typename std::conditional<MODE == BICG_XR || MODE == BICG_XP, TM, double>::type TA;
+  const TA* __restrict__ a0 = (const TA*)a0v;
+  const TA* __restrict__ a1 = (const TA*)a1v;
   CellSet<CPT> cs;
   make_cells<CPT>(L, false, cs);
   double sa = 0.0, sb = 0.0;
@@ -162,10 +168,10 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
         sa += s * s;
       } else if (MODE == BICG_XP) {
         // v0 = x, a0 = p: x += p * alpha (early exit, BiCGStab.h:162-167)
-        v0[i] += a0[i] * alpha;
+        v0[i] += double(a0[i]) * alpha;
       } else if (MODE == BICG_XR) {
         // v0 = x, v1 = r, a0 = mp, a1 = ms, a2 = ap, a3 = as, a4 = rhat
-        v0[i] += a0[i] * alpha + a1[i] * omega;
+        v0[i] += double(a0[i]) * alpha + double(a1[i]) * omega;
         double r = v1[i];
         r += a2[i] * -alpha + a3[i] * -omega;
         v1[i] = r;
@@ -265,7 +271,7 @@ int krylov_blocks(const te_domain* d)
 
 // out = A u with the dot w.out (w required) and optionally out.out (pb != null).
 // fly: compute neighbour ghosts on the fly (2D, launch_star_dot_2d) instead of reading them from u.
-int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const double* u,
+int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const void* u, int udt,
                     double* out, const double* w, double* pa, double* pb, bool fly, hipStream_t s)
 {
   const te_domain* d = g->d;
@@ -277,11 +283,15 @@ int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const do
   }
   int grid = krylov_blocks(d);
   // the partial sums cover krylov_blocks(d) workgroups whichever kernel runs
-  if (fly) return launch_star_dot_2d(g, op_kind, neumann, coeffs, u, out, w, pa, pb, grid, s);
+  if (fly) return launch_star_dot_2d(g, op_kind, neumann, coeffs, u, udt, out, w, pa, pb, grid, s);
+  if (udt != TE_F64) {
+    set_error("internal: the Krylov apply reads FP32 input only with ghosts computed in the launch");
+    return TE_ERR_UNSUPPORTED;
+  }
   bool sq = pb != nullptr;
 #define TE_SDC(C_, D_, OP_, Q_)                                                                               \
   hipLaunchKernelGGL((star_dot_kernel<C_, D_, OP_, true, Q_>), dim3(grid), dim3(kThreads), 0, s, d->d_patches, \
-                     u, coeffs, out, w, pa, pb, L, neumann)
+                     (const double*)u, coeffs, out, w, pa, pb, L, neumann)
 #define TE_SD(D_, OP_, Q_)                      \
   do {                                          \
     switch (cpt) {                              \
@@ -311,10 +321,14 @@ int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const do
   return TE_OK;
 }
 
-int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const double* a0,
-                const double* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
-                hipStream_t s)
+int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const void* a0,
+                const void* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
+                hipStream_t s, int dtm)
 {
+  if (dtm != TE_F64 && mode != BICG_XR && mode != BICG_XP) {
+    set_error("internal: FP32 operands only for the x updates");
+    return TE_ERR_ARG;
+  }
   Layout L = make_layout(d->dim, d->ns, 1, d->np);
   int cpt = pick_cpt(d->ns[0] * d->ns[1] * (d->dim == 3 ? d->ns[2] : 1));
   if (cpt < 0) {
@@ -322,9 +336,15 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
     return TE_ERR_UNSUPPORTED;
   }
   int grid = krylov_blocks(d);
-#define TE_BKC(C_, M_)                                                                                        \
-  hipLaunchKernelGGL((bicg_kernel<C_, M_>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0, a1, a2, a3, \
-                     a4, pa, pb)
+#define TE_BKC(C_, M_)                                                                                           \
+  do {                                                                                                           \
+    if (dtm == TE_F32)                                                                                           \
+      hipLaunchKernelGGL((bicg_kernel<C_, M_, float>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0, a1, \
+                         a2, a3, a4, pa, pb);                                                                    \
+    else                                                                                                         \
+      hipLaunchKernelGGL((bicg_kernel<C_, M_, double>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0,   \
+                         a1, a2, a3, a4, pa, pb);                                                                \
+  } while (0)
 #define TE_BK(M_)                      \
   do {                                 \
     switch (cpt) {                     \
