@@ -55,9 +55,10 @@ def time_launches(R, fn, n, stream):
 def kernel_rooflines(R, P, stream, reps=50):
     """Average launch time (HIP events on the launch stream) of the finest-level kernels
     of the mixed-precision V-cycle, launched standalone through the C ABI with the same
-    dtype combination as inside the solve, and their algorithmic bytes per launch (every
-    input read once, every output written once; ghost values computed from neighbours
-    are cache traffic, not algorithmic)."""
+    dtype combination as inside the solve (the BiCGStab hands the V-cycle FP32 copies of p
+    and s and keeps M p, M s in FP32), and their algorithmic bytes per launch (every input
+    read once, every output written once; ghost values computed from neighbours are cache
+    traffic, not algorithmic)."""
     import numpy as np
 
     dim, ns = P.dim, P.ns
@@ -70,9 +71,10 @@ def kernel_rooflines(R, P, stream, reps=50):
     g = R.GhostFiller(d0, P.filler, R.FACES)
     link = R.LevelLink(d0, d1)
     f64 = R.DeviceArray.from_host(rng.standard_normal(d0.shape(1)))
+    f32 = R.DeviceArray.from_host(rng.standard_normal(d0.shape(1)), np.float32)
     u32 = R.DeviceArray.from_host(rng.standard_normal(d0.shape(1)), np.float32)
     o32 = R.DeviceArray(d0.shape(1), np.float32)
-    o64 = R.DeviceArray(d0.shape(1), np.float64)
+    m32 = R.DeviceArray(d0.shape(1), np.float32)
     uc32 = R.DeviceArray.from_host(rng.standard_normal(d1.shape(1)), np.float32)
     fc32 = R.DeviceArray(d1.shape(1), np.float32)
 
@@ -85,14 +87,14 @@ def kernel_rooflines(R, P, stream, reps=50):
     if dim == 2 and P.smoother == 0:
         plan = R.DftPlan(g, op_kind=P.op_kind)
         fl = n0 * (8 * ns[0] ** 3 + N)  # four n^3 products of 2 n^3 flops, n^2 divides
-        # pre-smoothing from u = 0: f (f64) read, u (f32) written
-        rec("dft_apply_f32_in64", lambda: plan.apply_mixed(R.F32, f64, o32, stream=stream.h), n0 * N * (8 + 4), fl)
-        # Cycle::restrict: u (f32), f (f64) read; coarse f (f32) written
-        rec("residual_restrict_f32", lambda: link.residual_restrict(g, f64, u32, fc32, stream=stream.h),
-            n0 * N * (4 + 8) + n1 * N * 4)
-        # post-smoothing with the interpolation: f (f64), u (f32), coarse u (f32) read; u (f64) written
-        rec("dft_smooth_interp_f32_out64",
-            lambda: plan.smooth_interp(link, f64, u32, uc32, o64, stream=stream.h), n0 * N * (8 + 4 + 8) + n1 * N * 4,
+        # pre-smoothing from u = 0: f read, u written (FP32)
+        rec("dft_apply_f32", lambda: plan.apply_mixed(R.F32, f32, o32, stream=stream.h), n0 * N * (4 + 4), fl)
+        # Cycle::restrict: u, f read; coarse f written (FP32)
+        rec("residual_restrict_f32", lambda: link.residual_restrict(g, f32, u32, fc32, stream=stream.h),
+            n0 * N * (4 + 4) + n1 * N * 4)
+        # post-smoothing with the interpolation: f, u, coarse u read; M f written (FP32)
+        rec("dft_smooth_interp_f32",
+            lambda: plan.smooth_interp(link, f32, u32, uc32, m32, stream=stream.h), n0 * N * (4 + 4 + 4) + n1 * N * 4,
             fl)
     # copy of a vector of known size (calibrates the HBM byte counters)
     c64 = R.DeviceArray(d0.shape(1), np.float64)

@@ -57,7 +57,7 @@ struct te_gmg {
   hipStream_t s = nullptr;
   double *resid = nullptr, *rhat = nullptr, *p = nullptr, *ap = nullptr, *as = nullptr, *sv = nullptr,
          *ms = nullptr, *mp = nullptr;
-  double *scal = nullptr, *pa = nullptr, *pb = nullptr;
+  double *scal = nullptr, *pa = nullptr, *pb = nullptr, *pc = nullptr;  // pc: the s.s partials
   double* host_scal = nullptr;  // pinned
   // host-coherent copy of the scalars written by the device at the end of each BiCGStab
   // iteration (host and device addresses)
@@ -74,9 +74,12 @@ struct te_gmg {
   bool fly = false;    // outer operator computes ghosts on the fly
   // mixed precision on the fused path: M p and M s are FP32 values (the finest level's last
   // sweep computes in FP32), so the BiCGStab keeps them in FP32 buffers
+  // and hands it FP32 copies of p and s as right-hand sides (written by the p and s updates)
   bool m32 = false;
   float* mp32 = nullptr;
   float* ms32 = nullptr;
+  float* p32 = nullptr;
+  float* s32 = nullptr;
 };
 
 namespace {
@@ -232,14 +235,15 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
   return TE_OK;
 }
 
-// Cycle::apply (GMG/Cycle.h:94-98) enqueued on g->s.  udt: type of u (FP32 only on the
-// fused mixed-precision path, see visit_fused).
-int vcycle_enqueue(te_gmg* g, const double* f, void* u, int udt = TE_F64)
+// Cycle::apply (GMG/Cycle.h:94-98) enqueued on g->s.  dtf / udt: types of f and u (FP32
+// only on the fused mixed-precision path, where the V-cycle rounds f to FP32 as it reads it
+// and its last sweep computes u in FP32; see visit_fused).
+int vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt = TE_F64, int dtf = TE_F64)
 {
   GLevel& L0 = g->L[0];
-  if (g->fused) return visit_fused(g, 0, f, TE_F64, u, udt, false);
-  if (udt != TE_F64) {
-    set_error("internal: an FP32 V-cycle result needs the fused path");
+  if (g->fused) return visit_fused(g, 0, f, dtf, u, udt, false);
+  if (udt != TE_F64 || dtf != TE_F64) {
+    set_error("internal: FP32 V-cycle input or output needs the fused path");
     return TE_ERR_UNSUPPORTED;
   }
   double* ud = (double*)u;
@@ -308,10 +312,12 @@ void free_gmg(te_gmg* g)
     (void)hipFree(Lv.coeffs32);
     (void)hipFree(Lv.coeffs64_own);
   }
-  double* vs[] = { g->resid, g->rhat, g->p, g->ap, g->as, g->sv, g->ms, g->mp, g->scal, g->pa, g->pb };
+  double* vs[] = { g->resid, g->rhat, g->p, g->ap, g->as, g->sv, g->ms, g->mp, g->scal, g->pa, g->pb, g->pc };
   for (double* v : vs) (void)hipFree(v);
   (void)hipFree(g->mp32);
   (void)hipFree(g->ms32);
+  (void)hipFree(g->p32);
+  (void)hipFree(g->s32);
   (void)hipFree(g->d_flags);
   if (g->host_scal) (void)hipHostFree(g->host_scal);
   if (g->mirror_h) (void)hipHostFree(g->mirror_h);
@@ -445,7 +451,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
       (void)hipMemset(*v, 0, bytes);
     }
     if (g->m32) {
-      for (float** v : { &g->mp32, &g->ms32 }) {
+      for (float** v : { &g->mp32, &g->ms32, &g->p32, &g->s32 }) {
         if (e == hipSuccess) e = hipMalloc(v, sizeof(float) * (size_t)(n > 0 ? n : 1));
         if (e == hipSuccess) e = hipMemset(*v, 0, sizeof(float) * (size_t)(n > 0 ? n : 1));
       }
@@ -454,6 +460,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     if (e == hipSuccess) e = hipMalloc(&g->scal, sizeof(double) * 16);
     if (e == hipSuccess) e = hipMalloc(&g->pa, sizeof(double) * (g->nblk + 1));
     if (e == hipSuccess) e = hipMalloc(&g->pb, sizeof(double) * (g->nblk + 1));
+    if (e == hipSuccess) e = hipMalloc(&g->pc, sizeof(double) * (g->nblk + 1));
     if (e == hipSuccess) e = hipMalloc(&g->d_flags, sizeof(int32_t) * 4);
     if (e == hipSuccess) e = hipMemset(g->d_flags, 0, sizeof(int32_t) * 4);
     if (e == hipSuccess) e = hipMemset(g->scal, 0, sizeof(double) * 16);
@@ -525,32 +532,34 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // One iteration, enqueued as a whole: the host reads the scalars once per iteration,
   // from the copy FIN_RHO publishes (so the next iteration is enqueued while the p update
   // still runs).
-  //   mp = M p; ap = A mp (+ rhat.ap); alpha; s = r - alpha ap; ||s|| and the s-norm test;
-  //   if it fired: x += alpha p (mp with use_precond = 2), and the updates after it are
-  //   skipped on the device (the kernels read the state word); else
-  //   ms = M s; as = A ms (+ as.s, as.as); omega; x, r updates; rho, beta, ||r||; p update.
-  // The preconditioner and operator launches of the second half run even after the test
-  // fired (their results are then unused): that happens at most once per solve.
+  //   mp = M p; ap = A mp (+ rhat.ap); alpha; s = r - alpha ap (+ s.s);
+  //   ms = M s; as = A ms (+ as.s, as.as); ||s|| and the s-norm test, then omega; if the
+  //   test fired: x += alpha p (mp with use_precond = 2), and the updates after it are
+  //   skipped on the device (the kernels read the state word); else x, r updates; rho,
+  //   beta, ||r||; p update.
+  // The preconditioner and operator launches for s run before the test (the reference runs
+  // them after it): when it fires their results are unused; that happens at most once per
+  // solve.
   auto iteration = [&]() -> int {
-    if (use_precond) TE_RC(vcycle_enqueue(g, g->p, mp, dtm));
+    if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->p32 : g->p, mp, dtm, dtm));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, dtm, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
     TE_RC(launch_bicg_finalize(FIN_ALPHA, g->pa, nullptr, nb, g->scal, s));
-    TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pa, nullptr,
-                      s));
-    TE_RC(launch_bicg_finalize(FIN_SNORM, g->pa, nullptr, nb, g->scal, s));
-    // reference: alpha p (BiCGStab.h:162-167); use_precond = 2: alpha M p
-    const bool xp_m = use_precond == 2;
-    TE_RC(launch_bicg(BICG_XP, d, g->scal, x, nullptr, xp_m ? mp : (const void*)g->p, nullptr, nullptr, nullptr,
-                      nullptr, nullptr, nullptr, s, xp_m ? dtm : TE_F64));
-    if (use_precond) TE_RC(vcycle_enqueue(g, g->sv, ms, dtm));
+    TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pc, nullptr,
+                      s, TE_F64, m32 ? g->s32 : nullptr));
+    if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->s32 : g->sv, ms, dtm, dtm));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, ms, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, dtm, g->as, g->sv, g->pa, g->pb, g->fly, s));
-    TE_RC(launch_bicg_finalize(FIN_OMEGA, g->pa, g->pb, nb, g->scal, s));
-    TE_RC(launch_bicg(BICG_XR, d, g->scal, x, g->resid, mp, ms, g->ap, g->as, g->rhat, g->pa, g->pb, s, dtm));
+    // ||s|| and the s-norm test (its partials from the s update), then omega
+    TE_RC(launch_bicg_finalize(FIN_SOMEGA, g->pa, g->pb, nb, g->scal, s, 0.0, nullptr, g->pc));
+    // x and r updates; after the test fired, x += alpha p instead (BiCGStab.h:162-167;
+    // use_precond = 2: alpha M p)
+    const bool xp_m = use_precond == 2;
+    TE_RC(launch_bicg(BICG_XR, d, g->scal, x, g->resid, mp, ms, g->ap, g->as, g->rhat, g->pa, g->pb, s, dtm, nullptr,
+                      xp_m ? mp : (const void*)g->p, xp_m ? 1 : 0));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
     return launch_bicg(BICG_P, d, g->scal, g->p, nullptr, g->ap, g->resid, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, s);
+                       nullptr, s, TE_F64, m32 ? g->p32 : nullptr);
   };
   hipGraphExec_t gx = nullptr;
   if (g->o.use_graph) {
@@ -583,7 +592,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // resid = b - A x ; r0_norm = ||b|| ; rhat = p = resid ; rho = rhat.resid
   TE_RC(launch_ghost_fill(gf, TE_F64, x, 1, s));
   TE_RC(launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, b, x, g->resid, 1, s));
-  TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s));
+  TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s,
+                    TE_F64, m32 ? g->p32 : nullptr));
   TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s, tol));
   TE_RC(read_scal());
   double seq = h[SC_SEQ];

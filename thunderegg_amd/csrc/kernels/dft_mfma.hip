@@ -23,172 +23,13 @@
 //             (the result must then go to a buffer other than u).
 #include "../te_internal.h"
 #include "dft.h"
+#include "dft_wave.h"
 #include "ghost2d.h"
 
 namespace te {
 namespace {
 
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef double doublex4 __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mf;
-template <>
-struct Mf<float> {
-  typedef floatx4 V;
-  static __device__ __forceinline__ V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-  // row of accumulator register r in lane group g
-  static __device__ __forceinline__ int row(int r, int g) { return 4 * g + r; }
-};
-template <>
-struct Mf<double> {
-  typedef doublex4 V;
-  static __device__ __forceinline__ V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-  static __device__ __forceinline__ int row(int r, int g) { return 4 * r + g; }
-};
-
 constexpr int kWaves = 4;
-
-// u (+ injection of the coarser level's uc) at interior cell (x, y) of patch q
-template <typename T, typename TC, bool INTERP>
-struct UAcc {
-  const T* u;
-  const TC* uc;
-  const int32_t* parent;
-  const int32_t* porth;
-  Layout L;
-  __device__ __forceinline__ T operator()(int q, int x, int y) const
-  {
-    T v = u[q * L.ps + (y + 1) * L.sx + x + 1];
-    if (INTERP) {
-      int o = porth[q];
-      int cx = x, cy = y;
-      if (o >= 0) {
-        cx = (x + ((o & 1) ? L.n[0] : 0)) / 2;
-        cy = (y + ((o & 2) ? L.n[1] : 0)) / 2;
-      }
-      v = T(v + T(uc[parent[q] * L.ps + (cy + 1) * L.sx + cx + 1]));
-    }
-    return v;
-  }
-};
-
-template <typename T, typename TC>
-struct FusedArgs {
-  const TC* uc;
-  const int32_t* parent;
-  const int32_t* porth;
-};
-
-// The patch solve of one wavefront: X (LDS, n x ld, the modified right-hand side) ->
-// out[base + padded (x, y)] (interior only), lx = 4 / hx^2, ly = 4 / hy^2.  X is
-// overwritten (the wave's own LDS; no workgroup barrier).  active = false: the wave
-// skips the arithmetic.
-template <typename T, typename TO, int NT>
-__device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, const T* __restrict__ S3,
-                                               const T* __restrict__ s2, T lx, T ly, TO* __restrict__ out,
-                                               int base, int sx, bool active)
-{
-  constexpr int n = 16 * NT;
-  constexpr int ld = n + 1;
-  typedef typename Mf<T>::V V;
-  const int lane = threadIdx.x & 63;
-  const int c = lane & 15;
-  const int g = lane >> 4;
-  if (active) {
-    // ---- Y = F S^T : Y[i][j] = sum_k F[i][k] S[j][k]
-    V Y[NT][NT];
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++) {
-        V acc = { 0, 0, 0, 0 };
-#pragma unroll
-        for (int K = 0; K < NT; K++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            int k = 16 * K + 4 * s + g;
-            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], Sm[(16 * J + c) * ld + k], acc);
-          }
-        Y[I][J] = acc;
-      }
-    // ---- Z = S Y : the B operand comes from Y's registers (K index = row of register s)
-    V Z[NT][NT];
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++) {
-        V acc = { 0, 0, 0, 0 };
-#pragma unroll
-        for (int K = 0; K < NT; K++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            int k = 16 * K + Mf<T>::row(s, g);
-            acc = Mf<T>::mma(Sm[(16 * I + c) * ld + k], Y[K][J][s], acc);
-          }
-        Z[I][J] = acc;
-      }
-    // ---- divide by the eigenvalues -4/hx^2 s2[kx] - 4/hy^2 s2[ky] (getEigenValues, :320-361)
-    {
-#pragma unroll
-      for (int I = 0; I < NT; I++)
-#pragma unroll
-        for (int J = 0; J < NT; J++)
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            int ky = 16 * I + Mf<T>::row(r, g), kx = 16 * J + c;
-            T eig = T(0);
-            eig -= lx * s2[kx];
-            eig -= ly * s2[ky];
-            Z[I][J][r] = Z[I][J][r] / eig;
-          }
-    }
-    // ---- V = S3 Z'
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++) {
-        V acc = { 0, 0, 0, 0 };
-#pragma unroll
-        for (int K = 0; K < NT; K++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            int k = 16 * K + Mf<T>::row(s, g);
-            acc = Mf<T>::mma(S3[(16 * I + c) * ld + k], Z[K][J][s], acc);
-          }
-        Y[I][J] = acc;  // reuse Y for V
-      }
-    // V -> LDS (accumulator layout), then U = V S3^T with V in the A layout
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) X[(16 * I + Mf<T>::row(r, g)) * ld + 16 * J + c] = Y[I][J][r];
-  }
-  wave_sync();  // V in LDS, read back in the A-operand layout
-  if (active) {
-    const T scale = T(4.0 / double(n * n));  // prod over axes of 2/n (:442-449)
-#pragma unroll
-    for (int I = 0; I < NT; I++)
-#pragma unroll
-      for (int J = 0; J < NT; J++) {
-        V acc = { 0, 0, 0, 0 };
-#pragma unroll
-        for (int K = 0; K < NT; K++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            int k = 16 * K + 4 * s + g;
-            acc = Mf<T>::mma(X[(16 * I + c) * ld + k], S3[(16 * J + c) * ld + k], acc);
-          }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int y = 16 * I + Mf<T>::row(r, g), x = 16 * J + c;
-          out[base + (y + 1) * sx + x + 1] = TO(acc[r] * scale);
-        }
-      }
-  }
-}
 
 template <typename T, typename TF, typename TO, typename TC, int NT, int OP, bool UZERO, int GM, int KIND>
 __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM == 0 ? 5 : 4) : 1) dft_mfma_kernel(const DevPatch* __restrict__ dp,
@@ -504,7 +345,8 @@ __global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __r
                                                              const DevPatch* __restrict__ dpc,
                                                              const TC* __restrict__ mats,
                                                              const TC* __restrict__ s2tab, TC* __restrict__ fc,
-                                                             TC* __restrict__ uc, Layout Lc)
+                                                             TC* __restrict__ uc, Layout Lc,
+                                                             const int4* __restrict__ ring)
 {
   constexpr int n = 16;
   constexpr int ld = n + 1;
@@ -531,27 +373,38 @@ __global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __r
     // wave j handles child j of P (P itself when it was not coarsened: one child)
     const bool act = wave < nchild[P];
     const int c = act ? child[P * 8 + wave] : 0;
-    SideInfo si[1];
+    const int Xb = lane % h, Yb = lane / h;  // a lane per 2x2 block (h * h = 64)
     int o = -1;
+    int4 rr = make_int4(-1, 0, -1, 0);  // ring record of the lane's ghost cell (ring cell = lane)
+    T gv[4];                            // the neighbour values of that ghost
+    TF fv[4];                           // f on the lane's 2x2 block
     if (act) {
-      tile_interior_2d<T>(c, n, n, U, tile, lane);
-      ring_prefetch<1>(dpf[c], n, n, lane, si);
+      // every global read of the child is issued before the first barrier: its ring
+      // record, own values, f and orth, then the neighbour values the record names
+      int rs, rt;
+      rr = ring[ring_index(c, lane, n, n, &rs, &rt)];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int x = 2 * Xb + (q & 1), y = 2 * Yb + (q >> 1);
+        fv[q] = f[c * Lf.ps + (y + 1) * Lf.sx + x + 1];
+      }
       o = porth[c];
+      tile_interior_2d<T>(c, n, n, U, tile, lane);
+      ring_gather_plain<T, KIND>(rr, lane, n, n, u, Lf, gv);
     }
     __syncthreads();
-    if (act) tile_ghosts_2d<T, KIND>(dpf[c], si[0], c, n, n, U, tile, phys, lane);
+    if (act) tile_ghost_gathered<T, KIND>(rr, lane, n, n, tile, phys, gv);
     __syncthreads();
     if (act) {
       const DevPatch& Pf = dpf[c];
       const T ih2[2] = { T(Pf.inv_h2[0]), T(Pf.inv_h2[1]) };
       const int ox = (o & 1) ? h : 0, oy = (o & 2) ? h : 0;
-      const int Xb = lane % h, Yb = lane / h;  // a lane per 2x2 block (h * h = 64)
       T r[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int x = 2 * Xb + (q & 1), y = 2 * Yb + (q >> 1);
         const int ci = (y + 1) * Lf.sx + x + 1;
-        r[q] = T(f[c * Lf.ps + ci]) - star_tile<T, OP>(tile, w, x, y, ih2, cf, c * Lf.cs + ci, Lf.sx);
+        r[q] = T(fv[q]) - star_tile<T, OP>(tile, w, x, y, ih2, cf, c * Lf.cs + ci, Lf.sx);
       }
       if (o < 0) {  // copyToParent
 #pragma unroll
@@ -595,7 +448,7 @@ void launch_rp(const te_ghost_filler* g, const te_level_link* link, int op_kind,
 #define TE_RP(OP_, K_)                                                                                          \
   hipLaunchKernelGGL((rr_pre_kernel<T, TF, TC, OP_, K_>), dim3(grid), dim3(64 * kWaves), shmem, s, df->d_patches, \
                      (const T*)u, (const T*)coeffs, (const TF*)f, Lf, neumann, link->d_child, link->d_nchild,      \
-                     link->d_orth, dc->d_patches, mats, s2, (TC*)fc, (TC*)uc, Lc)
+                     link->d_orth, dc->d_patches, mats, s2, (TC*)fc, (TC*)uc, Lc, (const int4*)df->d_ring)
   if (op_kind == 0) {
     if (g->kind == TE_BILINEAR) TE_RP(0, TE_BILINEAR);
     else TE_RP(0, TE_BIQUADRATIC);

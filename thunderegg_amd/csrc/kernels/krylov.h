@@ -14,7 +14,8 @@ enum BicgScalar {
 // (ST_EARLY); from then on only the early-exit update x += alpha p runs.
 enum BicgState { ST_RUN = 0, ST_EARLY = 1 };
 enum BicgMode { BICG_INIT = 0, BICG_S, BICG_XP, BICG_XR, BICG_P };
-enum BicgFinalize { FIN_INIT = 0, FIN_ALPHA, FIN_SNORM, FIN_OMEGA, FIN_RHO };
+// FIN_SOMEGA = FIN_SNORM (from the s.s partials pc) followed by FIN_OMEGA unless the test fired
+enum BicgFinalize { FIN_INIT = 0, FIN_ALPHA, FIN_SNORM, FIN_OMEGA, FIN_RHO, FIN_SOMEGA };
 
 int krylov_blocks(const te_domain* d);
 bool star_dot_fly_supported(const te_ghost_filler* g);
@@ -24,16 +25,20 @@ int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const do
 // 2D, ghosts computed in the launch; writes `grid` partials (fused2d.hip)
 int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const void* u,
                        int udt, double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s);
-// dtm: type of a0 and a1 for BICG_XR (M p, M s) and of a0 for BICG_XP; FP64 for the other modes
+// dtm: type of a0 and a1 for BICG_XR (M p, M s) and of a0 for BICG_XP; FP64 for the other modes.
+// w32: FP32 copy of the new p (BICG_INIT, BICG_P) or s (BICG_S), or null.
+// xp_dir (BICG_XR): after the s-norm test fired, the launch does x += alpha xp_dir instead
+// (xp_dir of type dtm when xp_dir_m, else FP64).
 int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const void* a0,
                 const void* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
-                hipStream_t s, int dtm = TE_F64);
+                hipStream_t s, int dtm = TE_F64, float* w32 = nullptr, const void* xp_dir = nullptr,
+                int xp_dir_m = 0);
 // tol: the relative tolerance, stored by FIN_INIT for the s-norm test of FIN_SNORM.
 // mirror (FIN_RHO): device pointer to host-coherent memory of SC_COUNT doubles; FIN_RHO
 // copies the scalars there and then bumps SC_SEQ, so the host can read an iteration's
 // outcome while the rest of the iteration is still running.
 int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s,
-                         double tol = 0.0, double* mirror = nullptr);
+                         double tol = 0.0, double* mirror = nullptr, const double* pc = nullptr);
 int launch_patch_bicgstab(const te_domain* d, int dtype, int op_kind, int neumann, const void* coeffs, const void* f,
                           void* u, int max_it, double tol, int32_t* iters, int32_t* bd, hipStream_t s);
 

@@ -124,8 +124,11 @@ __global__ void __launch_bounds__(kThreads) star_dot_kernel(const DevPatch* __re
 }
 
 // Vector updates of one BiCGStab iteration (interior cells only).  After the s-norm test
-// has fired (state ST_EARLY) only BICG_XP runs, and BICG_XP runs only then.
+// has fired (state ST_EARLY) only the early-exit update x += alpha dir runs: BICG_XP, or
+// BICG_XR given dirv (dir of type TM when dir_m, else FP64), which otherwise updates x and r.
 // TM: type of M p and M s (a0, a1 of BICG_XR; a0 of BICG_XP), widened to FP64 as read.
+// w32 (BICG_INIT, BICG_S, BICG_P; may be null): an FP32 copy of the new p or s, the
+// right-hand side the mixed-precision V-cycle reads (it rounds its input to FP32 anyway).
 template <int CPT, int MODE, typename TM>
 __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* __restrict__ scal,
                                                         double* __restrict__ v0, double* __restrict__ v1,
@@ -134,19 +137,34 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
                                                         const double* __restrict__ a2,
                                                         const double* __restrict__ a3,
                                                         const double* __restrict__ a4, double* __restrict__ pa,
-                                                        double* __restrict__ pb)
+                                                        double* __restrict__ pb, float* __restrict__ w32,
+                                                        const void* __restrict__ dirv, int dir_m)
 {
+  CellSet<CPT> cs;
+  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], beta = scal[SC_BETA];
   if (MODE != BICG_INIT) {
     const double st = scal[SC_STATE];
+    if (MODE == BICG_XR && dirv && st == ST_EARLY) {
+      // x += alpha dir (BiCGStab.h:162-167)
+      make_cells<CPT>(L, false, cs);
+      for (int p = blockIdx.x; p < L.np; p += gridDim.x) {
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+          if (k >= cs.n) continue;
+          const int i = p * L.ps + cs.off[k];
+          const double dv = dir_m ? double(((const TM*)dirv)[i]) : ((const double*)dirv)[i];
+          v0[i] += dv * alpha;
+        }
+      }
+      return;
+    }
     if (MODE == BICG_XP ? st != ST_EARLY : st != ST_RUN) return;
   }
   typedef typename std::conditional<MODE == BICG_XR || MODE == BICG_XP, TM, double>::type TA;
   const TA* __restrict__ a0 = (const TA*)a0v;
   const TA* __restrict__ a1 = (const TA*)a1v;
-  CellSet<CPT> cs;
   make_cells<CPT>(L, false, cs);
   double sa = 0.0, sb = 0.0;
-  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], beta = scal[SC_BETA];
   for (int p = blockIdx.x; p < L.np; p += gridDim.x) {
     const int base = p * L.ps;
 #pragma unroll
@@ -158,6 +176,7 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
         double r = a0[i];
         v0[i] = r;
         v1[i] = r;
+        if (w32) w32[i] = float(r);
         sa += a1[i] * a1[i];
         sb += r * r;
       } else if (MODE == BICG_S) {
@@ -165,6 +184,7 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
         double s = a0[i];
         s += a1[i] * -alpha;
         v0[i] = s;
+        if (w32) w32[i] = float(s);
         sa += s * s;
       } else if (MODE == BICG_XP) {
         // v0 = x, a0 = p: x += p * alpha (early exit, BiCGStab.h:162-167)
@@ -180,7 +200,9 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
       } else {  // BICG_P: v0 = p, a0 = ap, a1 = r
         double pv = v0[i];
         pv += a0[i] * -omega;
-        v0[i] = pv * beta + a1[i];
+        const double pn = pv * beta + a1[i];
+        v0[i] = pn;
+        if (w32) w32[i] = float(pn);
       }
     }
   }
@@ -203,17 +225,20 @@ __device__ double fold(const double* __restrict__ part, int n)
 }
 
 // After the s-norm test has fired only FIN_RHO runs, to publish the state.
+// FIN_SOMEGA: the s-norm test from pc (the s.s partials), then omega unless it fired.
 __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double* __restrict__ pa,
-                                                          const double* __restrict__ pb, int n,
+                                                          const double* __restrict__ pb,
+                                                          const double* __restrict__ pc, int n,
                                                           double* __restrict__ scal, double tol,
                                                           double* __restrict__ mirror)
 {
   const bool run = mode == FIN_INIT || scal[SC_STATE] == ST_RUN;
   if (!run && mode != FIN_RHO) return;
-  double A = 0.0, B = 0.0;
+  double A = 0.0, B = 0.0, C = 0.0;
   if (run) {
     A = fold(pa, n);
-    B = (mode == FIN_INIT || mode == FIN_OMEGA || mode == FIN_RHO) ? fold(pb, n) : 0.0;
+    B = (mode == FIN_INIT || mode == FIN_OMEGA || mode == FIN_SOMEGA || mode == FIN_RHO) ? fold(pb, n) : 0.0;
+    if (mode == FIN_SOMEGA) C = fold(pc, n);
   }
   if (threadIdx.x != 0) return;
   switch (mode) {
@@ -233,6 +258,13 @@ __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double
       break;
     }
     case FIN_OMEGA: scal[SC_OMEGA] = A / B; break;
+    case FIN_SOMEGA: {
+      const double sn = sqrt(C);
+      scal[SC_SNORM] = sn;
+      if (sn / scal[SC_R0] <= scal[SC_TOL]) scal[SC_STATE] = ST_EARLY;
+      else scal[SC_OMEGA] = A / B;
+      break;
+    }
     default: {  // FIN_RHO
       if (run) {
         double rho_new = A;
@@ -323,10 +355,11 @@ int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const do
 
 int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const void* a0,
                 const void* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
-                hipStream_t s, int dtm)
+                hipStream_t s, int dtm, float* w32, const void* xp_dir, int xp_dir_m)
 {
-  if (dtm != TE_F64 && mode != BICG_XR && mode != BICG_XP) {
-    set_error("internal: FP32 operands only for the x updates");
+  if ((dtm != TE_F64 && mode != BICG_XR && mode != BICG_XP) ||
+      (w32 && mode != BICG_INIT && mode != BICG_S && mode != BICG_P)) {
+    set_error("internal: FP32 operands only for the x updates, FP32 copies only of p and s");
     return TE_ERR_ARG;
   }
   Layout L = make_layout(d->dim, d->ns, 1, d->np);
@@ -340,10 +373,10 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
   do {                                                                                                           \
     if (dtm == TE_F32)                                                                                           \
       hipLaunchKernelGGL((bicg_kernel<C_, M_, float>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0, a1, \
-                         a2, a3, a4, pa, pb);                                                                    \
+                         a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m);                                             \
     else                                                                                                         \
       hipLaunchKernelGGL((bicg_kernel<C_, M_, double>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0,   \
-                         a1, a2, a3, a4, pa, pb);                                                                \
+                         a1, a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m);                                         \
   } while (0)
 #define TE_BK(M_)                      \
   do {                                 \
@@ -369,9 +402,9 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
 }
 
 int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s, double tol,
-                         double* mirror)
+                         double* mirror, const double* pc)
 {
-  hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, n, scal, tol, mirror);
+  hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, pc, n, scal, tol, mirror);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
