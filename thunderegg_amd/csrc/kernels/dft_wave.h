@@ -119,7 +119,9 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
             T eig = T(0);
             eig -= lx * s2[kx];
             eig -= ly * s2[ky];
-            Z[I][J][r] = Z[I][J][r] / eig;
+            // FP32: the hardware reciprocal (1 ulp) instead of a correctly rounded division
+            if (sizeof(T) == 4) Z[I][J][r] = Z[I][J][r] * T(__builtin_amdgcn_rcpf(float(eig)));
+            else Z[I][J][r] = Z[I][J][r] / eig;
           }
     }
     // ---- V = S3 Z'

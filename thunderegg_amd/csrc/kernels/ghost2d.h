@@ -100,6 +100,15 @@ __device__ __forceinline__ void ghost2d_gather(int type, int orth, int q, int s,
   }
 }
 
+// x / c for the constant divisors of the ghost formulas: a division in FP64 (as the
+// reference), a product with the rounded reciprocal in FP32, where the cycle is a
+// preconditioner and the two differ by an ulp.
+template <typename T>
+__device__ __forceinline__ T divc(T x, int c)
+{
+  return sizeof(T) == 8 ? x / T(c) : x * T(1.0 / c);
+}
+
 template <typename T, int KIND, class Self>
 __device__ __forceinline__ T ghost2d_combine(int type, int orth, int t, int nt, Self S, const T (&v)[4])
 {
@@ -120,15 +129,15 @@ __device__ __forceinline__ T ghost2d_combine(int type, int orth, int t, int nt, 
       }
       return G + T(2.0 / 3.0) * v[0];
     }
-    T G = T(2) * S(0, t) / T(3) - S(1, t) / T(5);
+    T G = divc(T(2) * S(0, t), 3) - divc(S(1, t), 5);
     int i0, di, cs;
     coarse_quad_cells(orth, t, nt, &i0, &di, &cs);
     const T c0 = v[0], c1 = v[1], c2 = v[2];
     T add;
-    if (cs == 0) add = T(3) * c0 / T(4) - T(3) * c1 / T(10) + c2 / T(12);
-    else if (cs == 1) add = T(7) * c0 / T(20) + T(7) * c1 / T(30) - c2 / T(20);
-    else if (cs == 2) add = c0 / T(12) + c1 / T(2) - c2 / T(20);
-    else add = -c0 / T(20) + c1 / T(2) + c2 / T(12);
+    if (cs == 0) add = divc(T(3) * c0, 4) - divc(T(3) * c1, 10) + divc(c2, 12);
+    else if (cs == 1) add = divc(T(7) * c0, 20) + divc(T(7) * c1, 30) - divc(c2, 20);
+    else if (cs == 2) add = divc(c0, 12) + divc(c1, 2) - divc(c2, 20);
+    else add = divc(-c0, 20) + divc(c1, 2) + divc(c2, 12);
     return G + add;
   }
   // finer neighbours (ghost t lies against fine cells 2t, 2t + 1 of neighbour 0 when
@@ -139,15 +148,15 @@ __device__ __forceinline__ T ghost2d_combine(int type, int orth, int t, int nt, 
   } else {
     if (t == 0 || t == nt - 1) {
       const int i0 = t == 0 ? 0 : nt - 1, di = t == 0 ? 1 : -1;
-      G = -S(0, i0) / T(10) + S(0, i0 + di) / T(15) - S(0, i0 + 2 * di) / T(30);
+      G = divc(-S(0, i0), 10) + divc(S(0, i0 + di), 15) - divc(S(0, i0 + 2 * di), 30);
     } else {
-      G = -S(0, t - 1) / T(30) - S(0, t + 1) / T(30);
+      G = divc(-S(0, t - 1), 30) - divc(S(0, t + 1), 30);
     }
   }
 #pragma unroll
   for (int j = 0; j < 2; j++) {
     if (KIND == TE_BILINEAR) G += T(2.0 / 3.0) * v[j];
-    else G += v[2 * j] / T(3) + v[2 * j + 1] / T(5);
+    else G += divc(v[2 * j], 3) + divc(v[2 * j + 1], 5);
   }
   return G;
 }
