@@ -41,6 +41,14 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
 {
   constexpr int n = 16 * NT;
   constexpr int ld = n + 1;  // padded LDS row
+  // the padded layout of a one-component vector of n x n patches (L.sx, L.ps = L.cs) as
+  // compile-time constants: the address arithmetic folds into shifts and adds
+  constexpr int SX = n + 2, PS = SX * SX;
+  L.sx = SX;  // the same constants for the accessors that take the layout
+  L.ps = PS;
+  L.cs = PS;
+  L.n[0] = n;
+  L.n[1] = n;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // the transform matrices with rows padded to n + 1 (the MFMA operand loads of a lane
   // group walk down a column: an unpadded pitch of n puts 8 lanes on one LDS bank)
@@ -76,11 +84,11 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
   constexpr bool PREF = NT == 1;
   TF fn[PREF ? FPL : 1];
   auto fload = [&](int p) {
-    const int base = p * L.ps;
+    const int base = p * PS;
 #pragma unroll
     for (int k = 0; k < (PREF ? FPL : 1); k++) {
       const int q = lane + 64 * k, y = q / n, x = q - y * n;
-      fn[k] = f[base + (y + 1) * L.sx + x + 1];
+      fn[k] = at(f, base + (y + 1) * SX + x + 1);
     }
   };
   auto fput = [&]() {
@@ -107,13 +115,13 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
 #pragma unroll
     for (int k = 0; k < (PREF_T ? FPL : 1); k++) {
       const int q = lane + 64 * k, y = q / n, x = q - y * n;
-      un[k] = u[p * L.ps + (y + 1) * L.sx + x + 1];
+      un[k] = at(u, p * PS + (y + 1) * SX + x + 1);
     }
     if (GM == 2) {
-      on = fa.porth[p];
-      parn = fa.parent[p];
+      on = at(fa.porth, p);
+      parn = at(fa.parent, p);
     }
-    rn = ring[(p * 4 + rs) * 2 + (2 * rt >= n ? 1 : 0)];
+    rn = at(ring, (p * 4 + rs) * 2 + (2 * rt >= n ? 1 : 0));
   };
   auto tload_b = [&]() {
     if (GM == 2) {
@@ -122,10 +130,10 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
         const int q = lane + 64 * k, y = q / n, x = q - y * n;
         int cx = x, cy = y;
         if (on >= 0) {
-          cx = (x + ((on & 1) ? L.n[0] : 0)) / 2;
-          cy = (y + ((on & 2) ? L.n[1] : 0)) / 2;
+          cx = (x + ((on & 1) ? n : 0)) / 2;
+          cy = (y + ((on & 2) ? n : 0)) / 2;
         }
-        ucn[k] = fa.uc[parn * L.ps + (cy + 1) * L.sx + cx + 1];
+        ucn[k] = at(fa.uc, parn * PS + (cy + 1) * SX + cx + 1);
       }
     }
 #pragma unroll
@@ -135,15 +143,15 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
     }
     const int pq = rn.w;
     ghost2d_gather<KIND>(rn.x, rn.y, rn.z, rs, rt, n, n, [&](int k, int q, int x, int y) {
-      gu[k] = u[q * L.ps + (y + 1) * L.sx + x + 1];
+      gu[k] = at(u, q * PS + (y + 1) * SX + x + 1);
       if (GM == 2) {
         const int o = (pq & 7) - 1;
         int cx = x, cy = y;
         if (o >= 0) {
-          cx = (x + ((o & 1) ? L.n[0] : 0)) / 2;
-          cy = (y + ((o & 2) ? L.n[1] : 0)) / 2;
+          cx = (x + ((o & 1) ? n : 0)) / 2;
+          cy = (y + ((o & 2) ? n : 0)) / 2;
         }
-        gc[k] = fa.uc[(pq >> 3) * L.ps + (cy + 1) * L.sx + cx + 1];
+        gc[k] = at(fa.uc, (pq >> 3) * PS + (cy + 1) * SX + cx + 1);
       }
     });
   };
@@ -166,10 +174,10 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
     if (PREF) {
       fput();
     } else {
-      const int base = p * L.ps;
+      const int base = p * PS;
       for (int q = lane; q < n * n; q += 64) {
         int y = q / n, x = q - y * n;
-        X[y * ld + x] = T(f[base + (y + 1) * L.sx + x + 1]);
+        X[y * ld + x] = T(f[base + (y + 1) * SX + x + 1]);
       }
     }
     if (TILE) tile_interior_2d<T>(p, n, n, U, Tl, lane);
@@ -194,7 +202,7 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
     const bool has_next = p + xr.step < xr.end;
     if (PREF && has_next) fload(p + xr.step);
     if (PREF_T && has_next) tload_a(p + xr.step);
-    const int base = p * L.ps;
+    const int base = p * PS;
     const DevPatch& P = dp[p];
     // ---- modifyRHSForInternalBoundaryConditions: the interface term of every side cell
     //      is formed first (a lane per cell, so all its loads are in flight together),
@@ -211,13 +219,13 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
           int gi, bi;  // padded index of the ghost and of the boundary cell
           int bx, by;  // boundary cell
           if (ph == 0) {
-            gi = base + (t + 1) * L.sx + (up ? n + 1 : 0);
-            bi = base + (t + 1) * L.sx + (up ? n : 1);
+            gi = base + (t + 1) * SX + (up ? n + 1 : 0);
+            bi = base + (t + 1) * SX + (up ? n : 1);
             bx = up ? n - 1 : 0;
             by = t;
           } else {
-            gi = base + (up ? n + 1 : 0) * L.sx + t + 1;
-            bi = base + (up ? n : 1) * L.sx + t + 1;
+            gi = base + (up ? n + 1 : 0) * SX + t + 1;
+            bi = base + (up ? n : 1) * SX + t + 1;
             bx = t;
             by = up ? n - 1 : 0;
           }
@@ -247,7 +255,7 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
           if (OP == 0) {
             term = (ug + ub) * ih2;
           } else {
-            const int cg = gi - p * L.ps + p * L.cs, cb = bi - p * L.ps + p * L.cs;
+            const int cg = gi - p * PS + p * PS, cb = bi - p * PS + p * PS;
             term = (ug + ub) * (coeffs[cb] + coeffs[cg]) * (ih2 * T(0.5));
           }
         }
@@ -268,7 +276,7 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
       }
     }
     const T lx = T(4.0 / (P.h[0] * P.h[0])), ly = T(4.0 / (P.h[1] * P.h[1]));
-    dft_solve_wave<T, TO, NT>(X, Sm, S3, s2, lx, ly, out, base, L.sx, true);
+    dft_solve_wave<T, TO, NT>(X, Sm, S3, s2, lx, ly, out, base, SX, true);
     wave_sync();  // X and the tile are rewritten by the next patch
   }
 }
@@ -352,6 +360,12 @@ __global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __r
   constexpr int ld = n + 1;
   constexpr int w = n + 2;
   constexpr int h = n / 2;
+  constexpr int SX = n + 2, PS = SX * SX;  // padded one-component layout of both levels
+  Lf.sx = SX;  // the same constants for the accessors that take the layout
+  Lf.ps = PS;
+  Lf.cs = PS;
+  Lf.n[0] = n;
+  Lf.n[1] = n;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TC* Sm = reinterpret_cast<TC*>(smem_raw);  // rows padded to ld, as in dft_mfma_kernel
   TC* S3 = Sm + n * ld;
@@ -382,11 +396,11 @@ __global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __r
       // every global read of the child is issued before the first barrier: its ring
       // record, own values, f and orth, then the neighbour values the record names
       int rs, rt;
-      rr = ring[ring_index(c, lane, n, n, &rs, &rt)];
+      rr = at(ring, ring_index(c, lane, n, n, &rs, &rt));
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int x = 2 * Xb + (q & 1), y = 2 * Yb + (q >> 1);
-        fv[q] = f[c * Lf.ps + (y + 1) * Lf.sx + x + 1];
+        fv[q] = at(f, c * PS + (y + 1) * SX + x + 1);
       }
       o = porth[c];
       tile_interior_2d<T>(c, n, n, U, tile, lane);
@@ -403,8 +417,8 @@ __global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __r
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int x = 2 * Xb + (q & 1), y = 2 * Yb + (q >> 1);
-        const int ci = (y + 1) * Lf.sx + x + 1;
-        r[q] = T(fv[q]) - star_tile<T, OP>(tile, w, x, y, ih2, cf, c * Lf.cs + ci, Lf.sx);
+        const int ci = (y + 1) * SX + x + 1;
+        r[q] = T(fv[q]) - star_tile<T, OP>(tile, w, x, y, ih2, cf, c * PS + ci, SX);
       }
       if (o < 0) {  // copyToParent
 #pragma unroll
@@ -412,7 +426,7 @@ __global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __r
           const int x = 2 * Xb + (q & 1), y = 2 * Yb + (q >> 1);
           const TC v = TC(r[q]);
           X[y * ld + x] = v;
-          fc[P * Lc.ps + (y + 1) * Lc.sx + x + 1] = v;
+          fc[P * PS + (y + 1) * SX + x + 1] = v;
         }
       } else {  // restrictToCoarserParent, fine cells in x-fastest order
         T acc = T(0);
@@ -420,14 +434,14 @@ __global__ void __launch_bounds__(64 * kWaves) rr_pre_kernel(const DevPatch* __r
         for (int q = 0; q < 4; q++) acc += r[q] * T(0.25);
         const TC v = TC(acc);
         X[(Yb + oy) * ld + Xb + ox] = v;
-        fc[P * Lc.ps + (Yb + oy + 1) * Lc.sx + Xb + ox + 1] = v;
+        fc[P * PS + (Yb + oy + 1) * SX + Xb + ox + 1] = v;
       }
     }
     __syncthreads();
     // the coarse patch solve from u = 0 by wave 0 (the others take part in its barrier)
     const DevPatch& Pc = dpc[P];
     const TC lx = TC(4.0 / (Pc.h[0] * Pc.h[0])), ly = TC(4.0 / (Pc.h[1] * Pc.h[1]));
-    dft_solve_wave<TC, TC, 1>(X, Sm, S3, s2, lx, ly, uc, P * Lc.ps, Lc.sx, wave == 0);
+    dft_solve_wave<TC, TC, 1>(X, Sm, S3, s2, lx, ly, uc, P * PS, SX, wave == 0);
     __syncthreads();  // X is rewritten by the next patch
   }
 }

@@ -165,7 +165,7 @@ __device__ __forceinline__ void dft_solve_wave(T* X, const T* __restrict__ Sm, c
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           int y = 16 * I + Mf<T>::row(r, g), x = 16 * J + c;
-          out[base + (y + 1) * sx + x + 1] = TO(acc[r] * scale);
+          at(out, base + (y + 1) * sx + x + 1) = TO(acc[r] * scale);
         }
       }
   }

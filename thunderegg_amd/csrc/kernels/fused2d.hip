@@ -32,6 +32,11 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
                                                                  const int32_t* __restrict__ porth, Layout L,
                                                                  int neumann, const int4* __restrict__ ring)
 {
+  if (P16) {  // compile-time layout constants: the address arithmetic folds
+    L.n[0] = L.n[1] = 16;
+    L.sx = 18;
+    L.ps = L.cs = 18 * 18;
+  }
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n0 = L.n[0], n1 = L.n[1], w = n0 + 2;
   const int h0 = n0 / 2, h1 = n1 / 2;
@@ -50,15 +55,15 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
   int4 rn = make_int4(-1, 0, -1, 0);
   auto load_a = [&](int p) {
     int rs, rt;
-    rn = ring[ring_index(p, lane, n0, n1, &rs, &rt)];
-    on = porth[p];
-    cpn = parent[p];
+    rn = at(ring, ring_index(p, lane, n0, n1, &rs, &rt));
+    on = at(porth, p);
+    cpn = at(parent, p);
 #pragma unroll
     for (int k = 0; k < (P16 ? 4 : 1); k++) {
       const int q = lane + 64 * k;
-      un[k] = u[p * L.ps + ((q >> 4) + 1) * L.sx + (q & 15) + 1];
+      un[k] = at(u, p * L.ps + ((q >> 4) + 1) * L.sx + (q & 15) + 1);
       const int x = 2 * (lane & 7) + (k & 1), y = 2 * (lane >> 3) + (k >> 1);
-      fn[k] = f[p * L.ps + (y + 1) * L.sx + x + 1];
+      fn[k] = at(f, p * L.ps + (y + 1) * L.sx + x + 1);
     }
   };
   auto load_b = [&]() { ring_gather_plain<T, KIND>(rn, lane, n0, n1, u, L, gu); };
@@ -106,14 +111,14 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
         for (int j = 0; j < 4; j++) {
           const int x = 2 * X + (j & 1), y = 2 * Y + (j >> 1);
           const int c = (y + 1) * L.sx + x + 1;
-          const TF fv = P16 ? fcur[j] : f[p * L.ps + c];
+          const TF fv = P16 ? fcur[j] : at(f, p * L.ps + c);
           r[j] = T(fv) - star_tile<T, OP>(tile, w, x, y, ih2, cf, p * L.cs + c, L.sx);
         }
         if (o < 0) {  // copyToParent
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             const int x = 2 * X + (j & 1), y = 2 * Y + (j >> 1);
-            fc[cp * L.ps + (y + 1) * L.sx + x + 1] = TC(r[j]);
+            at(fc, cp * L.ps + (y + 1) * L.sx + x + 1) = TC(r[j]);
           }
         } else {
           // restrictToCoarserParent: coarse[(i + s) / 2] += fine[i] / 4 over the fine
@@ -121,7 +126,7 @@ __global__ void __launch_bounds__(64 * kWaves) resid_restrict_2d(const DevPatch*
           T acc = T(0);
 #pragma unroll
           for (int j = 0; j < 4; j++) acc += r[j] * T(0.25);
-          fc[cp * L.ps + (Y + oy + 1) * L.sx + X + ox + 1] = TC(acc);
+          at(fc, cp * L.ps + (Y + oy + 1) * L.sx + X + ox + 1) = TC(acc);
         }
       }
     }
@@ -134,7 +139,7 @@ template <typename T, typename TU>
 struct WideAcc {
   const TU* u;
   Layout L;
-  __device__ __forceinline__ T operator()(int q, int x, int y) const { return T(u[q * L.ps + (y + 1) * L.sx + x + 1]); }
+  __device__ __forceinline__ T operator()(int q, int x, int y) const { return T(at(u, q * L.ps + (y + 1) * L.sx + x + 1)); }
 };
 
 // out = A u (ghost cells of out written 0); pa[block] = sum w * out, pb[block] = sum out * out.
@@ -147,6 +152,11 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
                                                            double* __restrict__ pb, Layout L, int neumann,
                                                            const int4* __restrict__ ring)
 {
+  if (P16) {  // compile-time layout constants: the address arithmetic folds
+    L.n[0] = L.n[1] = 16;
+    L.sx = 18;
+    L.ps = L.cs = 18 * 18;
+  }
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n0 = L.n[0], n1 = L.n[1], w = n0 + 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -163,13 +173,13 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
   double gu[4];
   auto load_a = [&](int p) {
     int rs, rt;
-    rn = ring[ring_index(p, lane, n0, n1, &rs, &rt)];
+    rn = at(ring, ring_index(p, lane, n0, n1, &rs, &rt));
 #pragma unroll
     for (int k = 0; k < (P16 ? 4 : 1); k++) {
       const int q = lane + 64 * k;
       const int c = p * L.ps + ((q >> 4) + 1) * L.sx + (q & 15) + 1;
-      un[k] = double(u[c]);
-      wn[k] = wv[c];
+      un[k] = double(at(u, c));
+      wn[k] = at(wv, c);
     }
   };
   auto load_b = [&]() { ring_gather_plain<double, KIND>(rn, lane, n0, n1, u, L, gu); };
@@ -206,8 +216,8 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
         const int y = q / n0, x = q - y * n0;
         const int c = (y + 1) * L.sx + x + 1;
         const double acc = star_tile<double, OP>(tile, w, x, y, ih2, cf, p * L.cs + c, L.sx);
-        out[base + c] = acc;
-        const double wq = P16 ? wcur[k & 3] : wv[base + c];
+        at(out, base + c) = acc;
+        const double wq = P16 ? wcur[k & 3] : at(wv, base + c);
         sa += wq * acc;  // rhat.dot(ap) / as.dot(s): w first, as the reference
         if (SQ) sq += acc * acc;
       }
@@ -222,7 +232,7 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
           x = h < n1 ? -1 : n0;
           y = h < n1 ? h : h - n1;
         }
-        out[base + (y + 1) * L.sx + x + 1] = 0.0;
+        at(out, base + (y + 1) * L.sx + x + 1) = 0.0;
       }
     }
     wave_sync();

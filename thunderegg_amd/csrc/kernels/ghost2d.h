@@ -187,7 +187,7 @@ template <typename T>
 struct PlainAcc {  // interior value of patch q at (x, y)
   const T* u;
   Layout L;
-  __device__ __forceinline__ T operator()(int q, int x, int y) const { return u[q * L.ps + (y + 1) * L.sx + x + 1]; }
+  __device__ __forceinline__ T operator()(int q, int x, int y) const { return at(u, q * L.ps + (y + 1) * L.sx + x + 1); }
 };
 
 // (A u)(x, y) from the staged tile; ci: padded index of the cell in the coefficient vector
@@ -206,7 +206,7 @@ __device__ __forceinline__ T star_tile(const T* tile, int w, int x, int y, const
       acc += (upper - T(2) * mid + lower) * ih2[a];
     } else {
       const int cst = a == 0 ? 1 : sx;
-      const T cm = cf[ci], cl = cf[ci - cst], cu = cf[ci + cst];
+      const T cm = at(cf, ci), cl = at(cf, ci - cst), cu = at(cf, ci + cst);
       acc += ((cu + cm) * (upper - mid) - (cl + cm) * (mid - lower)) * (ih2[a] * T(0.5));
     }
   }
@@ -347,7 +347,7 @@ __device__ __forceinline__ void ring_gather_plain(const int4& r, int g, int n0, 
 #pragma unroll
   for (int k = 0; k < 4; k++) v[k] = T(0);
   ghost2d_gather<KIND>(r.x, r.y, r.z, s, t, na, nt,
-                       [&](int k, int q, int x, int y) { v[k] = T(u[q * L.ps + (y + 1) * L.sx + x + 1]); });
+                       [&](int k, int q, int x, int y) { v[k] = T(at(u, q * L.ps + (y + 1) * L.sx + x + 1)); });
 }
 
 // The ghost of ring cell g (record r, gathered neighbour values v) written into the tile

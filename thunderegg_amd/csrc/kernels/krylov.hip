@@ -152,8 +152,8 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
         for (int k = 0; k < CPT; k++) {
           if (k >= cs.n) continue;
           const int i = p * L.ps + cs.off[k];
-          const double dv = dir_m ? double(((const TM*)dirv)[i]) : ((const double*)dirv)[i];
-          v0[i] += dv * alpha;
+          const double dv = dir_m ? double(at((const TM*)dirv, i)) : at((const double*)dirv, i);
+          at(v0, i) += dv * alpha;
         }
       }
       return;
@@ -173,36 +173,36 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
       const int i = base + cs.off[k];
       if (MODE == BICG_INIT) {
         // v0 = rhat, v1 = p, a0 = r, a1 = b: rhat = r; p = r; sums b.b and rhat.r
-        double r = a0[i];
-        v0[i] = r;
-        v1[i] = r;
-        if (w32) w32[i] = float(r);
-        sa += a1[i] * a1[i];
+        double r = at(a0, i);
+        at(v0, i) = r;
+        at(v1, i) = r;
+        if (w32) at(w32, i) = float(r);
+        sa += at(a1, i) * at(a1, i);
         sb += r * r;
       } else if (MODE == BICG_S) {
         // v0 = s, a0 = r, a1 = ap: s = r; s += ap * -alpha; sum s.s
-        double s = a0[i];
-        s += a1[i] * -alpha;
-        v0[i] = s;
-        if (w32) w32[i] = float(s);
+        double s = at(a0, i);
+        s += at(a1, i) * -alpha;
+        at(v0, i) = s;
+        if (w32) at(w32, i) = float(s);
         sa += s * s;
       } else if (MODE == BICG_XP) {
         // v0 = x, a0 = p: x += p * alpha (early exit, BiCGStab.h:162-167)
-        v0[i] += double(a0[i]) * alpha;
+        at(v0, i) += double(at(a0, i)) * alpha;
       } else if (MODE == BICG_XR) {
         // v0 = x, v1 = r, a0 = mp, a1 = ms, a2 = ap, a3 = as, a4 = rhat
-        v0[i] += double(a0[i]) * alpha + double(a1[i]) * omega;
-        double r = v1[i];
-        r += a2[i] * -alpha + a3[i] * -omega;
-        v1[i] = r;
-        sa += r * a4[i];
+        at(v0, i) += double(at(a0, i)) * alpha + double(at(a1, i)) * omega;
+        double r = at(v1, i);
+        r += at(a2, i) * -alpha + at(a3, i) * -omega;
+        at(v1, i) = r;
+        sa += r * at(a4, i);
         sb += r * r;
       } else {  // BICG_P: v0 = p, a0 = ap, a1 = r
-        double pv = v0[i];
-        pv += a0[i] * -omega;
-        const double pn = pv * beta + a1[i];
-        v0[i] = pn;
-        if (w32) w32[i] = float(pn);
+        double pv = at(v0, i);
+        pv += at(a0, i) * -omega;
+        const double pn = pv * beta + at(a1, i);
+        at(v0, i) = pn;
+        if (w32) at(w32, i) = float(pn);
       }
     }
   }

@@ -62,6 +62,20 @@ inline Layout make_layout(int dim, const int* n, int ncomp, int np)
   return L;
 }
 
+// base[idx] for idx >= 0 with the byte offset formed in 32 bits (vectors and tables stay
+// below 2 GiB, checked when domains are created), so a load or store can address from the
+// kernel argument's scalar base plus a 32-bit vector offset instead of a 64-bit vector address.
+template <typename T>
+__device__ __forceinline__ const T& at(const T* base, int idx)
+{
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T& at(T* base, int idx)
+{
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)idx * (unsigned)sizeof(T));
+}
+
 __host__ __device__ inline int lin(const Layout& L, int p, int c, int x, int y, int z)
 {
   return p * L.ps + c * L.cs + (x + 1) + (y + 1) * L.sx + (L.dim == 3 ? (z + 1) * L.sy : 0);
