@@ -31,8 +31,11 @@ namespace {
 
 constexpr int kWaves = 4;
 
-template <typename T, typename TF, typename TO, typename TC, int NT, int OP, bool UZERO, int GM, int KIND>
-__global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM == 0 ? 5 : 4) : 1) dft_mfma_kernel(const DevPatch* __restrict__ dp,
+// PF (n = 16): each wave walks several patches and loads the next one while it solves the
+// current one; without PF (levels where each wave gets about one patch) a patch's loads come
+// right before its solve, which needs fewer registers, so more waves fit a SIMD.
+template <typename T, typename TF, typename TO, typename TC, int NT, int OP, bool UZERO, int GM, int KIND, bool PF>
+__global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (PF ? (GM == 0 ? 5 : 4) : (GM == 0 ? 8 : (KIND == TE_BILINEAR ? 5 : 4))) : 1) dft_mfma_kernel(const DevPatch* __restrict__ dp,
                                                                const T* __restrict__ mats, const T* __restrict__ s2tab,
                                                                const TF* __restrict__ f, const T* __restrict__ u,
                                                                const T* __restrict__ coeffs, TO* __restrict__ out,
@@ -185,7 +188,7 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
   };
   const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
   int p = xr.first + wave;
-  if (p < xr.end) {
+  if (PF && p < xr.end) {
     if (PREF) fload(p);
     if (PREF_T) {
       tload_a(p);
@@ -195,11 +198,19 @@ __global__ void __launch_bounds__(64 * kWaves, (NT == 1 && sizeof(T) == 4) ? (GM
   }
   __syncthreads();
   for (bool first = true; p < xr.end; p += xr.step, first = false) {
-    if (!first) {
+    if (!PF) {
+      if (PREF) fload(p);
+      if (PREF_T) {
+        tload_a(p);
+        tload_b();
+      }
+      stage(p);
+      wave_sync();
+    } else if (!first) {
       stage(p);
       wave_sync();
     }
-    const bool has_next = p + xr.step < xr.end;
+    const bool has_next = PF && p + xr.step < xr.end;
     if (PREF && has_next) fload(p + xr.step);
     if (PREF_T && has_next) tload_a(p + xr.step);
     const int base = p * PS;
@@ -302,10 +313,21 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   // several patches and its prefetch of the next one overlaps the current solve
   const int cap = NT == 1 ? 2048 : 4096;
   int grid = ngroups < cap ? ngroups : cap;
+  // the next patch is prefetched where waves get more than one patch each
+  const bool pf = NT == 1 && ngroups > cap;
   const int kind = pl->g->kind;
-#define TE_MK(OP_, UZ_, GM_, K_)                                                                            \
-  hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_>), dim3(grid), dim3(64 * kWaves), \
+#define TE_MK1(OP_, UZ_, GM_, K_, PF_)                                                                          \
+  hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_, PF_>), dim3(grid), dim3(64 * kWaves), \
                      shmem, s, d->d_patches, mats, s2, f, u, cf, out, L, fa, ring)
+#define TE_MK(OP_, UZ_, GM_, K_)                      \
+  do {                                                \
+    if constexpr (NT == 1) {                          \
+      if (pf) TE_MK1(OP_, UZ_, GM_, K_, true);        \
+      else TE_MK1(OP_, UZ_, GM_, K_, false);          \
+    } else {                                          \
+      TE_MK1(OP_, UZ_, GM_, K_, false);               \
+    }                                                 \
+  } while (0)
 #define TE_MKG(OP_)                                                         \
   if (uzero) TE_MK(OP_, true, 0, 0);                                        \
   else if (gm == 0) TE_MK(OP_, false, 0, 0);                                \
@@ -320,6 +342,7 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   }
 #undef TE_MKG
 #undef TE_MK
+#undef TE_MK1
 }
 
 template <typename T, typename TF, typename TO, typename TC>
