@@ -216,12 +216,41 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
   }
 }
 
-// fixed-order fold of n partials by one workgroup (strided, coalesced reads)
-__device__ double fold(const double* __restrict__ part, int n)
+// Fixed-order folds of n partials by one workgroup (thread t sums partials t, t + 256, ...
+// in that order, then the block reduction): up to three arrays, every load issued before
+// the first add (n <= kFoldMax, the Krylov grid cap), one block reduction pass for all.
+constexpr int kFoldPer = 8;
+constexpr int kFoldMax = kThreads * kFoldPer;
+static_assert(kFoldMax >= 2048, "the Krylov kernels write at most 2048 partials");
+
+template <int NA>
+__device__ void fold_n(const double* const (&part)[3], int n, double (&out)[3])
 {
-  double acc = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) acc += part[k];
-  return block_reduce<RED_SUM>(acc);
+  double v[NA][kFoldPer];
+#pragma unroll
+  for (int a = 0; a < NA; a++)
+#pragma unroll
+    for (int j = 0; j < kFoldPer; j++) {
+      const int k = threadIdx.x + kThreads * j;
+      v[a][j] = k < n ? part[a][k] : 0.0;
+    }
+  __shared__ double wsum[3][kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kFoldPer; j++) acc += v[a][j];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) wsum[a][w] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    double r = wsum[a][0];
+    for (int k = 1; k < kThreads / 64; k++) r += wsum[a][k];
+    out[a] = r;  // valid in every thread
+  }
 }
 
 // After the s-norm test has fired only FIN_RHO runs, to publish the state.
@@ -236,9 +265,14 @@ __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double
   if (!run && mode != FIN_RHO) return;
   double A = 0.0, B = 0.0, C = 0.0;
   if (run) {
-    A = fold(pa, n);
-    B = (mode == FIN_INIT || mode == FIN_OMEGA || mode == FIN_SOMEGA || mode == FIN_RHO) ? fold(pb, n) : 0.0;
-    if (mode == FIN_SOMEGA) C = fold(pc, n);
+    const double* const parts[3] = { pa, pb, pc };
+    double r[3] = { 0.0, 0.0, 0.0 };
+    if (mode == FIN_SOMEGA) fold_n<3>(parts, n, r);
+    else if (mode == FIN_ALPHA || mode == FIN_SNORM) fold_n<1>(parts, n, r);
+    else fold_n<2>(parts, n, r);
+    A = r[0];
+    B = r[1];
+    C = r[2];
   }
   if (threadIdx.x != 0) return;
   switch (mode) {
@@ -404,6 +438,10 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
 int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, double* scal, hipStream_t s, double tol,
                          double* mirror, const double* pc)
 {
+  if (n > kFoldMax) {
+    set_error("internal: too many partial sums for the BiCGStab finalize");
+    return TE_ERR_ARG;
+  }
   hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, pc, n, scal, tol, mirror);
   TE_HIP(hipGetLastError());
   return TE_OK;
