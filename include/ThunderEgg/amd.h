@@ -543,6 +543,23 @@ class PatchSolver : public ThunderEgg::PatchSolver<D> {
   bool continue_;
 };
 
+/* Iterative/CG.h:40-194 */
+template <int D>
+class CG : public Solver<D> {
+ public:
+  void setMaxIterations(int n) { max_iterations_ = n; }
+  int getMaxIterations() const { return max_iterations_; }
+  void setTolerance(double t) { tolerance_ = t; }
+  double getTolerance() const { return tolerance_; }
+  CG* clone() const override { return new CG(*this); }
+  int solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, const Operator<D>* Mr = nullptr,
+            bool output = false, std::ostream& os = std::cout) const override;
+
+ private:
+  int max_iterations_ = 1000;
+  double tolerance_ = 1e-12;
+};
+
 /* Iterative/BiCGStab.h:40-203 */
 template <int D>
 class BiCGStab : public Solver<D> {
@@ -598,7 +615,8 @@ struct Level {
   std::shared_ptr<const Interpolator<D>> interpolator;  // from the next coarser level
 };
 
-/* GMG/Cycle.h:40-98 + GMG/VCycle.h:46-70 */
+/* GMG/Cycle.h:40-98 + GMG/VCycle.h:46-70, GMG/WCycle.h:50-84, GMG/FMGCycle.h:43-104
+ * (CycleOpts::cycle_type "V", "W", "F", as CycleBuilder.h:182-189 dispatches) */
 template <int D>
 class Cycle : public Operator<D> {
  public:
@@ -607,7 +625,7 @@ class Cycle : public Operator<D> {
     build_device_cycle();
   }
   Cycle* clone() const override { return new Cycle(*this); }
-  /* Cycle::apply (GMG/Cycle.h:94-98): u := V-cycle(f) from u = 0 */
+  /* Cycle::apply (GMG/Cycle.h:94-98): u := cycle(f) from u = 0 */
   void apply(const Vector<D>& f, Vector<D>& u) const override
   {
     if (g_) {
@@ -615,7 +633,7 @@ class Cycle : public Operator<D> {
       return;
     }
     u.setWithGhost(0.0);
-    visit(0, f, u);
+    visit(0, f, u, false);
   }
   /* the fused device cycle, or NULL when a component is user-defined */
   te_gmg* device() const { return g_.get(); }
@@ -623,7 +641,9 @@ class Cycle : public Operator<D> {
   int numLevels() const { return (int)levels_.size(); }
 
  private:
-  void visit(size_t i, const Vector<D>& f, Vector<D>& u) const
+  /* vonly: FMGCycle::v_visit (one coarse visit).  The W- and F-cycles' second coarse
+   * visit starts from the first one's coarse u, as in the reference. */
+  void visit(size_t i, const Vector<D>& f, Vector<D>& u, bool vonly) const
   {
     const Level<D>& L = levels_[i];
     if (i + 1 == levels_.size()) {
@@ -631,13 +651,22 @@ class Cycle : public Operator<D> {
       return;
     }
     for (int k = 0; k < opts_.pre_sweeps; k++) L.smoother->smooth(f, u);
-    Vector<D> r = f.getZeroClone();  // Cycle::restrict: r = f - A u
-    L.op->apply(u, r);
-    r.scaleThenAdd(-1.0, f);
-    Vector<D> fc = L.restrictor->restrict(r);
+    auto restrict_residual = [&]() {  // Cycle::restrict: r = f - A u, restricted
+      Vector<D> r = f.getZeroClone();
+      L.op->apply(u, r);
+      r.scaleThenAdd(-1.0, f);
+      return L.restrictor->restrict(r);
+    };
+    Vector<D> fc = restrict_residual();
     Vector<D> uc = fc.getZeroClone();
-    visit(i + 1, fc, uc);
+    visit(i + 1, fc, uc, false);
     levels_[i + 1].interpolator->interpolate(uc, u);
+    if (opts_.cycle_type != "V" && !vonly) {
+      for (int k = 0; k < opts_.mid_sweeps; k++) L.smoother->smooth(f, u);
+      fc = restrict_residual();
+      visit(i + 1, fc, uc, opts_.cycle_type == "F");
+      levels_[i + 1].interpolator->interpolate(uc, u);
+    }
     for (int k = 0; k < opts_.post_sweeps; k++) L.smoother->smooth(f, u);
   }
   void build_device_cycle();
@@ -651,7 +680,8 @@ class Cycle : public Operator<D> {
 template <int D>
 void Cycle<D>::build_device_cycle()
 {
-  if (opts_.cycle_type != "V") throw RuntimeError("CycleBuilder: only the V-cycle is provided by this build");
+  if (opts_.cycle_type != "V" && opts_.cycle_type != "W" && opts_.cycle_type != "F")
+    throw RuntimeError("Unsupported Cycle type: " + opts_.cycle_type);
   if (opts_.precision != "double" && opts_.precision != "mixed")
     throw RuntimeError("CycleOpts::precision must be \"double\" or \"mixed\"");
   te_gmg_options o{};
@@ -714,6 +744,8 @@ void Cycle<D>::build_device_cycle()
   o.pre_sweeps = opts_.pre_sweeps;
   o.post_sweeps = opts_.post_sweeps;
   o.coarse_sweeps = opts_.coarse_sweeps;
+  o.mid_sweeps = opts_.mid_sweeps;
+  o.cycle_type = opts_.cycle_type == "W" ? TE_CYCLE_W : (opts_.cycle_type == "F" ? TE_CYCLE_F : TE_CYCLE_V);
   o.precision = opts_.precision == "mixed" ? TE_F32 : TE_F64;
   o.use_graph = 1;
   te_gmg* h = nullptr;
@@ -862,6 +894,61 @@ int BiCGStab<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, c
     residual = resid.twoNorm() / r0_norm;
     if (output) detail_bicg::log_line(os, num_its, residual);
   }
+  return num_its;
+}
+
+template <int D>
+int CG<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, const Operator<D>* Mr, bool output,
+                 std::ostream& os) const
+{
+  // Device solve when Mr is a device cycle whose finest operator is A.
+  auto cyc = dynamic_cast<const GMG::Cycle<D>*>(Mr);
+  if (cyc && cyc->device() && same_operator(A, cyc->finestOperator()) && b.getNumComponents() == 1) {
+    std::vector<double> hist((size_t)max_iterations_ + 2, 0.0);
+    int its = 0;
+    detail::check(te_gmg_cg(cyc->device(), b.data(), x.data(), 1, max_iterations_, tolerance_, &its, hist.data(),
+                            (int)hist.size(), nullptr));
+    if (output)
+      for (int k = 0; k <= its && k < (int)hist.size(); k++) detail_bicg::log_line(os, k, hist[(size_t)k]);
+    return its;
+  }
+  // Generic right-preconditioned CG over the virtual interfaces (Iterative/CG.h:126-194).
+  Vector<D> resid = b.getZeroClone();
+  A.apply(x, resid);
+  resid.scaleThenAdd(-1.0, b);
+  Vector<D> initial_guess(x);
+  x.set(0.0);
+  const double r0_norm = b.twoNorm();
+  Vector<D> p(resid), ap = b.getZeroClone(), tmp = b.getZeroClone();
+  double rho = resid.dot(resid);
+  int num_its = 0;
+  if (r0_norm == 0) return num_its;
+  double residual = resid.twoNorm() / r0_norm;
+  if (output) detail_bicg::log_line(os, num_its, residual);
+  while (residual > tolerance_ && num_its < max_iterations_) {
+    if (rho == 0) throw BreakdownError("CG broke down, rho was 0 on iteration " + std::to_string(num_its));
+    if (Mr) {
+      Mr->apply(p, tmp);
+      A.apply(tmp, ap);
+    } else {
+      A.apply(p, ap);
+    }
+    const double alpha = rho / p.dot(ap);
+    x.addScaled(alpha, p);
+    resid.addScaled(-alpha, ap);
+    const double rho_new = resid.dot(resid);
+    const double beta = rho_new / rho;
+    p.scaleThenAdd(beta, resid);
+    num_its++;
+    rho = rho_new;
+    residual = resid.twoNorm() / r0_norm;
+    if (output) detail_bicg::log_line(os, num_its, residual);
+  }
+  if (Mr) {
+    Mr->apply(x, resid);
+    x.copy(resid);
+  }
+  x.add(initial_guess);
   return num_its;
 }
 }  // namespace Iterative

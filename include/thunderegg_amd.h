@@ -240,6 +240,11 @@ int te_vec_convert(const te_domain* d, int dst_dtype, void* dst, int src_dtype, 
  * chain of domains (finest first), with per-level device workspaces allocated here,
  * once.  precision: TE_F64 = FP64 everywhere; TE_F32 = FP32 V-cycle (ghost fill,
  * stencil, smoother, restrict/interpolate) with the coarsest-level solve in FP64. */
+/* GMG::VCycle (GMG/VCycle.h:46-70), GMG::WCycle (GMG/WCycle.h:50-84), GMG::FMGCycle
+ * (GMG/FMGCycle.h:43-104).  W and F run one launch per step (the fused launches are
+ * V-cycle only); 0 (V) is the zero-initialised default. */
+enum te_cycle_type { TE_CYCLE_V = 0, TE_CYCLE_W = 1, TE_CYCLE_F = 2 };
+
 typedef struct te_gmg_options {
   int filler_kind;    /* te_filler_kind */
   int fill_type;      /* te_fill_type */
@@ -253,6 +258,8 @@ typedef struct te_gmg_options {
   int pre_sweeps, post_sweeps, coarse_sweeps; /* CycleOpts (GMG/CycleOpts.h:33-55) */
   int precision;      /* TE_F64 or TE_F32 */
   int use_graph;      /* capture the V-cycle as a hipGraph */
+  int cycle_type;     /* te_cycle_type: CycleOpts::cycle_type "V" / "W" / "F" (GMG/CycleBuilder.h:182-189) */
+  int mid_sweeps;     /* CycleOpts::mid_sweeps (W and F cycles) */
 } te_gmg_options;
 
 /* coeffs: VarPoisson beta per level (device float64 pointers, ghost-filled), or NULL. */
@@ -272,7 +279,13 @@ int te_gmg_vcycle(te_gmg* g, const double* f, double* u, te_stream stream);
  * return value.  Returns TE_ERR_BREAKDOWN when rho == 0. */
 int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int max_it, double tol,
                     int* iterations, double* history, int history_cap, te_stream stream);
-/* Seconds spent in the last te_gmg_bicgstab, from the first A x to convergence. */
+/* Iterative::CG::solve(A, x, b, Mr) (Iterative/CG.h:126-194), right-preconditioned by
+ * this cycle (use_precond 1) or plain (0); x holds the initial guess and receives
+ * Mr x + guess.  Like the reference, a zero b returns 0 iterations with x set to zero.
+ * history as for te_gmg_bicgstab.  Returns TE_ERR_BREAKDOWN when rho == 0. */
+int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it, double tol, int* iterations,
+              double* history, int history_cap, te_stream stream);
+/* Seconds spent in the last te_gmg_bicgstab / te_gmg_cg, from the first A x to convergence. */
 double te_gmg_last_solve_seconds(const te_gmg* g);
 
 /* ----------------------------------------------- mesh generation (host) --

@@ -40,6 +40,9 @@ def lib():
         L.ora_restrict.argtypes = [vp, vp, dp, dp, C.c_int]
         L.ora_interpolate.argtypes = [vp, vp, dp, dp, C.c_int]
         L.ora_vcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.ora_cycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.ora_cg_solve.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp,
+                                   C.c_int, C.c_double, dp, C.c_int]
         L.ora_bicgstab_solve.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int,
                                          C.c_double, dp, C.c_int]
         for f in ("ora_vec_dot",):
@@ -195,6 +198,25 @@ class Levels:
         _check(lib().ora_vcycle_apply(C.cast(self.arr, C.c_void_p), len(self.domains), pre, post, coarse, _dp(f),
                                       _dp(u)), "vcycle")
         return u
+
+    def cycle(self, f, cycle_type="V", pre=1, post=1, mid=1, coarse=1):
+        """Cycle::apply with the reference's VCycle, WCycle or FMGCycle (cycle_type "V" / "W" / "F")."""
+        u = np.zeros_like(f)
+        t = {"V": 0, "W": 1, "F": 2}[cycle_type]
+        _check(lib().ora_cycle_apply(C.cast(self.arr, C.c_void_p), len(self.domains), t, pre, post, mid, coarse,
+                                     _dp(f), _dp(u)), "cycle")
+        return u
+
+    def cg(self, b, x=None, precond=True, cycle_type="V", pre=1, post=1, mid=1, coarse=1, max_it=1000, tol=1e-10):
+        """Iterative::CG::solve, right-preconditioned by the cycle (or plain)."""
+        x = np.zeros_like(b) if x is None else x
+        hist = np.full(max_it + 2, np.nan)
+        t = {"V": 0, "W": 1, "F": 2}[cycle_type]
+        its = lib().ora_cg_solve(C.cast(self.arr, C.c_void_p), len(self.domains), int(precond), t, pre, post, mid,
+                                 coarse, _dp(b), _dp(x), max_it, tol, _dp(hist), len(hist))
+        if its < 0:
+            raise RuntimeError(f"oracle CG failed ({its})")
+        return x, its, hist[: its + 1]
 
     def bicgstab(self, b, x=None, precond=True, pre=1, post=1, coarse=1, max_it=1000, tol=1e-10):
         """precond: False, True (the reference's solve) or 2 (early exit with alpha M p)."""

@@ -1244,11 +1244,13 @@ int ora_interpolate(const ora_domain* coarse, const ora_domain* fine, const doub
   return 0;
 }
 
-/* --------------------------------------------------------------- V-cycle -- */
+/* --------------------------------------------------------------- cycles -- */
 typedef struct {
   const ora_level* lv;
   int nl, pre, post, coarse;
   int err;
+  int type; /* 0 V (VCycle.h), 1 W (WCycle.h), 2 F (FMGCycle.h) */
+  int mid;  /* CycleOpts::mid_sweeps (W and F) */
 } vcctx;
 
 static int level_smooth(const ora_level* L, const double* f, double* u)
@@ -1263,8 +1265,28 @@ static int level_apply(const ora_level* L, double* u, double* f)
   return op_apply(&o, u, f, 1);
 }
 
-/* VCycle::visit (GMG/VCycle.h:46-70) with Cycle::restrict (GMG/Cycle.h:57-65). */
-static int visit(vcctx* c, int li, const double* f, double* u)
+/* Cycle::restrict (GMG/Cycle.h:57-65): cf = restriction of f - A u. */
+static int cycle_restrict(vcctx* c, int li, const double* f, double* u, double* cf)
+{
+  const ora_level* L = &c->lv[li];
+  long len = ora_domain_vec_len(L->dom, 1);
+  double* r = (double*)calloc(len, sizeof(double));
+  int rc = level_apply(L, u, r);
+  if (!rc) {
+    vspace V = { L->dom, L->dom->np, 1 };
+    FOR_INTERIOR(&V, i) r[i] = r[i] * -1 + f[i];
+    rc = ora_restrict(L->dom, c->lv[li + 1].dom, r, cf, 1);
+  }
+  free(r);
+  return rc;
+}
+
+/* VCycle::visit (GMG/VCycle.h:46-70), WCycle::visit (GMG/WCycle.h:50-84) and
+ * FMGCycle::visit / v_visit (GMG/FMGCycle.h:43-104).  vonly: FMGCycle::v_visit (one
+ * coarse visit, its recursion again FMGCycle::visit).  W and F visit the coarser level a
+ * second time after the mid sweeps, re-restricting the residual into the coarse f but
+ * keeping the coarse u of the first visit as the initial guess, as the reference does. */
+static int visit(vcctx* c, int li, const double* f, double* u, int vonly)
 {
   const ora_level* L = &c->lv[li];
   int rc;
@@ -1275,22 +1297,22 @@ static int visit(vcctx* c, int li, const double* f, double* u)
   }
   for (int i = 0; i < c->pre; i++)
     if ((rc = level_smooth(L, f, u))) return rc;
-  long len = ora_domain_vec_len(L->dom, 1);
-  double* r = (double*)calloc(len, sizeof(double));
-  if ((rc = level_apply(L, u, r))) {
-    free(r);
-    return rc;
-  }
-  vspace V = { L->dom, L->dom->np, 1 };
-  FOR_INTERIOR(&V, i) r[i] = r[i] * -1 + f[i];
   const ora_level* C = &c->lv[li + 1];
   long clen = ora_domain_vec_len(C->dom, 1);
   double* cf = (double*)calloc(clen, sizeof(double));
   double* cu = (double*)calloc(clen, sizeof(double));
-  rc = ora_restrict(L->dom, C->dom, r, cf, 1);
-  free(r);
-  if (!rc) rc = visit(c, li + 1, cf, cu);
+  rc = cycle_restrict(c, li, f, u, cf);
+  if (!rc) rc = visit(c, li + 1, cf, cu, 0);
   if (!rc) rc = ora_interpolate(C->dom, L->dom, cu, u, 1);
+  if (!rc && c->type != 0 && !vonly) {
+    for (int i = 0; i < c->mid && !rc; i++) rc = level_smooth(L, f, u);
+    if (!rc) {
+      memset(cf, 0, sizeof(double) * clen);
+      rc = cycle_restrict(c, li, f, u, cf);
+    }
+    if (!rc) rc = visit(c, li + 1, cf, cu, c->type == 2);
+    if (!rc) rc = ora_interpolate(C->dom, L->dom, cu, u, 1);
+  }
   free(cf);
   free(cu);
   if (rc) return rc;
@@ -1299,18 +1321,24 @@ static int visit(vcctx* c, int li, const double* f, double* u)
   return 0;
 }
 
+int ora_cycle_apply(const ora_level* levels, int nlevels, int type, int pre, int post, int mid, int coarse,
+                    const double* f, double* u)
+{
+  memset(u, 0, sizeof(double) * ora_domain_vec_len(levels[0].dom, 1)); /* Cycle::apply: u.setWithGhost(0) */
+  vcctx c = { levels, nlevels, pre, post, coarse, 0, type, mid };
+  return visit(&c, 0, f, u, 0);
+}
+
 int ora_vcycle_apply(const ora_level* levels, int nlevels, int pre, int post, int coarse, const double* f,
                      double* u)
 {
-  memset(u, 0, sizeof(double) * ora_domain_vec_len(levels[0].dom, 1)); /* Cycle::apply: u.setWithGhost(0) */
-  vcctx c = { levels, nlevels, pre, post, coarse, 0 };
-  return visit(&c, 0, f, u);
+  return ora_cycle_apply(levels, nlevels, 0, pre, post, 1, coarse, f, u);
 }
 
 static int vcycle_cb(void* ctx, double* in, double* out)
 {
   vcctx* c = (vcctx*)ctx;
-  int rc = ora_vcycle_apply(c->lv, c->nl, c->pre, c->post, c->coarse, in, out);
+  int rc = ora_cycle_apply(c->lv, c->nl, c->type, c->pre, c->post, c->mid, c->coarse, in, out);
   if (rc) c->err = rc;
   return rc;
 }
@@ -1323,11 +1351,77 @@ int ora_bicgstab_solve(const ora_level* levels, int nlevels, int use_precond, in
                        const double* b, double* x, int max_it, double tol, double* hist, int hist_cap)
 {
   vspace V = { levels[0].dom, levels[0].dom->np, 1 };
-  vcctx c = { levels, nlevels, pre, post, coarse, 0 };
+  vcctx c = { levels, nlevels, pre, post, coarse, 0, 0, 1 };
   int bd = 0;
   int its = bicgstab(&V, level_apply_cb, (void*)&levels[0], x, b, use_precond ? vcycle_cb : NULL, &c, max_it, tol,
                      hist, hist_cap, &bd, use_precond == 2);
   if (bd) return -1;
   if (c.err) return -2;
   return its;
+}
+
+/* ------------------------------------------------------------------ CG --
+ * Iterative::CG::solve (Iterative/CG.h:126-194), right-preconditioned: ap = A (Mr p)
+ * (Solver.h applyWithPreconditioner), x accumulates the unpreconditioned iterate from 0,
+ * and at the end x = Mr x + the initial guess. */
+int ora_cg_solve(const ora_level* levels, int nlevels, int use_precond, int type, int pre, int post, int mid,
+                 int coarse, const double* b, double* x, int max_it, double tol, double* hist, int hist_cap)
+{
+  const ora_level* L0 = &levels[0];
+  vspace V = { L0->dom, L0->dom->np, 1 };
+  vcctx c = { levels, nlevels, pre, post, coarse, 0, type, mid };
+  long len = vs_len(&V);
+  double* resid = (double*)calloc(len, sizeof(double));
+  double* x0 = (double*)malloc(sizeof(double) * len);
+  double* p = (double*)malloc(sizeof(double) * len);
+  double* ap = (double*)calloc(len, sizeof(double));
+  double* tmp = (double*)calloc(len, sizeof(double));
+  int rc = 0, num_its = 0;
+  level_apply(L0, x, resid);
+  FOR_INTERIOR(&V, i) resid[i] = resid[i] * -1 + b[i]; /* resid.scaleThenAdd(-1, b) */
+  memcpy(x0, x, sizeof(double) * len);                 /* initial_guess = x */
+  FOR_INTERIOR(&V, i) x[i] = 0.0;                      /* x.set(0) */
+  double r0_norm = vs_norm(&V, b);
+  memcpy(p, resid, sizeof(double) * len);
+  double rho = vs_dot(&V, resid, resid);
+  if (r0_norm == 0) goto done;
+  double residual = vs_norm(&V, resid) / r0_norm;
+  if (hist && hist_cap > 0) hist[0] = residual;
+  while (residual > tol && num_its < max_it) {
+    if (rho == 0) {
+      rc = -1;
+      goto done;
+    }
+    if (use_precond) {
+      memset(tmp, 0, sizeof(double) * len);
+      if ((rc = vcycle_cb(&c, p, tmp))) goto done;
+      level_apply(L0, tmp, ap);
+    } else {
+      level_apply(L0, p, ap);
+    }
+    double alpha = rho / vs_dot(&V, p, ap);
+    FOR_INTERIOR(&V, i) x[i] += p[i] * alpha;
+    FOR_INTERIOR(&V, i) resid[i] += ap[i] * -alpha;
+    double rho_new = vs_dot(&V, resid, resid);
+    double beta = rho_new / rho;
+    FOR_INTERIOR(&V, i) p[i] = p[i] * beta + resid[i]; /* p.scaleThenAdd(beta, resid) */
+    num_its++;
+    rho = rho_new;
+    residual = vs_norm(&V, resid) / r0_norm;
+    if (hist && num_its < hist_cap) hist[num_its] = residual;
+  }
+  if (use_precond) {
+    if ((rc = vcycle_cb(&c, x, resid))) goto done;
+    FOR_INTERIOR(&V, i) x[i] = resid[i]; /* x.copy(resid) */
+  }
+  FOR_INTERIOR(&V, i) x[i] += x0[i]; /* x.add(initial_guess) */
+done:
+  free(resid);
+  free(x0);
+  free(p);
+  free(ap);
+  free(tmp);
+  if (rc == -1) return -1;
+  if (rc) return -2;
+  return num_its;
 }

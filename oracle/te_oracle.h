@@ -104,6 +104,16 @@ typedef struct ora_level {
 int ora_vcycle_apply(const ora_level* levels, int nlevels, int pre, int post, int coarse,
                      const double* f, double* u);
 
+/* GMG::Cycle::apply with VCycle (type 0), WCycle (1, GMG/WCycle.h:50-84) or FMGCycle
+ * (2, GMG/FMGCycle.h:43-104) visits; mid = CycleOpts::mid_sweeps. */
+int ora_cycle_apply(const ora_level* levels, int nlevels, int type, int pre, int post, int mid, int coarse,
+                    const double* f, double* u);
+
+/* Iterative::CG::solve (Iterative/CG.h:126-194), right-preconditioned by the cycle over
+ * `levels` (use_precond 1) or plain (0).  Returns iterations, -1 on breakdown (rho == 0). */
+int ora_cg_solve(const ora_level* levels, int nlevels, int use_precond, int type, int pre, int post, int mid,
+                 int coarse, const double* b, double* x, int max_it, double tol, double* hist, int hist_cap);
+
 /* Iterative::BiCGStab::solve (Iterative/BiCGStab.h:111-203) with A = level 0's operator
  * and Mr = the V-cycle over `levels` (use_precond 1), or none (0).  use_precond 2 is the
  * V-cycle with the early exit updating x by alpha M p instead of the reference's alpha p.

@@ -1,11 +1,13 @@
 // Drives the C++ API (include/ThunderEgg/) the way a program written against the
 // reference's hot-path classes does: domains from a forest, per-level fillers,
 // operators, DFT smoothers, restrictors and interpolators, a CycleBuilder V-cycle, and
-// Iterative::BiCGStab.  Checks: the device cycle and the generic (virtual-call) cycle
-// give the same V-cycle and the same solve; mixed precision reaches the tolerance; the
-// reference's log format; the builder's error behaviour.  Exit status 0 = pass.
+// Iterative::BiCGStab / Iterative::CG.  Checks: the device cycle and the generic
+// (virtual-call) cycle give the same V-, W- and F-cycle and the same solves; mixed
+// precision reaches the tolerance; the reference's log format; the builder's error
+// behaviour.  Exit status 0 = pass.
 #include <ThunderEgg/GMG/CycleBuilder.h>
 #include <ThunderEgg/Iterative/BiCGStab.h>
+#include <ThunderEgg/Iterative/CG.h>
 #include <ThunderEgg/Poisson/DFTPatchSolver.h>
 
 #include <cmath>
@@ -143,11 +145,86 @@ int main()
   CHECK(solver.solve(A, x4, z, dev.get()) == 0);
   CHECK(std::fabs(x4.infNorm() - 1.0) < 1e-15);
 
+  // W- and F-cycles (GMG/WCycle.h, GMG/FMGCycle.h): device == generic
+  for (const char* type : { "W", "F" }) {
+    GMG::CycleOpts wopts;
+    wopts.cycle_type = type;
+    auto wd = build(doms, wopts, false);
+    auto wg = build(doms, wopts, true);
+    CHECK(wd->device() != nullptr);
+    Vector<2> w1(doms[0], 1), w2(doms[0], 1);
+    wd->apply(f, w1);
+    wg->apply(f, w2);
+    double dw = rel_diff(w1, w2);
+    std::printf("%s-cycle device vs generic: %.3e\n", type, dw);
+    CHECK(dw <= 1e-12);
+    CHECK(rel_diff(w1, u1) > 1e-6);  // not the V-cycle
+  }
+
+  // CG (Iterative/CG.h) preconditioned by the V-cycle, on a uniform forest (CG needs the
+  // symmetric operator of a mesh without coarse/fine faces): device == generic == BiCGStab
+  {
+    ForestDomainGenerator<2> ugen(3, { 16, 16 });
+    std::vector<Domain<2>> ud{ ugen.getFinestDomain() };
+    while (ugen.hasCoarserDomain()) ud.push_back(ugen.getCoarserDomain());
+    Vector<2> fu(ud[0], 1);
+    {
+      std::vector<double> h = fu.copyToHost();
+      const auto& P = ud[0].getPatchInfoVector();
+      const int n = 16, w = n + 2;
+      for (size_t p = 0; p < P.size(); p++)
+        for (int j = 0; j < n; j++)
+          for (int i = 0; i < n; i++) {
+            double x = P[p].starts[0] + (i + 0.5) * P[p].spacings[0];
+            double y = P[p].starts[1] + (j + 0.5) * P[p].spacings[1];
+            h[p * w * w + (size_t)(j + 1) * w + i + 1] = std::sin(3 * M_PI * x) * x * y * (1 - y);
+          }
+      fu.copyFromHost(h);
+    }
+    BiQuadraticGhostFiller gfu(ud[0], GhostFillingType::Faces);
+    Poisson::StarPatchOperator<2> Au(ud[0], gfu);
+    auto udev = build(ud, opts, false);
+    auto ugen_cycle = build(ud, opts, true);
+    // preconditioned: the device solve and the generic one take the same steps (the
+    // reference's right-preconditioned CG needs A M symmetric, which this cycle is not, so
+    // 12 iterations are compared rather than a converged solve)
+    Iterative::CG<2> cg;
+    cg.setMaxIterations(12);
+    cg.setTolerance(1e-30);
+    Vector<2> c1(ud[0], 1), c2(ud[0], 1);
+    std::ostringstream l1, l2;
+    int ic1 = cg.solve(Au, c1, fu, udev.get(), true, l1);
+    int ic2 = cg.solve(Au, c2, fu, ugen_cycle.get(), true, l2);
+    double dc = rel_diff(c1, c2);
+    std::printf("cg (12 steps) device vs generic: %.3e\n", dc);
+    CHECK(ic1 == 12 && ic2 == 12);
+    CHECK(dc <= 1e-9);
+    // plain CG converges to the BiCGStab solution
+    Iterative::CG<2> plain;
+    plain.setTolerance(1e-11);
+    Vector<2> c3(ud[0], 1), c4(ud[0], 1);
+    int ic3 = plain.solve(Au, c3, fu);
+    int ic4 = solver.solve(Au, c4, fu, udev.get());
+    double db = rel_diff(c3, c4);
+    std::printf("plain cg %d iterations, bicgstab %d, solution diff %.3e\n", ic3, ic4, db);
+    CHECK(ic3 > 0 && ic3 < 1000);
+    CHECK(db <= 1e-7);
+  }
+
   // builder errors are RuntimeErrors, as in the reference
   bool threw = false;
   try {
     GMG::CycleBuilder<2> b(opts);
     b.getCycle();
+  } catch (const RuntimeError&) {
+    threw = true;
+  }
+  CHECK(threw);
+  threw = false;
+  try {
+    GMG::CycleOpts bad;
+    bad.cycle_type = "X";
+    build(doms, bad, false);
   } catch (const RuntimeError&) {
     threw = true;
   }

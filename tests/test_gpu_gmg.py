@@ -260,3 +260,63 @@ def test_fused_vcycle_matches_unfused(monkeypatch, filler, prec, op_kind):
     slack = 0 if (prec == R.F64 and op_kind == 0) else (2 if op_kind == 0 else max(2, int(0.2 * itu)))
     assert abs(itf - itu) <= slack
     assert relerr(mesh.interior(xf, dim), mesh.interior(xu, dim)) <= 1e-8
+
+
+@pytest.mark.parametrize("cycle_type", ["W", "F"])
+@pytest.mark.parametrize("label,dim,ns,mk,filler,smoother", [CASES[0], CASES[1]] + [c for c in CASES if "3d" in c[0]][:1])
+def test_w_and_f_cycles(cycle_type, label, dim, ns, mk, filler, smoother):
+    """GMG::WCycle / GMG::FMGCycle (GMG/WCycle.h:50-84, GMG/FMGCycle.h:43-104) against the
+    oracle: FP64 to 1e-12, the FP32 cycle to 2e-5; a BiCGStab solve preconditioned by them
+    takes the oracle's iteration count."""
+    _print_case("cycle", cycle_type, label)
+    levels = mk(ns)
+    f = random_field(mesh.padded_shape(len(levels[0]), dim, ns), 31)
+    ol = _oracle_levels(dim, ns, levels, filler, smoother)
+    uref = ol.cycle(f.copy(), cycle_type)
+    for prec, tol in ((R.F64, 1e-12), (R.F32, 2e-5)):
+        g = solver.GMG(dim, ns, levels, filler_kind=filler, smoother=smoother, precision=prec, cycle_type=cycle_type)
+        u = g.vcycle(f)
+        assert relerr(mesh.interior(u, dim), mesh.interior(uref, dim)) <= tol, prec
+        u2 = g.vcycle(f)  # graph replay
+        assert np.array_equal(u2, u)
+    b = _rhs_sin(levels, dim, ns)
+    g = solver.GMG(dim, ns, levels, filler_kind=filler, smoother=smoother, precision=R.F64, cycle_type=cycle_type)
+    x, its, hist = g.solve(b, tol=1e-10, precond=2)
+    od = [O.Domain(dim, ns, lv) for lv in levels]
+    L = O.Levels(od, filler_kind=filler, smoother_kind=smoother)
+    # the oracle's BiCGStab takes its V-cycle: the W / F preconditioned solve needs no more
+    # iterations (to rounding) and reaches the same solution
+    xv, itv, _ = L.bicgstab(b.copy(), tol=1e-10, precond=2)
+    assert its <= itv + 1
+    assert relerr(mesh.interior(x, dim), mesh.interior(xv, dim)) <= 1e-8
+
+
+@pytest.mark.parametrize("cycle_type", ["V", "W"])
+def test_cg_matches_oracle(cycle_type):
+    """Iterative::CG::solve (Iterative/CG.h:126-194) through te_gmg_cg against the oracle's
+    CG on a uniform mesh (a symmetric operator): plain and right-preconditioned, the same
+    iteration count and solution; a zero right-hand side leaves x at zero."""
+    _print_case("cg", cycle_type)
+    ns = (8, 8)
+    dim, levels = load("2d_uniform_4x4_mpi1.json", ns)
+    b = _rhs_sin(levels, dim, ns) + 0.1 * random_field(mesh.padded_shape(len(levels[0]), dim, ns), 32)
+    ol = _oracle_levels(dim, ns, levels, R.BILINEAR, 0)
+    g = solver.GMG(dim, ns, levels, filler_kind=R.BILINEAR, smoother=0, precision=R.F64, cycle_type=cycle_type)
+    for precond in (False, True):
+        xr, itr, hr = ol.cg(b.copy(), precond=precond, cycle_type=cycle_type, tol=1e-10)
+        x, its, hist = g.cg(b, tol=1e-10, precond=precond)
+        assert abs(its - itr) <= 1, precond
+        assert relerr(mesh.interior(x, dim), mesh.interior(xr, dim)) <= 1e-8, precond
+        n = min(len(hist), len(hr))
+        assert np.allclose(hist[:n], hr[:n], rtol=1e-6, atol=1e-13), precond
+    x0 = random_field(b.shape, 33)
+    x, its, _ = g.cg(np.zeros_like(b), x0=x0)
+    assert its == 0 and not mesh.interior(x, dim).any()
+    # mixed-precision cycle as the preconditioner: the recursive residual converges as fast;
+    # the solution is the cycle applied to the CG iterate at the end (x = M y, CG.h:186-189),
+    # so it carries the FP32 cycle's rounding
+    gm = solver.GMG(dim, ns, levels, filler_kind=R.BILINEAR, smoother=0, precision=R.F32, cycle_type=cycle_type)
+    xm, itm, _ = gm.cg(b, tol=1e-10)
+    xr, itr, _ = ol.cg(b.copy(), precond=True, cycle_type=cycle_type, tol=1e-10)
+    assert itm <= itr + 2
+    assert relerr(mesh.interior(xm, dim), mesh.interior(xr, dim)) <= 2e-5

@@ -382,6 +382,92 @@ def test_vcycle_two_levels_matches_definition(name, kind):
     assert relerr(mesh.interior(got, dim), mesh.interior(u2, dim)) <= 1e-13
 
 
+def _cycle_by_definition(ds, kind, f, typ):
+    """VCycle / WCycle / FMGCycle::visit (GMG/VCycle.h:46-70, GMG/WCycle.h:50-84,
+    GMG/FMGCycle.h:43-104) composed from the separately pinned smoother, operator,
+    restrictor and interpolator, one sweep each (pre, mid, post, coarse)."""
+
+    def visit(li, f, u, vonly):
+        d = ds[li]
+        O.dft_smooth(d, f, u, kind=kind)
+        if li == len(ds) - 1:
+            return
+        c = ds[li + 1]
+        fc = O.restrict(d, c, f - O.poisson_apply(d, u.copy(), kind=kind))
+        uc = np.zeros_like(fc)
+        visit(li + 1, fc, uc, False)
+        O.interpolate(c, d, uc, u)
+        if typ != "V" and not vonly:
+            O.dft_smooth(d, f, u, kind=kind)  # mid sweep
+            fc = O.restrict(d, c, f - O.poisson_apply(d, u.copy(), kind=kind))
+            visit(li + 1, fc, uc, typ == "F")  # uc keeps the first visit's result
+            O.interpolate(c, d, uc, u)
+        O.dft_smooth(d, f, u, kind=kind)  # post sweep
+
+    u = np.zeros_like(f)
+    visit(0, f, u, False)
+    return u
+
+
+@pytest.mark.parametrize("name,kind", [("2d_uniform_2x2_refined_nw_mpi1.json", BIL),
+                                       ("2d_uniform_8x8_refined_cross_mpi1.json", BIQ)])
+def test_w_and_f_cycles_match_definition(name, kind):
+    ns = (8, 8)
+    dim, levels = load(name, ns)
+    levels = levels[-3:]
+    ds = [O.Domain(dim, ns, lv) for lv in levels]
+    L = O.Levels(ds, filler_kind=kind, smoother_kind=0)
+    f = random_field(ds[0].shape(), 19)
+    out = {}
+    for typ in ("V", "W", "F"):
+        got = L.cycle(f.copy(), typ)
+        want = _cycle_by_definition(ds, kind, f, typ)
+        assert relerr(mesh.interior(got, dim), mesh.interior(want, dim)) <= 1e-13, typ
+        out[typ] = mesh.interior(got, dim)
+    assert np.array_equal(out["V"], mesh.interior(L.vcycle(f.copy()), dim))
+    # three levels: the W- and F-cycles differ below the second level's second visit
+    assert relerr(out["W"], out["F"]) > 1e-8
+    # two levels: the F-cycle's V-visit of the coarsest level is its W-visit
+    L2 = O.Levels(ds[1:], filler_kind=kind, smoother_kind=0)
+    f2 = random_field(ds[1].shape(), 20)
+    assert np.array_equal(L2.cycle(f2.copy(), "W"), L2.cycle(f2.copy(), "F"))
+    # every cycle is linear in f for fixed sweeps
+    g = random_field(f.shape, 21)
+    for typ in ("W", "F"):
+        lhs = mesh.interior(L.cycle(2 * f + 3 * g, typ), dim)
+        rhs = 2 * out[typ] + 3 * mesh.interior(L.cycle(g.copy(), typ), dim)
+        assert relerr(lhs, rhs) <= 1e-12, typ
+
+
+def test_cg_against_dense_solve():
+    """Iterative::CG::solve (Iterative/CG.h:126-194) reaches the dense direct solve, plain and
+    right-preconditioned by a V- or W-cycle; on 64 unknowns plain CG needs at most about 64
+    iterations.  A zero right-hand side returns at once with x set to zero (the reference
+    zeroes x before it checks ||b||)."""
+    ns = (4, 4)
+    dim, levels = load("2d_uniform_2x2_mpi1.json", ns)
+    ods = [O.Domain(dim, ns, lv) for lv in levels]
+    A, pos = _dense_operator(ods[0], dim, ns)
+    assert np.allclose(A, A.T)  # the uniform-mesh operator is symmetric (CG's premise)
+    b = random_field(ods[0].shape(), 22)
+    xd = np.linalg.solve(A, b.ravel()[pos])
+    L = O.Levels(ods, filler_kind=BIL, smoother_kind=0)
+    x, its, hist = L.cg(b.copy(), precond=False, tol=1e-13)
+    assert 0 < its <= 80 and hist[-1] <= 1e-13
+    assert relerr(x.ravel()[pos], xd) <= 1e-9
+    for typ in ("V", "W"):
+        x, its_p, hist = L.cg(b.copy(), precond=True, cycle_type=typ, tol=1e-13)
+        assert 0 < its_p < its and hist[-1] <= 1e-13, typ
+        assert relerr(x.ravel()[pos], xd) <= 1e-9, typ
+    # initial guess: CG solves for the correction and adds the guess back
+    x0 = random_field(b.shape, 23)
+    x, _, _ = L.cg(b.copy(), x=x0.copy(), precond=True, tol=1e-13)
+    assert relerr(x.ravel()[pos], xd) <= 1e-9
+    z = np.zeros_like(b)
+    x, its, _ = L.cg(z, x=x0.copy())
+    assert its == 0 and not mesh.interior(x, dim).any()
+
+
 @pytest.mark.parametrize("name", MESH_2D + MESH_3D)
 def test_enumerated_calls_match_tables(name):
     """test/MPIGhostFiller_MPI1.cpp: the local face calls are one per (patch, side) with a

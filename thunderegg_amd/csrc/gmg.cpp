@@ -116,8 +116,12 @@ int smooth(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void*
   return TE_OK;
 }
 
-// VCycle::visit (GMG/VCycle.h:46-70)
-int visit(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
+// VCycle::visit (GMG/VCycle.h:46-70), WCycle::visit (GMG/WCycle.h:50-84) and
+// FMGCycle::visit / v_visit (GMG/FMGCycle.h:43-104), one launch per step.  u_zero: this
+// level's u starts at zero (the reference's fresh getZeroClone); false for the second
+// coarse visit of the W- and F-cycles, which keeps the first visit's coarse u as its
+// initial guess, as the reference does.  vonly: FMGCycle::v_visit (one coarse visit).
+int visit(te_gmg* g, int li, const void* f, int dtf, double* u_ext, bool u_zero = true, bool vonly = false)
 {
   GLevel& Lv = g->L[li];
   const int nl = (int)g->L.size();
@@ -127,25 +131,34 @@ int visit(te_gmg* g, int li, const void* f, int dtf, double* u_ext)
   if (li == nl - 1) {
     int n = g->o.coarse_sweeps;
     if (n == 0) {
-      TE_RC(zero_with_ghost(g, li, u, Lv.dt));
+      if (u_zero) TE_RC(zero_with_ghost(g, li, u, Lv.dt));
       if (mixed_out) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), TE_F64, u_ext, Lv.dt, u, g->s));
     }
     for (int k = 0; k < n; k++) {
       bool last = k == n - 1;
-      TE_RC(smooth(g, li, f, dtf, u, k == 0, (mixed_out && last) ? (void*)u_ext : u,
+      TE_RC(smooth(g, li, f, dtf, u, u_zero && k == 0, (mixed_out && last) ? (void*)u_ext : u,
                    (mixed_out && last) ? TE_F64 : Lv.dt));
     }
     return TE_OK;
   }
-  for (int k = 0; k < g->o.pre_sweeps; k++) TE_RC(smooth(g, li, f, dtf, u, k == 0, u, Lv.dt));
-  if (g->o.pre_sweeps == 0) TE_RC(zero_with_ghost(g, li, u, Lv.dt));
-  // Cycle::restrict: r = f - A u, then restrict into the coarser level's f
-  TE_RC(launch_ghost_fill(Lv.gf, Lv.dt, u, 1, g->s));
-  TE_RC(launch_star(Lv.gf, Lv.dt, dtf, Lv.dt, g->o.op_kind, g->o.neumann, Lv.coeffs, f, u, Lv.r, 1, g->s));
+  for (int k = 0; k < g->o.pre_sweeps; k++) TE_RC(smooth(g, li, f, dtf, u, u_zero && k == 0, u, Lv.dt));
+  if (g->o.pre_sweeps == 0 && u_zero) TE_RC(zero_with_ghost(g, li, u, Lv.dt));
   GLevel& Cv = g->L[li + 1];
-  TE_RC(launch_restrict(Lv.link, Lv.dt, Lv.r, Cv.dt, Cv.f, 1, g->s));
-  TE_RC(visit(g, li + 1, Cv.f, Cv.dt, u_ext));
+  // Cycle::restrict: r = f - A u, then restrict into the coarser level's f
+  auto restrict_to_coarse = [&]() -> int {
+    TE_RC(launch_ghost_fill(Lv.gf, Lv.dt, u, 1, g->s));
+    TE_RC(launch_star(Lv.gf, Lv.dt, dtf, Lv.dt, g->o.op_kind, g->o.neumann, Lv.coeffs, f, u, Lv.r, 1, g->s));
+    return launch_restrict(Lv.link, Lv.dt, Lv.r, Cv.dt, Cv.f, 1, g->s);
+  };
+  TE_RC(restrict_to_coarse());
+  TE_RC(visit(g, li + 1, Cv.f, Cv.dt, u_ext, true, false));
   TE_RC(launch_interpolate(Lv.link, Cv.dt, Cv.u, Lv.dt, u, 1, g->s));
+  if (g->o.cycle_type != TE_CYCLE_V && !vonly) {
+    for (int k = 0; k < g->o.mid_sweeps; k++) TE_RC(smooth(g, li, f, dtf, u, false, u, Lv.dt));
+    TE_RC(restrict_to_coarse());
+    TE_RC(visit(g, li + 1, Cv.f, Cv.dt, u_ext, false, g->o.cycle_type == TE_CYCLE_F));
+    TE_RC(launch_interpolate(Lv.link, Cv.dt, Cv.u, Lv.dt, u, 1, g->s));
+  }
   int n = g->o.post_sweeps;
   for (int k = 0; k < n; k++) {
     bool last = k == n - 1;
@@ -348,6 +361,15 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     set_error("te_gmg_create: smoother must be 0 (DFT) or 1 (BiCGStab patch solver)");
     return TE_ERR_ARG;
   }
+  if (opts->cycle_type < TE_CYCLE_V || opts->cycle_type > TE_CYCLE_F) {
+    set_error("te_gmg_create: Unsupported Cycle type " + std::to_string(opts->cycle_type) +
+              " (0 V, 1 W, 2 F as CycleBuilder accepts)");
+    return TE_ERR_ARG;
+  }
+  if (opts->pre_sweeps < 0 || opts->post_sweeps < 0 || opts->coarse_sweeps < 0 || opts->mid_sweeps < 0) {
+    set_error("te_gmg_create: sweep counts must be >= 0");
+    return TE_ERR_ARG;
+  }
   auto g = new te_gmg();
   g->o = *opts;
   hipError_t e = hipStreamCreate(&g->s);
@@ -419,7 +441,8 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     // one-kernel-per-step path (same arithmetic, for comparison and timing).
     const char* nf = std::getenv("TE_NO_FUSE");
     const bool allow = !(nf && nf[0] == '1');
-    bool fused = allow && opts->smoother == 0;
+    // the fused launches implement the V-cycle; W- and F-cycles run one launch per step
+    bool fused = allow && opts->smoother == 0 && opts->cycle_type == TE_CYCLE_V;
     for (int i = 0; i < num_levels && fused; i++) {
       const GLevel& Lv = g->L[i];
       fused = Lv.plan && Lv.plan->use_mfma && dft_mfma_supported(Lv.plan) &&
@@ -644,6 +667,97 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   TE_HIP(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
   g->last_seconds = std::chrono::duration<double>(t1 - t0).count();
+  if (iterations) *iterations = num_its;
+  if (rc) return rc;
+  return check_flags(g);
+}
+
+// Iterative::CG::solve (Iterative/CG.h:126-194): right-preconditioned CG, ap = A (M p)
+// (Solver.h, applyWithPreconditioner), x accumulates from zero and becomes M x plus the
+// initial guess at the end.  The scalars go to the host every iteration (the path is not
+// the headline one; it reuses the vector, operator and cycle launches).
+int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it, double tol, int* iterations,
+              double* history, int history_cap, te_stream stream)
+{
+  if (!g || !b || !x) {
+    set_error("te_gmg_cg: bad argument");
+    return TE_ERR_ARG;
+  }
+  TE_HIP(hipStreamSynchronize((hipStream_t)stream));
+  GLevel& L0 = g->L[0];
+  const te_domain* d = L0.d;
+  te_ghost_filler* gf = L0.gf;
+  const double* cf = g->o.op_kind == 1 ? L0.coeffs64 : nullptr;
+  hipStream_t s = g->s;
+  double* resid = g->resid;
+  double* x0 = g->rhat;  // the initial guess
+  double* p = g->p;
+  double* ap = g->ap;
+  double* tmp = g->mp;
+  auto vop = [&](int op, double* v, const double* a, double alpha) {
+    return launch_vec_op(d, TE_F64, op, v, a, nullptr, alpha, 0.0, 0.0, 1, s);
+  };
+  // dot product (RED_DOT) or two-norm (RED_SUMSQ: the launch returns the square root)
+  auto reduce = [&](int op, const double* a, const double* bv, double* out) -> int {
+    TE_RC(launch_reduce(d, TE_F64, op, a, bv, 1, d->d_result, s));
+    TE_HIP(hipMemcpyAsync(out, d->d_result, sizeof(double), hipMemcpyDeviceToHost, s));
+    TE_HIP(hipStreamSynchronize(s));
+    return TE_OK;
+  };
+  auto apply_a = [&](double* u, double* out) -> int {  // A.apply (PatchOperator.h:126-143)
+    TE_RC(launch_ghost_fill(gf, TE_F64, u, 1, s));
+    return launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, nullptr, u, out, 1, s);
+  };
+  auto t0 = std::chrono::steady_clock::now();
+  int num_its = 0, rc = TE_OK;
+  // resid = b - A x (A.apply then scaleThenAdd(-1, b), as one residual launch)
+  TE_RC(launch_ghost_fill(gf, TE_F64, x, 1, s));
+  TE_RC(launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, b, x, resid, 1, s));
+  TE_RC(vop(VOP_COPY, x0, x, 0.0));    // initial_guess = x
+  TE_RC(vop(VOP_SET, x, nullptr, 0.0));  // x.set(0)
+  double r0 = 0.0, rho = 0.0, nr = 0.0;
+  TE_RC(reduce(RED_SUMSQ, b, nullptr, &r0));  // b.twoNorm()
+  TE_RC(vop(VOP_COPY, p, resid, 0.0));
+  TE_RC(reduce(RED_DOT, resid, resid, &rho));
+  if (r0 != 0.0) {
+    TE_RC(reduce(RED_SUMSQ, resid, nullptr, &nr));
+    double residual = nr / r0;
+    if (history && history_cap > 0) history[0] = residual;
+    while (residual > tol && num_its < max_it) {
+      if (rho == 0.0) {
+        set_error("CG broke down, rho was 0 on iteration " + std::to_string(num_its));
+        rc = TE_ERR_BREAKDOWN;
+        break;
+      }
+      if (use_precond) {
+        TE_RC(vcycle_enqueue(g, p, tmp));
+        TE_RC(apply_a(tmp, ap));
+      } else {
+        TE_RC(apply_a(p, ap));
+      }
+      double pap = 0.0;
+      TE_RC(reduce(RED_DOT, p, ap, &pap));
+      const double alpha = rho / pap;
+      TE_RC(vop(VOP_ADD_SCALED, x, p, alpha));         // x.addScaled(alpha, p)
+      TE_RC(vop(VOP_ADD_SCALED, resid, ap, -alpha));   // resid.addScaled(-alpha, ap)
+      double rho_new = 0.0;
+      TE_RC(reduce(RED_DOT, resid, resid, &rho_new));
+      const double beta = rho_new / rho;
+      TE_RC(vop(VOP_SCALE_THEN_ADD, p, resid, beta));  // p.scaleThenAdd(beta, resid)
+      num_its++;
+      rho = rho_new;
+      TE_RC(reduce(RED_SUMSQ, resid, nullptr, &nr));
+      residual = nr / r0;
+      if (history && num_its < history_cap) history[num_its] = residual;
+    }
+    if (!rc && use_precond) {
+      TE_RC(vcycle_enqueue(g, x, tmp));  // Mr->apply(x, resid); x.copy(resid)
+      TE_RC(vop(VOP_COPY, x, tmp, 0.0));
+    }
+    if (!rc) TE_RC(vop(VOP_ADD, x, x0, 0.0));  // x.add(initial_guess)
+  }
+  TE_HIP(hipStreamSynchronize(s));
+  g->last_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (iterations) *iterations = num_its;
   if (rc) return rc;
   return check_flags(g);

@@ -89,6 +89,7 @@ SIGNATURES = {
     "te_gmg_destroy": (C.c_int, [_vp]),
     "te_gmg_vcycle": (C.c_int, [_vp, _vp, _vp, _vp]),
     "te_gmg_bicgstab": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp, C.c_int, _vp]),
+    "te_gmg_cg": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp, C.c_int, _vp]),
     "te_gmg_last_solve_seconds": (C.c_double, [_vp]),
     "te_forest_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
     "te_forest_destroy": (C.c_int, [_vp]),

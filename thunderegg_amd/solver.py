@@ -87,19 +87,26 @@ class _Opts(C.Structure):
         ("coarse_sweeps", C.c_int),
         ("precision", C.c_int),
         ("use_graph", C.c_int),
+        ("cycle_type", C.c_int),
+        ("mid_sweeps", C.c_int),
     ]
 
 
+CYCLE_TYPES = {"V": 0, "W": 1, "F": 2}
+
+
 class GMG:
-    """V-cycle hierarchy + outer BiCGStab on the device (te_gmg).
+    """Multigrid hierarchy + outer BiCGStab / CG on the device (te_gmg).
 
     levels: patch tables finest first.  smoother 0 = DFTPatchSolver,
-    1 = Iterative::PatchSolver(BiCGStab).  precision F64 or F32 (FP32 V-cycle,
-    FP64 coarsest solve, FP64 outer iteration)."""
+    1 = Iterative::PatchSolver(BiCGStab).  precision F64 or F32 (FP32 cycle,
+    FP64 coarsest solve, FP64 outer iteration).  cycle_type "V", "W" or "F"
+    (GMG::VCycle, WCycle, FMGCycle; mid_sweeps for W and F)."""
 
     def __init__(self, dim, ns, levels, filler_kind=R.BILINEAR, fill_type=R.FACES, op_kind=0, neumann=False,
                  smoother=0, neumann_mask=0, inner_max_it=1000, inner_tol=1e-12, continue_on_breakdown=False,
-                 pre_sweeps=1, post_sweeps=1, coarse_sweeps=1, precision=R.F64, use_graph=True, coeffs=None):
+                 pre_sweeps=1, post_sweeps=1, coarse_sweeps=1, precision=R.F64, use_graph=True, coeffs=None,
+                 cycle_type="V", mid_sweeps=1):
         self.dim, self.ns = dim, tuple(ns)
         self.domains = [R.Domain(dim, ns, lv) for lv in levels]
         self.coeffs = None
@@ -108,7 +115,8 @@ class GMG:
             self.coeffs = [R.DeviceArray.from_host(np.ascontiguousarray(c, dtype=np.float64)) for c in coeffs]
             cptrs = (C.c_void_p * len(levels))(*[c.ptr for c in self.coeffs])
         o = _Opts(filler_kind, fill_type, op_kind, int(neumann), smoother, neumann_mask, inner_max_it, inner_tol,
-                  int(continue_on_breakdown), pre_sweeps, post_sweeps, coarse_sweeps, precision, int(use_graph))
+                  int(continue_on_breakdown), pre_sweeps, post_sweeps, coarse_sweeps, precision, int(use_graph),
+                  CYCLE_TYPES[cycle_type], mid_sweeps)
         hs = (C.c_void_p * len(self.domains))(*[d.h for d in self.domains])
         h = C.c_void_p()
         R.check(R.lib().te_gmg_create(hs, len(self.domains), C.byref(o), cptrs, C.byref(h)))
@@ -144,6 +152,20 @@ class GMG:
         db = R.DeviceArray.from_host(np.ascontiguousarray(b, dtype=np.float64))
         dx = R.DeviceArray.from_host(np.zeros(self.shape) if x0 is None else np.ascontiguousarray(x0, np.float64))
         its, hist = self.solve_device(db, dx, tol=tol, max_it=max_it, precond=precond, history_cap=max_it + 2)
+        return dx.get(), its, hist
+
+    def cg_device(self, b: R.DeviceArray, x: R.DeviceArray, tol=1e-10, max_it=1000, precond=True, history_cap=0):
+        """Iterative::CG::solve, right-preconditioned by the cycle (precond True) or plain."""
+        its = C.c_int(0)
+        hist = np.full(max(history_cap, 1), np.nan)
+        R.check(R.lib().te_gmg_cg(self.h, b.ptr, x.ptr, int(bool(precond)), max_it, tol, C.byref(its),
+                                  hist.ctypes.data_as(C.POINTER(C.c_double)), history_cap, None))
+        return its.value, hist[: its.value + 1] if history_cap else None
+
+    def cg(self, b: np.ndarray, x0: np.ndarray | None = None, tol=1e-10, max_it=1000, precond=True):
+        db = R.DeviceArray.from_host(np.ascontiguousarray(b, dtype=np.float64))
+        dx = R.DeviceArray.from_host(np.zeros(self.shape) if x0 is None else np.ascontiguousarray(x0, np.float64))
+        its, hist = self.cg_device(db, dx, tol=tol, max_it=max_it, precond=precond, history_cap=max_it + 2)
         return dx.get(), its, hist
 
     def last_solve_seconds(self):
