@@ -39,6 +39,12 @@
  *  - BiCGStab: converges to the dense direct solve of the assembled operator
  *    (plain, preconditioned, corrected early exit); zero right-hand side and a
  *    converged initial guess take 0 iterations (test/Iterative/BiCGStab_MPI1.cpp).
+ *  - W- and F-cycles: equal the WCycle / FMGCycle::visit sequences composed from the
+ *    pinned parts on three levels; V through the general entry equals ora_vcycle_apply;
+ *    W and F coincide on two levels and differ on three; both are linear in f.
+ *  - CG: converges to the dense direct solve (plain, V- and W-preconditioned) on a
+ *    symmetric uniform-mesh operator, the preconditioned ones in fewer iterations; a zero
+ *    right-hand side leaves x at zero (the reference zeroes x before testing ||b||).
  *  Parity unpinned: the Neumann variants (DCT transforms inside the solve, Neumann
  *  physical ghosts) are checked only through the matrices above, and the inner
  *  BiCGStab patch smoother only through its convergence to the patch solve.
