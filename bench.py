@@ -136,6 +136,72 @@ def cpu_baseline(P, gpu_iters, sample_iters=3):
     }
 
 
+def run_config5(args):
+    """Config 5: independent problems solved concurrently (one stream per problem); value =
+    aggregate finest-level DOF x rounds / wall time, every problem to 1e-10, mixed precision,
+    with the FP64 aggregate beside it."""
+    import numpy as np
+
+    from thunderegg_amd import configs
+    from thunderegg_amd import runtime as R
+    from thunderegg_amd import solver
+
+    R.lib()
+    t0 = time.perf_counter()
+    probs = [configs.config5_problem(k) for k in range(args.problems)]
+    t_mesh = time.perf_counter() - t0
+    dof = sum(P.dof for P in probs)
+    log(f"config 5: {len(probs)} problems, {dof} DOF, patch sizes "
+        f"{sorted({P.ns[0] for P in probs})}; meshes {t_mesh:.1f}s")
+
+    def run(prec):
+        gs = [solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, precision=prec,
+                         use_graph=True) for P in probs]
+        bs = [R.DeviceArray.from_host(P.rhs) for P in probs]
+        xs = [R.DeviceArray(g.shape, np.float64) for g in gs]
+
+        def round_():
+            for x in xs:
+                x.zero()
+            R.sync()
+            return solver.solve_many(gs, bs, xs, tol=1e-10, max_it=500, precond=2, threads=args.threads)
+
+        for _ in range(args.warmup):
+            round_()
+        R.sync()
+        t = time.perf_counter()
+        its = None
+        for _ in range(args.steps):
+            its = round_()
+        R.sync()
+        return time.perf_counter() - t, its
+
+    T32, its32 = run(R.F32)
+    T64, its64 = run(R.F64)
+    out = {
+        "metric": "DOF/s to 1e-10 rel residual, FP32-precond+FP64 refine vs FP64 GPU; HBM GB/s",
+        "value": dof * args.steps / T32,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": T32 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle / f64 BiCGStab",
+        "data": "synthetic (seeded meshes and right-hand sides)",
+        "config": {"workload": f"{len(probs)} independent 2D adaptive problems (config 5), solved concurrently",
+                   "config_id": 5, "problems": len(probs), "dof": dof, "threads": args.threads,
+                   "patch_sizes": sorted({P.ns[0] for P in probs})},
+        "fp64_dof_per_s": dof * args.steps / T64,
+        "mixed_over_fp64": T64 / T32,
+        "iterations_mixed": its32,
+        "iterations_fp64": its64,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,11 +213,16 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the standalone kernel timings (for counter runs under rocprofv3)")
+    ap.add_argument("--problems", type=int, default=64, help="config 5: number of problems (512 in full)")
+    ap.add_argument("--threads", type=int, default=8, help="config 5: concurrent solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
                           "n_gpus_requested": args.gpus}))
         sys.exit(2)
+    if args.config == 5:
+        run_config5(args)
+        return
 
     import numpy as np
 

@@ -189,4 +189,40 @@ def config4(ns=(32, 32), seed=SEED0 + 4, min_level=5, max_level=9, contrast=1e3)
                    info={"seed": seed, "contrast": contrast})
 
 
+def config5_problem(k, seed=SEED0 + 5, max_level=11):
+    """Problem k of config 5 (512 independent solves): seed + 1000 k; a quadtree refined
+    level by level around 8 random points (nearest leaves first) until it holds about a
+    patch count drawn log-uniform in [16, 4096]; patch size drawn from {8, 16, 32};
+    BiQuadratic filling, DFT smoother, the Gaussian-sum right-hand side of config 2."""
+    rng = np.random.default_rng(seed + 1000 * k)
+    target = int(round(np.exp(rng.uniform(np.log(16), np.log(4096)))))
+    n = int(rng.choice([8, 16, 32]))
+    pts, amp, wid = _gaussian_points(rng, 8, 2)
+    forest = Forest(2, 2)
+    for lev in range(2, max_level):
+        lv, _ = forest.leaves()
+        if len(lv) >= target:
+            break
+        lo, size = forest.leaf_boxes()
+        c = lo + size[:, None] / 2
+        dist = np.min(np.sqrt(((c[:, None, :] - pts[None, :, :]) ** 2).sum(axis=-1)), axis=1)
+        cand = np.flatnonzero(lv == lev)
+        cand = cand[np.argsort(dist[cand], kind="stable")]
+        take = cand[: max(1, (target - len(lv) + 2) // 3)]  # a refinement adds 3 leaves
+        flag = np.zeros(len(lv), dtype=np.uint8)
+        flag[take] = 1
+        forest.refine(flag)
+    forest.balance()
+    ns = (n, n)
+    levels = forest.build_domains(ns)
+    rhs = _gaussian_rhs(levels, 2, ns, pts, amp, wid)
+    return Problem(f"2D adaptive, {len(levels[0])} patches of {n}x{n}, BiQuadratic/DFT", 2, ns, levels, rhs,
+                   R.BIQUADRATIC, 0, info={"seed": seed + 1000 * k, "target_patches": target})
+
+
+def config5(num=512, seed=SEED0 + 5):
+    """Config 5: `num` independent problems (config5_problem)."""
+    return [config5_problem(k, seed) for k in range(num)]
+
+
 CONFIGS = {1: config1, 2: config2, 3: config3, 4: config4}

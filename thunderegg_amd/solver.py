@@ -170,3 +170,14 @@ class GMG:
 
     def last_solve_seconds(self):
         return R.lib().te_gmg_last_solve_seconds(self.h)
+
+
+def solve_many(solvers, bs, xs, tol=1e-10, max_it=1000, precond=2, threads=8):
+    """Independent solves run concurrently: each GMG owns a stream, and the C calls release
+    the interpreter lock, so up to `threads` solves keep the GPU busy together (config 5's
+    many small problems).  Returns the iteration counts."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        futs = [ex.submit(g.solve_device, b, x, tol, max_it, precond) for g, b, x in zip(solvers, bs, xs)]
+        return [f.result()[0] for f in futs]
