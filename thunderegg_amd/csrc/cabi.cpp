@@ -2,6 +2,7 @@
 // operators, smoothers, transfers and vector algebra.  Each function checks
 // its arguments on the host before anything is enqueued.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -18,6 +19,19 @@
 namespace te {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
+
+int cu_count()
+{
+  static std::atomic<int> cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int v = cus[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cus[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
 }  // namespace te
 
 using namespace te;

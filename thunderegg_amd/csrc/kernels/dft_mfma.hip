@@ -309,9 +309,11 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   // ghosts from the ring table: the link's (with the coarse parents) when interpolating
   const int4* ring = (const int4*)(gm == 2 ? link->d_ring : d->d_ring);
   int ngroups = (d->np + kWaves - 1) / kWaves;
-  // n = 16: about as many workgroups as are resident at once, so each wave solves
-  // several patches and its prefetch of the next one overlaps the current solve
-  const int cap = NT == 1 ? 2048 : 4096;
+  // n = 16: as many workgroups as are resident at once (kWaves waves each, at the
+  // prefetching variant's occupancy: 5 waves per SIMD without interpolation, 4 with), so
+  // each wave solves several patches and its prefetch of the next one overlaps the
+  // current solve
+  const int cap = NT == 1 ? cu_count() * 4 / kWaves * (gm == 0 ? 5 : 4) : 4096;
   int grid = ngroups < cap ? ngroups : cap;
   // the next patch is prefetched where waves get more than one patch each
   const bool pf = NT == 1 && ngroups > cap;

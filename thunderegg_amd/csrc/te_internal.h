@@ -128,6 +128,15 @@ namespace te {
 // issue the neighbour reads of its ghost cell.
 std::vector<int32_t> ring_table(const te_domain* d, const int32_t* parent, const int32_t* orth_on_parent);
 
+// compute units of the current device (queried once per device)
+int cu_count();
+
+// Timings on a stream for the solvers (timer.cpp): a domain timing (domain_id, or INT_MAX
+// for none) started / stopped by events on s.  A null timer makes both no-ops.
+int timer_start_stream(te_timer* t, const char* name, int domain_id, hipStream_t s);
+int timer_stop_stream(te_timer* t, const char* name, int domain_id, hipStream_t s);
+bool timer_has_domain(const te_timer* t, int domain_id);
+
 // ---- launchers (defined in kernels/*.hip) ----
 int launch_ghost_fill(const te_ghost_filler* g, int dtype, void* u, int ncomp, hipStream_t s);
 int launch_star(const te_ghost_filler* g, int dt, int dtf, int dto, int op_kind, int neumann, const void* coeffs,
