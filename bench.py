@@ -215,6 +215,10 @@ def main():
                     help="only the standalone kernel timings (for counter runs under rocprofv3)")
     ap.add_argument("--problems", type=int, default=64, help="config 5: number of problems (512 in full)")
     ap.add_argument("--threads", type=int, default=8, help="config 5: concurrent solves")
+    ap.add_argument("--timer", metavar="FILE",
+                    help="after the timed solves, one more mixed-precision solve with a Timer attached: its "
+                         "timing tree (Iteration, per-level Total Patch Smooth) saved as JSON to FILE and "
+                         "the report printed on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
@@ -288,6 +292,24 @@ def main():
 
         ia, ic = mesh.interior(a, P.dim), mesh.interior(c, P.dim)
         res["mixed_vs_fp64_rel_l2"] = float(np.sqrt(((ia - ic) ** 2).sum() / (ic ** 2).sum()))
+    if args.timer:
+        # outside the timed region: the timings come from events on the solver's stream
+        timer = R.Timer()
+        for k, d in enumerate(g_mixed.domains):
+            timer.add_domain(k, d)
+        g_mixed.set_timer(timer, iterations=True, domain_ids=list(range(len(g_mixed.domains))))
+        xm.zero()
+        g_mixed.solve_device(b, xm, tol=1e-10, max_it=500, precond=2)
+        g_mixed.set_timer(None)
+        timer.save_to_file(args.timer)
+        log(timer.report())
+        tj = timer.to_json()
+        for it in tj["timings"]:
+            log(f"{it['name']}: {it['num_calls']} calls, {it['sum'] * 1e3:.3f} ms")
+            for t in it.get("timings", []):
+                k = t["domain_id"]
+                log(f"  level {k} ({len(P.levels[k])} patches) {t['name']}: {t['num_calls']} calls, "
+                    f"{t['sum'] * 1e3:.3f} ms, {t['sum'] / t['num_calls'] * 1e6:.1f} us each")
     ms = T_mixed / args.steps * 1e3
     value = P.dof * args.steps / T_mixed
     out = {

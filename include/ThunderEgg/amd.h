@@ -68,6 +68,87 @@ struct Deleter {
 };
 }  // namespace detail
 
+/* Timer (Timer.h:28-204) over te_timer.  addDomain takes a Domain (the reference takes
+ * its JSON); to_json becomes json(indent) returning the text.  The *On variants time
+ * device work by events on a stream. */
+class Timer {
+ public:
+  Timer()
+  {
+    te_timer* h = nullptr;
+    detail::check(te_timer_create(&h));
+    h_ = std::shared_ptr<te_timer>(h, detail::Deleter<te_timer, te_timer_destroy>());
+  }
+  template <class Comm>
+  explicit Timer(const Comm&) : Timer()
+  {
+  }
+  void start(const std::string& name) { detail::check(te_timer_start(h_.get(), name.c_str(), TE_TIMER_NONE, TE_TIMER_NONE)); }
+  void stop(const std::string& name) { detail::check(te_timer_stop(h_.get(), name.c_str(), TE_TIMER_NONE, TE_TIMER_NONE)); }
+  template <class Dom>
+  void addDomain(int domain_id, const Dom& domain)
+  {
+    std::array<int, 3> n3{ 1, 1, 1 };
+    for (size_t a = 0; a < domain.getNs().size(); a++) n3[a] = domain.getNs()[a];
+    const auto& pv = domain.getPatchInfoVector();
+    detail::check(te_timer_add_domain(h_.get(), domain_id, (int)domain.getNs().size(), n3.data(), (int)pv.size(),
+                                      pv.data()));
+  }
+  void startDomainTiming(int domain_id, const std::string& name)
+  {
+    detail::check(te_timer_start(h_.get(), name.c_str(), domain_id, TE_TIMER_NONE));
+  }
+  void stopDomainTiming(int domain_id, const std::string& name)
+  {
+    detail::check(te_timer_stop(h_.get(), name.c_str(), domain_id, TE_TIMER_NONE));
+  }
+  void startPatchTiming(int patch_id, int domain_id, const std::string& name)
+  {
+    detail::check(te_timer_start(h_.get(), name.c_str(), domain_id, patch_id));
+  }
+  void stopPatchTiming(int patch_id, int domain_id, const std::string& name)
+  {
+    detail::check(te_timer_stop(h_.get(), name.c_str(), domain_id, patch_id));
+  }
+  void startOn(const std::string& name, te_stream s, int domain_id = TE_TIMER_NONE, int patch_id = TE_TIMER_NONE)
+  {
+    detail::check(te_timer_start_on(h_.get(), name.c_str(), domain_id, patch_id, s));
+  }
+  void stopOn(const std::string& name, te_stream s, int domain_id = TE_TIMER_NONE, int patch_id = TE_TIMER_NONE)
+  {
+    detail::check(te_timer_stop_on(h_.get(), name.c_str(), domain_id, patch_id, s));
+  }
+  void addIntInfo(const std::string& name, int info) { detail::check(te_timer_add_int_info(h_.get(), name.c_str(), info)); }
+  void addDoubleInfo(const std::string& name, double info)
+  {
+    detail::check(te_timer_add_double_info(h_.get(), name.c_str(), info));
+  }
+  /* to_json(json&, const Timer&) as text: compact (indent < 0) or indented */
+  std::string json(int indent = -1) const
+  {
+    size_t n = 0;
+    detail::check(te_timer_to_json(h_.get(), indent, nullptr, 0, &n));
+    std::string out(n + 1, '\0');
+    detail::check(te_timer_to_json(h_.get(), indent, &out[0], out.size(), &n));
+    out.resize(n);
+    return out;
+  }
+  void saveToFile(const std::string& filename) const { detail::check(te_timer_save(h_.get(), filename.c_str())); }
+  friend std::ostream& operator<<(std::ostream& os, const Timer& t)
+  {
+    size_t n = 0;
+    detail::check(te_timer_print(t.h_.get(), nullptr, 0, &n));
+    std::string out(n + 1, '\0');
+    detail::check(te_timer_print(t.h_.get(), &out[0], out.size(), &n));
+    out.resize(n);
+    return os << out;
+  }
+  te_timer* handle() const { return h_.get(); }
+
+ private:
+  std::shared_ptr<te_timer> h_;
+};
+
 /* Single-rank stand-in for the reference's MPI communicator (Communicator.h). */
 class Communicator {
  public:
@@ -109,8 +190,18 @@ class Domain {
   Communicator getCommunicator() const { return {}; }
   te_domain* handle() const { return h_.get(); }
   bool operator==(const Domain& o) const { return h_ == o.h_; }
+  /* Domain::setTimer / getTimer / hasTimer (Domain.h:316-332, json.cpp:357-362); the
+   * timer is shared by every copy of this domain */
+  void setTimer(std::shared_ptr<Timer> timer) const
+  {
+    *timer_ = timer;
+    timer->addDomain(id_, *this);
+  }
+  std::shared_ptr<Timer> getTimer() const { return *timer_; }
+  bool hasTimer() const { return *timer_ != nullptr; }
 
  private:
+  std::shared_ptr<std::shared_ptr<Timer>> timer_{ std::make_shared<std::shared_ptr<Timer>>() };
   int id_ = -1;
   std::array<int, D> ns_{};
   int num_ghost_ = 1;
@@ -414,6 +505,23 @@ class PatchSolver : public virtual Operator<D>, public virtual GMG::Smoother<D> 
   std::shared_ptr<const GhostFiller<D>> filler_;
 };
 
+namespace detail {
+/* PatchSolver::smooth's "Total Patch Smooth" domain timing (PatchSolver.h:153-168), by
+ * events on the null stream the smoother's launches use; stop() ends it */
+template <int D>
+struct SmoothTiming {
+  explicit SmoothTiming(const Domain<D>& d) : d_(d)
+  {
+    if (d_.hasTimer()) d_.getTimer()->startOn("Total Patch Smooth", nullptr, d_.getId());
+  }
+  void stop()
+  {
+    if (d_.hasTimer()) d_.getTimer()->stopOn("Total Patch Smooth", nullptr, d_.getId());
+  }
+  const Domain<D>& d_;
+};
+}  // namespace detail
+
 namespace Poisson {
 /* Poisson/DFTPatchSolver.h:369-450 */
 template <int D>
@@ -437,7 +545,9 @@ class DFTPatchSolver : public PatchSolver<D> {
   DFTPatchSolver* clone() const override { return new DFTPatchSolver(*this); }
   void smooth(const Vector<D>& f, Vector<D>& u) const override
   {
+    detail::SmoothTiming<D> t(this->getDomain());
     detail::check(te_dft_smooth(h_.get(), TE_F64, f.data(), u.data(), nullptr));
+    t.stop();
   }
   const PatchOperator<D>& getOperator() const { return *op_; }
   std::bitset<2 * D> getNeumann() const { return neumann_; }
@@ -526,9 +636,11 @@ class PatchSolver : public ThunderEgg::PatchSolver<D> {
   PatchSolver* clone() const override { return new PatchSolver(*this); }
   void smooth(const Vector<D>& f, Vector<D>& u) const override
   {
+    ThunderEgg::detail::SmoothTiming<D> t(this->getDomain());
     detail::check(te_patch_bicgstab_smooth(op_->deviceFiller().handle(), TE_F64, op_kind_, neumann_, coeffs_,
                                            max_it_, tol_, continue_ ? 1 : 0, f.data(), u.data(), nullptr,
                                            nullptr));
+    t.stop();
   }
   int maxIterations() const { return max_it_; }
   double tolerance() const { return tol_; }
@@ -552,12 +664,16 @@ class CG : public Solver<D> {
   void setTolerance(double t) { tolerance_ = t; }
   double getTolerance() const { return tolerance_; }
   CG* clone() const override { return new CG(*this); }
+  /* "Iteration" timings (Iterative/CG.h) */
+  void setTimer(std::shared_ptr<Timer> timer) { timer_ = timer; }
+  std::shared_ptr<Timer> getTimer() const { return timer_; }
   int solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, const Operator<D>* Mr = nullptr,
             bool output = false, std::ostream& os = std::cout) const override;
 
  private:
   int max_iterations_ = 1000;
   double tolerance_ = 1e-12;
+  std::shared_ptr<Timer> timer_;
 };
 
 /* Iterative/BiCGStab.h:40-203 */
@@ -569,12 +685,16 @@ class BiCGStab : public Solver<D> {
   void setTolerance(double t) { tolerance_ = t; }
   double getTolerance() const { return tolerance_; }
   BiCGStab* clone() const override { return new BiCGStab(*this); }
+  /* "Iteration" timings (Iterative/BiCGStab.h) */
+  void setTimer(std::shared_ptr<Timer> timer) { timer_ = timer; }
+  std::shared_ptr<Timer> getTimer() const { return timer_; }
   int solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, const Operator<D>* Mr = nullptr,
             bool output = false, std::ostream& os = std::cout) const override;
 
  private:
   int max_iterations_ = 1000;
   double tolerance_ = 1e-12;
+  std::shared_ptr<Timer> timer_;
 };
 
 template <int D>
@@ -629,7 +749,10 @@ class Cycle : public Operator<D> {
   void apply(const Vector<D>& f, Vector<D>& u) const override
   {
     if (g_) {
-      detail::check(te_gmg_vcycle(g_.get(), f.data(), u.data(), nullptr));
+      attachTimers(nullptr);
+      int rc = te_gmg_vcycle(g_.get(), f.data(), u.data(), nullptr);
+      detachTimers();
+      detail::check(rc);
       return;
     }
     u.setWithGhost(0.0);
@@ -637,6 +760,23 @@ class Cycle : public Operator<D> {
   }
   /* the fused device cycle, or NULL when a component is user-defined */
   te_gmg* device() const { return g_.get(); }
+  /* Attach to the device cycle the timer of a solver (its "Iteration" timings; may be
+   * null) and, for the levels whose domain carries the same timer (or, without a solver
+   * timer, the first timer found on a level), the levels' sweep timings; detachTimers
+   * undoes it. */
+  void attachTimers(const std::shared_ptr<Timer>& solver_timer) const
+  {
+    std::shared_ptr<Timer> t = solver_timer;
+    for (size_t i = 0; !t && i < domains_.size(); i++) t = domains_[i].getTimer();
+    if (!t || !g_) return;
+    std::vector<int> ids;
+    for (const Domain<D>& d : domains_) ids.push_back(d.getTimer() == t ? d.getId() : TE_TIMER_NONE);
+    detail::check(te_gmg_set_timer(g_.get(), t->handle(), solver_timer == t ? 1 : 0, ids.data()));
+  }
+  void detachTimers() const
+  {
+    if (g_) te_gmg_set_timer(g_.get(), nullptr, 0, nullptr);
+  }
   const Operator<D>& finestOperator() const { return *levels_[0].op; }
   int numLevels() const { return (int)levels_.size(); }
 
@@ -675,6 +815,7 @@ class Cycle : public Operator<D> {
   std::vector<Level<D>> levels_;
   std::shared_ptr<te_gmg> g_;
   std::vector<Vector<D>> coeffs_;  // keeps VarPoisson coefficients alive
+  std::vector<Domain<D>> domains_;  // the device cycle's levels (for their timers)
 };
 
 template <int D>
@@ -732,6 +873,7 @@ void Cycle<D>::build_device_cycle()
     doms.push_back(po->getDomain().handle());
     coeffs.push_back(c);
   }
+  for (const Level<D>& L : levels_) domains_.push_back(dynamic_cast<const PatchOperator<D>*>(L.op.get())->getDomain());
   o.filler_kind = filler_kind;
   o.fill_type = fill_type;
   o.op_kind = op_kind;
@@ -837,8 +979,11 @@ int BiCGStab<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, c
     if (same_operator(A, cyc->finestOperator()) && b.getNumComponents() == 1) {
       std::vector<double> hist((size_t)max_iterations_ + 2, 0.0);
       int its = 0;
-      detail::check(te_gmg_bicgstab(cyc->device(), b.data(), x.data(), 1, max_iterations_, tolerance_, &its,
-                                    hist.data(), (int)hist.size(), nullptr));
+      cyc->attachTimers(timer_);
+      int rc = te_gmg_bicgstab(cyc->device(), b.data(), x.data(), 1, max_iterations_, tolerance_, &its, hist.data(),
+                               (int)hist.size(), nullptr);
+      cyc->detachTimers();
+      detail::check(rc);
       if (output)
         for (int k = 0; k <= its && k < (int)hist.size(); k++) detail_bicg::log_line(os, k, hist[(size_t)k]);
       return its;
@@ -861,6 +1006,7 @@ int BiCGStab<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, c
   if (output) detail_bicg::log_line(os, num_its, residual);
   if (r0_norm == 0) return num_its;
   while (residual > tolerance_ && num_its < max_iterations_) {
+    if (timer_) timer_->start("Iteration");
     if (rho == 0) throw BreakdownError("BiCGStab broke down, rho was 0 on iteration " + std::to_string(num_its));
     if (Mr) {
       Mr->apply(p, mp);
@@ -873,6 +1019,7 @@ int BiCGStab<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, c
     s.addScaled(-alpha, ap);
     if (s.twoNorm() / r0_norm <= tolerance_) {
       x.addScaled(alpha, p);  // the reference's early exit (Iterative/BiCGStab.h:162-167)
+      if (timer_) timer_->stop("Iteration");
       break;
     }
     if (Mr) {
@@ -893,6 +1040,7 @@ int BiCGStab<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, c
     rho = rho_new;
     residual = resid.twoNorm() / r0_norm;
     if (output) detail_bicg::log_line(os, num_its, residual);
+    if (timer_) timer_->stop("Iteration");
   }
   return num_its;
 }
@@ -906,8 +1054,11 @@ int CG<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, const O
   if (cyc && cyc->device() && same_operator(A, cyc->finestOperator()) && b.getNumComponents() == 1) {
     std::vector<double> hist((size_t)max_iterations_ + 2, 0.0);
     int its = 0;
-    detail::check(te_gmg_cg(cyc->device(), b.data(), x.data(), 1, max_iterations_, tolerance_, &its, hist.data(),
-                            (int)hist.size(), nullptr));
+    cyc->attachTimers(timer_);
+    int rc = te_gmg_cg(cyc->device(), b.data(), x.data(), 1, max_iterations_, tolerance_, &its, hist.data(),
+                       (int)hist.size(), nullptr);
+    cyc->detachTimers();
+    detail::check(rc);
     if (output)
       for (int k = 0; k <= its && k < (int)hist.size(); k++) detail_bicg::log_line(os, k, hist[(size_t)k]);
     return its;
@@ -926,6 +1077,7 @@ int CG<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, const O
   double residual = resid.twoNorm() / r0_norm;
   if (output) detail_bicg::log_line(os, num_its, residual);
   while (residual > tolerance_ && num_its < max_iterations_) {
+    if (timer_) timer_->start("Iteration");
     if (rho == 0) throw BreakdownError("CG broke down, rho was 0 on iteration " + std::to_string(num_its));
     if (Mr) {
       Mr->apply(p, tmp);
@@ -943,6 +1095,7 @@ int CG<D>::solve(const Operator<D>& A, Vector<D>& x, const Vector<D>& b, const O
     rho = rho_new;
     residual = resid.twoNorm() / r0_norm;
     if (output) detail_bicg::log_line(os, num_its, residual);
+    if (timer_) timer_->stop("Iteration");
   }
   if (Mr) {
     Mr->apply(x, resid);

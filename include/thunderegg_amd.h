@@ -288,6 +288,56 @@ int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it
 /* Seconds spent in the last te_gmg_bicgstab / te_gmg_cg, from the first A x to convergence. */
 double te_gmg_last_solve_seconds(const te_gmg* g);
 
+/* ------------------------------------------------------------- timings --
+ * Timer (Timer.h:28-204, Timer.cpp:160-781): nested named timings, each optionally tied to
+ * a domain id and a patch id (TE_TIMER_NONE = none, the reference's INT_MAX), with call
+ * count, sum, min and max in seconds and named int / double infos.  te_timer_start /
+ * te_timer_stop time on the host clock (Timer::start / stop, startDomainTiming,
+ * startPatchTiming and their stops, with name, domain id and patch id in one call);
+ * the _on variants record HIP events on a stream instead, read when the timer is
+ * serialized or printed.  Errors follow the reference's RuntimeErrors: a stop that does not
+ * match the innermost started timing, a domain id that was not added, an info with no
+ * timing started or of the other type, adding a domain id twice.  A timer is not
+ * thread-safe. */
+#define TE_TIMER_NONE 2147483647
+typedef struct te_timer te_timer;
+int te_timer_create(te_timer** out);
+int te_timer_destroy(te_timer* t);
+/* Timer::addDomain(domain_id, domain) (Timer.h:71): the domain's patches (dim 2 or 3, ns
+ * cells per side) serialized in the reference's mesh schema (json.cpp:161-247). */
+int te_timer_add_domain(te_timer* t, int domain_id, int dim, const int* ns, int num_patches,
+                        const te_patch_info* patches);
+int te_timer_start(te_timer* t, const char* name, int domain_id, int patch_id);
+int te_timer_stop(te_timer* t, const char* name, int domain_id, int patch_id);
+int te_timer_start_on(te_timer* t, const char* name, int domain_id, int patch_id, te_stream stream);
+int te_timer_stop_on(te_timer* t, const char* name, int domain_id, int patch_id, te_stream stream);
+/* Timer::addIntInfo / addDoubleInfo (Timer.h:120-145): to the innermost started timing. */
+int te_timer_add_int_info(te_timer* t, const char* name, int info);
+int te_timer_add_double_info(te_timer* t, const char* name, double info);
+/* to_json(json&, const Timer&) (Timer.cpp:700-745) as text: indent < 0 compact, else
+ * pretty-printed with that many spaces; "null" for an empty timer.  *len = the text's length;
+ * with buf != NULL the text and a terminating 0 are copied when cap > *len (else
+ * TE_ERR_ARG). */
+int te_timer_to_json(te_timer* t, int indent, char* buf, size_t cap, size_t* len);
+/* operator<<(ostream&, const Timer&) (Timer.cpp:608-640): the text report, sibling timings
+ * merged by name; buffer protocol of te_timer_to_json.  Fails with unfinished timings. */
+int te_timer_print(te_timer* t, char* buf, size_t cap, size_t* len);
+/* Timer::saveToFile (Timer.cpp:747-759): the JSON, indented by 4. */
+int te_timer_save(te_timer* t, const char* filename);
+/* Attach a timer to a solver (NULL detaches).  iterations != 0: te_gmg_bicgstab and
+ * te_gmg_cg time each outer iteration as "Iteration" (Iterative/BiCGStab.h:145-200,
+ * Iterative/CG.h:158-186).  domain_ids (num_levels entries, or NULL for none): every
+ * smoothing sweep of level l is timed as the domain timing "Total Patch Smooth" of
+ * domain_ids[l] (PatchSolver.h:153-168), which must have been added to the timer;
+ * TE_TIMER_NONE leaves a level untimed.  Timings are recorded by events on the solver's
+ * stream, with its launches enqueued one by one instead of as graphs; the arithmetic is
+ * unchanged.  On the fused 2D path a sweep's launch also does the transfer fused into it
+ * (interpolation into the first post-smoothing sweep, the finer level's residual and
+ * restriction into the first sweep of a level of at most 1024 patches), so its timing
+ * includes that work; the reference's per-patch "Single Patch Solve" timings have no
+ * counterpart (a sweep solves all patches of the level in one launch). */
+int te_gmg_set_timer(te_gmg* g, te_timer* t, int iterations, const int* domain_ids);
+
 /* ----------------------------------------------- mesh generation (host) --
  * Quadtree / octree forest on the unit square / cube, replacing
  * P4estDomainGenerator / P8estDomainGenerator (P4estDomainGenerator.cpp:205-738):

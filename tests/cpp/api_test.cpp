@@ -125,6 +125,39 @@ int main()
   CHECK(log1.str().substr(0, 6) == "    0 ");
   std::printf("%s", log1.str().c_str());
 
+  // Timer (Timer.h): the solver's "Iteration" timings and the levels' "Total Patch Smooth"
+  // domain timings, on the device cycle (same arithmetic as without a timer) and on the
+  // generic one
+  {
+    auto timer = std::make_shared<Timer>(Communicator());
+    for (auto& d : doms) d.setTimer(timer);
+    bool threw = false;
+    try {
+      doms[0].setTimer(timer);  // the domain id is already added
+    } catch (const RuntimeError&) {
+      threw = true;
+    }
+    CHECK(threw);
+    Iterative::BiCGStab<2> ts;
+    ts.setTolerance(1e-10);
+    ts.setTimer(timer);
+    CHECK(ts.getTimer() == timer);
+    Vector<2> xt(doms[0], 1), xg(doms[0], 1);
+    int itt = ts.solve(A, xt, f, dev.get());
+    CHECK(itt == it1);
+    CHECK(rel_diff(xt, x1) == 0.0);
+    CHECK(ts.solve(A, xg, f, gen_cycle.get()) == it2);
+    const std::string js = timer->json();
+    std::ostringstream rep;
+    rep << *timer;
+    CHECK(js.find("\"Iteration\"") != std::string::npos);
+    CHECK(js.find("\"Total Patch Smooth\"") != std::string::npos);
+    CHECK(rep.str().find("Iteration -> Total Patch Smooth") != std::string::npos);
+    std::printf("%s", rep.str().c_str());
+    Domain<2> copy(doms[1]);  // copies share the timer
+    CHECK(copy.getTimer() == timer && copy.hasTimer());
+  }
+
   // mixed precision cycle reaches the tolerance in the true residual
   GMG::CycleOpts mopts;
   mopts.precision = "mixed";

@@ -18,6 +18,7 @@ SOURCES = [
     "cabi.cpp",
     "gmg.cpp",
     "forest.cpp",
+    "timer.cpp",
     "kernels/ghost.hip",
     "kernels/stencil.hip",
     "kernels/transfer.hip",

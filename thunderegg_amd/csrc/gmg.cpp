@@ -80,6 +80,11 @@ struct te_gmg {
   float* ms32 = nullptr;
   float* p32 = nullptr;
   float* s32 = nullptr;
+  // te_gmg_set_timer: "Iteration" timings (time_its) and per-level "Total Patch Smooth"
+  // domain timings (timer_ids[level], TE_TIMER_NONE = untimed), events on s
+  te_timer* timer = nullptr;
+  int time_its = 0;
+  std::vector<int> timer_ids;
 };
 
 namespace {
@@ -95,10 +100,29 @@ int zero_with_ghost(te_gmg* g, int li, void* v, int dt)
   return launch_vec_op(g->L[li].d, dt, VOP_SET_WITH_GHOST, v, nullptr, nullptr, 0.0, 0, 0, 1, g->s);
 }
 
+// start (stop = false) or stop the "Total Patch Smooth" timing of level li's sweep
+// (PatchSolver.h:153-168) when a timer is attached and the level is timed
+int sweep_timing(te_gmg* g, int li, bool stop)
+{
+  if (!g->timer || g->timer_ids.empty() || g->timer_ids[li] == TE_TIMER_NONE) return TE_OK;
+  const char* name = "Total Patch Smooth";
+  return stop ? timer_stop_stream(g->timer, name, g->timer_ids[li], g->s)
+              : timer_start_stream(g->timer, name, g->timer_ids[li], g->s);
+}
+
+int smooth_sweep(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void* out, int dto);
+
 // One smoothing sweep on level li: PatchSolver::smooth (PatchSolver.h:142-170).
 // uzero: u is known to be zero (fresh coarse-level vector).  out/dto: where the
 // new u goes (== u except for the finest level's last mixed-precision sweep).
 int smooth(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void* out, int dto)
+{
+  TE_RC(sweep_timing(g, li, false));
+  TE_RC(smooth_sweep(g, li, f, dtf, u, uzero, out, dto));
+  return sweep_timing(g, li, true);
+}
+
+int smooth_sweep(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void* out, int dto)
 {
   GLevel& Lv = g->L[li];
   if (g->o.smoother == 0) {
@@ -196,9 +220,10 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
       uc = g->L[li + 1].cur;
       dtc = g->L[li + 1].dt;
     }
-    int rc = launch_dft_mfma_fused(Lv.plan, Lv.dt, dtf, dto, f, buf[cur], out, mode == 0, mode == 0 ? 0 : mode,
-                                   dtc, uc, mode == 2 ? Lv.link : nullptr, g->s);
-    if (rc) return rc;
+    TE_RC(sweep_timing(g, li, false));
+    TE_RC(launch_dft_mfma_fused(Lv.plan, Lv.dt, dtf, dto, f, buf[cur], out, mode == 0, mode == 0 ? 0 : mode, dtc,
+                                uc, mode == 2 ? Lv.link : nullptr, g->s));
+    TE_RC(sweep_timing(g, li, true));
     if (to_ext) {
       Lv.cur = u_ext;
     } else {
@@ -227,8 +252,10 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
   const int next_first = li + 1 == nl - 1 ? g->o.coarse_sweeps : g->o.pre_sweeps;
   if (next_first > 0 && g->rr_pre[li]) {
     // Cycle::restrict and the coarser level's first sweep (from zero) in one launch
+    TE_RC(sweep_timing(g, li + 1, false));
     TE_RC(launch_rr_pre(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur], Cv.plan,
                         Cv.dt, Cv.f, Cv.u, g->s));
+    TE_RC(sweep_timing(g, li + 1, true));
     Cv.cur = Cv.u;
     TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext, ext_dt, true));
   } else {
@@ -270,7 +297,7 @@ int vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt = TE_F64, int dtf 
 
 int vcycle_launch(te_gmg* g, const double* f, double* u)
 {
-  if (!g->o.use_graph) return vcycle_enqueue(g, f, u);
+  if (!g->o.use_graph || g->timer) return vcycle_enqueue(g, f, u);
   auto key = std::make_pair((const void*)f, (const void*)u);
   auto it = g->vgraphs.find(key);
   if (it == g->vgraphs.end()) {
@@ -585,7 +612,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
                        nullptr, s, TE_F64, m32 ? g->p32 : nullptr);
   };
   hipGraphExec_t gx = nullptr;
-  if (g->o.use_graph) {
+  if (g->o.use_graph && !g->timer) {
     auto key = std::make_tuple((const void*)b, (const void*)x, use_precond);
     auto it = g->bgraphs.find(key);
     if (it == g->bgraphs.end()) {
@@ -609,6 +636,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     }
     gx = it->second;
   }
+  const bool time_its = g->timer && g->time_its;
   auto t0 = std::chrono::steady_clock::now();
   int rc = TE_OK;
   int num_its = 0;
@@ -653,8 +681,10 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
         rc = TE_ERR_BREAKDOWN;
         break;
       }
+      if (time_its) TE_RC(timer_start_stream(g->timer, "Iteration", TE_TIMER_NONE, s));
       if (gx) TE_HIP(hipGraphLaunch(gx, s));
       else TE_RC(iteration());
+      if (time_its) TE_RC(timer_stop_stream(g->timer, "Iteration", TE_TIMER_NONE, s));
       seq += 1.0;
       TE_RC(wait_published(seq));
       if (h[SC_STATE] == ST_EARLY) break;  // x was updated on the device
@@ -708,6 +738,7 @@ int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it
     TE_RC(launch_ghost_fill(gf, TE_F64, u, 1, s));
     return launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, nullptr, u, out, 1, s);
   };
+  const bool time_its = g->timer && g->time_its;
   auto t0 = std::chrono::steady_clock::now();
   int num_its = 0, rc = TE_OK;
   // resid = b - A x (A.apply then scaleThenAdd(-1, b), as one residual launch)
@@ -729,6 +760,7 @@ int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it
         rc = TE_ERR_BREAKDOWN;
         break;
       }
+      if (time_its) TE_RC(timer_start_stream(g->timer, "Iteration", TE_TIMER_NONE, s));
       if (use_precond) {
         TE_RC(vcycle_enqueue(g, p, tmp));
         TE_RC(apply_a(tmp, ap));
@@ -749,6 +781,7 @@ int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it
       TE_RC(reduce(RED_SUMSQ, resid, nullptr, &nr));
       residual = nr / r0;
       if (history && num_its < history_cap) history[num_its] = residual;
+      if (time_its) TE_RC(timer_stop_stream(g->timer, "Iteration", TE_TIMER_NONE, s));
     }
     if (!rc && use_precond) {
       TE_RC(vcycle_enqueue(g, x, tmp));  // Mr->apply(x, resid); x.copy(resid)
@@ -764,5 +797,27 @@ int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it
 }
 
 double te_gmg_last_solve_seconds(const te_gmg* g) { return g ? g->last_seconds : -1.0; }
+
+int te_gmg_set_timer(te_gmg* g, te_timer* t, int iterations, const int* domain_ids)
+{
+  if (!g) {
+    set_error("te_gmg_set_timer: bad argument");
+    return TE_ERR_ARG;
+  }
+  std::vector<int> ids;
+  if (t && domain_ids) {
+    ids.assign(domain_ids, domain_ids + g->L.size());
+    for (int id : ids) {
+      if (id != TE_TIMER_NONE && !timer_has_domain(t, id)) {
+        set_error("Domain with id " + std::to_string(id) + " was not added to timer");
+        return TE_ERR_ARG;
+      }
+    }
+  }
+  g->timer = t;
+  g->time_its = t && iterations;
+  g->timer_ids = ids;
+  return TE_OK;
+}
 
 }  // extern "C"

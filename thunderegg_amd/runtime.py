@@ -91,6 +91,19 @@ SIGNATURES = {
     "te_gmg_bicgstab": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp, C.c_int, _vp]),
     "te_gmg_cg": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp, C.c_int, _vp]),
     "te_gmg_last_solve_seconds": (C.c_double, [_vp]),
+    "te_gmg_set_timer": (C.c_int, [_vp, _vp, C.c_int, _ip]),
+    "te_timer_create": (C.c_int, [C.POINTER(_vp)]),
+    "te_timer_destroy": (C.c_int, [_vp]),
+    "te_timer_add_domain": (C.c_int, [_vp, C.c_int, C.c_int, _ip, C.c_int, _vp]),
+    "te_timer_start": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int]),
+    "te_timer_stop": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int]),
+    "te_timer_start_on": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int, _vp]),
+    "te_timer_stop_on": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int, _vp]),
+    "te_timer_add_int_info": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "te_timer_add_double_info": (C.c_int, [_vp, C.c_char_p, C.c_double]),
+    "te_timer_to_json": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "te_timer_print": (C.c_int, [_vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "te_timer_save": (C.c_int, [_vp, C.c_char_p]),
     "te_forest_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
     "te_forest_destroy": (C.c_int, [_vp]),
     "te_forest_num_leaves": (C.c_int, [_vp]),
@@ -381,6 +394,98 @@ class Event:
         if getattr(self, "h", None):
             lib().te_event_destroy(self.h)
             self.h = None
+
+
+TIMER_NONE = 2147483647  # no domain / no patch (the reference's INT_MAX)
+
+
+class Timer:
+    """``Timer`` (reference ``Timer.h:28-204``) over te_timer.
+
+    Timings without a stream use the host clock, as the reference does; with
+    ``stream=`` (a Stream, a raw handle, or 0 for the null stream) they are timed by
+    HIP events recorded on that stream and read when the timer is serialized or
+    printed.  Errors raise TEError with the reference's messages.
+    """
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().te_timer_create(C.byref(h)))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().te_timer_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def _s(stream):
+        return stream.h if isinstance(stream, Stream) else stream
+
+    def _go(self, begin, name, domain_id, patch_id, stream):
+        n = name.encode()
+        if stream is None:
+            fn = lib().te_timer_start if begin else lib().te_timer_stop
+            check(fn(self.h, n, domain_id, patch_id))
+        else:
+            fn = lib().te_timer_start_on if begin else lib().te_timer_stop_on
+            check(fn(self.h, n, domain_id, patch_id, self._s(stream)))
+
+    def add_domain(self, domain_id: int, domain):
+        """Timer::addDomain: the domain's patches in the reference's mesh schema.  domain:
+        a Domain, or anything with its dim, ns and patches (PATCH_DTYPE records)."""
+        patches = np.ascontiguousarray(domain.patches, dtype=mesh.PATCH_DTYPE)
+        nsa = (C.c_int * 3)(*(list(domain.ns) + [1] * (3 - domain.dim)))
+        check(lib().te_timer_add_domain(self.h, domain_id, domain.dim, nsa, len(patches), patches.ctypes.data))
+
+    def start(self, name, stream=None):
+        self._go(True, name, TIMER_NONE, TIMER_NONE, stream)
+
+    def stop(self, name, stream=None):
+        self._go(False, name, TIMER_NONE, TIMER_NONE, stream)
+
+    def start_domain_timing(self, domain_id, name, stream=None):
+        self._go(True, name, domain_id, TIMER_NONE, stream)
+
+    def stop_domain_timing(self, domain_id, name, stream=None):
+        self._go(False, name, domain_id, TIMER_NONE, stream)
+
+    def start_patch_timing(self, patch_id, domain_id, name, stream=None):
+        self._go(True, name, domain_id, patch_id, stream)
+
+    def stop_patch_timing(self, patch_id, domain_id, name, stream=None):
+        self._go(False, name, domain_id, patch_id, stream)
+
+    def add_int_info(self, name, info: int):
+        check(lib().te_timer_add_int_info(self.h, name.encode(), int(info)))
+
+    def add_double_info(self, name, info: float):
+        check(lib().te_timer_add_double_info(self.h, name.encode(), float(info)))
+
+    def _text(self, fn, *args) -> str:
+        n = C.c_size_t()
+        check(fn(self.h, *args, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(fn(self.h, *args, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
+    def dumps(self, indent: int = -1) -> str:
+        """The JSON text (to_json): compact, or indented by ``indent`` spaces."""
+        return self._text(lib().te_timer_to_json, indent)
+
+    def to_json(self):
+        import json
+
+        return json.loads(self.dumps())
+
+    def report(self) -> str:
+        """operator<<: the text report."""
+        return self._text(lib().te_timer_print)
+
+    __str__ = report
+
+    def save_to_file(self, filename: str):
+        check(lib().te_timer_save(self.h, filename.encode()))
 
 
 def device_count():

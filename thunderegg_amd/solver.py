@@ -171,6 +171,19 @@ class GMG:
     def last_solve_seconds(self):
         return R.lib().te_gmg_last_solve_seconds(self.h)
 
+    def set_timer(self, timer: "R.Timer | None", iterations=True, domain_ids=None):
+        """Attach a Timer (None detaches): "Iteration" timings of the outer solves
+        (iterations=True) and "Total Patch Smooth" domain timings of every sweep of the
+        levels whose entry in domain_ids (one per level, TIMER_NONE = untimed; None: no
+        level timings) names a domain added to the timer.  Keeps a reference to the timer."""
+        ids = None
+        if timer is not None and domain_ids is not None:
+            if len(domain_ids) != len(self.domains):
+                raise ValueError("domain_ids needs one entry per level")
+            ids = (C.c_int * len(domain_ids))(*[int(v) for v in domain_ids])
+        R.check(R.lib().te_gmg_set_timer(self.h, timer.h if timer is not None else None, int(bool(iterations)), ids))
+        self._timer = timer
+
 
 def solve_many(solvers, bs, xs, tol=1e-10, max_it=1000, precond=2, threads=8):
     """Independent solves run concurrently: each GMG owns a stream, and the C calls release
