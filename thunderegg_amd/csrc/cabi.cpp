@@ -7,7 +7,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -30,6 +32,27 @@ int cu_count()
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
     cus[dev].store(v, std::memory_order_relaxed);
   }
+  return v;
+}
+
+int resident_blocks(const void* kernel, int block, size_t shmem)
+{
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void*, int, size_t>, int> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  auto key = std::make_tuple(dev, kernel, block, shmem);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, shmem) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const int v = per_cu * cu_count();
+  std::lock_guard<std::mutex> lk(mu);
+  cache.emplace(key, v);
   return v;
 }
 }  // namespace te

@@ -142,7 +142,8 @@ struct WideAcc {
   __device__ __forceinline__ T operator()(int q, int x, int y) const { return T(at(u, q * L.ps + (y + 1) * L.sx + x + 1)); }
 };
 
-// out = A u (ghost cells of out written 0); pa[block] = sum w * out, pb[block] = sum out * out.
+// out = A u (ghost cells of out written 0); pa[block] = sum w * out, pb[block] = sum out * out,
+// and 0 in the slots from gridDim.x to nslots (the partial count the finalize folds).
 // TU: type of u (FP32 values are widened to FP64 as they are read: the arithmetic is FP64).
 template <int OP, int KIND, bool SQ, bool P16, typename TU>
 __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __restrict__ dp,
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
                                                            const double* __restrict__ cf, double* __restrict__ out,
                                                            const double* __restrict__ wv, double* __restrict__ pa,
                                                            double* __restrict__ pb, Layout L, int neumann,
-                                                           const int4* __restrict__ ring)
+                                                           const int4* __restrict__ ring, int nslots)
 {
   if (P16) {  // compile-time layout constants: the address arithmetic folds
     L.n[0] = L.n[1] = 16;
@@ -243,6 +244,12 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
     r = block_reduce<RED_SUM>(sq);
     if (threadIdx.x == 0) pb[blockIdx.x] = r;
   }
+  if (blockIdx.x == 0) {
+    for (int i = (int)gridDim.x + (int)threadIdx.x; i < nslots; i += (int)blockDim.x) {
+      pa[i] = 0.0;
+      if (SQ) pb[i] = 0.0;
+    }
+  }
 }
 
 size_t tile_bytes(const te_domain* d, size_t esz) { return esz * kWaves * (d->ns[0] + 2) * (d->ns[1] + 2); }
@@ -256,14 +263,17 @@ void launch_rr(const te_ghost_filler* g, const te_level_link* link, int op_kind,
   size_t shmem = tile_bytes(d, sizeof(T));
   const bool p16 = d->ns[0] == 16 && d->ns[1] == 16;
   int ngroups = (d->np + kWaves - 1) / kWaves;
-  // 16 x 16: about as many workgroups as are resident at once, so that each wave takes
-  // several patches and its loads of the next one overlap the current one
-  const int cap = p16 ? 2048 : 8192;
-  int grid = ngroups < cap ? ngroups : cap;
-#define TE_RR(OP_, K_, P_)                                                                                    \
-  hipLaunchKernelGGL((resid_restrict_2d<T, TF, TC, OP_, K_, P_>), dim3(grid), dim3(64 * kWaves), shmem, s,   \
-                     d->d_patches, (const T*)u, (const T*)coeffs, (const TF*)f, (TC*)fc, link->d_parent,     \
-                     link->d_orth, L, neumann, (const int4*)d->d_ring)
+  // 16 x 16: as many workgroups as are resident at once, so that each wave takes several
+  // patches (all in one round) and its loads of the next one overlap the current one
+  auto grid_of = [&](const void* k) {
+    const int cap = p16 ? resident_blocks(k, 64 * kWaves, shmem) : 8192;
+    return ngroups < cap ? ngroups : cap;
+  };
+#define TE_RR(OP_, K_, P_)                                                                                   \
+  hipLaunchKernelGGL((resid_restrict_2d<T, TF, TC, OP_, K_, P_>),                                            \
+                     dim3(grid_of((const void*)resid_restrict_2d<T, TF, TC, OP_, K_, P_>)), dim3(64 * kWaves), \
+                     shmem, s, d->d_patches, (const T*)u, (const T*)coeffs, (const TF*)f, (TC*)fc,            \
+                     link->d_parent, link->d_orth, L, neumann, (const int4*)d->d_ring)
 #define TE_RRP(OP_, K_)             \
   do {                              \
     if (p16) TE_RR(OP_, K_, true);  \
@@ -333,15 +343,24 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
   Layout L = make_layout(2, d->ns, 1, d->np);
   size_t shmem = tile_bytes(d, sizeof(double));
   const bool p16 = d->ns[0] == 16 && d->ns[1] == 16;
-#define TE_SD(OP_, K_, Q_, P_)                                                                                      \
-  do {                                                                                                             \
-    if (udt == TE_F32)                                                                                             \
-      hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_, float>), dim3(grid), dim3(64 * kWaves), shmem, s,           \
-                         d->d_patches, (const float*)u, coeffs, out, w, pa, pb, L, neumann, (const int4*)d->d_ring); \
-    else                                                                                                           \
-      hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_, double>), dim3(grid), dim3(64 * kWaves), shmem, s,          \
-                         d->d_patches, (const double*)u, coeffs, out, w, pa, pb, L, neumann,                       \
-                         (const int4*)d->d_ring);                                                                  \
+  // as many workgroups as are resident at once (each wave takes several patches, all in one
+  // round), at most the `grid` partial slots
+  auto fit = [&](const void* k) {
+    const int r = resident_blocks(k, 64 * kWaves, shmem);
+    return r < grid ? r : grid;
+  };
+#define TE_SD(OP_, K_, Q_, P_)                                                                                  \
+  do {                                                                                                         \
+    if (udt == TE_F32)                                                                                         \
+      hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_, float>),                                                \
+                         dim3(fit((const void*)star_dot_2d<OP_, K_, Q_, P_, float>)), dim3(64 * kWaves), shmem, \
+                         s, d->d_patches, (const float*)u, coeffs, out, w, pa, pb, L, neumann,                  \
+                         (const int4*)d->d_ring, grid);                                                         \
+    else                                                                                                       \
+      hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_, double>),                                               \
+                         dim3(fit((const void*)star_dot_2d<OP_, K_, Q_, P_, double>)), dim3(64 * kWaves),      \
+                         shmem, s, d->d_patches, (const double*)u, coeffs, out, w, pa, pb, L, neumann,          \
+                         (const int4*)d->d_ring, grid);                                                         \
   } while (0)
 #define TE_SDQ(OP_, K_)                      \
   do {                                       \

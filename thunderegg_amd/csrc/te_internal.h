@@ -130,6 +130,9 @@ std::vector<int32_t> ring_table(const te_domain* d, const int32_t* parent, const
 
 // compute units of the current device (queried once per device)
 int cu_count();
+// workgroups of `block` threads and `shmem` dynamic LDS bytes of `kernel` resident at once
+// on the whole current device (occupancy query, cached)
+int resident_blocks(const void* kernel, int block, size_t shmem);
 
 // Timings on a stream for the solvers (timer.cpp): a domain timing (domain_id, or INT_MAX
 // for none) started / stopped by events on s.  A null timer makes both no-ops.
