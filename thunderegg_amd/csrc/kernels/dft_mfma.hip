@@ -308,27 +308,29 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   FusedArgs<T, TC> fa{ uc, link ? link->d_parent : nullptr, link ? link->d_orth : nullptr };
   // ghosts from the ring table: the link's (with the coarse parents) when interpolating
   const int4* ring = (const int4*)(gm == 2 ? link->d_ring : d->d_ring);
-  int ngroups = (d->np + kWaves - 1) / kWaves;
-  // n = 16: as many workgroups as are resident at once (kWaves waves each, at the
-  // prefetching variant's occupancy: 5 waves per SIMD without interpolation, 4 with), so
-  // each wave solves several patches and its prefetch of the next one overlaps the
-  // current solve
-  const int cap = NT == 1 ? cu_count() * 4 / kWaves * (gm == 0 ? 5 : 4) : 4096;
-  int grid = ngroups < cap ? ngroups : cap;
-  // the next patch is prefetched where waves get more than one patch each
-  const bool pf = NT == 1 && ngroups > cap;
+  const int ngroups = (d->np + kWaves - 1) / kWaves;
   const int kind = pl->g->kind;
-#define TE_MK1(OP_, UZ_, GM_, K_, PF_)                                                                          \
-  hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_, PF_>), dim3(grid), dim3(64 * kWaves), \
+  // n = 16: when every workgroup (one patch per wave) is resident at once, one round of
+  // the plain variant; else the prefetching variant (the next patch loaded during the
+  // current solve) on as many workgroups as are resident at once, each wave solving
+  // several patches.  n = 32: at most 4096 workgroups.
+#define TE_MK1(OP_, UZ_, GM_, K_, PF_, G_)                                                                    \
+  hipLaunchKernelGGL((dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_, PF_>), dim3(G_), dim3(64 * kWaves), \
                      shmem, s, d->d_patches, mats, s2, f, u, cf, out, L, fa, ring)
-#define TE_MK(OP_, UZ_, GM_, K_)                      \
-  do {                                                \
-    if constexpr (NT == 1) {                          \
-      if (pf) TE_MK1(OP_, UZ_, GM_, K_, true);        \
-      else TE_MK1(OP_, UZ_, GM_, K_, false);          \
-    } else {                                          \
-      TE_MK1(OP_, UZ_, GM_, K_, false);               \
-    }                                                 \
+#define TE_MK(OP_, UZ_, GM_, K_)                                                                            \
+  do {                                                                                                      \
+    if constexpr (NT == 1) {                                                                                \
+      const void* plain = (const void*)dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_, false>;        \
+      if (ngroups <= resident_blocks(plain, 64 * kWaves, shmem)) {                                          \
+        TE_MK1(OP_, UZ_, GM_, K_, false, ngroups);                                                          \
+      } else {                                                                                              \
+        const int r = resident_blocks((const void*)dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_, true>, \
+                                      64 * kWaves, shmem);                                                  \
+        TE_MK1(OP_, UZ_, GM_, K_, true, ngroups < r ? ngroups : r);                                         \
+      }                                                                                                     \
+    } else {                                                                                                \
+      TE_MK1(OP_, UZ_, GM_, K_, false, ngroups < 4096 ? ngroups : 4096);                                    \
+    }                                                                                                       \
   } while (0)
 #define TE_MKG(OP_)                                                         \
   if (uzero) TE_MK(OP_, true, 0, 0);                                        \
