@@ -262,6 +262,42 @@ def test_fused_vcycle_matches_unfused(monkeypatch, filler, prec, op_kind):
     assert relerr(mesh.interior(xf, dim), mesh.interior(xu, dim)) <= 1e-8
 
 
+@pytest.mark.parametrize("prec", [R.F64, R.F32])
+@pytest.mark.parametrize("name,base_level", [("2d_uniform_8x8_refined_cross_mpi1.json", None), (None, 4)])
+def test_tail_matches_per_level_launches(monkeypatch, prec, name, base_level):
+    """The coarsest levels in one launch (levels of at most 16 patches of 16x16, kept in LDS)
+    against the per-level launches (TE_NO_TAIL=1): the same arithmetic in the same order,
+    so V-cycles and solves agree bit for bit."""
+    _print_case("tail", prec, name, base_level)
+    ns = (16, 16)
+    if name:
+        dim, levels = load(name, ns)
+    else:  # a uniform level-4 quadtree: 256, 64, 16, 4, 1 patches
+        dim, levels = 2, solver.uniform_levels(2, base_level, ns)
+    nps = [len(lv) for lv in levels]
+    assert any(n <= 16 for n in nps[1:-1]), nps  # a tail of two or more levels exists
+    kw = dict(filler_kind=R.BIQUADRATIC, smoother=0, precision=prec)
+    gt = solver.GMG(dim, ns, levels, **kw)
+    monkeypatch.setenv("TE_NO_TAIL", "1")
+    gn = solver.GMG(dim, ns, levels, **kw)
+    f = random_field(mesh.padded_shape(nps[0], dim, ns), 51)
+    vt, vn = gt.vcycle(f), gn.vcycle(f)
+    assert np.array_equal(mesh.interior(vt, dim), mesh.interior(vn, dim))
+    assert np.array_equal(gt.vcycle(f), vt)  # graph replay
+    b = _rhs_sin(levels, dim, ns)
+    xt, itt, ht = gt.solve(b, tol=1e-10, precond=2)
+    xn, itn, hn = gn.solve(b, tol=1e-10, precond=2)
+    assert itt == itn and np.array_equal(ht, hn)
+    assert np.array_equal(mesh.interior(xt, dim), mesh.interior(xn, dim))
+    # pre / post / coarse sweep counts other than one
+    kw2 = dict(kw, pre_sweeps=2, post_sweeps=0, coarse_sweeps=3)
+    monkeypatch.delenv("TE_NO_TAIL")
+    g2t = solver.GMG(dim, ns, levels, **kw2)
+    monkeypatch.setenv("TE_NO_TAIL", "1")
+    g2n = solver.GMG(dim, ns, levels, **kw2)
+    assert np.array_equal(mesh.interior(g2t.vcycle(f), dim), mesh.interior(g2n.vcycle(f), dim))
+
+
 @pytest.mark.parametrize("cycle_type", ["W", "F"])
 @pytest.mark.parametrize("label,dim,ns,mk,filler,smoother", [CASES[0], CASES[1]] + [c for c in CASES if "3d" in c[0]][:1])
 def test_w_and_f_cycles(cycle_type, label, dim, ns, mk, filler, smoother):

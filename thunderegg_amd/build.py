@@ -26,6 +26,7 @@ SOURCES = [
     "kernels/dft.hip",
     "kernels/dft_mfma.hip",
     "kernels/fused2d.hip",
+    "kernels/tail2d.hip",
     "kernels/patch_bicgstab.hip",
     "kernels/krylov.hip",
 ]

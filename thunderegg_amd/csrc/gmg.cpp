@@ -24,6 +24,7 @@
 #include "kernels/blas.h"
 #include "kernels/dft.h"
 #include "kernels/krylov.h"
+#include "kernels/tail2d.h"
 #include "te_internal.h"
 
 using namespace te;
@@ -85,6 +86,10 @@ struct te_gmg {
   te_timer* timer = nullptr;
   int time_its = 0;
   std::vector<int> timer_ids;
+  // fused path: the coarsest levels from tail_from down in one launch (launch_dft_tail),
+  // -1 when none qualify
+  int tail_from = -1;
+  TailArgs tail{};
 };
 
 namespace {
@@ -111,6 +116,15 @@ int sweep_timing(te_gmg* g, int li, bool stop)
 }
 
 int smooth_sweep(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void* out, int dto);
+
+// per-level sweep timings requested for a level of the tail: it then runs as per-level launches
+bool tail_timed(const te_gmg* g)
+{
+  if (!g->timer || g->timer_ids.empty() || g->tail_from < 0) return false;
+  for (size_t l = (size_t)g->tail_from; l < g->timer_ids.size(); l++)
+    if (g->timer_ids[l] != TE_TIMER_NONE) return true;
+  return false;
+}
 
 // One smoothing sweep on level li: PatchSolver::smooth (PatchSolver.h:142-170).
 // uzero: u is known to be zero (fresh coarse-level vector).  out/dto: where the
@@ -250,7 +264,13 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
   }
   GLevel& Cv = g->L[li + 1];
   const int next_first = li + 1 == nl - 1 ? g->o.coarse_sweeps : g->o.pre_sweeps;
-  if (next_first > 0 && g->rr_pre[li]) {
+  if (li + 1 == g->tail_from && !tail_timed(g)) {
+    // Cycle::restrict, then the V-cycle of the levels from li + 1 down in one launch
+    TE_RC(launch_resid_restrict(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur],
+                                Cv.dt, Cv.f, g->s));
+    TE_RC(launch_dft_tail(g->tail, Cv.dt, g->L[nl - 1].dt, g->o.filler_kind, g->s));
+    Cv.cur = Cv.u;
+  } else if (next_first > 0 && g->rr_pre[li]) {
     // Cycle::restrict and the coarser level's first sweep (from zero) in one launch
     TE_RC(sweep_timing(g, li + 1, false));
     TE_RC(launch_rr_pre(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur], Cv.plan,
@@ -490,6 +510,47 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
                       te_domain_num_patches(g->L[i + 1].d) <= kRrPreMaxCoarse)
                        ? 1
                        : 0;
+    // the tail: the first level below the finest with at most kTailMaxPatches patches whose
+    // levels (two or more) fit the tail's LDS; Poisson, 16 x 16 patches.  TE_NO_TAIL=1
+    // keeps the per-level launches (same arithmetic, for comparison).
+    const char* nt = std::getenv("TE_NO_TAIL");
+    if (fused && opts->op_kind == 0 && !(nt && nt[0] == '1') && levels[0]->dim == 2 && levels[0]->ns[0] == 16 &&
+        levels[0]->ns[1] == 16) {
+      for (int t = 1; t + 1 < num_levels; t++) {
+        const int nl = num_levels - t;
+        if (nl > kTailMaxLevels || te_domain_num_patches(g->L[t].d) > kTailMaxPatches) continue;
+        int np[kTailMaxLevels];
+        for (int l = 0; l < nl; l++) np[l] = te_domain_num_patches(g->L[t + l].d);
+        TailArgs a{};
+        if (tail_layout(&a, np, nl, g->L[t].dt, g->L[num_levels - 1].dt) > kTailLds) continue;
+        for (int l = 0; l < nl; l++) {
+          const GLevel& Lv = g->L[t + l];
+          a.lv[l].dp = Lv.d->d_patches;
+          a.lv[l].ring = (const int4*)Lv.d->d_ring;
+          a.lv[l].parent = Lv.link ? Lv.link->d_parent : nullptr;
+          a.lv[l].porth = Lv.link ? Lv.link->d_orth : nullptr;
+          a.lv[l].np = np[l];
+        }
+        a.nl = nl;
+        a.pre = opts->pre_sweeps;
+        a.post = opts->post_sweeps;
+        a.coarse = opts->coarse_sweeps;
+        a.neumann = opts->neumann;
+        a.f_in = g->L[t].f;
+        a.u_out = g->L[t].u;
+        const te_dft_plan* pt = g->L[t].plan;
+        const te_dft_plan* pl = g->L[num_levels - 1].plan;
+        const bool t64 = g->L[t].dt == TE_F64, l64 = g->L[num_levels - 1].dt == TE_F64;
+        a.mats_t = t64 ? (const void*)pt->d_mats64 : (const void*)pt->d_mats32;
+        a.s2_t = t64 ? (const void*)pt->d_s2_64 : (const void*)pt->d_s2_32;
+        a.mats_l = l64 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32;
+        a.s2_l = l64 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32;
+        if (!a.mats_t || !a.s2_t || !a.mats_l || !a.s2_l) break;
+        g->tail = a;
+        g->tail_from = t;
+        break;
+      }
+    }
   }
   if (!rc) {
     const te_domain* d0 = levels[0];
