@@ -23,6 +23,7 @@ struct TailLevelArgs {
   const int32_t* porth;    // per patch: orth_on_parent, -1 for a copy
   int np;
   int u0, u1, f;           // LDS byte offsets of the two solution buffers and of f
+  int meta, ringl;         // LDS byte offsets of the per-patch records and of the ring table
 };
 
 struct TailArgs {

@@ -23,6 +23,25 @@ namespace {
 constexpr int kN = 16, kLd = kN + 1, kSX = kN + 2, kPS = kSX * kSX;
 constexpr int kScr = kN * kLd + 4 * kN + kPS;  // per-wave scratch: X, interface terms, tile
 
+// what the steps read of a patch, copied into LDS at the start with the ring table, so
+// that after the first loads every step works from LDS alone
+struct TailPatch {
+  double h[2];
+  double ih2[2];
+  int type[4];  // side neighbour types (PatchInfo slots 4..7)
+  int parent, porth;
+};
+
+struct LevelLds {
+  const TailPatch* meta;
+  const int4* ring;
+};
+
+__device__ __forceinline__ LevelLds level_lds(const unsigned char* lds, const TailLevelArgs& lv)
+{
+  return LevelLds{ reinterpret_cast<const TailPatch*>(lds + lv.meta), reinterpret_cast<const int4*>(lds + lv.ringl) };
+}
+
 template <typename S>
 struct Scratch {
   S* X;
@@ -39,11 +58,11 @@ __device__ __forceinline__ Scratch<S> scratch(unsigned char* lds, int off, int w
 
 // one DFT sweep of patch p: out = solve(f - interface terms of u's ghosts); uzero: u = 0
 template <typename S, int KIND>
-__device__ void sweep_patch(int p, bool uzero, const TailLevelArgs& lv, const S* F, const S* U, S* out,
+__device__ void sweep_patch(int p, bool uzero, const LevelLds& lv, const S* F, const S* U, S* out,
                             const S* Sm, const S* S3, const S* s2, const Scratch<S>& w, int lane)
 {
   const int base = p * kPS;
-  const DevPatch& P = lv.dp[p];
+  const TailPatch& P = lv.meta[p];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int q = lane + 64 * k, y = q / kN, x = q - y * kN;
@@ -82,7 +101,7 @@ __device__ void sweep_patch(int p, bool uzero, const TailLevelArgs& lv, const S*
                                             },
                                             gv);
       const S ub = w.Tl[(by + 1) * kSX + bx + 1];
-      term = (ug + ub) * S(P.inv_h2[ph]);
+      term = (ug + ub) * S(P.ih2[ph]);
     }
     w.Bt[lane] = term;
     wave_sync();
@@ -92,7 +111,7 @@ __device__ void sweep_patch(int p, bool uzero, const TailLevelArgs& lv, const S*
       if (lane < 2 * kN) {
         const int up = lane / kN, t = lane - up * kN;
         const int s = 2 * ph + up;
-        if (P.type[4 + s] >= 0) {
+        if (P.type[s] >= 0) {
           const int xi = ph == 0 ? t * kLd + (up ? kN - 1 : 0) : (up ? kN - 1 : 0) * kLd + t;
           w.X[xi] -= w.Bt[s * kN + t];
         }
@@ -108,7 +127,7 @@ __device__ void sweep_patch(int p, bool uzero, const TailLevelArgs& lv, const S*
 
 // r = f - A u on patch p of level lv, restricted into fc (the next level's f)
 template <typename S, typename SC, int KIND>
-__device__ void residual_patch(int p, const TailLevelArgs& lv, const S* F, const S* U, SC* fc, S phys,
+__device__ void residual_patch(int p, const LevelLds& lv, const S* F, const S* U, SC* fc, S phys,
                                const Scratch<S>& w, int lane)
 {
   constexpr int h = kN / 2;
@@ -119,7 +138,7 @@ __device__ void residual_patch(int p, const TailLevelArgs& lv, const S* F, const
   L.ps = L.cs = kPS;
   int rs, rt;
   const int4 rn = at(lv.ring, ring_index(p, lane, kN, kN, &rs, &rt));
-  const int o = lv.porth[p], cp = lv.parent[p];
+  const int o = lv.meta[p].porth, cp = lv.meta[p].parent;
   S fv[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -133,8 +152,8 @@ __device__ void residual_patch(int p, const TailLevelArgs& lv, const S* F, const
   wave_sync();
   tile_ghost_gathered<S, KIND>(rn, lane, kN, kN, w.Tl, phys, gu);
   wave_sync();
-  const DevPatch& P = lv.dp[p];
-  const S ih2[2] = { S(P.inv_h2[0]), S(P.inv_h2[1]) };
+  const TailPatch& P = lv.meta[p];
+  const S ih2[2] = { S(P.ih2[0]), S(P.ih2[1]) };
   const int ox = (o & 1) ? h : 0, oy = (o & 2) ? h : 0;
   const int X = lane & 7, Y = lane >> 3;  // a lane per 2x2 block of fine cells
   S r[4];
@@ -160,9 +179,9 @@ __device__ void residual_patch(int p, const TailLevelArgs& lv, const S* F, const
 
 // out = u + P uc on patch p's interior (out may be u)
 template <typename S, typename SC>
-__device__ void interp_patch(int p, const TailLevelArgs& lv, const S* U, const SC* Uc, S* out, int lane)
+__device__ void interp_patch(int p, const LevelLds& lv, const S* U, const SC* Uc, S* out, int lane)
 {
-  const int o = lv.porth[p], cp = lv.parent[p];
+  const int o = lv.meta[p].porth, cp = lv.meta[p].parent;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int q = lane + 64 * k, y = q / kN, x = q - y * kN;
@@ -217,6 +236,24 @@ __global__ void __launch_bounds__(64 * kTailWaves) dft_tail_kernel(TailArgs a)
     T* F0 = fbuf<T>(lds, a.lv[0]);
     for (int i = threadIdx.x; i < a.lv[0].np * kPS; i += blockDim.x) F0[i] = fin[i];
   }
+  for (int l = 0; l < nl; l++) {
+    const TailLevelArgs& lv = a.lv[l];
+    TailPatch* meta = reinterpret_cast<TailPatch*>(lds + lv.meta);
+    int4* ring = reinterpret_cast<int4*>(lds + lv.ringl);
+    for (int i = threadIdx.x; i < lv.np * 8; i += blockDim.x) ring[i] = lv.ring[i];
+    for (int p = threadIdx.x; p < lv.np; p += blockDim.x) {
+      const DevPatch& P = lv.dp[p];
+      TailPatch m;
+      m.h[0] = P.h[0];
+      m.h[1] = P.h[1];
+      m.ih2[0] = P.inv_h2[0];
+      m.ih2[1] = P.inv_h2[1];
+      for (int s = 0; s < 4; s++) m.type[s] = P.type[4 + s];
+      m.parent = l < nl - 1 ? lv.parent[p] : 0;
+      m.porth = l < nl - 1 ? lv.porth[p] : -1;
+      meta[p] = m;
+    }
+  }
   __syncthreads();
   const T* SmT = reinterpret_cast<const T*>(lds + a.mat_t);
   const TL* SmL = reinterpret_cast<const TL*>(lds + a.mat_l);
@@ -233,11 +270,11 @@ __global__ void __launch_bounds__(64 * kTailWaves) dft_tail_kernel(TailArgs a)
     if (l == last) {
       if (wave < lw)
         for (int p = wave; p < lv.np; p += lw)
-          sweep_patch<TL, KIND>(p, uzero, lv, fbuf<TL>(lds, lv), buf<TL>(lds, lv, cur(l)), buf<TL>(lds, lv, cur(l) ^ 1),
+          sweep_patch<TL, KIND>(p, uzero, level_lds(lds, lv), fbuf<TL>(lds, lv), buf<TL>(lds, lv, cur(l)), buf<TL>(lds, lv, cur(l) ^ 1),
                                 SmL, SmL + kN * kLd, SmL + 2 * kN * kLd, wl, lane);
     } else {
       for (int p = wave; p < lv.np; p += kTailWaves)
-        sweep_patch<T, KIND>(p, uzero, lv, fbuf<T>(lds, lv), buf<T>(lds, lv, cur(l)), buf<T>(lds, lv, cur(l) ^ 1),
+        sweep_patch<T, KIND>(p, uzero, level_lds(lds, lv), fbuf<T>(lds, lv), buf<T>(lds, lv, cur(l)), buf<T>(lds, lv, cur(l) ^ 1),
                              SmT, SmT + kN * kLd, SmT + 2 * kN * kLd, wt, lane);
     }
     __syncthreads();
@@ -262,10 +299,10 @@ __global__ void __launch_bounds__(64 * kTailWaves) dft_tail_kernel(TailArgs a)
       const TailLevelArgs& lv = a.lv[l];
       for (int p = wave; p < lv.np; p += kTailWaves) {
         if (l + 1 == last)
-          residual_patch<T, TL, KIND>(p, lv, fbuf<T>(lds, lv), buf<T>(lds, lv, cur(l)), fbuf<TL>(lds, a.lv[l + 1]),
+          residual_patch<T, TL, KIND>(p, level_lds(lds, lv), fbuf<T>(lds, lv), buf<T>(lds, lv, cur(l)), fbuf<TL>(lds, a.lv[l + 1]),
                                       physT, wt, lane);
         else
-          residual_patch<T, T, KIND>(p, lv, fbuf<T>(lds, lv), buf<T>(lds, lv, cur(l)), fbuf<T>(lds, a.lv[l + 1]),
+          residual_patch<T, T, KIND>(p, level_lds(lds, lv), fbuf<T>(lds, lv), buf<T>(lds, lv, cur(l)), fbuf<T>(lds, a.lv[l + 1]),
                                      physT, wt, lane);
       }
       __syncthreads();
@@ -277,10 +314,10 @@ __global__ void __launch_bounds__(64 * kTailWaves) dft_tail_kernel(TailArgs a)
     const int dst = a.post == 0 ? cur(l) : cur(l) ^ 1;
     for (int p = wave; p < lv.np; p += kTailWaves) {
       if (l + 1 == last)
-        interp_patch<T, TL>(p, lv, buf<T>(lds, lv, cur(l)), buf<TL>(lds, a.lv[l + 1], cur(l + 1)),
+        interp_patch<T, TL>(p, level_lds(lds, lv), buf<T>(lds, lv, cur(l)), buf<TL>(lds, a.lv[l + 1], cur(l + 1)),
                             buf<T>(lds, lv, dst), lane);
       else
-        interp_patch<T, T>(p, lv, buf<T>(lds, lv, cur(l)), buf<T>(lds, a.lv[l + 1], cur(l + 1)), buf<T>(lds, lv, dst),
+        interp_patch<T, T>(p, level_lds(lds, lv), buf<T>(lds, lv, cur(l)), buf<T>(lds, a.lv[l + 1], cur(l + 1)), buf<T>(lds, lv, dst),
                            lane);
     }
     __syncthreads();
@@ -324,6 +361,10 @@ int tail_layout(TailArgs* a, const int* np, int nl, int dt, int dtl)
     a->lv[l].u1 = off + b;
     a->lv[l].f = off + 2 * b;
     off += 3 * b;
+    a->lv[l].meta = off;
+    off = align16(off + np[l] * (int)sizeof(TailPatch));
+    a->lv[l].ringl = off;
+    off = align16(off + np[l] * 8 * (int)sizeof(int4));
   }
   return off;
 }
