@@ -298,6 +298,32 @@ def test_tail_matches_per_level_launches(monkeypatch, prec, name, base_level):
     assert np.array_equal(mesh.interior(g2t.vcycle(f), dim), mesh.interior(g2n.vcycle(f), dim))
 
 
+@pytest.mark.parametrize("name", ["2d_uniform_8x8_refined_cross_mpi1.json", None])
+def test_fused_update_sweep_matches_separate(monkeypatch, name):
+    """Mixed precision, 16x16 patches: the BiCGStab s and p updates that also run the finest
+    level's zero-guess sweep give the same solve, bit for bit, as the separate update and
+    sweep launches (TE_NO_BSWEEP=1); only the grouping of the s.s partial sums differs, and
+    that feeds the early-exit test alone."""
+    _print_case("bsweep", name)
+    ns = (16, 16)
+    if name:
+        dim, levels = _fixture_levels(name, ns)
+    else:  # a uniform level-4 quadtree
+        dim, levels = 2, solver.uniform_levels(2, 4, ns)
+    kw = dict(filler_kind=R.BIQUADRATIC, smoother=0, precision=R.F32)
+    gf = solver.GMG(dim, ns, levels, **kw)
+    monkeypatch.setenv("TE_NO_BSWEEP", "1")
+    gs = solver.GMG(dim, ns, levels, **kw)
+    for b in (_rhs_sin(levels, dim, ns), random_field(mesh.padded_shape(len(levels[0]), dim, ns), 61)):
+        xf, itf, hf = gf.solve(b, tol=1e-10, precond=2)
+        xs, its, hs = gs.solve(b, tol=1e-10, precond=2)
+        assert itf == its and np.array_equal(hf, hs)
+        assert np.array_equal(mesh.interior(xf, dim), mesh.interior(xs, dim))
+        # and a second solve from the cached graph
+        xf2, itf2, _ = gf.solve(b, tol=1e-10, precond=2)
+        assert itf2 == itf and np.array_equal(xf2, xf)
+
+
 @pytest.mark.parametrize("cycle_type", ["W", "F"])
 @pytest.mark.parametrize("label,dim,ns,mk,filler,smoother", [CASES[0], CASES[1]] + [c for c in CASES if "3d" in c[0]][:1])
 def test_w_and_f_cycles(cycle_type, label, dim, ns, mk, filler, smoother):

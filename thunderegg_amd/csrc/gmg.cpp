@@ -90,6 +90,9 @@ struct te_gmg {
   // -1 when none qualify
   int tail_from = -1;
   TailArgs tail{};
+  // fused mixed-precision path: the BiCGStab s and p updates run the finest level's
+  // zero-guess sweep too (launch_bicg_sweep); TE_NO_BSWEEP=1 keeps them apart
+  bool bsweep = false;
 };
 
 namespace {
@@ -116,6 +119,9 @@ int sweep_timing(te_gmg* g, int li, bool stop)
 }
 
 int smooth_sweep(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void* out, int dto);
+
+// a sweep timing requested for the finest level: its first sweep then runs in the V-cycle
+bool level0_timed(const te_gmg* g) { return g->timer && !g->timer_ids.empty() && g->timer_ids[0] != TE_TIMER_NONE; }
 
 // per-level sweep timings requested for a level of the tail: it then runs as per-level launches
 bool tail_timed(const te_gmg* g)
@@ -298,10 +304,12 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
 // Cycle::apply (GMG/Cycle.h:94-98) enqueued on g->s.  dtf / udt: types of f and u (FP32
 // only on the fused mixed-precision path, where the V-cycle rounds f to FP32 as it reads it
 // and its last sweep computes u in FP32; see visit_fused).
-int vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt = TE_F64, int dtf = TE_F64)
+// l0_done: the finest level's zero-guess sweep already ran (launch_bicg_sweep), its result
+// in L[0].u.
+int vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt = TE_F64, int dtf = TE_F64, bool l0_done = false)
 {
   GLevel& L0 = g->L[0];
-  if (g->fused) return visit_fused(g, 0, f, dtf, u, udt, false);
+  if (g->fused) return visit_fused(g, 0, f, dtf, u, udt, l0_done);
   if (udt != TE_F64 || dtf != TE_F64) {
     set_error("internal: FP32 V-cycle input or output needs the fused path");
     return TE_ERR_UNSUPPORTED;
@@ -551,6 +559,9 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
         break;
       }
     }
+    const char* nb = std::getenv("TE_NO_BSWEEP");
+    g->bsweep = g->m32 && num_levels > 1 && opts->pre_sweeps >= 1 && bicg_sweep_supported(g->L[0].plan) &&
+                !(nb && nb[0] == '1');
   }
   if (!rc) {
     const te_domain* d0 = levels[0];
@@ -651,14 +662,21 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // The preconditioner and operator launches for s run before the test (the reference runs
   // them after it): when it fires their results are unused; that happens at most once per
   // solve.
+  // the s and p updates also run the finest level's zero-guess sweep of the V-cycle that
+  // follows them (the first iteration's comes from a separate sweep after the setup)
+  const bool l0f = m32 && g->bsweep && !level0_timed(g);
+  float* u0 = (float*)L0.u;
   auto iteration = [&]() -> int {
-    if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->p32 : g->p, mp, dtm, dtm));
+    if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->p32 : g->p, mp, dtm, dtm, l0f));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, dtm, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
     TE_RC(launch_bicg_finalize(FIN_ALPHA, g->pa, nullptr, nb, g->scal, s));
-    TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pc, nullptr,
-                      s, TE_F64, m32 ? g->s32 : nullptr));
-    if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->s32 : g->sv, ms, dtm, dtm));
+    if (l0f)
+      TE_RC(launch_bicg_sweep(BICG_S, L0.plan, g->scal, g->sv, g->resid, g->ap, g->s32, u0, g->pc, nb, s));
+    else
+      TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pc,
+                        nullptr, s, TE_F64, m32 ? g->s32 : nullptr));
+    if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->s32 : g->sv, ms, dtm, dtm, l0f));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, ms, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, dtm, g->as, g->sv, g->pa, g->pb, g->fly, s));
     // ||s|| and the s-norm test (its partials from the s update), then omega
@@ -669,6 +687,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     TE_RC(launch_bicg(BICG_XR, d, g->scal, x, g->resid, mp, ms, g->ap, g->as, g->rhat, g->pa, g->pb, s, dtm, nullptr,
                       xp_m ? mp : (const void*)g->p, xp_m ? 1 : 0));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
+    if (l0f) return launch_bicg_sweep(BICG_P, L0.plan, g->scal, g->p, g->ap, g->resid, g->p32, u0, nullptr, 0, s);
     return launch_bicg(BICG_P, d, g->scal, g->p, nullptr, g->ap, g->resid, nullptr, nullptr, nullptr, nullptr,
                        nullptr, s, TE_F64, m32 ? g->p32 : nullptr);
   };
@@ -707,6 +726,10 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s,
                     TE_F64, m32 ? g->p32 : nullptr));
   TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s, tol));
+  // the first V-cycle's zero-guess sweep on the finest level (the later ones run in the p
+  // updates)
+  if (l0f) TE_RC(launch_dft_mfma_fused(L0.plan, TE_F32, TE_F32, TE_F32, g->p32, u0, u0, true, 0, TE_F64, nullptr,
+                                       nullptr, s));
   TE_RC(read_scal());
   double seq = h[SC_SEQ];
   // wait for the iteration with sequence number `want` to publish its scalars into h

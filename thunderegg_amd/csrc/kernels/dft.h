@@ -6,12 +6,6 @@
 namespace te {
 constexpr int kDftMatDim = 64;                      // largest supported n
 constexpr int kDftMatStride = kDftMatDim * kDftMatDim;
-// Cycle::restrict of the level of g fused with the zero-guess sweep of the next coarser
-// level (plan pc): writes the coarse f (fc) and the coarse u (uc); 2D, 16x16 patches.
-bool rr_pre_supported(const te_ghost_filler* g, const te_dft_plan* pc);
-int launch_rr_pre(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
-                  const void* coeffs, int dt, int dtf, const void* f, const void* u, const te_dft_plan* pc, int dtc,
-                  void* fc, void* uc, hipStream_t s);
 }  // namespace te
 
 struct te_dft_plan {
@@ -48,4 +42,11 @@ bool rr_pre_supported(const te_ghost_filler* g, const te_dft_plan* pc);
 int launch_rr_pre(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
                   const void* coeffs, int dt, int dtf, const void* f, const void* u, const te_dft_plan* pc, int dtc,
                   void* fc, void* uc, hipStream_t s);
+// BiCGStab's s update (mode BICG_S, with the s.s partials in pc: nslots slots, those past
+// the launch's workgroups zeroed) or p update (BICG_P) fused with the zero-guess FP32 sweep of
+// plan pl's level (2D, 16 x 16) on the updated vector: v0 (FP64) and its FP32 copy w32 are
+// written as by launch_bicg, and the sweep's result goes to out.
+bool bicg_sweep_supported(const te_dft_plan* pl);
+int launch_bicg_sweep(int mode, const te_dft_plan* pl, const double* scal, double* v0, const double* a0,
+                      const double* a1, float* w32, float* out, double* pc, int nslots, hipStream_t s);
 }  // namespace te

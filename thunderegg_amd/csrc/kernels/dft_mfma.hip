@@ -22,9 +22,11 @@
 //             interpolation, ghost fill and the post-smoothing sweep are one launch
 //             (the result must then go to a buffer other than u).
 #include "../te_internal.h"
+#include "blas.h"
 #include "dft.h"
 #include "dft_wave.h"
 #include "ghost2d.h"
+#include "krylov.h"
 
 namespace te {
 namespace {
@@ -500,7 +502,112 @@ void launch_rp(const te_ghost_filler* g, const te_level_link* link, int op_kind,
 #undef TE_RP
 }
 
+// The outer BiCGStab's s update (MODE BICG_S: s = r - alpha ap, with the s.s partials) or p
+// update (BICG_P: p = (p - omega ap) beta + r) fused with the zero-guess sweep of the finest
+// level (16 x 16 patches, FP32 cycle) that the next V-cycle starts with: each wave updates a
+// patch in FP64, writes it and its FP32 copy, and solves the patch on the FP32 values it has
+// just formed (the same values the separate sweep would read back from the copy), writing
+// the level's solution buffer.  The updates are those of bicg_kernel.
+template <int MODE>
+__global__ void __launch_bounds__(64 * kWaves) bicg_sweep_kernel(const DevPatch* __restrict__ dp,
+                                                                 const float* __restrict__ mats,
+                                                                 const float* __restrict__ s2tab,
+                                                                 const double* __restrict__ scal,
+                                                                 double* __restrict__ v0, const double* __restrict__ a0,
+                                                                 const double* __restrict__ a1, float* __restrict__ w32,
+                                                                 float* __restrict__ out, double* __restrict__ pc,
+                                                                 int nslots, int np)
+{
+  constexpr int n = 16, ld = n + 1, SX = n + 2, PS = SX * SX;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* Sm = reinterpret_cast<float*>(smem_raw);
+  float* S3 = Sm + n * ld;
+  float* s2 = S3 + n * ld;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float* X = s2 + n + wave * n * ld;
+  for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+    const int row = k / n, col = k - row * n;
+    Sm[row * ld + col] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
+    S3[row * ld + col] = mats[(4 * 3 + 0) * kDftMatStride + k];  // DST_III, axis 0
+  }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) s2[k] = s2tab[(2 * 3 + 0) * kDftMatDim + k];
+  __syncthreads();
+  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], beta = scal[SC_BETA];
+  const bool run = scal[SC_STATE] == ST_RUN;
+  double sa = 0.0;
+  if (run) {
+    const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, np, kWaves);
+    for (int p = xr.first + wave; p < xr.end; p += xr.step) {
+      const int base = p * PS;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int q = lane + 64 * k, y = q >> 4, x = q & 15;
+        const int i = base + (y + 1) * SX + x + 1;
+        double v;
+        if (MODE == BICG_S) {  // v0 = s, a0 = r, a1 = ap
+          v = at(a0, i);
+          v += at(a1, i) * -alpha;
+          sa += v * v;
+        } else {  // v0 = p, a0 = ap, a1 = r
+          double pv = at(v0, i);
+          pv += at(a0, i) * -omega;
+          v = pv * beta + at(a1, i);
+        }
+        at(v0, i) = v;
+        const float v32 = float(v);
+        at(w32, i) = v32;
+        X[y * ld + x] = v32;
+      }
+      wave_sync();
+      const DevPatch& P = dp[p];
+      const float lx = float(4.0 / (P.h[0] * P.h[0])), ly = float(4.0 / (P.h[1] * P.h[1]));
+      dft_solve_wave<float, float, 1>(X, Sm, S3, s2, lx, ly, out, base, SX, true);
+      wave_sync();  // X is rewritten by the next patch
+    }
+  }
+  if (MODE == BICG_S) {
+    const double r = block_reduce<RED_SUM>(sa);
+    if (threadIdx.x == 0) pc[blockIdx.x] = r;
+    if (blockIdx.x == 0)
+      for (int i = (int)gridDim.x + (int)threadIdx.x; i < nslots; i += (int)blockDim.x) pc[i] = 0.0;
+  }
+}
+
 }  // namespace
+
+bool bicg_sweep_supported(const te_dft_plan* pl)
+{
+  const te_domain* d = pl->g->d;
+  return d->dim == 2 && d->ns[0] == 16 && d->ns[1] == 16 && pl->use_mfma && pl->mask == 0 && pl->d_mats32 &&
+         pl->d_s2_32;
+}
+
+int launch_bicg_sweep(int mode, const te_dft_plan* pl, const double* scal, double* v0, const double* a0,
+                      const double* a1, float* w32, float* out, double* pc, int nslots, hipStream_t s)
+{
+  const te_domain* d = pl->g->d;
+  if (d->np == 0) return TE_OK;
+  if (!bicg_sweep_supported(pl) || (mode != BICG_S && mode != BICG_P) || !w32 || !out ||
+      (mode == BICG_S && (!pc || nslots < 1))) {
+    set_error("internal: fused BiCGStab update + sweep: 16 x 16 matrix-core plan, s or p update, FP32 buffers");
+    return TE_ERR_ARG;
+  }
+  constexpr int n = 16;
+  const size_t shmem = sizeof(float) * (2 * n * (n + 1) + n + kWaves * n * (n + 1));
+  const int ngroups = (d->np + kWaves - 1) / kWaves;
+  const void* k = mode == BICG_S ? (const void*)bicg_sweep_kernel<BICG_S> : (const void*)bicg_sweep_kernel<BICG_P>;
+  int grid = resident_blocks(k, 64 * kWaves, shmem);
+  if (grid > ngroups) grid = ngroups;
+  if (mode == BICG_S && grid > nslots) grid = nslots;
+  if (mode == BICG_S)
+    hipLaunchKernelGGL((bicg_sweep_kernel<BICG_S>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches,
+                       pl->d_mats32, pl->d_s2_32, scal, v0, a0, a1, w32, out, pc, nslots, d->np);
+  else
+    hipLaunchKernelGGL((bicg_sweep_kernel<BICG_P>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches,
+                       pl->d_mats32, pl->d_s2_32, scal, v0, a0, a1, w32, out, pc, nslots, d->np);
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
 
 bool dft_mfma_supported(const te_dft_plan* pl)
 {
