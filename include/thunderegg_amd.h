@@ -335,7 +335,8 @@ int te_timer_save(te_timer* t, const char* filename);
  * (interpolation into the first post-smoothing sweep, the finer level's residual and
  * restriction into the first sweep of a level of at most 1024 patches), so its timing
  * includes that work; the reference's per-patch "Single Patch Solve" timings have no
- * counterpart (a sweep solves all patches of the level in one launch). */
+ * counterpart (a sweep solves all patches of the level in one launch).  The solver keeps
+ * the pointer: detach (t = NULL) before te_timer_destroy. */
 int te_gmg_set_timer(te_gmg* g, te_timer* t, int iterations, const int* domain_ids);
 
 /* ----------------------------------------------- mesh generation (host) --
