@@ -186,6 +186,50 @@ def test_dft_smooth(name, ns, neumann_mask):
         assert relerr(mesh.interior(got, dim), mesh.interior(uref, dim)) <= tol, dtype
 
 
+@pytest.mark.parametrize("name", ["3d_refined_bnw_2x2x2_mpi1.json", "3d_mid_refine_4x4x4_mpi1.json"])
+@pytest.mark.parametrize("op_kind", [0, 1])
+@pytest.mark.parametrize("neumann_mask", [0, 0b100101])
+def test_dft3_mfma_matches_generic(name, op_kind, neumann_mask, monkeypatch):
+    """The 3D matrix-core patch solve (16^3 cells) against the generic kernel and the oracle:
+    Dirichlet and mixed Neumann transform types, Poisson and VarPoisson interface terms, the
+    smoothing sweep and the zero-u apply."""
+    _print_case("dft3-mfma", name, op_kind, neumann_mask)
+    ns = (16, 16, 16)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    od = O.Domain(dim, ns, lv)
+    dd = _dev(dim, ns, lv)
+    g = R.GhostFiller(dd, R.TRILINEAR, R.FACES)
+    coeffs = None
+    if op_kind == 1:
+        X = mesh.cell_centers(lv, dim, ns)
+        coeffs = np.exp(np.sin(3 * X[..., 0]) * np.cos(2 * X[..., 1]) + 0.5 * X[..., 2])[:, None].copy()
+        O.fill_ghost(od, coeffs, kind=R.TRILINEAR)
+    dc = R.DeviceArray.from_host(coeffs) if coeffs is not None else None
+    plan_mfma = R.DftPlan(g, neumann_mask=neumann_mask, op_kind=op_kind, coeffs=dc)
+    monkeypatch.setenv("TE_DFT_VALU", "1")
+    plan_valu = R.DftPlan(g, neumann_mask=neumann_mask, op_kind=op_kind, coeffs=dc)
+    for dtype, tol in [(R.F64, 1e-12), (R.F32, 2e-5)]:
+        f0 = random_field(od.shape(1), 71, R.np_dtype(dtype))
+        u0 = random_field(od.shape(1), 72, R.np_dtype(dtype))
+        uref = u0.astype(np.float64)
+        O.dft_smooth(od, f0.astype(np.float64), uref, kind=R.TRILINEAR, neumann_mask=neumann_mask, op_kind=op_kind,
+                     coeffs=coeffs)
+        outs = []
+        for plan in (plan_mfma, plan_valu):
+            du = R.DeviceArray.from_host(u0)
+            plan.smooth(R.DeviceArray.from_host(f0), du)
+            outs.append(mesh.interior(du.get(), dim))
+        assert relerr(outs[0], mesh.interior(uref, dim)) <= tol, dtype
+        assert relerr(outs[0], outs[1]) <= tol, dtype
+        du = R.DeviceArray.from_host(u0)
+        plan_mfma.apply(R.DeviceArray.from_host(f0), du)
+        uz = np.zeros_like(uref)
+        O.dft_smooth(od, f0.astype(np.float64), uz, kind=R.TRILINEAR, neumann_mask=neumann_mask, op_kind=op_kind,
+                     coeffs=coeffs)
+        assert relerr(mesh.interior(du.get(), dim), mesh.interior(uz, dim)) <= tol, dtype
+
+
 @pytest.mark.parametrize("ns", [(16, 16), (32, 32)])
 @pytest.mark.parametrize("op_kind", [0, 1])
 def test_dft_mfma_matches_generic(ns, op_kind, monkeypatch):

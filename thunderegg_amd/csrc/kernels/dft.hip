@@ -10,6 +10,7 @@
 // modifyRHSForInternalBoundaryConditions vanish and u is not read.
 #include "../te_internal.h"
 #include "dft.h"
+#include "dft_wave.h"
 
 namespace te {
 namespace {
@@ -134,6 +135,139 @@ __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ d
   }
 }
 
+// The same solve for 3D patches of 16^3 cells on the matrix cores: every axis transform is
+// 16 products of 16 x 16 matrices (the lines of one slice, or of one row of slices, against
+// the axis matrix), 4 per wave, each a chain of 16x16x4 MFMAs with operands read from LDS.
+// The patch lives in LDS with padded pitches (rows of 17, slices of 16 * 17 + 1) so that a
+// lane group's operand reads spread over the banks.  Interface terms, eigenvalues, the
+// per-patch transform types and the all-Neumann zero mode are the generic kernel's.
+constexpr int k3N = 16, k3Row = 17, k3Slice = 16 * 17 + 1, k3Cells = 16 * k3Slice, k3Mat = 16 * 17;
+
+__device__ __forceinline__ int c3(int x, int y, int z) { return x + k3Row * y + k3Slice * z; }
+
+// one 16 x 16 x 16 product, D[i][j] = sum_k A(i, k) B(k, j), written by put(i, j, value)
+template <typename T, class GA, class GB, class Put>
+__device__ __forceinline__ void mm16(GA A, GB B, Put put, int lane)
+{
+  typedef typename Mf<T>::V V;
+  const int c = lane & 15, g = lane >> 4;
+  V acc = { 0, 0, 0, 0 };
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int k = 4 * s + g;
+    acc = Mf<T>::mma(A(c, k), B(k, c), acc);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) put(Mf<T>::row(r, g), c, acc[r]);
+}
+
+// dst = the transform of src along axis a with matrix M (16 x 16, rows of k3Row)
+template <typename T>
+__device__ __forceinline__ void axis16(int a, const T* M, const T* src, T* dst, int wave, int lane)
+{
+  for (int q = wave; q < k3N; q += 4) {
+    if (a == 0) {  // slice z = q: D[y][k] = sum_x X[y][x] M[k][x]
+      mm16<T>([&](int y, int x) { return src[c3(x, y, q)]; }, [&](int x, int k) { return M[k * k3Row + x]; },
+              [&](int y, int k, T v) { dst[c3(k, y, q)] = v; }, lane);
+    } else if (a == 1) {  // slice z = q: D[k][x] = sum_y M[k][y] X[y][x]
+      mm16<T>([&](int k, int y) { return M[k * k3Row + y]; }, [&](int y, int x) { return src[c3(x, y, q)]; },
+              [&](int k, int x, T v) { dst[c3(x, k, q)] = v; }, lane);
+    } else {  // row y = q: D[k][x] = sum_z M[k][z] X[z][x]
+      mm16<T>([&](int k, int z) { return M[k * k3Row + z]; }, [&](int z, int x) { return src[c3(x, q, z)]; },
+              [&](int k, int x, T v) { dst[c3(x, q, k)] = v; }, lane);
+    }
+  }
+}
+
+template <typename T, typename TF, typename TO, int OP, bool UZERO>
+__global__ void __launch_bounds__(256) dft3_mfma_kernel(const DevPatch* __restrict__ dp, const int* __restrict__ ptype,
+                                                        const T* __restrict__ mats, const T* __restrict__ s2tab,
+                                                        const TF* __restrict__ f, const T* __restrict__ u,
+                                                        const T* __restrict__ coeffs, TO* __restrict__ out, Layout L)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* A = reinterpret_cast<T*>(smem_raw);
+  T* B = A + k3Cells;
+  T* MF = B + k3Cells;       // forward matrices of the three axes, rows of k3Row
+  T* MI = MF + 3 * k3Mat;    // inverse matrices
+  const int p = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int* pt = ptype + p * 8;  // fwd type per axis [0..2], eig type per axis [3..5], zero mode [6]
+  const DevPatch& P = dp[p];
+  for (int a = 0; a < 3; a++) {
+    const int tf = pt[a];
+    const int ti = tf == 0 ? 1 : (tf == 3 ? 4 : tf);  // DCT_II->DCT_III, DST_II->DST_III, IV types self-inverse
+    for (int k = threadIdx.x; k < k3N * k3N; k += blockDim.x) {
+      const int row = k >> 4, col = k & 15;
+      MF[a * k3Mat + row * k3Row + col] = mats[(tf * 3 + a) * kDftMatStride + k];
+      MI[a * k3Mat + row * k3Row + col] = mats[(ti * 3 + a) * kDftMatStride + k];
+    }
+  }
+  // f_copy = f, then modifyRHSForInternalBoundaryConditions (Poisson :165-180, VarPoisson :168-186)
+  for (int o = threadIdx.x; o < k3N * k3N * k3N; o += blockDim.x) {
+    const int x = o & 15, y = (o >> 4) & 15, z = o >> 8;
+    const int gi = lin(L, p, 0, x, y, z);
+    T v = T(f[gi]);
+    if (!UZERO) {
+      const int cc[3] = { x, y, z };
+      const int st[3] = { 1, L.sx, L.sy };
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int up = 0; up < 2; up++) {
+          const bool edge = up ? (cc[a] == k3N - 1) : (cc[a] == 0);
+          if (!edge || P.type[20 + 2 * a + up] < 0) continue;
+          const int gidx = up ? gi + st[a] : gi - st[a];
+          const T ih2 = T(P.inv_h2[a]);
+          if (OP == 0) {
+            v -= (u[gidx] + u[gi]) * ih2;
+          } else {
+            const int cq = gi - p * L.ps;
+            const int cg = up ? cq + st[a] : cq - st[a];
+            const T cb = coeffs[p * L.cs + cq], cgv = coeffs[p * L.cs + cg];
+            v -= (u[gidx] + u[gi]) * (cb + cgv) * (ih2 * T(0.5));
+          }
+        }
+      }
+    }
+    A[c3(x, y, z)] = v;
+  }
+  __syncthreads();
+  T* src = A;
+  T* dst = B;
+  for (int a = 0; a < 3; a++) {  // forward transform along each axis
+    axis16<T>(a, MF + a * k3Mat, src, dst, wave, lane);
+    __syncthreads();
+    T* t = src;
+    src = dst;
+    dst = t;
+  }
+  for (int o = threadIdx.x; o < k3N * k3N * k3N; o += blockDim.x) {  // divide by the eigenvalues
+    const int k3[3] = { o & 15, (o >> 4) & 15, o >> 8 };
+    T eig = T(0);
+    for (int a = 0; a < 3; a++) {
+      const double h = P.h[a];
+      eig -= T(4.0 / (h * h)) * s2tab[(pt[3 + a] * 3 + a) * kDftMatDim + k3[a]];
+    }
+    src[c3(k3[0], k3[1], k3[2])] /= eig;
+  }
+  __syncthreads();
+  if (pt[6] && threadIdx.x == 0) src[0] = T(0);  // all-Neumann patch without neighbours (:438-440)
+  __syncthreads();
+  for (int a = 0; a < 3; a++) {
+    axis16<T>(a, MI + a * k3Mat, src, dst, wave, lane);
+    __syncthreads();
+    T* t = src;
+    src = dst;
+    dst = t;
+  }
+  const T scale = T(2.0) / T(16) * (T(2.0) / T(16)) * (T(2.0) / T(16));
+  for (int o = threadIdx.x; o < k3N * k3N * k3N; o += blockDim.x) {
+    const int x = o & 15, y = (o >> 4) & 15, z = o >> 8;
+    out[lin(L, p, 0, x, y, z)] = TO(src[c3(x, y, z)] * scale);
+  }
+}
+
 template <typename T, typename TF, typename TO, int D>
 void launch_d(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzero, hipStream_t s)
 {
@@ -149,6 +283,19 @@ void launch_d(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzer
   const T* mats = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32);
   const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
   const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);
+  if (D == 3 && pl->use_mfma && d->ns[0] == 16 && d->ns[1] == 16 && d->ns[2] == 16) {
+    const size_t sh3 = sizeof(T) * (2 * (size_t)k3Cells + 6 * (size_t)k3Mat);
+#define TE_D3(OP_, UZ_)                                                                                         \
+  hipLaunchKernelGGL((dft3_mfma_kernel<T, TF, TO, OP_, UZ_>), dim3(d->np), dim3(256), sh3, s, d->d_patches,     \
+                     pl->d_ptype, mats, s2, f, u, cf, out, L)
+    if (pl->op_kind == 0) {
+      if (uzero) TE_D3(0, true); else TE_D3(0, false);
+    } else {
+      if (uzero) TE_D3(1, true); else TE_D3(1, false);
+    }
+#undef TE_D3
+    return;
+  }
 #define TE_DL(OP_, UZ_)                                                                                      \
   hipLaunchKernelGGL((dft_kernel<T, TF, TO, D, OP_, UZ_>), dim3(d->np), dim3(256), shmem, s, d->d_patches, \
                      pl->d_ptype, mats, s2, f, u, cf, out, L)
