@@ -139,8 +139,11 @@ __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ d
 // 16 products of 16 x 16 matrices (the lines of one slice, or of one row of slices, against
 // the axis matrix), 4 per wave, each a chain of 16x16x4 MFMAs with operands read from LDS.
 // The patch lives in LDS with padded pitches (rows of 17, slices of 16 * 17 + 1) so that a
-// lane group's operand reads spread over the banks.  Interface terms, eigenvalues, the
-// per-patch transform types and the all-Neumann zero mode are the generic kernel's.
+// lane group's operand reads spread over the banks, in one buffer: a wave's products read
+// and write only its own slices (x and y transforms) or rows (z), and each product reads
+// all its operands before it writes, so the transforms run in place.  Interface terms,
+// eigenvalues, the per-patch transform types and the all-Neumann zero mode are the generic
+// kernel's.
 constexpr int k3N = 16, k3Row = 17, k3Slice = 16 * 17 + 1, k3Cells = 16 * k3Slice, k3Mat = 16 * 17;
 
 __device__ __forceinline__ int c3(int x, int y, int z) { return x + k3Row * y + k3Slice * z; }
@@ -161,7 +164,7 @@ __device__ __forceinline__ void mm16(GA A, GB B, Put put, int lane)
   for (int r = 0; r < 4; r++) put(Mf<T>::row(r, g), c, acc[r]);
 }
 
-// dst = the transform of src along axis a with matrix M (16 x 16, rows of k3Row)
+// dst = the transform of src along axis a with matrix M (16 x 16, rows of k3Row); dst may be src
 template <typename T>
 __device__ __forceinline__ void axis16(int a, const T* M, const T* src, T* dst, int wave, int lane)
 {
@@ -187,8 +190,7 @@ __global__ void __launch_bounds__(256) dft3_mfma_kernel(const DevPatch* __restri
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* A = reinterpret_cast<T*>(smem_raw);
-  T* B = A + k3Cells;
-  T* MF = B + k3Cells;       // forward matrices of the three axes, rows of k3Row
+  T* MF = A + k3Cells;       // forward matrices of the three axes, rows of k3Row
   T* MI = MF + 3 * k3Mat;    // inverse matrices
   const int p = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -233,15 +235,11 @@ __global__ void __launch_bounds__(256) dft3_mfma_kernel(const DevPatch* __restri
     A[c3(x, y, z)] = v;
   }
   __syncthreads();
-  T* src = A;
-  T* dst = B;
-  for (int a = 0; a < 3; a++) {  // forward transform along each axis
-    axis16<T>(a, MF + a * k3Mat, src, dst, wave, lane);
+  for (int a = 0; a < 3; a++) {  // forward transform along each axis, in place
+    axis16<T>(a, MF + a * k3Mat, A, A, wave, lane);
     __syncthreads();
-    T* t = src;
-    src = dst;
-    dst = t;
   }
+  T* src = A;
   for (int o = threadIdx.x; o < k3N * k3N * k3N; o += blockDim.x) {  // divide by the eigenvalues
     const int k3[3] = { o & 15, (o >> 4) & 15, o >> 8 };
     T eig = T(0);
@@ -255,11 +253,8 @@ __global__ void __launch_bounds__(256) dft3_mfma_kernel(const DevPatch* __restri
   if (pt[6] && threadIdx.x == 0) src[0] = T(0);  // all-Neumann patch without neighbours (:438-440)
   __syncthreads();
   for (int a = 0; a < 3; a++) {
-    axis16<T>(a, MI + a * k3Mat, src, dst, wave, lane);
+    axis16<T>(a, MI + a * k3Mat, A, A, wave, lane);
     __syncthreads();
-    T* t = src;
-    src = dst;
-    dst = t;
   }
   const T scale = T(2.0) / T(16) * (T(2.0) / T(16)) * (T(2.0) / T(16));
   for (int o = threadIdx.x; o < k3N * k3N * k3N; o += blockDim.x) {
@@ -284,7 +279,7 @@ void launch_d(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzer
   const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
   const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);
   if (D == 3 && pl->use_mfma && d->ns[0] == 16 && d->ns[1] == 16 && d->ns[2] == 16) {
-    const size_t sh3 = sizeof(T) * (2 * (size_t)k3Cells + 6 * (size_t)k3Mat);
+    const size_t sh3 = sizeof(T) * ((size_t)k3Cells + 6 * (size_t)k3Mat);
 #define TE_D3(OP_, UZ_)                                                                                         \
   hipLaunchKernelGGL((dft3_mfma_kernel<T, TF, TO, OP_, UZ_>), dim3(d->np), dim3(256), sh3, s, d->d_patches,     \
                      pl->d_ptype, mats, s2, f, u, cf, out, L)
