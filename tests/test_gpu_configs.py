@@ -100,15 +100,22 @@ def test_config2_full_size():
 
 
 def test_config3_full_size():
-    """3D, 134M DOF: TriLinear ghosts, DFT smoother, 2 pre / 2 post sweeps, 5 levels."""
+    """3D, 134M DOF: TriLinear ghosts, DFT smoother, 2 pre / 2 post sweeps, 5 levels.  The
+    solve takes the iterations the oracle's cycle gives on this problem (8 in FP64, 9 mixed;
+    a race between patches shows up as more, and as solves that differ), and a second solve
+    is bit-identical."""
     _print_case("config3")
     P = configs.config3()
     for prec in (R.F64, R.F32):
         g = _gmg(P, prec)
         b, x, its, hist = _solve(P, g, 2)
         print("config3", "f64" if prec == R.F64 else "mixed", its, "iterations", flush=True)
-        assert its < 60
+        assert its <= (8 if prec == R.F64 else 10)
         assert _true_residual(P, g, b, x) <= 1.5e-10
+        x1 = mesh.interior(x.get(), 3)  # (the true residual above filled x's ghost cells)
+        x.zero()
+        its2, hist2 = g.solve_device(b, x, tol=1e-10, max_it=500, precond=2, history_cap=502)
+        assert its2 == its and np.array_equal(hist2, hist) and np.array_equal(mesh.interior(x.get(), 3), x1)
         del g, b, x
 
 
