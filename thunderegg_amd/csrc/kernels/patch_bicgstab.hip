@@ -4,8 +4,10 @@
 // single-patch operator with internal boundary conditions, started from the
 // current u.  One workgroup per patch keeps the whole Krylov state on chip:
 // each thread owns CPT cells of every Krylov vector in registers, the operand
-// of each operator apply is staged in LDS with its boundary ghosts, and the
-// inner products are workgroup reductions.  Patches iterate independently
+// of each operator apply is staged in LDS (VarPoisson: the patch's coefficients
+// too, once), and the inner products are workgroup reductions, two at a time
+// where the recurrence allows (a.s with a.a, r.rhat with r.r), each over
+// double-buffered slots with one barrier.  Patches iterate independently
 // (per-patch termination), so a level is one launch.
 #include "../te_internal.h"
 #include "blas.h"
@@ -15,17 +17,30 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// sum over the workgroup, returned to every thread
-__device__ __forceinline__ double wg_sum(double v, double* red)
+constexpr int kWaves = kThreads / 64;
+
+// sums over the workgroup of NV values per thread, returned to every thread; `buf` alternates
+// between the two halves of `red` from call to call, so that a call's writes cannot meet the
+// previous call's reads and one barrier suffices (the one before it orders the call before)
+template <int NV>
+__device__ __forceinline__ void wg_sums(double (&v)[NV], double (*red)[2 * kWaves], int& buf)
 {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; j++)
+    for (int off = 32; off > 0; off >>= 1) v[j] += __shfl_down(v[j], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double* rb = red[buf];
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < NV; j++) rb[j * kWaves + w] = v[j];
   __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double r = red[0];
-  for (int k = 1; k < kThreads / 64; k++) r += red[k];
-  return r;
+#pragma unroll
+  for (int j = 0; j < NV; j++) {
+    double r = rb[j * kWaves];
+    for (int k = 1; k < kWaves; k++) r += rb[j * kWaves + k];
+    v[j] = r;
+  }
+  buf ^= 1;
 }
 
 template <typename T, int D, int OP, int CPT>
@@ -35,7 +50,9 @@ __global__ void __launch_bounds__(kThreads) patch_bicgstab_kernel(
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* pad = reinterpret_cast<T*>(smem_raw);  // padded patch of the operand being applied
-  __shared__ double red[kThreads / 64];
+  T* cfl = pad + L.cs;                      // VarPoisson: the patch's coefficients (with ghosts)
+  __shared__ double red[2][2 * kWaves];
+  int rbuf = 0;
   const int p = blockIdx.x;
   const DevPatch& P = dp[p];
   const int n0 = L.n[0], n1 = L.n[1], n2 = D == 3 ? L.n[2] : 1;
@@ -56,12 +73,15 @@ __global__ void __launch_bounds__(kThreads) patch_bicgstab_kernel(
     gidx[k] = lin(L, 0, 0, x, y, z);
   }
   const int base = p * L.ps;
+  if (OP == 1)
+    for (int k = threadIdx.x; k < L.cs; k += kThreads) cfl[k] = coeffs[base + k];
 
   // Operator apply on the padded LDS copy: applySinglePatchWithInternalBoundaryConditions
   // (Poisson/StarPatchOperator.h:75-101 with internal=true): every side ghost is
   // -interior, except physical Neumann sides (+interior).
+  // (no barrier before the writes: a reduction, with its barrier, separates any two
+  // applies, so the previous apply's reads are done)
   auto apply = [&](const T (&v)[CPT], T (&out)[CPT]) {
-    __syncthreads();
     for (int k = 0; k < CPT; k++)
       if (act[k]) pad[gidx[k]] = v[k];
     __syncthreads();
@@ -91,7 +111,7 @@ __global__ void __launch_bounds__(kThreads) patch_bicgstab_kernel(
         if (OP == 0) {
           acc += (upper - T(2) * mid + lower) * ih2[a];
         } else {
-          T cm = coeffs[base + g], cl = coeffs[base + g - st[a]], cu = coeffs[base + g + st[a]];
+          T cm = cfl[g], cl = cfl[g - st[a]], cu = cfl[g + st[a]];
           acc += ((cu + cm) * (upper - mid) - (cl + cm) * (mid - lower)) * (ih2[a] * T(0.5));
         }
       }
@@ -99,10 +119,23 @@ __global__ void __launch_bounds__(kThreads) patch_bicgstab_kernel(
     }
   };
   auto dot = [&](const T (&a)[CPT], const T (&b)[CPT]) {
-    double s = 0.0;
+    double v[1] = { 0.0 };
     for (int k = 0; k < CPT; k++)
-      if (act[k]) s += double(a[k]) * double(b[k]);
-    return wg_sum(s, red);
+      if (act[k]) v[0] += double(a[k]) * double(b[k]);
+    wg_sums<1>(v, red, rbuf);
+    return v[0];
+  };
+  // a.b and a.c together
+  auto dot2 = [&](const T (&a)[CPT], const T (&b)[CPT], const T (&c)[CPT], double* ab, double* ac) {
+    double v[2] = { 0.0, 0.0 };
+    for (int k = 0; k < CPT; k++)
+      if (act[k]) {
+        v[0] += double(a[k]) * double(b[k]);
+        v[1] += double(a[k]) * double(c[k]);
+      }
+    wg_sums<2>(v, red, rbuf);
+    *ab = v[0];
+    *ac = v[1];
   };
 
   T x[CPT], b[CPT], r[CPT], rhat[CPT], pv[CPT], ap[CPT], s[CPT], as[CPT];
@@ -154,12 +187,15 @@ __global__ void __launch_bounds__(kThreads) patch_bicgstab_kernel(
         break;
       }
       apply(s, as);
-      double omega = dot(as, s) / dot(as, as);
+      double as_s, as_as;
+      dot2(as, s, as, &as_s, &as_as);
+      double omega = as_s / as_as;
       for (int k = 0; k < CPT; k++) {
         x[k] += pv[k] * T(alpha) + s[k] * T(omega);
         r[k] += ap[k] * T(-alpha) + as[k] * T(-omega);
       }
-      double rho_new = dot(r, rhat);
+      double rho_new, rr;
+      dot2(r, rhat, r, &rho_new, &rr);
       double beta = rho_new * alpha / (rho * omega);
       for (int k = 0; k < CPT; k++) {
         pv[k] += ap[k] * T(-omega);
@@ -167,7 +203,7 @@ __global__ void __launch_bounds__(kThreads) patch_bicgstab_kernel(
       }
       its++;
       rho = rho_new;
-      residual = sqrt(dot(r, r)) / r0_norm;
+      residual = sqrt(rr) / r0_norm;
     }
   }
   for (int k = 0; k < CPT; k++)
@@ -183,7 +219,7 @@ int launch_cpt(const te_domain* d, int cpt, const T* coeffs, const T* f, T* u, i
                int32_t* iters, int32_t* bd, hipStream_t s)
 {
   Layout L = make_layout(d->dim, d->ns, 1, d->np);
-  size_t shmem = sizeof(T) * (size_t)L.cs;
+  size_t shmem = sizeof(T) * (size_t)L.cs * (OP == 1 ? 2 : 1);
 #define TE_PB(C_)                                                                                            \
   hipLaunchKernelGGL((patch_bicgstab_kernel<T, D, OP, C_>), dim3(d->np), dim3(kThreads), shmem, s, d->d_patches, \
                      coeffs, f, u, L, neumann, max_it, tol, iters, bd)
