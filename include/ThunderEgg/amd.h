@@ -159,6 +159,47 @@ class Communicator {
 /* GhostFillingType.h */
 enum class GhostFillingType { Faces, Edges, Corners };
 
+/* Side<D> = Face<D, D - 1> (Face.h:41-584): 0 west, 1 east, 2 south, 3 north, 4 bottom,
+ * 5 top; axis = value / 2, lower side = even value. */
+template <int D>
+class Side {
+ public:
+  constexpr explicit Side(int value = 0) : v_(value) {}
+  static constexpr Side west() { return Side(0); }
+  static constexpr Side east() { return Side(1); }
+  static constexpr Side south() { return Side(2); }
+  static constexpr Side north() { return Side(3); }
+  static constexpr Side bottom() { return Side(4); }
+  static constexpr Side top() { return Side(5); }
+  size_t getIndex() const { return (size_t)v_; }
+  size_t getAxisIndex() const { return (size_t)(v_ >> 1); }
+  bool isLowerOnAxis() const { return (v_ & 1) == 0; }
+  bool isHigherOnAxis() const { return (v_ & 1) == 1; }
+  bool operator==(const Side& o) const { return v_ == o.v_; }
+
+ private:
+  int v_;
+};
+
+/* The PatchInfo<D> fields that locate a patch's cells (PatchInfo.h:50-140); the default
+ * is one cell of unit size at the origin, as the reference's default constructor. */
+template <int D>
+struct PatchInfo {
+  int id = 0;
+  int local_index = 0;
+  int refine_level = -1;
+  int num_ghost_cells = 1;
+  std::array<int, D> ns;
+  std::array<double, D> starts;
+  std::array<double, D> spacings;
+  PatchInfo()
+  {
+    ns.fill(1);
+    starts.fill(0.0);
+    spacings.fill(1.0);
+  }
+};
+
 /* Domain<D> (Domain.h:94-248): the patches of one level, local index = position. */
 template <int D>
 class Domain {
@@ -187,6 +228,22 @@ class Domain {
   }
   long getNumGlobalCells() const { return getNumLocalCells(); }
   const std::vector<te_patch_info>& getPatchInfoVector() const { return patches_; }
+  /* the cell-locating fields of one patch, with the domain's ns and ghost width */
+  PatchInfo<D> getPatchInfo(int local_index) const
+  {
+    const te_patch_info& r = patches_.at((size_t)local_index);
+    PatchInfo<D> pi;
+    pi.id = r.id;
+    pi.local_index = local_index;
+    pi.refine_level = r.refine_level;
+    pi.num_ghost_cells = num_ghost_;
+    for (int a = 0; a < D; a++) {
+      pi.ns[a] = ns_[a];
+      pi.starts[a] = r.starts[a];
+      pi.spacings[a] = r.spacings[a];
+    }
+    return pi;
+  }
   Communicator getCommunicator() const { return {}; }
   te_domain* handle() const { return h_.get(); }
   bool operator==(const Domain& o) const { return h_ == o.h_; }

@@ -231,6 +231,10 @@ int te_vec_dot(const te_domain* d, int dtype, const void* a, const void* b, int 
                te_stream s);
 int te_vec_two_norm(const te_domain* d, int dtype, const void* a, int ncomp, double* out, te_stream s);
 int te_vec_inf_norm(const te_domain* d, int dtype, const void* a, int ncomp, double* out, te_stream s);
+/* DomainTools::Integrate (DomainTools.h:513-536): the sum over patches and components of
+ * the interior values times the patch's cell volume (product of its spacings);
+ * synchronous, result in *out. */
+int te_vec_integrate(const te_domain* d, int dtype, const void* u, int ncomp, double* out, te_stream s);
 /* Element-type conversion of a whole padded vector (dst_dtype <- src_dtype). */
 int te_vec_convert(const te_domain* d, int dst_dtype, void* dst, int src_dtype, const void* src, int ncomp,
                    te_stream s);

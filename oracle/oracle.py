@@ -48,7 +48,7 @@ def lib():
         for f in ("ora_vec_dot",):
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [vp, dp, dp, C.c_int]
-        for f in ("ora_vec_two_norm", "ora_vec_inf_norm"):
+        for f in ("ora_vec_two_norm", "ora_vec_inf_norm", "ora_vec_integrate"):
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [vp, dp, C.c_int]
         L.ora_dft_matrix.argtypes = [C.c_int, C.c_int, dp]
@@ -239,6 +239,10 @@ def two_norm(dom, a):
 
 def inf_norm(dom, a):
     return lib().ora_vec_inf_norm(dom.h, _dp(a), a.shape[1])
+
+
+def integrate(dom, u):
+    return lib().ora_vec_integrate(dom.h, _dp(u), u.shape[1])
 
 
 def dft_matrix(kind: int, n: int):

@@ -45,6 +45,9 @@
  *  - CG: converges to the dense direct solve (plain, V- and W-preconditioned) on a
  *    symmetric uniform-mesh operator, the preconditioned ones in fewer iterations; a zero
  *    right-hand side leaves x at zero (the reference zeroes x before testing ||b||).
+ *  - Integrate: the midpoint rule's closed forms on every level of the fixture meshes
+ *    (the integral of 1 is the meshes' box volume; products of affine functions exact;
+ *    x^2 short by h^2 / 12 per unit volume of each patch); ghosts do not enter.
  *  Parity unpinned: the Neumann variants (DCT transforms inside the solve, Neumann
  *  physical ghosts) are checked only through the matrices above, and the inner
  *  BiCGStab patch smoother only through its convergence to the patch solve.
@@ -898,6 +901,23 @@ double ora_vec_two_norm(const ora_domain* d, const double* a, int ncomp)
 {
   vspace v = { d, d->np, ncomp };
   return vs_norm(&v, a);
+}
+/* DomainTools::Integrate (DomainTools.h:513-536): per patch and component, the sum of the
+ * interior values, multiplied by each spacing in axis order, added over patches. */
+double ora_vec_integrate(const ora_domain* d, const double* u, int ncomp)
+{
+  double sum = 0;
+  for (int p = 0; p < d->np; p++) {
+    for (int c = 0; c < ncomp; c++) {
+      double ps = 0;
+      for (int z = 0; z < (d->dim == 3 ? d->ns[2] : 1); z++)
+        for (int y = 0; y < d->ns[1]; y++)
+          for (int x = 0; x < d->ns[0]; x++) ps += u[cell_index(d, p, c, ncomp, (int[3]){ x, y, z })];
+      for (int a = 0; a < d->dim; a++) ps *= d->p[p].spacings[a];
+      sum += ps;
+    }
+  }
+  return sum;
 }
 /* Vector::infNorm (Vector.h:568-579) is max(0, max v), not max|v|. */
 double ora_vec_inf_norm(const ora_domain* d, const double* a, int ncomp)

@@ -127,6 +127,8 @@ int ora_bicgstab_solve(const ora_level* levels, int nlevels, int use_precond, in
 double ora_vec_dot(const ora_domain* d, const double* a, const double* b, int ncomp);
 double ora_vec_two_norm(const ora_domain* d, const double* a, int ncomp);
 double ora_vec_inf_norm(const ora_domain* d, const double* a, int ncomp);
+/* DomainTools::Integrate (DomainTools.h:513-536). */
+double ora_vec_integrate(const ora_domain* d, const double* u, int ncomp);
 
 /* DFTPatchSolver transform matrices (Poisson/DFTPatchSolver.h:131-196): type 0 DCT_II,
  * 1 DCT_III, 2 DCT_IV, 3 DST_II, 4 DST_III, 5 DST_IV; out[i*n+j] row-major. */

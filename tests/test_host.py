@@ -178,13 +178,13 @@ def test_uniform_levels_shape():
     assert (p["nbr_type"][:, 4:8] >= -1).all()
 
 
-def _build_api_test(tmp):
-    """g++ build of tests/cpp/api_test.cpp against include/ThunderEgg (the reference-shaped
+def _build_api_test(tmp, name="api_test"):
+    """g++ build of tests/cpp/<name>.cpp against include/ThunderEgg (the reference-shaped
     C++ API) and the C ABI library; host compiler only."""
     import subprocess
-    exe = os.path.join(str(tmp), "api_test")
+    exe = os.path.join(str(tmp), name)
     cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(ROOT, "include"),
-           os.path.join(ROOT, "tests", "cpp", "api_test.cpp"), "-L" + os.path.join(ROOT, "thunderegg_amd"),
+           os.path.join(ROOT, "tests", "cpp", name + ".cpp"), "-L" + os.path.join(ROOT, "thunderegg_amd"),
            "-lthunderegg_amd", "-Wl,-rpath," + os.path.join(ROOT, "thunderegg_amd"), "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr

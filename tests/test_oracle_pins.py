@@ -21,7 +21,9 @@ transposed operand fails it:
                   (test/Iterative/BiCGStab_MPI1.cpp);
   V-cycle         one patch and one level: the cycle is the exact inverse; two levels:
                   the cycle equals its definition (VCycle.h:46-70) composed from the
-                  separately pinned smoother, operator, restrictor and interpolator.
+                  separately pinned smoother, operator, restrictor and interpolator;
+  Integrate       the midpoint rule's closed forms on the fixture meshes (exact for
+                  products of affine functions, the known h^2/12 shortfall for x^2).
 """
 import numpy as np
 import pytest
@@ -560,3 +562,51 @@ def test_varpoisson_second_order():
         e = mesh.interior(f - fe, dim)
         errs.append(np.sqrt((e * e).sum() / (mesh.interior(fe, dim) ** 2).sum()))
     assert np.log(errs[0] / errs[1]) / np.log(2) > 1.8
+
+
+def _box(level, dim, ns):
+    lo = level["starts"][:, :dim].min(axis=0)
+    hi = (level["starts"][:, :dim] + level["spacings"][:, :dim] * np.array(ns[:dim])).max(axis=0)
+    return lo, hi
+
+
+@pytest.mark.parametrize("name", MESH_2D + MESH_3D)
+def test_integrate_midpoint_rule(name):
+    """DomainTools::Integrate (DomainTools.h:513-536) is the midpoint rule over each cell,
+    with each patch's own cell volume.  Every level of a fixture mesh tiles its bounding
+    box, so the integral of 1 is the box volume; products of affine functions of each
+    coordinate are integrated exactly; x0^2 is short by h0^2 / 12 times each patch's
+    volume (the midpoint rule's known error, which differs between refinement levels);
+    ghost values and the component split do not enter."""
+    dim = 2 if name.startswith("2d") else 3
+    ns = (4, 6) if dim == 2 else (4, 4, 2)
+    dim, levels = load(name, ns)
+    coef = (1.0, 2.0, -3.0)
+    for lv in levels:
+        dom = O.Domain(dim, ns, lv)
+        lo, hi = _box(lv, dim, ns)
+        X = mesh.cell_centers(lv, dim, ns)
+        vol = np.prod(hi - lo)
+        one = np.ones(mesh.padded_shape(len(lv), dim, ns))
+        assert abs(O.integrate(dom, one) - vol) <= 1e-13 * vol
+        f = np.ones(X.shape[:-1])
+        exact = 1.0
+        for a in range(dim):
+            f = f * (1.0 + coef[a] * X[..., a])
+            exact *= (hi[a] - lo[a]) + coef[a] * (hi[a] ** 2 - lo[a] ** 2) / 2.0
+        fv = f[:, None].copy()
+        garbage = random_field(fv.shape, 7) * 1e6
+        fg = garbage.copy()
+        mesh.interior(fg, dim)[...] = mesh.interior(fv, dim)
+        assert abs(O.integrate(dom, fg) - exact) <= 1e-12 * abs(exact)
+        # x0^2: exact minus h0^2/12 per unit volume of each patch
+        q = (X[..., 0] ** 2)[:, None].copy()
+        pvol = np.prod(lv["spacings"][:, :dim] * np.array(ns[:dim]), axis=1)
+        exact_q = (hi[0] ** 3 - lo[0] ** 3) / 3.0 * np.prod((hi - lo)[1:])
+        midpoint_q = exact_q - np.sum(lv["spacings"][:, 0] ** 2 / 12.0 * pvol)
+        assert abs(O.integrate(dom, q) - midpoint_q) <= 1e-12 * abs(exact_q)
+        if len(set(lv["refine_level"])) > 1:
+            assert abs(midpoint_q - exact_q) > 1e-6 * abs(exact_q)
+        # two components: the sum of the two integrals
+        two = np.concatenate([fg, q], axis=1)
+        assert abs(O.integrate(dom, two) - (exact + midpoint_q)) <= 1e-12 * (abs(exact) + abs(midpoint_q))

@@ -751,6 +751,10 @@ int te_vec_inf_norm(const te_domain* d, int dtype, const void* a, int ncomp, dou
 {
   return reduce_sync(d, dtype, RED_MAX, a, nullptr, ncomp, out, s);
 }
+int te_vec_integrate(const te_domain* d, int dtype, const void* u, int ncomp, double* out, te_stream s)
+{
+  return reduce_sync(d, dtype, RED_INTEGRAL, u, nullptr, ncomp, out, s);
+}
 int te_vec_convert(const te_domain* d, int dst_dtype, void* dst, int src_dtype, const void* src, int ncomp,
                    te_stream s)
 {

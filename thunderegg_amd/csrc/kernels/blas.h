@@ -20,7 +20,8 @@ enum VecOp {
   VOP_SCALE_THEN_ADD_SCALED2,
 };
 
-enum RedOp { RED_SUM = 0, RED_DOT = 0, RED_SUMSQ = 1, RED_MAX = 2 };
+// RED_INTEGRAL: sum of the interior values times their patch's cell volume
+enum RedOp { RED_SUM = 0, RED_DOT = 0, RED_SUMSQ = 1, RED_MAX = 2, RED_INTEGRAL = 3 };
 
 constexpr int kMaxPartials = 1024;
 

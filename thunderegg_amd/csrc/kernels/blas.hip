@@ -74,7 +74,8 @@ namespace {
 
 template <typename T, int OP>
 __global__ void __launch_bounds__(256) reduce_partial_kernel(Layout L, const T* __restrict__ a,
-                                                             const T* __restrict__ b, double* __restrict__ partial)
+                                                             const T* __restrict__ b, const DevPatch* __restrict__ dp,
+                                                             double* __restrict__ partial)
 {
   long total = (long)L.np * L.ps;
   double acc = 0.0;  // max starts at 0: Vector::infNorm is max(0, max v)
@@ -82,9 +83,16 @@ __global__ void __launch_bounds__(256) reduce_partial_kernel(Layout L, const T* 
     int q;
     if (!interior_of(L, (int)i, &q)) continue;
     double va = double(a[i]);
-    if (OP == RED_DOT) acc += va * double(b[i]);
-    else if (OP == RED_SUMSQ) acc += va * va;
-    else acc = fmax(acc, va);
+    if (OP == RED_DOT) {
+      acc += va * double(b[i]);
+    } else if (OP == RED_SUMSQ) {
+      acc += va * va;
+    } else if (OP == RED_INTEGRAL) {
+      const double* h = dp[(int)i / L.ps].h;  // h[2] = 1 in 2D
+      acc += va * (h[0] * h[1] * h[2]);
+    } else {
+      acc = fmax(acc, va);
+    }
   }
   double r = block_reduce<OP>(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = r;
@@ -137,15 +145,18 @@ int launch_reduce(const te_domain* d, int dtype, int op, const void* a, const vo
   Layout L = make_layout(d->dim, d->ns, ncomp, d->np);
   int g = reduce_grid(L);
   double* part = d->d_scratch;
-#define TE_RP(T, OP_)                                                                                         \
-  hipLaunchKernelGGL((reduce_partial_kernel<T, OP_>), dim3(g), dim3(256), 0, s, L, (const T*)a, (const T*)b, part)
+#define TE_RP(T, OP_)                                                                                     \
+  hipLaunchKernelGGL((reduce_partial_kernel<T, OP_>), dim3(g), dim3(256), 0, s, L, (const T*)a, (const T*)b, \
+                     d->d_patches, part)
   if (dtype == TE_F64) {
     if (op == RED_DOT) TE_RP(double, RED_DOT);
     else if (op == RED_SUMSQ) TE_RP(double, RED_SUMSQ);
+    else if (op == RED_INTEGRAL) TE_RP(double, RED_INTEGRAL);
     else TE_RP(double, RED_MAX);
   } else {
     if (op == RED_DOT) TE_RP(float, RED_DOT);
     else if (op == RED_SUMSQ) TE_RP(float, RED_SUMSQ);
+    else if (op == RED_INTEGRAL) TE_RP(float, RED_INTEGRAL);
     else TE_RP(float, RED_MAX);
   }
 #undef TE_RP

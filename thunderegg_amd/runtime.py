@@ -84,6 +84,7 @@ SIGNATURES = {
     "te_vec_dot": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _dp, _vp]),
     "te_vec_two_norm": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _dp, _vp]),
     "te_vec_inf_norm": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _dp, _vp]),
+    "te_vec_integrate": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _dp, _vp]),
     "te_vec_convert": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp]),
     "te_gmg_create": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "te_gmg_destroy": (C.c_int, [_vp]),
@@ -248,6 +249,13 @@ class Domain:
     def inf_norm(self, a: DeviceArray, stream=None):
         out = C.c_double()
         check(lib().te_vec_inf_norm(self.h, _dt(a), a.ptr, a.shape[1], C.byref(out), stream))
+        return out.value
+
+    def integrate(self, u: DeviceArray, stream=None):
+        """DomainTools::Integrate (DomainTools.h:513-536): interior values times cell
+        volume, summed over patches and components."""
+        out = C.c_double()
+        check(lib().te_vec_integrate(self.h, _dt(u), u.ptr, u.shape[1], C.byref(out), stream))
         return out.value
 
 
