@@ -103,7 +103,7 @@ def kernel_rooflines(R, P, stream, reps=50):
     return out
 
 
-def cpu_baseline(P, gpu_iters, sample_iters=3):
+def cpu_baseline(P, gpu_iters, sample_iters=5):
     """The oracle (FP64 C restatement of the reference algorithm, one core) on the same
     problem: a bounded sample of `sample_iters` outer BiCGStab iterations, extrapolated to
     the GPU FP64 solve's iteration count."""
