@@ -45,12 +45,19 @@
  *  - CG: converges to the dense direct solve (plain, V- and W-preconditioned) on a
  *    symmetric uniform-mesh operator, the preconditioned ones in fewer iterations; a zero
  *    right-hand side leaves x at zero (the reference zeroes x before testing ||b||).
+ *  - vector reductions: ghosts excluded, symmetry and bilinearity of dot, twoNorm^2 =
+ *    dot(a, a), infNorm = max(0, max v).
  *  - Integrate: the midpoint rule's closed forms on every level of the fixture meshes
  *    (the integral of 1 is the meshes' box volume; products of affine functions exact;
  *    x^2 short by h^2 / 12 per unit volume of each patch); ghosts do not enter.
- *  Parity unpinned: the Neumann variants (DCT transforms inside the solve, Neumann
- *  physical ghosts) are checked only through the matrices above, and the inner
- *  BiCGStab patch smoother only through its convergence to the patch solve.
+ *  - Neumann variants of the DFT solve: with any set of Neumann sides (2D and 3D) the
+ *    solve is the exact inverse of an independently assembled dense one-patch operator
+ *    (ghost = -u on Dirichlet sides, u on Neumann sides); the zero-mean solution when
+ *    every side is Neumann.
+ *  - inner BiCGStab patch smoother (PatchSolver.h:153-168): one sweep at a tight inner
+ *    tolerance equals one DFT sweep on refined meshes (Dirichlet, Neumann, VarPoisson
+ *    with beta = 1), so its interface terms and internal boundary conditions are those
+ *    of the pinned DFT path.
  */
 #include "te_oracle.h"
 

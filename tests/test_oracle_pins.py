@@ -19,9 +19,17 @@ transposed operand fails it:
   BiCGStab        against a dense direct solve of the assembled operator (brute force
                   on a 64-DOF mesh), zero right-hand side, converged initial guess
                   (test/Iterative/BiCGStab_MPI1.cpp);
+  Neumann DFT     the solve with any set of Neumann sides (2D, 3D) is the exact inverse of
+                  an independently assembled dense one-patch operator (zero-mean solution
+                  when every side is Neumann);
+  patch BiCGStab  one sweep of the inner-BiCGStab patch smoother at a tight tolerance
+                  equals one DFT sweep on refined meshes (Dirichlet, Neumann, and
+                  VarPoisson with beta = 1);
   V-cycle         one patch and one level: the cycle is the exact inverse; two levels:
                   the cycle equals its definition (VCycle.h:46-70) composed from the
                   separately pinned smoother, operator, restrictor and interpolator;
+  reductions      dot / twoNorm / infNorm: ghosts excluded, symmetry, bilinearity,
+                  max(0, max v);
   Integrate       the midpoint rule's closed forms on the fixture meshes (exact for
                   products of affine functions, the known h^2/12 shortfall for x^2).
 """
@@ -610,3 +618,120 @@ def test_integrate_midpoint_rule(name):
         # two components: the sum of the two integrals
         two = np.concatenate([fg, q], axis=1)
         assert abs(O.integrate(dom, two) - (exact + midpoint_q)) <= 1e-12 * (abs(exact) + abs(midpoint_q))
+
+
+def _lap1d(n, h, lo_neumann, hi_neumann):
+    """Second difference on n cell centres; a Dirichlet end reflects with ghost = -u,
+    a Neumann end with ghost = u (the star operator's physical-boundary ghosts)."""
+    t = np.diag(-2.0 * np.ones(n)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    t[0, 0] += 1.0 if lo_neumann else -1.0
+    t[-1, -1] += 1.0 if hi_neumann else -1.0
+    return t / (h * h)
+
+
+def _dense_laplacian(ns, hs, mask):
+    """Dense n x n operator on one patch, x fastest; side s is Neumann when bit s of mask
+    is set (sides W, E, S, N, B, T: axis s // 2, lower when s is even)."""
+    dim = len(ns)
+    A = np.zeros((int(np.prod(ns)),) * 2)
+    for a in range(dim):
+        t = _lap1d(ns[a], hs[a], (mask >> (2 * a)) & 1, (mask >> (2 * a + 1)) & 1)
+        term = np.ones((1, 1))
+        for b in reversed(range(dim)):  # slowest axis first in the Kronecker product
+            term = np.kron(term, t if b == a else np.eye(ns[b]))
+        A += term
+    return A
+
+
+@pytest.mark.parametrize("dim,mask", [(2, 0b0001), (2, 0b0010), (2, 0b0101), (2, 0b0011), (2, 0b1100),
+                                      (2, 0b1111), (3, 0b000011), (3, 0b100101), (3, 0b111111)])
+def test_dft_neumann_solve_is_exact_inverse(dim, mask):
+    """The DFT patch solve with Neumann sides (DCT-II/III, DCT-IV / DST-IV transforms and
+    their eigenvalues, Poisson/DFTPatchSolver.h:272-361, 424-440) is the exact inverse of
+    the one-patch operator assembled here independently (Dirichlet ghost = -u, Neumann
+    ghost = u), anisotropic spacing.  All sides Neumann: the operator is singular; for a
+    zero-mean f the solve gives the zero-mean solution (the constant mode set to 0)."""
+    if dim == 2:
+        ns, name = (12, 8), "2d_uniform_2x2_mpi1.json"
+    else:
+        ns, name = (4, 6, 8), "3d_uniform_2x2x2_mpi1.json"  # TriLinear: even n
+    _, levels = load(name, ns)
+    lv = levels[-1]
+    assert len(lv) == 1
+    d = O.Domain(dim, ns, lv)
+    hs = [lv["spacings"][0][a] for a in range(dim)]
+    A = _dense_laplacian(ns, hs, mask)
+    f = random_field(d.shape(), 30 + mask)
+    fi = mesh.interior(f, dim)[0, 0].reshape(-1)
+    singular = mask == (1 << (2 * dim)) - 1
+    if singular:
+        fi -= fi.mean()
+        mesh.interior(f, dim)[0, 0] = fi.reshape(mesh.interior(f, dim)[0, 0].shape)
+    u = np.zeros_like(f)
+    O.dft_smooth(d, f, u, kind=BIL if dim == 2 else TRI, neumann_mask=mask)
+    ui = mesh.interior(u, dim)[0, 0].reshape(-1)
+    if singular:
+        expect = np.linalg.lstsq(A, fi, rcond=None)[0]
+        expect -= expect.mean()
+        assert abs(ui.mean()) <= 1e-12 * np.abs(ui).max()
+    else:
+        expect = np.linalg.solve(A, fi)
+    assert relerr(ui, expect) <= 1e-11
+    assert relerr(A @ ui, fi) <= 1e-11
+
+
+@pytest.mark.parametrize("name", ["2d_uniform_2x2_refined_nw_mpi1.json", "2d_uniform_8x8_refined_cross_mpi1.json"])
+@pytest.mark.parametrize("neumann", [False, True])
+def test_patch_bicgstab_smoother_matches_dft_smoother(name, neumann):
+    """Iterative::PatchSolver (inner BiCGStab per patch, PatchSolver.h:153-168) and the DFT
+    patch solver solve the same local problems (interface terms from the neighbours'
+    ghosts, internal boundary conditions): with a tight inner tolerance one Poisson sweep
+    of each agrees on refined meshes, Dirichlet or Neumann physical sides; the DFT sweep
+    is pinned above (exact inverse, second-order convergence)."""
+    ns = (8, 6)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    f = random_field(d.shape(), 51)
+    u0 = random_field(d.shape(), 52)
+    ud = u0.copy()
+    O.dft_smooth(d, f, ud, kind=BIQ, neumann_mask=0b1111 if neumann else 0)
+    ub = u0.copy()
+    its = O.bicgstab_smooth(d, f, ub, kind=BIQ, neumann=neumann, max_it=2000, tol=1e-14)
+    assert (its > 0).all()
+    assert relerr(mesh.interior(ub, dim), mesh.interior(ud, dim)) <= 1e-9
+    if not neumann:
+        # VarPoisson with beta = 1 (ghosts included) is the same local problem
+        uv = u0.copy()
+        ones = np.ones(d.shape())
+        O.bicgstab_smooth(d, f, uv, kind=BIQ, op_kind=1, coeffs=ones, max_it=2000, tol=1e-14)
+        assert relerr(mesh.interior(uv, dim), mesh.interior(ud, dim)) <= 1e-9
+
+
+def test_vector_reductions():
+    """Vector::dot / twoNorm / infNorm (Vector.h:391-611) over interior cells only: ghost
+    values do not enter; twoNorm^2 = dot(a, a); dot is symmetric and bilinear; infNorm is
+    max(0, max v), so an all-negative vector has infNorm 0 (Vector.h:568-579)."""
+    ns = (6, 4)
+    dim, levels = load("2d_uniform_2x2_refined_nw_mpi1.json", ns)
+    d = O.Domain(dim, ns, levels[0])
+    a = random_field(d.shape(2), 61)
+    b = random_field(d.shape(2), 62)
+    ga, gb = a.copy(), b.copy()
+    ga[...] = 1e9
+    gb[...] = -1e9
+    mesh.interior(ga, dim)[...] = mesh.interior(a, dim)
+    mesh.interior(gb, dim)[...] = mesh.interior(b, dim)
+    assert O.dot(d, ga, gb) == pytest.approx(O.dot(d, a, b), rel=1e-14)
+    assert O.dot(d, a, b) == pytest.approx(O.dot(d, b, a), rel=1e-14)
+    assert O.dot(d, a, 2 * a + b) == pytest.approx(2 * O.dot(d, a, a) + O.dot(d, a, b), rel=1e-12)
+    assert O.two_norm(d, ga) ** 2 == pytest.approx(O.dot(d, a, a), rel=1e-14)
+    ones = np.ones(d.shape(2))
+    assert O.dot(d, ones, ones) == 2 * len(levels[0]) * ns[0] * ns[1]
+    neg = -np.abs(a) - 1.0
+    neg_g = neg.copy()
+    neg_g[...] = 5.0
+    mesh.interior(neg_g, dim)[...] = mesh.interior(neg, dim)
+    assert O.inf_norm(d, neg_g) == 0.0
+    mesh.interior(neg_g, dim)[1, 1, 2, 3] = 0.25
+    assert O.inf_norm(d, neg_g) == 0.25
