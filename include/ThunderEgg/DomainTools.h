@@ -9,6 +9,7 @@
 #pragma once
 
 #include <functional>
+#include <type_traits>
 
 #include "amd.h"
 
@@ -126,10 +127,11 @@ class DomainTools {
             [&](const std::array<double, 3>& c) { return func(c[0], c[1], c[2]); });
   }
   /* one function per component, from component 0; more functions than components
-   * throw RuntimeError once the components run out (DomainTools.h:51-88, 340-346) */
-  template <int D, typename... Args>
-  static void SetValues(const Domain<D>& domain, Vector<D>& vec,
-                        std::function<double(const std::array<double, D>&)> func, Args... args)
+   * throw RuntimeError once the components run out (DomainTools.h:51-88, 340-346).  The
+   * first function's type is a template parameter kept off integers, so that
+   * SetValues<D>(domain, vec, 0, f) selects the component-index overload. */
+  template <int D, typename F, typename... Args, typename = typename std::enable_if<!std::is_integral<F>::value>::type>
+  static void SetValues(const Domain<D>& domain, Vector<D>& vec, F func, Args... args)
   {
     setAll<D>(domain, vec, 0, false, func, args...);
   }
@@ -153,9 +155,8 @@ class DomainTools {
     fill<3>(domain, vec, component_index, true,
             [&](const std::array<double, 3>& c) { return func(c[0], c[1], c[2]); });
   }
-  template <int D, typename... Args>
-  static void SetValuesWithGhost(const Domain<D>& domain, Vector<D>& vec,
-                                 std::function<double(const std::array<double, D>&)> func, Args... args)
+  template <int D, typename F, typename... Args, typename = typename std::enable_if<!std::is_integral<F>::value>::type>
+  static void SetValuesWithGhost(const Domain<D>& domain, Vector<D>& vec, F func, Args... args)
   {
     setAll<D>(domain, vec, 0, true, func, args...);
   }

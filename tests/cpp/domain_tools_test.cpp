@@ -181,6 +181,9 @@ static void device()
   CHECK(near(in, 0.0, 1e-13));
   DomainTools::SetValues(d, u, 0, [](double x, double y) { return (1 + x) * (3 + 2 * y); });
   CHECK(near(DomainTools::Integrate<2>(d, u), 1.5 * 4.0, 1e-13));
+  // a literal component index with an array-argument function selects the indexed overload
+  DomainTools::SetValues<2>(d, u, 0, [](const std::array<double, 2>& c) { return c[0] * c[1]; });
+  CHECK(near(DomainTools::Integrate<2>(d, u), 0.25, 1e-13));
   u.setWithGhost(7.0);
   u.set(1.0);  // ghosts stay 7: they must not enter
   CHECK(near(DomainTools::Integrate<2>(d, u), 1.0, 1e-13));
