@@ -261,19 +261,17 @@ __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double
                                                           double* __restrict__ scal, double tol,
                                                           double* __restrict__ mirror)
 {
-  const bool run = mode == FIN_INIT || scal[SC_STATE] == ST_RUN;
+  // the state and the partials are loaded together (the partial buffers are always valid
+  // memory; after the s-norm test has fired their fold is simply not used)
+  const double state = scal[SC_STATE];
+  const double* const parts[3] = { pa, pb, pc };
+  double r[3] = { 0.0, 0.0, 0.0 };
+  if (mode == FIN_SOMEGA) fold_n<3>(parts, n, r);
+  else if (mode == FIN_ALPHA || mode == FIN_SNORM) fold_n<1>(parts, n, r);
+  else fold_n<2>(parts, n, r);
+  const bool run = mode == FIN_INIT || state == ST_RUN;
   if (!run && mode != FIN_RHO) return;
-  double A = 0.0, B = 0.0, C = 0.0;
-  if (run) {
-    const double* const parts[3] = { pa, pb, pc };
-    double r[3] = { 0.0, 0.0, 0.0 };
-    if (mode == FIN_SOMEGA) fold_n<3>(parts, n, r);
-    else if (mode == FIN_ALPHA || mode == FIN_SNORM) fold_n<1>(parts, n, r);
-    else fold_n<2>(parts, n, r);
-    A = r[0];
-    B = r[1];
-    C = r[2];
-  }
+  const double A = r[0], B = r[1], C = r[2];
   if (threadIdx.x != 0) return;
   switch (mode) {
     case FIN_INIT:
