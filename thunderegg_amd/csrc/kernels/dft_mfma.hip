@@ -525,6 +525,9 @@ __global__ void __launch_bounds__(64 * kWaves) bicg_sweep_kernel(const DevPatch*
   float* s2 = S3 + n * ld;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float* X = s2 + n + wave * n * ld;
+  // the scalars are loaded together with the transform matrices
+  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], beta = scal[SC_BETA];
+  const double state = scal[SC_STATE];
   for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
     const int row = k / n, col = k - row * n;
     Sm[row * ld + col] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
@@ -532,8 +535,7 @@ __global__ void __launch_bounds__(64 * kWaves) bicg_sweep_kernel(const DevPatch*
   }
   for (int k = threadIdx.x; k < n; k += blockDim.x) s2[k] = s2tab[(2 * 3 + 0) * kDftMatDim + k];
   __syncthreads();
-  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], beta = scal[SC_BETA];
-  const bool run = scal[SC_STATE] == ST_RUN;
+  const bool run = state == ST_RUN;
   double sa = 0.0;
   if (run) {
     const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, np, kWaves);
