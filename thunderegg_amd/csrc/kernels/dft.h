@@ -45,8 +45,11 @@ int launch_rr_pre(const te_ghost_filler* g, const te_level_link* link, int op_ki
 // BiCGStab's s update (mode BICG_S, with the s.s partials in pc: nslots slots, those past
 // the launch's workgroups zeroed) or p update (BICG_P) fused with the zero-guess FP32 sweep of
 // plan pl's level (2D, 16 x 16) on the updated vector: v0 (FP64) and its FP32 copy w32 are
-// written as by launch_bicg, and the sweep's result goes to out.
+// written as by launch_bicg, and the sweep's result goes to out.  pfold (BICG_S only, may be
+// null): the nfold rhat.ap partials; the launch then also does the FIN_ALPHA finalize (alpha
+// = rho / their fold, stored in scal), so that finalize is not launched.
 bool bicg_sweep_supported(const te_dft_plan* pl);
-int launch_bicg_sweep(int mode, const te_dft_plan* pl, const double* scal, double* v0, const double* a0,
-                      const double* a1, float* w32, float* out, double* pc, int nslots, hipStream_t s);
+int launch_bicg_sweep(int mode, const te_dft_plan* pl, double* scal, double* v0, const double* a0,
+                      const double* a1, float* w32, float* out, double* pc, int nslots, hipStream_t s,
+                      const double* pfold = nullptr, int nfold = 0);
 }  // namespace te

@@ -27,6 +27,7 @@
 #include "dft_wave.h"
 #include "ghost2d.h"
 #include "krylov.h"
+#include "fold.h"
 
 namespace te {
 namespace {
@@ -508,16 +509,21 @@ void launch_rp(const te_ghost_filler* g, const te_level_link* link, int op_kind,
 // patch in FP64, writes it and its FP32 copy, and solves the patch on the FP32 values it has
 // just formed (the same values the separate sweep would read back from the copy), writing
 // the level's solution buffer.  The updates are those of bicg_kernel.
+// pfold (BICG_S, may be null): the rhat.ap partials of the operator launch; each workgroup
+// folds them (as the finalize would, same order and bits) into alpha = rho / rhat.ap, and
+// workgroup 0 stores alpha, so that no separate finalize launch is needed.
 template <int MODE>
 __global__ void __launch_bounds__(64 * kWaves) bicg_sweep_kernel(const DevPatch* __restrict__ dp,
                                                                  const float* __restrict__ mats,
                                                                  const float* __restrict__ s2tab,
-                                                                 const double* __restrict__ scal,
+                                                                 double* __restrict__ scal,
                                                                  double* __restrict__ v0, const double* __restrict__ a0,
                                                                  const double* __restrict__ a1, float* __restrict__ w32,
                                                                  float* __restrict__ out, double* __restrict__ pc,
-                                                                 int nslots, int np)
+                                                                 int nslots, int np, const double* __restrict__ pfold,
+                                                                 int nfold)
 {
+  static_assert(64 * kWaves == kFoldThreads, "the in-kernel fold needs the fold's block size");
   constexpr int n = 16, ld = n + 1, SX = n + 2, PS = SX * SX;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float* Sm = reinterpret_cast<float*>(smem_raw);
@@ -525,9 +531,16 @@ __global__ void __launch_bounds__(64 * kWaves) bicg_sweep_kernel(const DevPatch*
   float* s2 = S3 + n * ld;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float* X = s2 + n + wave * n * ld;
-  // the scalars are loaded together with the transform matrices
-  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], beta = scal[SC_BETA];
+  // the scalars (and the partials of alpha) are loaded together with the transform matrices
+  const bool fold = MODE == BICG_S && pfold != nullptr;
+  double alpha = fold ? 0.0 : scal[SC_ALPHA];
+  const double omega = scal[SC_OMEGA], beta = scal[SC_BETA], rho = scal[SC_RHO];
   const double state = scal[SC_STATE];
+  FoldRegs<1> fr;
+  if (fold) {
+    const double* const parts[3] = { pfold, nullptr, nullptr };
+    fold_load<1>(parts, nfold, fr);
+  }
   for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
     const int row = k / n, col = k - row * n;
     Sm[row * ld + col] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
@@ -536,6 +549,12 @@ __global__ void __launch_bounds__(64 * kWaves) bicg_sweep_kernel(const DevPatch*
   for (int k = threadIdx.x; k < n; k += blockDim.x) s2[k] = s2tab[(2 * 3 + 0) * kDftMatDim + k];
   __syncthreads();
   const bool run = state == ST_RUN;
+  if (fold) {
+    double r[3] = { 0.0, 0.0, 0.0 };
+    fold_reduce<1>(fr, r);
+    alpha = rho / r[0];  // FIN_ALPHA: alpha = rho / rhat.dot(ap)
+    if (run && blockIdx.x == 0 && threadIdx.x == 0) scal[SC_ALPHA] = alpha;
+  }
   double sa = 0.0;
   if (run) {
     const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, np, kWaves);
@@ -584,13 +603,14 @@ bool bicg_sweep_supported(const te_dft_plan* pl)
          pl->d_s2_32;
 }
 
-int launch_bicg_sweep(int mode, const te_dft_plan* pl, const double* scal, double* v0, const double* a0,
-                      const double* a1, float* w32, float* out, double* pc, int nslots, hipStream_t s)
+int launch_bicg_sweep(int mode, const te_dft_plan* pl, double* scal, double* v0, const double* a0,
+                      const double* a1, float* w32, float* out, double* pc, int nslots, hipStream_t s,
+                      const double* pfold, int nfold)
 {
   const te_domain* d = pl->g->d;
   if (d->np == 0) return TE_OK;
   if (!bicg_sweep_supported(pl) || (mode != BICG_S && mode != BICG_P) || !w32 || !out ||
-      (mode == BICG_S && (!pc || nslots < 1))) {
+      (mode == BICG_S && (!pc || nslots < 1)) || (pfold && (mode != BICG_S || nfold < 1 || nfold > kFoldMax))) {
     set_error("internal: fused BiCGStab update + sweep: 16 x 16 matrix-core plan, s or p update, FP32 buffers");
     return TE_ERR_ARG;
   }
@@ -603,10 +623,10 @@ int launch_bicg_sweep(int mode, const te_dft_plan* pl, const double* scal, doubl
   if (mode == BICG_S && grid > nslots) grid = nslots;
   if (mode == BICG_S)
     hipLaunchKernelGGL((bicg_sweep_kernel<BICG_S>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches,
-                       pl->d_mats32, pl->d_s2_32, scal, v0, a0, a1, w32, out, pc, nslots, d->np);
+                       pl->d_mats32, pl->d_s2_32, scal, v0, a0, a1, w32, out, pc, nslots, d->np, pfold, nfold);
   else
     hipLaunchKernelGGL((bicg_sweep_kernel<BICG_P>), dim3(grid), dim3(64 * kWaves), shmem, s, d->d_patches,
-                       pl->d_mats32, pl->d_s2_32, scal, v0, a0, a1, w32, out, pc, nslots, d->np);
+                       pl->d_mats32, pl->d_s2_32, scal, v0, a0, a1, w32, out, pc, nslots, d->np, nullptr, 0);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

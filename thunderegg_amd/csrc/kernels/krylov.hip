@@ -13,11 +13,13 @@
 #include "../te_internal.h"
 #include "blas.h"
 #include "krylov.h"
+#include "fold.h"
 
 namespace te {
 namespace {
 
 constexpr int kThreads = 256;
+static_assert(kThreads == kFoldThreads, "the finalize folds with the Krylov block size");
 constexpr int kMaxCells = 24;  // largest per-thread cell count instantiated (3D n = 16: 18^3 / 256 -> 23)
 
 // The cells a thread owns in every patch (at most CPT; unrolled so they stay in registers).
@@ -213,43 +215,6 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
   if (MODE == BICG_INIT || MODE == BICG_XR) {
     double r = block_reduce<RED_SUM>(sb);
     if (threadIdx.x == 0) pb[blockIdx.x] = r;
-  }
-}
-
-// Fixed-order folds of n partials by one workgroup (thread t sums partials t, t + 256, ...
-// in that order, then the block reduction): up to three arrays, every load issued before
-// the first add (n <= kFoldMax, the Krylov grid cap), one block reduction pass for all.
-constexpr int kFoldPer = 8;
-constexpr int kFoldMax = kThreads * kFoldPer;
-static_assert(kFoldMax >= 2048, "the Krylov kernels write at most 2048 partials");
-
-template <int NA>
-__device__ void fold_n(const double* const (&part)[3], int n, double (&out)[3])
-{
-  double v[NA][kFoldPer];
-#pragma unroll
-  for (int a = 0; a < NA; a++)
-#pragma unroll
-    for (int j = 0; j < kFoldPer; j++) {
-      const int k = threadIdx.x + kThreads * j;
-      v[a][j] = k < n ? part[a][k] : 0.0;
-    }
-  __shared__ double wsum[3][kThreads / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int a = 0; a < NA; a++) {
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < kFoldPer; j++) acc += v[a][j];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) wsum[a][w] = acc;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int a = 0; a < NA; a++) {
-    double r = wsum[a][0];
-    for (int k = 1; k < kThreads / 64; k++) r += wsum[a][k];
-    out[a] = r;  // valid in every thread
   }
 }
 
