@@ -531,16 +531,12 @@ __global__ void __launch_bounds__(64 * kWaves) bicg_sweep_kernel(const DevPatch*
   float* s2 = S3 + n * ld;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float* X = s2 + n + wave * n * ld;
-  // the scalars (and the partials of alpha) are loaded together with the transform matrices
+  // the scalars are loaded together with the transform matrices (the partials of alpha
+  // after them: held across the staging they would cost a wave per SIMD of occupancy)
   const bool fold = MODE == BICG_S && pfold != nullptr;
   double alpha = fold ? 0.0 : scal[SC_ALPHA];
   const double omega = scal[SC_OMEGA], beta = scal[SC_BETA], rho = scal[SC_RHO];
   const double state = scal[SC_STATE];
-  FoldRegs<1> fr;
-  if (fold) {
-    const double* const parts[3] = { pfold, nullptr, nullptr };
-    fold_load<1>(parts, nfold, fr);
-  }
   for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
     const int row = k / n, col = k - row * n;
     Sm[row * ld + col] = mats[(3 * 3 + 0) * kDftMatStride + k];  // DST_II, axis 0
@@ -550,8 +546,9 @@ __global__ void __launch_bounds__(64 * kWaves) bicg_sweep_kernel(const DevPatch*
   __syncthreads();
   const bool run = state == ST_RUN;
   if (fold) {
+    const double* const parts[3] = { pfold, nullptr, nullptr };
     double r[3] = { 0.0, 0.0, 0.0 };
-    fold_reduce<1>(fr, r);
+    fold_n<1>(parts, nfold, r);
     alpha = rho / r[0];  // FIN_ALPHA: alpha = rho / rhat.dot(ap)
     if (run && blockIdx.x == 0 && threadIdx.x == 0) scal[SC_ALPHA] = alpha;
   }
