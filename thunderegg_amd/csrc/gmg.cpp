@@ -670,13 +670,12 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->p32 : g->p, mp, dtm, dtm, l0f));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, dtm, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
-    if (l0f) {  // alpha folded inside the fused s update + sweep
+    // the s update folds alpha's partials itself (no FIN_ALPHA launch)
+    if (l0f)
       TE_RC(launch_bicg_sweep(BICG_S, L0.plan, g->scal, g->sv, g->resid, g->ap, g->s32, u0, g->pc, nb, s, g->pa, nb));
-    } else {
-      TE_RC(launch_bicg_finalize(FIN_ALPHA, g->pa, nullptr, nb, g->scal, s));
+    else
       TE_RC(launch_bicg(BICG_S, d, g->scal, g->sv, nullptr, g->resid, g->ap, nullptr, nullptr, nullptr, g->pc,
-                        nullptr, s, TE_F64, m32 ? g->s32 : nullptr));
-    }
+                        nullptr, s, TE_F64, m32 ? g->s32 : nullptr, nullptr, 0, g->pa, nb));
     if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->s32 : g->sv, ms, dtm, dtm, l0f));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, ms, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, ms, dtm, g->as, g->sv, g->pa, g->pb, g->fly, s));

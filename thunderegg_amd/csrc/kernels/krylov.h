@@ -29,10 +29,12 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
 // w32: FP32 copy of the new p (BICG_INIT, BICG_P) or s (BICG_S), or null.
 // xp_dir (BICG_XR): after the s-norm test fired, the launch does x += alpha xp_dir instead
 // (xp_dir of type dtm when xp_dir_m, else FP64).
-int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const void* a0,
+// pfold / nfold (BICG_S only): the rhat.ap partials; the launch then also does the FIN_ALPHA
+// finalize (alpha = rho / their fold, stored in scal), so that finalize is not launched.
+int launch_bicg(int mode, const te_domain* d, double* scal, double* v0, double* v1, const void* a0,
                 const void* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
                 hipStream_t s, int dtm = TE_F64, float* w32 = nullptr, const void* xp_dir = nullptr,
-                int xp_dir_m = 0);
+                int xp_dir_m = 0, const double* pfold = nullptr, int nfold = 0);
 // tol: the relative tolerance, stored by FIN_INIT for the s-norm test of FIN_SNORM.
 // mirror (FIN_RHO): device pointer to host-coherent memory of SC_COUNT doubles; FIN_RHO
 // copies the scalars there and then bumps SC_SEQ, so the host can read an iteration's

@@ -131,8 +131,11 @@ __global__ void __launch_bounds__(kThreads) star_dot_kernel(const DevPatch* __re
 // TM: type of M p and M s (a0, a1 of BICG_XR; a0 of BICG_XP), widened to FP64 as read.
 // w32 (BICG_INIT, BICG_S, BICG_P; may be null): an FP32 copy of the new p or s, the
 // right-hand side the mixed-precision V-cycle reads (it rounds its input to FP32 anyway).
+// pfold (BICG_S, may be null): the nfold rhat.ap partials; every workgroup folds them into
+// alpha = rho / rhat.ap as the finalize would (same order and bits) and workgroup 0 stores
+// alpha, so that no separate finalize launch is needed.
 template <int CPT, int MODE, typename TM>
-__global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* __restrict__ scal,
+__global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, double* __restrict__ scal,
                                                         double* __restrict__ v0, double* __restrict__ v1,
                                                         const void* __restrict__ a0v,
                                                         const void* __restrict__ a1v,
@@ -140,10 +143,13 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
                                                         const double* __restrict__ a3,
                                                         const double* __restrict__ a4, double* __restrict__ pa,
                                                         double* __restrict__ pb, float* __restrict__ w32,
-                                                        const void* __restrict__ dirv, int dir_m)
+                                                        const void* __restrict__ dirv, int dir_m,
+                                                        const double* __restrict__ pfold, int nfold)
 {
   CellSet<CPT> cs;
-  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], beta = scal[SC_BETA];
+  const bool fold = MODE == BICG_S && pfold != nullptr;
+  double alpha = fold ? 0.0 : scal[SC_ALPHA];
+  const double omega = scal[SC_OMEGA], beta = scal[SC_BETA], rho = scal[SC_RHO];
   if (MODE != BICG_INIT) {
     const double st = scal[SC_STATE];
     if (MODE == BICG_XR && dirv && st == ST_EARLY) {
@@ -161,6 +167,13 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, const double* 
       return;
     }
     if (MODE == BICG_XP ? st != ST_EARLY : st != ST_RUN) return;
+  }
+  if (fold) {  // FIN_ALPHA: alpha = rho / rhat.dot(ap)
+    const double* const parts[3] = { pfold, nullptr, nullptr };
+    double r[3] = { 0.0, 0.0, 0.0 };
+    fold_n<1>(parts, nfold, r);
+    alpha = rho / r[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[SC_ALPHA] = alpha;
   }
   typedef typename std::conditional<MODE == BICG_XR || MODE == BICG_XP, TM, double>::type TA;
   const TA* __restrict__ a0 = (const TA*)a0v;
@@ -350,10 +363,14 @@ int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const do
   return TE_OK;
 }
 
-int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, double* v1, const void* a0,
+int launch_bicg(int mode, const te_domain* d, double* scal, double* v0, double* v1, const void* a0,
                 const void* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
-                hipStream_t s, int dtm, float* w32, const void* xp_dir, int xp_dir_m)
+                hipStream_t s, int dtm, float* w32, const void* xp_dir, int xp_dir_m, const double* pfold, int nfold)
 {
+  if (pfold && (mode != BICG_S || nfold < 1 || nfold > kFoldMax)) {
+    set_error("internal: only the s update folds partials (at most kFoldMax)");
+    return TE_ERR_ARG;
+  }
   if ((dtm != TE_F64 && mode != BICG_XR && mode != BICG_XP) ||
       (w32 && mode != BICG_INIT && mode != BICG_S && mode != BICG_P)) {
     set_error("internal: FP32 operands only for the x updates, FP32 copies only of p and s");
@@ -370,10 +387,10 @@ int launch_bicg(int mode, const te_domain* d, const double* scal, double* v0, do
   do {                                                                                                           \
     if (dtm == TE_F32)                                                                                           \
       hipLaunchKernelGGL((bicg_kernel<C_, M_, float>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0, a1, \
-                         a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m);                                             \
+                         a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m, pfold, nfold);                               \
     else                                                                                                         \
       hipLaunchKernelGGL((bicg_kernel<C_, M_, double>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0,   \
-                         a1, a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m);                                         \
+                         a1, a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m, pfold, nfold);                           \
   } while (0)
 #define TE_BK(M_)                      \
   do {                                 \
