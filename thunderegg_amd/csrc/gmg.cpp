@@ -654,7 +654,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // One iteration, enqueued as a whole: the host reads the scalars once per iteration,
   // from the copy FIN_RHO publishes (so the next iteration is enqueued while the p update
   // still runs).
-  //   mp = M p; ap = A mp (+ rhat.ap); alpha; s = r - alpha ap (+ s.s);
+  //   mp = M p; ap = A mp (+ rhat.ap); alpha (folded by the s update's launch);
+  //   s = r - alpha ap (+ s.s);
   //   ms = M s; as = A ms (+ as.s, as.as); ||s|| and the s-norm test, then omega; if the
   //   test fired: x += alpha p (mp with use_precond = 2), and the updates after it are
   //   skipped on the device (the kernels read the state word); else x, r updates; rho,
