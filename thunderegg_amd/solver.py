@@ -1,8 +1,9 @@
 """Host-side driver objects over the C ABI: the forest (mesh generator), the
 multigrid hierarchy and the outer BiCGStab.
 
-These mirror the reference's setup sequence (SURVEY call stack F): a domain
-generator produces the finest and successively coarser domains; every level gets
+These mirror the reference's setup sequence (GMG/CycleBuilder.h:97-192 with
+Iterative/BiCGStab.h:111-203): a domain generator produces the finest and
+successively coarser domains; every level gets
 a ghost filler, operator and smoother; CycleBuilder assembles the V-cycle;
 BiCGStab solves with the cycle as right preconditioner.
 """
