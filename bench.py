@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: DOF/s to a 1e-10 relative residual on one MI355X.
 
-Workload (default --config 2): 2D adaptive Poisson, quadtree levels 4-9 refined
+Workload (default, --config 2 of the numbered workloads below): 2D adaptive Poisson, quadtree levels 4-9 refined
 within 0.05 of 8 seeded random points, 2:1 balanced, 19,930 patches of 16x16
 cells (5.1M DOF), BiQuadratic ghost filling, 5-point star operator, DFT patch
 smoother (1 pre / 1 post / 1 coarse sweep) in a 10-level V-cycle preconditioning
@@ -136,7 +136,7 @@ def cpu_baseline(P, gpu_iters, sample_iters=5):
     }
 
 
-def run_config5(args):
+def run_many_problems(args):
     """Config 5: independent problems solved concurrently (one stream per problem); value =
     aggregate finest-level DOF x rounds / wall time, every problem to 1e-10, mixed precision,
     with the FP64 aggregate beside it."""
@@ -148,10 +148,10 @@ def run_config5(args):
 
     R.lib()
     t0 = time.perf_counter()
-    probs = [configs.config5_problem(k) for k in range(args.problems)]
+    probs = [configs.small_adaptive2d_problem(k) for k in range(args.problems)]
     t_mesh = time.perf_counter() - t0
     dof = sum(P.dof for P in probs)
-    log(f"config 5: {len(probs)} problems, {dof} DOF, patch sizes "
+    log(f"many problems: {len(probs)} problems, {dof} DOF, patch sizes "
         f"{sorted({P.ns[0] for P in probs})}; meshes {t_mesh:.1f}s")
 
     def run(prec):
@@ -191,8 +191,8 @@ def run_config5(args):
         "vs_baseline": None,
         "dtype": "f32 V-cycle / f64 BiCGStab",
         "data": "synthetic (seeded meshes and right-hand sides)",
-        "config": {"workload": f"{len(probs)} independent 2D adaptive problems (config 5), solved concurrently",
-                   "config_id": 5, "problems": len(probs), "dof": dof, "threads": args.threads,
+        "config": {"workload": f"{len(probs)} independent 2D adaptive problems , solved concurrently",
+                   "problems": len(probs), "dof": dof, "threads": args.threads,
                    "patch_sizes": sorted({P.ns[0] for P in probs})},
         "fp64_dof_per_s": dof * args.steps / T64,
         "mixed_over_fp64": T64 / T32,
@@ -207,14 +207,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2,
+                    help="workload: 1 uniform 2D (64 x 32^2, BiCGStab smoother), 2 adaptive 2D (19,930 x 16^2, "
+                         "DFT smoother), 3 uniform 3D (32,768 x 16^3), 4 VarPoisson 2D, 5 independent small problems")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the standalone kernel timings (for counter runs under rocprofv3)")
-    ap.add_argument("--problems", type=int, default=64, help="config 5: number of problems (512 in full)")
-    ap.add_argument("--threads", type=int, default=8, help="config 5: concurrent solves")
+    ap.add_argument("--problems", type=int, default=64, help="many problems: number of problems (512 in full)")
+    ap.add_argument("--threads", type=int, default=8, help="many problems: concurrent solves")
     ap.add_argument("--timer", metavar="FILE",
                     help="after the timed solves, one more mixed-precision solve with a Timer attached: its "
                          "timing tree (Iteration, per-level Total Patch Smooth) saved as JSON to FILE and "
@@ -225,7 +227,7 @@ def main():
                           "n_gpus_requested": args.gpus}))
         sys.exit(2)
     if args.config == 5:
-        run_config5(args)
+        run_many_problems(args)
         return
 
     import numpy as np
@@ -327,7 +329,6 @@ def main():
         "data": "synthetic (seeded mesh and right-hand side)",
         "config": {
             "workload": P.name,
-            "config_id": args.config,
             "patches_finest": len(P.levels[0]),
             "levels": [len(l) for l in P.levels],
             "dof": P.dof,

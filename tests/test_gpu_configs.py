@@ -1,9 +1,10 @@
 """The benchmark's workloads end to end on the GPU, through the C ABI.
 
-config 1 (2D uniform 64 x 32^2, BiLinear, BiCGStab patch smoother) is small enough for
-the oracle: the FP64 solve matches it iteration for iteration.  Configs 2-4 run at their
-full sizes and are checked through size-independent properties: the oracle's V-cycle on
-the same right-hand side (config 2), linearity of the cycle, the true residual of the
+The uniform 2D workload (64 patches of 32^2, BiLinear, BiCGStab patch smoother) is small
+enough for the oracle: the FP64 solve matches it iteration for iteration.  The adaptive 2D
+(19,930 patches of 16^2), uniform 3D (32,768 patches of 16^3) and VarPoisson 2D workloads
+run at their full sizes and are checked through size-independent properties: the
+oracle's V-cycle on the same right-hand side (adaptive 2D), linearity of the cycle, the true residual of the
 FP64 and mixed-precision solves, and agreement of the two solutions.
 """
 import numpy as np
@@ -48,9 +49,9 @@ def _solve(P, g, precond, tol=1e-10, max_it=500):
     return b, x, its, hist
 
 
-def test_config1_matches_oracle():
-    _print_case("config1")
-    P = configs.config1()
+def test_uniform2d_bicgstab_smoother_matches_oracle():
+    _print_case("uniform2d_bicgstab_smoother")
+    P = configs.uniform2d_bicgstab_smoother()
     od = [O.Domain(P.dim, P.ns, lv) for lv in P.levels]
     L = O.Levels(od, filler_kind=P.filler, smoother_kind=P.smoother)
     xref, its_ref, hist_ref = L.bicgstab(P.rhs.copy(), tol=1e-10)
@@ -72,9 +73,9 @@ def test_config1_matches_oracle():
     assert _true_residual(P, gm, b, xm) <= 1.5e-10
 
 
-def test_config2_full_size():
-    _print_case("config2")
-    P = configs.config2()
+def test_adaptive2d_dft_full_size():
+    _print_case("adaptive2d_dft")
+    P = configs.adaptive2d_dft()
     g = _gmg(P, R.F64)
     # one V-cycle at full size against the oracle's
     f = P.rhs
@@ -99,17 +100,17 @@ def test_config2_full_size():
     assert np.sqrt(((a - c) ** 2).sum() / (c ** 2).sum()) <= 1e-8
 
 
-def test_config3_full_size():
+def test_uniform3d_dft_full_size():
     """3D, 134M DOF: TriLinear ghosts, DFT smoother, 2 pre / 2 post sweeps, 5 levels.  The
     solve takes the iterations the oracle's cycle gives on this problem (8 in FP64, 9 mixed;
     a race between patches shows up as more, and as solves that differ), and a second solve
     is bit-identical."""
-    _print_case("config3")
-    P = configs.config3()
+    _print_case("uniform3d_dft")
+    P = configs.uniform3d_dft()
     for prec in (R.F64, R.F32):
         g = _gmg(P, prec)
         b, x, its, hist = _solve(P, g, 2)
-        print("config3", "f64" if prec == R.F64 else "mixed", its, "iterations", flush=True)
+        print("uniform3d_dft", "f64" if prec == R.F64 else "mixed", its, "iterations", flush=True)
         assert its <= (8 if prec == R.F64 else 10)
         assert _true_residual(P, g, b, x) <= 1.5e-10
         x1 = mesh.interior(x.get(), 3)  # (the true residual above filled x's ghost cells)
@@ -119,29 +120,29 @@ def test_config3_full_size():
         del g, b, x
 
 
-def test_config4_full_size():
+def test_varpoisson2d_full_size():
     """2D VarPoisson, 17.5M DOF, beta contrast 1e3, BiCGStab patch smoother: FP64 and mixed
     reach 1e-10 in the true residual."""
-    _print_case("config4")
-    P = configs.config4()
+    _print_case("varpoisson2d_bicgstab_smoother")
+    P = configs.varpoisson2d_bicgstab_smoother()
     sols = []
     for prec in (R.F64, R.F32):
         g = _gmg(P, prec)
         b, x, its, hist = _solve(P, g, 2, max_it=1000)
-        print("config4", "f64" if prec == R.F64 else "mixed", its, "iterations", flush=True)
+        print("varpoisson2d_bicgstab_smoother", "f64" if prec == R.F64 else "mixed", its, "iterations", flush=True)
         assert _true_residual(P, g, b, x) <= 1.5e-10
         sols.append(mesh.interior(x.get(), 2))
     a, c = sols[1], sols[0]
     assert np.sqrt(((a - c) ** 2).sum() / (c ** 2).sum()) <= 1e-7
 
 
-def test_config5_concurrent_problems():
+def test_small_adaptive2d_problems_concurrently():
     """Config 5's independent problems (patch sizes 8, 16 and 32, 30 to 3800 patches) solved
     concurrently, one stream each: the FP64 solves take exactly the iterations of one-at-a-
     time solves (same arithmetic), and FP64 and mixed precision all reach 1e-10 in the true
     residual."""
-    _print_case("config5")
-    probs = [configs.config5_problem(k) for k in range(10)]
+    _print_case("small_adaptive2d_problems")
+    probs = [configs.small_adaptive2d_problem(k) for k in range(10)]
     assert {P.ns[0] for P in probs} >= {8, 32}
     seq = []
     for P in probs:
@@ -158,7 +159,7 @@ def test_config5_concurrent_problems():
             x.zero()
             xs.append(x)
         its = solver.solve_many(gs, bs, xs, tol=1e-10, max_it=500, precond=2, threads=4)
-        print("config5", "f64" if prec == R.F64 else "mixed", its, flush=True)
+        print("small_adaptive2d_problems", "f64" if prec == R.F64 else "mixed", its, flush=True)
         for P, g, b, x, it, it0 in zip(probs, gs, bs, xs, its, seq):
             assert _true_residual(P, g, b, x) <= 1.5e-10
             if prec == R.F64:

@@ -1,9 +1,9 @@
-"""Synthetic problem generators for the five benchmark configurations.
+"""Synthetic problem generators for the five benchmark workloads.
 
 Each returns a Problem: dimension, cells per patch side, the patch tables of
 every multigrid level (finest first), the right-hand side (padded FP64 vector of
 the finest level), solver settings, and the exact solution when one is known.
-Seeds: numpy default_rng(20261200 + config id).
+Seeds: numpy default_rng(20261200 + workload number).
 """
 from __future__ import annotations
 
@@ -46,7 +46,7 @@ def _centers(levels, dim, ns):
     return mesh.cell_centers(levels[0], dim, ns)
 
 
-def config1():
+def uniform2d_bicgstab_smoother():
     """2D Poisson, uniform level 3 (64 patches of 32x32), BiLinear, BiCGStab patch
     smoother, f = -8 pi^2 sin(2 pi x) sin(2 pi y), u = sin(2 pi x) sin(2 pi y)."""
     ns = (32, 32)
@@ -92,7 +92,7 @@ def _gaussian_rhs(levels, dim, ns, pts, amp, wid):
     return f[:, None].copy()
 
 
-def config2(ns=(16, 16), max_level=9, seed=SEED0 + 2):
+def adaptive2d_dft(ns=(16, 16), max_level=9, seed=SEED0 + 2):
     """2D adaptive Poisson: levels 4-9 refined within 0.05 of 8 random points, 2:1
     balanced, 16x16 patches (~20k patches), BiQuadratic filling, DFT smoother, Gaussian-sum RHS."""
     rng = np.random.default_rng(seed)
@@ -105,7 +105,7 @@ def config2(ns=(16, 16), max_level=9, seed=SEED0 + 2):
                    info={"points": pts.tolist(), "amplitudes": amp.tolist(), "widths": wid.tolist(), "seed": seed})
 
 
-def config3(level=5, ns=(16, 16, 16), seed=SEED0 + 3):
+def uniform3d_dft(level=5, ns=(16, 16, 16), seed=SEED0 + 3):
     """3D Poisson, uniform octree level 5 (32^3 patches of 16^3), TriLinear, DFT smoother,
     2 pre / 2 post sweeps, f = -12 pi^2 prod sin(2 pi x_i) + N(0, 1e-3) noise, V-cycle over
     5 levels (32768 -> 8 patches)."""
@@ -119,7 +119,7 @@ def config3(level=5, ns=(16, 16, 16), seed=SEED0 + 3):
                    post=2, coarse=1, info={"seed": seed, "noise_sigma": 1e-3})
 
 
-def config4(ns=(32, 32), seed=SEED0 + 4, min_level=5, max_level=9, contrast=1e3):
+def varpoisson2d_bicgstab_smoother(ns=(32, 32), seed=SEED0 + 4, min_level=5, max_level=9, contrast=1e3):
     """2D variable-coefficient Poisson: beta = exp(g), g a 64-mode random Fourier field
     (|k| <= 16) scaled to max/min beta ~ contrast; levels 5-9 refined where |grad g| is above
     its 90th percentile; 32x32 patches; BiLinear filling; random RHS, zero mean.
@@ -189,11 +189,11 @@ def config4(ns=(32, 32), seed=SEED0 + 4, min_level=5, max_level=9, contrast=1e3)
                    info={"seed": seed, "contrast": contrast})
 
 
-def config5_problem(k, seed=SEED0 + 5, max_level=11):
-    """Problem k of config 5 (512 independent solves): seed + 1000 k; a quadtree refined
+def small_adaptive2d_problem(k, seed=SEED0 + 5, max_level=11):
+    """Problem k of the 512 independent small problems: seed + 1000 k; a quadtree refined
     level by level around 8 random points (nearest leaves first) until it holds about a
     patch count drawn log-uniform in [16, 4096]; patch size drawn from {8, 16, 32};
-    BiQuadratic filling, DFT smoother, the Gaussian-sum right-hand side of config 2."""
+    BiQuadratic filling, DFT smoother, the Gaussian-sum right-hand side of adaptive2d_dft."""
     rng = np.random.default_rng(seed + 1000 * k)
     target = int(round(np.exp(rng.uniform(np.log(16), np.log(4096)))))
     n = int(rng.choice([8, 16, 32]))
@@ -220,9 +220,9 @@ def config5_problem(k, seed=SEED0 + 5, max_level=11):
                    R.BIQUADRATIC, 0, info={"seed": seed + 1000 * k, "target_patches": target})
 
 
-def config5(num=512, seed=SEED0 + 5):
-    """Config 5: `num` independent problems (config5_problem)."""
-    return [config5_problem(k, seed) for k in range(num)]
+def small_adaptive2d_problems(num=512, seed=SEED0 + 5):
+    """`num` independent small problems (small_adaptive2d_problem)."""
+    return [small_adaptive2d_problem(k, seed) for k in range(num)]
 
 
-CONFIGS = {1: config1, 2: config2, 3: config3, 4: config4}
+CONFIGS = {1: uniform2d_bicgstab_smoother, 2: adaptive2d_dft, 3: uniform3d_dft, 4: varpoisson2d_bicgstab_smoother}

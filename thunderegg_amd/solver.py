@@ -188,8 +188,8 @@ class GMG:
 
 def solve_many(solvers, bs, xs, tol=1e-10, max_it=1000, precond=2, threads=8):
     """Independent solves run concurrently: each GMG owns a stream, and the C calls release
-    the interpreter lock, so up to `threads` solves keep the GPU busy together (config 5's
-    many small problems).  Returns the iteration counts."""
+    the interpreter lock, so up to `threads` solves keep the GPU busy together (many small
+    independent problems).  Returns the iteration counts."""
     from concurrent.futures import ThreadPoolExecutor
 
     with ThreadPoolExecutor(max_workers=threads) as ex:
