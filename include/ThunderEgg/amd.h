@@ -512,9 +512,13 @@ class StarPatchOperator : public PatchOperator<D> {
                                    x.getNumComponents(), nullptr));
   }
   const Vector<D>& getCoefficients() const { return coeffs_; }
+  /* shared by this operator and its clones (each clone deep-copies the coefficients, so
+   * their buffers differ while their values are the same) */
+  const void* coefficientIdentity() const { return ident_.get(); }
 
  private:
   Vector<D> coeffs_;
+  std::shared_ptr<const char> ident_{ std::make_shared<const char>(0) };
 };
 }  // namespace VarPoisson
 
@@ -588,11 +592,11 @@ class DFTPatchSolver : public PatchSolver<D> {
       : PatchSolver<D>(op.getDomain(), op.getGhostFiller()), op_(op.clone()), neumann_(neumann)
   {
     int op_kind = 0;
-    const double* coeffs = nullptr;
-    if (auto v = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(&op)) {
+    const double* coeffs = nullptr;  // copied by te_dft_plan_create
+    if (auto v = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(op_.get())) {
       op_kind = 1;
       coeffs = v->getCoefficients().data();
-    } else if (!dynamic_cast<const Poisson::StarPatchOperator<D>*>(&op)) {
+    } else if (!dynamic_cast<const Poisson::StarPatchOperator<D>*>(op_.get())) {
       throw RuntimeError("DFTPatchSolver: the operator must be a Poisson or VarPoisson StarPatchOperator");
     }
     te_dft_plan* h = nullptr;
@@ -762,10 +766,11 @@ PatchSolver<D>::PatchSolver(const Solver<D>& solver, const PatchOperator<D>& op,
   if (!bs) throw RuntimeError("Iterative::PatchSolver: the inner solver must be Iterative::BiCGStab");
   max_it_ = bs->getMaxIterations();
   tol_ = bs->getTolerance();
-  if (auto v = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(&op)) {
+  // the coefficients come from the owned clone: the caller's operator may go away first
+  if (auto v = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(op_.get())) {
     op_kind_ = 1;
     coeffs_ = v->getCoefficients().data();
-  } else if (auto p = dynamic_cast<const Poisson::StarPatchOperator<D>*>(&op)) {
+  } else if (auto p = dynamic_cast<const Poisson::StarPatchOperator<D>*>(op_.get())) {
     neumann_ = p->neumann() ? 1 : 0;
   } else {
     throw RuntimeError("Iterative::PatchSolver: the operator must be a Poisson or VarPoisson StarPatchOperator");
@@ -885,10 +890,12 @@ void Cycle<D>::build_device_cycle()
   te_gmg_options o{};
   std::vector<te_domain*> doms;
   std::vector<const double*> coeffs;
-  int smoother_kind = -1, op_kind = -1, filler_kind = -1, fill_type = -1, neumann = 0;
+  int op_kind = -1, filler_kind = -1, fill_type = -1, neumann = 0, cont = -1;
   unsigned mask = 0;
-  int inner_max = 1000, cont = 0;
-  double inner_tol = 1e-12;
+  bool any_dft = false;
+  // one smoother per level (CycleBuilder takes one per level): kind and inner settings
+  std::vector<int> lsm, lmax;
+  std::vector<double> ltol;
   for (size_t i = 0; i < levels_.size(); i++) {
     const Level<D>& L = levels_[i];
     const PatchOperator<D>* po = dynamic_cast<const PatchOperator<D>*>(L.op.get());
@@ -904,29 +911,37 @@ void Cycle<D>::build_device_cycle()
     } else {
       return;
     }
-    int sk = -1;
-    unsigned mk = 0;
+    int sk = -1, im = 1000, ct = -1;
+    double it = 1e-12;
     if (auto s = dynamic_cast<const Poisson::DFTPatchSolver<D>*>(L.smoother.get())) {
       sk = 0;
-      mk = (unsigned)s->getNeumann().to_ulong();
+      const unsigned mk = (unsigned)s->getNeumann().to_ulong();
+      if (any_dft && mk != mask) return;  // one Neumann mask for every DFT level
+      mask = mk;
+      any_dft = true;
     } else if (auto s = dynamic_cast<const Iterative::PatchSolver<D>*>(L.smoother.get())) {
       sk = 1;
-      inner_max = s->maxIterations();
-      inner_tol = s->tolerance();
-      cont = s->continueOnBreakdown() ? 1 : 0;
+      im = s->maxIterations();
+      it = s->tolerance();
+      ct = s->continueOnBreakdown() ? 1 : 0;
     } else {
       return;
+    }
+    if (ct >= 0) {
+      if (cont >= 0 && ct != cont) return;  // one breakdown policy for every level
+      cont = ct;
     }
     if (i + 1 < levels_.size() && !dynamic_cast<const LinearRestrictor<D>*>(L.restrictor.get())) return;
     if (i > 0 && !dynamic_cast<const DirectInterpolator<D>*>(L.interpolator.get())) return;
     const DeviceGhostFiller<D>& gf = po->deviceFiller();
     if (i == 0) {
-      smoother_kind = sk, op_kind = ok, filler_kind = gf.kind(), fill_type = (int)gf.getFillType(), neumann = nm;
-      mask = mk;
-    } else if (sk != smoother_kind || ok != op_kind || gf.kind() != filler_kind || (int)gf.getFillType() != fill_type ||
-               nm != neumann || mk != mask) {
-      return;  // heterogeneous levels: generic cycle
+      op_kind = ok, filler_kind = gf.kind(), fill_type = (int)gf.getFillType(), neumann = nm;
+    } else if (ok != op_kind || gf.kind() != filler_kind || (int)gf.getFillType() != fill_type || nm != neumann) {
+      return;  // levels with different operators or fillers: generic cycle
     }
+    lsm.push_back(sk);
+    lmax.push_back(im);
+    ltol.push_back(it);
     doms.push_back(po->getDomain().handle());
     coeffs.push_back(c);
   }
@@ -935,11 +950,14 @@ void Cycle<D>::build_device_cycle()
   o.fill_type = fill_type;
   o.op_kind = op_kind;
   o.neumann = neumann;
-  o.smoother = smoother_kind;
+  o.smoother = lsm[0];
   o.neumann_mask = mask;
-  o.inner_max_it = inner_max;
-  o.inner_tol = inner_tol;
-  o.continue_on_breakdown = cont;
+  o.inner_max_it = lmax[0];
+  o.inner_tol = ltol[0];
+  o.continue_on_breakdown = cont > 0 ? 1 : 0;
+  o.level_smoother = lsm.data();
+  o.level_inner_max_it = lmax.data();
+  o.level_inner_tol = ltol.data();
   o.pre_sweeps = opts_.pre_sweeps;
   o.post_sweeps = opts_.post_sweeps;
   o.coarse_sweeps = opts_.coarse_sweeps;
@@ -1019,7 +1037,7 @@ bool same_operator(const Operator<D>& A, const Operator<D>& B)
   }
   if (auto a = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(&A)) {
     auto b = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(&B);
-    return a->getDomain() == b->getDomain() && a->getCoefficients().data() == b->getCoefficients().data() &&
+    return a->getDomain() == b->getDomain() && a->coefficientIdentity() == b->coefficientIdentity() &&
            a->deviceFiller().kind() == b->deviceFiller().kind() &&
            a->deviceFiller().getFillType() == b->deviceFiller().getFillType();
   }

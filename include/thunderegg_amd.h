@@ -145,7 +145,8 @@ int te_residual(const te_ghost_filler* g, int dtype, int op_kind, int neumann, c
  * Poisson::DFTPatchSolver (Poisson/DFTPatchSolver.h:369-450).  neumann_mask is the
  * reference's std::bitset<Side<D>::number_of> (bit s = side s).  op_kind selects the
  * modifyRHSForInternalBoundaryConditions of the wrapped operator: 0 Poisson, 1
- * VarPoisson (coeffs: ghost-filled beta, float64, ncomp 1). */
+ * VarPoisson (coeffs: ghost-filled beta, float64, ncomp 1; the plan keeps its own copy, so
+ * the caller's buffer may be freed after this call returns). */
 int te_dft_plan_create(const te_ghost_filler* g, unsigned neumann_mask, int op_kind, const double* coeffs,
                        te_dft_plan** out);
 int te_dft_plan_destroy(te_dft_plan* p);
@@ -264,13 +265,27 @@ typedef struct te_gmg_options {
   int use_graph;      /* capture the V-cycle as a hipGraph */
   int cycle_type;     /* te_cycle_type: CycleOpts::cycle_type "V" / "W" / "F" (GMG/CycleBuilder.h:182-189) */
   int mid_sweeps;     /* CycleOpts::mid_sweeps (W and F cycles) */
+  /* A smoother per level, as CycleBuilder::addFinestLevel / addIntermediateLevel /
+   * addCoarsestLevel take one each (GMG/CycleBuilder.h:97-160): NULL = `smoother`,
+   * `inner_max_it` and `inner_tol` on every level, else num_levels entries (finest first).
+   * The fused 2D launches need the DFT smoother on every level; other mixes run one
+   * launch per step. */
+  const int* level_smoother;
+  const int* level_inner_max_it;
+  const double* level_inner_tol;
 } te_gmg_options;
 
 /* coeffs: VarPoisson beta per level (device float64 pointers, ghost-filled), or NULL. */
 int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_options* opts,
                   const double* const* coeffs, te_gmg** out);
 int te_gmg_destroy(te_gmg* g);
-/* Cycle::apply(f, u) (GMG/Cycle.h:94-98): u := V-cycle(f); float64 vectors of level 0. */
+/* Cycle::apply(f, u) (GMG/Cycle.h:94-98): u := V-cycle(f); float64 vectors of level 0.
+ * Stream-ordered and asynchronous: the cycle starts after the work enqueued on `stream` and
+ * `stream`'s later work waits for it (events; the launches run on the solver's own stream
+ * as one cached graph).  When `stream` is being captured into a graph, the launches are
+ * enqueued on it directly, so the cycle becomes part of the caller's graph.  Exception: a
+ * level with the BiCGStab patch smoother and continue_on_breakdown == 0 makes the call
+ * synchronous (it reads the breakdown flag back) and not capturable. */
 int te_gmg_vcycle(te_gmg* g, const double* f, double* u, te_stream stream);
 /* Iterative::BiCGStab::solve(A, x, b, Mr) (Iterative/BiCGStab.h:111-203), A = level-0
  * operator, Mr = this V-cycle (use_precond 1) or none (0).  x holds the initial guess.
@@ -280,13 +295,18 @@ int te_gmg_vcycle(te_gmg* g, const double* f, double* u, te_stream stream);
  * tolerance in the true residual.
  * history (host, may be NULL) receives the recursive relative residual per iteration
  * (entry 0 = initial), up to history_cap entries.  *iterations = the reference's
- * return value.  Returns TE_ERR_BREAKDOWN when rho == 0. */
+ * return value.  Returns TE_ERR_BREAKDOWN when rho == 0.
+ * Synchronous, like the reference's solve: it starts after the work enqueued on `stream`
+ * (event), runs on the solver's own stream, reads the residual norm back once per
+ * iteration and returns when x is final.  Not capturable into a graph (each iteration is
+ * itself a cached graph). */
 int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int max_it, double tol,
                     int* iterations, double* history, int history_cap, te_stream stream);
 /* Iterative::CG::solve(A, x, b, Mr) (Iterative/CG.h:126-194), right-preconditioned by
  * this cycle (use_precond 1) or plain (0); x holds the initial guess and receives
  * Mr x + guess.  Like the reference, a zero b returns 0 iterations with x set to zero.
- * history as for te_gmg_bicgstab.  Returns TE_ERR_BREAKDOWN when rho == 0. */
+ * history as for te_gmg_bicgstab.  Returns TE_ERR_BREAKDOWN when rho == 0.  Synchronous, as
+ * te_gmg_bicgstab. */
 int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it, double tol, int* iterations,
               double* history, int history_cap, te_stream stream);
 /* Seconds spent in the last te_gmg_bicgstab / te_gmg_cg, from the first A x to convergence. */

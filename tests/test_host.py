@@ -123,6 +123,61 @@ def test_forest_matches_3d_fixtures(name, base):
         assert _sig3(mine[k]) == _sig3(fx[k]), k
 
 
+def _morton_key(dim, starts, lmax):
+    """z-order key of a cell's lower corner at the finest resolution 2^-lmax: the bits of
+    the integer coordinates interleaved, x lowest (p4est's quadrant order in one tree)."""
+    c = [int(round(starts[a] * (1 << lmax))) for a in range(dim)]
+    key = 0
+    for b in range(lmax):
+        for a in range(dim):
+            key |= ((c[a] >> b) & 1) << (b * dim + a)
+    return key
+
+
+@pytest.mark.parametrize("dim,base,seed", [(2, 2, 3), (2, 3, 4), (3, 1, 5), (3, 2, 6)])
+def test_forest_ids_follow_p4est_rules(dim, base, seed):
+    """Ids and patch order of every level follow the reference generator's rules: the
+    finest level's ids are the leaves' positions in Morton order (SetIds,
+    P4estDomainGenerator.cpp:180-198: an MPI_Scan prefix plus the position in p4est's
+    z-order iteration, one rank); a coarser level merges each complete family into one
+    patch that takes the id of its first (SW / orthant-0) child and lists the family's ids in
+    orthant order, while a patch that is not merged keeps its id and is its own parent
+    (coarsenTree, :300-319).  The key is computed here from the patch geometry alone."""
+    rng = np.random.default_rng(seed)
+    f = solver.Forest(dim, base)
+    for _ in range(2):
+        lv, _ = f.leaves()
+        f.refine((rng.random(len(lv)) < 0.3).astype(np.uint8))
+    f.balance()
+    ns = (4,) * dim
+    levels = f.build_domains(ns)
+    h0 = np.array([r["spacings"][0] * ns[0] for r in levels[0]])
+    lmax = int(round(-np.log2(h0.min())))
+    keys0 = [_morton_key(dim, r["starts"], lmax) for r in levels[0]]
+    assert list(levels[0]["id"]) == list(range(len(levels[0])))
+    assert all(a < b for a, b in zip(keys0, keys0[1:]))
+    for k in range(1, len(levels)):
+        fine, coarse = levels[k - 1], levels[k]
+        fid = {int(r["id"]): r for r in fine}
+        keys = [_morton_key(dim, r["starts"], lmax) for r in coarse]
+        assert all(a < b for a, b in zip(keys, keys[1:])), k
+        for r in coarse:
+            kids = [int(c) for c in r["child_ids"][: 1 << dim]]
+            if kids[1] == -1:  # not merged: the same patch one level down
+                q = fid[int(r["id"])]
+                assert kids[0] == int(r["id"]) and int(q["parent_id"]) == int(r["id"])
+                assert int(q["orth_on_parent"]) == -1
+                assert np.allclose(q["starts"][:dim], r["starts"][:dim])
+                continue
+            assert int(r["id"]) == kids[0]
+            for o, c in enumerate(kids):
+                q = fid[c]
+                assert int(q["parent_id"]) == int(r["id"]) and int(q["orth_on_parent"]) == o
+                hq = q["spacings"][0] * ns[0]
+                off = [(o >> a) & 1 for a in range(dim)]
+                assert np.allclose(q["starts"][:dim], np.asarray(r["starts"][:dim]) + hq * np.array(off))
+
+
 def test_forest_balance_is_2to1():
     """After balance(), face and corner neighbours differ by at most one level."""
     f = solver.Forest(2, 2)

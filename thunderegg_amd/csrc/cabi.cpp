@@ -536,9 +536,11 @@ int te_dft_plan_create(const te_ghost_filler* g, unsigned neumann_mask, int op_k
   if (e == hipSuccess) e = hipMemcpy(pl->d_s2_32, s2_32.data(), sizeof(float) * s2.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(pl->d_ptype, pt.data(), sizeof(int) * pt.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess && op_kind == 1) {
-    pl->d_coeffs64 = coeffs;
+    // the plan keeps its own copies: the caller's coefficient vector may be freed first
     long n = te_domain_vector_length(d, 1);
-    e = hipMalloc(&pl->d_coeffs32, sizeof(float) * std::max(n, 1L));
+    e = hipMalloc(&pl->d_coeffs64, sizeof(double) * std::max(n, 1L));
+    if (e == hipSuccess) e = hipMalloc(&pl->d_coeffs32, sizeof(float) * std::max(n, 1L));
+    if (e == hipSuccess && n) e = hipMemcpy(pl->d_coeffs64, coeffs, sizeof(double) * n, hipMemcpyDeviceToDevice);
     if (e == hipSuccess && launch_convert(n, TE_F32, pl->d_coeffs32, TE_F64, coeffs, 0) != TE_OK) e = hipErrorUnknown;
     if (e == hipSuccess) e = hipDeviceSynchronize();
   }
@@ -559,6 +561,7 @@ int te_dft_plan_destroy(te_dft_plan* p)
   (void)hipFree(p->d_s2_64);
   (void)hipFree(p->d_s2_32);
   (void)hipFree(p->d_ptype);
+  if (p->d_coeffs64) (void)hipFree(p->d_coeffs64);
   if (p->d_coeffs32) (void)hipFree(p->d_coeffs32);
   delete p;
   return TE_OK;

@@ -11,6 +11,7 @@
 //    writes the FP64 result directly.
 //  - Iterative::BiCGStab::solve (Iterative/BiCGStab.h:111-203) in FP64 with the
 //    scalars on the device; the host reads ||s|| and ||r|| once each per iteration.
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -46,9 +47,45 @@ struct GLevel {
   const double* coeffs64 = nullptr;  // FP64 coefficients (ghost-filled)
   float* coeffs32 = nullptr;         // owned when dt == F32 and VarPoisson
   double* coeffs64_own = nullptr;    // ghost-filled copy of the caller's coefficients
+  int smoother = 0;                  // 0 DFTPatchSolver, 1 Iterative::PatchSolver(BiCGStab)
+  int inner_max_it = 1000;           // inner BiCGStab settings (smoother 1)
+  double inner_tol = 1e-12;
 };
 
 size_t esize(int dt) { return dt == TE_F64 ? 8 : 4; }
+
+// Instantiated graphs keyed by the buffers they were captured with, most recently used
+// first; at most kCap are kept (callers that pass fresh buffers every call would otherwise
+// accumulate one per call).
+struct GraphCache {
+  static constexpr size_t kCap = 4;
+  using Key = std::array<uintptr_t, 3>;
+  std::vector<std::pair<Key, hipGraphExec_t>> e;
+  hipGraphExec_t find(const Key& k)
+  {
+    for (size_t i = 0; i < e.size(); i++)
+      if (e[i].first == k) {
+        auto hit = e[i];
+        e.erase(e.begin() + (long)i);
+        e.insert(e.begin(), hit);
+        return hit.second;
+      }
+    return nullptr;
+  }
+  void put(const Key& k, hipGraphExec_t x)
+  {
+    if (e.size() >= kCap) {
+      (void)hipGraphExecDestroy(e.back().second);
+      e.pop_back();
+    }
+    e.insert(e.begin(), { k, x });
+  }
+  void clear()
+  {
+    for (auto& kv : e) (void)hipGraphExecDestroy(kv.second);
+    e.clear();
+  }
+};
 
 }  // namespace
 
@@ -66,9 +103,9 @@ struct te_gmg {
   double* mirror_d = nullptr;
   int nblk = 0;
   int32_t* d_flags = nullptr;
-  std::map<std::pair<const void*, const void*>, hipGraphExec_t> vgraphs;
-  // one BiCGStab iteration per (b, x, use_precond)
-  std::map<std::tuple<const void*, const void*, int>, hipGraphExec_t> bgraphs;
+  // the V-cycle per (f, u); one BiCGStab iteration per (x, use_precond) (b is not read
+  // inside the iteration)
+  GraphCache vgraphs, bgraphs;
   double last_seconds = 0.0;
   bool fused = false;  // 2D matrix-core path: ghost fills fused into the stencil / smoother launches
   std::vector<char> rr_pre;  // per level: restriction fused with the coarser level's first sweep
@@ -93,6 +130,8 @@ struct te_gmg {
   // fused mixed-precision path: the BiCGStab s and p updates run the finest level's
   // zero-guess sweep too (launch_bicg_sweep); TE_NO_BSWEEP=1 keeps them apart
   bool bsweep = false;
+  // stream ordering with the caller's stream (no host blocking)
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 namespace {
@@ -145,16 +184,16 @@ int smooth(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void*
 int smooth_sweep(te_gmg* g, int li, const void* f, int dtf, void* u, bool uzero, void* out, int dto)
 {
   GLevel& Lv = g->L[li];
-  if (g->o.smoother == 0) {
+  if (Lv.smoother == 0) {
     if (!uzero) TE_RC(launch_ghost_fill(Lv.gf, Lv.dt, u, 1, g->s));
     return launch_dft(Lv.plan, Lv.dt, dtf, dto, f, u, out, uzero, g->s);
   }
   // Iterative::PatchSolver(BiCGStab): f and u share the level dtype here
   if (uzero) TE_RC(zero_with_ghost(g, li, u, Lv.dt));
   else TE_RC(launch_ghost_fill(Lv.gf, Lv.dt, u, 1, g->s));
-  double tol = g->o.inner_tol;
+  double tol = Lv.inner_tol;
   if (Lv.dt == TE_F32 && tol < 1e-6) tol = 1e-6;  // an FP32 inner solve cannot reach tighter
-  TE_RC(launch_patch_bicgstab(Lv.d, Lv.dt, g->o.op_kind, g->o.neumann, Lv.coeffs, f, u, g->o.inner_max_it, tol,
+  TE_RC(launch_patch_bicgstab(Lv.d, Lv.dt, g->o.op_kind, g->o.neumann, Lv.coeffs, f, u, Lv.inner_max_it, tol,
                               nullptr, g->d_flags, g->s));
   if (out != u) TE_RC(launch_convert(te_domain_vector_length(Lv.d, 1), dto, out, Lv.dt, u, g->s));
   return TE_OK;
@@ -315,7 +354,7 @@ int vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt = TE_F64, int dtf 
     return TE_ERR_UNSUPPORTED;
   }
   double* ud = (double*)u;
-  if (L0.dt != TE_F64 && g->o.smoother != 0) {
+  if (L0.dt != TE_F64 && L0.smoother != 0) {
     // the inner-Krylov smoother works on same-dtype f and u: convert f once
     TE_RC(launch_convert(te_domain_vector_length(L0.d, 1), L0.dt, L0.f, TE_F64, f, g->s));
     return visit(g, 0, L0.f, L0.dt, ud);
@@ -326,9 +365,9 @@ int vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt = TE_F64, int dtf 
 int vcycle_launch(te_gmg* g, const double* f, double* u)
 {
   if (!g->o.use_graph || g->timer) return vcycle_enqueue(g, f, u);
-  auto key = std::make_pair((const void*)f, (const void*)u);
-  auto it = g->vgraphs.find(key);
-  if (it == g->vgraphs.end()) {
+  const GraphCache::Key key{ (uintptr_t)f, (uintptr_t)u, 0 };
+  hipGraphExec_t it = g->vgraphs.find(key);
+  if (!it) {
     hipGraph_t graph;
     TE_HIP(hipStreamBeginCapture(g->s, hipStreamCaptureModeThreadLocal));
     int rc = vcycle_enqueue(g, f, u);
@@ -345,10 +384,40 @@ int vcycle_launch(te_gmg* g, const double* f, double* u)
       set_error(std::string("V-cycle graph instantiate: ") + hipGetErrorString(e));
       return TE_ERR_HIP;
     }
-    it = g->vgraphs.emplace(key, exec).first;
+    g->vgraphs.put(key, exec);
+    it = exec;
   }
-  TE_HIP(hipGraphLaunch(it->second, g->s));
+  TE_HIP(hipGraphLaunch(it, g->s));
   return TE_OK;
+}
+
+// The solver's stream waits for the work enqueued so far on `caller` (an event recorded
+// there), without blocking the host.
+int join_from(te_gmg* g, hipStream_t caller)
+{
+  if (caller == g->s) return TE_OK;
+  TE_HIP(hipEventRecord(g->ev_in, caller));
+  TE_HIP(hipStreamWaitEvent(g->s, g->ev_in, 0));
+  return TE_OK;
+}
+
+// `caller` waits for the work enqueued so far on the solver's stream.
+int join_to(te_gmg* g, hipStream_t caller)
+{
+  if (caller == g->s) return TE_OK;
+  TE_HIP(hipEventRecord(g->ev_out, g->s));
+  TE_HIP(hipStreamWaitEvent(caller, g->ev_out, 0));
+  return TE_OK;
+}
+
+// a level smooths with the inner BiCGStab and a breakdown there must be reported: the
+// breakdown flag has to be read back after the cycle
+bool needs_breakdown_check(const te_gmg* g)
+{
+  if (g->o.continue_on_breakdown) return false;
+  for (const GLevel& Lv : g->L)
+    if (Lv.smoother == 1) return true;
+  return false;
 }
 
 int check_flags(te_gmg* g)
@@ -367,8 +436,8 @@ int check_flags(te_gmg* g)
 void free_gmg(te_gmg* g)
 {
   if (!g) return;
-  for (auto& kv : g->vgraphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto& kv : g->bgraphs) (void)hipGraphExecDestroy(kv.second);
+  g->vgraphs.clear();
+  g->bgraphs.clear();
   for (auto& Lv : g->L) {
     if (Lv.gf) te_ghost_filler_destroy(Lv.gf);
     if (Lv.plan) te_dft_plan_destroy(Lv.plan);
@@ -389,6 +458,8 @@ void free_gmg(te_gmg* g)
   (void)hipFree(g->d_flags);
   if (g->host_scal) (void)hipHostFree(g->host_scal);
   if (g->mirror_h) (void)hipHostFree(g->mirror_h);
+  if (g->ev_in) (void)hipEventDestroy(g->ev_in);
+  if (g->ev_out) (void)hipEventDestroy(g->ev_out);
   if (g->s) (void)hipStreamDestroy(g->s);
   delete g;
 }
@@ -412,9 +483,17 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     set_error("te_gmg_create: VarPoisson needs per-level coefficients");
     return TE_ERR_ARG;
   }
-  if (opts->smoother != 0 && opts->smoother != 1) {
-    set_error("te_gmg_create: smoother must be 0 (DFT) or 1 (BiCGStab patch solver)");
-    return TE_ERR_ARG;
+  for (int i = 0; i < num_levels; i++) {
+    const int sk = opts->level_smoother ? opts->level_smoother[i] : opts->smoother;
+    if (sk != 0 && sk != 1) {
+      set_error("te_gmg_create: smoother must be 0 (DFT) or 1 (BiCGStab patch solver)");
+      return TE_ERR_ARG;
+    }
+    const int mi = opts->level_inner_max_it ? opts->level_inner_max_it[i] : opts->inner_max_it;
+    if (sk == 1 && mi < 0) {
+      set_error("te_gmg_create: bad inner iteration limit");
+      return TE_ERR_ARG;
+    }
   }
   if (opts->cycle_type < TE_CYCLE_V || opts->cycle_type > TE_CYCLE_F) {
     set_error("te_gmg_create: Unsupported Cycle type " + std::to_string(opts->cycle_type) +
@@ -427,10 +506,15 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
   }
   auto g = new te_gmg();
   g->o = *opts;
+  g->o.level_smoother = nullptr;  // the caller's arrays are not kept (copied into the levels)
+  g->o.level_inner_max_it = nullptr;
+  g->o.level_inner_tol = nullptr;
   hipError_t e = hipStreamCreate(&g->s);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_in, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_out, hipEventDisableTiming);
   if (e != hipSuccess) {
-    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    delete g;
+    set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
+    free_gmg(g);
     return TE_ERR_HIP;
   }
   g->L.resize(num_levels);
@@ -440,6 +524,9 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     Lv.d = levels[i];
     bool coarsest = i == num_levels - 1;
     Lv.dt = (opts->precision == TE_F32 && !coarsest) ? TE_F32 : TE_F64;
+    Lv.smoother = opts->level_smoother ? opts->level_smoother[i] : opts->smoother;
+    Lv.inner_max_it = opts->level_inner_max_it ? opts->level_inner_max_it[i] : opts->inner_max_it;
+    Lv.inner_tol = opts->level_inner_tol ? opts->level_inner_tol[i] : opts->inner_tol;
     rc = te_ghost_filler_create(Lv.d, opts->filler_kind, opts->fill_type, &Lv.gf);
     if (rc) break;
     long n = te_domain_vector_length(Lv.d, 1);
@@ -486,7 +573,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
         rc = TE_ERR_HIP;
       }
     }
-    if (opts->smoother == 0 && !rc)
+    if (Lv.smoother == 0 && !rc)
       rc = te_dft_plan_create(Lv.gf, opts->neumann_mask, opts->op_kind, opts->op_kind == 1 ? Lv.coeffs64 : nullptr,
                               &Lv.plan);
     if (i > 0 && !rc) rc = te_level_link_create(g->L[i - 1].d, Lv.d, &g->L[i - 1].link);
@@ -497,10 +584,10 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     const char* nf = std::getenv("TE_NO_FUSE");
     const bool allow = !(nf && nf[0] == '1');
     // the fused launches implement the V-cycle; W- and F-cycles run one launch per step
-    bool fused = allow && opts->smoother == 0 && opts->cycle_type == TE_CYCLE_V;
+    bool fused = allow && opts->cycle_type == TE_CYCLE_V;
     for (int i = 0; i < num_levels && fused; i++) {
       const GLevel& Lv = g->L[i];
-      fused = Lv.plan && Lv.plan->use_mfma && dft_mfma_supported(Lv.plan) &&
+      fused = Lv.smoother == 0 && Lv.plan && Lv.plan->use_mfma && dft_mfma_supported(Lv.plan) &&
               (i == num_levels - 1 || resid_restrict_fused_supported(Lv.gf));
     }
     g->fused = fused;
@@ -619,8 +706,26 @@ int te_gmg_vcycle(te_gmg* g, const double* f, double* u, te_stream stream)
     set_error("te_gmg_vcycle: bad argument");
     return TE_ERR_ARG;
   }
-  TE_HIP(hipStreamSynchronize((hipStream_t)stream));
+  hipStream_t cs = (hipStream_t)stream;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (cs) TE_HIP(hipStreamIsCapturing(cs, &st));
+  if (st == hipStreamCaptureStatusActive) {
+    // the caller captures its stream into a graph: the cycle's launches go straight onto it
+    if (needs_breakdown_check(g)) {
+      set_error("te_gmg_vcycle: a cycle whose BiCGStab patch smoother stops on breakdown reads its breakdown "
+                "flag back and cannot be captured (use continue_on_breakdown)");
+      return TE_ERR_UNSUPPORTED;
+    }
+    hipStream_t own = g->s;
+    g->s = cs;
+    int rc = vcycle_enqueue(g, f, u);
+    g->s = own;
+    return rc;
+  }
+  TE_RC(join_from(g, cs));
   TE_RC(vcycle_launch(g, f, u));
+  TE_RC(join_to(g, cs));
+  if (!needs_breakdown_check(g)) return TE_OK;
   TE_HIP(hipStreamSynchronize(g->s));
   return check_flags(g);
 }
@@ -633,7 +738,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     set_error("te_gmg_bicgstab: bad argument");
     return TE_ERR_ARG;
   }
-  TE_HIP(hipStreamSynchronize((hipStream_t)stream));
+  TE_RC(join_from(g, (hipStream_t)stream));
   GLevel& L0 = g->L[0];
   const te_domain* d = L0.d;
   te_ghost_filler* gf = L0.gf;
@@ -694,9 +799,9 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   };
   hipGraphExec_t gx = nullptr;
   if (g->o.use_graph && !g->timer) {
-    auto key = std::make_tuple((const void*)b, (const void*)x, use_precond);
-    auto it = g->bgraphs.find(key);
-    if (it == g->bgraphs.end()) {
+    const GraphCache::Key key{ (uintptr_t)x, (uintptr_t)use_precond, 1 };
+    gx = g->bgraphs.find(key);
+    if (!gx) {
       hipGraph_t graph;
       TE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       int rc = iteration();
@@ -713,9 +818,9 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
         set_error(std::string("BiCGStab graph instantiate: ") + hipGetErrorString(e));
         return TE_ERR_HIP;
       }
-      it = g->bgraphs.emplace(key, ex).first;
+      g->bgraphs.put(key, ex);
+      gx = ex;
     }
-    gx = it->second;
   }
   const bool time_its = g->timer && g->time_its;
   auto t0 = std::chrono::steady_clock::now();
@@ -798,7 +903,7 @@ int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it
     set_error("te_gmg_cg: bad argument");
     return TE_ERR_ARG;
   }
-  TE_HIP(hipStreamSynchronize((hipStream_t)stream));
+  TE_RC(join_from(g, (hipStream_t)stream));
   GLevel& L0 = g->L[0];
   const te_domain* d = L0.d;
   te_ghost_filler* gf = L0.gf;

@@ -17,8 +17,8 @@ struct te_dft_plan {
   float* d_mats32 = nullptr;
   double* d_s2_64 = nullptr;    // [3 eig types][3 axes][kDftMatDim]
   float* d_s2_32 = nullptr;
-  const double* d_coeffs64 = nullptr;  // VarPoisson beta (caller-owned)
-  float* d_coeffs32 = nullptr;         // plan-owned float copy
+  double* d_coeffs64 = nullptr;  // VarPoisson beta: plan-owned copy of the caller's values
+  float* d_coeffs32 = nullptr;   // plan-owned float copy
   int use_mfma = 1;                    // matrix-core kernel where supported (2D, n 16/32, Dirichlet)
 };
 

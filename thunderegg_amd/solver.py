@@ -90,6 +90,9 @@ class _Opts(C.Structure):
         ("use_graph", C.c_int),
         ("cycle_type", C.c_int),
         ("mid_sweeps", C.c_int),
+        ("level_smoother", C.POINTER(C.c_int)),
+        ("level_inner_max_it", C.POINTER(C.c_int)),
+        ("level_inner_tol", C.POINTER(C.c_double)),
     ]
 
 
@@ -100,7 +103,8 @@ class GMG:
     """Multigrid hierarchy + outer BiCGStab / CG on the device (te_gmg).
 
     levels: patch tables finest first.  smoother 0 = DFTPatchSolver,
-    1 = Iterative::PatchSolver(BiCGStab).  precision F64 or F32 (FP32 cycle,
+    1 = Iterative::PatchSolver(BiCGStab), or a list with one kind per level (finest first;
+    inner_max_it / inner_tol may be lists too).  precision F64 or F32 (FP32 cycle,
     FP64 coarsest solve, FP64 outer iteration).  cycle_type "V", "W" or "F"
     (GMG::VCycle, WCycle, FMGCycle; mid_sweeps for W and F)."""
 
@@ -115,9 +119,24 @@ class GMG:
         if coeffs is not None:
             self.coeffs = [R.DeviceArray.from_host(np.ascontiguousarray(c, dtype=np.float64)) for c in coeffs]
             cptrs = (C.c_void_p * len(levels))(*[c.ptr for c in self.coeffs])
+        nl = len(levels)
+
+        def per_level(v, ctype):
+            if np.isscalar(v):
+                return v, None
+            if len(v) != nl:
+                raise ValueError("per-level settings need one entry per level")
+            return v[0], (ctype * nl)(*v)
+
+        smoother, self._lsm = per_level(smoother, C.c_int)
+        inner_max_it, self._lmi = per_level(inner_max_it, C.c_int)
+        inner_tol, self._ltol = per_level(inner_tol, C.c_double)
         o = _Opts(filler_kind, fill_type, op_kind, int(neumann), smoother, neumann_mask, inner_max_it, inner_tol,
                   int(continue_on_breakdown), pre_sweeps, post_sweeps, coarse_sweeps, precision, int(use_graph),
-                  CYCLE_TYPES[cycle_type], mid_sweeps)
+                  CYCLE_TYPES[cycle_type], mid_sweeps,
+                  C.cast(self._lsm, C.POINTER(C.c_int)) if self._lsm is not None else None,
+                  C.cast(self._lmi, C.POINTER(C.c_int)) if self._lmi is not None else None,
+                  C.cast(self._ltol, C.POINTER(C.c_double)) if self._ltol is not None else None)
         hs = (C.c_void_p * len(self.domains))(*[d.h for d in self.domains])
         h = C.c_void_p()
         R.check(R.lib().te_gmg_create(hs, len(self.domains), C.byref(o), cptrs, C.byref(h)))
