@@ -856,6 +856,8 @@ static int op_apply(const opctx* o, double* u, double* f, int nc)
   memset(f, 0, sizeof(double) * len);
   int rc = ora_fill_ghost(d, o->kind, o->ft, u, nc);
   if (rc) return rc;
+  /* patches are independent here (each writes its own f and its own physical ghosts) */
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < d->np; p++) apply_patch(o, p, p, u, f, nc, 0);
   return 0;
 }
@@ -1103,9 +1105,6 @@ static int dft_smooth(const opctx* o, unsigned mask, const double* f, double* u)
   if (rc) return rc;
   int n[3] = { d->ns[0], d->ns[1], D == 3 ? d->ns[2] : 1 };
   long tot = (long)n[0] * n[1] * n[2];
-  double* blk = (double*)malloc(sizeof(double) * tot);
-  double* tmp = (double*)malloc(sizeof(double) * tot);
-  double* fcopy = (double*)malloc(sizeof(double) * d->cells_ghost);
   double* mats[6][3];
   for (int t = 0; t < 6; t++)
     for (int a = 0; a < D; a++) {
@@ -1115,6 +1114,14 @@ static int dft_smooth(const opctx* o, unsigned mask, const double* f, double* u)
   int all_neumann = 1;
   for (int s = 0; s < 2 * D; s++)
     if (!((mask >> s) & 1)) all_neumann = 0;
+  /* the patch solves are independent (u's ghosts were filled above and only interior
+   * cells are written), so they run in parallel with per-thread work buffers */
+#pragma omp parallel
+  {
+  double* blk = (double*)malloc(sizeof(double) * tot);
+  double* tmp = (double*)malloc(sizeof(double) * tot);
+  double* fcopy = (double*)malloc(sizeof(double) * d->cells_ghost);
+#pragma omp for schedule(static)
   for (int p = 0; p < d->np; p++) {
     /* f_copy = f ; modifyRHS (:424-430) */
     memset(fcopy, 0, sizeof(double) * d->cells_ghost);
@@ -1170,11 +1177,12 @@ static int dft_smooth(const opctx* o, unsigned mask, const double* f, double* u)
           u[cell_index(d, p, 0, 1, xc)] = blk[k++] * eig_scale; /* :442-449 */
         }
   }
-  for (int t = 0; t < 6; t++)
-    for (int a = 0; a < D; a++) free(mats[t][a]);
   free(blk);
   free(tmp);
   free(fcopy);
+  }
+  for (int t = 0; t < 6; t++)
+    for (int a = 0; a < D; a++) free(mats[t][a]);
   return 0;
 }
 
@@ -1194,25 +1202,29 @@ static int bicgstab_smooth(const opctx* o, int max_it, double tol, int cont, con
   int rc = ora_fill_ghost(d, o->kind, o->ft, u, nc);
   if (rc) return rc;
   long plen = (long)nc * d->cells_ghost;
-  double* fcopy = (double*)malloc(sizeof(double) * plen);
   vspace V1 = { d, 1, nc };
-  int total = 0;
-  for (int p = 0; p < d->np; p++) {
-    memset(fcopy, 0, sizeof(double) * plen);
-    FOR_INTERIOR(&V1, i) fcopy[i] = f[p * plen + i];
-    modify_rhs(o, p, u, fcopy, nc, p, 0);
-    spctx sc = { o, p, nc };
-    int bd = 0;
-    int its = bicgstab(&V1, single_patch_apply, &sc, u + p * plen, fcopy, NULL, NULL, max_it, tol, NULL, 0, &bd, 0);
-    if (bd && !cont) {
-      free(fcopy);
-      return -10;
+  int failed = 0;
+  /* independent patch solves (each reads the filled ghosts and writes its own interior) */
+#pragma omp parallel
+  {
+    double* fcopy = (double*)malloc(sizeof(double) * plen);
+#pragma omp for schedule(dynamic, 4)
+    for (int p = 0; p < d->np; p++) {
+      memset(fcopy, 0, sizeof(double) * plen);
+      FOR_INTERIOR(&V1, i) fcopy[i] = f[p * plen + i];
+      modify_rhs(o, p, u, fcopy, nc, p, 0);
+      spctx sc = { o, p, nc };
+      int bd = 0;
+      int its = bicgstab(&V1, single_patch_apply, &sc, u + p * plen, fcopy, NULL, NULL, max_it, tol, NULL, 0, &bd, 0);
+      if (bd && !cont) {
+#pragma omp atomic write
+        failed = 1;
+      }
+      if (iters_out) iters_out[p] = its;
     }
-    if (iters_out) iters_out[p] = its;
-    total += its;
+    free(fcopy);
   }
-  free(fcopy);
-  return 0;
+  return failed ? -10 : 0;
 }
 
 int ora_bicgstab_smooth(const ora_domain* d, int kind, int fill_type, int op_kind, const double* coeffs,
@@ -1229,9 +1241,13 @@ int ora_restrict(const ora_domain* fine, const ora_domain* coarse, const double*
   int D = fine->dim;
   memset(cv, 0, sizeof(double) * ora_domain_vec_len(coarse, nc)); /* coarse.setWithGhost(0) */
   int n[3] = { fine->ns[0], fine->ns[1], D == 3 ? fine->ns[2] : 1 };
+  for (int p = 0; p < fine->np; p++)
+    if (fine->parent_local[p] < 0) return -1;
+  /* each coarse cell receives the values of exactly one fine patch, so the fine patches
+   * are independent */
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < fine->np; p++) {
     int cp = fine->parent_local[p];
-    if (cp < 0) return -1;
     int orth = fine->p[p].orth_on_parent;
     int st[3] = { 0, 0, 0 };
     if (orth >= 0)
@@ -1257,9 +1273,11 @@ int ora_interpolate(const ora_domain* coarse, const ora_domain* fine, const doub
 {
   int D = fine->dim;
   int n[3] = { fine->ns[0], fine->ns[1], D == 3 ? fine->ns[2] : 1 };
+  for (int p = 0; p < fine->np; p++)
+    if (fine->parent_local[p] < 0) return -1;
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < fine->np; p++) {
     int cp = fine->parent_local[p];
-    if (cp < 0) return -1;
     int orth = fine->p[p].orth_on_parent;
     int st[3] = { 0, 0, 0 };
     if (orth >= 0)
