@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark: DOF/s to a 1e-10 relative residual on one MI355X.
 
-Workload (default, --config 2 of the numbered workloads below): 2D adaptive Poisson, quadtree levels 4-9 refined
-within 0.05 of 8 seeded random points, 2:1 balanced, 19,930 patches of 16x16
-cells (5.1M DOF), BiQuadratic ghost filling, 5-point star operator, DFT patch
-smoother (1 pre / 1 post / 1 coarse sweep) in a 10-level V-cycle preconditioning
-an FP64 BiCGStab to 1e-10.
+Workload (default): 3D Poisson on a uniform octree at level 5, 32,768 patches of 16^3
+cells (134,217,728 DOF), TriLinear ghost filling, 7-point star operator, DFT patch
+smoother with 2 pre / 2 post sweeps and 1 coarsest sweep in a 5-level V-cycle (32,768 ->
+8 patches) preconditioning an FP64 BiCGStab to 1e-10; right-hand side
+-12 pi^2 prod sin(2 pi x_i) plus N(0, 1e-3) noise.  The largest single-GPU solve of the
+benchmark workloads.  `--config N` selects another one (1 uniform 2D with the BiCGStab
+patch smoother, 2 adaptive 2D, 4 VarPoisson 2D, 5 independent small problems).
 
-A "step" is one complete solve from x = 0 to ||r||/||b|| <= 1e-10 with the
-V-cycle in FP32 (coarsest level FP64) under FP64 BiCGStab refinement; `value`
-is finest-level DOF x steps / wall time of the timed steps, inputs resident in
-HBM.  The plain FP64 GPU solve is timed the same way and reported beside it.
-The BiCGStab early exit uses x += alpha M p (the reference adds alpha p, which
-leaves the true residual above the tolerance whenever that exit fires).
+A "step" is one complete solve from x = 0 to ||r||/||b|| <= 1e-10 with the V-cycle in
+FP32 (coarsest level FP64) under FP64 BiCGStab refinement; `value` is finest-level DOF x
+steps / wall time of the timed steps, inputs resident in HBM.  The plain FP64 GPU solve is
+timed the same way and reported beside it (`fp64_dof_per_s`, `mixed_over_fp64`), with the
+true FP64 residuals of both.  The BiCGStab early exit uses x += alpha M p (the reference
+adds alpha p, which leaves the true residual above the tolerance whenever that exit fires).
+The adaptive 2D workload (19,930 patches of 16^2) is measured the same way and reported
+under `secondary`.
 
 Prints one JSON line.  --gpus N > 1 exits: this build measures one GPU.
 """
@@ -21,6 +25,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import statistics
 import sys
 import time
@@ -29,6 +34,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFS = {"f32": 157.3, "f64": 78.6}
+METRIC = "DOF/s to 1e-10 rel residual, FP32-precond+FP64 refine vs FP64 GPU; HBM GB/s"
 
 
 def log(*a):
@@ -52,13 +59,11 @@ def time_launches(R, fn, n, stream):
     return R.Event.elapsed_ms(e0, e1) / n
 
 
-def kernel_rooflines(R, P, stream, reps=50):
-    """Average launch time (HIP events on the launch stream) of the finest-level kernels
-    of the mixed-precision V-cycle, launched standalone through the C ABI with the same
-    dtype combination as inside the solve (the BiCGStab hands the V-cycle FP32 copies of p
-    and s and keeps M p, M s in FP32), and their algorithmic bytes per launch (every input
-    read once, every output written once; ghost values computed from neighbours are cache
-    traffic, not algorithmic)."""
+def kernel_rooflines_2d(R, P, stream, reps=50):
+    """Average launch time (HIP events on the launch stream) of the finest-level kernels of
+    the 2D mixed-precision V-cycle, launched standalone through the C ABI with the dtype
+    combination of the solve, and their algorithmic bytes per launch (every input read
+    once, every output written once)."""
     import numpy as np
 
     dim, ns = P.dim, P.ns
@@ -84,31 +89,83 @@ def kernel_rooflines(R, P, stream, reps=50):
         if flops:
             out[name].update({"flops": flops, "TFs": flops / t / 1e9})
 
-    if dim == 2 and P.smoother == 0:
+    if P.smoother == 0:
         plan = R.DftPlan(g, op_kind=P.op_kind)
         fl = n0 * (8 * ns[0] ** 3 + N)  # four n^3 products of 2 n^3 flops, n^2 divides
-        # pre-smoothing from u = 0: f read, u written (FP32)
         rec("dft_apply_f32", lambda: plan.apply_mixed(R.F32, f32, o32, stream=stream.h), n0 * N * (4 + 4), fl)
-        # Cycle::restrict: u, f read; coarse f written (FP32)
         rec("residual_restrict_f32", lambda: link.residual_restrict(g, f32, u32, fc32, stream=stream.h),
             n0 * N * (4 + 4) + n1 * N * 4)
-        # post-smoothing with the interpolation: f, u, coarse u read; M f written (FP32)
         rec("dft_smooth_interp_f32",
             lambda: plan.smooth_interp(link, f32, u32, uc32, m32, stream=stream.h), n0 * N * (4 + 4 + 4) + n1 * N * 4,
             fl)
-    # copy of a vector of known size (calibrates the HBM byte counters)
     c64 = R.DeviceArray(d0.shape(1), np.float64)
     rec("vec_copy_f64", lambda: R.check(R.lib().te_vec_copy(d0.h, R.F64, c64.ptr, f64.ptr, 1, stream.h)),
         2 * n0 * N * 8)
     return out
 
 
-def cpu_baseline(P, gpu_iters, sample_iters=5):
-    """The oracle (FP64 C restatement of the reference algorithm, one core) on the same
-    problem: a bounded sample of `sample_iters` outer BiCGStab iterations, extrapolated to
-    the GPU FP64 solve's iteration count."""
+def kernel_rooflines_3d(g, P, its, reps=20):
+    """The finest-level kernels of the compact 3D cycle and the outer solver, each launched
+    standalone on the mixed-precision solver's own buffers and timed with HIP events on its
+    stream (te_gmg_time_kernel), with algorithmic bytes and flops per launch.  Per 16^3
+    patch: N = 4096 cells, B = 1352 cells of the boundary layer (the cells a neighbour's
+    ghost formula or the patch's own interface terms read), Bc = 296 coarse cells under the
+    boundary layer; FP32 levels (s = 4), FP64 outer vectors.  Bytes count every value the
+    result depends on once:
+      sweep_zero      f, out                        s (2N)
+      sweep           f, u's boundary layer, out    s (2N + B)
+      sweep_interp    + the coarse boundary cells   s (2N + B + Bc)
+      resid_restrict  f, u, coarse f                s (2N + N/8)
+      operator        M s (FP32), s, A M s (FP64)   4N + 8N + 8N
+      update_xr       x, r read + written; M p, M s (FP32); A M p, A M s, rhat  (8 + 8) 2 + 4 2 + 8 3 per cell
+    Flops of the patch solve: six 16 x 16 x 16 transform products per axis pass, 12 n^4."""
+    np_ = len(P.levels[0])
+    N, B, Bc = 4096, 16 ** 3 - 14 ** 3, 8 ** 3 - 6 ** 3
+    s = 4
+    fl = np_ * 12 * 16 ** 4
+    pre, post = P.pre, P.post
+    specs = {
+        # name: (bytes, flops, launches per solve)
+        "sweep_zero": (np_ * s * 2 * N, fl, 2 * its),
+        "sweep": (np_ * s * (2 * N + B), fl, 2 * its * max(pre - 1 + post - 1, 0)),
+        "sweep_interp": (np_ * s * (2 * N + B + Bc), fl, 2 * its),
+        "resid_restrict": (np_ * s * (2 * N + N // 8), None, 2 * its),
+        "operator": (np_ * N * (4 + 8 + 8), None, 2 * its),
+        "update_xr": (np_ * N * (16 * 2 + 4 * 2 + 8 * 3), None, its),
+    }
+    out = {}
+    for name, (byt, flops, per_solve) in specs.items():
+        ms = g.time_kernel(name, reps)
+        d = {"ms": ms, "bytes": byt, "GBs": byt / ms / 1e6, "launches_per_solve": per_solve,
+             "ms_per_solve": ms * per_solve}
+        if flops:
+            d.update({"flops": flops, "TFs": flops / ms / 1e9})
+        out[name] = d
+    return out
+
+
+def cpu_info():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(P, gpu_iters, budget_s=30.0):
+    """The oracle (the FP64 C restatement of the reference algorithm; its per-patch loops run
+    under OpenMP, the ghost fills and reductions serially) on the same problem, on this
+    host's cores: outer BiCGStab iterations until about `budget_s` of CPU work (at least
+    one), extrapolated to the GPU FP64 solve's iteration count."""
     from oracle import oracle as O
 
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     od = [O.Domain(P.dim, P.ns, lv) for lv in P.levels]
     kw = {}
     if P.op_kind == 1:
@@ -120,24 +177,35 @@ def cpu_baseline(P, gpu_iters, sample_iters=5):
         kw = {"op_kind": 1, "coeffs": coeffs}
     L = O.Levels(od, filler_kind=P.filler, smoother_kind=P.smoother, **kw)
     t0 = time.perf_counter()
-    _, its, _ = L.bicgstab(P.rhs.copy(), tol=1e-30, precond=2, pre=P.pre, post=P.post, coarse=P.coarse,
-                           max_it=sample_iters)
+    _, its, _ = L.bicgstab(P.rhs.copy(), tol=1e-30, precond=2, pre=P.pre, post=P.post, coarse=P.coarse, max_it=1)
     dt = time.perf_counter() - t0
-    per_it = dt / max(its, 1)
+    k = max(its, 1)
+    if dt * 2 < budget_s:
+        more = max(1, min(int(budget_s / max(dt, 1e-3)) - 1, 5))
+        t0 = time.perf_counter()
+        _, its2, _ = L.bicgstab(P.rhs.copy(), tol=1e-30, precond=2, pre=P.pre, post=P.post, coarse=P.coarse,
+                                max_it=more)
+        dt2 = time.perf_counter() - t0
+        if its2 > 0:
+            dt, k = dt2, its2
+    per_it = dt / k
     est = per_it * max(gpu_iters, 1)
+    model, ncpu = cpu_info()
     return {
         "value": P.dof / est,
         "unit": "DOF/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle FP64 BiCGStab+V-cycle on the full {P.dof}-DOF problem, {its} outer iterations "
-                   f"in {dt:.2f} s ({per_it:.3f} s/iteration), extrapolated to the {gpu_iters} iterations "
-                   f"the FP64 GPU solve needs"),
+        "cpu_model": model,
+        "host_logical_cpus": ncpu,
+        "sample": (f"oracle FP64 BiCGStab + V-cycle on the full {P.dof}-DOF problem with {threads} OpenMP threads "
+                   f"on {model}: {k} outer iteration(s) in {dt:.2f} s ({per_it:.2f} s/iteration, setup excluded), "
+                   f"extrapolated to the {gpu_iters} iterations of the FP64 GPU solve"),
     }
 
 
 def run_many_problems(args):
-    """Config 5: independent problems solved concurrently (one stream per problem); value =
+    """The independent small problems solved concurrently (one stream per problem); value =
     aggregate finest-level DOF x rounds / wall time, every problem to 1e-10, mixed precision,
     with the FP64 aggregate beside it."""
     import numpy as np
@@ -179,7 +247,7 @@ def run_many_problems(args):
     T32, its32 = run(R.F32)
     T64, its64 = run(R.F64)
     out = {
-        "metric": "DOF/s to 1e-10 rel residual, FP32-precond+FP64 refine vs FP64 GPU; HBM GB/s",
+        "metric": METRIC,
         "value": dof * args.steps / T32,
         "unit": "DOF/s",
         "n_gpus": 1,
@@ -191,7 +259,7 @@ def run_many_problems(args):
         "vs_baseline": None,
         "dtype": "f32 V-cycle / f64 BiCGStab",
         "data": "synthetic (seeded meshes and right-hand sides)",
-        "config": {"workload": f"{len(probs)} independent 2D adaptive problems , solved concurrently",
+        "config": {"workload": f"{len(probs)} independent 2D adaptive problems, solved concurrently",
                    "problems": len(probs), "dof": dof, "threads": args.threads,
                    "patch_sizes": sorted({P.ns[0] for P in probs})},
         "fp64_dof_per_s": dof * args.steps / T64,
@@ -202,17 +270,84 @@ def run_many_problems(args):
     print(json.dumps(out), flush=True)
 
 
+def measure(P, args, R, solver, with_fp64=True):
+    """Timed mixed-precision (and FP64) solves of problem P; returns the numbers and the solvers."""
+    import numpy as np
+
+    from thunderegg_amd import mesh
+
+    def make(precision):
+        return solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, op_kind=P.op_kind,
+                          coeffs=P.coeffs, pre_sweeps=P.pre, post_sweeps=P.post, coarse_sweeps=P.coarse,
+                          inner_max_it=P.inner_max_it, inner_tol=P.inner_tol, precision=precision, use_graph=True)
+
+    t0 = time.perf_counter()
+    g_mixed = make(R.F32)
+    g_fp64 = make(R.F64) if with_fp64 else None
+    t_setup = time.perf_counter() - t0
+    dom = g_mixed.domains[0]
+    b = R.DeviceArray.from_host(P.rhs)
+    xm = R.DeviceArray(dom.shape(1), np.float64)
+    x64 = R.DeviceArray(dom.shape(1), np.float64)
+    r = R.DeviceArray(dom.shape(1), np.float64)
+    filler = R.GhostFiller(dom, P.filler, R.FACES)
+
+    def run(g, x, k):
+        its, ts = [], []
+        R.sync()
+        t_start = time.perf_counter()
+        for _ in range(k):
+            t1 = time.perf_counter()
+            x.zero()
+            it, _ = g.solve_device(b, x, tol=1e-10, max_it=500, precond=2)
+            R.sync()
+            ts.append(time.perf_counter() - t1)
+            its.append(it)
+        R.sync()
+        return time.perf_counter() - t_start, its, ts
+
+    # warmup (includes the hipGraph capture of the iteration)
+    run(g_mixed, xm, args.warmup)
+    if g_fp64:
+        run(g_fp64, x64, args.warmup)
+    T_mixed, its_mixed, ts_mixed = run(g_mixed, xm, args.steps)
+    log(f"{P.name}: mixed {args.steps} solves in {T_mixed:.3f}s, iterations {its_mixed}")
+    res = {
+        "value": P.dof * args.steps / T_mixed,
+        "ms_per_step": T_mixed / args.steps * 1e3,
+        "iterations_mixed": its_mixed,
+        "solve_ms_median_mixed": statistics.median(ts_mixed) * 1e3,
+        "setup_seconds": {"plans": t_setup},
+        "mixed_true_residual": true_residual(R, filler, dom, b, xm, r),
+    }
+    if g_fp64:
+        T_64, its_64, ts_64 = run(g_fp64, x64, args.steps)
+        log(f"{P.name}: fp64 {args.steps} solves in {T_64:.3f}s, iterations {its_64}")
+        res["fp64_true_residual"] = true_residual(R, filler, dom, b, x64, r)
+        ia, ic = mesh.interior(xm.get(), P.dim), mesh.interior(x64.get(), P.dim)
+        res["mixed_vs_fp64_rel_l2"] = float(np.sqrt(((ia - ic) ** 2).sum() / (ic ** 2).sum()))
+        res["fp64_dof_per_s"] = P.dof * args.steps / T_64
+        res["iterations_fp64"] = its_64
+        res["solve_ms_median_fp64"] = statistics.median(ts_64) * 1e3
+        res["mixed_over_fp64"] = res["value"] / res["fp64_dof_per_s"]
+        if P.exact is not None:
+            e = mesh.interior(x64.get(), P.dim) - mesh.interior(P.exact, P.dim)
+            res["fp64_max_error_vs_exact"] = float(np.abs(e).max())
+    return res, g_mixed, g_fp64, (b, xm, r, filler, dom)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2,
-                    help="workload: 1 uniform 2D (64 x 32^2, BiCGStab smoother), 2 adaptive 2D (19,930 x 16^2, "
-                         "DFT smoother), 3 uniform 3D (32,768 x 16^3), 4 VarPoisson 2D, 5 independent small problems")
+    ap.add_argument("--config", type=int, default=3,
+                    help="workload: 1 uniform 2D (64 x 32^2, BiCGStab smoother), 2 adaptive 2D (19,930 x 16^2), "
+                         "3 uniform 3D (32,768 x 16^3, default), 4 VarPoisson 2D, 5 independent small problems")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the adaptive 2D secondary measurement")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the standalone kernel timings (for counter runs under rocprofv3)")
     ap.add_argument("--problems", type=int, default=64, help="many problems: number of problems (512 in full)")
@@ -230,8 +365,6 @@ def main():
         run_many_problems(args)
         return
 
-    import numpy as np
-
     from thunderegg_amd import configs
     from thunderegg_amd import runtime as R
     from thunderegg_amd import solver
@@ -242,58 +375,15 @@ def main():
     t_mesh = time.perf_counter() - t0
     log(f"config {args.config}: {P.name}; levels {[len(l) for l in P.levels]}; DOF {P.dof}; mesh {t_mesh:.1f}s")
     if args.kernels_only:
-        kr = kernel_rooflines(R, P, R.Stream(), reps=args.steps)
-        print(json.dumps({"kernels": kr}), flush=True)
+        if P.dim == 3:
+            g = solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, pre_sweeps=P.pre,
+                           post_sweeps=P.post, coarse_sweeps=P.coarse, precision=R.F32)
+            print(json.dumps({"kernels": kernel_rooflines_3d(g, P, 9, reps=args.steps)}), flush=True)
+        else:
+            print(json.dumps({"kernels": kernel_rooflines_2d(R, P, R.Stream(), reps=args.steps)}), flush=True)
         return
 
-    def make(precision):
-        return solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, op_kind=P.op_kind,
-                          coeffs=P.coeffs, pre_sweeps=P.pre, post_sweeps=P.post, coarse_sweeps=P.coarse,
-                          inner_max_it=P.inner_max_it, inner_tol=P.inner_tol, precision=precision, use_graph=True)
-
-    t0 = time.perf_counter()
-    g_mixed = make(R.F32)
-    g_fp64 = None if args.no_fp64 else make(R.F64)
-    t_setup = time.perf_counter() - t0
-    dom = g_mixed.domains[0]
-    b = R.DeviceArray.from_host(P.rhs)
-    xm = R.DeviceArray(dom.shape(1), np.float64)
-    x64 = R.DeviceArray(dom.shape(1), np.float64)
-    r = R.DeviceArray(dom.shape(1), np.float64)
-    filler = R.GhostFiller(dom, P.filler, R.FACES)
-
-    def run(g, x, k):
-        its = []
-        R.sync()
-        ts = []
-        t_start = time.perf_counter()
-        for _ in range(k):
-            t1 = time.perf_counter()
-            x.zero()
-            it, _ = g.solve_device(b, x, tol=1e-10, max_it=500, precond=2)
-            R.sync()
-            ts.append(time.perf_counter() - t1)
-            its.append(it)
-        R.sync()
-        return time.perf_counter() - t_start, its, ts
-
-    # warmup (includes hipGraph capture of both BiCGStab phases)
-    run(g_mixed, xm, args.warmup)
-    if g_fp64:
-        run(g_fp64, x64, args.warmup)
-    log("warmup done")
-    T_mixed, its_mixed, ts_mixed = run(g_mixed, xm, args.steps)
-    log(f"mixed: {args.steps} solves in {T_mixed:.3f}s, iterations {its_mixed}")
-    res = {"mixed_true_residual": true_residual(R, filler, dom, b, xm, r)}
-    if g_fp64:
-        T_64, its_64, ts_64 = run(g_fp64, x64, args.steps)
-        log(f"fp64: {args.steps} solves in {T_64:.3f}s, iterations {its_64}")
-        res["fp64_true_residual"] = true_residual(R, filler, dom, b, x64, r)
-        a, c = xm.get(), x64.get()
-        from thunderegg_amd import mesh
-
-        ia, ic = mesh.interior(a, P.dim), mesh.interior(c, P.dim)
-        res["mixed_vs_fp64_rel_l2"] = float(np.sqrt(((ia - ic) ** 2).sum() / (ic ** 2).sum()))
+    res, g_mixed, g_fp64, (b, xm, r, filler, dom) = measure(P, args, R, solver, with_fp64=not args.no_fp64)
     if args.timer:
         # outside the timed region: the timings come from events on the solver's stream
         timer = R.Timer()
@@ -305,23 +395,14 @@ def main():
         g_mixed.set_timer(None)
         timer.save_to_file(args.timer)
         log(timer.report())
-        tj = timer.to_json()
-        for it in tj["timings"]:
-            log(f"{it['name']}: {it['num_calls']} calls, {it['sum'] * 1e3:.3f} ms")
-            for t in it.get("timings", []):
-                k = t["domain_id"]
-                log(f"  level {k} ({len(P.levels[k])} patches) {t['name']}: {t['num_calls']} calls, "
-                    f"{t['sum'] * 1e3:.3f} ms, {t['sum'] / t['num_calls'] * 1e6:.1f} us each")
-    ms = T_mixed / args.steps * 1e3
-    value = P.dof * args.steps / T_mixed
     out = {
-        "metric": "DOF/s to 1e-10 rel residual, FP32-precond+FP64 refine vs FP64 GPU; HBM GB/s",
-        "value": value,
+        "metric": METRIC,
+        "value": res.pop("value"),
         "unit": "DOF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": res.pop("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -336,35 +417,47 @@ def main():
             "tolerance": 1e-10,
             "smoothing": [P.pre, P.post, P.coarse],
         },
-        "iterations_mixed": its_mixed,
-        "solve_ms_median_mixed": statistics.median(ts_mixed) * 1e3,
-        "setup_seconds": {"mesh": t_mesh, "plans": t_setup},
     }
+    res["setup_seconds"]["mesh"] = t_mesh
     out.update(res)
-    if g_fp64:
-        out["fp64_dof_per_s"] = P.dof * args.steps / T_64
-        out["iterations_fp64"] = its_64
-        out["solve_ms_median_fp64"] = statistics.median(ts_64) * 1e3
-        out["mixed_over_fp64"] = out["value"] / out["fp64_dof_per_s"]
+    its_m = out["iterations_mixed"][0]
     if not args.no_roofline:
-        st = R.Stream()
-        kr = kernel_rooflines(R, P, st)
+        if P.dim == 3:
+            kr = kernel_rooflines_3d(g_mixed, P, its_m)
+            dom_k = max(kr, key=lambda k: kr[k]["ms_per_solve"])
+        else:
+            kr = kernel_rooflines_2d(R, P, R.Stream())
+            cands = [k for k in kr if k != "vec_copy_f64"]
+            dom_k = max(cands, key=lambda k: kr[k]["ms"])
         out["kernels"] = kr
-        # the dominant kernel: the longest of the finest-level V-cycle launches
-        cands = [k for k in kr if k != "vec_copy_f64"]
-        dom_k = max(cands, key=lambda k: kr[k]["ms"])
         k = kr[dom_k]
         traffic = None
-        tf = os.path.join(ROOT, "profiles", f"traffic_config{args.config}.json")
+        tf = os.path.join(ROOT, "profiles", f"traffic_{'uniform3d' if P.dim == 3 else 'adaptive2d'}.json")
         if os.path.exists(tf):
             with open(tf) as fh:
                 traffic = json.load(fh).get(dom_k, {}).get("bytes_per_launch")
         out["roofline"] = {"bound": "hbm", "achieved": k["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom_k,
                            "bytes_per_launch": k["bytes"], "ms_per_launch": k["ms"]}
+        if "TFs" in k:
+            out["roofline"]["mfma_tflops"] = k["TFs"]
+            out["roofline"]["mfma_frac"] = k["TFs"] / MFMA_PEAK_TFS["f32"]
+    del g_mixed, g_fp64
+    if args.config == 3 and not args.no_secondary:
+        P2 = configs.adaptive2d_dft()
+        sargs = argparse.Namespace(steps=max(args.steps, 10), warmup=max(args.warmup, 2))
+        r2, g2m, g2d, _ = measure(P2, sargs, R, solver)
+        out["secondary"] = {"workload": P2.name, "dof": P2.dof, "value": r2["value"],
+                            "fp64_dof_per_s": r2["fp64_dof_per_s"], "mixed_over_fp64": r2["mixed_over_fp64"],
+                            "iterations_mixed": r2["iterations_mixed"][0], "iterations_fp64": r2["iterations_fp64"][0],
+                            "solve_ms_median_mixed": r2["solve_ms_median_mixed"],
+                            "solve_ms_median_fp64": r2["solve_ms_median_fp64"],
+                            "mixed_true_residual": r2["mixed_true_residual"],
+                            "fp64_true_residual": r2["fp64_true_residual"]}
+        del g2m, g2d
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(P, out.get("iterations_fp64", its_mixed)[0])
+        out["cpu_baseline"] = cpu_baseline(P, out.get("iterations_fp64", out["iterations_mixed"])[0])
     print(json.dumps(out), flush=True)
 
 

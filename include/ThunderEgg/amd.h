@@ -26,12 +26,15 @@
 
 #include <array>
 #include <bitset>
+#include <cmath>
 #include <cstdio>
+#include <functional>
 #include <iostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <typeinfo>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -90,7 +93,7 @@ class Timer {
   {
     std::array<int, 3> n3{ 1, 1, 1 };
     for (size_t a = 0; a < domain.getNs().size(); a++) n3[a] = domain.getNs()[a];
-    const auto& pv = domain.getPatchInfoVector();
+    const auto& pv = domain.getPatchTable();
     detail::check(te_timer_add_domain(h_.get(), domain_id, (int)domain.getNs().size(), n3.data(), (int)pv.size(),
                                       pv.data()));
   }
@@ -159,36 +162,341 @@ class Communicator {
 /* GhostFillingType.h */
 enum class GhostFillingType { Faces, Edges, Corners };
 
-/* Side<D> = Face<D, D - 1> (Face.h:41-584): 0 west, 1 east, 2 south, 3 north, 4 bottom,
- * 5 top; axis = value / 2, lower side = even value. */
-template <int D>
-class Side {
+/* Face<D, M> (Face.h:41-584): an M-dimensional face of a D-dimensional patch, with the
+ * reference's encoding.  Side<D> = Face<D, D - 1> (value 2 axis + upper), Edge = Face<3, 1>
+ * (value >> 2 = the axis parallel to the edge, low bits = upper on the other axes in
+ * increasing order), Corner<D> = Face<D, 0> (bit a = upper on axis a). */
+template <int D, int M>
+class Face {
+  static constexpr int choose(int n, int k) { return k == 0 ? 1 : choose(n - 1, k - 1) * n / k; }
+
  public:
-  constexpr explicit Side(int value = 0) : v_(value) {}
-  static constexpr Side west() { return Side(0); }
-  static constexpr Side east() { return Side(1); }
-  static constexpr Side south() { return Side(2); }
-  static constexpr Side north() { return Side(3); }
-  static constexpr Side bottom() { return Side(4); }
-  static constexpr Side top() { return Side(5); }
-  size_t getIndex() const { return (size_t)v_; }
+  static constexpr int dimensionality = M;
+  static constexpr size_t number_of = (size_t)(1 << (D - M)) * (size_t)choose(D, M);
+  constexpr Face() : v_((unsigned char)number_of) {}
+  constexpr explicit Face(unsigned char value) : v_(value) {}
+  static constexpr Face null() { return Face((unsigned char)number_of); }
+  size_t getIndex() const { return v_; }
+  /* the D - M sides that meet in this face, in axis order (Face.h:496-524) */
+  std::array<Face<D, D - 1>, D - M> getSides() const
+  {
+    std::array<Face<D, D - 1>, D - M> out;
+    if (M == D - 1) {
+      out[0] = Face<D, D - 1>(v_);
+    } else if (M == 0) {
+      for (int a = 0; a < D; a++) out[(size_t)a] = Face<D, D - 1>((unsigned char)(2 * a + ((v_ >> a) & 1)));
+    } else {  // D == 3, M == 1
+      const int par = v_ >> 2;
+      for (int a = 0, k = 0; a < 3; a++)
+        if (a != par) {
+          out[(size_t)k] = Face<D, D - 1>((unsigned char)(2 * a + ((v_ >> k) & 1)));
+          k++;
+        }
+    }
+    return out;
+  }
+  Face opposite() const { return Face((unsigned char)(v_ ^ ~((~0u) << (D - M)))); }
+  /* Side only */
   size_t getAxisIndex() const { return (size_t)(v_ >> 1); }
   bool isLowerOnAxis() const { return (v_ & 1) == 0; }
   bool isHigherOnAxis() const { return (v_ & 1) == 1; }
-  bool operator==(const Side& o) const { return v_ == o.v_; }
+  /* Edge only: the axis parallel to the edge */
+  size_t getAxisIndexOfEdge() const { return (size_t)(v_ >> 2); }
+  bool operator==(const Face& o) const { return v_ == o.v_; }
+  bool operator!=(const Face& o) const { return v_ != o.v_; }
+  bool operator<(const Face& o) const { return v_ < o.v_; }
+  /* all faces of this dimension, in value order */
+  static std::vector<Face> getValues()
+  {
+    std::vector<Face> out;
+    for (size_t k = 0; k < number_of; k++) out.push_back(Face((unsigned char)k));
+    return out;
+  }
+  // named faces (Face.h:106-440)
+  static constexpr Face west() { return Face(0); }
+  static constexpr Face east() { return Face(1); }
+  static constexpr Face south() { return Face(2); }
+  static constexpr Face north() { return Face(3); }
+  static constexpr Face bottom() { return Face(4); }
+  static constexpr Face top() { return Face(5); }
+  static constexpr Face sw() { return Face(0); }
+  static constexpr Face se() { return Face(1); }
+  static constexpr Face nw() { return Face(2); }
+  static constexpr Face ne() { return Face(3); }
+  static constexpr Face bsw() { return Face(0); }
+  static constexpr Face bse() { return Face(1); }
+  static constexpr Face bnw() { return Face(2); }
+  static constexpr Face bne() { return Face(3); }
+  static constexpr Face tsw() { return Face(4); }
+  static constexpr Face tse() { return Face(5); }
+  static constexpr Face tnw() { return Face(6); }
+  static constexpr Face tne() { return Face(7); }
+  static constexpr Face bs() { return Face(0); }
+  static constexpr Face bn() { return Face(1); }
+  static constexpr Face ts() { return Face(2); }
+  static constexpr Face tn() { return Face(3); }
+  static constexpr Face bw() { return Face(4); }
+  static constexpr Face be() { return Face(5); }
+  static constexpr Face tw() { return Face(6); }
+  static constexpr Face te() { return Face(7); }
+  static constexpr Face sw_edge() { return Face(8); }
+  static constexpr Face se_edge() { return Face(9); }
+  static constexpr Face nw_edge() { return Face(10); }
+  static constexpr Face ne_edge() { return Face(11); }
 
  private:
-  int v_;
+  unsigned char v_;
+};
+template <int D>
+using Side = Face<D, D - 1>;
+using Edge = Face<3, 1>;
+template <int D>
+using Corner = Face<D, 0>;
+
+/* Orthant<D> (Orthant.h:42-348): bit a = upper half on axis a. */
+template <int D>
+class Orthant {
+ public:
+  static constexpr size_t num_orthants = (size_t)1 << D;
+  constexpr Orthant() : v_((unsigned char)num_orthants) {}
+  constexpr explicit Orthant(unsigned char value) : v_(value) {}
+  static constexpr Orthant null() { return Orthant((unsigned char)num_orthants); }
+  static constexpr Orthant lower() { return Orthant(0); }
+  static constexpr Orthant upper() { return Orthant(1); }
+  static constexpr Orthant sw() { return Orthant(0); }
+  static constexpr Orthant se() { return Orthant(1); }
+  static constexpr Orthant nw() { return Orthant(2); }
+  static constexpr Orthant ne() { return Orthant(3); }
+  size_t getIndex() const { return v_; }
+  bool isOnSide(Side<D> s) const { return (((v_ >> s.getAxisIndex()) & 1) != 0) == s.isHigherOnAxis(); }
+  static std::vector<Orthant> getValues()
+  {
+    std::vector<Orthant> out;
+    for (size_t k = 0; k < num_orthants; k++) out.push_back(Orthant((unsigned char)k));
+    return out;
+  }
+  bool operator==(const Orthant& o) const { return v_ == o.v_; }
+  bool operator!=(const Orthant& o) const { return v_ != o.v_; }
+
+ private:
+  unsigned char v_;
 };
 
-/* The PatchInfo<D> fields that locate a patch's cells (PatchInfo.h:50-140); the default
- * is one cell of unit size at the origin, as the reference's default constructor. */
+/* NbrType.h:33-48 */
+enum class NbrType { Normal, Coarse, Fine };
+
+/* View<T, D> (View.h:40-226): a strided D-dimensional view of patch data with a ghost
+ * range; coordinates run from getGhostStart() (= -g on a padded axis) to getGhostEnd(). */
+template <typename T, int D>
+class View {
+ public:
+  using T_ptr = T*;
+  View() = default;
+  View(T* data, const std::array<int, D>& strides, const std::array<int, D>& ghost_start,
+       const std::array<int, D>& start, const std::array<int, D>& end, const std::array<int, D>& ghost_end)
+      : data_(data), strides_(strides), gs_(ghost_start), s_(start), e_(end), ge_(ghost_end)
+  {
+  }
+  template <typename U, typename = typename std::enable_if<std::is_same<const U, T>::value>::type>
+  View(const View<U, D>& o)
+      : data_(o.getPointer()), strides_(o.getStrides()), gs_(o.getGhostStart()), s_(o.getStart()), e_(o.getEnd()),
+        ge_(o.getGhostEnd())
+  {
+  }
+  T& operator[](const std::array<int, D>& coord) const
+  {
+    long off = 0;
+    for (int a = 0; a < D; a++) off += (long)coord[(size_t)a] * strides_[(size_t)a];
+    return data_[off];
+  }
+  template <typename... I>
+  T& operator()(I... coord) const
+  {
+    static_assert(sizeof...(I) == D, "one index per axis");
+    return (*this)[std::array<int, D>{ { (int)coord... } }];
+  }
+  const std::array<int, D>& getStrides() const { return strides_; }
+  const std::array<int, D>& getGhostStart() const { return gs_; }
+  const std::array<int, D>& getStart() const { return s_; }
+  const std::array<int, D>& getEnd() const { return e_; }
+  const std::array<int, D>& getGhostEnd() const { return ge_; }
+  /* pointer to the element at coordinate 0 */
+  T* getPointer() const { return data_; }
+
+ private:
+  T* data_ = nullptr;
+  std::array<int, D> strides_{}, gs_{}, s_{}, e_{}, ge_{};
+};
+
+namespace detail {
+/* the view of face f (M-dimensional) of a view with D spatial axes (and, when C = 1, a
+ * trailing component axis kept as the last axis of the result); offset[k] = layer
+ * counted inward from the face's k-th side (0 = first interior layer, -1 = first ghost) */
+template <typename T, int D, int M, int C>
+View<T, M + C> face_slice(const View<T, D + C>& v, Face<D, M> f, const std::array<int, D - M>& offset)
+{
+  auto sides = f.getSides();
+  T* p = v.getPointer();
+  std::array<bool, D> fixed{};
+  for (int k = 0; k < D - M; k++) {
+    const Face<D, D - 1> sd = sides[(size_t)k];
+    const int a = (int)sd.getAxisIndex();
+    fixed[(size_t)a] = true;
+    const int idx = sd.isLowerOnAxis() ? offset[(size_t)k] : v.getEnd()[(size_t)a] - offset[(size_t)k];
+    p += (long)idx * v.getStrides()[(size_t)a];
+  }
+  std::array<int, M + C> st{}, gs{}, s{}, e{}, ge{};
+  int j = 0;
+  for (int a = 0; a < D + C; a++) {
+    if (a < D && fixed[(size_t)a]) continue;
+    st[(size_t)j] = v.getStrides()[(size_t)a];
+    gs[(size_t)j] = v.getGhostStart()[(size_t)a];
+    s[(size_t)j] = v.getStart()[(size_t)a];
+    e[(size_t)j] = v.getEnd()[(size_t)a];
+    ge[(size_t)j] = v.getGhostEnd()[(size_t)a];
+    j++;
+  }
+  return View<T, M + C>(p, st, gs, s, e, ge);
+}
+}  // namespace detail
+
+/* ComponentView<T, D> (ComponentView.h:38-266): one component of one patch */
+template <typename T, int D>
+class ComponentView : public View<T, D> {
+ public:
+  ComponentView() = default;
+  explicit ComponentView(const View<T, D>& v) : View<T, D>(v) {}
+  template <int M>
+  View<T, M> getSliceOn(Face<D, M> f, const std::array<int, D - M>& offset) const
+  {
+    return detail::face_slice<T, D, M, 0>(*this, f, offset);
+  }
+  /* the ghost layers are writable through a view of const data (PatchView.h:261-262) */
+  template <int M>
+  View<typename std::remove_const<T>::type, M> getGhostSliceOn(Face<D, M> f, const std::array<size_t, D - M>& offset) const
+  {
+    std::array<int, D - M> o;
+    for (size_t k = 0; k < (size_t)(D - M); k++) o[k] = -1 - (int)offset[k];
+    View<typename std::remove_const<T>::type, D> w(const_cast<typename std::remove_const<T>::type*>(this->getPointer()),
+                                                   this->getStrides(), this->getGhostStart(), this->getStart(),
+                                                   this->getEnd(), this->getGhostEnd());
+    return detail::face_slice<typename std::remove_const<T>::type, D, M, 0>(w, f, o);
+  }
+};
+
+/* PatchView<T, D> (PatchView.h:38-306): all components of one patch; the last axis is the
+ * component */
+template <typename T, int D>
+class PatchView : public View<T, D + 1> {
+ public:
+  PatchView() = default;
+  explicit PatchView(const View<T, D + 1>& v) : View<T, D + 1>(v) {}
+  template <typename U, typename = typename std::enable_if<std::is_same<const U, T>::value>::type>
+  PatchView(const PatchView<U, D>& o) : View<T, D + 1>(o)
+  {
+  }
+  template <int M>
+  View<T, M + 1> getSliceOn(Face<D, M> f, const std::array<int, D - M>& offset) const
+  {
+    return detail::face_slice<T, D, M, 1>(*this, f, offset);
+  }
+  template <int M>
+  View<typename std::remove_const<T>::type, M + 1> getGhostSliceOn(Face<D, M> f,
+                                                                  const std::array<size_t, D - M>& offset) const
+  {
+    using U = typename std::remove_const<T>::type;
+    std::array<int, D - M> o;
+    for (size_t k = 0; k < (size_t)(D - M); k++) o[k] = -1 - (int)offset[k];
+    View<U, D + 1> w(const_cast<U*>(this->getPointer()), this->getStrides(), this->getGhostStart(),
+                     this->getStart(), this->getEnd(), this->getGhostEnd());
+    return detail::face_slice<U, D, M, 1>(w, f, o);
+  }
+  ComponentView<T, D> getComponentView(int component) const
+  {
+    std::array<int, D> st, gs, s, e, ge;
+    for (size_t a = 0; a < (size_t)D; a++) {
+      st[a] = this->getStrides()[a];
+      gs[a] = this->getGhostStart()[a];
+      s[a] = this->getStart()[a];
+      e[a] = this->getEnd()[a];
+      ge[a] = this->getGhostEnd()[a];
+    }
+    return ComponentView<T, D>(
+      View<T, D>(this->getPointer() + (long)component * this->getStrides()[(size_t)D], st, gs, s, e, ge));
+  }
+};
+
+/* Loops.h:38-158 */
+namespace Loop {
+template <int D, class F>
+void Nested(const std::array<int, D>& start, const std::array<int, D>& end, F f)
+{
+  std::array<int, D> c = start;
+  if (D == 0) {
+    f(c);
+    return;
+  }
+  for (int a = 0; a < D; a++)
+    if (end[(size_t)a] < start[(size_t)a]) return;
+  while (true) {
+    f(c);
+    int a = 0;
+    while (a < D) {
+      if (c[(size_t)a] < end[(size_t)a]) {
+        c[(size_t)a]++;
+        break;
+      }
+      c[(size_t)a] = start[(size_t)a];
+      a++;
+    }
+    if (a == D) return;
+  }
+}
+template <int D, class V, class F>
+void OverInteriorIndexes(const V& v, F f)
+{
+  Nested<D>(v.getStart(), v.getEnd(), f);
+}
+template <int D, class V, class F>
+void OverAllIndexes(const V& v, F f)
+{
+  Nested<D>(v.getGhostStart(), v.getGhostEnd(), f);
+}
+}  // namespace Loop
+
+/* Neighbour records of PatchInfo (NormalNbrInfo.h:38-100, CoarseNbrInfo.h:38-113,
+ * FineNbrInfo.h:38-127); one rank, so every rank is 0 */
+template <int M>
+struct NormalNbrInfo {
+  int id = -1, rank = 0, local_index = -1;
+};
+template <int M>
+struct CoarseNbrInfo {
+  int id = -1, rank = 0, local_index = -1;
+  Orthant<M> orth_on_coarse;
+};
+template <int M>
+struct FineNbrInfo {
+  std::array<int, Orthant<M>::num_orthants> ids{}, ranks{}, local_indexes{};
+};
+
+/* PatchInfo<D> (PatchInfo.h:52-463): geometry, tree links and one neighbour slot per face,
+ * edge and corner (slot = Face<D, M>::sum_of_faces + face value, PatchInfo.h:222-289).  The
+ * default is one cell of unit size at the origin with no neighbours, as the reference's
+ * default constructor. */
 template <int D>
 struct PatchInfo {
   int id = 0;
   int local_index = 0;
+  int global_index = 0;
+  int rank = 0;
   int refine_level = -1;
+  int parent_id = -1;
+  int parent_rank = -1;
+  Orthant<D> orth_on_parent = Orthant<D>::null();
   int num_ghost_cells = 1;
+  std::array<int, (1 << D)> child_ids;
+  std::array<int, (1 << D)> child_ranks;
   std::array<int, D> ns;
   std::array<double, D> starts;
   std::array<double, D> spacings;
@@ -197,6 +505,81 @@ struct PatchInfo {
     ns.fill(1);
     starts.fill(0.0);
     spacings.fill(1.0);
+    child_ids.fill(-1);
+    child_ranks.fill(-1);
+    nbr_type_.fill(-1);
+    nbr_orth_.fill(-1);
+    for (auto& a : nbr_id_) a.fill(-1);
+    for (auto& a : nbr_local_) a.fill(-1);
+  }
+  template <int M>
+  static constexpr int slot(Face<D, M> f)
+  {
+    return (D == 2 ? (M == 0 ? 0 : 4) : (M == 0 ? 0 : (M == 1 ? 8 : 20))) + (int)f.getIndex();
+  }
+  template <int M>
+  bool hasNbr(Face<D, M> f) const
+  {
+    return nbr_type_[(size_t)slot(f)] >= 0;
+  }
+  /* any neighbour on a face, edge or corner */
+  bool hasNbr() const
+  {
+    for (int t : nbr_type_)
+      if (t >= 0) return true;
+    return false;
+  }
+  template <int M>
+  NbrType getNbrType(Face<D, M> f) const
+  {
+    const int t = nbr_type_[(size_t)slot(f)];
+    if (t < 0) throw RuntimeError("PatchInfo: no neighbour on that face");
+    return t == TE_NBR_NORMAL ? NbrType::Normal : (t == TE_NBR_COARSE ? NbrType::Coarse : NbrType::Fine);
+  }
+  template <int M>
+  NormalNbrInfo<M> getNormalNbrInfo(Face<D, M> f) const
+  {
+    check_type(f, TE_NBR_NORMAL);
+    NormalNbrInfo<M> r;
+    r.id = nbr_id_[(size_t)slot(f)][0];
+    r.local_index = nbr_local_[(size_t)slot(f)][0];
+    return r;
+  }
+  template <int M>
+  CoarseNbrInfo<M> getCoarseNbrInfo(Face<D, M> f) const
+  {
+    check_type(f, TE_NBR_COARSE);
+    CoarseNbrInfo<M> r;
+    r.id = nbr_id_[(size_t)slot(f)][0];
+    r.local_index = nbr_local_[(size_t)slot(f)][0];
+    r.orth_on_coarse = Orthant<M>((unsigned char)(nbr_orth_[(size_t)slot(f)] < 0 ? 0 : nbr_orth_[(size_t)slot(f)]));
+    return r;
+  }
+  template <int M>
+  FineNbrInfo<M> getFineNbrInfo(Face<D, M> f) const
+  {
+    check_type(f, TE_NBR_FINE);
+    FineNbrInfo<M> r;
+    for (size_t k = 0; k < Orthant<M>::num_orthants; k++) {
+      r.ids[k] = nbr_id_[(size_t)slot(f)][k];
+      r.local_indexes[k] = nbr_local_[(size_t)slot(f)][k];
+      r.ranks[k] = 0;
+    }
+    return r;
+  }
+  bool operator<(const PatchInfo& o) const { return id < o.id; }
+  bool operator==(const PatchInfo& o) const { return id == o.id; }
+  /* the neighbour slots, filled by Domain */
+  std::array<int, 26> nbr_type_;
+  std::array<int, 26> nbr_orth_;
+  std::array<std::array<int, 4>, 26> nbr_id_;
+  std::array<std::array<int, 4>, 26> nbr_local_;
+
+ private:
+  template <int M>
+  void check_type(Face<D, M> f, int t) const
+  {
+    if (nbr_type_[(size_t)slot(f)] != t) throw RuntimeError("PatchInfo: the neighbour is not of the requested type");
   }
 };
 
@@ -214,6 +597,41 @@ class Domain {
     te_domain* h = nullptr;
     detail::check(te_domain_create(D, n3.data(), num_ghost_cells, (int)patches_.size(), patches_.data(), &h));
     h_ = std::shared_ptr<te_domain>(h, detail::Deleter<te_domain, te_domain_destroy>());
+    // the reference's PatchInfo objects, neighbour local indexes resolved (Domain.h:94-109)
+    std::vector<int32_t> nl(patches_.size() * 26 * 4);
+    if (!patches_.empty()) detail::check(te_domain_get_nbr_local(h, nl.data()));
+    infos_ = std::make_shared<std::vector<PatchInfo<D>>>();
+    infos_->reserve(patches_.size());
+    for (size_t i = 0; i < patches_.size(); i++) {
+      const te_patch_info& r = patches_[i];
+      PatchInfo<D> pi;
+      pi.id = r.id;
+      pi.local_index = pi.global_index = (int)i;
+      pi.rank = r.rank;
+      pi.refine_level = r.refine_level;
+      pi.parent_id = r.parent_id;
+      pi.parent_rank = r.parent_rank;
+      pi.orth_on_parent = r.orth_on_parent < 0 ? Orthant<D>::null() : Orthant<D>((unsigned char)r.orth_on_parent);
+      pi.num_ghost_cells = num_ghost_;
+      for (int k = 0; k < (1 << D); k++) {
+        pi.child_ids[(size_t)k] = r.child_ids[k];
+        pi.child_ranks[(size_t)k] = r.child_ranks[k];
+      }
+      for (int a = 0; a < D; a++) {
+        pi.ns[(size_t)a] = ns_[(size_t)a];
+        pi.starts[(size_t)a] = r.starts[a];
+        pi.spacings[(size_t)a] = r.spacings[a];
+      }
+      for (int sl = 0; sl < 26; sl++) {
+        pi.nbr_type_[(size_t)sl] = (D == 2 && sl >= 8) ? -1 : r.nbr_type[sl];
+        pi.nbr_orth_[(size_t)sl] = r.nbr_orth[sl];
+        for (int k = 0; k < 4; k++) {
+          pi.nbr_id_[(size_t)sl][(size_t)k] = pi.nbr_type_[(size_t)sl] >= 0 ? r.nbr_ids[sl][k] : -1;
+          pi.nbr_local_[(size_t)sl][(size_t)k] = nl[(i * 26 + (size_t)sl) * 4 + (size_t)k];
+        }
+      }
+      infos_->push_back(pi);
+    }
   }
   int getId() const { return id_; }
   const std::array<int, D>& getNs() const { return ns_; }
@@ -227,23 +645,15 @@ class Domain {
     return c * (long)patches_.size();
   }
   long getNumGlobalCells() const { return getNumLocalCells(); }
-  const std::vector<te_patch_info>& getPatchInfoVector() const { return patches_; }
-  /* the cell-locating fields of one patch, with the domain's ns and ghost width */
-  PatchInfo<D> getPatchInfo(int local_index) const
+  /* Domain::getPatchInfoVector (Domain.h:257): the patches in local-index order */
+  const std::vector<PatchInfo<D>>& getPatchInfoVector() const
   {
-    const te_patch_info& r = patches_.at((size_t)local_index);
-    PatchInfo<D> pi;
-    pi.id = r.id;
-    pi.local_index = local_index;
-    pi.refine_level = r.refine_level;
-    pi.num_ghost_cells = num_ghost_;
-    for (int a = 0; a < D; a++) {
-      pi.ns[a] = ns_[a];
-      pi.starts[a] = r.starts[a];
-      pi.spacings[a] = r.spacings[a];
-    }
-    return pi;
+    static const std::vector<PatchInfo<D>> none;
+    return infos_ ? *infos_ : none;
   }
+  PatchInfo<D> getPatchInfo(int local_index) const { return getPatchInfoVector().at((size_t)local_index); }
+  /* the flattened te_patch_info records this domain was made from */
+  const std::vector<te_patch_info>& getPatchTable() const { return patches_; }
   Communicator getCommunicator() const { return {}; }
   te_domain* handle() const { return h_.get(); }
   bool operator==(const Domain& o) const { return h_ == o.h_; }
@@ -263,6 +673,7 @@ class Domain {
   std::array<int, D> ns_{};
   int num_ghost_ = 1;
   std::vector<te_patch_info> patches_;
+  std::shared_ptr<std::vector<PatchInfo<D>>> infos_;
   std::shared_ptr<te_domain> h_;
 };
 
@@ -306,9 +717,35 @@ class Vector {
   long getNumLocalCells() const { return domain_.getNumLocalCells(); }
   const Domain<D>& getDomain() const { return domain_; }
   Communicator getCommunicator() const { return {}; }
-  /* device pointer to the padded data */
-  double* data() const { return data_.get(); }
+  /* device pointer to the padded data.  Writes made through patch views (a host copy) are
+   * copied back first, and the host copy is dropped: the caller may change the data on
+   * the device. */
+  double* data() const
+  {
+    to_device();
+    host_->valid = false;
+    return data_.get();
+  }
   long size() const { return len_; }
+  /* Vector::getPatchView / getComponentView (Vector.h:336-375).  The views address a host
+   * copy of the vector (the data lives in device memory): it is fetched on the first view
+   * and copied back before the next device operation on the vector (every other method,
+   * and every operator, smoother or solver the vector is passed to), after which views taken
+   * before that operation are stale.  Ghost cells are writable through views of a const
+   * vector, as in the reference (PatchView.h:261-262). */
+  PatchView<double, D> getPatchView(int local_index) { return PatchView<double, D>(host_view(local_index)); }
+  PatchView<const double, D> getPatchView(int local_index) const
+  {
+    return PatchView<const double, D>(View<const double, D + 1>(host_view(local_index)));
+  }
+  ComponentView<double, D> getComponentView(int component, int local_index)
+  {
+    return getPatchView(local_index).getComponentView(component);
+  }
+  ComponentView<const double, D> getComponentView(int component, int local_index) const
+  {
+    return getPatchView(local_index).getComponentView(component);
+  }
 
   void set(double alpha) { chk(te_vec_set(dh(), TE_F64, data(), alpha, ncomp_, nullptr)); }
   void setWithGhost(double alpha) { chk(te_vec_set_with_ghost(dh(), TE_F64, data(), alpha, ncomp_, nullptr)); }
@@ -378,10 +815,55 @@ class Vector {
   {
     chk(te_memcpy_d2d(data(), o.data(), sizeof(double) * (size_t)len_, nullptr));
   }
+  // the host copy behind the patch views: valid = it holds the current values, dirty = it
+  // may hold writes the device copy lacks
+  struct HostCopy {
+    std::vector<double> h;
+    bool valid = false;
+    bool dirty = false;
+  };
+  void to_device() const
+  {
+    if (!host_->dirty) return;
+    chk(te_stream_synchronize(nullptr));
+    chk(te_memcpy_h2d(data_.get(), host_->h.data(), sizeof(double) * (size_t)len_));
+    host_->dirty = false;
+  }
+  View<double, D + 1> host_view(int local_index) const
+  {
+    if (local_index < 0 || local_index >= getNumLocalPatches()) throw RuntimeError("getPatchView: bad patch index");
+    if (!host_->valid) {
+      host_->h.resize((size_t)len_);
+      chk(te_stream_synchronize(nullptr));
+      chk(te_memcpy_d2h(host_->h.data(), data_.get(), sizeof(double) * (size_t)len_));
+      host_->valid = true;
+    }
+    host_->dirty = true;
+    const auto& ns = domain_.getNs();
+    std::array<int, D + 1> st, gs, s, e, ge;
+    long stride = 1, patch = 1;
+    for (int a = 0; a < D; a++) {
+      st[(size_t)a] = (int)stride;
+      gs[(size_t)a] = -1;
+      s[(size_t)a] = 0;
+      e[(size_t)a] = ns[(size_t)a] - 1;
+      ge[(size_t)a] = ns[(size_t)a];
+      stride *= ns[(size_t)a] + 2;
+    }
+    st[(size_t)D] = (int)stride;  // one component's padded cells
+    gs[(size_t)D] = s[(size_t)D] = 0;
+    e[(size_t)D] = ge[(size_t)D] = ncomp_ - 1;
+    patch = stride * ncomp_;
+    // the element at coordinate 0: one ghost layer in on every axis
+    long first = (long)local_index * patch;
+    for (int a = 0; a < D; a++) first += st[(size_t)a];
+    return View<double, D + 1>(host_->h.data() + first, st, gs, s, e, ge);
+  }
   Domain<D> domain_;
   int ncomp_ = 0;
   long len_ = 0;
   std::shared_ptr<double> data_;
+  mutable std::shared_ptr<HostCopy> host_ = std::make_shared<HostCopy>();
 };
 
 /* Operator.h:40-58 */
@@ -450,7 +932,58 @@ class TriLinearGhostFiller : public DeviceGhostFiller<3> {
   TriLinearGhostFiller* clone() const override { return new TriLinearGhostFiller(*this); }
 };
 
-/* PatchOperator.h:45-143 */
+namespace detail {
+/* A function sampled at the face points of every side of every patch (the points of
+ * DomainTools::GetRealCoordBound, DomainTools.h:178-210), in the boundary-data layout of
+ * te_domain_boundary_length: per patch, per side, the side's face cells with the first of the
+ * other axes fastest.  func(side axis, point). */
+template <int D, class F>
+std::vector<double> boundary_samples(const Domain<D>& domain, F func)
+{
+  std::vector<double> out;
+  out.reserve((size_t)te_domain_boundary_length(domain.handle()));
+  const auto& ns = domain.getNs();
+  for (const auto& r : domain.getPatchInfoVector()) {
+    for (int s = 0; s < 2 * D; s++) {
+      const int a = s >> 1;
+      int fr[2] = { 0, 0 }, nf = 0;
+      for (int b = 0; b < D; b++)
+        if (b != a) fr[nf++] = b;
+      const int n0 = nf > 0 ? ns[fr[0]] : 1, n1 = nf > 1 ? ns[fr[1]] : 1;
+      for (int j = 0; j < n1; j++)
+        for (int i = 0; i < n0; i++) {
+          std::array<double, D> x;
+          x[a] = (s & 1) ? r.starts[a] + r.spacings[a] * ns[a] : r.starts[a];
+          const int idx[2] = { i, j };
+          for (int k = 0; k < nf; k++)
+            x[fr[k]] = r.starts[fr[k]] + r.spacings[fr[k]] / 2 + r.spacings[fr[k]] * idx[k];
+          out.push_back(func(a, x));
+        }
+    }
+  }
+  return out;
+}
+
+/* a host array copied to a temporary device buffer */
+struct DeviceCopy {
+  explicit DeviceCopy(const std::vector<double>& h)
+  {
+    check(te_device_alloc(sizeof(double) * (h.empty() ? 1 : h.size()), &p));
+    if (!h.empty()) check(te_memcpy_h2d(p, h.data(), sizeof(double) * h.size()));
+  }
+  ~DeviceCopy() { te_device_free(p); }
+  DeviceCopy(const DeviceCopy&) = delete;
+  DeviceCopy& operator=(const DeviceCopy&) = delete;
+  const double* get() const { return static_cast<const double*>(p); }
+  void* p = nullptr;
+};
+}  // namespace detail
+
+/* PatchOperator.h:45-143.  The per-patch virtuals take host views of one patch (see
+ * Vector::getPatchView).  apply() fills the ghosts of u and calls applySinglePatch on every
+ * patch; the built-in operators override it with their device kernels (when they are used
+ * as they are, not through a subclass), so a user-derived operator runs through the
+ * virtuals and a built-in one on the device. */
 template <int D>
 class PatchOperator : public Operator<D> {
  public:
@@ -458,12 +991,34 @@ class PatchOperator : public Operator<D> {
       : domain_(domain), filler_(ghost_filler.clone())
   {
     dev_ = dynamic_cast<const DeviceGhostFiller<D>*>(filler_.get());
-    if (!dev_) throw RuntimeError("PatchOperator: the ghost filler must be one of the built-in fillers");
   }
   PatchOperator<D>* clone() const override = 0;
+  /* f = A u on one patch; u's ghost values are filled (PatchOperator.h:85-93) */
+  virtual void applySinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view,
+                                const PatchView<double, D>& f_view) const = 0;
+  /* the patch's internal boundaries treated as domain boundaries (:96-107) */
+  virtual void modifyRHSForInternalBoundaryConditions(const PatchInfo<D>& pinfo,
+                                                      const PatchView<const double, D>& u_view,
+                                                      const PatchView<double, D>& f_view) const = 0;
+  virtual void applySinglePatchWithInternalBoundaryConditions(const PatchInfo<D>& pinfo,
+                                                              const PatchView<const double, D>& u_view,
+                                                              const PatchView<double, D>& f_view) const = 0;
+  /* PatchOperator::apply (:126-143): f := 0 with ghosts, fill u's ghosts, every patch */
+  void apply(const Vector<D>& u, Vector<D>& f) const override
+  {
+    f.setWithGhost(0.0);
+    filler_->fillGhost(u);
+    for (const PatchInfo<D>& pinfo : domain_.getPatchInfoVector())
+      applySinglePatch(pinfo, u.getPatchView(pinfo.local_index), f.getPatchView(pinfo.local_index));
+  }
   const Domain<D>& getDomain() const { return domain_; }
   const GhostFiller<D>& getGhostFiller() const { return *filler_; }
-  const DeviceGhostFiller<D>& deviceFiller() const { return *dev_; }
+  bool hasDeviceFiller() const { return dev_ != nullptr; }
+  const DeviceGhostFiller<D>& deviceFiller() const
+  {
+    if (!dev_) throw RuntimeError("PatchOperator: the device path needs one of the built-in ghost fillers");
+    return *dev_;
+  }
 
  private:
   Domain<D> domain_;
@@ -471,8 +1026,25 @@ class PatchOperator : public Operator<D> {
   const DeviceGhostFiller<D>* dev_ = nullptr;
 };
 
+namespace detail {
+/* the physical-side (internal = false) or neighbour-side (internal = true) ghost values
+ * of u set to sign * the first interior layer (StarPatchOperator::enforceBoundaryConditions
+ * / enforceInternalBoundaryConditions, Poisson/StarPatchOperator.h:115-164) */
+template <int D>
+void set_side_ghosts(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view, bool internal,
+                     double sign)
+{
+  for (Side<D> s : Side<D>::getValues()) {
+    if (pinfo.hasNbr(s) != internal) continue;
+    View<double, D> g = u_view.getGhostSliceOn(s, { 0 });
+    View<const double, D> mid = u_view.getSliceOn(s, { 0 });
+    Loop::OverInteriorIndexes<D>(mid, [&](const std::array<int, D>& c) { g[c] = sign * mid[c]; });
+  }
+}
+}  // namespace detail
+
 namespace Poisson {
-/* Poisson/StarPatchOperator.h:39-204: f = sum_a (u+ - 2u + u-)/h_a^2 */
+/* Poisson/StarPatchOperator.h:39-243: f = sum_a (u+ - 2u + u-)/h_a^2 */
 template <int D>
 class StarPatchOperator : public PatchOperator<D> {
  public:
@@ -484,18 +1056,96 @@ class StarPatchOperator : public PatchOperator<D> {
   StarPatchOperator* clone() const override { return new StarPatchOperator(*this); }
   void apply(const Vector<D>& x, Vector<D>& b) const override
   {
+    if (typeid(*this) != typeid(StarPatchOperator) || !this->hasDeviceFiller()) {
+      PatchOperator<D>::apply(x, b);  // a subclass or a user filler: the per-patch virtuals
+      return;
+    }
     detail::check(te_star_apply(this->deviceFiller().handle(), TE_F64, neumann_, x.data(), b.data(),
                                 x.getNumComponents(), nullptr));
   }
   bool neumann() const { return neumann_; }
+  void applySinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view,
+                        const PatchView<double, D>& f_view) const override
+  {
+    applySinglePatch(pinfo, u_view, f_view, false);
+  }
+  void applySinglePatchWithInternalBoundaryConditions(const PatchInfo<D>& pinfo,
+                                                      const PatchView<const double, D>& u_view,
+                                                      const PatchView<double, D>& f_view) const override
+  {
+    applySinglePatch(pinfo, u_view, f_view, true);
+  }
+  /* physical ghosts: -u (Dirichlet) or +u (Neumann) (:115-144) */
+  void enforceBoundaryConditions(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view) const
+  {
+    detail::set_side_ghosts<D>(pinfo, u_view, false, neumann_ ? 1.0 : -1.0);
+  }
+  /* neighbour-side ghosts: -u (:145-164) */
+  void enforceInternalBoundaryConditions(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view) const
+  {
+    detail::set_side_ghosts<D>(pinfo, u_view, true, -1.0);
+  }
+  /* f_b -= (u_g + u_b) / h^2 on sides with a neighbour (:165-180) */
+  void modifyRHSForInternalBoundaryConditions(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view,
+                                              const PatchView<double, D>& f_view) const override
+  {
+    for (Side<D> s : Side<D>::getValues()) {
+      if (!pinfo.hasNbr(s)) continue;
+      const double h2 = pinfo.spacings[s.getAxisIndex()] * pinfo.spacings[s.getAxisIndex()];
+      View<double, D> f_inner = f_view.getSliceOn(s, { 0 });
+      View<const double, D> u_ghost = u_view.getSliceOn(s, { -1 });
+      View<const double, D> u_inner = u_view.getSliceOn(s, { 0 });
+      Loop::OverInteriorIndexes<D>(f_inner, [&](const std::array<int, D>& c) {
+        f_inner[c] -= (u_ghost[c] + u_inner[c]) / h2;
+      });
+    }
+  }
+  /* addDrichletBCToRHS (Poisson/StarPatchOperator.h:187-204): on every side without a
+   * neighbour, f_b += -2 g(face point) / h^2 (gfunc sampled on the host, added on the device) */
+  void addDrichletBCToRHS(Vector<D>& f, std::function<double(const std::array<double, D>&)> gfunc) const
+  {
+    const detail::DeviceCopy g(
+      detail::boundary_samples<D>(this->getDomain(), [&](int, const std::array<double, D>& x) { return gfunc(x); }));
+    chk_sync(te_star_add_dirichlet_bc(this->getDomain().handle(), TE_F64, f.data(), g.get(), nullptr, nullptr));
+  }
+  /* addNeumannBCToRHS (:206-242): f_b += dg/dx_a / h on lower sides, -= on upper sides, with
+   * dg/dx_a = gfunc_grad[a] of the side's axis a */
+  void addNeumannBCToRHS(Vector<D>& f, std::function<double(const std::array<double, D>&)> gfunc,
+                         std::array<std::function<double(const std::array<double, D>&)>, D> gfunc_grad) const
+  {
+    (void)gfunc;  // the reference takes it too and uses only the gradient
+    const detail::DeviceCopy dg(detail::boundary_samples<D>(
+      this->getDomain(), [&](int a, const std::array<double, D>& x) { return gfunc_grad[(size_t)a](x); }));
+    chk_sync(te_star_add_neumann_bc(this->getDomain().handle(), TE_F64, f.data(), dg.get(), nullptr));
+  }
 
  private:
+  /* the stencil on one patch (:75-101), axis by axis: f = (axis 0 ? 0 : f) + (u+ - 2u + u-)/h^2 */
+  void applySinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view,
+                        const PatchView<double, D>& f_view, bool internal) const
+  {
+    if (internal) enforceInternalBoundaryConditions(pinfo, u_view);
+    enforceBoundaryConditions(pinfo, u_view);
+    for (int a = 0; a < D; a++) {
+      const double h2 = pinfo.spacings[(size_t)a] * pinfo.spacings[(size_t)a];
+      const int st = u_view.getStrides()[(size_t)a];
+      Loop::OverInteriorIndexes<D + 1>(u_view, [&](const std::array<int, D + 1>& c) {
+        const double* q = &u_view[c];
+        f_view[c] = (a == 0 ? 0.0 : 1.0) * f_view[c] + (q[st] - 2 * q[0] + q[-st]) / h2;
+      });
+    }
+  }
+  static void chk_sync(int rc)
+  {
+    detail::check(rc);
+    detail::check(te_stream_synchronize(nullptr));  // before the temporary boundary data is freed
+  }
   bool neumann_;
 };
 }  // namespace Poisson
 
 namespace VarPoisson {
-/* VarPoisson/StarPatchOperator.h:40-186; the coefficients' ghosts are filled once here (:70) */
+/* VarPoisson/StarPatchOperator.h:40-220; the coefficients' ghosts are filled once here (:70) */
 template <int D>
 class StarPatchOperator : public PatchOperator<D> {
  public:
@@ -508,19 +1158,193 @@ class StarPatchOperator : public PatchOperator<D> {
   StarPatchOperator* clone() const override { return new StarPatchOperator(*this); }
   void apply(const Vector<D>& x, Vector<D>& b) const override
   {
+    if (typeid(*this) != typeid(StarPatchOperator) || !this->hasDeviceFiller()) {
+      PatchOperator<D>::apply(x, b);
+      return;
+    }
     detail::check(te_varstar_apply(this->deviceFiller().handle(), TE_F64, coeffs_.data(), x.data(), b.data(),
                                    x.getNumComponents(), nullptr));
   }
   const Vector<D>& getCoefficients() const { return coeffs_; }
+  void applySinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view,
+                        const PatchView<double, D>& f_view) const override
+  {
+    applySinglePatch(pinfo, u_view, f_view, false);
+  }
+  void applySinglePatchWithInternalBoundaryConditions(const PatchInfo<D>& pinfo,
+                                                      const PatchView<const double, D>& u_view,
+                                                      const PatchView<double, D>& f_view) const override
+  {
+    applySinglePatch(pinfo, u_view, f_view, true);
+  }
+  /* physical ghosts -u (:132-150) */
+  void enforceBoundaryConditions(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view) const
+  {
+    detail::set_side_ghosts<D>(pinfo, u_view, false, -1.0);
+  }
+  void enforceInternalBoundaryConditions(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view) const
+  {
+    detail::set_side_ghosts<D>(pinfo, u_view, true, -1.0);
+  }
+  /* f_b -= (u_g + u_b)(c_b + c_g) / (2 h^2) on sides with a neighbour (:168-186) */
+  void modifyRHSForInternalBoundaryConditions(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view,
+                                              const PatchView<double, D>& f_view) const override
+  {
+    ComponentView<const double, D> c = coeffs_.getComponentView(0, pinfo.local_index);
+    for (Side<D> s : Side<D>::getValues()) {
+      if (!pinfo.hasNbr(s)) continue;
+      const double h2 = pinfo.spacings[s.getAxisIndex()] * pinfo.spacings[s.getAxisIndex()];
+      View<double, D> f_inner = f_view.getSliceOn(s, { 0 });
+      View<double, D> u_ghost = u_view.getGhostSliceOn(s, { 0 });
+      View<const double, D> u_inner = u_view.getSliceOn(s, { 0 });
+      View<const double, D - 1> c_ghost = c.getSliceOn(s, { -1 });
+      View<const double, D - 1> c_inner = c.getSliceOn(s, { 0 });
+      Loop::OverInteriorIndexes<D>(f_inner, [&](const std::array<int, D>& k) {
+        std::array<int, D - 1> kc;
+        for (int a = 0; a < D - 1; a++) kc[(size_t)a] = k[(size_t)a];
+        f_inner[k] -= (u_ghost[k] + u_inner[k]) * (c_inner[kc] + c_ghost[kc]) / (2 * h2);
+      });
+    }
+  }
+  /* addDrichletBCToRHS (VarPoisson/StarPatchOperator.h:194-220): on every side without a
+   * neighbour, f_b -= 2 g c / h^2 with g = gfunc and c = hfunc at the face point */
+  void addDrichletBCToRHS(Vector<D>& f, std::function<double(const std::array<double, D>&)> gfunc,
+                          std::function<double(const std::array<double, D>&)> hfunc) const
+  {
+    const detail::DeviceCopy g(
+      detail::boundary_samples<D>(this->getDomain(), [&](int, const std::array<double, D>& x) { return gfunc(x); }));
+    const detail::DeviceCopy c(
+      detail::boundary_samples<D>(this->getDomain(), [&](int, const std::array<double, D>& x) { return hfunc(x); }));
+    detail::check(te_star_add_dirichlet_bc(this->getDomain().handle(), TE_F64, f.data(), g.get(), c.get(), nullptr));
+    detail::check(te_stream_synchronize(nullptr));
+  }
   /* shared by this operator and its clones (each clone deep-copies the coefficients, so
    * their buffers differ while their values are the same) */
   const void* coefficientIdentity() const { return ident_.get(); }
 
  private:
+  /* :61-109: f = sum_a [(c+ + c)(u+ - u) - (c- + c)(u - u-)] / (2 h_a^2) */
+  void applySinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& u_view,
+                        const PatchView<double, D>& f_view, bool internal) const
+  {
+    if (internal) enforceInternalBoundaryConditions(pinfo, u_view);
+    enforceBoundaryConditions(pinfo, u_view);
+    ComponentView<const double, D> c = coeffs_.getComponentView(0, pinfo.local_index);
+    for (int a = 0; a < D; a++) {
+      const double h2 = pinfo.spacings[(size_t)a] * pinfo.spacings[(size_t)a];
+      const int st = u_view.getStrides()[(size_t)a], cst = c.getStrides()[(size_t)a];
+      Loop::OverInteriorIndexes<D + 1>(u_view, [&](const std::array<int, D + 1>& k) {
+        std::array<int, D> kc;
+        for (int b = 0; b < D; b++) kc[(size_t)b] = k[(size_t)b];
+        const double* q = &u_view[k];
+        const double* cq = &c[kc];
+        f_view[k] = (a == 0 ? 0.0 : 1.0) * f_view[k] +
+                    ((cq[cst] + cq[0]) * (q[st] - q[0]) - (cq[-cst] + cq[0]) * (q[0] - q[-st])) / (2 * h2);
+      });
+    }
+  }
   Vector<D> coeffs_;
   std::shared_ptr<const char> ident_{ std::make_shared<const char>(0) };
 };
 }  // namespace VarPoisson
+
+/* MPIGhostFiller<D> (MPIGhostFiller.h:44-1107) on one rank, for user-written fillers: the
+ * four per-neighbour virtuals on host views, called in the reference's order (zero the
+ * ghosts facing neighbours, the local fill of every patch, then per face dimension -- sides,
+ * edges, corners as the fill type includes them -- per patch, per face in value order, per
+ * neighbour: the patch pushes into that neighbour's ghosts, :1055-1092, :821-895). */
+template <int D>
+class MPIGhostFiller : public GhostFiller<D> {
+ public:
+  MPIGhostFiller(const Domain<D>& domain, GhostFillingType fill_type) : domain_(domain), fill_type_(fill_type) {}
+  void fillGhost(const Vector<D>& u) const override
+  {
+    const auto& infos = domain_.getPatchInfoVector();
+    for (const PatchInfo<D>& pinfo : infos) zero_ghosts(pinfo, u.getPatchView(pinfo.local_index));
+    for (const PatchInfo<D>& pinfo : infos) fillGhostCellsForLocalPatch(pinfo, u.getPatchView(pinfo.local_index));
+    calls<D - 1>(u);
+    if (D == 3 && fill_type_ != GhostFillingType::Faces) calls<(D == 3 ? 1 : 0)>(u);
+    if (fill_type_ == GhostFillingType::Corners) calls<0>(u);
+  }
+  GhostFillingType getFillType() const { return fill_type_; }
+  const Domain<D>& getDomain() const { return domain_; }
+
+ protected:
+  virtual void fillGhostCellsForNbrPatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& local_view,
+                                         const PatchView<const double, D>& nbr_view, Side<D> side, NbrType nbr_type,
+                                         Orthant<D - 1> orthant_on_coarse) const = 0;
+  virtual void fillGhostCellsForEdgeNbrPatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& local_view,
+                                             const PatchView<const double, D>& nbr_view, Edge edge, NbrType nbr_type,
+                                             Orthant<1> orthant_on_coarse) const = 0;
+  virtual void fillGhostCellsForCornerNbrPatch(const PatchInfo<D>& pinfo,
+                                               const PatchView<const double, D>& local_view,
+                                               const PatchView<const double, D>& nbr_view, Corner<D> corner,
+                                               NbrType nbr_type) const = 0;
+  virtual void fillGhostCellsForLocalPatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& view) const = 0;
+
+ private:
+  template <int M>
+  void zero_faces(const PatchInfo<D>& pinfo, const PatchView<const double, D>& v) const
+  {
+    for (Face<D, M> f : Face<D, M>::getValues()) {
+      if (!pinfo.hasNbr(f)) continue;
+      std::array<size_t, D - M> off{};
+      View<double, M + 1> g = v.getGhostSliceOn(f, off);
+      Loop::OverInteriorIndexes<M + 1>(g, [&](const std::array<int, M + 1>& c) { g[c] = 0.0; });
+    }
+  }
+  /* zeroGhostCells (:904-947) */
+  void zero_ghosts(const PatchInfo<D>& pinfo, const PatchView<const double, D>& v) const
+  {
+    if (fill_type_ == GhostFillingType::Corners) zero_faces<0>(pinfo, v);
+    if (D == 3 && fill_type_ != GhostFillingType::Faces) zero_faces<(D == 3 ? 1 : 0)>(pinfo, v);
+    zero_faces<D - 1>(pinfo, v);
+  }
+  template <int M>
+  void call(const PatchInfo<D>& pinfo, const PatchView<const double, D>& lv, const PatchView<const double, D>& nv,
+            Face<D, M> f, NbrType t, int orth) const
+  {
+    if constexpr (M == D - 1) {
+      fillGhostCellsForNbrPatch(pinfo, lv, nv, f, t, Orthant<D - 1>((unsigned char)orth));
+    } else if constexpr (M == 0) {
+      fillGhostCellsForCornerNbrPatch(pinfo, lv, nv, f, t);
+    } else {
+      fillGhostCellsForEdgeNbrPatch(pinfo, lv, nv, Edge((unsigned char)f.getIndex()), t,
+                                    Orthant<1>((unsigned char)orth));
+    }
+  }
+  /* the local calls of face dimension M (enumerateCalls, :821-895; processLocalFills,
+   * :704-717) */
+  template <int M>
+  void calls(const Vector<D>& u) const
+  {
+    for (const PatchInfo<D>& pinfo : domain_.getPatchInfoVector()) {
+      PatchView<const double, D> lv = u.getPatchView(pinfo.local_index);
+      for (Face<D, M> f : Face<D, M>::getValues()) {
+        if (!pinfo.hasNbr(f)) continue;
+        switch (pinfo.getNbrType(f)) {
+          case NbrType::Normal:
+            call<M>(pinfo, lv, u.getPatchView(pinfo.getNormalNbrInfo(f).local_index), f, NbrType::Normal, 0);
+            break;
+          case NbrType::Coarse: {
+            auto ni = pinfo.getCoarseNbrInfo(f);
+            call<M>(pinfo, lv, u.getPatchView(ni.local_index), f, NbrType::Coarse,
+                    (int)ni.orth_on_coarse.getIndex());
+            break;
+          }
+          case NbrType::Fine: {
+            auto ni = pinfo.getFineNbrInfo(f);
+            for (size_t k = 0; k < Orthant<M>::num_orthants; k++)
+              call<M>(pinfo, lv, u.getPatchView(ni.local_indexes[k]), f, NbrType::Fine, (int)k);
+            break;
+          }
+        }
+      }
+    }
+  }
+  Domain<D> domain_;
+  GhostFillingType fill_type_;
+};
 
 namespace GMG {
 /* GMG/Smoother.h, GMG/Restrictor.h, GMG/Interpolator.h */
@@ -547,16 +1371,94 @@ class Interpolator {
 };
 }  // namespace GMG
 
-/* PatchSolver.h:40-170: apply = smooth from u = 0 */
+namespace detail {
+/* PatchSolver::smooth's "Total Patch Smooth" domain timing (PatchSolver.h:153-168): by
+ * events on the null stream for the device launches, by the host clock for the per-patch
+ * virtuals; stop() ends it */
+template <int D>
+struct SmoothTiming {
+  SmoothTiming(const Domain<D>& d, bool host = false) : d_(d), host_(host)
+  {
+    if (!d_.hasTimer()) return;
+    if (host_) d_.getTimer()->startDomainTiming(d_.getId(), "Total Patch Smooth");
+    else d_.getTimer()->startOn("Total Patch Smooth", nullptr, d_.getId());
+  }
+  void stop()
+  {
+    if (!d_.hasTimer()) return;
+    if (host_) d_.getTimer()->stopDomainTiming(d_.getId(), "Total Patch Smooth");
+    else d_.getTimer()->stopOn("Total Patch Smooth", nullptr, d_.getId());
+  }
+  const Domain<D>& d_;
+  bool host_;
+};
+
+/* one padded patch of one component in host memory, with the views of Vector::getPatchView */
+template <int D>
+struct PatchBuf {
+  explicit PatchBuf(const std::array<int, D>& ns) : ns_(ns)
+  {
+    size_t n = 1;
+    for (int a = 0; a < D; a++) n *= (size_t)(ns[(size_t)a] + 2);
+    v.assign(n, 0.0);
+  }
+  PatchView<double, D> view()
+  {
+    std::array<int, D + 1> st, gs, s, e, ge;
+    long stride = 1, first = 0;
+    for (int a = 0; a < D; a++) {
+      st[(size_t)a] = (int)stride;
+      gs[(size_t)a] = -1;
+      s[(size_t)a] = 0;
+      e[(size_t)a] = ns_[(size_t)a] - 1;
+      ge[(size_t)a] = ns_[(size_t)a];
+      first += stride;
+      stride *= ns_[(size_t)a] + 2;
+    }
+    st[(size_t)D] = (int)stride;
+    gs[(size_t)D] = s[(size_t)D] = e[(size_t)D] = ge[(size_t)D] = 0;
+    return PatchView<double, D>(View<double, D + 1>(v.data() + first, st, gs, s, e, ge));
+  }
+  std::array<int, D> ns_;
+  std::vector<double> v;
+};
+
+/* the interior cells of a patch of n cells per axis, x fastest, with their running index */
+template <int D, class F>
+void each_cell(const std::array<int, D>& n, F fn)
+{
+  std::array<int, D> e;
+  for (int a = 0; a < D; a++) e[(size_t)a] = n[(size_t)a] - 1;
+  long i = 0;
+  Loop::Nested<D>(std::array<int, D>{}, e, [&](const std::array<int, D>& c) { fn(c, i++); });
+}
+}  // namespace detail
+
+/* PatchSolver.h:40-171.  solveSinglePatch takes host views of one patch; smooth() fills
+ * u's ghosts and solves every patch through it.  The built-in solvers override smooth()
+ * with their device kernels when their operator is a built-in one. */
 template <int D>
 class PatchSolver : public virtual Operator<D>, public virtual GMG::Smoother<D> {
  public:
   PatchSolver(const Domain<D>& domain, const GhostFiller<D>& filler) : domain_(domain), filler_(filler.clone()) {}
   PatchSolver* clone() const override = 0;
+  /* u = the patch solution for f, u's ghosts filled (PatchSolver.h:98) */
+  virtual void solveSinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& f_view,
+                                const PatchView<double, D>& u_view) const = 0;
+  /* PatchSolver::apply (:107-134): u := 0, then smooth */
   void apply(const Vector<D>& f, Vector<D>& u) const override
   {
     u.setWithGhost(0.0);
     this->smooth(f, u);
+  }
+  /* PatchSolver::smooth (:142-170) */
+  void smooth(const Vector<D>& f, Vector<D>& u) const override
+  {
+    detail::SmoothTiming<D> t(domain_, true);
+    filler_->fillGhost(u);
+    for (const PatchInfo<D>& pinfo : domain_.getPatchInfoVector())
+      solveSinglePatch(pinfo, f.getPatchView(pinfo.local_index), u.getPatchView(pinfo.local_index));
+    t.stop();
   }
   const Domain<D>& getDomain() const { return domain_; }
   const GhostFiller<D>& getGhostFiller() const { return *filler_; }
@@ -567,51 +1469,133 @@ class PatchSolver : public virtual Operator<D>, public virtual GMG::Smoother<D> 
 };
 
 namespace detail {
-/* PatchSolver::smooth's "Total Patch Smooth" domain timing (PatchSolver.h:153-168), by
- * events on the null stream the smoother's launches use; stop() ends it */
+/* DFTPatchSolver::getTransformArray (Poisson/DFTPatchSolver.h:131-196): row-major n x n,
+ * y_i = sum_j M[i][j] x_j; type 0 DCT-II, 1 DCT-III, 2 DCT-IV, 3 DST-II, 4 DST-III, 5 DST-IV */
+inline std::vector<double> dft_matrix(int type, int n)
+{
+  const double pi = 3.14159265358979323846;
+  std::vector<double> m((size_t)n * n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      double v;
+      switch (type) {
+        case 0: v = std::cos(pi / n * (i * (j + 0.5))); break;
+        case 1: v = j == 0 ? 0.5 : std::cos(pi / n * ((i + 0.5) * j)); break;
+        case 2: v = std::cos(pi / n * ((i + 0.5) * (j + 0.5))); break;
+        case 3: v = std::sin(pi / n * ((i + 1) * (j + 0.5))); break;
+        case 4: v = j == n - 1 ? ((i % 2 == 0) ? 0.5 : -0.5) : std::sin(pi / n * ((i + 0.5) * (j + 1))); break;
+        default: v = std::sin(pi / n * ((i + 0.5) * (j + 0.5))); break;
+      }
+      m[(size_t)i * n + j] = v;
+    }
+  return m;
+}
+
+/* the transform along axis a of an interior block (x fastest): y = M x on every line */
 template <int D>
-struct SmoothTiming {
-  explicit SmoothTiming(const Domain<D>& d) : d_(d)
-  {
-    if (d_.hasTimer()) d_.getTimer()->startOn("Total Patch Smooth", nullptr, d_.getId());
+void transform_axis(const std::array<int, D>& n, int a, const std::vector<double>& M, std::vector<double>& blk)
+{
+  long st = 1;
+  for (int b = 0; b < a; b++) st *= n[(size_t)b];
+  const int na = n[(size_t)a];
+  std::vector<double> line((size_t)na);
+  for (long base = 0; base < (long)blk.size(); base++) {
+    if ((base / st) % na != 0) continue;
+    for (int i = 0; i < na; i++) {
+      double acc = 0;
+      for (int j = 0; j < na; j++) acc += M[(size_t)i * na + j] * blk[(size_t)(base + j * st)];
+      line[(size_t)i] = acc;
+    }
+    for (int i = 0; i < na; i++) blk[(size_t)(base + i * st)] = line[(size_t)i];
   }
-  void stop()
-  {
-    if (d_.hasTimer()) d_.getTimer()->stopOn("Total Patch Smooth", nullptr, d_.getId());
-  }
-  const Domain<D>& d_;
-};
+}
 }  // namespace detail
 
 namespace Poisson {
-/* Poisson/DFTPatchSolver.h:369-450 */
+/* Poisson/DFTPatchSolver.h:47-451: on the device for the built-in operators; for any other
+ * PatchOperator the per-patch solve below (host views, the operator's own
+ * modifyRHSForInternalBoundaryConditions) */
 template <int D>
 class DFTPatchSolver : public PatchSolver<D> {
  public:
   DFTPatchSolver(const PatchOperator<D>& op, std::bitset<2 * D> neumann)
       : PatchSolver<D>(op.getDomain(), op.getGhostFiller()), op_(op.clone()), neumann_(neumann)
   {
-    int op_kind = 0;
+    int op_kind = -1;
     const double* coeffs = nullptr;  // copied by te_dft_plan_create
-    if (auto v = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(op_.get())) {
+    if (typeid(*op_) == typeid(VarPoisson::StarPatchOperator<D>)) {
       op_kind = 1;
-      coeffs = v->getCoefficients().data();
-    } else if (!dynamic_cast<const Poisson::StarPatchOperator<D>*>(op_.get())) {
-      throw RuntimeError("DFTPatchSolver: the operator must be a Poisson or VarPoisson StarPatchOperator");
+      coeffs = static_cast<const VarPoisson::StarPatchOperator<D>*>(op_.get())->getCoefficients().data();
+    } else if (typeid(*op_) == typeid(Poisson::StarPatchOperator<D>)) {
+      op_kind = 0;
     }
-    te_dft_plan* h = nullptr;
-    detail::check(te_dft_plan_create(op.deviceFiller().handle(), (unsigned)neumann.to_ulong(), op_kind, coeffs, &h));
-    h_ = std::shared_ptr<te_dft_plan>(h, detail::Deleter<te_dft_plan, te_dft_plan_destroy>());
+    if (op_kind >= 0 && op_->hasDeviceFiller()) {
+      te_dft_plan* h = nullptr;
+      detail::check(
+        te_dft_plan_create(op_->deviceFiller().handle(), (unsigned)neumann.to_ulong(), op_kind, coeffs, &h));
+      h_ = std::shared_ptr<te_dft_plan>(h, detail::Deleter<te_dft_plan, te_dft_plan_destroy>());
+    }
   }
   DFTPatchSolver* clone() const override { return new DFTPatchSolver(*this); }
   void smooth(const Vector<D>& f, Vector<D>& u) const override
   {
+    if (!h_) {
+      PatchSolver<D>::smooth(f, u);
+      return;
+    }
     detail::SmoothTiming<D> t(this->getDomain());
     detail::check(te_dft_smooth(h_.get(), TE_F64, f.data(), u.data(), nullptr));
     t.stop();
   }
+  /* solveSinglePatch (:420-450): f_copy = f with the interface terms, transforms along each
+   * axis, division by the eigenvalues, inverse transforms, scaled by prod 2/n */
+  void solveSinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& f_view,
+                        const PatchView<double, D>& u_view) const override
+  {
+    const auto& ns = this->getDomain().getNs();
+    detail::PatchBuf<D> fc(ns);
+    PatchView<double, D> fv = fc.view();
+    ComponentView<double, D> fcv = fv.getComponentView(0);
+    ComponentView<const double, D> fin = f_view.getComponentView(0);
+    Loop::OverInteriorIndexes<D>(fcv, [&](const std::array<int, D>& c) { fcv[c] = fin[c]; });
+    op_->modifyRHSForInternalBoundaryConditions(pinfo, u_view, fv);
+    std::array<int, D> n;
+    long N = 1;
+    for (int a = 0; a < D; a++) N *= (n[(size_t)a] = ns[(size_t)a]);
+    std::vector<double> blk((size_t)N);
+    detail::each_cell<D>(n, [&](const std::array<int, D>& c, long i) { blk[(size_t)i] = fcv[c]; });
+    std::array<int, D> tf, ti;
+    std::array<double, D> eshift;  // the eigenvalue index shift: 0 both Neumann, 1/2 one, 1 none
+    for (int a = 0; a < D; a++) {
+      // getTransformsForPatch / getInverseTransformsForPatch (:272-313)
+      const bool lo = !pinfo.hasNbr(Side<D>((unsigned char)(2 * a))) && neumann_[(size_t)(2 * a)];
+      const bool hi = !pinfo.hasNbr(Side<D>((unsigned char)(2 * a + 1))) && neumann_[(size_t)(2 * a + 1)];
+      tf[(size_t)a] = (lo && hi) ? 0 : lo ? 2 : hi ? 5 : 3;
+      ti[(size_t)a] = (lo && hi) ? 1 : lo ? 2 : hi ? 5 : 4;
+      eshift[(size_t)a] = (lo && hi) ? 0.0 : (lo || hi) ? 0.5 : 1.0;
+    }
+    for (int a = 0; a < D; a++) detail::transform_axis<D>(n, a, detail::dft_matrix(tf[(size_t)a], n[(size_t)a]), blk);
+    // getEigenValues (:320-361): -sum_a 4/h_a^2 sin^2((k + shift) pi / (2 n_a))
+    const double pi = 3.14159265358979323846;
+    detail::each_cell<D>(n, [&](const std::array<int, D>& c, long i) {
+      double eig = 0;
+      for (int a = 0; a < D; a++) {
+        const double h = pinfo.spacings[(size_t)a];
+        eig -= 4 / (h * h) * std::pow(std::sin((c[(size_t)a] + eshift[(size_t)a]) * pi / (2 * n[(size_t)a])), 2);
+      }
+      blk[(size_t)i] /= eig;
+    });
+    if (neumann_.all() && !pinfo.hasNbr()) blk[0] = 0;  // (:438-440)
+    for (int a = 0; a < D; a++) detail::transform_axis<D>(n, a, detail::dft_matrix(ti[(size_t)a], n[(size_t)a]), blk);
+    double scale = 1;
+    for (int a = 0; a < D; a++) scale *= 2.0 / n[(size_t)a];
+    ComponentView<double, D> uc = u_view.getComponentView(0);
+    detail::each_cell<D>(n, [&](const std::array<int, D>& c, long i) { uc[c] = blk[(size_t)i] * scale; });
+  }
   const PatchOperator<D>& getOperator() const { return *op_; }
   std::bitset<2 * D> getNeumann() const { return neumann_; }
+  /* the device plan, or NULL when the per-patch virtuals are used */
+  te_dft_plan* device() const { return h_.get(); }
 
  private:
   std::shared_ptr<const PatchOperator<D>> op_;
@@ -687,7 +1671,9 @@ class DirectInterpolator : public Interpolator<D> {
 }  // namespace GMG
 
 namespace Iterative {
-/* Iterative/PatchSolver.h:53-175: inner solver per patch (BiCGStab only) */
+/* Iterative/PatchSolver.h:53-175: an inner BiCGStab per patch (the solver must be an
+ * Iterative::BiCGStab); on the device for the built-in operators, else on host views
+ * through the operator's per-patch virtuals */
 template <int D>
 class BiCGStab;
 template <int D>
@@ -697,18 +1683,28 @@ class PatchSolver : public ThunderEgg::PatchSolver<D> {
   PatchSolver* clone() const override { return new PatchSolver(*this); }
   void smooth(const Vector<D>& f, Vector<D>& u) const override
   {
+    if (!device_) {
+      ThunderEgg::PatchSolver<D>::smooth(f, u);
+      return;
+    }
     ThunderEgg::detail::SmoothTiming<D> t(this->getDomain());
     detail::check(te_patch_bicgstab_smooth(op_->deviceFiller().handle(), TE_F64, op_kind_, neumann_, coeffs_,
                                            max_it_, tol_, continue_ ? 1 : 0, f.data(), u.data(), nullptr,
                                            nullptr));
     t.stop();
   }
+  /* solveSinglePatch (Iterative/PatchSolver.h:134-175): f' = f with the interface terms, then
+   * BiCGStab (Iterative/BiCGStab.h:111-203) on the patch's operator with internal boundary
+   * conditions, started from u */
+  void solveSinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& f_view,
+                        const PatchView<double, D>& u_view) const override;
   int maxIterations() const { return max_it_; }
   double tolerance() const { return tol_; }
   bool continueOnBreakdown() const { return continue_; }
 
  private:
   std::shared_ptr<const PatchOperator<D>> op_;
+  bool device_ = false;
   int op_kind_ = 0, neumann_ = 0;
   const double* coeffs_ = nullptr;
   int max_it_ = 1000;
@@ -767,14 +1763,93 @@ PatchSolver<D>::PatchSolver(const Solver<D>& solver, const PatchOperator<D>& op,
   max_it_ = bs->getMaxIterations();
   tol_ = bs->getTolerance();
   // the coefficients come from the owned clone: the caller's operator may go away first
-  if (auto v = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(op_.get())) {
+  if (typeid(*op_) == typeid(VarPoisson::StarPatchOperator<D>)) {
     op_kind_ = 1;
-    coeffs_ = v->getCoefficients().data();
-  } else if (auto p = dynamic_cast<const Poisson::StarPatchOperator<D>*>(op_.get())) {
-    neumann_ = p->neumann() ? 1 : 0;
-  } else {
-    throw RuntimeError("Iterative::PatchSolver: the operator must be a Poisson or VarPoisson StarPatchOperator");
+    coeffs_ = static_cast<const VarPoisson::StarPatchOperator<D>*>(op_.get())->getCoefficients().data();
+    device_ = op_->hasDeviceFiller();
+  } else if (typeid(*op_) == typeid(Poisson::StarPatchOperator<D>)) {
+    neumann_ = static_cast<const Poisson::StarPatchOperator<D>*>(op_.get())->neumann() ? 1 : 0;
+    device_ = op_->hasDeviceFiller();
   }
+}
+
+template <int D>
+void PatchSolver<D>::solveSinglePatch(const PatchInfo<D>& pinfo, const PatchView<const double, D>& f_view,
+                                      const PatchView<double, D>& u_view) const
+{
+  const auto& ns = this->getDomain().getNs();
+  std::array<int, D> n;
+  for (int a = 0; a < D; a++) n[(size_t)a] = ns[(size_t)a];
+  using Buf = ThunderEgg::detail::PatchBuf<D>;
+  Buf fb(ns), xb(ns), resid(ns), rhat(ns), p(ns), ap(ns), sv(ns), as(ns);
+  auto cv = [](Buf& b) { return b.view().getComponentView(0); };
+  // f_copy = f, modifyRHSForInternalBoundaryConditions (:160-163); x = u
+  {
+    ComponentView<double, D> f = cv(fb), x = cv(xb);
+    ComponentView<const double, D> fi = f_view.getComponentView(0);
+    ComponentView<double, D> ui = u_view.getComponentView(0);
+    ThunderEgg::detail::each_cell<D>(n, [&](const std::array<int, D>& c, long) {
+      f[c] = fi[c];
+      x[c] = ui[c];
+    });
+  }
+  op_->modifyRHSForInternalBoundaryConditions(pinfo, u_view, fb.view());
+  auto A = [&](Buf& in, Buf& out) {  // SinglePatchOp::apply (:76-81)
+    std::fill(out.v.begin(), out.v.end(), 0.0);
+    op_->applySinglePatchWithInternalBoundaryConditions(pinfo, in.view(), out.view());
+  };
+  auto dot = [&](Buf& a, Buf& b) {
+    double r = 0;
+    ComponentView<double, D> av = cv(a), bv = cv(b);
+    ThunderEgg::detail::each_cell<D>(n, [&](const std::array<int, D>& c, long) { r += av[c] * bv[c]; });
+    return r;
+  };
+  auto axpby = [&](Buf& y, double alpha, Buf& x, double beta, Buf* z, double gamma) {  // y = alpha y + beta x + gamma z
+    ComponentView<double, D> yv = cv(y), xv = cv(x);
+    ThunderEgg::detail::each_cell<D>(n, [&](const std::array<int, D>& c, long) {
+      double v = yv[c] * alpha + xv[c] * beta;
+      if (z) v += cv(*z)[c] * gamma;
+      yv[c] = v;
+    });
+  };
+  // Iterative::BiCGStab::solve (Iterative/BiCGStab.h:111-203), unpreconditioned
+  A(xb, resid);
+  axpby(resid, -1.0, fb, 1.0, nullptr, 0.0);
+  const double r0 = std::sqrt(dot(fb, fb));
+  rhat.v = resid.v;
+  p.v = resid.v;
+  double rho = dot(rhat, resid);
+  int its = 0;
+  double residual = std::sqrt(dot(resid, resid)) / r0;
+  if (r0 != 0) {
+    while (residual > tol_ && its < max_it_) {
+      if (rho == 0) {
+        if (continue_) break;
+        throw BreakdownError("BiCGStab broke down, rho was 0 on iteration " + std::to_string(its));
+      }
+      A(p, ap);
+      const double alpha = rho / dot(rhat, ap);
+      sv.v = resid.v;
+      axpby(sv, 1.0, ap, -alpha, nullptr, 0.0);
+      if (std::sqrt(dot(sv, sv)) / r0 <= tol_) {
+        axpby(xb, 1.0, p, alpha, nullptr, 0.0);
+        break;
+      }
+      A(sv, as);
+      const double omega = dot(as, sv) / dot(as, as);
+      axpby(xb, 1.0, p, alpha, &sv, omega);
+      axpby(resid, 1.0, ap, -alpha, &as, -omega);
+      const double rho_new = dot(resid, rhat);
+      const double beta = rho_new * alpha / (rho * omega);
+      axpby(p, 1.0, ap, -omega, nullptr, 0.0);
+      axpby(p, beta, resid, 1.0, nullptr, 0.0);
+      its++;
+      rho = rho_new;
+      residual = std::sqrt(dot(resid, resid)) / r0;
+    }
+  }
+  ComponentView<double, D> ui = u_view.getComponentView(0), x = cv(xb);
+  ThunderEgg::detail::each_cell<D>(n, [&](const std::array<int, D>& c, long) { ui[c] = x[c]; });
 }
 }  // namespace Iterative
 

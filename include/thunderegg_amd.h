@@ -136,6 +136,22 @@ int te_star_apply_patches(const te_ghost_filler* g, int dtype, int neumann, cons
  * constructor does (:70). */
 int te_varstar_apply(const te_ghost_filler* g, int dtype, const void* coeffs, void* u, void* f, int ncomp,
                      te_stream stream);
+/* Boundary data: one value per face cell of every side of every patch, for patch p (local
+ * order) and side s (Side<D> order: west, east, south, north, bottom, top) the side's face
+ * cells with the first of the other axes fastest -- the points DomainTools::GetRealCoordBound
+ * gives for that side (DomainTools.h:178-210).  Values on sides with a neighbour are not read.
+ * Length (doubles) for the whole domain: */
+int64_t te_domain_boundary_length(const te_domain* d);
+/* Poisson::StarPatchOperator::addDrichletBCToRHS (Poisson/StarPatchOperator.h:187-204):
+ * on every side without a neighbour f_b += -2 g / h^2; with c != NULL the VarPoisson form
+ * (VarPoisson/StarPatchOperator.h:194-220) f_b -= 2 g c / h^2.  g (Dirichlet values) and c
+ * (coefficient values) are device FP64 boundary data; f a device vector (component 0). */
+int te_star_add_dirichlet_bc(const te_domain* d, int dtype, void* f, const double* g, const double* c,
+                             te_stream stream);
+/* Poisson::StarPatchOperator::addNeumannBCToRHS (Poisson/StarPatchOperator.h:206-242): on
+ * every side without a neighbour f_b += dg / h on lower sides and f_b -= dg / h on upper
+ * sides; dg: device FP64 boundary data of the derivative of g along each side's axis. */
+int te_star_add_neumann_bc(const te_domain* d, int dtype, void* f, const double* dg, te_stream stream);
 /* Cycle::restrict residual (GMG/Cycle.h:57-65): fills ghosts of u, then r = f - A u.
  * op_kind 0 Poisson (neumann flag), 1 VarPoisson (coeffs). */
 int te_residual(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs, const void* f,
@@ -311,6 +327,21 @@ int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it
               double* history, int history_cap, te_stream stream);
 /* Seconds spent in the last te_gmg_bicgstab / te_gmg_cg, from the first A x to convergence. */
 double te_gmg_last_solve_seconds(const te_gmg* g);
+/* Measurement hook, no counterpart in the reference: the average duration in milliseconds
+ * of `reps` back-to-back launches of one kernel of the finest level's cycle or of the
+ * solver, each launched standalone on the solver's own work buffers and stream and timed
+ * with HIP events recorded on that stream (one untimed launch first).  The work buffers'
+ * contents are overwritten: call it between solves.  Supported where the cycle runs on
+ * compact vectors (3D levels of 16^3 patches); TE_ERR_UNSUPPORTED otherwise. */
+enum te_kernel_id {
+  TE_K_SWEEP_ZERO = 0,      /* patch-solve sweep from u = 0 (first pre-smoothing sweep) */
+  TE_K_SWEEP = 1,           /* sweep with the interface terms from u (later sweeps) */
+  TE_K_SWEEP_INTERP = 2,    /* sweep on u + P uc (interpolation fused, first post-smoothing) */
+  TE_K_RESID_RESTRICT = 3,  /* r = f - A u restricted to the coarser level */
+  TE_K_OPERATOR = 4,        /* outer FP64 A (M s) with the dots as.s and as.as */
+  TE_K_UPDATE_XR = 5        /* outer x and r updates with the dots r.rhat and r.r */
+};
+int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms);
 
 /* ------------------------------------------------------------- timings --
  * Timer (Timer.h:28-204, Timer.cpp:160-781): nested named timings, each optionally tied to

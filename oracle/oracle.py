@@ -52,6 +52,7 @@ def lib():
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [vp, dp, C.c_int]
         L.ora_dft_matrix.argtypes = [C.c_int, C.c_int, dp]
+        L.ora_add_bc.argtypes = [vp, dp, dp, dp, C.c_int]
         _LIB = L
     return _LIB
 
@@ -243,6 +244,16 @@ def inf_norm(dom, a):
 
 def integrate(dom, u):
     return lib().ora_vec_integrate(dom.h, _dp(u), u.shape[1])
+
+
+def add_bc(dom: Domain, f, g, c=None, neumann=False):
+    """StarPatchOperator::addDrichletBCToRHS (c: the VarPoisson coefficient samples) or
+    addNeumannBCToRHS (neumann=True, g: the normal-axis derivative samples) on f in place,
+    with boundary data sampled in the layout of thunderegg_amd.domain_tools.boundary_points."""
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    c = None if c is None else np.ascontiguousarray(c, dtype=np.float64)
+    _check(lib().ora_add_bc(dom.h, _dp(f), _dp(g), _dp(c) if c is not None else None, int(neumann)), "add_bc")
+    return f
 
 
 def dft_matrix(kind: int, n: int):

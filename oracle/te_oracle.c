@@ -876,6 +876,45 @@ int ora_varpoisson_apply(const ora_domain* d, int kind, int fill_type, const dou
   return op_apply(&o, u, f, ncomp);
 }
 
+/* Poisson::StarPatchOperator::addDrichletBCToRHS (Poisson/StarPatchOperator.h:187-204),
+ * VarPoisson::StarPatchOperator::addDrichletBCToRHS (VarPoisson/StarPatchOperator.h:194-220)
+ * and Poisson::StarPatchOperator::addNeumannBCToRHS (:206-242), with the functions sampled
+ * at the face points of every side (g: Dirichlet values or, neumann != 0, the derivative
+ * along the side's axis; c: the VarPoisson coefficient there, or NULL).  Sample layout:
+ * per patch, per side in order, the side's face cells with the first free axis fastest. */
+int ora_add_bc(const ora_domain* d, double* f, const double* g, const double* c, int neumann)
+{
+  int D = d->dim, M = D - 1;
+  long off = 0;
+  for (int p = 0; p < d->np; p++) {
+    for (int s = 0; s < 2 * D; s++) {
+      int fa[2];
+      int nfr = face_free_axes(D, M, s, fa);
+      int cnt = 1;
+      for (int k = 0; k < nfr; k++) cnt *= d->ns[fa[k]];
+      if (!has_nbr(d, p, side_slot(D, s))) {
+        double h = d->p[p].spacings[s >> 1];
+        double h2 = pow(h, 2);
+        for (int li = 0; li < cnt; li++) {
+          int fr[2] = { li % d->ns[fa[0]], nfr > 1 ? li / d->ns[fa[0]] : 0 };
+          double* fb = sl(d, f, 1, p, 0, M, s, 0, fr);
+          double gv = g[off + li];
+          if (neumann) {
+            if ((s & 1) == 0) *fb += gv / h;
+            else *fb -= gv / h;
+          } else if (c) {
+            *fb -= 2 * gv * c[off + li] / h2;
+          } else {
+            *fb += -2.0 * gv / h2;
+          }
+        }
+      }
+      off += cnt;
+    }
+  }
+  return 0;
+}
+
 /* ---------------------------------------------------------- vector ops -- */
 typedef struct {
   const ora_domain* d;

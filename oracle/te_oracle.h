@@ -129,6 +129,8 @@ double ora_vec_two_norm(const ora_domain* d, const double* a, int ncomp);
 double ora_vec_inf_norm(const ora_domain* d, const double* a, int ncomp);
 /* DomainTools::Integrate (DomainTools.h:513-536). */
 double ora_vec_integrate(const ora_domain* d, const double* u, int ncomp);
+/* StarPatchOperator::addDrichletBCToRHS / addNeumannBCToRHS with sampled boundary data */
+int ora_add_bc(const ora_domain* d, double* f, const double* g, const double* c, int neumann);
 
 /* DFTPatchSolver transform matrices (Poisson/DFTPatchSolver.h:131-196): type 0 DCT_II,
  * 1 DCT_III, 2 DCT_IV, 3 DST_II, 4 DST_III, 5 DST_IV; out[i*n+j] row-major. */

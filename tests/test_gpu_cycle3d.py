@@ -1,0 +1,151 @@
+"""GPU parity of the compact 3D path (uniform octree levels of 16^3 patches: the V-cycle and
+the outer BiCGStab on interior-only work vectors, ghost values computed in the launches)
+against the oracle and against the one-launch-per-step cycle, through the C ABI.
+
+FP64: max-norm relative difference <= 1e-12 against the oracle's cycle; FP32 cycle (FP64
+coarsest level): <= 2e-5 (the patch solves' FP32 matrix-core transforms)."""
+import numpy as np
+import pytest
+
+from helpers import load, random_field, relerr
+from oracle import oracle as O
+from thunderegg_amd import mesh
+from thunderegg_amd import runtime as R
+from thunderegg_amd import solver
+
+pytestmark = pytest.mark.gpu
+
+NS = (16, 16, 16)
+
+
+def _print_case(*a):
+    print("case:", *a, flush=True)
+
+
+def _levels(base):
+    return solver.uniform_levels(3, base, NS)
+
+
+def _rhs_sin(levels):
+    X = mesh.cell_centers(levels[0], 3, NS)
+    k = 2 * np.pi
+    f = -3 * k * k * np.sin(k * X[..., 0]) * np.sin(k * X[..., 1]) * np.sin(k * X[..., 2])
+    return f[:, None].copy()
+
+
+@pytest.mark.parametrize("prec,tol", [(R.F64, 1e-12), (R.F32, 2e-5)])
+@pytest.mark.parametrize("sweeps", [(1, 1, 1), (2, 2, 1), (0, 1, 2), (1, 0, 1), (2, 1, 0)])
+def test_compact3d_vcycle_matches_oracle(prec, tol, sweeps):
+    pre, post, coarse = sweeps
+    _print_case("compact3d vcycle", prec, sweeps)
+    levels = _levels(2)
+    assert [len(lv) for lv in levels] == [64, 8, 1]
+    f = random_field(mesh.padded_shape(64, 3, NS), 91)
+    od = [O.Domain(3, NS, lv) for lv in levels]
+    ol = O.Levels(od, filler_kind=R.TRILINEAR, smoother_kind=0)
+    vref = ol.vcycle(f.copy(), pre=pre, post=post, coarse=coarse)
+    for graph in (False, True):
+        g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=prec, pre_sweeps=pre,
+                       post_sweeps=post, coarse_sweeps=coarse, use_graph=graph)
+        v = g.vcycle(f)
+        assert relerr(mesh.interior(v, 3), mesh.interior(vref, 3)) <= tol, graph
+        v2 = g.vcycle(f)  # replay (a cached graph with use_graph)
+        assert np.array_equal(mesh.interior(v2, 3), mesh.interior(v, 3))
+
+
+@pytest.mark.parametrize("prec,tol", [(R.F64, 1e-12), (R.F32, 2e-5)])
+def test_compact3d_neumann_vcycle_matches_oracle(prec, tol):
+    """Neumann physical sides: the operator's ghosts are +u there and the patch solves use
+    the DCT / DST-IV transforms on those sides (and drop the zero mode of the one patch of the
+    coarsest level, which has no neighbours)."""
+    _print_case("compact3d neumann vcycle", prec)
+    levels = _levels(2)
+    f = random_field(mesh.padded_shape(64, 3, NS), 93)
+    od = [O.Domain(3, NS, lv) for lv in levels]
+    for mask in (0b111111, 0b000101):
+        ol = O.Levels(od, filler_kind=R.TRILINEAR, smoother_kind=0, neumann=True, neumann_mask=mask)
+        vref = ol.vcycle(f.copy())
+        g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=prec, neumann=True,
+                       neumann_mask=mask)
+        v = g.vcycle(f)
+        assert relerr(mesh.interior(v, 3), mesh.interior(vref, 3)) <= tol, mask
+
+
+def test_compact3d_bicgstab_matches_oracle():
+    """FP64: the oracle's iteration count and residual history, both early-exit variants;
+    mixed precision: the tolerance in the true residual and the FP64 solution."""
+    _print_case("compact3d bicgstab")
+    levels = _levels(2)
+    f = _rhs_sin(levels)
+    od = [O.Domain(3, NS, lv) for lv in levels]
+    ol = O.Levels(od, filler_kind=R.TRILINEAR, smoother_kind=0)
+    g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=R.F64, pre_sweeps=2, post_sweeps=2)
+    for precond in (True, 2):
+        xref, its_ref, hist_ref = ol.bicgstab(f.copy(), tol=1e-10, precond=precond, pre=2, post=2)
+        x, its, hist = g.solve(f, tol=1e-10, precond=precond)
+        assert its == its_ref, precond
+        assert np.allclose(hist, hist_ref, rtol=1e-6, atol=0)
+        assert relerr(mesh.interior(x, 3), mesh.interior(xref, 3)) <= 1e-10
+    x2, _, _ = g.solve(f, tol=1e-10, precond=2)
+    x0, its0, _ = g.solve(f, tol=1e-10, precond=False, max_it=400)
+    xr0, itsr0, _ = ol.bicgstab(f.copy(), tol=1e-10, precond=False, max_it=400)
+    # unpreconditioned: a long, rounding-sensitive iteration (sums in a different order)
+    assert abs(its0 - itsr0) <= 2 and relerr(mesh.interior(x0, 3), mesh.interior(xr0, 3)) <= 1e-6
+
+    def true_resid(xs):
+        ax = O.poisson_apply(od[0], xs.copy(), kind=R.TRILINEAR)
+        r = mesh.interior(f - ax, 3)
+        return np.sqrt((r * r).sum()) / np.sqrt((mesh.interior(f, 3) ** 2).sum())
+
+    assert true_resid(x2) <= 1.5e-10
+    gm = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=R.F32, pre_sweeps=2,
+                    post_sweeps=2)
+    xm, itm, _ = gm.solve(f, tol=1e-10, precond=2)
+    assert true_resid(xm) <= 1.5e-10
+    assert relerr(mesh.interior(xm, 3), mesh.interior(x2, 3)) <= 1e-8
+
+
+@pytest.mark.parametrize("prec", [R.F64, R.F32])
+def test_compact3d_matches_unfused(monkeypatch, prec):
+    """The compact path and the one-launch-per-step cycle (TE_NO_FUSE=1) on a 4-level
+    hierarchy (512 patches): the same cycle to rounding, and FP64 solves with the same
+    iteration count."""
+    _print_case("compact3d vs unfused", prec)
+    levels = _levels(3)
+    assert [len(lv) for lv in levels] == [512, 64, 8, 1]
+    f = random_field(mesh.padded_shape(512, 3, NS), 95)
+    b = _rhs_sin(levels)
+    kw = dict(filler_kind=R.TRILINEAR, smoother=0, precision=prec, pre_sweeps=2, post_sweeps=2)
+    gc = solver.GMG(3, NS, levels, **kw)
+    vc = gc.vcycle(f)
+    xc, itc, _ = gc.solve(b, tol=1e-10, precond=2)
+    monkeypatch.setenv("TE_NO_FUSE", "1")
+    gu = solver.GMG(3, NS, levels, **kw)
+    vu = gu.vcycle(f)
+    xu, itu, _ = gu.solve(b, tol=1e-10, precond=2)
+    tol = 1e-12 if prec == R.F64 else 2e-5
+    assert relerr(mesh.interior(vc, 3), mesh.interior(vu, 3)) <= tol
+    if prec == R.F64:
+        assert itc == itu
+        assert relerr(mesh.interior(xc, 3), mesh.interior(xu, 3)) <= 1e-9
+    else:
+        assert abs(itc - itu) <= 1
+        assert relerr(mesh.interior(xc, 3), mesh.interior(xu, 3)) <= 1e-8
+
+
+def test_compact3d_fixture_mesh_and_cg():
+    """A reference fixture mesh (3d_uniform_2x2x2: 8 patches, two levels) through the compact
+    path, and the CG driver on top of the compact cycle."""
+    _print_case("compact3d fixture + cg")
+    dim, levels = load("3d_uniform_2x2x2_mpi1.json", NS)
+    f = random_field(mesh.padded_shape(len(levels[0]), 3, NS), 97)
+    od = [O.Domain(3, NS, lv) for lv in levels]
+    ol = O.Levels(od, filler_kind=R.TRILINEAR, smoother_kind=0)
+    vref = ol.vcycle(f.copy())
+    g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=R.F64)
+    assert relerr(mesh.interior(g.vcycle(f), 3), mesh.interior(vref, 3)) <= 1e-12
+    b = _rhs_sin(levels)
+    xr, itr, _ = ol.cg(b.copy(), tol=1e-10)
+    x, it, _ = g.cg(b, tol=1e-10)
+    assert it == itr
+    assert relerr(mesh.interior(x, 3), mesh.interior(xr, 3)) <= 1e-8

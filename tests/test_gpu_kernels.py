@@ -361,3 +361,58 @@ def test_vector_algebra(dtype):
     R.check(L.te_vec_set(dd.h, dtype, x.ptr, 0.0, 2, None))
     got = x.get()
     assert np.all(mesh.interior(got, 2) == 0.0) and got.max() == 1.5
+
+
+@pytest.mark.parametrize("name,dim,ns", [("2d_uniform_2x2_refined_nw_mpi1.json", 2, (10, 6)),
+                                         ("2d_uniform_8x8_refined_cross_mpi1.json", 2, (16, 16)),
+                                         ("3d_refined_bnw_2x2x2_mpi1.json", 3, (4, 6, 8))])
+def test_boundary_data_rhs(name, dim, ns):
+    """te_star_add_dirichlet_bc (Poisson and VarPoisson forms) and te_star_add_neumann_bc
+    against the oracle on the same sampled boundary data (FP64 <= 1e-14, FP32 storage
+    <= 1e-6), and the reference's linear-exactness identity on the device operator: for a
+    linear g, A g (physical ghosts -g_b, or +g_b with Neumann) equals the boundary terms."""
+    from thunderegg_amd import domain_tools as DT
+
+    _print_case("boundary data", name, ns)
+    _, levels = load(name, ns)
+    lv = levels[0]
+    od = O.Domain(dim, ns, lv)
+    dd = _dev(dim, ns, lv)
+    pts = DT.boundary_points(lv, dim, ns)
+    assert len(pts) == dd.boundary_length()
+    rng = np.random.default_rng(3)
+    g = rng.standard_normal(len(pts))
+    cval = rng.uniform(0.5, 2.0, len(pts))
+    dg, dc = R.DeviceArray.from_host(g), R.DeviceArray.from_host(cval)
+    for dtype, tol in [(R.F64, 1e-14), (R.F32, 1e-6)]:
+        f0 = random_field(od.shape(1), 21, R.np_dtype(dtype))
+        for form in ("dirichlet", "varpoisson", "neumann"):
+            ref = f0.astype(np.float64)
+            df = R.DeviceArray.from_host(f0)
+            if form == "dirichlet":
+                O.add_bc(od, ref, g)
+                dd.add_dirichlet_bc(df, dg)
+            elif form == "varpoisson":
+                O.add_bc(od, ref, g, c=cval)
+                dd.add_dirichlet_bc(df, dg, dc)
+            else:
+                O.add_bc(od, ref, g, neumann=True)
+                dd.add_neumann_bc(df, dg)
+            assert relerr(df.get(), ref) <= tol, (dtype, form)
+    # linear exactness through the device operator
+    kind = R.TRILINEAR if dim == 3 else R.BILINEAR
+    coef = [5.0, 13.0, 7.0][:dim]
+    Xg = mesh.cell_centers(lv, dim, ns, with_ghost=True)
+    gv = (1 + sum(coef[a] * Xg[..., a] for a in range(dim)))[:, None].copy()
+    filler = R.GhostFiller(dd, kind, R.FACES)
+    for neumann in (False, True):
+        du = R.DeviceArray.from_host(gv)
+        df = R.DeviceArray(dd.shape(1), np.float64)
+        filler.star_apply(du, df, neumann=neumann)
+        fe = R.DeviceArray.from_host(np.zeros_like(gv))
+        if neumann:
+            DT.add_neumann_bc(dd, fe, [(lambda c, k=k: np.full_like(c[0], coef[k])) for k in range(dim)])
+        else:
+            DT.add_dirichlet_bc(dd, fe, lambda c: 1 + sum(coef[a] * c[a] for a in range(dim)))
+        a, b = mesh.interior(df.get(), dim), mesh.interior(fe.get(), dim)
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max(), neumann

@@ -735,3 +735,124 @@ def test_vector_reductions():
     assert O.inf_norm(d, neg_g) == 0.0
     mesh.interior(neg_g, dim)[1, 1, 2, 3] = 0.25
     assert O.inf_norm(d, neg_g) == 0.25
+
+
+# ---- boundary data on the right-hand side (addDrichletBCToRHS / addNeumannBCToRHS) --------
+MESHES_BC = ["2d_uniform_2x2_mpi1.json", "2d_uniform_4x4_mpi1.json", "2d_uniform_2x2_refined_nw_mpi1.json",
+             "2d_uniform_8x8_refined_cross_mpi1.json"]
+
+
+def _bc_samples(lv, dim, ns, func):
+    from thunderegg_amd.domain_tools import boundary_points
+
+    pts = boundary_points(lv, dim, ns)
+    return np.ascontiguousarray(func(*[pts[:, a] for a in range(dim)]), dtype=np.float64), pts
+
+
+def _side_axis(lv, dim, ns):
+    """the axis of the side of every boundary-data entry"""
+    per = np.concatenate([np.full(int(np.prod([ns[b] for b in range(dim) if b != (s >> 1)])), s >> 1)
+                          for s in range(2 * dim)])
+    return np.tile(per, len(lv))
+
+
+@pytest.mark.parametrize("name", MESHES_BC)
+@pytest.mark.parametrize("ns", [(2, 2), (2, 10), (10, 2), (10, 10)])
+@pytest.mark.parametrize("neumann", [False, True])
+def test_bc_rhs_linear_exactness_2d(name, ns, neumann):
+    """test/Poisson/StarPatchOperator_MPI1.cpp, 'apply on linear lhs constant coeff' (and its
+    Neumann twin): for g = 1 + 5x + 13y the operator with the physical ghosts -g_b (+g_b)
+    gives exactly the boundary terms, 0 + addDrichletBCToRHS(g) (addNeumannBCToRHS(g,
+    grad g)), at every cell -- interfaces, refined meshes and corners included."""
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    Xg = mesh.cell_centers(lv, dim, ns, with_ghost=True)
+    g = (1 + 5 * Xg[..., 0] + 13 * Xg[..., 1])[:, None].copy()
+    f = O.poisson_apply(d, g.copy(), kind=BIL, neumann=neumann)
+    fexp = np.zeros_like(g)
+    if neumann:
+        axis = _side_axis(lv, dim, ns)
+        O.add_bc(d, fexp, np.where(axis == 0, 5.0, 13.0), neumann=True)
+    else:
+        gs, _ = _bc_samples(lv, dim, ns, lambda x, y: 1 + 5 * x + 13 * y)
+        O.add_bc(d, fexp, gs)
+    a, b = mesh.interior(f, dim), mesh.interior(fexp, dim)
+    assert np.abs(b).max() > 0
+    assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("name", MESH_3D)
+@pytest.mark.parametrize("neumann", [False, True])
+def test_bc_rhs_linear_exactness_3d(name, neumann):
+    """The same identity in 3D (TriLinear ghosts, 7-point operator), g = 1 + 5x + 13y + 7z."""
+    ns = (4, 6, 8)
+    dim, levels = load(name, ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    Xg = mesh.cell_centers(lv, dim, ns, with_ghost=True)
+    g = (1 + 5 * Xg[..., 0] + 13 * Xg[..., 1] + 7 * Xg[..., 2])[:, None].copy()
+    f = O.poisson_apply(d, g.copy(), kind=TRI, neumann=neumann)
+    fexp = np.zeros_like(g)
+    if neumann:
+        axis = _side_axis(lv, dim, ns)
+        O.add_bc(d, fexp, np.choose(axis, [5.0, 13.0, 7.0]), neumann=True)
+    else:
+        gs, _ = _bc_samples(lv, dim, ns, lambda x, y, z: 1 + 5 * x + 13 * y + 7 * z)
+        O.add_bc(d, fexp, gs)
+    a, b = mesh.interior(f, dim), mesh.interior(fexp, dim)
+    assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max()
+
+
+def test_bc_rhs_varpoisson_constant_coefficient():
+    """The VarPoisson form -2 g c / h^2 with c = 1 is the Poisson form, and it scales with c."""
+    ns = (8, 8)
+    dim, levels = load("2d_uniform_2x2_refined_nw_mpi1.json", ns)
+    lv = levels[0]
+    d = O.Domain(dim, ns, lv)
+    gs, _ = _bc_samples(lv, dim, ns, lambda x, y: np.sin(3 * x) + y * y)
+    f0 = random_field(mesh.padded_shape(len(lv), dim, ns), 5)
+    a, b, c = f0.copy(), f0.copy(), f0.copy()
+    O.add_bc(d, a, gs)
+    O.add_bc(d, b, gs, c=np.ones_like(gs))
+    O.add_bc(d, c, gs, c=np.full_like(gs, 2.5))
+    assert np.allclose(a, b, rtol=1e-15, atol=0)
+    assert np.allclose(c - f0, 2.5 * (a - f0), rtol=1e-13, atol=1e-12)
+    # interior cells change (on physical sides), ghost cells do not
+    changed = np.abs(a - f0) > 0
+    inner = np.zeros(a.shape, dtype=bool)
+    mesh.interior(inner, dim)[...] = True
+    assert changed.any() and not changed[~inner].any()
+
+
+@pytest.mark.parametrize("name", MESHES_BC)
+@pytest.mark.parametrize("neumann", [False, True])
+def test_bc_rhs_second_order(name, neumann):
+    """test/Poisson/StarPatchOperator_MPI1.cpp, 'gets 2nd order convergence' (with Neumann
+    BC, BiQuadratic ghosts): A g against f + the boundary terms for g = sin(pi y) cos(2 pi x),
+    f = -5 pi^2 g, at n = 32 and 64 -- order > 1.8."""
+    errs = []
+    for n in (32, 64):
+        ns = (n, n)
+        dim, levels = load(name, ns)
+        lv = levels[0]
+        d = O.Domain(dim, ns, lv)
+        X = mesh.cell_centers(lv, dim, ns)
+        x, y = X[..., 0], X[..., 1]
+        g = (np.sin(np.pi * y) * np.cos(2 * np.pi * x))[:, None]
+        fexp = (-5 * np.pi ** 2 * np.sin(np.pi * y) * np.cos(2 * np.pi * x))[:, None].copy()
+        if neumann:
+            axis = _side_axis(lv, dim, ns)
+            _, pts = _bc_samples(lv, dim, ns, lambda x, y: x)
+            px, py = pts[:, 0], pts[:, 1]
+            dg = np.where(axis == 0, -2 * np.pi * np.sin(np.pi * py) * np.sin(2 * np.pi * px),
+                          np.pi * np.cos(np.pi * py) * np.cos(2 * np.pi * px))
+            O.add_bc(d, fexp, dg, neumann=True)
+            f = O.poisson_apply(d, g.copy(), kind=BIQ, neumann=True)
+        else:
+            gs, _ = _bc_samples(lv, dim, ns, lambda x, y: np.sin(np.pi * y) * np.cos(2 * np.pi * x))
+            O.add_bc(d, fexp, gs)
+            f = O.poisson_apply(d, g.copy(), kind=BIL)
+        e = mesh.interior(f - fexp, dim)
+        errs.append(np.sqrt((e * e).sum() / (mesh.interior(fexp, dim) ** 2).sum()))
+    assert np.log(errs[0] / errs[1]) / np.log(2) > 1.8

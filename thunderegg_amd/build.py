@@ -29,6 +29,8 @@ SOURCES = [
     "kernels/tail2d.hip",
     "kernels/patch_bicgstab.hip",
     "kernels/krylov.hip",
+    "kernels/cycle3d.hip",
+    "kernels/bc.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -428,6 +428,22 @@ int te_varstar_apply(const te_ghost_filler* g, int dtype, const void* coeffs, vo
   return launch_star(g, dtype, dtype, dtype, 1, 0, coeffs, nullptr, u, f, ncomp, S(s));
 }
 
+int64_t te_domain_boundary_length(const te_domain* d) { return d ? (int64_t)d->np * boundary_stride(d) : -1; }
+
+int te_star_add_dirichlet_bc(const te_domain* d, int dtype, void* f, const double* g, const double* c, te_stream s)
+{
+  TE_ARG(d && f && g, "bad argument");
+  TE_ARG(dtype == TE_F64 || dtype == TE_F32, "bad dtype");
+  return launch_bc_rhs(d, dtype, f, g, c, false, S(s));
+}
+
+int te_star_add_neumann_bc(const te_domain* d, int dtype, void* f, const double* dg, te_stream s)
+{
+  TE_ARG(d && f && dg, "bad argument");
+  TE_ARG(dtype == TE_F64 || dtype == TE_F32, "bad dtype");
+  return launch_bc_rhs(d, dtype, f, dg, nullptr, true, S(s));
+}
+
 int te_residual(const te_ghost_filler* g, int dtype, int op_kind, int neumann, const void* coeffs, const void* f,
                 void* u, void* r, int ncomp, te_stream s)
 {

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "kernels/blas.h"
+#include "kernels/cycle3d.h"
 #include "kernels/dft.h"
 #include "kernels/krylov.h"
 #include "kernels/tail2d.h"
@@ -132,6 +133,22 @@ struct te_gmg {
   bool bsweep = false;
   // stream ordering with the caller's stream (no host blocking)
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // 3D hierarchies of 16^3 patches whose face neighbours are all on the same level: the
+  // cycle and the BiCGStab work on compact vectors (interior cells only, cycle3d.h)
+  bool c3 = false;
+  struct C3Level {
+    void* f = nullptr;
+    void* u = nullptr;
+    void* u2 = nullptr;
+    void* cur = nullptr;  // the buffer holding this level's solution after its visit
+  };
+  std::vector<C3Level> C3;
+  double *c_r = nullptr, *c_rhat = nullptr, *c_p = nullptr, *c_ap = nullptr, *c_as = nullptr, *c_s = nullptr;
+  void* c_mp = nullptr;  // M p, M s: the finest level's type (FP32 in mixed precision)
+  void* c_ms = nullptr;
+  float* c_p32 = nullptr;  // FP32 copies of p and s (mixed precision: the cycle's input)
+  float* c_s32 = nullptr;
+  void* c_tmp = nullptr;   // the cycle's output for padded callers
 };
 
 namespace {
@@ -340,6 +357,84 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
   return TE_OK;
 }
 
+// VCycle::visit (GMG/VCycle.h:46-70) on compact vectors (3D, cycle3d.h): per level one launch
+// per smoothing sweep and one for Cycle::restrict (residual and restriction); the
+// interpolation and the ghost values of every sweep are computed inside the sweeps'
+// launches.  Sweeps after the first read the solution from one buffer and write the other;
+// the finest level's last sweep writes u_ext (compact, type ext_dt).
+int visit_c3(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_dt)
+{
+  GLevel& Lv = g->L[li];
+  te_gmg::C3Level& Cl = g->C3[li];
+  const int nl = (int)g->L.size();
+  const bool finest = li == 0;
+  const long ncell = (long)Lv.d->np * kC3Cells;
+  void* buf[2] = { Cl.u, Cl.u2 };
+  int cur = 0;
+  auto sweep = [&](int mode, bool last_of_level) -> int {
+    const bool to_ext = finest && last_of_level;
+    void* out = to_ext ? u_ext : buf[cur ^ 1];
+    const int dto = to_ext ? ext_dt : Lv.dt;
+    const void* uc = mode == 2 ? g->C3[li + 1].cur : nullptr;
+    const int dtc = mode == 2 ? g->L[li + 1].dt : TE_F64;
+    TE_RC(sweep_timing(g, li, false));
+    TE_RC(launch_c3_sweep(Lv.plan, Lv.dt, dtf, f, buf[cur], dto, out, mode, dtc, uc, mode == 2 ? Lv.link : nullptr,
+                          g->s));
+    TE_RC(sweep_timing(g, li, true));
+    if (to_ext) {
+      Cl.cur = u_ext;
+    } else {
+      cur ^= 1;
+      Cl.cur = buf[cur];
+    }
+    return TE_OK;
+  };
+  auto zero_level = [&]() -> int {
+    TE_RC(launch_c3_zero(ncell, Lv.dt, buf[0], g->s));
+    cur = 0;
+    Cl.cur = buf[0];
+    return TE_OK;
+  };
+  if (li == nl - 1) {
+    const int n = g->o.coarse_sweeps;
+    if (n == 0) {
+      TE_RC(zero_level());
+      if (finest) TE_RC(launch_convert(ncell, ext_dt, u_ext, Lv.dt, buf[0], g->s));
+      return TE_OK;
+    }
+    for (int k = 0; k < n; k++) TE_RC(sweep(k == 0 ? 0 : 1, k == n - 1));
+    return TE_OK;
+  }
+  for (int k = 0; k < g->o.pre_sweeps; k++) TE_RC(sweep(k == 0 ? 0 : 1, false));
+  if (g->o.pre_sweeps == 0) TE_RC(zero_level());
+  GLevel& Cv = g->L[li + 1];
+  TE_RC(launch_c3_resid_restrict(Lv.d, Lv.link, g->o.neumann, Lv.dt, dtf, f, Cl.cur, Cv.dt, g->C3[li + 1].f, g->s));
+  TE_RC(visit_c3(g, li + 1, g->C3[li + 1].f, Cv.dt, u_ext, ext_dt));
+  const int n = g->o.post_sweeps;
+  if (n == 0) {
+    // DirectInterpolator alone (GMG/DirectInterpolator.h:60-91)
+    TE_RC(launch_c3_interp(Lv.link, Cv.dt, g->C3[li + 1].cur, Lv.dt, Cl.cur, buf[cur ^ 1], g->s));
+    cur ^= 1;
+    Cl.cur = buf[cur];
+    if (finest) {
+      TE_RC(launch_convert(ncell, ext_dt, u_ext, Lv.dt, buf[cur], g->s));
+      Cl.cur = u_ext;
+    }
+    return TE_OK;
+  }
+  for (int k = 0; k < n; k++) TE_RC(sweep(k == 0 ? 2 : 1, k == n - 1));
+  return TE_OK;
+}
+
+// Cycle::apply on the caller's padded FP64 vectors through the compact cycle
+int vcycle_c3_padded(te_gmg* g, const double* f, double* u)
+{
+  const GLevel& L0 = g->L[0];
+  TE_RC(launch_c3_pack(L0.d, L0.dt, g->C3[0].f, TE_F64, f, g->s));
+  TE_RC(visit_c3(g, 0, g->C3[0].f, L0.dt, g->c_tmp, L0.dt));
+  return launch_c3_unpack(L0.d, TE_F64, u, L0.dt, g->c_tmp, g->s);
+}
+
 // Cycle::apply (GMG/Cycle.h:94-98) enqueued on g->s.  dtf / udt: types of f and u (FP32
 // only on the fused mixed-precision path, where the V-cycle rounds f to FP32 as it reads it
 // and its last sweep computes u in FP32; see visit_fused).
@@ -349,6 +444,13 @@ int vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt = TE_F64, int dtf 
 {
   GLevel& L0 = g->L[0];
   if (g->fused) return visit_fused(g, 0, f, dtf, u, udt, l0_done);
+  if (g->c3) {
+    if (udt != TE_F64 || dtf != TE_F64) {
+      set_error("internal: the padded entry of the 3D compact cycle takes FP64 vectors");
+      return TE_ERR_UNSUPPORTED;
+    }
+    return vcycle_c3_padded(g, (const double*)f, (double*)u);
+  }
   if (udt != TE_F64 || dtf != TE_F64) {
     set_error("internal: FP32 V-cycle input or output needs the fused path");
     return TE_ERR_UNSUPPORTED;
@@ -456,6 +558,14 @@ void free_gmg(te_gmg* g)
   (void)hipFree(g->p32);
   (void)hipFree(g->s32);
   (void)hipFree(g->d_flags);
+  for (auto& c : g->C3) {
+    (void)hipFree(c.f);
+    (void)hipFree(c.u);
+    (void)hipFree(c.u2);
+  }
+  for (void* v : { (void*)g->c_r, (void*)g->c_rhat, (void*)g->c_p, (void*)g->c_ap, (void*)g->c_as, (void*)g->c_s,
+                   g->c_mp, g->c_ms, (void*)g->c_p32, (void*)g->c_s32, g->c_tmp })
+    (void)hipFree(v);
   if (g->host_scal) (void)hipHostFree(g->host_scal);
   if (g->mirror_h) (void)hipHostFree(g->mirror_h);
   if (g->ev_in) (void)hipEventDestroy(g->ev_in);
@@ -519,6 +629,20 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
   }
   g->L.resize(num_levels);
   int rc = TE_OK;
+  {
+    // the compact 3D path: 16^3 patches with same-level face neighbours on every level, the
+    // DFT smoother everywhere, the Poisson operator, a V-cycle (TE_NO_FUSE=1 or TE_DFT_VALU=1
+    // keep the one-launch-per-step cycle, for comparison)
+    const char* nf = std::getenv("TE_NO_FUSE");
+    const char* valu = std::getenv("TE_DFT_VALU");
+    bool c3 = !(nf && nf[0] == '1') && !(valu && valu[0] == '1') && opts->cycle_type == TE_CYCLE_V &&
+              opts->op_kind == 0;
+    for (int i = 0; i < num_levels && c3; i++) {
+      const int sk = opts->level_smoother ? opts->level_smoother[i] : opts->smoother;
+      c3 = sk == 0 && c3_supported(levels[i]);
+    }
+    g->c3 = c3;
+  }
   for (int i = 0; i < num_levels && !rc; i++) {
     GLevel& Lv = g->L[i];
     Lv.d = levels[i];
@@ -530,17 +654,31 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     rc = te_ghost_filler_create(Lv.d, opts->filler_kind, opts->fill_type, &Lv.gf);
     if (rc) break;
     long n = te_domain_vector_length(Lv.d, 1);
-    size_t bytes = esize(Lv.dt) * (size_t)(n > 0 ? n : 1);
-    if ((e = hipMalloc(&Lv.f, bytes)) != hipSuccess || (e = hipMalloc(&Lv.u, bytes)) != hipSuccess ||
-        (e = hipMalloc(&Lv.r, bytes)) != hipSuccess || (e = hipMalloc(&Lv.u2, bytes)) != hipSuccess) {
-      set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
-      rc = TE_ERR_HIP;
-      break;
+    if (g->c3) {
+      // compact level vectors only (the padded ones serve the one-launch-per-step cycle)
+      g->C3.resize(num_levels);
+      const size_t cb = esize(Lv.dt) * (size_t)std::max<long>((long)Lv.d->np * kC3Cells, 1);
+      te_gmg::C3Level& Cl = g->C3[i];
+      if ((e = hipMalloc(&Cl.f, cb)) != hipSuccess || (e = hipMalloc(&Cl.u, cb)) != hipSuccess ||
+          (e = hipMalloc(&Cl.u2, cb)) != hipSuccess || (e = hipMemset(Cl.f, 0, cb)) != hipSuccess ||
+          (e = hipMemset(Cl.u, 0, cb)) != hipSuccess || (e = hipMemset(Cl.u2, 0, cb)) != hipSuccess) {
+        set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
+        rc = TE_ERR_HIP;
+        break;
+      }
+    } else {
+      size_t bytes = esize(Lv.dt) * (size_t)(n > 0 ? n : 1);
+      if ((e = hipMalloc(&Lv.f, bytes)) != hipSuccess || (e = hipMalloc(&Lv.u, bytes)) != hipSuccess ||
+          (e = hipMalloc(&Lv.r, bytes)) != hipSuccess || (e = hipMalloc(&Lv.u2, bytes)) != hipSuccess) {
+        set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
+        rc = TE_ERR_HIP;
+        break;
+      }
+      (void)hipMemset(Lv.f, 0, bytes);
+      (void)hipMemset(Lv.u, 0, bytes);
+      (void)hipMemset(Lv.r, 0, bytes);
+      (void)hipMemset(Lv.u2, 0, bytes);
     }
-    (void)hipMemset(Lv.f, 0, bytes);
-    (void)hipMemset(Lv.u, 0, bytes);
-    (void)hipMemset(Lv.r, 0, bytes);
-    (void)hipMemset(Lv.u2, 0, bytes);
     if (opts->op_kind == 1) {
       if (!coeffs[i]) {
         set_error("te_gmg_create: missing coefficients for a level");
@@ -666,6 +804,24 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
       }
     }
     g->nblk = krylov_blocks(d0);
+    if (g->c3 && e == hipSuccess) {
+      const size_t nc = (size_t)std::max<long>((long)d0->np * kC3Cells, 1);
+      const size_t em = esize(g->L[0].dt);
+      for (double** v : { &g->c_r, &g->c_rhat, &g->c_p, &g->c_ap, &g->c_as, &g->c_s }) {
+        if (e == hipSuccess) e = hipMalloc(v, sizeof(double) * nc);
+        if (e == hipSuccess) e = hipMemset(*v, 0, sizeof(double) * nc);
+      }
+      for (void** v : { &g->c_mp, &g->c_ms, &g->c_tmp }) {
+        if (e == hipSuccess) e = hipMalloc(v, em * nc);
+        if (e == hipSuccess) e = hipMemset(*v, 0, em * nc);
+      }
+      if (g->L[0].dt == TE_F32) {
+        for (float** v : { &g->c_p32, &g->c_s32 }) {
+          if (e == hipSuccess) e = hipMalloc(v, sizeof(float) * nc);
+          if (e == hipSuccess) e = hipMemset(*v, 0, sizeof(float) * nc);
+        }
+      }
+    }
     if (e == hipSuccess) e = hipMalloc(&g->scal, sizeof(double) * 16);
     if (e == hipSuccess) e = hipMalloc(&g->pa, sizeof(double) * (g->nblk + 1));
     if (e == hipSuccess) e = hipMalloc(&g->pb, sizeof(double) * (g->nblk + 1));
@@ -772,7 +928,56 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // follows them (the first iteration's comes from a separate sweep after the setup)
   const bool l0f = m32 && g->bsweep && !level0_timed(g);
   float* u0 = (float*)L0.u;
+  // the compact 3D path: the same iteration on compact work vectors (cycle3d.h); x and b
+  // keep the caller's padded layout.  M p and M s are of the finest level's type (the last
+  // sweep computes in it), and in mixed precision the cycle's right-hand sides are FP32
+  // copies of p and s written by the p and s updates.
+  const int c3m = use_precond ? L0.dt : TE_F64;
+  void* c3_mp = use_precond ? g->c_mp : (void*)g->c_p;
+  void* c3_ms = use_precond ? g->c_ms : (void*)g->c_s;
+  const bool c3_32 = use_precond && L0.dt == TE_F32;
+  auto iteration_c3 = [&]() -> int {
+    C3Bicg a{};
+    a.scal = g->scal;
+    a.x = x;
+    a.r = g->c_r;
+    a.rhat = g->c_rhat;
+    a.p = g->c_p;
+    a.sv = g->c_s;
+    a.ap = g->c_ap;
+    a.as = g->c_as;
+    a.mp = c3_mp;
+    a.ms = c3_ms;
+    a.dtm = c3m;
+    if (use_precond)
+      TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_p32 : (const void*)g->c_p, c3_32 ? TE_F32 : TE_F64, c3_mp, c3m));
+    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_mp, g->c_ap, g->c_rhat, g->pa, nullptr, nb, s));
+    // s = r - alpha ap, alpha folded from the rhat.ap partials; s.s partials into pc
+    a.w32 = c3_32 ? g->c_s32 : nullptr;
+    a.pa = g->pc;
+    a.pfold = g->pa;
+    a.nfold = nb;
+    TE_RC(launch_c3_bicg(BICG_S, d, a, nb, s));
+    if (use_precond)
+      TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_s32 : (const void*)g->c_s, c3_32 ? TE_F32 : TE_F64, c3_ms, c3m));
+    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_ms, g->c_as, g->c_s, g->pa, g->pb, nb, s));
+    TE_RC(launch_bicg_finalize(FIN_SOMEGA, g->pa, g->pb, nb, g->scal, s, 0.0, nullptr, g->pc));
+    // x and r updates; after the s-norm test fired, x += alpha p (use_precond 2: alpha M p)
+    a.w32 = nullptr;
+    a.pfold = nullptr;
+    a.nfold = 0;
+    a.pa = g->pa;
+    a.pb = g->pb;
+    a.dir = use_precond == 2 ? (const void*)c3_mp : (const void*)g->c_p;
+    a.dir_m = use_precond == 2 ? 1 : 0;
+    TE_RC(launch_c3_bicg(BICG_XR, d, a, nb, s));
+    TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
+    a.dir = nullptr;
+    a.w32 = c3_32 ? g->c_p32 : nullptr;
+    return launch_c3_bicg(BICG_P, d, a, nb, s);
+  };
   auto iteration = [&]() -> int {
+    if (g->c3) return iteration_c3();
     if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->p32 : g->p, mp, dtm, dtm, l0f));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, dtm, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
@@ -829,8 +1034,23 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // resid = b - A x ; r0_norm = ||b|| ; rhat = p = resid ; rho = rhat.resid
   TE_RC(launch_ghost_fill(gf, TE_F64, x, 1, s));
   TE_RC(launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, b, x, g->resid, 1, s));
-  TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s,
-                    TE_F64, m32 ? g->p32 : nullptr));
+  if (g->c3) {
+    C3Bicg a{};
+    a.scal = g->scal;
+    a.b = b;
+    a.rpad = g->resid;
+    a.r = g->c_r;
+    a.rhat = g->c_rhat;
+    a.p = g->c_p;
+    a.w32 = c3_32 ? g->c_p32 : nullptr;
+    a.pa = g->pa;
+    a.pb = g->pb;
+    a.dtm = TE_F64;
+    TE_RC(launch_c3_bicg(BICG_INIT, d, a, nb, s));
+  } else {
+    TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s,
+                      TE_F64, m32 ? g->p32 : nullptr));
+  }
   TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s, tol));
   // the first V-cycle's zero-guess sweep on the finest level (the later ones run in the p
   // updates)
@@ -987,6 +1207,74 @@ int te_gmg_cg(te_gmg* g, const double* b, double* x, int use_precond, int max_it
 }
 
 double te_gmg_last_solve_seconds(const te_gmg* g) { return g ? g->last_seconds : -1.0; }
+
+int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
+{
+  if (!g || !ms || reps < 1 || which < TE_K_SWEEP_ZERO || which > TE_K_UPDATE_XR) {
+    set_error("te_gmg_time_kernel: bad argument");
+    return TE_ERR_ARG;
+  }
+  if (!g->c3) {
+    set_error("te_gmg_time_kernel: only the compact 3D cycle has standalone kernel timings");
+    return TE_ERR_UNSUPPORTED;
+  }
+  const bool two = g->L.size() > 1;
+  if (!two && (which == TE_K_SWEEP_INTERP || which == TE_K_RESID_RESTRICT)) {
+    set_error("te_gmg_time_kernel: the transfer kernels need a coarser level");
+    return TE_ERR_UNSUPPORTED;
+  }
+  GLevel& L0 = g->L[0];
+  te_gmg::C3Level& C0 = g->C3[0];
+  hipStream_t s = g->s;
+  const int nb = g->nblk;
+  auto launch = [&]() -> int {
+    switch (which) {
+      case TE_K_SWEEP_ZERO:
+        return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 0, TE_F64, nullptr, nullptr, s);
+      case TE_K_SWEEP:
+        return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 1, TE_F64, nullptr, nullptr, s);
+      case TE_K_SWEEP_INTERP:
+        return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 2, g->L[1].dt, g->C3[1].u, L0.link,
+                               s);
+      case TE_K_RESID_RESTRICT:
+        return launch_c3_resid_restrict(L0.d, L0.link, g->o.neumann, L0.dt, L0.dt, C0.f, C0.u, g->L[1].dt,
+                                        g->C3[1].f, s);
+      case TE_K_OPERATOR:
+        return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_ms, g->c_as, g->c_s, g->pa, g->pb, nb, s);
+      default: {
+        C3Bicg a{};
+        a.scal = g->scal;
+        a.x = g->resid;  // a padded FP64 work vector stands in for x
+        a.r = g->c_r;
+        a.rhat = g->c_rhat;
+        a.ap = g->c_ap;
+        a.as = g->c_as;
+        a.mp = g->c_mp;
+        a.ms = g->c_ms;
+        a.dtm = L0.dt;
+        a.pa = g->pa;
+        a.pb = g->pb;
+        return launch_c3_bicg(BICG_XR, L0.d, a, nb, s);
+      }
+    }
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  TE_HIP(hipEventCreate(&e0));
+  TE_HIP(hipEventCreate(&e1));
+  int rc = launch();
+  if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = TE_ERR_HIP;
+  for (int k = 0; k < reps && !rc; k++) rc = launch();
+  float t = 0.0f;
+  if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&t, e0, e1) != hipSuccess)) {
+    set_error("te_gmg_time_kernel: event timing failed");
+    rc = TE_ERR_HIP;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc) *ms = (double)t / reps;
+  return rc;
+}
 
 int te_gmg_set_timer(te_gmg* g, te_timer* t, int iterations, const int* domain_ids)
 {

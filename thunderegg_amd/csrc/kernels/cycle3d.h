@@ -1,0 +1,78 @@
+// The 3D V-cycle and the outer BiCGStab's vector work on compact vectors: interior cells
+// only, [patch][z][y][x], 16^3 cells per patch, for levels whose face neighbours are all on
+// the same level (or a physical boundary).  The caller's vectors keep the reference's
+// padded layout; the solver's own work vectors need no ghost cells, because every kernel
+// here computes the ghost values it needs from the neighbours' boundary cells (the
+// TriLinearGhostFiller's same-level case is a copy, TriLinearGhostFiller.cpp:33-175).
+#pragma once
+
+#include "../te_internal.h"
+#include "dft.h"
+
+namespace te {
+
+constexpr int kC3N = 16;
+constexpr int kC3Cells = kC3N * kC3N * kC3N;
+
+// 3D, 16^3 cells, every face neighbour on the same level
+bool c3_supported(const te_domain* d);
+
+// One PatchSolver::smooth sweep of the DFT patch solver (Poisson/DFTPatchSolver.h:420-450)
+// on compact vectors.  mode 0: u is zero (out = solve(f)); 1: the interface terms of
+// modifyRHSForInternalBoundaryConditions from uin's boundary cells and the neighbours';
+// 2: the same on uin + P uc (DirectInterpolator, GMG/DirectInterpolator.h:60-91; uc is the
+// coarser level's compact solution of type dtc, link: this level -> the coarser one).
+// out must not alias uin.  dt: arithmetic type; dtf / dto: types of f and out.
+int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const void* uin, int dto, void* out,
+                    int mode, int dtc, const void* uc, const te_level_link* link, hipStream_t s);
+// Cycle::restrict (GMG/Cycle.h:57-65): fc = restriction (GMG/LinearRestrictor.h:101-155) of
+// f - A u, A the Poisson star operator (neumann: Neumann physical sides); fc compact, of the
+// coarser level, type dtc.
+int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int neumann, int dt, int dtf,
+                             const void* f, const void* u, int dtc, void* fc, hipStream_t s);
+// out = A u (FP64, compact; u compact of type udt, widened as read) with the per-workgroup
+// partial sums pa = sum w out and, when pb is not null, pb = sum out out; nslots partial
+// slots, those past the launch's workgroups written zero.
+int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, double* out, const double* w,
+                       double* pa, double* pb, int nslots, hipStream_t s);
+// BiCGStab vector updates (Iterative/BiCGStab.h:111-203) on compact vectors, the modes and
+// scalar slots of krylov.h.  x and b are the caller's padded FP64 vectors.
+//   BICG_INIT: r_pad (padded residual b - A x) -> r, rhat, p (+ p32); sums b.b, r.r
+//   BICG_S:    s = r - alpha ap (+ s32), sum s.s (alpha folded from pfold when given)
+//   BICG_XR:   x += alpha mp + omega ms; r -= alpha ap + omega as; sums r.rhat, r.r; after
+//              the s-norm test fired: x += alpha dir (dir of type dtm when dir_m, else FP64)
+//   BICG_P:    p = r + beta (p - omega ap) (+ p32)
+struct C3Bicg {
+  double* scal;
+  double* x;         // padded FP64 (XR)
+  const double* b;   // padded FP64 (INIT)
+  const double* rpad;  // padded FP64 (INIT)
+  double* r;
+  double* rhat;
+  double* p;
+  double* sv;
+  const double* ap;
+  const double* as;
+  const void* mp;    // type dtm
+  const void* ms;
+  float* w32;        // FP32 copy of the new p or s, or null
+  const void* dir;   // early-exit direction (XR)
+  int dir_m;
+  int dtm;
+  double* pa;
+  double* pb;
+  const double* pfold;
+  int nfold;
+};
+int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hipStream_t s);
+// compact <-> padded copies with element-type conversion (interior cells; the padded
+// vector's ghost cells are not written)
+int launch_c3_pack(const te_domain* d, int dtd, void* dst, int dts, const void* src_padded, hipStream_t s);
+int launch_c3_unpack(const te_domain* d, int dtd, void* dst_padded, int dts, const void* src, hipStream_t s);
+// out = u + P uc (compact, this level's type dt; uc of type dtc)
+int launch_c3_interp(const te_level_link* link, int dtc, const void* uc, int dt, const void* u, void* out,
+                     hipStream_t s);
+// x := 0 (n elements of type dt), a kernel (no memset node in captured graphs)
+int launch_c3_zero(long n, int dt, void* x, hipStream_t s);
+
+}  // namespace te

@@ -1,0 +1,740 @@
+// The 3D V-cycle and the outer BiCGStab's vector work on compact vectors (see cycle3d.h).
+//
+// Lane layout shared by every kernel here: a wavefront works on 16 x 16 slices of a patch;
+// lane l = c + 16 g holds the four cells x = 4g .. 4g + 3 of row y = c, so a row of a slice
+// is one 64-byte (FP32) or 128-byte (FP64) load by four lanes and a slice is one load
+// instruction per wavefront.
+//
+// Patch solve (c3_sweep_kernel, one 256-thread workgroup per patch): the reference applies
+// a dense n x n transform along x, then y, then z, divides by the eigenvalues and applies
+// the inverse transforms (Poisson/DFTPatchSolver.h:205-265, 320-361, 420-450).  Here each
+// transform is a 16 x 16 x 16 matrix product on the matrix cores (v_mfma_*_16x16x4), with
+// the transform matrices held in registers:
+//   per z-slice:  D1 = F Sx^T, D2 = Sy D1          (D1 stays in registers: its accumulator
+//                                                   layout is the B operand of the next
+//                                                   product with a matching K order)
+//   per y-row:    W = Sz D2, W /= lambda, W' = Szi W (the slab passes through LDS once)
+//   per z-slice:  R = Q Sxi^T, U = Syi R           (R again register-resident)
+// so the 16^3 patch crosses LDS twice (after the forward x-y pass, after the z pass).
+// In a product whose operands come from memory the K index of MFMA step s in lane group
+// g is 4g + s (four consecutive values per lane: one vector load); in a product fed by
+// the previous accumulator it is the accumulator's row of register s (Mf<T>::row).
+//
+// Stencils (c3_resid_restrict_kernel, c3_star_dot_kernel, one wavefront per patch): the
+// wavefront walks the 16 slices keeping slices z - 1, z, z + 1 in registers; x and y
+// neighbours come from other lanes (shuffles), the patch's face ghosts from the same-level
+// neighbour's boundary slice or the physical boundary condition.
+#include <type_traits>
+
+#include "../te_internal.h"
+#include "blas.h"
+#include "cycle3d.h"
+#include "dft_wave.h"
+#include "fold.h"
+#include "krylov.h"
+
+namespace te {
+namespace {
+
+constexpr int N = kC3N;
+constexpr int NC = kC3Cells;
+constexpr int kYP = 20;               // LDS pitch of a spectral row (16 + 4: 16-byte aligned rows)
+constexpr int kZP = 16 * kYP + 4;     // LDS pitch of a spectral slice
+constexpr int kSlab = 16 * kZP;
+
+// four consecutive values starting at an offset that is a multiple of 4, converted to T
+template <typename T, typename TS>
+__device__ __forceinline__ void ld4(const TS* __restrict__ base, int off, T (&r)[4])
+{
+  if constexpr (sizeof(TS) == 4) {
+    const float4 v = *reinterpret_cast<const float4*>(base + off);
+    r[0] = T(v.x);
+    r[1] = T(v.y);
+    r[2] = T(v.z);
+    r[3] = T(v.w);
+  } else {
+    const double2 a = *reinterpret_cast<const double2*>(base + off);
+    const double2 b = *reinterpret_cast<const double2*>(base + off + 2);
+    r[0] = T(a.x);
+    r[1] = T(a.y);
+    r[2] = T(b.x);
+    r[3] = T(b.y);
+  }
+}
+
+template <typename TD, typename T>
+__device__ __forceinline__ void st4(TD* __restrict__ base, int off, const T (&r)[4])
+{
+  if constexpr (sizeof(TD) == 4) {
+    *reinterpret_cast<float4*>(base + off) = make_float4(float(r[0]), float(r[1]), float(r[2]), float(r[3]));
+  } else {
+    *reinterpret_cast<double2*>(base + off) = make_double2(double(r[0]), double(r[1]));
+    *reinterpret_cast<double2*>(base + off + 2) = make_double2(double(r[2]), double(r[3]));
+  }
+}
+
+__device__ __forceinline__ int cidx(int x, int y, int z) { return x + 16 * y + 256 * z; }
+
+// u (+ the coarser level's correction uc, INTERP) at cells of patch q, compact vectors
+template <typename T, typename TC, bool INTERP>
+struct C3U {
+  const T* __restrict__ u;
+  const TC* __restrict__ uc;
+  const int32_t* __restrict__ parent;
+  const int32_t* __restrict__ porth;
+  // coarse cell of fine cell (x, y, z) of a patch with orthant o on its parent (o < 0: copy)
+  __device__ __forceinline__ int coarse(int o, int x, int y, int z) const
+  {
+    if (o < 0) return cidx(x, y, z);
+    return cidx((x + ((o & 1) ? N : 0)) >> 1, (y + ((o & 2) ? N : 0)) >> 1, (z + ((o & 4) ? N : 0)) >> 1);
+  }
+  __device__ __forceinline__ T one(int q, int x, int y, int z) const
+  {
+    T v = u[q * NC + cidx(x, y, z)];
+    if (INTERP) v = T(v + T(uc[parent[q] * NC + coarse(porth[q], x, y, z)]));
+    return v;
+  }
+  // cells x0 .. x0 + 3 of row (y, z), x0 a multiple of 4
+  __device__ __forceinline__ void row4(int q, int x0, int y, int z, T (&r)[4]) const
+  {
+    ld4(u, q * NC + cidx(x0, y, z), r);
+    if (INTERP) {
+      const int o = porth[q];
+      const TC* b = uc + parent[q] * NC;
+      if (o >= 0) {
+        const int k = coarse(o, x0, y, z);
+        const T a0 = T(b[k]), a1 = T(b[k + 1]);
+        r[0] = T(r[0] + a0);
+        r[1] = T(r[1] + a0);
+        r[2] = T(r[2] + a1);
+        r[3] = T(r[3] + a1);
+      } else {
+        T a[4];
+        ld4(b, cidx(x0, y, z), a);
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[k] = T(r[k] + a[k]);
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// The patch solve.  MODE 0: u = 0 (f only); 1: interface terms from uin; 2: from uin + P uc.
+template <typename T, typename TF, typename TO, typename TC, int MODE>
+__global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restrict__ dp,
+                                                       const int* __restrict__ ptype, const T* __restrict__ mats,
+                                                       const T* __restrict__ s2tab, const TF* __restrict__ f,
+                                                       const T* __restrict__ uin, TO* __restrict__ out,
+                                                       const TC* __restrict__ uc, const int32_t* __restrict__ parent,
+                                                       const int32_t* __restrict__ porth)
+{
+  __shared__ __align__(16) T slab[kSlab];
+  typedef typename Mf<T>::V V;
+  const int p = blockIdx.x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const int* pt = ptype + p * 8;  // forward type per axis [0..2], eigenvalue type per axis [3..5], zero mode [6]
+  int tf[3], ti[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    tf[a] = pt[a];
+    ti[a] = tf[a] == 0 ? 1 : (tf[a] == 3 ? 4 : tf[a]);  // DCT-II -> DCT-III, DST-II -> DST-III, IV types self-inverse
+  }
+  // the transform matrices M[i][j] (y_i = sum_j M[i][j] x_j, getTransformArray :131-196) as
+  // MFMA operands: lane (c, g), step s holds M[c][K] with K the product's K order
+  auto mat = [&](int t, int a, int i, int j) { return mats[(t * 3 + a) * kDftMatStride + i * N + j]; };
+  T SxB[4], SyA[4], SzA[4], SziA[4], SxiB[4], SyiA[4];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int k1 = 4 * g + s, k2 = Mf<T>::row(s, g);
+    SxB[s] = mat(tf[0], 0, c, k1);
+    SyA[s] = mat(tf[1], 1, c, k2);
+    SzA[s] = mat(tf[2], 2, c, k1);
+    SziA[s] = mat(ti[2], 2, c, k2);
+    SxiB[s] = mat(ti[0], 0, c, k1);
+    SyiA[s] = mat(ti[1], 1, c, k2);
+  }
+  const DevPatch& P = dp[p];
+  const C3U<T, TC, MODE == 2> U{ uin, uc, parent, porth };
+
+  // ---- forward transforms along x and y, one z-slice at a time
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int z = w + 4 * i;
+    T F[4];
+    ld4(f, p * NC + cidx(4 * g, c, z), F);
+    if (MODE != 0) {
+      // modifyRHSForInternalBoundaryConditions (Poisson/StarPatchOperator.h:165-180): on
+      // every side with a neighbour, f_b -= (ghost + u_b) / h^2, sides in the reference's order
+      const T ihx = T(P.inv_h2[0]), ihy = T(P.inv_h2[1]), ihz = T(P.inv_h2[2]);
+      if (g == 0 && P.type[20] >= 0) F[0] -= (U.one(P.nbr[20][0], N - 1, c, z) + U.one(p, 0, c, z)) * ihx;
+      if (g == 3 && P.type[21] >= 0) F[3] -= (U.one(P.nbr[21][0], 0, c, z) + U.one(p, N - 1, c, z)) * ihx;
+      if (c == 0 && P.type[22] >= 0) {
+        T gq[4], ub[4];
+        U.row4(P.nbr[22][0], 4 * g, N - 1, z, gq);
+        U.row4(p, 4 * g, 0, z, ub);
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] -= (gq[k] + ub[k]) * ihy;
+      }
+      if (c == N - 1 && P.type[23] >= 0) {
+        T gq[4], ub[4];
+        U.row4(P.nbr[23][0], 4 * g, 0, z, gq);
+        U.row4(p, 4 * g, N - 1, z, ub);
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] -= (gq[k] + ub[k]) * ihy;
+      }
+      if (z == 0 && P.type[24] >= 0) {
+        T gq[4], ub[4];
+        U.row4(P.nbr[24][0], 4 * g, c, N - 1, gq);
+        U.row4(p, 4 * g, c, 0, ub);
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] -= (gq[k] + ub[k]) * ihz;
+      }
+      if (z == N - 1 && P.type[25] >= 0) {
+        T gq[4], ub[4];
+        U.row4(P.nbr[25][0], 4 * g, c, 0, gq);
+        U.row4(p, 4 * g, c, N - 1, ub);
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] -= (gq[k] + ub[k]) * ihz;
+      }
+    }
+    V d1 = { 0, 0, 0, 0 };  // D1[y][kx] = sum_x F[y][x] Sx[kx][x]
+#pragma unroll
+    for (int s = 0; s < 4; s++) d1 = Mf<T>::mma(F[s], SxB[s], d1);
+    V d2 = { 0, 0, 0, 0 };  // D2[ky][kx] = sum_y Sy[ky][y] D1[y][kx]
+#pragma unroll
+    for (int s = 0; s < 4; s++) d2 = Mf<T>::mma(SyA[s], d1[s], d2);
+#pragma unroll
+    for (int r = 0; r < 4; r++) slab[z * kZP + Mf<T>::row(r, g) * kYP + c] = d2[r];
+  }
+  __syncthreads();
+
+  // ---- along z, one ky-row at a time: forward, eigenvalues, inverse
+  const int et0 = pt[3], et1 = pt[4], et2 = pt[5];
+  const T lx = T(4.0 / (P.h[0] * P.h[0])), ly = T(4.0 / (P.h[1] * P.h[1])), lz = T(4.0 / (P.h[2] * P.h[2]));
+  const T s2x = s2tab[(et0 * 3 + 0) * kDftMatDim + c];
+  T s2z[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) s2z[r] = s2tab[(et2 * 3 + 2) * kDftMatDim + Mf<T>::row(r, g)];
+  const bool zero_mode = pt[6] != 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int ky = w + 4 * i;
+    V b = { 0, 0, 0, 0 };  // W[kz][kx] = sum_z Sz[kz][z] D2[z][ky][kx]
+#pragma unroll
+    for (int s = 0; s < 4; s++) b = Mf<T>::mma(SzA[s], slab[(4 * g + s) * kZP + ky * kYP + c], b);
+    const T s2y = s2tab[(et1 * 3 + 1) * kDftMatDim + ky];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      // getEigenValues (:320-361): -sum_a 4/h_a^2 sin^2(...)
+      T eig = T(0);
+      eig -= lx * s2x;
+      eig -= ly * s2y;
+      eig -= lz * s2z[r];
+      if constexpr (sizeof(T) == 4) b[r] = b[r] * T(__builtin_amdgcn_rcpf(float(eig)));
+      else b[r] = b[r] / eig;
+      // all-Neumann patch without neighbours: the zero mode is set to 0 (:438-440)
+      if (zero_mode && ky == 0 && c == 0 && Mf<T>::row(r, g) == 0) b[r] = T(0);
+    }
+    V b2 = { 0, 0, 0, 0 };  // W'[z][kx] = sum_kz Szi[z][kz] W[kz][kx]
+#pragma unroll
+    for (int s = 0; s < 4; s++) b2 = Mf<T>::mma(SziA[s], b[s], b2);
+#pragma unroll
+    for (int r = 0; r < 4; r++) slab[Mf<T>::row(r, g) * kZP + ky * kYP + c] = b2[r];
+  }
+  __syncthreads();
+
+  // ---- inverse transforms along x and y, one z-slice at a time, scaled by prod 2/n (:442-449)
+  const T scale = T(2.0 / N) * T(2.0 / N) * T(2.0 / N);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int z = w + 4 * i;
+    T Q[4];
+    const T* qrow = slab + z * kZP + c * kYP + 4 * g;
+#pragma unroll
+    for (int k = 0; k < 4; k++) Q[k] = qrow[k];
+    V r1 = { 0, 0, 0, 0 };  // R[ky][x] = sum_kx Q[ky][kx] Sxi[x][kx]
+#pragma unroll
+    for (int s = 0; s < 4; s++) r1 = Mf<T>::mma(Q[s], SxiB[s], r1);
+    V u2 = { 0, 0, 0, 0 };  // U[y][x] = sum_ky Syi[y][ky] R[ky][x]
+#pragma unroll
+    for (int s = 0; s < 4; s++) u2 = Mf<T>::mma(SyiA[s], r1[s], u2);
+#pragma unroll
+    for (int r = 0; r < 4; r++) out[p * NC + cidx(c, Mf<T>::row(r, g), z)] = TO(u2[r] * scale);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// The Poisson star operator (Poisson/StarPatchOperator.h:75-101) of one slice of a patch,
+// wavefront-wide.  cur / prv / nxt: the lane's row in slices z, z - 1, z + 1 (with the face
+// ghosts or boundary values of the z faces already in place).  bc: the physical ghost factor
+// (-1 Dirichlet, +1 Neumann, :115-144).  q: the six face neighbours (-1: physical side).
+template <typename T, typename TU>
+__device__ __forceinline__ void star_slice(const TU* __restrict__ u, const int (&q)[6], int lane, int c, int g, int z,
+                                           T bc, T ihx, T ihy, T ihz, const T (&cur)[4], const T (&prv)[4],
+                                           const T (&nxt)[4], T (&acc)[4])
+{
+  T xm = __shfl(cur[3], (lane - 16) & 63, 64);
+  T xp = __shfl(cur[0], (lane + 16) & 63, 64);
+  T ym[4], yp[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    ym[k] = __shfl(cur[k], (lane - 1) & 63, 64);
+    yp[k] = __shfl(cur[k], (lane + 1) & 63, 64);
+  }
+  if (g == 0) xm = q[0] >= 0 ? T(u[q[0] * NC + cidx(N - 1, c, z)]) : bc * cur[0];
+  if (g == 3) xp = q[1] >= 0 ? T(u[q[1] * NC + cidx(0, c, z)]) : bc * cur[3];
+  if (c == 0) {
+    if (q[2] >= 0) ld4(u, q[2] * NC + cidx(4 * g, N - 1, z), ym);
+    else
+#pragma unroll
+      for (int k = 0; k < 4; k++) ym[k] = bc * cur[k];
+  }
+  if (c == N - 1) {
+    if (q[3] >= 0) ld4(u, q[3] * NC + cidx(4 * g, 0, z), yp);
+    else
+#pragma unroll
+      for (int k = 0; k < 4; k++) yp[k] = bc * cur[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const T mid = cur[k];
+    const T lo = k == 0 ? xm : cur[k - 1];
+    const T hi = k == 3 ? xp : cur[k + 1];
+    T a = T(0);
+    a += (hi - T(2) * mid + lo) * ihx;
+    a += (yp[k] - T(2) * mid + ym[k]) * ihy;
+    a += (nxt[k] - T(2) * mid + prv[k]) * ihz;
+    acc[k] = a;
+  }
+}
+
+// the lane's row of slice z - 1 at z = 0 (bottom ghost) and of slice z + 1 at z = 15 (top)
+template <typename T, typename TU>
+__device__ __forceinline__ void z_ghost(const TU* __restrict__ u, int qz, int zsrc, int ro, T bc, const T (&cur)[4],
+                                        T (&r)[4])
+{
+  if (qz >= 0) {
+    ld4(u, qz * NC + 256 * zsrc + ro, r);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) r[k] = bc * cur[k];
+  }
+}
+
+// Cycle::restrict: fc = restriction of f - A u (one wavefront per fine patch)
+template <typename T, typename TF, typename TC>
+__global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* __restrict__ dp,
+                                                                const T* __restrict__ u, const TF* __restrict__ f,
+                                                                TC* __restrict__ fc,
+                                                                const int32_t* __restrict__ parent,
+                                                                const int32_t* __restrict__ porth, int np, int neumann)
+{
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const int ro = cidx(4 * g, c, 0);
+  const int p = blockIdx.x * 4 + wv;
+  if (p >= np) return;
+  const DevPatch& P = dp[p];
+  int q[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++) q[s] = P.type[20 + s] >= 0 ? P.nbr[20 + s][0] : -1;
+  const T ihx = T(P.inv_h2[0]), ihy = T(P.inv_h2[1]), ihz = T(P.inv_h2[2]);
+  const T bc = neumann ? T(1) : T(-1);
+  const int base = p * NC;
+  const int cp = parent[p], o = porth[p];
+  const T wgt = T(1.0 / 8);
+  T cur[4], prv[4], nxt[4];
+  ld4(u, base + ro, cur);
+  z_ghost(u, q[4], N - 1, ro, bc, cur, prv);
+  T hold0 = T(0), hold1 = T(0);
+  for (int z = 0; z < N; z++) {
+    if (z < N - 1) ld4(u, base + 256 * (z + 1) + ro, nxt);
+    else z_ghost(u, q[5], 0, ro, bc, cur, nxt);
+    T fv[4];
+    ld4(f, base + 256 * z + ro, fv);
+    T acc[4];
+    star_slice(u, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
+    T r[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) r[k] = fv[k] - acc[k];
+    if (o < 0) {  // copyToParent (GMG/LinearRestrictor.h:138-155)
+      st4(fc, cp * NC + 256 * z + ro, r);
+    } else {      // restrictToCoarserParent (:101-128): coarse[(i + s) / 2] += fine[i] / 2^D
+      const T sx0 = r[0] * wgt + r[1] * wgt, sx1 = r[2] * wgt + r[3] * wgt;
+      const T sy0 = sx0 + __shfl(sx0, (lane + 1) & 63, 64);
+      const T sy1 = sx1 + __shfl(sx1, (lane + 1) & 63, 64);
+      if ((z & 1) == 0) {
+        hold0 = sy0;
+        hold1 = sy1;
+      } else if ((c & 1) == 0) {
+        const int k = cp * NC + cidx(2 * g + ((o & 1) ? 8 : 0), (c >> 1) + ((o & 2) ? 8 : 0),
+                                     (z >> 1) + ((o & 4) ? 8 : 0));
+        fc[k] = TC(hold0 + sy0);
+        fc[k + 1] = TC(hold1 + sy1);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      prv[k] = cur[k];
+      cur[k] = nxt[k];
+    }
+  }
+}
+
+// out = A u with the partial sums w.out (and out.out); persistent wavefronts over patches
+template <typename TU, bool SQ>
+__global__ void __launch_bounds__(256) c3_star_dot_kernel(const DevPatch* __restrict__ dp, const TU* __restrict__ u,
+                                                          double* __restrict__ out, const double* __restrict__ w,
+                                                          double* __restrict__ pa, double* __restrict__ pb, int np,
+                                                          int nslots, int neumann)
+{
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const int ro = cidx(4 * g, c, 0);
+  const double bc = neumann ? 1.0 : -1.0;
+  double sa = 0.0, sq = 0.0;
+  for (int p = blockIdx.x * 4 + wv; p < np; p += gridDim.x * 4) {
+    const DevPatch& P = dp[p];
+    int q[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) q[s] = P.type[20 + s] >= 0 ? P.nbr[20 + s][0] : -1;
+    const double ihx = P.inv_h2[0], ihy = P.inv_h2[1], ihz = P.inv_h2[2];
+    const int base = p * NC;
+    double cur[4], prv[4], nxt[4];
+    ld4(u, base + ro, cur);
+    z_ghost(u, q[4], N - 1, ro, bc, cur, prv);
+    for (int z = 0; z < N; z++) {
+      if (z < N - 1) ld4(u, base + 256 * (z + 1) + ro, nxt);
+      else z_ghost(u, q[5], 0, ro, bc, cur, nxt);
+      double wv4[4];
+      ld4(w, base + 256 * z + ro, wv4);
+      double acc[4];
+      star_slice(u, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
+      st4(out, base + 256 * z + ro, acc);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        sa += wv4[k] * acc[k];  // rhat.dot(ap) / as.dot(s): w first, as the reference
+        if (SQ) sq += acc[k] * acc[k];
+        prv[k] = cur[k];
+        cur[k] = nxt[k];
+      }
+    }
+  }
+  const double ra = block_reduce<RED_SUM>(sa);
+  if (threadIdx.x == 0) pa[blockIdx.x] = ra;
+  if (SQ) {
+    const double rb = block_reduce<RED_SUM>(sq);
+    if (threadIdx.x == 0) pb[blockIdx.x] = rb;
+  }
+  if (blockIdx.x == 0)
+    for (int k = (int)gridDim.x + threadIdx.x; k < nslots; k += blockDim.x) {
+      pa[k] = 0.0;
+      if (SQ) pb[k] = 0.0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// padded offset of compact cell i (16^3 patches: 18^3 padded cells)
+__device__ __forceinline__ int pad_of(int i)
+{
+  const int p = i >> 12, x = i & 15, y = (i >> 4) & 15, z = i >> 8 & 15;
+  return p * (18 * 18 * 18) + (x + 1) + 18 * (y + 1) + 324 * (z + 1);
+}
+
+// BiCGStab vector updates on compact vectors; one partial per workgroup, nslots slots
+template <int MODE, typename TM>
+__global__ void __launch_bounds__(256) c3_bicg_kernel(C3Bicg a, long n, int nslots)
+{
+  const bool fold = MODE == BICG_S && a.pfold != nullptr;
+  double* __restrict__ scal = a.scal;
+  double alpha = fold ? 0.0 : scal[SC_ALPHA];
+  const double omega = scal[SC_OMEGA], beta = scal[SC_BETA], rho = scal[SC_RHO];
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (MODE != BICG_INIT) {
+    const double st = scal[SC_STATE];
+    if (MODE == BICG_XR && a.dir && st == ST_EARLY) {
+      // x += alpha dir (BiCGStab.h:162-167)
+      for (long i = i0; i < n; i += stride) {
+        const double dv = a.dir_m ? double(((const TM*)a.dir)[i]) : ((const double*)a.dir)[i];
+        a.x[pad_of((int)i)] += dv * alpha;
+      }
+      return;
+    }
+    if (st != ST_RUN) return;
+  }
+  if (fold) {  // FIN_ALPHA: alpha = rho / rhat.dot(ap)
+    const double* const parts[3] = { a.pfold, nullptr, nullptr };
+    double r[3] = { 0.0, 0.0, 0.0 };
+    fold_n<1>(parts, a.nfold, r);
+    alpha = rho / r[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[SC_ALPHA] = alpha;
+  }
+  double sa = 0.0, sb = 0.0;
+  for (long i = i0; i < n; i += stride) {
+    if (MODE == BICG_INIT) {
+      // rhat = r; p = r; sums b.b and rhat.r
+      const int ip = pad_of((int)i);
+      const double r = a.rpad[ip], bv = a.b[ip];
+      a.r[i] = r;
+      a.rhat[i] = r;
+      a.p[i] = r;
+      if (a.w32) a.w32[i] = float(r);
+      sa += bv * bv;
+      sb += r * r;
+    } else if (MODE == BICG_S) {
+      // s = r; s += ap * -alpha; sum s.s
+      double s = a.r[i];
+      s += a.ap[i] * -alpha;
+      a.sv[i] = s;
+      if (a.w32) a.w32[i] = float(s);
+      sa += s * s;
+    } else if (MODE == BICG_XR) {
+      const int ip = pad_of((int)i);
+      a.x[ip] += double(((const TM*)a.mp)[i]) * alpha + double(((const TM*)a.ms)[i]) * omega;
+      double r = a.r[i];
+      r += a.ap[i] * -alpha + a.as[i] * -omega;
+      a.r[i] = r;
+      sa += r * a.rhat[i];
+      sb += r * r;
+    } else {  // BICG_P: p = (p - omega ap) beta + r
+      double pv = a.p[i];
+      pv += a.ap[i] * -omega;
+      const double pn = pv * beta + a.r[i];
+      a.p[i] = pn;
+      if (a.w32) a.w32[i] = float(pn);
+    }
+  }
+  if (MODE == BICG_INIT || MODE == BICG_S || MODE == BICG_XR) {
+    const double r = block_reduce<RED_SUM>(sa);
+    if (threadIdx.x == 0) a.pa[blockIdx.x] = r;
+  }
+  if (MODE == BICG_INIT || MODE == BICG_XR) {
+    const double r = block_reduce<RED_SUM>(sb);
+    if (threadIdx.x == 0) a.pb[blockIdx.x] = r;
+  }
+  if ((MODE == BICG_INIT || MODE == BICG_S || MODE == BICG_XR) && blockIdx.x == 0) {
+    for (int k = (int)gridDim.x + threadIdx.x; k < nslots; k += blockDim.x) {
+      a.pa[k] = 0.0;
+      if (MODE != BICG_S) a.pb[k] = 0.0;
+    }
+  }
+}
+
+template <typename TD, typename TS, bool TO_PADDED>
+__global__ void __launch_bounds__(256) c3_copy_kernel(TD* __restrict__ dst, const TS* __restrict__ src, long n)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (TO_PADDED) dst[pad_of((int)i)] = TD(src[i]);
+  else dst[i] = TD(src[pad_of((int)i)]);
+}
+
+template <typename T, typename TC>
+__global__ void __launch_bounds__(256) c3_interp_kernel(const int32_t* __restrict__ parent,
+                                                        const int32_t* __restrict__ porth, const TC* __restrict__ uc,
+                                                        const T* __restrict__ u, T* __restrict__ out, long n)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int p = (int)(i >> 12), r = (int)(i & 4095);
+  const int x = r & 15, y = (r >> 4) & 15, z = r >> 8;
+  const C3U<T, TC, true> U{ u, uc, parent, porth };
+  out[i] = U.one(p, x, y, z);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) c3_zero_kernel(T* __restrict__ x, long n)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = T(0);
+}
+
+int blocks_for(long n) { return (int)((n + 255) / 256); }
+
+}  // namespace
+
+bool c3_supported(const te_domain* d)
+{
+  if (d->dim != 3 || d->ns[0] != N || d->ns[1] != N || d->ns[2] != N) return false;
+  for (int p = 0; p < d->np; p++)
+    for (int s = 0; s < 6; s++) {
+      const int t = d->patches[p].nbr_type[20 + s];
+      if (t != TE_NBR_NONE && t != TE_NBR_NORMAL) return false;
+    }
+  return true;
+}
+
+int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const void* uin, int dto, void* out,
+                    int mode, int dtc, const void* uc, const te_level_link* link, hipStream_t s)
+{
+  const te_domain* d = pl->g->d;
+  if (d->np == 0) return TE_OK;
+  if (mode == 2 && (!link || !uc)) {
+    set_error("internal: the interpolating sweep needs the coarser level");
+    return TE_ERR_ARG;
+  }
+  const int32_t* par = link ? link->d_parent : nullptr;
+  const int32_t* por = link ? link->d_orth : nullptr;
+#define TE_C3S(T_, TF_, TO_, TC_, M_)                                                                                \
+  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_>), dim3(d->np), dim3(256), 0, s, d->d_patches,         \
+                     pl->d_ptype, (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32), \
+                     (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32),             \
+                     (const TF_*)f, (const T_*)uin, (TO_*)out, (const TC_*)uc, par, por)
+#define TE_C3M(T_, TF_, TO_, TC_)               \
+  do {                                          \
+    if (mode == 0) TE_C3S(T_, TF_, TO_, TC_, 0); \
+    else if (mode == 1) TE_C3S(T_, TF_, TO_, TC_, 1); \
+    else TE_C3S(T_, TF_, TO_, TC_, 2);          \
+  } while (0)
+  if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64 && dtc == TE_F64) TE_C3M(double, double, double, double);
+  else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32 && dtc == TE_F32) TE_C3M(float, float, float, float);
+  else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32 && dtc == TE_F64) TE_C3M(float, float, float, double);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F64 && dtc == TE_F32) TE_C3M(float, double, double, float);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F64 && dtc == TE_F64) TE_C3M(float, double, double, double);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32 && dtc == TE_F32) TE_C3M(float, double, float, float);
+  else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32 && dtc == TE_F64) TE_C3M(float, double, float, double);
+  else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F64 && dtc == TE_F32) TE_C3M(float, float, double, float);
+  else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F64 && dtc == TE_F64) TE_C3M(float, float, double, double);
+  else {
+    set_error("unsupported dtype combination for the 3D patch solve");
+    return TE_ERR_UNSUPPORTED;
+  }
+#undef TE_C3M
+#undef TE_C3S
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int neumann, int dt, int dtf,
+                             const void* f, const void* u, int dtc, void* fc, hipStream_t s)
+{
+  if (d->np == 0) return TE_OK;
+  const int grid = (d->np + 3) / 4;
+#define TE_C3R(T_, TF_, TC_)                                                                                     \
+  hipLaunchKernelGGL((c3_resid_restrict_kernel<T_, TF_, TC_>), dim3(grid), dim3(256), 0, s, d->d_patches,       \
+                     (const T_*)u, (const TF_*)f, (TC_*)fc, link->d_parent, link->d_orth, d->np, neumann)
+  if (dt == TE_F64 && dtf == TE_F64 && dtc == TE_F64) TE_C3R(double, double, double);
+  else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F32) TE_C3R(float, float, float);
+  else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F64) TE_C3R(float, float, double);
+  else if (dt == TE_F32 && dtf == TE_F64 && dtc == TE_F32) TE_C3R(float, double, float);
+  else if (dt == TE_F32 && dtf == TE_F64 && dtc == TE_F64) TE_C3R(float, double, double);
+  else {
+    set_error("unsupported dtype combination for the 3D residual / restriction");
+    return TE_ERR_UNSUPPORTED;
+  }
+#undef TE_C3R
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, double* out, const double* w,
+                       double* pa, double* pb, int nslots, hipStream_t s)
+{
+  if (nslots < 1 || nslots > kFoldMax) {
+    set_error("internal: bad partial count");
+    return TE_ERR_ARG;
+  }
+  int grid = (d->np + 3) / 4;
+  if (grid > nslots) grid = nslots;
+  if (grid < 1) grid = 1;
+#define TE_C3D(TU_, Q_)                                                                                         \
+  hipLaunchKernelGGL((c3_star_dot_kernel<TU_, Q_>), dim3(grid), dim3(256), 0, s, d->d_patches, (const TU_*)u, out, \
+                     w, pa, pb, d->np, nslots, neumann)
+  if (udt == TE_F64) {
+    if (pb) TE_C3D(double, true);
+    else TE_C3D(double, false);
+  } else {
+    if (pb) TE_C3D(float, true);
+    else TE_C3D(float, false);
+  }
+#undef TE_C3D
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hipStream_t s)
+{
+  if (nslots < 1 || nslots > kFoldMax || (a.pfold && (mode != BICG_S || a.nfold < 1 || a.nfold > kFoldMax))) {
+    set_error("internal: bad partial counts for the 3D BiCGStab updates");
+    return TE_ERR_ARG;
+  }
+  const long n = (long)d->np * NC;
+  int grid = blocks_for(n);
+  if (grid > nslots) grid = nslots;
+  if (grid < 1) grid = 1;
+#define TE_C3B(M_)                                                                                        \
+  do {                                                                                                    \
+    if (a.dtm == TE_F32) hipLaunchKernelGGL((c3_bicg_kernel<M_, float>), dim3(grid), dim3(256), 0, s, a, n, nslots);  \
+    else hipLaunchKernelGGL((c3_bicg_kernel<M_, double>), dim3(grid), dim3(256), 0, s, a, n, nslots);    \
+  } while (0)
+  switch (mode) {
+    case BICG_INIT: TE_C3B(BICG_INIT); break;
+    case BICG_S: TE_C3B(BICG_S); break;
+    case BICG_XR: TE_C3B(BICG_XR); break;
+    case BICG_P: TE_C3B(BICG_P); break;
+    default:
+      set_error("internal: unsupported 3D BiCGStab update");
+      return TE_ERR_ARG;
+  }
+#undef TE_C3B
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+template <bool TO_PADDED>
+static int copy_t(const te_domain* d, int dtd, void* dst, int dts, const void* src, hipStream_t s)
+{
+  const long n = (long)d->np * NC;
+  if (n == 0) return TE_OK;
+  const int grid = blocks_for(n);
+#define TE_C3C(TD_, TS_) \
+  hipLaunchKernelGGL((c3_copy_kernel<TD_, TS_, TO_PADDED>), dim3(grid), dim3(256), 0, s, (TD_*)dst, (const TS_*)src, n)
+  if (dtd == TE_F64 && dts == TE_F64) TE_C3C(double, double);
+  else if (dtd == TE_F64 && dts == TE_F32) TE_C3C(double, float);
+  else if (dtd == TE_F32 && dts == TE_F64) TE_C3C(float, double);
+  else TE_C3C(float, float);
+#undef TE_C3C
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_pack(const te_domain* d, int dtd, void* dst, int dts, const void* src_padded, hipStream_t s)
+{
+  return copy_t<false>(d, dtd, dst, dts, src_padded, s);
+}
+
+int launch_c3_unpack(const te_domain* d, int dtd, void* dst_padded, int dts, const void* src, hipStream_t s)
+{
+  return copy_t<true>(d, dtd, dst_padded, dts, src, s);
+}
+
+int launch_c3_interp(const te_level_link* link, int dtc, const void* uc, int dt, const void* u, void* out,
+                     hipStream_t s)
+{
+  const long n = (long)link->fine->np * NC;
+  if (n == 0) return TE_OK;
+  const int grid = blocks_for(n);
+#define TE_C3I(T_, TC_)                                                                                          \
+  hipLaunchKernelGGL((c3_interp_kernel<T_, TC_>), dim3(grid), dim3(256), 0, s, link->d_parent, link->d_orth,     \
+                     (const TC_*)uc, (const T_*)u, (T_*)out, n)
+  if (dt == TE_F64 && dtc == TE_F64) TE_C3I(double, double);
+  else if (dt == TE_F32 && dtc == TE_F64) TE_C3I(float, double);
+  else if (dt == TE_F32 && dtc == TE_F32) TE_C3I(float, float);
+  else {
+    set_error("unsupported dtype combination for the 3D interpolation");
+    return TE_ERR_UNSUPPORTED;
+  }
+#undef TE_C3I
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_zero(long n, int dt, void* x, hipStream_t s)
+{
+  if (n == 0) return TE_OK;
+  if (dt == TE_F64) hipLaunchKernelGGL((c3_zero_kernel<double>), dim3(blocks_for(n)), dim3(256), 0, s, (double*)x, n);
+  else hipLaunchKernelGGL((c3_zero_kernel<float>), dim3(blocks_for(n)), dim3(256), 0, s, (float*)x, n);
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+}  // namespace te

@@ -148,5 +148,9 @@ int launch_restrict(const te_level_link* l, int dtype_in, const void* fine, int 
                     hipStream_t s);
 int launch_interpolate(const te_level_link* l, int dtype_c, const void* coarse, int dtype_f, void* fine, int ncomp,
                        hipStream_t s);
+// boundary data (kernels/bc.hip): values per patch, sum over sides of the face cell counts
+long boundary_stride(const te_domain* d);
+int launch_bc_rhs(const te_domain* d, int dtype, void* f, const double* g, const double* c, bool neumann,
+                  hipStream_t s);
 
 }  // namespace te

@@ -137,6 +137,60 @@ def set_values_with_ghost(domain: R.Domain, vec: R.DeviceArray, *funcs, componen
     _set(domain, vec, funcs, component_index, True, vectorized)
 
 
+def boundary_points(patches, dim: int, ns) -> np.ndarray:
+    """The face points of every side of every patch (``GetRealCoordBound``), in the layout of
+    the boundary data that ``te_star_add_dirichlet_bc`` / ``te_star_add_neumann_bc`` read:
+    per patch, per side (west, east, south, north, bottom, top), the side's face cells with
+    the first of the other axes fastest.  Shape (te_domain_boundary_length, dim)."""
+    starts = np.array([r["starts"][:dim] for r in patches], dtype=np.float64).reshape(-1, dim)
+    h = np.array([r["spacings"][:dim] for r in patches], dtype=np.float64).reshape(-1, dim)
+    sides = []
+    for s in range(2 * dim):
+        a = s >> 1
+        free = [b for b in range(dim) if b != a]
+        idx = np.indices([ns[b] for b in free][::-1]).reshape(len(free), -1)[::-1]
+        pts = np.empty((len(starts), idx.shape[1], dim))
+        pts[:, :, a] = starts[:, a:a + 1] + (h[:, a:a + 1] * ns[a] if s & 1 else 0.0)
+        for k, b in enumerate(free):
+            pts[:, :, b] = starts[:, b:b + 1] + h[:, b:b + 1] / 2.0 + h[:, b:b + 1] * idx[k][None, :]
+        sides.append(pts)
+    return np.concatenate(sides, axis=1).reshape(-1, dim)
+
+
+def boundary_values(domain: R.Domain, func, vectorized: bool = True) -> np.ndarray:
+    """func at the boundary face points of ``domain`` (``boundary_points``)."""
+    pts = boundary_points(domain.patches, domain.dim, domain.ns)
+    return np.ascontiguousarray(_evaluate(func, pts, vectorized), dtype=np.float64)
+
+
+def add_dirichlet_bc(domain: R.Domain, f: R.DeviceArray, gfunc, hfunc=None, vectorized: bool = True):
+    """``Poisson::StarPatchOperator::addDrichletBCToRHS(f, gfunc)`` (and the VarPoisson form
+    with the coefficient function hfunc): the functions are sampled here at the face points
+    and the right-hand side is updated on the device."""
+    g = R.DeviceArray.from_host(boundary_values(domain, gfunc, vectorized))
+    c = R.DeviceArray.from_host(boundary_values(domain, hfunc, vectorized)) if hfunc is not None else None
+    domain.add_dirichlet_bc(f, g, c)
+    R.sync()
+
+
+def add_neumann_bc(domain: R.Domain, f: R.DeviceArray, gfunc_grad, vectorized: bool = True):
+    """``Poisson::StarPatchOperator::addNeumannBCToRHS(f, gfunc, gfunc_grad)``: gfunc_grad is
+    one function per axis (the reference's gradient array); each side uses its axis' one."""
+    pts = boundary_points(domain.patches, domain.dim, domain.ns)
+    stride = len(pts) // max(len(domain.patches), 1)
+    side = np.concatenate([np.full(int(np.prod([domain.ns[b] for b in range(domain.dim) if b != (s >> 1)])), s)
+                           for s in range(2 * domain.dim)])
+    side = np.tile(side, len(domain.patches))
+    assert len(side) == len(pts) and stride * len(domain.patches) == len(pts)
+    dg = np.empty(len(pts))
+    for a in range(domain.dim):
+        m = (side >> 1) == a
+        dg[m] = _evaluate(gfunc_grad[a], pts[m], vectorized)
+    d = R.DeviceArray.from_host(dg)
+    domain.add_neumann_bc(f, d)
+    R.sync()
+
+
 def integrate(domain: R.Domain, u: R.DeviceArray) -> float:
     """``Integrate`` (``DomainTools.h:513-536``): interior values times the patch's cell
     volume, summed over patches and components on the device."""
@@ -152,3 +206,4 @@ class DomainTools:
     SetValues = staticmethod(set_values)
     SetValuesWithGhost = staticmethod(set_values_with_ghost)
     Integrate = staticmethod(integrate)
+    BoundaryPoints = staticmethod(boundary_points)

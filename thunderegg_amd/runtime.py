@@ -55,6 +55,9 @@ SIGNATURES = {
     "te_star_apply": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp]),
     "te_star_apply_patches": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp]),
     "te_varstar_apply": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp]),
+    "te_domain_boundary_length": (C.c_int64, [_vp]),
+    "te_star_add_dirichlet_bc": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "te_star_add_neumann_bc": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
     "te_residual": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, C.c_int, _vp]),
     "te_dft_plan_create": (C.c_int, [_vp, C.c_uint, C.c_int, _vp, C.POINTER(_vp)]),
     "te_dft_plan_destroy": (C.c_int, [_vp]),
@@ -92,6 +95,7 @@ SIGNATURES = {
     "te_gmg_bicgstab": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp, C.c_int, _vp]),
     "te_gmg_cg": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp, C.c_int, _vp]),
     "te_gmg_last_solve_seconds": (C.c_double, [_vp]),
+    "te_gmg_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "te_gmg_set_timer": (C.c_int, [_vp, _vp, C.c_int, _ip]),
     "te_timer_create": (C.c_int, [C.POINTER(_vp)]),
     "te_timer_destroy": (C.c_int, [_vp]),
@@ -250,6 +254,19 @@ class Domain:
         out = C.c_double()
         check(lib().te_vec_inf_norm(self.h, _dt(a), a.ptr, a.shape[1], C.byref(out), stream))
         return out.value
+
+    def boundary_length(self):
+        """Length of the boundary data (te_domain_boundary_length)."""
+        return int(lib().te_domain_boundary_length(self.h))
+
+    def add_dirichlet_bc(self, f: DeviceArray, g: DeviceArray, c: DeviceArray | None = None, stream=None):
+        """StarPatchOperator::addDrichletBCToRHS with sampled boundary data g (and the
+        VarPoisson coefficient c)."""
+        check(lib().te_star_add_dirichlet_bc(self.h, _dt(f), f.ptr, g.ptr, c.ptr if c is not None else None, stream))
+
+    def add_neumann_bc(self, f: DeviceArray, dg: DeviceArray, stream=None):
+        """StarPatchOperator::addNeumannBCToRHS with the sampled normal-axis derivative dg."""
+        check(lib().te_star_add_neumann_bc(self.h, _dt(f), f.ptr, dg.ptr, stream))
 
     def integrate(self, u: DeviceArray, stream=None):
         """DomainTools::Integrate (DomainTools.h:513-536): interior values times cell

@@ -191,6 +191,16 @@ class GMG:
     def last_solve_seconds(self):
         return R.lib().te_gmg_last_solve_seconds(self.h)
 
+    # te_kernel_id: the finest level's kernels of the compact 3D cycle and the solver
+    KERNELS = {"sweep_zero": 0, "sweep": 1, "sweep_interp": 2, "resid_restrict": 3, "operator": 4, "update_xr": 5}
+
+    def time_kernel(self, name, reps=20):
+        """Average milliseconds of `reps` standalone launches of one kernel on the solver's
+        own buffers and stream (te_gmg_time_kernel; overwrites the work buffers)."""
+        ms = C.c_double()
+        R.check(R.lib().te_gmg_time_kernel(self.h, self.KERNELS[name], reps, C.byref(ms)))
+        return ms.value
+
     def set_timer(self, timer: "R.Timer | None", iterations=True, domain_ids=None):
         """Attach a Timer (None detaches): "Iteration" timings of the outer solves
         (iterations=True) and "Total Patch Smooth" domain timings of every sweep of the
