@@ -1701,6 +1701,8 @@ class PatchSolver : public ThunderEgg::PatchSolver<D> {
   int maxIterations() const { return max_it_; }
   double tolerance() const { return tol_; }
   bool continueOnBreakdown() const { return continue_; }
+  /* the per-patch solves run on the device (a built-in operator) */
+  bool onDevice() const { return device_; }
 
  private:
   std::shared_ptr<const PatchOperator<D>> op_;
@@ -1973,28 +1975,35 @@ void Cycle<D>::build_device_cycle()
   std::vector<double> ltol;
   for (size_t i = 0; i < levels_.size(); i++) {
     const Level<D>& L = levels_[i];
+    // only the built-in classes themselves (not subclasses, which may override the per-patch
+    // virtuals) run as the device cycle
     const PatchOperator<D>* po = dynamic_cast<const PatchOperator<D>*>(L.op.get());
-    if (!po) return;
+    if (!po || !po->hasDeviceFiller()) return;
     int ok = -1, nm = 0;
     const double* c = nullptr;
-    if (auto v = dynamic_cast<const VarPoisson::StarPatchOperator<D>*>(po)) {
+    if (typeid(*po) == typeid(VarPoisson::StarPatchOperator<D>)) {
       ok = 1;
-      c = v->getCoefficients().data();
-    } else if (auto p = dynamic_cast<const Poisson::StarPatchOperator<D>*>(po)) {
+      c = static_cast<const VarPoisson::StarPatchOperator<D>*>(po)->getCoefficients().data();
+    } else if (typeid(*po) == typeid(Poisson::StarPatchOperator<D>)) {
       ok = 0;
-      nm = p->neumann() ? 1 : 0;
+      nm = static_cast<const Poisson::StarPatchOperator<D>*>(po)->neumann() ? 1 : 0;
     } else {
       return;
     }
     int sk = -1, im = 1000, ct = -1;
     double it = 1e-12;
-    if (auto s = dynamic_cast<const Poisson::DFTPatchSolver<D>*>(L.smoother.get())) {
+    const GMG::Smoother<D>& sref = *L.smoother;
+    if (typeid(sref) == typeid(Poisson::DFTPatchSolver<D>)) {
+      auto s = dynamic_cast<const Poisson::DFTPatchSolver<D>*>(L.smoother.get());
+      if (!s->device()) return;
       sk = 0;
       const unsigned mk = (unsigned)s->getNeumann().to_ulong();
       if (any_dft && mk != mask) return;  // one Neumann mask for every DFT level
       mask = mk;
       any_dft = true;
-    } else if (auto s = dynamic_cast<const Iterative::PatchSolver<D>*>(L.smoother.get())) {
+    } else if (typeid(sref) == typeid(Iterative::PatchSolver<D>)) {
+      auto s = dynamic_cast<const Iterative::PatchSolver<D>*>(L.smoother.get());
+      if (!s->onDevice()) return;
       sk = 1;
       im = s->maxIterations();
       it = s->tolerance();
@@ -2006,8 +2015,8 @@ void Cycle<D>::build_device_cycle()
       if (cont >= 0 && ct != cont) return;  // one breakdown policy for every level
       cont = ct;
     }
-    if (i + 1 < levels_.size() && !dynamic_cast<const LinearRestrictor<D>*>(L.restrictor.get())) return;
-    if (i > 0 && !dynamic_cast<const DirectInterpolator<D>*>(L.interpolator.get())) return;
+    if (i + 1 < levels_.size() && typeid(*L.restrictor) != typeid(LinearRestrictor<D>)) return;
+    if (i > 0 && typeid(*L.interpolator) != typeid(DirectInterpolator<D>)) return;
     const DeviceGhostFiller<D>& gf = po->deviceFiller();
     if (i == 0) {
       op_kind = ok, filler_kind = gf.kind(), fill_type = (int)gf.getFillType(), neumann = nm;

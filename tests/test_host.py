@@ -248,6 +248,7 @@ def _build_api_test(tmp, name="api_test"):
 
 def test_cpp_api_compiles(tmp_path):
     _build_api_test(tmp_path)
+    _build_api_test(tmp_path, "patch_api_test")
     # every forwarding header compiles on its own
     import subprocess
     inc = os.path.join(ROOT, "include")

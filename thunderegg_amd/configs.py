@@ -85,10 +85,20 @@ def _refine_near_points(forest: Forest, dim, pts, radius, min_level, max_level):
 
 def _gaussian_rhs(levels, dim, ns, pts, amp, wid):
     X = _centers(levels, dim, ns)
+    axes = [np.ascontiguousarray(X[..., a]) for a in range(dim)]
     f = np.zeros(X.shape[:-1])
+    r2 = np.empty_like(f)
+    t = np.empty_like(f)
     for p, a, w in zip(pts, amp, wid):
-        r2 = ((X - p) ** 2).sum(axis=-1)
-        f += a * np.exp(-r2 / (2 * w * w))
+        np.subtract(axes[0], p[0], out=r2)
+        np.multiply(r2, r2, out=r2)
+        for ax in range(1, dim):
+            np.subtract(axes[ax], p[ax], out=t)
+            np.multiply(t, t, out=t)
+            r2 += t
+        np.divide(r2, -(2 * w * w), out=r2)
+        np.exp(r2, out=r2)
+        f += a * r2
     return f[:, None].copy()
 
 
