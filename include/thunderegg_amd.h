@@ -252,6 +252,17 @@ int te_vec_inf_norm(const te_domain* d, int dtype, const void* a, int ncomp, dou
  * the interior values times the patch's cell volume (product of its spacings);
  * synchronous, result in *out. */
 int te_vec_integrate(const te_domain* d, int dtype, const void* u, int ncomp, double* out, te_stream s);
+/* Vector::dot for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The same dot product as te_vec_dot, but its partial sums (one per patch) live in a fixed
+ * device array of the library instead of the domain's reduction workspace, the result is
+ * written to the caller's device double *d_out, and the call is asynchronous on `stream`;
+ * it allocates nothing.  Rejects domains of more than TE_NOALLOC_MAX_PATCHES patches.
+ * The array is one per GPU and shared by every call on that GPU: calls that overlap on one
+ * GPU race. */
+#define TE_NOALLOC_MAX_PATCHES 65536
+int te_vec_dot_noalloc(const te_domain* d, int dtype, const void* a, const void* b, int ncomp, double* d_out,
+                       te_stream stream);
 /* Element-type conversion of a whole padded vector (dst_dtype <- src_dtype). */
 int te_vec_convert(const te_domain* d, int dst_dtype, void* dst, int src_dtype, const void* src, int ncomp,
                    te_stream s);

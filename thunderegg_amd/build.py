@@ -31,6 +31,7 @@ SOURCES = [
     "kernels/krylov.hip",
     "kernels/cycle3d.hip",
     "kernels/bc.hip",
+    "kernels/noalloc.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

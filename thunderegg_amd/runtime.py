@@ -85,6 +85,7 @@ SIGNATURES = {
     "te_vec_scale_then_add_scaled2": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_double, _vp, C.c_double, _vp,
                                                 C.c_int, _vp]),
     "te_vec_dot": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _dp, _vp]),
+    "te_vec_dot_noalloc": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp, _vp]),
     "te_vec_two_norm": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _dp, _vp]),
     "te_vec_inf_norm": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _dp, _vp]),
     "te_vec_integrate": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _dp, _vp]),
