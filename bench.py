@@ -141,6 +141,21 @@ def kernel_rooflines_3d(g, P, its, reps=20):
         if flops:
             d.update({"flops": flops, "TFs": flops / ms / 1e9})
         out[name] = d
+    # a plain copy of the interior of one FP64 vector of the finest level: the reference
+    # point that scales the HBM counters (and the bandwidth a plain stream reaches)
+    import numpy as np
+
+    from thunderegg_amd import runtime as R
+
+    d0 = g.domains[0]
+    stream = R.Stream()
+    a = R.DeviceArray(d0.shape(1), np.float64)
+    c = R.DeviceArray(d0.shape(1), np.float64)
+    a.zero()
+    byt = 2 * np_ * N * 8
+    ms = time_launches(R, lambda: R.check(R.lib().te_vec_copy(d0.h, R.F64, c.ptr, a.ptr, 1, stream.h)), reps, stream)
+    out["vec_copy_f64"] = {"ms": ms, "bytes": byt, "GBs": byt / ms / 1e6, "launches_per_solve": 0,
+                           "ms_per_solve": 0.0}
     return out
 
 
@@ -204,10 +219,13 @@ def cpu_baseline(P, gpu_iters, budget_s=30.0):
     }
 
 
-def run_many_problems(args):
-    """The independent small problems solved concurrently (one stream per problem); value =
-    aggregate finest-level DOF x rounds / wall time, every problem to 1e-10, mixed precision,
-    with the FP64 aggregate beside it."""
+def run_many_problems(args, nprob=None, emit=True):
+    """The independent small problems (512 in full): patch counts log-uniform in [16, 4096],
+    patch sizes 8, 16 and 32.  Batched (te_batch: one launch per cycle and BiCGStab step for
+    all problems of a patch size, one graph per outer iteration) in mixed precision and FP64,
+    beside the same problems solved one te_gmg per problem on concurrent streams (the
+    per-problem GPU solve).  value = aggregate finest-level DOF x rounds / wall time of the
+    batched mixed-precision rounds, every problem to 1e-10."""
     import numpy as np
 
     from thunderegg_amd import configs
@@ -215,25 +233,16 @@ def run_many_problems(args):
     from thunderegg_amd import solver
 
     R.lib()
+    nprob = nprob or args.problems
     t0 = time.perf_counter()
-    probs = [configs.small_adaptive2d_problem(k) for k in range(args.problems)]
+    probs = [configs.small_adaptive2d_problem(k) for k in range(nprob)]
     t_mesh = time.perf_counter() - t0
     dof = sum(P.dof for P in probs)
     log(f"many problems: {len(probs)} problems, {dof} DOF, patch sizes "
         f"{sorted({P.ns[0] for P in probs})}; meshes {t_mesh:.1f}s")
+    bs = [R.DeviceArray.from_host(P.rhs) for P in probs]
 
-    def run(prec):
-        gs = [solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, precision=prec,
-                         use_graph=True) for P in probs]
-        bs = [R.DeviceArray.from_host(P.rhs) for P in probs]
-        xs = [R.DeviceArray(g.shape, np.float64) for g in gs]
-
-        def round_():
-            for x in xs:
-                x.zero()
-            R.sync()
-            return solver.solve_many(gs, bs, xs, tol=1e-10, max_it=500, precond=2, threads=args.threads)
-
+    def timed(round_):
         for _ in range(args.warmup):
             round_()
         R.sync()
@@ -244,30 +253,87 @@ def run_many_problems(args):
         R.sync()
         return time.perf_counter() - t, its
 
-    T32, its32 = run(R.F32)
-    T64, its64 = run(R.F64)
+    def batched(prec):
+        t = time.perf_counter()
+        B = solver.Batch([(P.ns, P.levels) for P in probs], filler_kind=probs[0].filler, precision=prec)
+        setup = time.perf_counter() - t
+        xs = [R.DeviceArray(P.rhs.shape, np.float64) for P in probs]
+
+        def round_():
+            for x in xs:
+                x.zero()
+            R.sync()
+            its, _ = B.solve_device(bs, xs, tol=1e-10, max_it=500, precond=2)
+            return its
+
+        T, its = timed(round_)
+        # every problem's true residual after the last round
+        worst = 0.0
+        for P, b, x in zip(probs, bs, xs):
+            dom = R.Domain(2, P.ns, P.levels[0])
+            filler = R.GhostFiller(dom, P.filler, R.FACES)
+            r = R.DeviceArray(dom.shape(1), np.float64)
+            filler.residual(b, x, r)
+            worst = max(worst, dom.two_norm(r) / dom.two_norm(b))
+        return T, [int(i) for i in its], setup, worst
+
+    def concurrent(prec):
+        gs = [solver.GMG(P.dim, P.ns, P.levels, filler_kind=P.filler, smoother=P.smoother, precision=prec,
+                         use_graph=True) for P in probs]
+        xs = [R.DeviceArray(g.shape, np.float64) for g in gs]
+
+        def round_():
+            for x in xs:
+                x.zero()
+            R.sync()
+            return solver.solve_many(gs, bs, xs, tol=1e-10, max_it=500, precond=2, threads=args.threads)
+
+        T, its = timed(round_)
+        del gs, xs
+        return T, [int(i) for i in its]
+
+    Tb32, itb32, setup32, worst32 = batched(R.F32)
+    log(f"many problems: batched mixed {args.steps} rounds in {Tb32:.3f}s")
+    Tb64, itb64, _, worst64 = batched(R.F64)
+    log(f"many problems: batched fp64 {args.steps} rounds in {Tb64:.3f}s")
+    Tc64, itc64 = concurrent(R.F64)
+    log(f"many problems: per-problem fp64 (concurrent streams) {args.steps} rounds in {Tc64:.3f}s")
+    Tc32, itc32 = concurrent(R.F32)
+    log(f"many problems: per-problem mixed (concurrent streams) {args.steps} rounds in {Tc32:.3f}s")
     out = {
         "metric": METRIC,
-        "value": dof * args.steps / T32,
+        "value": dof * args.steps / Tb32,
         "unit": "DOF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": T32 / args.steps * 1e3,
+        "ms_per_step": Tb32 / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 V-cycle / f64 BiCGStab",
         "data": "synthetic (seeded meshes and right-hand sides)",
-        "config": {"workload": f"{len(probs)} independent 2D adaptive problems, solved concurrently",
-                   "problems": len(probs), "dof": dof, "threads": args.threads,
-                   "patch_sizes": sorted({P.ns[0] for P in probs})},
-        "fp64_dof_per_s": dof * args.steps / T64,
-        "mixed_over_fp64": T64 / T32,
-        "iterations_mixed": its32,
-        "iterations_fp64": its64,
+        "config": {"workload": f"{len(probs)} independent 2D adaptive problems, batched",
+                   "problems": len(probs), "dof": dof,
+                   "patch_sizes": sorted({P.ns[0] for P in probs}),
+                   "patches_min_max": [min(len(P.levels[0]) for P in probs), max(len(P.levels[0]) for P in probs)]},
+        "fp64_dof_per_s": dof * args.steps / Tb64,
+        "mixed_over_fp64": Tb64 / Tb32,
+        "per_problem_fp64_dof_per_s": dof * args.steps / Tc64,
+        "per_problem_mixed_dof_per_s": dof * args.steps / Tc32,
+        "batched_over_per_problem_fp64": Tc64 / Tb32,
+        "concurrent_threads": args.threads,
+        "batch_setup_seconds": setup32,
+        "fp64_iterations_equal_per_problem": itb64 == itc64,
+        "worst_true_residual_mixed": worst32,
+        "worst_true_residual_fp64": worst64,
+        "iterations_mixed": itb32,
+        "iterations_fp64": itb64,
+        "iterations_per_problem_fp64": itc64,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def measure(P, args, R, solver, with_fp64=True):
@@ -350,7 +416,7 @@ def main():
     ap.add_argument("--no-secondary", action="store_true", help="skip the adaptive 2D secondary measurement")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the standalone kernel timings (for counter runs under rocprofv3)")
-    ap.add_argument("--problems", type=int, default=64, help="many problems: number of problems (512 in full)")
+    ap.add_argument("--problems", type=int, default=512, help="many problems: number of problems")
     ap.add_argument("--threads", type=int, default=8, help="many problems: concurrent solves")
     ap.add_argument("--timer", metavar="FILE",
                     help="after the timed solves, one more mixed-precision solve with a Timer attached: its "
@@ -455,6 +521,24 @@ def main():
                             "mixed_true_residual": r2["mixed_true_residual"],
                             "fp64_true_residual": r2["fp64_true_residual"]}
         del g2m, g2d
+        try:
+            r5 = run_many_problems(argparse.Namespace(steps=args.steps, warmup=args.warmup, problems=512,
+                                                      threads=args.threads), emit=False)
+            keep = ("value", "fp64_dof_per_s", "mixed_over_fp64", "per_problem_fp64_dof_per_s",
+                    "per_problem_mixed_dof_per_s", "batched_over_per_problem_fp64",
+                    "fp64_iterations_equal_per_problem", "ms_per_step", "worst_true_residual_mixed",
+                    "worst_true_residual_fp64")
+            out["batched_small_problems"] = {"workload": r5["config"]["workload"], "dof": r5["config"]["dof"],
+                                             **{k: r5[k] for k in keep},
+                                             "iterations_mixed_min_max_sum": [min(r5["iterations_mixed"]),
+                                                                              max(r5["iterations_mixed"]),
+                                                                              sum(r5["iterations_mixed"])],
+                                             "iterations_fp64_min_max_sum": [min(r5["iterations_fp64"]),
+                                                                             max(r5["iterations_fp64"]),
+                                                                             sum(r5["iterations_fp64"])]}
+        except Exception as e:  # the headline line must not depend on this measurement
+            log(f"many problems: failed: {e!r}")
+            out["batched_small_problems"] = {"error": repr(e)}
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(P, out.get("iterations_fp64", out["iterations_mixed"])[0])

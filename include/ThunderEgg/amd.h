@@ -1623,10 +1623,8 @@ class LinearRestrictor : public Restrictor<D> {
  public:
   LinearRestrictor(const Domain<D>& fine_domain, const Domain<D>& coarse_domain,
                    bool extrapolate_boundary_ghosts = false)
-      : fine_(fine_domain), coarse_(coarse_domain)
+      : fine_(fine_domain), coarse_(coarse_domain), extrapolate_(extrapolate_boundary_ghosts)
   {
-    if (extrapolate_boundary_ghosts)
-      throw RuntimeError("LinearRestrictor: boundary-ghost extrapolation is not provided by this build");
     te_level_link* h = nullptr;
     detail::check(te_level_link_create(fine_domain.handle(), coarse_domain.handle(), &h));
     h_ = std::shared_ptr<te_level_link>(h, detail::Deleter<te_level_link, te_level_link_destroy>());
@@ -1635,14 +1633,20 @@ class LinearRestrictor : public Restrictor<D> {
   Vector<D> restrict(const Vector<D>& fine) const override
   {
     Vector<D> coarse(coarse_, fine.getNumComponents());
-    detail::check(te_restrict(h_.get(), TE_F64, fine.data(), coarse.data(), fine.getNumComponents(), nullptr));
+    if (extrapolate_)
+      detail::check(
+        te_restrict_extrapolate(h_.get(), TE_F64, fine.data(), coarse.data(), fine.getNumComponents(), nullptr));
+    else
+      detail::check(te_restrict(h_.get(), TE_F64, fine.data(), coarse.data(), fine.getNumComponents(), nullptr));
     return coarse;
   }
   const Domain<D>& fineDomain() const { return fine_; }
   const Domain<D>& coarseDomain() const { return coarse_; }
+  bool extrapolatesBoundaryGhosts() const { return extrapolate_; }
 
  private:
   Domain<D> fine_, coarse_;
+  bool extrapolate_;
   std::shared_ptr<te_level_link> h_;
 };
 
@@ -2016,6 +2020,11 @@ void Cycle<D>::build_device_cycle()
       cont = ct;
     }
     if (i + 1 < levels_.size() && typeid(*L.restrictor) != typeid(LinearRestrictor<D>)) return;
+    // the device cycle's restriction leaves the coarse ghosts at zero: a restrictor that
+    // extrapolates them runs in the generic cycle
+    if (i + 1 < levels_.size() &&
+        static_cast<const LinearRestrictor<D>&>(*L.restrictor).extrapolatesBoundaryGhosts())
+      return;
     if (i > 0 && typeid(*L.interpolator) != typeid(DirectInterpolator<D>)) return;
     const DeviceGhostFiller<D>& gf = po->deviceFiller();
     if (i == 0) {

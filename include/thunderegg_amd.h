@@ -214,6 +214,12 @@ int te_patch_bicgstab_smooth(const te_ghost_filler* g, int dtype, int op_kind, i
  * GMG/LinearRestrictor.h:101-191), extrapolate_boundary_ghosts = false:
  * coarse := 0 (with ghosts), then 2^D-cell averages / copies. */
 int te_restrict(const te_level_link* l, int dtype, const void* fine, void* coarse, int ncomp, te_stream stream);
+/* The same with extrapolate_boundary_ghosts = true (GMG/LinearRestrictor.h:49-92, 138-155):
+ * the coarse ghost cells over each fine patch's exterior sides get (3 ghost - interior) / 2^D
+ * of its side cells, the coarse corner ghost of its orthant (when it has no neighbour across
+ * that corner) 3 ghost - interior of its corner cell, and copied patches keep their ghosts. */
+int te_restrict_extrapolate(const te_level_link* l, int dtype, const void* fine, void* coarse, int ncomp,
+                            te_stream stream);
 /* MPIInterpolator::interpolate + DirectInterpolator (GMG/MPIInterpolator.h:79-106,
  * GMG/DirectInterpolator.h:60-91): fine += injection of coarse. */
 int te_interpolate(const te_level_link* l, int dtype, const void* coarse, void* fine, int ncomp,
@@ -353,6 +359,59 @@ enum te_kernel_id {
   TE_K_UPDATE_XR = 5        /* outer x and r updates with the dots r.rhat and r.r */
 };
 int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms);
+
+/* ------------------------------------------- batches of independent problems --
+ * Many independent 2D problems solved together.  The reference solves each with its own
+ * GMG::CycleBuilder + Iterative::BiCGStab (GMG/CycleBuilder.h:97-192, GMG/VCycle.h:46-70,
+ * Iterative/BiCGStab.h:111-203); a batch runs exactly those steps for every problem, with one
+ * launch per cycle step and per BiCGStab step for all problems of one patch size and one
+ * graph per outer iteration.  Each problem's inner products have the bits of its own
+ * te_gmg_bicgstab, so its iterations and result are those of solving it alone (with the
+ * cycle's one-launch-per-level forms: TE_NO_TAIL, TE_NO_RRPRE, TE_NO_BSWEEP).
+ * Problem q's hierarchy is levels[first(q)] .. levels[first(q) + nlev[q] - 1], finest first
+ * as te_gmg_create takes it, first(q) = nlev[0] + ... + nlev[q - 1]; every problem uses
+ * `opts`, which must select the V-cycle, the Poisson operator and the DFT smoother.  The
+ * domains must outlive the batch. */
+typedef struct te_batch te_batch;
+int te_batch_create(int nprob, const te_domain* const* levels, const int* nlev, const te_gmg_options* opts,
+                    te_batch** out);
+int te_batch_destroy(te_batch* b);
+/* Patch-size groups of the batch (each runs its own launches, on its own stream). */
+int te_batch_num_groups(const te_batch* b);
+/* Cycle::apply for every problem: f[q], u[q] are problem q's own padded FP64 vectors of its
+ * finest level.  Stream-ordered after `stream`; `stream` waits for the cycle. */
+int te_batch_vcycle(te_batch* b, const double* const* f, double* const* u, te_stream stream);
+/* Iterative::BiCGStab::solve for every problem: b[q], x[q] problem q's own padded FP64
+ * vectors (x[q] the initial guess, overwritten), iterations[q] its iteration count,
+ * residuals[q] (may be null) its last relative residual norm.  A problem whose rho
+ * becomes 0 stops there; the call then returns TE_ERR_BREAKDOWN after all others have
+ * converged (te_last_error names the first such problem).  Synchronous. */
+int te_batch_bicgstab(te_batch* b, const double* const* bv, double* const* x, int use_precond, int max_it, double tol,
+                      int* iterations, double* residuals, te_stream stream);
+
+/* ------------------------------------------------------------ autotuner --
+ * No counterpart in the reference.  Launch variants per patch shape: every variant of a
+ * family computes each patch with the same arithmetic in the same order, only the mapping
+ * of patches onto workgroups and waves differs, so the choice never changes a result.
+ * thunderegg_amd/autotune.py times the variants (te_tune_time_dft) and writes the table,
+ * lines of "family dim n dtype variant"; the library loads tune_table.txt beside itself (or
+ * the file TE_TUNE_TABLE names; TE_TUNE_TABLE=none keeps the hand defaults, variant 0) on
+ * first use. */
+enum te_tune_family {
+  TE_TUNE_DFT_GENERIC = 0, /* VALU DFT patch solve: patches per workgroup x threads per patch */
+  TE_TUNE_DFT_MFMA_2D = 1  /* 2D matrix-core DFT patch solve (n 16, 32): workgroups per launch */
+};
+int te_tune_num_variants(int family);
+int te_tune_load(const char* path);
+int te_tune_clear(void);
+int te_tune_set(int family, int dim, int n, int dtype, int variant);
+int te_tune_get(int family, int dim, int n, int dtype);
+/* the table file loaded ("" for none) into buf; returns its length */
+int te_tune_table_path(char* buf, size_t cap);
+/* average milliseconds of `reps` launches of plan pl's patch solve (u's ghosts read from u)
+ * with variant `variant` forced, after one untimed launch; f, u, out of type dtype */
+int te_tune_time_dft(const te_dft_plan* pl, int dtype, const void* f, const void* u, void* out, int variant, int reps,
+                     double* ms);
 
 /* ------------------------------------------------------------- timings --
  * Timer (Timer.h:28-204, Timer.cpp:160-781): nested named timings, each optionally tied to

@@ -38,6 +38,7 @@ def lib():
         L.ora_bicgstab_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double,
                                           C.c_int, dp, dp, C.c_int, ip]
         L.ora_restrict.argtypes = [vp, vp, dp, dp, C.c_int]
+        L.ora_restrict_ex.argtypes = [vp, vp, dp, dp, C.c_int, C.c_int]
         L.ora_interpolate.argtypes = [vp, vp, dp, dp, C.c_int]
         L.ora_vcycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         L.ora_cycle_apply.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
@@ -158,9 +159,10 @@ def bicgstab_smooth(dom: Domain, f, u, kind=0, fill_type=0, op_kind=0, coeffs=No
     return iters
 
 
-def restrict(fine: Domain, coarse: Domain, fv):
+def restrict(fine: Domain, coarse: Domain, fv, extrapolate=False):
+    """LinearRestrictor::restrict; extrapolate: extrapolate_boundary_ghosts"""
     cv = coarse.zeros(fv.shape[1])
-    _check(lib().ora_restrict(fine.h, coarse.h, _dp(fv), _dp(cv), fv.shape[1]), "restrict")
+    _check(lib().ora_restrict_ex(fine.h, coarse.h, _dp(fv), _dp(cv), fv.shape[1], int(extrapolate)), "restrict")
     return cv
 
 
@@ -187,10 +189,12 @@ class Levels:
             L.op_kind = op_kind
             L.neumann = int(neumann)
             L.coeffs = _dp(coeffs[i]) if coeffs is not None else None
-            L.smoother_kind = smoother_kind
+            # smoother_kind, inner_max_it, inner_tol: one value, or one per level (finest first;
+            # CycleBuilder takes a smoother per level, GMG/CycleBuilder.h:97-160)
+            L.smoother_kind = smoother_kind[i] if np.ndim(smoother_kind) else smoother_kind
             L.neumann_mask = neumann_mask
-            L.inner_max_it = inner_max_it
-            L.inner_tol = inner_tol
+            L.inner_max_it = inner_max_it[i] if np.ndim(inner_max_it) else inner_max_it
+            L.inner_tol = inner_tol[i] if np.ndim(inner_tol) else inner_tol
             L.continue_on_breakdown = int(continue_on_breakdown)
         self.arr = arr
 

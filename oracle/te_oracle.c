@@ -1275,7 +1275,62 @@ int ora_bicgstab_smooth(const ora_domain* d, int kind, int fill_type, int op_kin
 }
 
 /* ------------------------------------------------- restrict / interpolate -- */
+/* LinearRestrictor::extrapolateBoundaries (GMG/LinearRestrictor.h:49-92) for fine patch p
+ * (orthant orth on its parent cp): on every exterior side s of the orthant, each ghost
+ * cell of the fine side adds (3 ghost - interior) / 2^D to the coarse ghost cell over it
+ * (the commented-out hasNbr test: whether or not s has a neighbour), the side's slices
+ * walked first free axis fastest; then, when the patch has no neighbour across the corner
+ * of its orthant, the coarse corner ghost gains 3 ghost - interior of the fine corner. */
+static void extrapolate_boundaries(const ora_domain* fine, const ora_domain* coarse, int p, int cp, int orth,
+                                   const double* fv, double* cv, int nc)
+{
+  const int D = fine->dim;
+  int n[3] = { fine->ns[0], fine->ns[1], D == 3 ? fine->ns[2] : 1 };
+  int st[3] = { 0, 0, 0 };
+  for (int a = 0; a < D; a++) st[a] = ((orth >> a) & 1) ? n[a] : 0;
+  for (int a = 0; a < D; a++) { /* Orthant::getExteriorSides: one side per axis, in axis order */
+    const int up = (orth >> a) & 1;
+    int fa[2], na = 0; /* the free axes, increasing */
+    for (int b = 0; b < D; b++)
+      if (b != a) fa[na++] = b;
+    const int n1 = na > 0 ? n[fa[0]] : 1, n2 = na > 1 ? n[fa[1]] : 1;
+    for (int cc = 0; cc < nc; cc++)
+      for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) {
+          int xg[3] = { 0, 0, 0 }, xi[3] = { 0, 0, 0 }, xc[3] = { 0, 0, 0 };
+          xg[a] = up ? n[a] : -1;
+          xi[a] = up ? n[a] - 1 : 0;
+          xc[a] = xg[a];
+          const int t[2] = { i, j };
+          for (int k = 0; k < na; k++) {
+            xg[fa[k]] = xi[fa[k]] = t[k];
+            xc[fa[k]] = (t[k] + st[fa[k]]) / 2;
+          }
+          const double g = fv[cell_index(fine, p, cc, nc, xg)], in = fv[cell_index(fine, p, cc, nc, xi)];
+          cv[cell_index(coarse, cp, cc, nc, xc)] += (3 * g - in) / (1 << D);
+        }
+  }
+  /* the corner of the orthant: Corner<D>(orth) has the orthant's index (slot orth) */
+  if (fine->p[p].nbr_type[orth] < 0) {
+    int xg[3] = { 0, 0, 0 }, xi[3] = { 0, 0, 0 };
+    for (int a = 0; a < D; a++) {
+      const int up = (orth >> a) & 1;
+      xg[a] = up ? n[a] : -1;
+      xi[a] = up ? n[a] - 1 : 0;
+    }
+    for (int cc = 0; cc < nc; cc++)
+      cv[cell_index(coarse, cp, cc, nc, xg)] += 3 * fv[cell_index(fine, p, cc, nc, xg)] -
+                                                fv[cell_index(fine, p, cc, nc, xi)];
+  }
+}
+
 int ora_restrict(const ora_domain* fine, const ora_domain* coarse, const double* fv, double* cv, int nc)
+{
+  return ora_restrict_ex(fine, coarse, fv, cv, nc, 0);
+}
+
+int ora_restrict_ex(const ora_domain* fine, const ora_domain* coarse, const double* fv, double* cv, int nc,
+                    int extrapolate)
 {
   int D = fine->dim;
   memset(cv, 0, sizeof(double) * ora_domain_vec_len(coarse, nc)); /* coarse.setWithGhost(0) */
@@ -1300,10 +1355,15 @@ int ora_restrict(const ora_domain* fine, const ora_domain* coarse, const double*
             if (orth >= 0) { /* restrictToCoarserParent (:101-128) */
               int xc[3] = { (x + st[0]) / 2, (y + st[1]) / 2, (z + st[2]) / 2 };
               cv[cell_index(coarse, cp, cc, nc, xc)] += v / (1 << D);
-            } else { /* copyToParent (:138-155) */
+            } else if (!extrapolate) { /* copyToParent (:138-155) */
               cv[cell_index(coarse, cp, cc, nc, xf)] += v;
             }
           }
+    if (extrapolate && orth >= 0) extrapolate_boundaries(fine, coarse, p, cp, orth, fv, cv, nc);
+    if (extrapolate && orth < 0) { /* copyToParent with extrapolation: every cell, ghosts included */
+      long len = (long)nc * fine->cells_ghost;
+      for (long k = 0; k < len; k++) cv[(long)cp * nc * coarse->cells_ghost + k] += fv[(long)p * nc * fine->cells_ghost + k];
+    }
   }
   return 0;
 }

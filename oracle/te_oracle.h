@@ -81,6 +81,11 @@ int ora_bicgstab_smooth(const ora_domain* d, int kind, int fill_type, int op_kin
  * GMG/MPIRestrictor.h:55-83), extrapolate_boundary_ghosts = false. */
 int ora_restrict(const ora_domain* fine, const ora_domain* coarse, const double* fine_v,
                  double* coarse_v, int ncomp);
+/* The same with extrapolate_boundary_ghosts (GMG/LinearRestrictor.h:49-92, 138-155) when
+ * extrapolate != 0: the coarse ghost cells over the fine patches' exterior sides and
+ * neighbourless orthant corners get the extrapolated fine ghosts; copies include ghosts. */
+int ora_restrict_ex(const ora_domain* fine, const ora_domain* coarse, const double* fine_v,
+                    double* coarse_v, int ncomp, int extrapolate);
 /* GMG::DirectInterpolator (GMG/DirectInterpolator.h:60-91): fine += P coarse. */
 int ora_interpolate(const ora_domain* coarse, const ora_domain* fine, const double* coarse_v,
                     double* fine_v, int ncomp);

@@ -4,11 +4,13 @@
 // virtuals), the DFT and inner-BiCGStab patch solvers wrapping the user operator, a
 // CycleBuilder cycle of user operators, the boundary-data helpers and the views.  Each is
 // checked against the built-in (device) classes on the same input.  Exit status 0 = pass.
+#include <ThunderEgg/DomainTools.h>
 #include <ThunderEgg/GMG/CycleBuilder.h>
 #include <ThunderEgg/Iterative/BiCGStab.h>
 #include <ThunderEgg/Iterative/PatchSolver.h>
 #include <ThunderEgg/Poisson/DFTPatchSolver.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -292,6 +294,34 @@ int main()
     A.addDrichletBCToRHS(bc, [](const std::array<double, 2>& c) { return 1 + 5 * c[0] + 13 * c[1]; });
     std::printf("addDrichletBCToRHS linear exactness: %.3e\n", rel_diff(ag, bc));
     CHECK(rel_diff(ag, bc) <= 1e-9);
+  }
+
+  // LinearRestrictor with extrapolate_boundary_ghosts, as test/GMG/LinearRestrictor_MPI1.cpp
+  // checks it: a linear function set with ghosts restricts to the same function on the coarse
+  // interior and on the ghosts of every side without a neighbour
+  {
+    auto lin = [](const std::array<double, 2>& c) { return 1 + (c[0] * 0.3 + c[1]); };
+    Vector<2> fine(doms[0], 1), expect(doms[1], 1);
+    DomainTools::SetValuesWithGhost<2>(doms[0], fine, lin);
+    DomainTools::SetValuesWithGhost<2>(doms[1], expect, lin);
+    GMG::LinearRestrictor<2> r(doms[0], doms[1], true);
+    Vector<2> got = r.restrict(fine);
+    double worst = 0.0;
+    int checked = 0;
+    for (const PatchInfo<2>& pi : doms[1].getPatchInfoVector()) {
+      auto g = got.getComponentView(0, pi.local_index);
+      auto e = expect.getComponentView(0, pi.local_index);
+      for (int j = 0; j < 8; j++)
+        for (int i = 0; i < 8; i++) worst = std::max(worst, std::fabs(g(i, j) - e(i, j)));
+      for (Side<2> sd : Side<2>::getValues()) {
+        if (pi.hasNbr(sd)) continue;
+        auto gs = g.getSliceOn(sd, { -1 });
+        auto es = e.getSliceOn(sd, { -1 });
+        for (int t = 0; t < 8; t++, checked++) worst = std::max(worst, std::fabs(gs(t) - es(t)));
+      }
+    }
+    std::printf("LinearRestrictor extrapolation: %d boundary ghosts, max error %.3e\n", checked, worst);
+    CHECK(checked > 0 && worst <= 1e-13);
   }
 
   if (failures) {

@@ -136,6 +136,46 @@ def test_varpoisson2d_full_size():
     assert np.sqrt(((a - c) ** 2).sum() / (c ** 2).sum()) <= 1e-7
 
 
+def test_varpoisson2d_specified_cycle_does_not_converge():
+    """The VarPoisson workload's specified cycle -- the BiCGStab patch smoother on the two
+    finest levels, the DFT patch solver on the coarser ones -- diverges: the DFT solve
+    inverts the constant-coefficient Laplacian (Poisson/DFTPatchSolver.h:420-450 with the
+    operator's modifyRHS), which at a 1e3 beta contrast makes the coarse correction wrong by
+    orders of magnitude.  On a reduced mesh of the same generator (levels 3-6, 166 patches)
+    the oracle's residual grows to ~1e2 after one iteration and to ~1e3 after a few; the
+    device solve does the same (the divergent iteration amplifies rounding-level differences
+    of the inner patch solves, so the histories agree in magnitude, not digit for digit),
+    and so does the device solve at full size.  (The workload therefore runs with the
+    BiCGStab patch smoother on every level, test_varpoisson2d_full_size.)"""
+    _print_case("varpoisson2d specified cycle")
+    P = configs.varpoisson2d_bicgstab_smoother(min_level=3, max_level=6)
+    nl = len(P.levels)
+    kinds = [1, 1] + [0] * (nl - 2)
+    ods = [O.Domain(2, P.ns, lv) for lv in P.levels]
+    cs = []
+    for od, c in zip(ods, P.coeffs):
+        c = c.copy()
+        O.fill_ghost(od, c, kind=P.filler)
+        cs.append(c)
+    ol = O.Levels(ods, filler_kind=P.filler, smoother_kind=kinds, op_kind=1, coeffs=cs, inner_max_it=[50] * nl,
+                  inner_tol=[1e-8] * nl)
+    _, _, href = ol.bicgstab(P.rhs.copy(), tol=1e-10, precond=2, max_it=12)
+    g = solver.GMG(2, P.ns, P.levels, filler_kind=P.filler, smoother=kinds, op_kind=1, coeffs=P.coeffs,
+                   inner_max_it=50, inner_tol=1e-8, precision=R.F64)
+    b, x, its, hist = _solve(P, g, 2, max_it=12)
+    print("reduced mesh: oracle", href[:6], "device", hist[:6], flush=True)
+    assert its == 12
+    assert abs(hist[1] / href[1] - 1.0) < 0.1 and hist[1] > 10.0
+    assert min(hist[3:]) > 100.0 and min(href[3:]) > 100.0
+    Pf = configs.varpoisson2d_bicgstab_smoother()
+    nlf = len(Pf.levels)
+    gf = solver.GMG(2, Pf.ns, Pf.levels, filler_kind=Pf.filler, smoother=[1, 1] + [0] * (nlf - 2), op_kind=1,
+                    coeffs=Pf.coeffs, inner_max_it=50, inner_tol=1e-8, precision=R.F32)
+    _, _, itf, histf = _solve(Pf, gf, 2, max_it=10)
+    print("full size, mixed:", len(Pf.levels[0]), "patches; history", histf, flush=True)
+    assert itf == 10 and histf[-1] > 1.0
+
+
 def test_small_adaptive2d_problems_concurrently():
     """Config 5's independent problems (patch sizes 8, 16 and 32, 30 to 3800 patches) solved
     concurrently, one stream each: the FP64 solves take exactly the iterations of one-at-a-

@@ -226,6 +226,50 @@ def test_varpoisson_gmg():
     assert np.sqrt((r * r).sum()) / np.sqrt((mesh.interior(f, dim) ** 2).sum()) <= 1.5e-10
 
 
+@pytest.mark.parametrize("split", ["bicg_fine", "dft_fine"])
+@pytest.mark.parametrize("op_kind", [0, 1])
+def test_heterogeneous_smoothers(split, op_kind):
+    """A smoother per level (CycleBuilder.h:97-160): the BiCGStab patch solver on the two
+    finest levels and the DFT patch solver below (config 4's cycle), and the reverse, against
+    the oracle's cycle with the same per-level smoothers (FP64; 1e-10, the bar of cycles with
+    inner BiCGStab solves), and the preconditioned solve (FP64 and mixed) to the
+    true-residual bar."""
+    _print_case("heterogeneous", split, op_kind)
+    ns = (16, 16)
+    dim, levels = load("2d_uniform_8x8_refined_cross_mpi1.json", ns)
+    nl = len(levels)
+    kinds = [1 if (k < 2) == (split == "bicg_fine") else 0 for k in range(nl)]
+    assert 0 in kinds and 1 in kinds
+    ods = [O.Domain(dim, ns, lv) for lv in levels]
+    coeffs = None
+    if op_kind == 1:
+        coeffs = []
+        for lv, od in zip(levels, ods):
+            X = mesh.cell_centers(lv, dim, ns)
+            c = np.exp(np.sin(3 * X[..., 0]) * np.cos(4 * X[..., 1]))[:, None].copy()
+            O.fill_ghost(od, c, kind=R.BILINEAR)
+            coeffs.append(c)
+    ol = O.Levels(ods, filler_kind=R.BILINEAR, smoother_kind=kinds, op_kind=op_kind, coeffs=coeffs,
+                  inner_max_it=[200] * nl, inner_tol=[1e-12] * nl)
+    kw = dict(filler_kind=R.BILINEAR, smoother=kinds, op_kind=op_kind, coeffs=coeffs, inner_max_it=[200] * nl,
+              inner_tol=[1e-12] * nl)
+    g = solver.GMG(dim, ns, levels, precision=R.F64, **kw)
+    r0 = random_field(mesh.padded_shape(len(levels[0]), dim, ns), 31)
+    # the inner patch solves stop at 1e-12 on a rounding-dependent iteration (as in
+    # test_patch_bicgstab_smoother), which moves the cycle's result by ~1e-11
+    assert relerr(mesh.interior(g.vcycle(r0), dim), mesh.interior(ol.vcycle(r0.copy()), dim)) <= 1e-10
+    f = _rhs_sin(levels, dim, ns)
+    for prec in (R.F64, R.F32):
+        gp = solver.GMG(dim, ns, levels, precision=prec, **kw)
+        x, its, _ = gp.solve(f, tol=1e-10, precond=2)
+        if op_kind == 0:
+            ax = O.poisson_apply(ods[0], x.copy(), kind=R.BILINEAR)
+        else:
+            ax = O.varpoisson_apply(ods[0], coeffs[0], x.copy(), kind=R.BILINEAR)
+        r = mesh.interior(f - ax, dim)
+        assert np.sqrt((r * r).sum()) / np.sqrt((mesh.interior(f, dim) ** 2).sum()) <= 1.5e-10, (prec, its)
+
+
 @pytest.mark.parametrize("filler", [R.BILINEAR, R.BIQUADRATIC])
 @pytest.mark.parametrize("prec", [R.F64, R.F32])
 @pytest.mark.parametrize("op_kind", [0, 1])

@@ -416,3 +416,33 @@ def test_boundary_data_rhs(name, dim, ns):
             DT.add_dirichlet_bc(dd, fe, lambda c: 1 + sum(coef[a] * c[a] for a in range(dim)))
         a, b = mesh.interior(df.get(), dim), mesh.interior(fe.get(), dim)
         assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max(), neumann
+
+
+@pytest.mark.parametrize("name,ns", [("2d_uniform_4x4_mpi1.json", (10, 2)), ("2d_uniform_2x2_refined_nw_mpi1.json", (4, 4)),
+                                     ("2d_uniform_8x8_refined_cross_mpi1.json", (8, 8)),
+                                     ("3d_refined_bnw_2x2x2_mpi1.json", (4, 4, 4)),
+                                     ("3d_mid_refine_4x4x4_mpi1.json", (2, 4, 6))])
+@pytest.mark.parametrize("ncomp", [1, 2])
+def test_restrict_extrapolate(name, ns, ncomp):
+    """LinearRestrictor with extrapolate_boundary_ghosts (te_restrict_extrapolate) against the
+    oracle, every cell of the coarse vector including its ghosts, FP64 bit-exact (the same
+    terms summed in the same order) and FP32 to rounding; without the flag the ghosts stay 0."""
+    print("case: restrict_extrapolate", name, ns, ncomp, flush=True)
+    dim, levels = load(name, ns)
+    of, oc = O.Domain(dim, ns, levels[0]), O.Domain(dim, ns, levels[1])
+    of.link_parents(oc)
+    df, dc = R.Domain(dim, ns, levels[0]), R.Domain(dim, ns, levels[1])
+    link = R.LevelLink(df, dc)
+    fv = random_field(of.shape(ncomp), 17)
+    ref = O.restrict(of, oc, fv, extrapolate=True)
+    assert np.any(ref[..., 0, 1:-1]) if dim == 2 else np.any(ref[..., 0, 1:-1, 1:-1])
+    dfv = R.DeviceArray.from_host(fv)
+    dcv = R.DeviceArray(oc.shape(ncomp), np.float64)
+    link.restrict_extrapolate(dfv, dcv)
+    assert np.array_equal(dcv.get(), ref)
+    d32 = R.DeviceArray.from_host(fv.astype(np.float32))
+    c32 = R.DeviceArray(oc.shape(ncomp), np.float32)
+    link.restrict_extrapolate(d32, c32)
+    assert np.max(np.abs(c32.get() - ref)) <= 1e-5 * np.max(np.abs(ref))
+    link.restrict(dfv, dcv)
+    assert np.array_equal(dcv.get(), O.restrict(of, oc, fv))

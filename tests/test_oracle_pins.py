@@ -275,6 +275,94 @@ def test_restrict_exact_on_linear(name, ns):
     assert relerr(mesh.interior(cv, dim), mesh.interior(cexp, dim)) <= 1e-13
 
 
+def _physical_side_ghosts(level, dim, ns, v):
+    """(patch, side, ghost values) of every side without a neighbour, 2D"""
+    out = []
+    for p in range(len(level)):
+        for s in range(4):
+            if level["nbr_type"][p][4 + s] >= 0:
+                continue
+            a, up = s >> 1, s & 1
+            if a == 0:
+                out.append((p, s, v[p, :, 1:-1, -1 if up else 0]))
+            else:
+                out.append((p, s, v[p, :, -1 if up else 0, 1:-1]))
+    return out
+
+
+@pytest.mark.parametrize("name", ["2d_uniform_4x4_mpi1.json", "2d_uniform_2x2_refined_nw_mpi1.json"])
+@pytest.mark.parametrize("nx,ny", [(2, 2), (10, 10), (2, 10), (10, 2)])
+@pytest.mark.parametrize("ncomp", [1, 2])
+def test_restrict_extrapolation_known_answer(name, nx, ny, ncomp):
+    """test/GMG/LinearRestrictor_MPI1.cpp "Linear Test LinearRestrictor" (and its two-component
+    case): 1 + 0.3 x + y (and 9 + 0.9 x + 4 y) set with ghosts on the fine level restricts with
+    extrapolate_boundary_ghosts to the same functions on the coarse interior and on the
+    ghosts of every coarse side without a neighbour; without extrapolation those ghosts stay 0."""
+    ns = (nx, ny)
+    dim, levels = load(name, ns)
+    fine, coarse = O.Domain(dim, ns, levels[0]), O.Domain(dim, ns, levels[1])
+    fine.link_parents(coarse)
+    funs = [lambda x, y: 1 + (x * 0.3 + y), lambda x, y: 9 + (x * 0.9 + y * 4)][:ncomp]
+    Xf = mesh.cell_centers(levels[0], dim, ns)
+    Xc = mesh.cell_centers(levels[1], dim, ns)
+    fv = np.stack([f(Xf[..., 0], Xf[..., 1]) for f in funs], axis=1)
+    cexp = np.stack([f(Xc[..., 0], Xc[..., 1]) for f in funs], axis=1)
+    cv = O.restrict(fine, coarse, fv, extrapolate=True)
+    assert relerr(mesh.interior(cv, dim), mesh.interior(cexp, dim)) <= 1e-13
+    ghosts = _physical_side_ghosts(levels[1], dim, ns, cv)
+    assert ghosts
+    for (p, s, got), (_, _, exp) in zip(ghosts, _physical_side_ghosts(levels[1], dim, ns, cexp)):
+        assert np.allclose(got, exp, rtol=1e-13, atol=1e-13), (p, s)
+    plain = O.restrict(fine, coarse, fv)
+    assert all(not np.any(g) for _, _, g in _physical_side_ghosts(levels[1], dim, ns, plain))
+    assert np.array_equal(mesh.interior(plain, dim), mesh.interior(cv, dim))
+
+
+def test_restrict_extrapolation_corners_and_copies():
+    """The other two pieces of extrapolateBoundaries / copyToParent (LinearRestrictor.h:79-91,
+    138-155) on the nw-refined mesh, where the coarse level holds both coarsened patches and
+    copies: a coarsened parent's corner ghost with no neighbour across it gets 3 g - i of the
+    fine corner (for a linear function: twice its value at the coarse corner ghost); a copy
+    keeps every fine value, ghosts included; 3D edge ghosts get nothing."""
+    ns = (4, 4)
+    dim, levels = load("2d_uniform_2x2_refined_nw_mpi1.json", ns)
+    fl, cl = levels[0], levels[1]
+    fine, coarse = O.Domain(dim, ns, fl), O.Domain(dim, ns, cl)
+    fine.link_parents(coarse)
+    fun = lambda x, y: 1 + (x * 0.3 + y)
+    Xf = mesh.cell_centers(fl, dim, ns)
+    Xc = mesh.cell_centers(cl, dim, ns)
+    fv = fun(Xf[..., 0], Xf[..., 1])[:, None]
+    cv = O.restrict(fine, coarse, fv, extrapolate=True)
+    ids = {int(i): k for k, i in enumerate(cl["id"])}
+    seen_corner = seen_copy = False
+    for p in range(len(fl)):
+        cp = ids[int(fl["parent_id"][p])]
+        o = int(fl["orth_on_parent"][p])
+        if o < 0:
+            assert np.array_equal(cv[cp], fv[p])
+            seen_copy = True
+            continue
+        if fl["nbr_type"][p][o] >= 0:
+            continue
+        iy, ix = (-1 if o & 2 else 0), (-1 if o & 1 else 0)
+        assert np.isclose(cv[cp, 0, iy, ix], 2 * fun(Xc[cp, iy, ix, 0], Xc[cp, iy, ix, 1]), rtol=1e-13)
+        seen_corner = True
+    assert seen_corner and seen_copy
+    dim3, lv3 = load("3d_refined_bnw_2x2x2_mpi1.json", (2, 2, 2))
+    f3, c3 = O.Domain(3, (2, 2, 2), lv3[0]), O.Domain(3, (2, 2, 2), lv3[1])
+    f3.link_parents(c3)
+    v3 = O.restrict(f3, c3, random_field(f3.shape(1), 3), extrapolate=True)
+    # edge ghosts (two axes outside the patch, the third inside) of the coarsened parents
+    ids3 = {int(i): k for k, i in enumerate(lv3[1]["id"])}
+    par = sorted({ids3[int(lv3[0]["parent_id"][p])] for p in range(len(lv3[0])) if lv3[0]["orth_on_parent"][p] >= 0})
+    assert par
+    for cp in par:
+        w = v3[cp, 0]
+        for a, b in ((0, 0), (0, -1), (-1, 0), (-1, -1)):
+            assert not np.any(w[a, b, 1:-1]) and not np.any(w[a, 1:-1, b]) and not np.any(w[1:-1, a, b])
+
+
 @pytest.mark.parametrize("name", ["2d_uniform_4x4_mpi1.json", "2d_uniform_2x2_refined_nw_mpi1.json"])
 @pytest.mark.parametrize("nx,ny", [(2, 2), (10, 10), (2, 10)])
 def test_interpolate_known_answer(name, nx, ny):

@@ -1,0 +1,94 @@
+"""Autotuner for the DFT patch-solve launches.
+
+For every patch shape the solvers run with the DFT smoother -- 2D n = 8, 16, 32 and 3D
+n = 8, in FP32 and FP64 -- it times each launch variant of the kernel that shape uses
+(tune.h: patches per workgroup x threads per patch for the VALU kernel, the workgroup count
+of the 2D matrix-core kernel) on a uniform level of about 4-16 thousand patches, with HIP
+events (te_tune_time_dft), checks that the variant's result has the same bits as the hand
+default's, and writes the table the library loads on first use (tune_table.txt beside
+libthunderegg_amd.so).  A variant replaces the default only when it is faster by more than
+`--margin` (3 %).
+
+Usage: python -m thunderegg_amd.autotune [--out PATH] [--reps N] [--margin F]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import runtime as R
+from .solver import uniform_levels
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(HERE, "tune_table.txt")
+
+# (dim, n, uniform level): about 4-16 thousand patches each
+SHAPES = [(2, 8, 7), (2, 16, 7), (2, 32, 6), (3, 8, 4)]
+
+
+def _level(dim, n, level):
+    lv = uniform_levels(dim, level, (n,) * dim)[0]
+    dom = R.Domain(dim, (n,) * dim, lv)
+    filler = R.GhostFiller(dom, R.BILINEAR if dim == 2 else R.TRILINEAR, R.FACES)
+    return dom, filler, R.DftPlan(filler)
+
+
+def tune_shape(dim, n, level, dtype, reps=20, margin=0.03, log=print):
+    """Times every variant on one shape; returns (family, best variant, {variant: ms})."""
+    dom, filler, plan = _level(dim, n, level)
+    fam = R.tune_family(dim, n)
+    npdt = np.float64 if dtype == R.F64 else np.float32
+    rng = np.random.default_rng(1000 * dim + n)
+    shape = dom.shape(1)
+    f = R.DeviceArray.from_host(rng.standard_normal(shape).astype(npdt))
+    u = R.DeviceArray.from_host(rng.standard_normal(shape).astype(npdt))
+    filler.fill_ghost(u)
+    outs = {}
+    times = {}
+    for v in range(R.tune_num_variants(fam)):
+        out = R.DeviceArray(shape, npdt)
+        out.zero()
+        times[v] = plan.time_variant(f, u, out, v, reps)
+        outs[v] = out.get()
+    ref = outs[0]
+    for v, o in outs.items():
+        if not np.array_equal(o, ref):
+            raise RuntimeError(f"variant {v} of family {fam} ({dim}D n={n}) changed the result")
+    best = min(times, key=times.get)
+    if times[best] > times[0] * (1.0 - margin):
+        best = 0
+    log(f"[autotune] {dim}D n={n} {'f64' if dtype == R.F64 else 'f32'} family {fam} ({dom.np} patches): "
+        + ", ".join(f"v{v} {t * 1e3:.1f}us" for v, t in times.items()) + f" -> v{best}")
+    return fam, best, times
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--margin", type=float, default=0.03)
+    args = ap.parse_args(argv)
+    R.lib()
+    R.tune_clear()  # time against the hand defaults, not a table loaded earlier
+    lines = ["# DFT patch-solve launch variants: family dim n dtype variant (thunderegg_amd/autotune.py)",
+             f"# written {time.strftime('%Y-%m-%d')} by: python -m thunderegg_amd.autotune --reps {args.reps} "
+             f"--margin {args.margin}",
+             "# timings per variant in microseconds (v0 = hand default)"]
+    for dim, n, level in SHAPES:
+        for dtype in (R.F32, R.F64):
+            fam, best, times = tune_shape(dim, n, level, dtype, args.reps, args.margin,
+                                          log=lambda *a: print(*a, file=sys.stderr, flush=True))
+            lines.append("# " + " ".join(f"v{v}={t * 1e3:.1f}" for v, t in times.items())
+                         + f"  gain {times[0] / times[best]:.3f}x")
+            lines.append(f"{fam} {dim} {n} {dtype} {best}")
+    with open(args.out, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(args.out)
+
+
+if __name__ == "__main__":
+    main()

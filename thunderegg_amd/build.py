@@ -17,6 +17,8 @@ LIB = os.path.join(HERE, "libthunderegg_amd.so")
 SOURCES = [
     "cabi.cpp",
     "gmg.cpp",
+    "batch.cpp",
+    "tune.cpp",
     "forest.cpp",
     "timer.cpp",
     "kernels/ghost.hip",

@@ -673,6 +673,13 @@ int te_restrict(const te_level_link* l, int dtype, const void* fine, void* coars
   return launch_restrict(l, dtype, fine, dtype, coarse, ncomp, S(s));
 }
 
+int te_restrict_extrapolate(const te_level_link* l, int dtype, const void* fine, void* coarse, int ncomp,
+                            te_stream s)
+{
+  TE_ARG(l && fine && coarse && ncomp >= 1 && ncomp <= 8, "bad argument");
+  return launch_restrict(l, dtype, fine, dtype, coarse, ncomp, S(s), true);
+}
+
 int te_interpolate(const te_level_link* l, int dtype, const void* coarse, void* fine, int ncomp, te_stream s)
 {
   TE_ARG(l && fine && coarse && ncomp >= 1 && ncomp <= 8, "bad argument");

@@ -27,129 +27,10 @@
 #include "kernels/dft.h"
 #include "kernels/krylov.h"
 #include "kernels/tail2d.h"
+#include "gmg_internal.h"
 #include "te_internal.h"
 
 using namespace te;
-
-namespace {
-
-struct GLevel {
-  const te_domain* d = nullptr;
-  te_ghost_filler* gf = nullptr;
-  te_dft_plan* plan = nullptr;
-  te_level_link* link = nullptr;  // this level -> next coarser level
-  int dt = TE_F64;
-  void* f = nullptr;
-  void* u = nullptr;
-  void* r = nullptr;
-  void* u2 = nullptr;  // second solution buffer (fused path ping-pongs between u and u2)
-  void* cur = nullptr; // buffer holding this level's solution after its visit
-  const void* coeffs = nullptr;      // in this level's dtype
-  const double* coeffs64 = nullptr;  // FP64 coefficients (ghost-filled)
-  float* coeffs32 = nullptr;         // owned when dt == F32 and VarPoisson
-  double* coeffs64_own = nullptr;    // ghost-filled copy of the caller's coefficients
-  int smoother = 0;                  // 0 DFTPatchSolver, 1 Iterative::PatchSolver(BiCGStab)
-  int inner_max_it = 1000;           // inner BiCGStab settings (smoother 1)
-  double inner_tol = 1e-12;
-};
-
-size_t esize(int dt) { return dt == TE_F64 ? 8 : 4; }
-
-// Instantiated graphs keyed by the buffers they were captured with, most recently used
-// first; at most kCap are kept (callers that pass fresh buffers every call would otherwise
-// accumulate one per call).
-struct GraphCache {
-  static constexpr size_t kCap = 4;
-  using Key = std::array<uintptr_t, 3>;
-  std::vector<std::pair<Key, hipGraphExec_t>> e;
-  hipGraphExec_t find(const Key& k)
-  {
-    for (size_t i = 0; i < e.size(); i++)
-      if (e[i].first == k) {
-        auto hit = e[i];
-        e.erase(e.begin() + (long)i);
-        e.insert(e.begin(), hit);
-        return hit.second;
-      }
-    return nullptr;
-  }
-  void put(const Key& k, hipGraphExec_t x)
-  {
-    if (e.size() >= kCap) {
-      (void)hipGraphExecDestroy(e.back().second);
-      e.pop_back();
-    }
-    e.insert(e.begin(), { k, x });
-  }
-  void clear()
-  {
-    for (auto& kv : e) (void)hipGraphExecDestroy(kv.second);
-    e.clear();
-  }
-};
-
-}  // namespace
-
-struct te_gmg {
-  te_gmg_options o{};
-  std::vector<GLevel> L;
-  hipStream_t s = nullptr;
-  double *resid = nullptr, *rhat = nullptr, *p = nullptr, *ap = nullptr, *as = nullptr, *sv = nullptr,
-         *ms = nullptr, *mp = nullptr;
-  double *scal = nullptr, *pa = nullptr, *pb = nullptr, *pc = nullptr;  // pc: the s.s partials
-  double* host_scal = nullptr;  // pinned
-  // host-coherent copy of the scalars written by the device at the end of each BiCGStab
-  // iteration (host and device addresses)
-  double* mirror_h = nullptr;
-  double* mirror_d = nullptr;
-  int nblk = 0;
-  int32_t* d_flags = nullptr;
-  // the V-cycle per (f, u); one BiCGStab iteration per (x, use_precond) (b is not read
-  // inside the iteration)
-  GraphCache vgraphs, bgraphs;
-  double last_seconds = 0.0;
-  bool fused = false;  // 2D matrix-core path: ghost fills fused into the stencil / smoother launches
-  std::vector<char> rr_pre;  // per level: restriction fused with the coarser level's first sweep
-  bool fly = false;    // outer operator computes ghosts on the fly
-  // mixed precision on the fused path: M p and M s are FP32 values (the finest level's last
-  // sweep computes in FP32), so the BiCGStab keeps them in FP32 buffers
-  // and hands it FP32 copies of p and s as right-hand sides (written by the p and s updates)
-  bool m32 = false;
-  float* mp32 = nullptr;
-  float* ms32 = nullptr;
-  float* p32 = nullptr;
-  float* s32 = nullptr;
-  // te_gmg_set_timer: "Iteration" timings (time_its) and per-level "Total Patch Smooth"
-  // domain timings (timer_ids[level], TE_TIMER_NONE = untimed), events on s
-  te_timer* timer = nullptr;
-  int time_its = 0;
-  std::vector<int> timer_ids;
-  // fused path: the coarsest levels from tail_from down in one launch (launch_dft_tail),
-  // -1 when none qualify
-  int tail_from = -1;
-  TailArgs tail{};
-  // fused mixed-precision path: the BiCGStab s and p updates run the finest level's
-  // zero-guess sweep too (launch_bicg_sweep); TE_NO_BSWEEP=1 keeps them apart
-  bool bsweep = false;
-  // stream ordering with the caller's stream (no host blocking)
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  // 3D hierarchies of 16^3 patches whose face neighbours are all on the same level: the
-  // cycle and the BiCGStab work on compact vectors (interior cells only, cycle3d.h)
-  bool c3 = false;
-  struct C3Level {
-    void* f = nullptr;
-    void* u = nullptr;
-    void* u2 = nullptr;
-    void* cur = nullptr;  // the buffer holding this level's solution after its visit
-  };
-  std::vector<C3Level> C3;
-  double *c_r = nullptr, *c_rhat = nullptr, *c_p = nullptr, *c_ap = nullptr, *c_as = nullptr, *c_s = nullptr;
-  void* c_mp = nullptr;  // M p, M s: the finest level's type (FP32 in mixed precision)
-  void* c_ms = nullptr;
-  float* c_p32 = nullptr;  // FP32 copies of p and s (mixed precision: the cycle's input)
-  float* c_s32 = nullptr;
-  void* c_tmp = nullptr;   // the cycle's output for padded callers
-};
 
 namespace {
 
@@ -162,6 +43,25 @@ namespace {
 int zero_with_ghost(te_gmg* g, int li, void* v, int dt)
 {
   return launch_vec_op(g->L[li].d, dt, VOP_SET_WITH_GHOST, v, nullptr, nullptr, 0.0, 0, 0, 1, g->s);
+}
+
+// padded elements of one patch of level li
+long patch_len(const te_gmg* g, int li)
+{
+  const Layout L = make_layout(g->L[li].d->dim, g->L[li].d->ns, 1, 1);
+  return L.ps;
+}
+
+// batch mode: the right-hand side of the finest-level patches of level li (li >= 1) from the
+// cycle's input, after the restriction from level li - 1 has written the rest of its f
+int batch_scatter(te_gmg* g, int li)
+{
+  if (!g->batch || g->bseg[li].count == 0) return TE_OK;
+  const te_gmg::BatchSeg& sg = g->bseg[li];
+  GLevel& Lv = g->L[li];
+  const long pl = patch_len(g, li);
+  return launch_convert((long)sg.count * pl, Lv.dt, (char*)Lv.f + esize(Lv.dt) * (size_t)(sg.from * pl), g->bf_dt,
+                        (const char*)g->bf + esize(g->bf_dt) * (size_t)(sg.foff * pl), g->s);
 }
 
 // start (stop = false) or stop the "Total Patch Smooth" timing of level li's sweep
@@ -251,6 +151,7 @@ int visit(te_gmg* g, int li, const void* f, int dtf, double* u_ext, bool u_zero 
     return launch_restrict(Lv.link, Lv.dt, Lv.r, Cv.dt, Cv.f, 1, g->s);
   };
   TE_RC(restrict_to_coarse());
+  TE_RC(batch_scatter(g, li + 1));
   TE_RC(visit(g, li + 1, Cv.f, Cv.dt, u_ext, true, false));
   TE_RC(launch_interpolate(Lv.link, Cv.dt, Cv.u, Lv.dt, u, 1, g->s));
   if (g->o.cycle_type != TE_CYCLE_V && !vonly) {
@@ -343,6 +244,7 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
   } else {
     TE_RC(launch_resid_restrict(Lv.gf, Lv.link, g->o.op_kind, g->o.neumann, Lv.coeffs, Lv.dt, dtf, f, buf[cur],
                                 Cv.dt, Cv.f, g->s));
+    TE_RC(batch_scatter(g, li + 1));
     TE_RC(visit_fused(g, li + 1, Cv.f, Cv.dt, u_ext, ext_dt, false));
   }
   int n = g->o.post_sweeps;
@@ -440,7 +342,30 @@ int vcycle_c3_padded(te_gmg* g, const double* f, double* u)
 // and its last sweep computes u in FP32; see visit_fused).
 // l0_done: the finest level's zero-guess sweep already ran (launch_bicg_sweep), its result
 // in L[0].u.
+int vcycle_enqueue_one(te_gmg* g, const void* f, void* u, int udt, int dtf, bool l0_done);
+
 int vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt = TE_F64, int dtf = TE_F64, bool l0_done = false)
+{
+  if (!g->batch) return vcycle_enqueue_one(g, f, u, udt, dtf, l0_done);
+  g->bf = f;
+  g->bf_dt = dtf;
+  int rc = vcycle_enqueue_one(g, f, u, udt, dtf, false);
+  g->bf = nullptr;
+  TE_RC(rc);
+  // the results of the finest-level patches below level 0 (level 0's last sweep wrote u)
+  for (size_t li = 1; li < g->L.size(); li++) {
+    const te_gmg::BatchSeg& sg = g->bseg[li];
+    if (sg.count == 0) continue;
+    const GLevel& Lv = g->L[li];
+    const void* src = g->fused ? Lv.cur : Lv.u;
+    const long pl = patch_len(g, (int)li);
+    TE_RC(launch_convert((long)sg.count * pl, udt, (char*)u + esize(udt) * (size_t)(sg.foff * pl), Lv.dt,
+                         (const char*)src + esize(Lv.dt) * (size_t)(sg.from * pl), g->s));
+  }
+  return TE_OK;
+}
+
+int vcycle_enqueue_one(te_gmg* g, const void* f, void* u, int udt, int dtf, bool l0_done)
 {
   GLevel& L0 = g->L[0];
   if (g->fused) return visit_fused(g, 0, f, dtf, u, udt, l0_done);
@@ -564,6 +489,7 @@ void free_gmg(te_gmg* g)
     (void)hipFree(c.u2);
   }
   for (void* v : { (void*)g->c_r, (void*)g->c_rhat, (void*)g->c_p, (void*)g->c_ap, (void*)g->c_as, (void*)g->c_s,
+                   (void*)g->c_x,
                    g->c_mp, g->c_ms, (void*)g->c_p32, (void*)g->c_s32, g->c_tmp })
     (void)hipFree(v);
   if (g->host_scal) (void)hipHostFree(g->host_scal);
@@ -576,10 +502,8 @@ void free_gmg(te_gmg* g)
 
 }  // namespace
 
-extern "C" {
-
-int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_options* opts,
-                  const double* const* coeffs, te_gmg** out)
+int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_options* opts,
+                   const double* const* coeffs, bool batch, te_gmg** out)
 {
   if (!levels || num_levels < 1 || !opts || !out) {
     set_error("te_gmg_create: bad argument");
@@ -615,6 +539,8 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     return TE_ERR_ARG;
   }
   auto g = new te_gmg();
+  g->batch = batch;
+  if (batch) g->bseg.assign(num_levels, te_gmg::BatchSeg{});
   g->o = *opts;
   g->o.level_smoother = nullptr;  // the caller's arrays are not kept (copied into the levels)
   g->o.level_inner_max_it = nullptr;
@@ -635,7 +561,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     // keep the one-launch-per-step cycle, for comparison)
     const char* nf = std::getenv("TE_NO_FUSE");
     const char* valu = std::getenv("TE_DFT_VALU");
-    bool c3 = !(nf && nf[0] == '1') && !(valu && valu[0] == '1') && opts->cycle_type == TE_CYCLE_V &&
+    bool c3 = !batch && !(nf && nf[0] == '1') && !(valu && valu[0] == '1') && opts->cycle_type == TE_CYCLE_V &&
               opts->op_kind == 0;
     for (int i = 0; i < num_levels && c3; i++) {
       const int sk = opts->level_smoother ? opts->level_smoother[i] : opts->smoother;
@@ -732,7 +658,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     g->fly = allow && star_dot_fly_supported(g->L[0].gf);
     g->m32 = g->fused && g->fly && g->L[0].dt == TE_F32;
     const char* nr = std::getenv("TE_NO_RRPRE");
-    const bool allow_rp = fused && !(nr && nr[0] == '1');
+    const bool allow_rp = fused && !batch && !(nr && nr[0] == '1');
     g->rr_pre.assign(num_levels, 0);
     // Only where the coarser level is small: there the launch and its dependent loads
     // dominate; on large levels the separate launches keep all four waves of a workgroup
@@ -747,7 +673,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     // levels (two or more) fit the tail's LDS; Poisson, 16 x 16 patches.  TE_NO_TAIL=1
     // keeps the per-level launches (same arithmetic, for comparison).
     const char* nt = std::getenv("TE_NO_TAIL");
-    if (fused && opts->op_kind == 0 && !(nt && nt[0] == '1') && levels[0]->dim == 2 && levels[0]->ns[0] == 16 &&
+    if (fused && !batch && opts->op_kind == 0 && !(nt && nt[0] == '1') && levels[0]->dim == 2 && levels[0]->ns[0] == 16 &&
         levels[0]->ns[1] == 16) {
       for (int t = 1; t + 1 < num_levels; t++) {
         const int nl = num_levels - t;
@@ -785,12 +711,12 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
       }
     }
     const char* nb = std::getenv("TE_NO_BSWEEP");
-    g->bsweep = g->m32 && num_levels > 1 && opts->pre_sweeps >= 1 && bicg_sweep_supported(g->L[0].plan) &&
-                !(nb && nb[0] == '1');
+    g->bsweep = !batch && g->m32 && num_levels > 1 && opts->pre_sweeps >= 1 &&
+                bicg_sweep_supported(g->L[0].plan) && !(nb && nb[0] == '1');
   }
   if (!rc) {
     const te_domain* d0 = levels[0];
-    long n = te_domain_vector_length(d0, 1);
+    long n = batch ? 1 : te_domain_vector_length(d0, 1);  // a batch owns its outer vectors
     size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 1);
     double** vs[] = { &g->resid, &g->rhat, &g->p, &g->ap, &g->as, &g->sv, &g->ms, &g->mp };
     for (double** v : vs) {
@@ -807,7 +733,7 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
     if (g->c3 && e == hipSuccess) {
       const size_t nc = (size_t)std::max<long>((long)d0->np * kC3Cells, 1);
       const size_t em = esize(g->L[0].dt);
-      for (double** v : { &g->c_r, &g->c_rhat, &g->c_p, &g->c_ap, &g->c_as, &g->c_s }) {
+      for (double** v : { &g->c_r, &g->c_rhat, &g->c_p, &g->c_ap, &g->c_as, &g->c_s, &g->c_x }) {
         if (e == hipSuccess) e = hipMalloc(v, sizeof(double) * nc);
         if (e == hipSuccess) e = hipMemset(*v, 0, sizeof(double) * nc);
       }
@@ -848,6 +774,21 @@ int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_o
   }
   *out = g;
   return TE_OK;
+}
+
+void te::gmg_free(te_gmg* g) { free_gmg(g); }
+
+int te::gmg_vcycle_enqueue(te_gmg* g, const void* f, void* u, int udt, int dtf)
+{
+  return vcycle_enqueue(g, f, u, udt, dtf, false);
+}
+
+extern "C" {
+
+int te_gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_options* opts,
+                  const double* const* coeffs, te_gmg** out)
+{
+  return gmg_create(levels, num_levels, opts, coeffs, false, out);
 }
 
 int te_gmg_destroy(te_gmg* g)
@@ -939,7 +880,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   auto iteration_c3 = [&]() -> int {
     C3Bicg a{};
     a.scal = g->scal;
-    a.x = x;
+    a.x = g->c_x;
     a.r = g->c_r;
     a.rhat = g->c_rhat;
     a.p = g->c_p;
@@ -1039,6 +980,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.scal = g->scal;
     a.b = b;
     a.rpad = g->resid;
+    a.x = g->c_x;
+    a.xpad = x;
     a.r = g->c_r;
     a.rhat = g->c_rhat;
     a.p = g->c_p;
@@ -1104,6 +1047,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
       if (history && num_its < history_cap) history[num_its] = residual;
     }
   }
+  if (g->c3) TE_RC(launch_c3_unpack(d, TE_F64, x, TE_F64, g->c_x, s));  // the compact x back
   TE_HIP(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
   g->last_seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1244,7 +1188,7 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
       default: {
         C3Bicg a{};
         a.scal = g->scal;
-        a.x = g->resid;  // a padded FP64 work vector stands in for x
+        a.x = g->c_x;
         a.r = g->c_r;
         a.rhat = g->c_rhat;
         a.ap = g->c_ap;

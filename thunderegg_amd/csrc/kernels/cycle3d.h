@@ -36,15 +36,17 @@ int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int 
 int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, double* out, const double* w,
                        double* pa, double* pb, int nslots, hipStream_t s);
 // BiCGStab vector updates (Iterative/BiCGStab.h:111-203) on compact vectors, the modes and
-// scalar slots of krylov.h.  x and b are the caller's padded FP64 vectors.
-//   BICG_INIT: r_pad (padded residual b - A x) -> r, rhat, p (+ p32); sums b.b, r.r
+// scalar slots of krylov.h.  b and xpad are the caller's padded FP64 vectors; x is a compact
+// FP64 copy of the caller's x that the iterations update (written back at the end).
+//   BICG_INIT: r_pad (padded residual b - A x) -> r, rhat, p (+ p32), xpad -> x; sums b.b, r.r
 //   BICG_S:    s = r - alpha ap (+ s32), sum s.s (alpha folded from pfold when given)
 //   BICG_XR:   x += alpha mp + omega ms; r -= alpha ap + omega as; sums r.rhat, r.r; after
 //              the s-norm test fired: x += alpha dir (dir of type dtm when dir_m, else FP64)
 //   BICG_P:    p = r + beta (p - omega ap) (+ p32)
 struct C3Bicg {
   double* scal;
-  double* x;         // padded FP64 (XR)
+  double* x;         // compact FP64 (INIT writes it from xpad, XR updates it)
+  const double* xpad;  // padded FP64 (INIT)
   const double* b;   // padded FP64 (INIT)
   const double* rpad;  // padded FP64 (INIT)
   double* r;

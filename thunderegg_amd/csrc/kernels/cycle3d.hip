@@ -155,56 +155,98 @@ __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restric
   }
   const DevPatch& P = dp[p];
   const C3U<T, TC, MODE == 2> U{ uin, uc, parent, porth };
-
-  // ---- forward transforms along x and y, one z-slice at a time
+  // the six face neighbours (the patch itself where a side is physical: its loads are then
+  // masked off) and, interpolating, their parents and orthants, read once per workgroup
+  int q[6];
+  bool has[6];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int z = w + 4 * i;
-    T F[4];
-    ld4(f, p * NC + cidx(4 * g, c, z), F);
-    if (MODE != 0) {
-      // modifyRHSForInternalBoundaryConditions (Poisson/StarPatchOperator.h:165-180): on
-      // every side with a neighbour, f_b -= (ghost + u_b) / h^2, sides in the reference's order
-      const T ihx = T(P.inv_h2[0]), ihy = T(P.inv_h2[1]), ihz = T(P.inv_h2[2]);
-      if (g == 0 && P.type[20] >= 0) F[0] -= (U.one(P.nbr[20][0], N - 1, c, z) + U.one(p, 0, c, z)) * ihx;
-      if (g == 3 && P.type[21] >= 0) F[3] -= (U.one(P.nbr[21][0], 0, c, z) + U.one(p, N - 1, c, z)) * ihx;
-      if (c == 0 && P.type[22] >= 0) {
-        T gq[4], ub[4];
-        U.row4(P.nbr[22][0], 4 * g, N - 1, z, gq);
-        U.row4(p, 4 * g, 0, z, ub);
+  for (int s = 0; s < 6; s++) {
+    has[s] = P.type[20 + s] >= 0;
+    q[s] = has[s] ? P.nbr[20 + s][0] : p;
+  }
+  int qpar[7] = { 0, 0, 0, 0, 0, 0, 0 }, qo[7] = { -1, -1, -1, -1, -1, -1, -1 };  // [6]: the patch itself
+  if (MODE == 2) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) F[k] -= (gq[k] + ub[k]) * ihy;
-      }
-      if (c == N - 1 && P.type[23] >= 0) {
-        T gq[4], ub[4];
-        U.row4(P.nbr[23][0], 4 * g, 0, z, gq);
-        U.row4(p, 4 * g, N - 1, z, ub);
+    for (int s = 0; s < 7; s++) {
+      const int qq = s < 6 ? q[s] : p;
+      qpar[s] = parent[qq];
+      qo[s] = porth[qq];
+    }
+  }
+  // cells x0 .. x0 + 3 of row (y, z) of patch qq (neighbour slot s, 6 = the patch), u + P uc
+  // when interpolating (C3U::row4 with the parent and orthant already in registers)
+  auto row = [&](int s, int qq, int x0, int y, int z, T(&r)[4]) {
+    ld4(uin, qq * NC + cidx(x0, y, z), r);
+    if (MODE == 2) {
+      const int o = qo[s];
+      const TC* b = uc + qpar[s] * NC;
+      if (o >= 0) {
+        const int k = U.coarse(o, x0, y, z);
+        const T a0 = T(b[k]), a1 = T(b[k + 1]);
+        r[0] = T(r[0] + a0);
+        r[1] = T(r[1] + a0);
+        r[2] = T(r[2] + a1);
+        r[3] = T(r[3] + a1);
+      } else {
+        T a[4];
+        ld4(b, cidx(x0, y, z), a);
 #pragma unroll
-        for (int k = 0; k < 4; k++) F[k] -= (gq[k] + ub[k]) * ihy;
-      }
-      if (z == 0 && P.type[24] >= 0) {
-        T gq[4], ub[4];
-        U.row4(P.nbr[24][0], 4 * g, c, N - 1, gq);
-        U.row4(p, 4 * g, c, 0, ub);
-#pragma unroll
-        for (int k = 0; k < 4; k++) F[k] -= (gq[k] + ub[k]) * ihz;
-      }
-      if (z == N - 1 && P.type[25] >= 0) {
-        T gq[4], ub[4];
-        U.row4(P.nbr[25][0], 4 * g, c, 0, gq);
-        U.row4(p, 4 * g, c, N - 1, ub);
-#pragma unroll
-        for (int k = 0; k < 4; k++) F[k] -= (gq[k] + ub[k]) * ihz;
+        for (int k = 0; k < 4; k++) r[k] = T(r[k] + a[k]);
       }
     }
-    V d1 = { 0, 0, 0, 0 };  // D1[y][kx] = sum_x F[y][x] Sx[kx][x]
+  };
+  const T ihx = T(P.inv_h2[0]), ihy = T(P.inv_h2[1]), ihz = T(P.inv_h2[2]);
+  // which of this lane's cells lie on a face with a neighbour (x: the lane's first or last
+  // cell, y: its row, z: its slice)
+  const bool onx = (g == 0 && has[0]) || (g == 3 && has[1]);
+  const bool ony = (c == 0 && has[2]) || (c == N - 1 && has[3]);
+  const int sx = g == 0 ? 0 : 1, sy = c == 0 ? 2 : 3;
+
+  // ---- forward transforms along x and y, one z-slice at a time; the loads of a batch of
+  // SB slices (f, and for the interface terms the lane's own boundary rows and the
+  // neighbours' facing rows) are all issued before its first product
+  constexpr int SB = MODE == 0 ? 4 : (sizeof(T) == 4 ? 2 : 1);
 #pragma unroll
-    for (int s = 0; s < 4; s++) d1 = Mf<T>::mma(F[s], SxB[s], d1);
-    V d2 = { 0, 0, 0, 0 };  // D2[ky][kx] = sum_y Sy[ky][y] D1[y][kx]
+  for (int i0 = 0; i0 < 4; i0 += SB) {
+    T F[SB][4], ub[SB][4], gx[SB][4], gy[SB][4], gz[SB][4];
 #pragma unroll
-    for (int s = 0; s < 4; s++) d2 = Mf<T>::mma(SyA[s], d1[s], d2);
+    for (int j = 0; j < SB; j++) {
+      const int z = w + 4 * (i0 + j);
+      ld4(f, p * NC + cidx(4 * g, c, z), F[j]);
+      if (MODE != 0) {
+        const bool onz = (z == 0 && has[4]) || (z == N - 1 && has[5]);
+        if (onx || ony || onz) row(6, p, 4 * g, c, z, ub[j]);
+        if (onx) row(sx, q[sx], g == 0 ? 12 : 0, c, z, gx[j]);
+        if (ony) row(sy, q[sy], 4 * g, c == 0 ? N - 1 : 0, z, gy[j]);
+        if (onz) row(z == 0 ? 4 : 5, q[z == 0 ? 4 : 5], 4 * g, c, z == 0 ? N - 1 : 0, gz[j]);
+      }
+    }
 #pragma unroll
-    for (int r = 0; r < 4; r++) slab[z * kZP + Mf<T>::row(r, g) * kYP + c] = d2[r];
+    for (int j = 0; j < SB; j++) {
+      const int z = w + 4 * (i0 + j);
+      if (MODE != 0) {
+        // modifyRHSForInternalBoundaryConditions (Poisson/StarPatchOperator.h:165-180): on
+        // every side with a neighbour, f_b -= (ghost + u_b) / h^2, sides in the reference's order
+        if (g == 0 && has[0]) F[j][0] -= (gx[j][3] + ub[j][0]) * ihx;
+        if (g == 3 && has[1]) F[j][3] -= (gx[j][0] + ub[j][3]) * ihx;
+        if (ony) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) F[j][k] -= (gy[j][k] + ub[j][k]) * ihy;
+        }
+        if ((z == 0 && has[4]) || (z == N - 1 && has[5])) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) F[j][k] -= (gz[j][k] + ub[j][k]) * ihz;
+        }
+      }
+      V d1 = { 0, 0, 0, 0 };  // D1[y][kx] = sum_x F[y][x] Sx[kx][x]
+#pragma unroll
+      for (int s = 0; s < 4; s++) d1 = Mf<T>::mma(F[j][s], SxB[s], d1);
+      V d2 = { 0, 0, 0, 0 };  // D2[ky][kx] = sum_y Sy[ky][y] D1[y][kx]
+#pragma unroll
+      for (int s = 0; s < 4; s++) d2 = Mf<T>::mma(SyA[s], d1[s], d2);
+#pragma unroll
+      for (int r = 0; r < 4; r++) slab[z * kZP + Mf<T>::row(r, g) * kYP + c] = d2[r];
+    }
   }
   __syncthreads();
 
@@ -455,7 +497,7 @@ __global__ void __launch_bounds__(256) c3_bicg_kernel(C3Bicg a, long n, int nslo
       // x += alpha dir (BiCGStab.h:162-167)
       for (long i = i0; i < n; i += stride) {
         const double dv = a.dir_m ? double(((const TM*)a.dir)[i]) : ((const double*)a.dir)[i];
-        a.x[pad_of((int)i)] += dv * alpha;
+        a.x[i] += dv * alpha;
       }
       return;
     }
@@ -474,6 +516,7 @@ __global__ void __launch_bounds__(256) c3_bicg_kernel(C3Bicg a, long n, int nslo
       // rhat = r; p = r; sums b.b and rhat.r
       const int ip = pad_of((int)i);
       const double r = a.rpad[ip], bv = a.b[ip];
+      a.x[i] = a.xpad[ip];
       a.r[i] = r;
       a.rhat[i] = r;
       a.p[i] = r;
@@ -488,8 +531,7 @@ __global__ void __launch_bounds__(256) c3_bicg_kernel(C3Bicg a, long n, int nslo
       if (a.w32) a.w32[i] = float(s);
       sa += s * s;
     } else if (MODE == BICG_XR) {
-      const int ip = pad_of((int)i);
-      a.x[ip] += double(((const TM*)a.mp)[i]) * alpha + double(((const TM*)a.ms)[i]) * omega;
+      a.x[i] += double(((const TM*)a.mp)[i]) * alpha + double(((const TM*)a.ms)[i]) * omega;
       double r = a.r[i];
       r += a.ap[i] * -alpha + a.as[i] * -omega;
       a.r[i] = r;

@@ -1,8 +1,9 @@
 // Poisson::DFTPatchSolver::solveSinglePatch (Poisson/DFTPatchSolver.h:420-450) for
-// every patch of a level in one launch, one workgroup per patch, generic
-// (VALU) form for any n.  The patch is staged in LDS; each axis transform
-// y = M x (executePlan, :205-265) is a dense n x n product over all lines of
-// that axis; eigenvalues (:320-361) come from a per-axis sin^2 table.
+// every patch of a level in one launch, generic (VALU) form for any n: a workgroup holds
+// PPW patches, TPP threads each (a launch variant chosen by the autotuner's table, tune.h).
+// Each patch is staged in LDS; each axis transform y = M x (executePlan, :205-265) is a
+// dense n x n product over all lines of that axis; eigenvalues (:320-361) come from a
+// per-axis sin^2 table.
 //
 // Template parameters: T = arithmetic / u type, TF = type of f, TO = type of
 // the output.  UZERO: u is known to be identically zero (the first smoothing
@@ -11,6 +12,7 @@
 #include "../te_internal.h"
 #include "dft.h"
 #include "dft_wave.h"
+#include "../tune.h"
 
 namespace te {
 namespace {
@@ -19,14 +21,19 @@ template <typename T, typename TF, typename TO, int D, int OP, bool UZERO>
 __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ dp, const int* __restrict__ ptype,
                                                   const T* __restrict__ mats, const T* __restrict__ s2tab,
                                                   const TF* __restrict__ f, const T* __restrict__ u,
-                                                  const T* __restrict__ coeffs, TO* __restrict__ out, Layout L)
+                                                  const T* __restrict__ coeffs, TO* __restrict__ out, Layout L,
+                                                  int ppw)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
-  const int p = blockIdx.x;
   const int n0 = L.n[0], n1 = L.n[1], n2 = D == 3 ? L.n[2] : 1;
   const int N = n0 * n1 * n2;
   const int nmax = n0 > n1 ? (n0 > n2 ? n0 : n2) : (n1 > n2 ? n1 : n2);
+  // this thread's patch (sub-block of tpp threads) and its index t within it; the threads
+  // of a patch past the last one take part in the barriers only
+  const int tpp = blockDim.x / ppw, sub = threadIdx.x / tpp, t = threadIdx.x - sub * tpp;
+  const bool live = blockIdx.x * ppw + sub < L.np;
+  const int p = live ? blockIdx.x * ppw + sub : L.np - 1;
+  T* sm = reinterpret_cast<T*>(smem_raw) + (size_t)sub * (2 * N + 2 * D * nmax * nmax);
   T* A = sm;
   T* B = sm + N;
   T* MF = sm + 2 * N;                 // forward matrices, D x nmax^2
@@ -39,13 +46,13 @@ __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ d
     int na = nn[a];
     int tf = pt[a];
     int ti = tf == 0 ? 1 : (tf == 3 ? 4 : tf);  // DCT_II->DCT_III, DST_II->DST_III, IV types self-inverse
-    for (int k = threadIdx.x; k < na * na; k += blockDim.x) {
+    for (int k = t; k < na * na; k += tpp) {
       MF[a * nmax * nmax + k] = mats[(tf * 3 + a) * kDftMatStride + k];
       MI[a * nmax * nmax + k] = mats[(ti * 3 + a) * kDftMatStride + k];
     }
   }
   // f_copy = f, then modifyRHSForInternalBoundaryConditions (Poisson :165-180, VarPoisson :168-186)
-  for (int o = threadIdx.x; o < N; o += blockDim.x) {
+  for (int o = t; o < N; o += tpp) {
     int x = o % n0;
     int y = (o / n0) % n1;
     int z = D == 3 ? o / (n0 * n1) : 0;
@@ -84,7 +91,7 @@ __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ d
     int na = nn[a];
     int sta = a == 0 ? 1 : (a == 1 ? n0 : n0 * n1);
     const T* M = MF + a * nmax * nmax;
-    for (int o = threadIdx.x; o < N; o += blockDim.x) {
+    for (int o = t; o < N; o += tpp) {
       int k = (o / sta) % na;
       int base = o - k * sta;
       T acc = T(0);
@@ -97,7 +104,7 @@ __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ d
     dst = t;
   }
   // divide by the eigenvalues
-  for (int o = threadIdx.x; o < N; o += blockDim.x) {
+  for (int o = t; o < N; o += tpp) {
     int k3[3] = { o % n0, (o / n0) % n1, D == 3 ? o / (n0 * n1) : 0 };
     T eig = T(0);
     for (int a = 0; a < D; a++) {
@@ -107,13 +114,13 @@ __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ d
     src[o] /= eig;
   }
   __syncthreads();
-  if (pt[6] && threadIdx.x == 0) src[0] = T(0);  // all-Neumann patch without neighbours (:438-440)
+  if (pt[6] && t == 0) src[0] = T(0);  // all-Neumann patch without neighbours (:438-440)
   __syncthreads();
   for (int a = 0; a < D; a++) {
     int na = nn[a];
     int sta = a == 0 ? 1 : (a == 1 ? n0 : n0 * n1);
     const T* M = MI + a * nmax * nmax;
-    for (int o = threadIdx.x; o < N; o += blockDim.x) {
+    for (int o = t; o < N; o += tpp) {
       int k = (o / sta) % na;
       int base = o - k * sta;
       T acc = T(0);
@@ -127,11 +134,11 @@ __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ d
   }
   T scale = T(1);
   for (int a = 0; a < D; a++) scale *= T(2.0) / T(nn[a]);
-  for (int o = threadIdx.x; o < N; o += blockDim.x) {
+  for (int o = t; o < N; o += tpp) {
     int x = o % n0;
     int y = (o / n0) % n1;
     int z = D == 3 ? o / (n0 * n1) : 0;
-    out[lin(L, p, 0, x, y, z)] = TO(src[o] * scale);
+    if (live) out[lin(L, p, 0, x, y, z)] = TO(src[o] * scale);
   }
 }
 
@@ -291,9 +298,13 @@ void launch_d(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzer
 #undef TE_D3
     return;
   }
-#define TE_DL(OP_, UZ_)                                                                                      \
-  hipLaunchKernelGGL((dft_kernel<T, TF, TO, D, OP_, UZ_>), dim3(d->np), dim3(256), shmem, s, d->d_patches, \
-                     pl->d_ptype, mats, s2, f, u, cf, out, L)
+  // patches per workgroup and threads per patch: the autotuner's choice for this shape
+  const DftGenericVariant v = dft_generic_variant(tune_variant(TUNE_DFT_GENERIC, D, d->ns[0], sizeof(T) == 8 ? TE_F64 : TE_F32));
+  const int grid = (d->np + v.ppw - 1) / v.ppw;
+  shmem *= (size_t)v.ppw;
+#define TE_DL(OP_, UZ_)                                                                                          \
+  hipLaunchKernelGGL((dft_kernel<T, TF, TO, D, OP_, UZ_>), dim3(grid), dim3(v.ppw * v.tpp), shmem, s,          \
+                     d->d_patches, pl->d_ptype, mats, s2, f, u, cf, out, L, v.ppw)
   if (pl->op_kind == 0) {
     if (uzero) TE_DL(0, true); else TE_DL(0, false);
   } else {

@@ -23,6 +23,7 @@
 //             (the result must then go to a buffer other than u).
 #include "../te_internal.h"
 #include "blas.h"
+#include "../tune.h"
 #include "dft.h"
 #include "dft_wave.h"
 #include "ghost2d.h"
@@ -313,6 +314,8 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   const int4* ring = (const int4*)(gm == 2 ? link->d_ring : d->d_ring);
   const int ngroups = (d->np + kWaves - 1) / kWaves;
   const int kind = pl->g->kind;
+  // the workgroup count: the autotuner's choice for this patch size and type (tune.h)
+  const int var = tune_variant(TUNE_DFT_MFMA_2D, 2, n, sizeof(T) == 8 ? TE_F64 : TE_F32);
   // n = 16: when every workgroup (one patch per wave) is resident at once, one round of
   // the plain variant; else the prefetching variant (the next patch loaded during the
   // current solve) on as many workgroups as are resident at once, each wave solving
@@ -324,15 +327,19 @@ void launch_nt(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uze
   do {                                                                                                      \
     if constexpr (NT == 1) {                                                                                \
       const void* plain = (const void*)dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_, false>;        \
-      if (ngroups <= resident_blocks(plain, 64 * kWaves, shmem)) {                                          \
+      if (var == MFMA_GRID_ALL ||                                                                           \
+          (var == MFMA_GRID_DEFAULT && ngroups <= resident_blocks(plain, 64 * kWaves, shmem))) {            \
         TE_MK1(OP_, UZ_, GM_, K_, false, ngroups);                                                          \
       } else {                                                                                              \
-        const int r = resident_blocks((const void*)dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_, true>, \
-                                      64 * kWaves, shmem);                                                  \
+        int r = resident_blocks((const void*)dft_mfma_kernel<T, TF, TO, TC, NT, OP_, UZ_, GM_, K_, true>,   \
+                                64 * kWaves, shmem);                                                        \
+        if (var == MFMA_GRID_HALF) r = r > 1 ? r / 2 : 1;                                                   \
         TE_MK1(OP_, UZ_, GM_, K_, true, ngroups < r ? ngroups : r);                                         \
       }                                                                                                     \
     } else {                                                                                                \
-      TE_MK1(OP_, UZ_, GM_, K_, false, ngroups < 4096 ? ngroups : 4096);                                    \
+      const int cap = var == MFMA_GRID_RESIDENT ? 2048 : var == MFMA_GRID_HALF ? 8192                      \
+                      : var == MFMA_GRID_ALL    ? ngroups : 4096;                                           \
+      TE_MK1(OP_, UZ_, GM_, K_, false, ngroups < cap ? ngroups : cap);                                      \
     }                                                                                                       \
   } while (0)
 #define TE_MKG(OP_)                                                         \

@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
                                                            const double* __restrict__ cf, double* __restrict__ out,
                                                            const double* __restrict__ wv, double* __restrict__ pa,
                                                            double* __restrict__ pb, Layout L, int neumann,
-                                                           const int4* __restrict__ ring, int nslots)
+                                                           const int4* __restrict__ ring, int nslots, BatchMap bm)
 {
   if (P16) {  // compile-time layout constants: the address arithmetic folds
     L.n[0] = L.n[1] = 16;
@@ -165,7 +165,24 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
   WideAcc<double, TU> U{ u, L };
   const double phys = (OP == 0 && neumann) ? 1.0 : -1.0;
   double sa = 0.0, sq = 0.0;
-  const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
+  // the patches of this workgroup and its partial slot; in a batch those of its problem,
+  // split as the launch on that problem alone splits them
+  XcdRange xr;
+  int slot = blockIdx.x, zero_from = gridDim.x, zero_to = nslots;
+  bool lead = blockIdx.x == 0;
+  if (bm.prob) {
+    const BatchProb P = bm.prob[bm.wq[blockIdx.x]];
+    const int b = (int)blockIdx.x - P.woff;
+    xr = xcd_range(b, P.nwg, P.np, kWaves);
+    xr.first += P.poff;
+    xr.end += P.poff;
+    slot = P.soff + b;
+    zero_from = P.soff + P.nwg;
+    zero_to = P.soff + P.nslot;
+    lead = b == 0;
+  } else {
+    xr = xcd_range(blockIdx.x, gridDim.x, L.np, kWaves);
+  }
   // P16 (16 x 16 cells: ring cell = lane): the next patch's u, w and ring records are
   // loaded during the current one, and the neighbour values of its ghosts as soon as
   // the records are in
@@ -239,13 +256,13 @@ __global__ void __launch_bounds__(64 * kWaves) star_dot_2d(const DevPatch* __res
     wave_sync();
   }
   double r = block_reduce<RED_SUM>(sa);
-  if (threadIdx.x == 0) pa[blockIdx.x] = r;
+  if (threadIdx.x == 0) pa[slot] = r;
   if (SQ) {
     r = block_reduce<RED_SUM>(sq);
-    if (threadIdx.x == 0) pb[blockIdx.x] = r;
+    if (threadIdx.x == 0) pb[slot] = r;
   }
-  if (blockIdx.x == 0) {
-    for (int i = (int)gridDim.x + (int)threadIdx.x; i < nslots; i += (int)blockDim.x) {
+  if (lead) {
+    for (int i = zero_from + (int)threadIdx.x; i < zero_to; i += (int)blockDim.x) {
       pa[i] = 0.0;
       if (SQ) pb[i] = 0.0;
     }
@@ -332,8 +349,41 @@ bool star_dot_fly_supported(const te_ghost_filler* g)
          (g->kind == TE_BILINEAR || g->kind == TE_BIQUADRATIC);
 }
 
+// the kernel instantiation a launch runs
+static const void* star_dot_2d_kernel(const te_ghost_filler* g, int op_kind, int udt, bool sq)
+{
+  const te_domain* d = g->d;
+  const bool p16 = d->ns[0] == 16 && d->ns[1] == 16;
+  const bool bq = g->kind != TE_BILINEAR;
+#define TE_K(OP_, K_, Q_, P_)                                                                         \
+  return udt == TE_F32 ? (const void*)star_dot_2d<OP_, K_, Q_, P_, float> \
+                       : (const void*)star_dot_2d<OP_, K_, Q_, P_, double>
+#define TE_KQ(OP_, K_)                          \
+  do {                                          \
+    if (sq && p16) TE_K(OP_, K_, true, true);   \
+    if (sq) TE_K(OP_, K_, true, false);         \
+    if (p16) TE_K(OP_, K_, false, true);        \
+    TE_K(OP_, K_, false, false);                \
+  } while (0)
+  if (op_kind == 0) {
+    if (!bq) TE_KQ(0, TE_BILINEAR);
+    TE_KQ(0, TE_BIQUADRATIC);
+  }
+  if (!bq) TE_KQ(1, TE_BILINEAR);
+  TE_KQ(1, TE_BIQUADRATIC);
+#undef TE_KQ
+#undef TE_K
+}
+
+int star_dot_2d_cap(const te_ghost_filler* g, int op_kind, int udt, bool sq)
+{
+  if (!star_dot_fly_supported(g)) return 0;
+  return resident_blocks(star_dot_2d_kernel(g, op_kind, udt, sq), 64 * kWaves, tile_bytes(g->d, sizeof(double)));
+}
+
 int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const double* coeffs, const void* u,
-                       int udt, double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s)
+                       int udt, double* out, const double* w, double* pa, double* pb, int grid, hipStream_t s,
+                       const BatchStarMaps* bsm)
 {
   const te_domain* d = g->d;
   if (!star_dot_fly_supported(g)) {
@@ -344,8 +394,21 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
   size_t shmem = tile_bytes(d, sizeof(double));
   const bool p16 = d->ns[0] == 16 && d->ns[1] == 16;
   // as many workgroups as are resident at once (each wave takes several patches, all in one
-  // round), at most the `grid` partial slots
+  // round), at most the `grid` partial slots; in a batch that choice per problem (the map)
+  BatchMap bm{};
+  if (bsm) {
+    const int cap = resident_blocks(star_dot_2d_kernel(g, op_kind, udt, pb != nullptr), 64 * kWaves, shmem);
+    int k = 0;
+    while (k < bsm->n && bsm->cap[k] != cap) k++;
+    if (k == bsm->n) {
+      set_error("internal: no batch map for this operator launch");
+      return TE_ERR_ARG;
+    }
+    bm = bsm->map[k];
+    if (bm.nwg < 1) return TE_OK;
+  }
   auto fit = [&](const void* k) {
+    if (bsm) return bm.nwg;
     const int r = resident_blocks(k, 64 * kWaves, shmem);
     return r < grid ? r : grid;
   };
@@ -355,12 +418,12 @@ int launch_star_dot_2d(const te_ghost_filler* g, int op_kind, int neumann, const
       hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_, float>),                                                \
                          dim3(fit((const void*)star_dot_2d<OP_, K_, Q_, P_, float>)), dim3(64 * kWaves), shmem, \
                          s, d->d_patches, (const float*)u, coeffs, out, w, pa, pb, L, neumann,                  \
-                         (const int4*)d->d_ring, grid);                                                         \
+                         (const int4*)d->d_ring, grid, bm);                                                     \
     else                                                                                                       \
       hipLaunchKernelGGL((star_dot_2d<OP_, K_, Q_, P_, double>),                                               \
                          dim3(fit((const void*)star_dot_2d<OP_, K_, Q_, P_, double>)), dim3(64 * kWaves),      \
                          shmem, s, d->d_patches, (const double*)u, coeffs, out, w, pa, pb, L, neumann,          \
-                         (const int4*)d->d_ring, grid);                                                         \
+                         (const int4*)d->d_ring, grid, bm);                                                     \
   } while (0)
 #define TE_SDQ(OP_, K_)                      \
   do {                                       \

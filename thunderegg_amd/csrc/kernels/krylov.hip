@@ -144,9 +144,26 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, double* __rest
                                                         const double* __restrict__ a4, double* __restrict__ pa,
                                                         double* __restrict__ pb, float* __restrict__ w32,
                                                         const void* __restrict__ dirv, int dir_m,
-                                                        const double* __restrict__ pfold, int nfold)
+                                                        const double* __restrict__ pfold, int nfold, BatchMap bm)
 {
   CellSet<CPT> cs;
+  // this workgroup's patches p0, p0 + pstep, ... below pend and its partial slot; in a
+  // batch those of its problem, whose scalars it reads
+  int p0 = blockIdx.x, pstep = gridDim.x, pend = L.np, slot = blockIdx.x;
+  bool lead = blockIdx.x == 0;
+  if (bm.prob) {
+    const int q = bm.wq[blockIdx.x];
+    const BatchProb P = bm.prob[q];
+    const int b = (int)blockIdx.x - P.woff;
+    p0 = P.poff + b;
+    pstep = P.nwg;
+    pend = P.poff + P.np;
+    slot = P.soff + b;
+    lead = b == 0;
+    scal += (long)q * SC_COUNT;
+    if (pfold) pfold += P.soff;
+    nfold = P.nslot;
+  }
   const bool fold = MODE == BICG_S && pfold != nullptr;
   double alpha = fold ? 0.0 : scal[SC_ALPHA];
   const double omega = scal[SC_OMEGA], beta = scal[SC_BETA], rho = scal[SC_RHO];
@@ -155,7 +172,7 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, double* __rest
     if (MODE == BICG_XR && dirv && st == ST_EARLY) {
       // x += alpha dir (BiCGStab.h:162-167)
       make_cells<CPT>(L, false, cs);
-      for (int p = blockIdx.x; p < L.np; p += gridDim.x) {
+      for (int p = p0; p < pend; p += pstep) {
 #pragma unroll
         for (int k = 0; k < CPT; k++) {
           if (k >= cs.n) continue;
@@ -173,14 +190,14 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, double* __rest
     double r[3] = { 0.0, 0.0, 0.0 };
     fold_n<1>(parts, nfold, r);
     alpha = rho / r[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0) scal[SC_ALPHA] = alpha;
+    if (lead && threadIdx.x == 0) scal[SC_ALPHA] = alpha;
   }
   typedef typename std::conditional<MODE == BICG_XR || MODE == BICG_XP, TM, double>::type TA;
   const TA* __restrict__ a0 = (const TA*)a0v;
   const TA* __restrict__ a1 = (const TA*)a1v;
   make_cells<CPT>(L, false, cs);
   double sa = 0.0, sb = 0.0;
-  for (int p = blockIdx.x; p < L.np; p += gridDim.x) {
+  for (int p = p0; p < pend; p += pstep) {
     const int base = p * L.ps;
 #pragma unroll
     for (int k = 0; k < CPT; k++) {
@@ -223,11 +240,11 @@ __global__ void __launch_bounds__(kThreads) bicg_kernel(Layout L, double* __rest
   }
   if (MODE == BICG_INIT || MODE == BICG_S || MODE == BICG_XR) {
     double r = block_reduce<RED_SUM>(sa);
-    if (threadIdx.x == 0) pa[blockIdx.x] = r;
+    if (threadIdx.x == 0) pa[slot] = r;
   }
   if (MODE == BICG_INIT || MODE == BICG_XR) {
     double r = block_reduce<RED_SUM>(sb);
-    if (threadIdx.x == 0) pb[blockIdx.x] = r;
+    if (threadIdx.x == 0) pb[slot] = r;
   }
 }
 
@@ -290,6 +307,127 @@ __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double
         *(volatile double*)&mirror[SC_SEQ] = seq;
       }
     }
+  }
+}
+
+// bicg_finalize for every problem of a batch: workgroup q folds problem q's slots and
+// updates its scalars; the loop control of BiCGStab.h:148-150 runs here (the host only
+// waits until no problem runs).
+__global__ void __launch_bounds__(kThreads) bicg_finalize_batch(int mode, const double* __restrict__ pa,
+                                                                const double* __restrict__ pb,
+                                                                const double* __restrict__ pc,
+                                                                const BatchProb* __restrict__ prob,
+                                                                double* __restrict__ scal_all, double tol,
+                                                                int max_it)
+{
+  const BatchProb P = prob[blockIdx.x];
+  double* __restrict__ scal = scal_all + (long)blockIdx.x * SC_COUNT;
+  const double state = scal[SC_STATE];
+  const double* const parts[3] = { pa + P.soff, pb ? pb + P.soff : nullptr, pc ? pc + P.soff : nullptr };
+  double r[3] = { 0.0, 0.0, 0.0 };
+  if (mode == FIN_SOMEGA) fold_n<3>(parts, P.nslot, r);
+  else if (mode == FIN_ALPHA || mode == FIN_SNORM) fold_n<1>(parts, P.nslot, r);
+  else fold_n<2>(parts, P.nslot, r);
+  const bool run = mode == FIN_INIT || state == ST_RUN;
+  if (!run && !(mode == FIN_RHO && state == ST_EARLY)) return;
+  const double A = r[0], B = r[1], C = r[2];
+  if (threadIdx.x != 0) return;
+  switch (mode) {
+    case FIN_INIT: {
+      const double r0 = sqrt(A), rn = sqrt(B);
+      scal[SC_R0] = r0;
+      scal[SC_RHO] = B;
+      scal[SC_RNORM] = rn;
+      scal[SC_TOL] = tol;
+      scal[SC_ITS] = 0.0;
+      scal[SC_MAXIT] = max_it;
+      // if (r0_norm == 0) no iteration; while (residual > tolerance && num_its < max_it),
+      // and rho == 0 at the top of an iteration is the breakdown error
+      double st = ST_RUN;
+      if (r0 == 0.0 || !(rn / r0 > tol) || max_it <= 0) st = ST_DONE;
+      else if (B == 0.0) st = ST_BREAK;
+      scal[SC_STATE] = st;
+      break;
+    }
+    case FIN_ALPHA: scal[SC_ALPHA] = scal[SC_RHO] / A; break;
+    case FIN_SNORM: {
+      const double sn = sqrt(A);
+      scal[SC_SNORM] = sn;
+      if (sn / scal[SC_R0] <= scal[SC_TOL]) scal[SC_STATE] = ST_EARLY;
+      break;
+    }
+    case FIN_OMEGA: scal[SC_OMEGA] = A / B; break;
+    case FIN_SOMEGA: {
+      const double sn = sqrt(C);
+      scal[SC_SNORM] = sn;
+      if (sn / scal[SC_R0] <= scal[SC_TOL]) scal[SC_STATE] = ST_EARLY;
+      else scal[SC_OMEGA] = A / B;
+      break;
+    }
+    default: {  // FIN_RHO
+      if (state == ST_EARLY) {  // x was updated; the loop ends without counting the iteration
+        scal[SC_STATE] = ST_DONE;
+        break;
+      }
+      const double rho_new = A;
+      scal[SC_BETA] = rho_new * scal[SC_ALPHA] / (scal[SC_RHO] * scal[SC_OMEGA]);
+      scal[SC_RHO] = rho_new;
+      const double rn = sqrt(B);
+      scal[SC_RNORM] = rn;
+      const double its = scal[SC_ITS] + 1.0;
+      scal[SC_ITS] = its;
+      if (!(rn / scal[SC_R0] > scal[SC_TOL]) || its >= scal[SC_MAXIT]) scal[SC_STATE] = ST_DONE;
+      else if (rho_new == 0.0) scal[SC_STATE] = ST_BREAK;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) batch_publish(const double* __restrict__ scal, int nprob,
+                                                          double* __restrict__ ctl, double* __restrict__ mirror)
+{
+  __shared__ int cnt[2][kThreads / 64];
+  int run = 0, brk = 0;
+  for (int q = threadIdx.x; q < nprob; q += kThreads) {
+    const double st = scal[(long)q * SC_COUNT + SC_STATE];
+    run += (st == ST_RUN || st == ST_EARLY) ? 1 : 0;
+    brk += st == ST_BREAK ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    run += __shfl_down(run, off, 64);
+    brk += __shfl_down(brk, off, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    cnt[0][w] = run;
+    cnt[1][w] = brk;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < kThreads / 64; k++) {
+    run += cnt[0][k];
+    brk += cnt[1][k];
+  }
+  // thread 0 of wave 0 holds its own wave's sums in run / brk already
+  const double seq = ctl[0] + 1.0;
+  ctl[0] = seq;
+  mirror[1] = run;
+  mirror[2] = brk;
+  __threadfence_system();  // the counts reach the host before the sequence number
+  *(volatile double*)&mirror[0] = seq;
+}
+
+// blockIdx.y = problem; workgroups stride over its elements
+__global__ void __launch_bounds__(kThreads) batch_copy(bool to_batch, double* __restrict__ batch,
+                                                       double* const* __restrict__ ptr, const long* __restrict__ off,
+                                                       const long* __restrict__ len)
+{
+  const int q = blockIdx.y;
+  const long n = len[q];
+  double* own = ptr[q];
+  double* bv = batch + off[q];
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) {
+    if (to_batch) bv[i] = own[i];
+    else own[i] = bv[i];
   }
 }
 
@@ -365,9 +503,11 @@ int launch_star_dot(const te_ghost_filler* g, int op_kind, int neumann, const do
 
 int launch_bicg(int mode, const te_domain* d, double* scal, double* v0, double* v1, const void* a0,
                 const void* a1, const double* a2, const double* a3, const double* a4, double* pa, double* pb,
-                hipStream_t s, int dtm, float* w32, const void* xp_dir, int xp_dir_m, const double* pfold, int nfold)
+                hipStream_t s, int dtm, float* w32, const void* xp_dir, int xp_dir_m, const double* pfold, int nfold,
+                const BatchMap* bm)
 {
-  if (pfold && (mode != BICG_S || nfold < 1 || nfold > kFoldMax)) {
+  // (in a batch each problem folds its own slots, at most krylov_blocks of its patches)
+  if (pfold && (mode != BICG_S || (!bm && (nfold < 1 || nfold > kFoldMax)))) {
     set_error("internal: only the s update folds partials (at most kFoldMax)");
     return TE_ERR_ARG;
   }
@@ -383,14 +523,20 @@ int launch_bicg(int mode, const te_domain* d, double* scal, double* v0, double* 
     return TE_ERR_UNSUPPORTED;
   }
   int grid = krylov_blocks(d);
+  BatchMap m{};
+  if (bm) {
+    m = *bm;
+    grid = m.nwg;
+    if (grid < 1) return TE_OK;
+  }
 #define TE_BKC(C_, M_)                                                                                           \
   do {                                                                                                           \
     if (dtm == TE_F32)                                                                                           \
       hipLaunchKernelGGL((bicg_kernel<C_, M_, float>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0, a1, \
-                         a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m, pfold, nfold);                               \
+                         a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m, pfold, nfold, m);                            \
     else                                                                                                         \
       hipLaunchKernelGGL((bicg_kernel<C_, M_, double>), dim3(grid), dim3(kThreads), 0, s, L, scal, v0, v1, a0,   \
-                         a1, a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m, pfold, nfold);                           \
+                         a1, a2, a3, a4, pa, pb, w32, xp_dir, xp_dir_m, pfold, nfold, m);                        \
   } while (0)
 #define TE_BK(M_)                      \
   do {                                 \
@@ -423,6 +569,39 @@ int launch_bicg_finalize(int mode, const double* pa, const double* pb, int n, do
     return TE_ERR_ARG;
   }
   hipLaunchKernelGGL(bicg_finalize, dim3(1), dim3(kThreads), 0, s, mode, pa, pb, pc, n, scal, tol, mirror);
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_bicg_finalize_batch(int mode, const double* pa, const double* pb, const double* pc, const BatchMap& m,
+                               double* scal, hipStream_t s, double tol, int max_it)
+{
+  if (m.nprob < 1) return TE_OK;
+  hipLaunchKernelGGL(bicg_finalize_batch, dim3(m.nprob), dim3(kThreads), 0, s, mode, pa, pb, pc, m.prob, scal, tol,
+                     max_it);
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_batch_publish(const double* scal, int nprob, double* ctl, double* mirror, hipStream_t s)
+{
+  hipLaunchKernelGGL(batch_publish, dim3(1), dim3(kThreads), 0, s, scal, nprob, ctl, mirror);
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_batch_copy(bool to_batch, double* batch, double* const* ptr, const long* off, const long* len, int nprob,
+                      long maxlen, hipStream_t s)
+{
+  if (nprob < 1 || maxlen < 1) return TE_OK;
+  if (nprob > 65535) {
+    set_error("internal: at most 65535 problems per batch copy");
+    return TE_ERR_ARG;
+  }
+  long gx = (maxlen + kThreads * 4 - 1) / (kThreads * 4);
+  if (gx > 256) gx = 256;
+  hipLaunchKernelGGL(batch_copy, dim3((unsigned)gx, (unsigned)nprob), dim3(kThreads), 0, s, to_batch, batch, ptr, off,
+                     len);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

@@ -3,7 +3,11 @@
 // interpolation is computed per fine cell.
 //  - LinearRestrictor (GMG/LinearRestrictor.h:101-155): coarse[(i+s)/2] += fine[i]/2^D
 //    for children with an orthant on the parent, coarse[i] += fine[i] for copies;
-//    MPIRestrictor::restrict zeroes the coarse vector with ghosts first (:73).
+//    MPIRestrictor::restrict zeroes the coarse vector with ghosts first (:73).  With
+//    extrapolate_boundary_ghosts (:49-92, 138-155) the coarse ghost cells over a child's
+//    exterior sides get (3 ghost - interior) / 2^D of the child's side cells, the corner of
+//    the child's orthant, when it has no neighbour there, 3 ghost - interior of the child's
+//    corner cell, and copies take the ghost cells too.
 //  - DirectInterpolator (GMG/DirectInterpolator.h:60-91): fine[i] += coarse[(i+s)/2].
 // Children of a coarse patch are visited in increasing fine local index and fine
 // cells in x-fastest order, the order in which the reference accumulates them.
@@ -12,11 +16,84 @@
 namespace te {
 namespace {
 
-template <typename TI, typename TO, int D>
+// the coarse ghost cell at (X, Y, Z) of patch p with extrapolate_boundary_ghosts: the sum
+// over p's children in increasing local order, each term as the reference forms it
+template <typename TI, int D>
+__device__ TI extrapolated_ghost(const DevPatch* __restrict__ fdp, const int32_t* __restrict__ child,
+                                 const int32_t* __restrict__ nchild, const int32_t* __restrict__ orth,
+                                 const TI* __restrict__ fine, const Layout& L, int p, int c, int X, int Y, int Z,
+                                 int rem)
+{
+  const int cx[3] = { X, Y, Z };
+  int outn = 0, bits = 0, axis = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    if (cx[a] < 0 || cx[a] >= L.n[a]) {
+      outn++;
+      axis = a;
+      if (cx[a] >= L.n[a]) bits |= 1 << a;
+    }
+  }
+  TI acc = TI(0);
+  const int nc = nchild[p];
+  for (int k = 0; k < nc; k++) {
+    const int f = child[p * 8 + k];
+    const int o = orth[f];
+    if (o < 0) {  // copyToParent: every cell, ghosts included
+      acc += fine[f * L.ps + rem];
+      continue;
+    }
+    if (outn == 1) {  // a side ghost: the child whose exterior side and half it lies over
+      const int up = (bits >> axis) & 1;
+      if (((o >> axis) & 1) != up) continue;
+      int lo[2] = { 0, 0 }, hi[2] = { 0, 0 }, fa[2] = { 0, 0 }, nf = 0;
+      bool mine = true;
+#pragma unroll
+      for (int b = 0; b < D; b++) {
+        if (b == axis) continue;
+        const int st = ((o >> b) & 1) ? L.n[b] : 0;
+        int t0 = 2 * cx[b] - st, t1 = t0 + 1;  // the fine cells with (t + st) / 2 == cx[b]
+        if (t0 < 0) t0 = 0;
+        if (t1 > L.n[b] - 1) t1 = L.n[b] - 1;
+        if (t0 > t1) mine = false;
+        fa[nf] = b;
+        lo[nf] = t0;
+        hi[nf] = t1;
+        nf++;
+      }
+      if (!mine) continue;
+      const int ga = up ? L.n[axis] : -1, ia = up ? L.n[axis] - 1 : 0;
+      // the side's slice, first free axis fastest
+      for (int j = (D == 3 ? lo[1] : 0); j <= (D == 3 ? hi[1] : 0); j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          int xg[3] = { 0, 0, 0 }, xi[3] = { 0, 0, 0 };
+          xg[axis] = ga;
+          xi[axis] = ia;
+          xg[fa[0]] = xi[fa[0]] = i;
+          if (D == 3) xg[fa[1]] = xi[fa[1]] = j;
+          const TI g = fine[lin(L, f, c, xg[0], xg[1], xg[2])], in = fine[lin(L, f, c, xi[0], xi[1], xi[2])];
+          acc += (TI(3) * g - in) / TI(1 << D);
+        }
+    } else if (outn == D) {  // the corner of the child's orthant, when nothing lies across it
+      if (o != bits || fdp[f].type[bits] >= 0) continue;
+      int xg[3] = { 0, 0, 0 }, xi[3] = { 0, 0, 0 };
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        xg[a] = ((bits >> a) & 1) ? L.n[a] : -1;
+        xi[a] = ((bits >> a) & 1) ? L.n[a] - 1 : 0;
+      }
+      acc += TI(3) * fine[lin(L, f, c, xg[0], xg[1], xg[2])] - fine[lin(L, f, c, xi[0], xi[1], xi[2])];
+    }
+  }
+  return acc;
+}
+
+template <typename TI, typename TO, int D, bool EX>
 __global__ void __launch_bounds__(256) restrict_kernel(const int32_t* __restrict__ child,
                                                        const int32_t* __restrict__ nchild,
                                                        const int32_t* __restrict__ orth, const TI* __restrict__ fine,
-                                                       TO* __restrict__ coarse, Layout L, int np_coarse)
+                                                       TO* __restrict__ coarse, Layout L, int np_coarse,
+                                                       const DevPatch* __restrict__ fdp)
 {
   long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long total = (long)np_coarse * L.ps;
@@ -41,7 +118,7 @@ __global__ void __launch_bounds__(256) restrict_kernel(const int32_t* __restrict
   Y -= 1;
   bool inter = X >= 0 && X < L.n[0] && Y >= 0 && Y < L.n[1] && (D == 2 || (Z >= 0 && Z < L.n[2]));
   if (!inter) {
-    coarse[idx] = TO(0);
+    coarse[idx] = EX ? TO(extrapolated_ghost<TI, D>(fdp, child, nchild, orth, fine, L, p, c, X, Y, Z, rem)) : TO(0);
     return;
   }
   int nc = nchild[p];
@@ -111,7 +188,7 @@ __global__ void __launch_bounds__(256) interp_kernel(const int32_t* __restrict__
 }
 
 template <typename TI, typename TO>
-void restrict_t(const te_level_link* l, const TI* fine, TO* coarse, int ncomp, hipStream_t s)
+void restrict_t(const te_level_link* l, const TI* fine, TO* coarse, int ncomp, bool ex, hipStream_t s)
 {
   const te_domain* c = l->coarse;
   Layout L = make_layout(c->dim, c->ns, ncomp, l->fine->np);
@@ -119,12 +196,17 @@ void restrict_t(const te_level_link* l, const TI* fine, TO* coarse, int ncomp, h
   if (total == 0) return;
   const int B = 256;
   int grid = (int)((total + B - 1) / B);
-  if (c->dim == 2)
-    hipLaunchKernelGGL((restrict_kernel<TI, TO, 2>), dim3(grid), dim3(B), 0, s, l->d_child, l->d_nchild, l->d_orth,
-                       fine, coarse, L, c->np);
-  else
-    hipLaunchKernelGGL((restrict_kernel<TI, TO, 3>), dim3(grid), dim3(B), 0, s, l->d_child, l->d_nchild, l->d_orth,
-                       fine, coarse, L, c->np);
+#define TE_RK(D_, EX_)                                                                                             \
+  hipLaunchKernelGGL((restrict_kernel<TI, TO, D_, EX_>), dim3(grid), dim3(B), 0, s, l->d_child, l->d_nchild,        \
+                     l->d_orth, fine, coarse, L, c->np, l->fine->d_patches)
+  if (c->dim == 2) {
+    if (ex) TE_RK(2, true);
+    else TE_RK(2, false);
+  } else {
+    if (ex) TE_RK(3, true);
+    else TE_RK(3, false);
+  }
+#undef TE_RK
 }
 
 template <typename TC, typename TF>
@@ -147,12 +229,13 @@ void interp_t(const te_level_link* l, const TC* coarse, TF* fine, int ncomp, hip
 }  // namespace
 
 int launch_restrict(const te_level_link* l, int dti, const void* fine, int dto, void* coarse, int ncomp,
-                    hipStream_t s)
+                    hipStream_t s, bool extrapolate)
 {
-  if (dti == TE_F64 && dto == TE_F64) restrict_t<double, double>(l, (const double*)fine, (double*)coarse, ncomp, s);
-  else if (dti == TE_F32 && dto == TE_F32) restrict_t<float, float>(l, (const float*)fine, (float*)coarse, ncomp, s);
-  else if (dti == TE_F32 && dto == TE_F64) restrict_t<float, double>(l, (const float*)fine, (double*)coarse, ncomp, s);
-  else restrict_t<double, float>(l, (const double*)fine, (float*)coarse, ncomp, s);
+  const bool ex = extrapolate;
+  if (dti == TE_F64 && dto == TE_F64) restrict_t<double, double>(l, (const double*)fine, (double*)coarse, ncomp, ex, s);
+  else if (dti == TE_F32 && dto == TE_F32) restrict_t<float, float>(l, (const float*)fine, (float*)coarse, ncomp, ex, s);
+  else if (dti == TE_F32 && dto == TE_F64) restrict_t<float, double>(l, (const float*)fine, (double*)coarse, ncomp, ex, s);
+  else restrict_t<double, float>(l, (const double*)fine, (float*)coarse, ncomp, ex, s);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

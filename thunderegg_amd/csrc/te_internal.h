@@ -144,8 +144,9 @@ bool timer_has_domain(const te_timer* t, int domain_id);
 int launch_ghost_fill(const te_ghost_filler* g, int dtype, void* u, int ncomp, hipStream_t s);
 int launch_star(const te_ghost_filler* g, int dt, int dtf, int dto, int op_kind, int neumann, const void* coeffs,
                 const void* fin, const void* u, void* out, int ncomp, hipStream_t s);
+// extrapolate: LinearRestrictor's extrapolate_boundary_ghosts
 int launch_restrict(const te_level_link* l, int dtype_in, const void* fine, int dtype_out, void* coarse, int ncomp,
-                    hipStream_t s);
+                    hipStream_t s, bool extrapolate = false);
 int launch_interpolate(const te_level_link* l, int dtype_c, const void* coarse, int dtype_f, void* fine, int ncomp,
                        hipStream_t s);
 // boundary data (kernels/bc.hip): values per patch, sum over sides of the face cell counts

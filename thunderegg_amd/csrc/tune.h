@@ -1,0 +1,38 @@
+// Launch variants chosen per patch shape by the autotuner's table.  Every variant of a
+// family computes each patch with the same arithmetic in the same order (only the mapping of
+// patches onto workgroups and waves changes), so results do not depend on the table.
+// thunderegg_amd/autotune.py times the variants on the GPU and writes the table; the library
+// loads it on first use (tune_table.txt beside the shared library, or the file named by
+// TE_TUNE_TABLE; TE_TUNE_TABLE=none keeps the hand defaults).
+#pragma once
+
+namespace te {
+
+enum TuneFamily {
+  TUNE_DFT_GENERIC = 0,  // DFT patch solve, VALU kernel (any n): patches per workgroup x threads per patch
+  TUNE_DFT_MFMA_2D = 1,  // DFT patch solve, 2D matrix-core kernel (n 16, 32): workgroups of the launch
+  TUNE_FAMILIES
+};
+
+// the variant for (family, dim, n, dtype): the one forced by a timing call on this thread,
+// else the table's, else 0 (the hand default)
+int tune_variant(int family, int dim, int n, int dtype);
+int tune_num_variants(int family);
+
+struct DftGenericVariant {
+  int ppw;  // patches per workgroup
+  int tpp;  // threads per patch
+};
+DftGenericVariant dft_generic_variant(int v);
+
+// 2D matrix-core DFT launches: the workgroup count policy of variant v (dft_mfma.hip)
+enum DftMfmaGrid {
+  MFMA_GRID_DEFAULT = 0,    // n 16: one round of the plain kernel when every workgroup is resident at
+                            // once, else the prefetching kernel on the resident count; n 32: <= 4096
+  MFMA_GRID_RESIDENT = 1,   // the prefetching kernel (n 16) on the resident workgroup count / <= 2048 (n 32)
+  MFMA_GRID_HALF = 2,       // half the resident count (n 16) / <= 8192 (n 32)
+  MFMA_GRID_ALL = 3,        // one workgroup per 4 patches, plain kernel
+  MFMA_GRID_VARIANTS
+};
+
+}  // namespace te

@@ -71,6 +71,7 @@ SIGNATURES = {
     "te_patch_bicgstab_smooth": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_double, C.c_int,
                                            _vp, _vp, _vp, _vp]),
     "te_restrict": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp]),
+    "te_restrict_extrapolate": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     "te_interpolate": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     "te_vec_set": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_int, _vp]),
     "te_vec_set_with_ghost": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_int, _vp]),
@@ -97,6 +98,19 @@ SIGNATURES = {
     "te_gmg_cg": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp, C.c_int, _vp]),
     "te_gmg_last_solve_seconds": (C.c_double, [_vp]),
     "te_gmg_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "te_tune_num_variants": (C.c_int, [C.c_int]),
+    "te_tune_load": (C.c_int, [C.c_char_p]),
+    "te_tune_clear": (C.c_int, []),
+    "te_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "te_tune_get": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "te_tune_table_path": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "te_tune_time_dft": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _dp]),
+    "te_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp), _ip, _vp, C.POINTER(_vp)]),
+    "te_batch_destroy": (C.c_int, [_vp]),
+    "te_batch_num_groups": (C.c_int, [_vp]),
+    "te_batch_vcycle": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp]),
+    "te_batch_bicgstab": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.c_int, C.c_int, C.c_double, _ip, _dp,
+                                    _vp]),
     "te_gmg_set_timer": (C.c_int, [_vp, _vp, C.c_int, _ip]),
     "te_timer_create": (C.c_int, [C.POINTER(_vp)]),
     "te_timer_destroy": (C.c_int, [_vp]),
@@ -303,6 +317,10 @@ class LevelLink:
     def restrict(self, fine: DeviceArray, coarse: DeviceArray, stream=None):
         check(lib().te_restrict(self.h, _dt(fine), fine.ptr, coarse.ptr, fine.shape[1], stream))
 
+    def restrict_extrapolate(self, fine: DeviceArray, coarse: DeviceArray, stream=None):
+        """LinearRestrictor::restrict with extrapolate_boundary_ghosts"""
+        check(lib().te_restrict_extrapolate(self.h, _dt(fine), fine.ptr, coarse.ptr, fine.shape[1], stream))
+
     def residual_restrict(self, filler: "GhostFiller", f: DeviceArray, u: DeviceArray, fc: DeviceArray,
                           op_kind=0, neumann=False, coeffs: DeviceArray | None = None, stream=None):
         """fc = restriction of f - A u, one launch (u's dtype is the arithmetic type)."""
@@ -375,11 +393,56 @@ class DftPlan:
         """From u = 0, computing in `dtype`; f and out keep their own dtypes."""
         check(lib().te_dft_apply_mixed(self.h, dtype, _dt(f), _dt(out), f.ptr, out.ptr, stream))
 
+    def time_variant(self, f: DeviceArray, u: DeviceArray, out: DeviceArray, variant: int, reps=20):
+        """Average milliseconds of the patch solve (ghosts read from u) with a launch variant
+        forced (te_tune_time_dft); u's dtype is the arithmetic type."""
+        ms = C.c_double()
+        check(lib().te_tune_time_dft(self.h, _dt(u), f.ptr, u.ptr, out.ptr, variant, reps, C.byref(ms)))
+        return ms.value
+
     def smooth_interp(self, link: "LevelLink", f: DeviceArray, u: DeviceArray, uc: DeviceArray, out: DeviceArray,
                       stream=None):
         """out = smooth(f, u + P uc), one launch (u's dtype is the arithmetic type)."""
         check(lib().te_dft_smooth_interp(self.h, link.h, _dt(u), _dt(f), _dt(out), f.ptr, u.ptr, _dt(uc), uc.ptr,
                                          out.ptr, stream))
+
+
+# the autotuner's table (te_tune_*): families
+TUNE_DFT_GENERIC, TUNE_DFT_MFMA_2D = 0, 1
+
+
+def tune_get(family, dim, n, dtype):
+    return lib().te_tune_get(family, dim, n, dtype)
+
+
+def tune_set(family, dim, n, dtype, variant):
+    check(lib().te_tune_set(family, dim, n, dtype, variant))
+
+
+def tune_num_variants(family):
+    return lib().te_tune_num_variants(family)
+
+
+def tune_load(path):
+    check(lib().te_tune_load(str(path).encode()))
+
+
+def tune_clear():
+    check(lib().te_tune_clear())
+
+
+def tune_table_path():
+    buf = C.create_string_buffer(4096)
+    lib().te_tune_table_path(buf, len(buf))
+    return buf.value.decode()
+
+
+def tune_family(dim, n, use_mfma=True):
+    """the family of the DFT patch solve a level of this shape runs (None: 3D 16^3 patches,
+    whose matrix-core kernels have no variants)"""
+    if use_mfma and dim == 3 and n == 16:
+        return None
+    return TUNE_DFT_MFMA_2D if (use_mfma and dim == 2 and n in (16, 32)) else TUNE_DFT_GENERIC
 
 
 class Stream:

@@ -215,6 +215,62 @@ class GMG:
         self._timer = timer
 
 
+class Batch:
+    """Many independent 2D problems solved together (te_batch): one launch per cycle step and
+    per BiCGStab step for all problems of a patch size, one graph per outer iteration.
+
+    problems: a list of (ns, levels) with levels the patch tables finest first; the options
+    are GMG's (V-cycle, Poisson operator, DFT smoother).  Each problem keeps its own vectors
+    (shape of its finest level); its iterations and result are those of solving it alone."""
+
+    def __init__(self, problems, filler_kind=R.BIQUADRATIC, fill_type=R.FACES, neumann=False, neumann_mask=0,
+                 pre_sweeps=1, post_sweeps=1, coarse_sweeps=1, precision=R.F64, use_graph=True):
+        self.domains = []
+        nlev = []
+        for ns, levels in problems:
+            nlev.append(len(levels))
+            self.domains += [R.Domain(2, ns, lv) for lv in levels]
+        self.n = len(problems)
+        firsts = np.concatenate([[0], np.cumsum(nlev)[:-1]]).astype(int)
+        self.shapes = [self.domains[f].shape(1) for f in firsts]
+        o = _Opts(filler_kind, fill_type, 0, int(neumann), 0, neumann_mask, 1000, 1e-12, 0, pre_sweeps, post_sweeps,
+                  coarse_sweeps, precision, int(use_graph), CYCLE_TYPES["V"], 1, None, None, None)
+        hs = (C.c_void_p * len(self.domains))(*[d.h for d in self.domains])
+        nl = (C.c_int * self.n)(*nlev)
+        h = C.c_void_p()
+        R.check(R.lib().te_batch_create(self.n, hs, nl, C.byref(o), C.byref(h)))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            R.lib().te_batch_destroy(self.h)
+            self.h = None
+
+    def num_groups(self):
+        return R.lib().te_batch_num_groups(self.h)
+
+    def _ptrs(self, arrays):
+        if len(arrays) != self.n:
+            raise ValueError("one vector per problem")
+        for a, shp in zip(arrays, self.shapes):
+            if a.dtype != np.float64 or tuple(a.shape) != tuple(shp):
+                raise ValueError(f"each problem's vector must be FP64 of its finest level's shape {shp}")
+        return (C.c_void_p * self.n)(*[a.ptr for a in arrays])
+
+    def vcycle_device(self, fs, us):
+        R.check(R.lib().te_batch_vcycle(self.h, self._ptrs(fs), self._ptrs(us), None))
+
+    def solve_device(self, bs, xs, tol=1e-10, max_it=1000, precond=2):
+        """BiCGStab on every problem (precond as GMG.solve_device); returns the iteration
+        counts and the last relative residual norms."""
+        its = np.zeros(self.n, dtype=np.int32)
+        res = np.zeros(self.n)
+        R.check(R.lib().te_batch_bicgstab(self.h, self._ptrs(bs), self._ptrs(xs), int(precond), max_it, tol,
+                                          its.ctypes.data_as(C.POINTER(C.c_int)),
+                                          res.ctypes.data_as(C.POINTER(C.c_double)), None))
+        return its, res
+
+
 def solve_many(solvers, bs, xs, tol=1e-10, max_it=1000, precond=2, threads=8):
     """Independent solves run concurrently: each GMG owns a stream, and the C calls release
     the interpreter lock, so up to `threads` solves keep the GPU busy together (many small
