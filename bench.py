@@ -101,7 +101,31 @@ def kernel_rooflines_2d(R, P, stream, reps=50):
     c64 = R.DeviceArray(d0.shape(1), np.float64)
     rec("vec_copy_f64", lambda: R.check(R.lib().te_vec_copy(d0.h, R.F64, c64.ptr, f64.ptr, 1, stream.h)),
         2 * n0 * N * 8)
+    # the standalone ghost fill (the path's other configurations use it; the fused cycle
+    # computes ghosts inside its launches)
+    u64 = R.DeviceArray.from_host(rng.standard_normal(d0.shape(1)))
+    rec("ghost_fill_f64", lambda: g.fill_ghost(u64, stream=stream.h), ghost_fill_bytes(lv0, ns, P.filler, 8))
     return out
+
+
+def ghost_fill_bytes(level, ns, kind, s):
+    """Algorithmic bytes of one 2D side-ghost fill: per side with a neighbour, its n ghost
+    cells written once and every cell their formulas read, once: the neighbour's facing row
+    (same level), the patch's own boundary row (and the one behind it, BiQuadratic) with the
+    n / 2 coarse cells (coarser neighbour), or the own row and the two finer neighbours'
+    facing rows (and the rows behind them, BiQuadratic)."""
+    import numpy as np
+
+    types = np.asarray(level["nbr_type"])[:, 4:8]
+    total = 0
+    for side in range(4):
+        n = ns[1] if side < 2 else ns[0]
+        t = types[:, side]
+        deep = 2 if kind == 1 else 1
+        total += int((t == 0).sum()) * 2 * n
+        total += int((t == 1).sum()) * (n + deep * n + n // 2)
+        total += int((t == 2).sum()) * (n + n + deep * 2 * n)
+    return total * s
 
 
 def kernel_rooflines_3d(g, P, its, reps=20):
@@ -156,6 +180,23 @@ def kernel_rooflines_3d(g, P, its, reps=20):
     ms = time_launches(R, lambda: R.check(R.lib().te_vec_copy(d0.h, R.F64, c.ptr, a.ptr, 1, stream.h)), reps, stream)
     out["vec_copy_f64"] = {"ms": ms, "bytes": byt, "GBs": byt / ms / 1e6, "launches_per_solve": 0,
                            "ms_per_solve": 0.0}
+    return out
+
+
+def tune_report(R, shapes):
+    """The autotuner's table as the library loaded it, and the launch variant each of the
+    given (dim, n) patch shapes runs, per precision (variant 0 = the hand default)."""
+    out = {"table": os.path.relpath(R.tune_table_path(), ROOT) if R.tune_table_path() else None, "variants": {}}
+    for dim, n in sorted(set(shapes)):
+        fam = R.tune_family(dim, n)
+        if fam is None:
+            out["variants"][f"{dim}d_n{n}"] = "no variants (compact 3D kernels)"
+            continue
+        out["variants"][f"{dim}d_n{n}"] = {
+            "family": ["dft_generic", "dft_mfma_2d"][fam],
+            "f32": R.tune_get(fam, dim, n, R.F32),
+            "f64": R.tune_get(fam, dim, n, R.F64),
+        }
     return out
 
 
@@ -325,6 +366,7 @@ def run_many_problems(args, nprob=None, emit=True):
         "concurrent_threads": args.threads,
         "batch_setup_seconds": setup32,
         "fp64_iterations_equal_per_problem": itb64 == itc64,
+        "tune": tune_report(R, [(2, P.ns[0]) for P in probs]),
         "worst_true_residual_mixed": worst32,
         "worst_true_residual_fp64": worst64,
         "iterations_mixed": itb32,
@@ -486,6 +528,7 @@ def main():
     }
     res["setup_seconds"]["mesh"] = t_mesh
     out.update(res)
+    out["tune"] = tune_report(R, [(P.dim, P.ns[0])] + ([(2, 16)] if args.config == 3 and not args.no_secondary else []))
     its_m = out["iterations_mixed"][0]
     if not args.no_roofline:
         if P.dim == 3:
@@ -493,7 +536,7 @@ def main():
             dom_k = max(kr, key=lambda k: kr[k]["ms_per_solve"])
         else:
             kr = kernel_rooflines_2d(R, P, R.Stream())
-            cands = [k for k in kr if k != "vec_copy_f64"]
+            cands = [k for k in kr if k not in ("vec_copy_f64", "ghost_fill_f64")]  # off the timed path
             dom_k = max(cands, key=lambda k: kr[k]["ms"])
         out["kernels"] = kr
         k = kr[dom_k]
@@ -527,7 +570,7 @@ def main():
             keep = ("value", "fp64_dof_per_s", "mixed_over_fp64", "per_problem_fp64_dof_per_s",
                     "per_problem_mixed_dof_per_s", "batched_over_per_problem_fp64",
                     "fp64_iterations_equal_per_problem", "ms_per_step", "worst_true_residual_mixed",
-                    "worst_true_residual_fp64")
+                    "worst_true_residual_fp64", "tune")
             out["batched_small_problems"] = {"workload": r5["config"]["workload"], "dof": r5["config"]["dof"],
                                              **{k: r5[k] for k in keep},
                                              "iterations_mixed_min_max_sum": [min(r5["iterations_mixed"]),

@@ -176,8 +176,9 @@ int te_dft_apply(const te_dft_plan* p, int dtype, const void* f, void* u, te_str
  * must already be filled.  One launch of the batched patch solve. */
 int te_dft_patch_solve(const te_dft_plan* p, int dtype, const void* f, void* u, te_stream stream);
 
-/* ---- fused launches of the V-cycle (2D, BiLinear / BiQuadratic ghosts; the DFT ones
- * need the matrix-core plan: n = 16 or 32, Dirichlet).  Each replaces a sequence of the
+/* ---- fused launches of the V-cycle (2D, even cells per side, BiLinear / BiQuadratic
+ * ghosts; the DFT ones run on the matrix cores for n = 16 or 32 with Dirichlet sides, and
+ * in the generic patch-solve kernel otherwise).  Each replaces a sequence of the
  * calls above by one launch that computes ghost values from the neighbours' interior
  * values instead of reading ghost cells; none of them reads or writes ghost cells.
  * dtype: arithmetic and u type; dtype_f / dtype_out / dtype_c: f, output and

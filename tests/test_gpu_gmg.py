@@ -273,11 +273,13 @@ def test_heterogeneous_smoothers(split, op_kind):
 @pytest.mark.parametrize("filler", [R.BILINEAR, R.BIQUADRATIC])
 @pytest.mark.parametrize("prec", [R.F64, R.F32])
 @pytest.mark.parametrize("op_kind", [0, 1])
-def test_fused_vcycle_matches_unfused(monkeypatch, filler, prec, op_kind):
+@pytest.mark.parametrize("ns", [(16, 16), (8, 8), (10, 6)])
+def test_fused_vcycle_matches_unfused(monkeypatch, filler, prec, op_kind, ns):
     """The fused 2D launches (ghosts computed inside the smoother / residual-restriction /
-    outer-operator kernels) give the same V-cycle and solve as one kernel per step."""
-    _print_case("fused-vs-unfused", filler, prec, op_kind)
-    ns = (16, 16)
+    outer-operator kernels; the matrix-core patch solve for 16 x 16, the generic one with
+    on-the-fly ghosts for other even sizes) give the same V-cycle and solve as one kernel
+    per step."""
+    _print_case("fused-vs-unfused", filler, prec, op_kind, ns)
     dim, levels = load("2d_uniform_8x8_refined_cross_mpi1.json", ns)
     coeffs = None
     if op_kind == 1:

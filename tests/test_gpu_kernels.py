@@ -271,13 +271,16 @@ def test_dft_mfma_matches_generic(ns, op_kind, monkeypatch):
 
 
 @pytest.mark.parametrize("name,ns", [("2d_uniform_8x8_refined_cross_mpi1.json", (16, 16)),
-                                     ("2d_uniform_2x2_refined_nw_mpi1.json", (32, 32))])
+                                     ("2d_uniform_2x2_refined_nw_mpi1.json", (32, 32)),
+                                     ("2d_uniform_8x8_refined_cross_mpi1.json", (8, 8)),
+                                     ("2d_uniform_2x2_refined_nw_mpi1.json", (10, 6))])
 @pytest.mark.parametrize("kind", [R.BILINEAR, R.BIQUADRATIC])
 @pytest.mark.parametrize("op_kind", [0, 1])
 def test_fused_vcycle_launches(name, ns, kind, op_kind):
     """te_dft_apply_mixed, te_dft_smooth_interp and te_residual_restrict against the oracle's
     unfused sequences (PatchSolver::apply; interpolate then smooth; residual then restrict),
-    FP64 and FP32 arithmetic, including the mixed-precision input / output types."""
+    FP64 and FP32 arithmetic, including the mixed-precision input / output types; 16 x 16 and
+    32 x 32 on the matrix cores, 8 x 8 and 10 x 6 in the generic patch-solve kernel."""
     _print_case("fused", name, ns, kind, op_kind)
     dim, levels = load(name, ns)
     fl, cl = levels[0], levels[1]

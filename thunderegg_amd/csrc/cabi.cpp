@@ -614,8 +614,9 @@ int te_dft_apply_mixed(const te_dft_plan* p, int dtype, int dtype_f, int dtype_o
 {
   TE_ARG(p && f && out, "null argument");
   TE_ARG(dtype_ok(dtype) && dtype_ok(dtype_f) && dtype_ok(dtype_out), "bad dtype");
-  TE_ARG(p->use_mfma && dft_mfma_supported(p), "fused DFT launches need the matrix-core plan (2D, n 16/32, Dirichlet)");
-  return launch_dft_mfma_fused(p, dtype, dtype_f, dtype_out, f, out, out, true, 0, TE_F64, nullptr, nullptr, S(s));
+  TE_ARG((p->use_mfma && dft_mfma_supported(p)) || dft_generic_fused_supported(p),
+         "fused DFT launches need a 2D plan with even cells per side and BiLinear or BiQuadratic ghosts");
+  return launch_dft_fused(p, dtype, dtype_f, dtype_out, f, out, out, true, 0, TE_F64, nullptr, nullptr, S(s));
 }
 
 int te_dft_smooth_interp(const te_dft_plan* p, const te_level_link* l, int dtype, int dtype_f, int dtype_out,
@@ -625,9 +626,10 @@ int te_dft_smooth_interp(const te_dft_plan* p, const te_level_link* l, int dtype
   TE_ARG(out != u, "out must not alias u");
   TE_ARG(dtype_ok(dtype) && dtype_ok(dtype_f) && dtype_ok(dtype_out) && dtype_ok(dtype_c), "bad dtype");
   TE_ARG(l->fine == p->g->d, "the link's fine level is not the plan's domain");
-  TE_ARG(p->use_mfma && dft_mfma_supported(p), "fused DFT launches need the matrix-core plan (2D, n 16/32, Dirichlet)");
+  TE_ARG((p->use_mfma && dft_mfma_supported(p)) || dft_generic_fused_supported(p),
+         "fused DFT launches need a 2D plan with even cells per side and BiLinear or BiQuadratic ghosts");
   TE_ARG(p->g->kind == TE_BILINEAR || p->g->kind == TE_BIQUADRATIC, "fused ghosts: BiLinear or BiQuadratic");
-  return launch_dft_mfma_fused(p, dtype, dtype_f, dtype_out, f, u, out, false, 2, dtype_c, uc, l, S(s));
+  return launch_dft_fused(p, dtype, dtype_f, dtype_out, f, u, out, false, 2, dtype_c, uc, l, S(s));
 }
 
 int te_residual_restrict(const te_ghost_filler* g, const te_level_link* l, int op_kind, int neumann,

@@ -198,7 +198,7 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
       dtc = g->L[li + 1].dt;
     }
     TE_RC(sweep_timing(g, li, false));
-    TE_RC(launch_dft_mfma_fused(Lv.plan, Lv.dt, dtf, dto, f, buf[cur], out, mode == 0, mode == 0 ? 0 : mode, dtc,
+    TE_RC(launch_dft_fused(Lv.plan, Lv.dt, dtf, dto, f, buf[cur], out, mode == 0, mode == 0 ? 0 : mode, dtc,
                                 uc, mode == 2 ? Lv.link : nullptr, g->s));
     TE_RC(sweep_timing(g, li, true));
     if (to_ext) {
@@ -651,7 +651,8 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
     bool fused = allow && opts->cycle_type == TE_CYCLE_V;
     for (int i = 0; i < num_levels && fused; i++) {
       const GLevel& Lv = g->L[i];
-      fused = Lv.smoother == 0 && Lv.plan && Lv.plan->use_mfma && dft_mfma_supported(Lv.plan) &&
+      fused = Lv.smoother == 0 && Lv.plan &&
+              ((Lv.plan->use_mfma && dft_mfma_supported(Lv.plan)) || dft_generic_fused_supported(Lv.plan)) &&
               (i == num_levels - 1 || resid_restrict_fused_supported(Lv.gf));
     }
     g->fused = fused;

@@ -32,6 +32,12 @@ int launch_dft_mfma(const te_dft_plan* pl, int dt, int dtf, int dto, const void*
 // (uc: the coarser level's solution of dtype dtc, link: fine -> coarse).
 int launch_dft_mfma_fused(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
                           bool uzero, int gm, int dtc, const void* uc, const te_level_link* link, hipStream_t s);
+// The same fused sweeps for any 2D plan with even cells per side: the matrix-core kernel
+// where it applies, else the generic kernel with the ghosts (and the interpolation) computed
+// in the launch.
+bool dft_generic_fused_supported(const te_dft_plan* pl);
+int launch_dft_fused(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
+                     bool uzero, int gm, int dtc, const void* uc, const te_level_link* link, hipStream_t s);
 bool resid_restrict_fused_supported(const te_ghost_filler* g);
 int launch_resid_restrict(const te_ghost_filler* g, const te_level_link* link, int op_kind, int neumann,
                           const void* coeffs, int dt, int dtf, const void* f, const void* u, int dtc, void* fc,

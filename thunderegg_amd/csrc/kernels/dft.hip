@@ -12,17 +12,22 @@
 #include "../te_internal.h"
 #include "dft.h"
 #include "dft_wave.h"
+#include "ghost2d.h"
 #include "../tune.h"
 
 namespace te {
 namespace {
 
-template <typename T, typename TF, typename TO, int D, int OP, bool UZERO>
+// GM (2D): 0 = the ghost values of u are read from u (filled beforehand); 1 = they are
+// computed here from the neighbours' cells (ghost2d, the filler's closed forms); 2 = the same
+// on u + P uc (DirectInterpolator applied on the fly: uc is the coarser level's solution).
+// With GM != 0, out must not alias u.
+template <typename T, typename TF, typename TO, int D, int OP, bool UZERO, int GM = 0, int KIND = 0, typename TC = T>
 __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ dp, const int* __restrict__ ptype,
                                                   const T* __restrict__ mats, const T* __restrict__ s2tab,
                                                   const TF* __restrict__ f, const T* __restrict__ u,
                                                   const T* __restrict__ coeffs, TO* __restrict__ out, Layout L,
-                                                  int ppw)
+                                                  int ppw, FusedArgs<T, TC> fa)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n0 = L.n[0], n1 = L.n[1], n2 = D == 3 ? L.n[2] : 1;
@@ -70,13 +75,23 @@ __global__ void __launch_bounds__(256) dft_kernel(const DevPatch* __restrict__ d
           if (!edge || P.type[sbase + 2 * a + up] < 0) continue;
           int gidx = up ? gi + st[a] : gi - st[a];
           T ih2 = T(P.inv_h2[a]);
+          T ug, ub;  // the ghost and the boundary cell
+          if (GM == 0) {
+            ug = u[gidx];
+            ub = u[gi];
+          } else {
+            const UAcc<T, TC, GM == 2> U{ u, fa.uc, fa.parent, fa.porth, L };
+            const int sd = 2 * a + up;
+            ug = ghost2d<T, KIND>(side_info(P, sd), p, sd, a == 0 ? y : x, nn[a], nn[1 - a], U);
+            ub = U(p, x, y);
+          }
           if (OP == 0) {
-            v -= (u[gidx] + u[gi]) * ih2;
+            v -= (ug + ub) * ih2;
           } else {
             int cq = gi - p * L.ps;
             int cg = up ? cq + st[a] : cq - st[a];
             T cb = coeffs[p * L.cs + cq], cgv = coeffs[p * L.cs + cg];
-            v -= (u[gidx] + u[gi]) * (cb + cgv) * (ih2 * T(0.5));
+            v -= (ug + ub) * (cb + cgv) * (ih2 * T(0.5));
           }
         }
       }
@@ -302,9 +317,10 @@ void launch_d(const te_dft_plan* pl, const TF* f, const T* u, TO* out, bool uzer
   const DftGenericVariant v = dft_generic_variant(tune_variant(TUNE_DFT_GENERIC, D, d->ns[0], sizeof(T) == 8 ? TE_F64 : TE_F32));
   const int grid = (d->np + v.ppw - 1) / v.ppw;
   shmem *= (size_t)v.ppw;
+  const FusedArgs<T, T> fa{ nullptr, nullptr, nullptr };
 #define TE_DL(OP_, UZ_)                                                                                          \
   hipLaunchKernelGGL((dft_kernel<T, TF, TO, D, OP_, UZ_>), dim3(grid), dim3(v.ppw * v.tpp), shmem, s,          \
-                     d->d_patches, pl->d_ptype, mats, s2, f, u, cf, out, L, v.ppw)
+                     d->d_patches, pl->d_ptype, mats, s2, f, u, cf, out, L, v.ppw, fa)
   if (pl->op_kind == 0) {
     if (uzero) TE_DL(0, true); else TE_DL(0, false);
   } else {
@@ -320,7 +336,102 @@ void launch_t(const te_dft_plan* pl, const void* f, const void* u, void* out, bo
   else launch_d<T, TF, TO, 3>(pl, (const TF*)f, (const T*)u, (TO*)out, uzero, s);
 }
 
+// the fused sweeps of the generic kernel, 2D: gm 0 (ghosts read from u), 1 (computed here),
+// 2 (computed here on u + P uc)
+template <typename T, typename TF, typename TO, typename TC>
+void launch_fused_t(const te_dft_plan* pl, const void* fv, const void* uv, void* outv, bool uzero, int gm,
+                    const void* ucv, const te_level_link* link, hipStream_t s)
+{
+  const te_domain* d = pl->g->d;
+  Layout L = make_layout(2, d->ns, 1, d->np);
+  const int nmax = d->ns[0] > d->ns[1] ? d->ns[0] : d->ns[1];
+  const int N = d->ns[0] * d->ns[1];
+  size_t shmem = sizeof(T) * (2 * (size_t)N + 4 * (size_t)nmax * nmax);
+  const T* mats = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32);
+  const T* s2 = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32);
+  const T* cf = (const T*)(sizeof(T) == 8 ? (const void*)pl->d_coeffs64 : (const void*)pl->d_coeffs32);
+  const DftGenericVariant v =
+    dft_generic_variant(tune_variant(TUNE_DFT_GENERIC, 2, d->ns[0], sizeof(T) == 8 ? TE_F64 : TE_F32));
+  const int grid = (d->np + v.ppw - 1) / v.ppw;
+  shmem *= (size_t)v.ppw;
+  const FusedArgs<T, TC> fa{ (const TC*)ucv, link ? link->d_parent : nullptr, link ? link->d_orth : nullptr };
+  const TF* f = (const TF*)fv;
+  const T* u = (const T*)uv;
+  TO* out = (TO*)outv;
+  const int kind = pl->g->kind;
+#define TE_DF(OP_, UZ_, GM_, K_)                                                                                 \
+  hipLaunchKernelGGL((dft_kernel<T, TF, TO, 2, OP_, UZ_, GM_, K_, TC>), dim3(grid), dim3(v.ppw * v.tpp), shmem, \
+                     s, d->d_patches, pl->d_ptype, mats, s2, f, u, cf, out, L, v.ppw, fa)
+#define TE_DFO(OP_)                                                        \
+  do {                                                                     \
+    if (uzero) TE_DF(OP_, true, 0, 0);                                     \
+    else if (gm == 0) TE_DF(OP_, false, 0, 0);                             \
+    else if (gm == 1 && kind == TE_BILINEAR) TE_DF(OP_, false, 1, TE_BILINEAR); \
+    else if (gm == 1) TE_DF(OP_, false, 1, TE_BIQUADRATIC);                \
+    else if (kind == TE_BILINEAR) TE_DF(OP_, false, 2, TE_BILINEAR);       \
+    else TE_DF(OP_, false, 2, TE_BIQUADRATIC);                             \
+  } while (0)
+  if (pl->op_kind == 0) TE_DFO(0);
+  else TE_DFO(1);
+#undef TE_DFO
+#undef TE_DF
+}
+
 }  // namespace
+
+bool dft_generic_fused_supported(const te_dft_plan* pl)
+{
+  const te_domain* d = pl->g->d;
+  return d->dim == 2 && d->ns[0] % 2 == 0 && d->ns[1] % 2 == 0 &&
+         (pl->g->kind == TE_BILINEAR || pl->g->kind == TE_BIQUADRATIC);
+}
+
+int launch_dft_fused(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out,
+                     bool uzero, int gm, int dtc, const void* uc, const te_level_link* link, hipStream_t s)
+{
+  if (pl->use_mfma && dft_mfma_supported(pl))
+    return launch_dft_mfma_fused(pl, dt, dtf, dto, f, u, out, uzero, gm, dtc, uc, link, s);
+  if (pl->g->d->np == 0) return TE_OK;
+  if (!dft_generic_fused_supported(pl)) {
+    set_error("fused DFT sweep: needs 2D, even cells per side, BiLinear or BiQuadratic ghosts");
+    return TE_ERR_UNSUPPORTED;
+  }
+  if (gm == 2 && (!uc || !link)) {
+    set_error("fused interpolation needs the coarser level's u and the level link");
+    return TE_ERR_ARG;
+  }
+  if (gm != 0 && !uzero && out == u) {
+    set_error("internal: a fused sweep that computes ghosts must not write its input");
+    return TE_ERR_ARG;
+  }
+  const bool c64 = gm != 2 || dtc == TE_F64;
+#define TE_LF(T_, TF_, TO_)                                                                      \
+  do {                                                                                           \
+    if (c64) launch_fused_t<T_, TF_, TO_, double>(pl, f, u, out, uzero, gm, uc, link, s);        \
+    else launch_fused_t<T_, TF_, TO_, float>(pl, f, u, out, uzero, gm, uc, link, s);             \
+  } while (0)
+  if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64) {
+    if (!c64) {
+      set_error("FP64 level with an FP32 coarser level is not supported");
+      return TE_ERR_UNSUPPORTED;
+    }
+    launch_fused_t<double, double, double, double>(pl, f, u, out, uzero, gm, uc, link, s);
+  } else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32) {
+    TE_LF(float, float, float);
+  } else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F32) {
+    TE_LF(float, double, float);
+  } else if (dt == TE_F32 && dtf == TE_F64 && dto == TE_F64) {
+    TE_LF(float, double, double);
+  } else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F64) {
+    TE_LF(float, float, double);
+  } else {
+    set_error("unsupported dtype combination for the DFT patch solver");
+    return TE_ERR_UNSUPPORTED;
+  }
+#undef TE_LF
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
 
 int launch_dft(const te_dft_plan* pl, int dt, int dtf, int dto, const void* f, const void* u, void* out, bool uzero,
                hipStream_t s)

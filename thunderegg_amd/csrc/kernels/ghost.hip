@@ -6,6 +6,7 @@
 // first (fillGhostCellsForLocalPatch runs before processLocalFills), then the
 // neighbour contributions in the order their calls are enumerated.
 #include "../te_internal.h"
+#include "ghost2d.h"
 
 namespace te {
 namespace {
@@ -30,99 +31,43 @@ __device__ __forceinline__ int side_cell3(const Layout& L, int p, int c, int s, 
   return lin(L, p, c, i, j, ca);
 }
 
+// 2D side ghosts: one wavefront per patch walks the patch's ring of side ghost cells
+// (ring_cell order: the two x faces, then the two y faces, so the y faces' ghost rows are
+// written by consecutive lanes).  One 16-byte ring-table record per cell names the
+// neighbour its formula reads; the closed forms are ghost2d.h's (self terms, then the
+// neighbour terms, in the reference's order).  Patches are split over the XCDs
+// (xcd_range), so a patch and most of its neighbours are read through one L2.
 template <typename T, int KIND>
-__global__ void __launch_bounds__(256) ghost_faces_2d(const DevPatch* __restrict__ dp, T* __restrict__ u, Layout L,
-                                                      int tmax)
+__global__ void __launch_bounds__(256) ghost_ring_2d(const int4* __restrict__ ring, T* __restrict__ u, Layout L)
 {
-  int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  int per = 4 * tmax;
-  int p = gid / per;
-  if (p >= L.np) return;
-  int r = gid - p * per;
-  int s = r / tmax;
-  int t = r - s * tmax;
-  int n = L.n[1 - (s >> 1)];
-  if (t >= n) return;
-  int slot = 4 + s;
-  int type = dp[p].type[slot];
-  if (type < 0) return;
-  const T c23 = T(2.0 / 3.0), m13 = T(-1.0 / 3.0);
-  for (int c = 0; c < L.ncomp; c++) {
-    T G;
-    if (type == NB_NORMAL) {
-      int q = dp[p].nbr[slot][0];
-      G = u[side_cell2(L, q, c, s ^ 1, 0, t)];
-    } else if (type == NB_COARSE) {
-      int q = dp[p].nbr[slot][0];
-      int o = dp[p].orth[slot];
-      if (KIND == TE_BILINEAR) {
-        // self (BiLinearGhostFiller.cpp:120-139): G[t] gets 2/3 S[t] and -1/3 S[t'] from
-        // the partner t' of t, in increasing t' order
-        int off = o ? n : 0;
-        G = T(0);
-        for (int tp = t - 1; tp <= t + 1; tp++) {
-          if (tp < 0 || tp >= n) continue;
-          T sv = u[side_cell2(L, p, c, s, 0, tp)];
-          if (tp == t) {
-            G += c23 * sv;
-          } else {
-            int partner = ((tp + off) % 2 == 0) ? tp + 1 : tp - 1;
-            if (partner == t) G += m13 * sv;
-          }
-        }
-        // pushed by the coarse neighbour (:92-107)
-        G += c23 * u[side_cell2(L, q, c, s ^ 1, 0, (t + off) / 2)];
-      } else {
-        // self (BiQuadraticGhostFiller.cpp:44-56)
-        T b = u[side_cell2(L, p, c, s, 0, t)];
-        T in = u[side_cell2(L, p, c, s, 1, t)];
-        G = T(2) * b / T(3) - in / T(5);
-        // pushed by the coarse neighbour (:170-226)
-#define CC(k) u[side_cell2(L, q, c, s ^ 1, 0, (k))]
-        T add;
-        if (o == 0) {
-          if (t == 0) add = T(3) * CC(0) / T(4) - T(3) * CC(1) / T(10) + CC(2) / T(12);
-          else if (t == 1) add = T(7) * CC(0) / T(20) + T(7) * CC(1) / T(30) - CC(2) / T(20);
-          else if (t % 2 == 0) add = CC(t / 2 - 1) / T(12) + CC(t / 2) / T(2) - CC(t / 2 + 1) / T(20);
-          else add = -CC(t / 2 - 1) / T(20) + CC(t / 2) / T(2) + CC(t / 2 + 1) / T(12);
-        } else {
-          if (t == n - 1) add = T(3) * CC(n - 1) / T(4) - T(3) * CC(n - 2) / T(10) + CC(n - 3) / T(12);
-          else if (t == n - 2) add = T(7) * CC(n - 1) / T(20) + T(7) * CC(n - 2) / T(30) - CC(n - 3) / T(20);
-          else {
-            int m = (t + n) / 2;
-            if ((t + n) % 2 == 0) add = CC(m - 1) / T(12) + CC(m) / T(2) - CC(m + 1) / T(20);
-            else add = -CC(m - 1) / T(20) + CC(m) / T(2) + CC(m + 1) / T(12);
-          }
-        }
-#undef CC
-        G += add;
-      }
-    } else {  // NB_FINE
-      if (KIND == TE_BILINEAR) {
-        G = m13 * u[side_cell2(L, p, c, s, 0, t)];  // :151-159
-      } else {
-        // BiQuadraticGhostFiller.cpp:58-81
-#define SB(k) u[side_cell2(L, p, c, s, 0, (k))]
-        if (t == 0) G = -SB(0) / T(10) + SB(1) / T(15) - SB(2) / T(30);
-        else if (t == n - 1) G = -SB(n - 1) / T(10) + SB(n - 2) / T(15) - SB(n - 3) / T(30);
-        else G = -SB(t - 1) / T(30) - SB(t + 1) / T(30);
-#undef SB
-      }
-      // pushed by the fine neighbours, in Orthant<1> order (:63-78 / :113-155)
-      for (int i = 0; i < 2; i++) {
-        int q = dp[p].nbr[slot][i];
-        int off = i * n;
-        for (int tf = 2 * t - off; tf <= 2 * t - off + 1; tf++) {
-          if (tf < 0 || tf >= n) continue;
-          if (KIND == TE_BILINEAR) {
-            G += c23 * u[side_cell2(L, q, c, s ^ 1, 0, tf)];
-          } else {
-            G += u[side_cell2(L, q, c, s ^ 1, 0, tf)] / T(3) + u[side_cell2(L, q, c, s ^ 1, 1, tf)] / T(5);
-          }
-        }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n0 = L.n[0], n1 = L.n[1], nr = 2 * (n0 + n1);
+  const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, L.np, 4);
+  for (int p = xr.first + wave; p < xr.end; p += xr.step) {
+    for (int g = lane; g < nr; g += 64) {
+      int s, t;
+      const int4 r = ring[ring_index(p, g, n0, n1, &s, &t)];
+      if (r.x < 0) continue;  // a physical side: the operators apply its condition themselves
+      const int a = s >> 1, up = s & 1;
+      const int na = a == 0 ? n0 : n1, nt = a == 0 ? n1 : n0;
+      int bx, by;
+      side_xy(s, 0, t, na, &bx, &by);
+      const int gx = a == 0 ? (up ? n0 : -1) : bx, gy = a == 0 ? by : (up ? n1 : -1);
+      for (int c = 0; c < L.ncomp; c++) {
+        T* uc = u + c * L.cs;
+        auto U = [&](int q, int x, int y) { return at(uc, q * L.ps + (y + 1) * L.sx + x + 1); };
+        T v[4] = { T(0), T(0), T(0), T(0) };
+        ghost2d_gather<KIND>(r.x, r.y, r.z, s, t, na, nt, [&](int k, int q, int x, int y) { v[k] = U(q, x, y); });
+        const T G = ghost2d_combine<T, KIND>(r.x, r.y, t, nt,
+                                             [&](int k, int tt) {
+                                               int x, y;
+                                               side_xy(s, k, tt, na, &x, &y);
+                                               return U(p, x, y);
+                                             },
+                                             v);
+        at(uc, p * L.ps + (gy + 1) * L.sx + gx + 1) = G;
       }
     }
-    u[side_cell2(L, p, c, s, -1, t)] = G;
   }
 }
 
@@ -315,13 +260,14 @@ int ghost_fill_t(const te_ghost_filler* g, T* u, int ncomp, hipStream_t s)
   Layout L = make_layout(d->dim, d->ns, ncomp, d->np);
   const int B = 256;
   if (d->dim == 2) {
-    int tmax = d->ns[0] > d->ns[1] ? d->ns[0] : d->ns[1];
-    long total = (long)d->np * 4 * tmax;
-    int grid = (int)((total + B - 1) / B);
+    // a wave per patch (4 per workgroup), at most 8192 workgroups walking the rest
+    const int ngroups = (d->np + 3) / 4;
+    const int grid = ngroups < 8192 ? ngroups : 8192;
     if (g->kind == TE_BILINEAR)
-      hipLaunchKernelGGL((ghost_faces_2d<T, TE_BILINEAR>), dim3(grid), dim3(B), 0, s, d->d_patches, u, L, tmax);
+      hipLaunchKernelGGL((ghost_ring_2d<T, TE_BILINEAR>), dim3(grid), dim3(B), 0, s, (const int4*)d->d_ring, u, L);
     else
-      hipLaunchKernelGGL((ghost_faces_2d<T, TE_BIQUADRATIC>), dim3(grid), dim3(B), 0, s, d->d_patches, u, L, tmax);
+      hipLaunchKernelGGL((ghost_ring_2d<T, TE_BIQUADRATIC>), dim3(grid), dim3(B), 0, s, (const int4*)d->d_ring, u,
+                         L);
     if (g->fill_type == TE_CORNERS) {
       int cg = (d->np * 4 + B - 1) / B;
       if (g->kind == TE_BILINEAR)

@@ -24,6 +24,8 @@ __device__ TI extrapolated_ghost(const DevPatch* __restrict__ fdp, const int32_t
                                  const TI* __restrict__ fine, const Layout& L, int p, int c, int X, int Y, int Z,
                                  int rem)
 {
+  // 3 g - i rounded after the product, as the reference's expression (no fused multiply-add)
+#pragma clang fp contract(off)
   const int cx[3] = { X, Y, Z };
   int outn = 0, bits = 0, axis = 0;
 #pragma unroll
