@@ -189,11 +189,8 @@ def tune_report(R, shapes):
     out = {"table": os.path.relpath(R.tune_table_path(), ROOT) if R.tune_table_path() else None, "variants": {}}
     for dim, n in sorted(set(shapes)):
         fam = R.tune_family(dim, n)
-        if fam is None:
-            out["variants"][f"{dim}d_n{n}"] = "no variants (compact 3D kernels)"
-            continue
         out["variants"][f"{dim}d_n{n}"] = {
-            "family": ["dft_generic", "dft_mfma_2d"][fam],
+            "family": ["dft_generic", "dft_mfma_2d", "c3_sweep"][fam],
             "f32": R.tune_get(fam, dim, n, R.F32),
             "f64": R.tune_get(fam, dim, n, R.F64),
         }

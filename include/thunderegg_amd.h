@@ -400,7 +400,8 @@ int te_batch_bicgstab(te_batch* b, const double* const* bv, double* const* x, in
  * first use. */
 enum te_tune_family {
   TE_TUNE_DFT_GENERIC = 0, /* VALU DFT patch solve: patches per workgroup x threads per patch */
-  TE_TUNE_DFT_MFMA_2D = 1  /* 2D matrix-core DFT patch solve (n 16, 32): workgroups per launch */
+  TE_TUNE_DFT_MFMA_2D = 1, /* 2D matrix-core DFT patch solve (n 16, 32): workgroups per launch */
+  TE_TUNE_C3_SWEEP = 2     /* 3D 16^3 fused sweeps: slices whose loads are issued together */
 };
 int te_tune_num_variants(int family);
 int te_tune_load(const char* path);
@@ -413,6 +414,8 @@ int te_tune_table_path(char* buf, size_t cap);
  * with variant `variant` forced, after one untimed launch; f, u, out of type dtype */
 int te_tune_time_dft(const te_dft_plan* pl, int dtype, const void* f, const void* u, void* out, int variant, int reps,
                      double* ms);
+/* te_gmg_time_kernel with a TE_TUNE_C3_SWEEP variant forced (the 3D compact cycle's sweeps) */
+int te_tune_time_c3(te_gmg* g, int which, int variant, int reps, double* ms);
 
 /* ------------------------------------------------------------- timings --
  * Timer (Timer.h:28-204, Timer.cpp:160-781): nested named timings, each optionally tied to

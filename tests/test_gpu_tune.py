@@ -73,3 +73,23 @@ def test_mfma_variants_in_the_fused_cycle(n, clean_table):
         res.append(g.vcycle(f))
     for k, r in enumerate(res[1:], 1):
         assert np.array_equal(r, res[0]), f"variant {k}"
+
+
+@pytest.mark.parametrize("prec", [R.F32, R.F64])
+def test_c3_sweep_variants_identical(prec, clean_table):
+    """the 3D compact cycle's sweeps (modes 1 and 2) under every load-batch variant give the
+    hand default's bits"""
+    from thunderegg_amd import solver
+
+    _print_case("tune c3", prec)
+    levels = uniform_levels(3, 2, (16, 16, 16))
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((len(levels[0]), 1, 18, 18, 18))
+    res = []
+    for v in range(R.tune_num_variants(R.TUNE_C3_SWEEP)):
+        R.tune_set(R.TUNE_C3_SWEEP, 3, 16, prec, v)
+        g = solver.GMG(3, (16, 16, 16), levels, filler_kind=R.TRILINEAR, smoother=0, pre_sweeps=2, post_sweeps=2,
+                       precision=prec)
+        res.append(g.vcycle(f))
+    for k, r in enumerate(res[1:], 1):
+        assert np.array_equal(r, res[0]), f"variant {k}"

@@ -14,6 +14,7 @@ Usage: python -m thunderegg_amd.autotune [--out PATH] [--reps N] [--margin F]
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import os
 import sys
 import time
@@ -28,6 +29,8 @@ DEFAULT_OUT = os.path.join(HERE, "tune_table.txt")
 
 # (dim, n, uniform level): about 4-16 thousand patches each
 SHAPES = [(2, 8, 7), (2, 16, 7), (2, 32, 6), (3, 8, 4)]
+# the 3D compact cycle (16^3 patches): its mode-1 and mode-2 sweeps on a uniform level
+C3_LEVEL = 4
 
 
 def _level(dim, n, level):
@@ -66,6 +69,40 @@ def tune_shape(dim, n, level, dtype, reps=20, margin=0.03, log=print):
     return fam, best, times
 
 
+def tune_c3(dtype, reps=20, margin=0.03, log=print):
+    """The 3D fused sweeps' variants (slices per load batch) timed on the compact cycle of a
+    uniform level of 16^3 patches (te_tune_time_c3: mode-1 plus mode-2 sweep); each
+    variant's cycle result must have the default's bits."""
+    from . import solver
+
+    levels = uniform_levels(3, C3_LEVEL, (16, 16, 16))
+    g = solver.GMG(3, (16, 16, 16), levels, filler_kind=R.TRILINEAR, smoother=0, pre_sweeps=2, post_sweeps=2,
+                   precision=dtype)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(g.shape)
+    fam = R.TUNE_C3_SWEEP
+    times, outs = {}, {}
+    for v in range(R.tune_num_variants(fam)):
+        t = 0.0
+        for name in ("sweep", "sweep_interp"):
+            ms = C.c_double()
+            R.check(R.lib().te_tune_time_c3(g.h, g.KERNELS[name], v, reps, C.byref(ms)))
+            t += ms.value
+        times[v] = t
+        R.tune_set(fam, 3, 16, dtype, v)
+        outs[v] = g.vcycle(f)
+    R.tune_clear()
+    for v, o in outs.items():
+        if not np.array_equal(o, outs[0]):
+            raise RuntimeError(f"3D sweep variant {v} changed the cycle's result")
+    best = min(times, key=times.get)
+    if times[best] > times[0] * (1.0 - margin):
+        best = 0
+    log(f"[autotune] 3D n=16 {'f64' if dtype == R.F64 else 'f32'} family {fam} ({len(levels[0])} patches, "
+        "mode-1 + mode-2 sweep): " + ", ".join(f"v{v} {t * 1e3:.1f}us" for v, t in times.items()) + f" -> v{best}")
+    return fam, best, times
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--out", default=DEFAULT_OUT)
@@ -85,6 +122,11 @@ def main(argv=None):
             lines.append("# " + " ".join(f"v{v}={t * 1e3:.1f}" for v, t in times.items())
                          + f"  gain {times[0] / times[best]:.3f}x")
             lines.append(f"{fam} {dim} {n} {dtype} {best}")
+    for dtype in (R.F32, R.F64):
+        fam, best, times = tune_c3(dtype, args.reps, args.margin, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+        lines.append("# " + " ".join(f"v{v}={t * 1e3:.1f}" for v, t in times.items())
+                     + f"  gain {times[0] / times[best]:.3f}x")
+        lines.append(f"{fam} 3 16 {dtype} {best}")
     with open(args.out, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     print(args.out)

@@ -29,6 +29,7 @@
 #include "kernels/tail2d.h"
 #include "gmg_internal.h"
 #include "te_internal.h"
+#include "tune.h"
 
 using namespace te;
 
@@ -1218,6 +1219,18 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (!rc) *ms = (double)t / reps;
+  return rc;
+}
+
+int te_tune_time_c3(te_gmg* g, int which, int variant, int reps, double* ms)
+{
+  if (!g || variant < 0 || variant >= tune_num_variants(TUNE_C3_SWEEP)) {
+    set_error("te_tune_time_c3: bad argument");
+    return TE_ERR_ARG;
+  }
+  tune_force(TUNE_C3_SWEEP, 3, kC3N, g->L[0].dt, variant);
+  const int rc = te_gmg_time_kernel(g, which, reps, ms);
+  tune_unforce();
   return rc;
 }
 

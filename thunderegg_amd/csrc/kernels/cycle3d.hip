@@ -32,6 +32,7 @@
 #include "dft_wave.h"
 #include "fold.h"
 #include "krylov.h"
+#include "../tune.h"
 
 namespace te {
 namespace {
@@ -120,7 +121,8 @@ struct C3U {
 
 // ------------------------------------------------------------------------------------------
 // The patch solve.  MODE 0: u = 0 (f only); 1: interface terms from uin; 2: from uin + P uc.
-template <typename T, typename TF, typename TO, typename TC, int MODE>
+// SB: slices per load batch of modes 1 / 2 (the autotuner's variants; mode 0 uses 4)
+template <typename T, typename TF, typename TO, typename TC, int MODE, int SB>
 __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restrict__ dp,
                                                        const int* __restrict__ ptype, const T* __restrict__ mats,
                                                        const T* __restrict__ s2tab, const TF* __restrict__ f,
@@ -204,8 +206,8 @@ __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restric
 
   // ---- forward transforms along x and y, one z-slice at a time; the loads of a batch of
   // SB slices (f, and for the interface terms the lane's own boundary rows and the
-  // neighbours' facing rows) are all issued before its first product
-  constexpr int SB = MODE == 0 ? 4 : (sizeof(T) == 4 ? 2 : 1);
+  // neighbours' facing rows) are all issued before its first product (more slices per
+  // batch: more loads in flight, more registers, fewer waves per SIMD)
 #pragma unroll
   for (int i0 = 0; i0 < 4; i0 += SB) {
     T F[SB][4], ub[SB][4], gx[SB][4], gy[SB][4], gz[SB][4];
@@ -616,16 +618,25 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
   }
   const int32_t* par = link ? link->d_parent : nullptr;
   const int32_t* por = link ? link->d_orth : nullptr;
-#define TE_C3S(T_, TF_, TO_, TC_, M_)                                                                                \
-  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_>), dim3(d->np), dim3(256), 0, s, d->d_patches,         \
+  // slices per load batch of modes 1 / 2: the autotuner's choice, else 2 (FP32) / 1 (FP64)
+  const int var = tune_variant(TUNE_C3_SWEEP, 3, N, dt);
+  const int sb = c3_sweep_batch(var, dt);
+#define TE_C3S(T_, TF_, TO_, TC_, M_, SB_)                                                                           \
+  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_, SB_>), dim3(d->np), dim3(256), 0, s, d->d_patches,    \
                      pl->d_ptype, (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32), \
                      (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32),             \
                      (const TF_*)f, (const T_*)uin, (TO_*)out, (const TC_*)uc, par, por)
-#define TE_C3M(T_, TF_, TO_, TC_)               \
-  do {                                          \
-    if (mode == 0) TE_C3S(T_, TF_, TO_, TC_, 0); \
-    else if (mode == 1) TE_C3S(T_, TF_, TO_, TC_, 1); \
-    else TE_C3S(T_, TF_, TO_, TC_, 2);          \
+#define TE_C3B(T_, TF_, TO_, TC_, M_)                \
+  do {                                               \
+    if (sb == 1) TE_C3S(T_, TF_, TO_, TC_, M_, 1);   \
+    else if (sb == 2) TE_C3S(T_, TF_, TO_, TC_, M_, 2); \
+    else TE_C3S(T_, TF_, TO_, TC_, M_, 4);           \
+  } while (0)
+#define TE_C3M(T_, TF_, TO_, TC_)                    \
+  do {                                               \
+    if (mode == 0) TE_C3S(T_, TF_, TO_, TC_, 0, 4);  \
+    else if (mode == 1) TE_C3B(T_, TF_, TO_, TC_, 1); \
+    else TE_C3B(T_, TF_, TO_, TC_, 2);               \
   } while (0)
   if (dt == TE_F64 && dtf == TE_F64 && dto == TE_F64 && dtc == TE_F64) TE_C3M(double, double, double, double);
   else if (dt == TE_F32 && dtf == TE_F32 && dto == TE_F32 && dtc == TE_F32) TE_C3M(float, float, float, float);
@@ -641,6 +652,7 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
     return TE_ERR_UNSUPPORTED;
   }
 #undef TE_C3M
+#undef TE_C3B
 #undef TE_C3S
   TE_HIP(hipGetLastError());
   return TE_OK;

@@ -129,10 +129,20 @@ int tune_num_variants(int family)
 {
   if (family == TUNE_DFT_GENERIC) return kGenericCount;
   if (family == TUNE_DFT_MFMA_2D) return MFMA_GRID_VARIANTS;
+  if (family == TUNE_C3_SWEEP) return 4;
   return 0;
 }
 
 DftGenericVariant dft_generic_variant(int v) { return kGeneric[v >= 0 && v < kGenericCount ? v : 0]; }
+
+void tune_force(int family, int dim, int n, int dtype, int variant)
+{
+  forced.on = true;
+  forced.k = Key{ family, dim, n, dtype };
+  forced.variant = variant;
+}
+
+void tune_unforce() { forced.on = false; }
 
 int tune_variant(int family, int dim, int n, int dtype)
 {

@@ -11,6 +11,7 @@ namespace te {
 enum TuneFamily {
   TUNE_DFT_GENERIC = 0,  // DFT patch solve, VALU kernel (any n): patches per workgroup x threads per patch
   TUNE_DFT_MFMA_2D = 1,  // DFT patch solve, 2D matrix-core kernel (n 16, 32): workgroups of the launch
+  TUNE_C3_SWEEP = 2,     // 3D 16^3 fused sweeps (modes 1, 2): slices per load batch
   TUNE_FAMILIES
 };
 
@@ -18,6 +19,9 @@ enum TuneFamily {
 // else the table's, else 0 (the hand default)
 int tune_variant(int family, int dim, int n, int dtype);
 int tune_num_variants(int family);
+// force `variant` for (family, dim, n, dtype) on this thread until tune_unforce (timing calls)
+void tune_force(int family, int dim, int n, int dtype, int variant);
+void tune_unforce();
 
 struct DftGenericVariant {
   int ppw;  // patches per workgroup
@@ -34,5 +38,15 @@ enum DftMfmaGrid {
   MFMA_GRID_ALL = 3,        // one workgroup per 4 patches, plain kernel
   MFMA_GRID_VARIANTS
 };
+
+// 3D fused sweeps: slices per load batch of variant v (0: 2 in FP32, 1 in FP64; 1, 2, 3:
+// 1, 2, 4 slices)
+inline int c3_sweep_batch(int v, int dtype)
+{
+  if (v == 1) return 1;
+  if (v == 2) return 2;
+  if (v == 3) return 4;
+  return dtype == 1 ? 2 : 1;  // TE_F32 = 1
+}
 
 }  // namespace te

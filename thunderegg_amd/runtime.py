@@ -105,6 +105,7 @@ SIGNATURES = {
     "te_tune_get": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "te_tune_table_path": (C.c_int, [C.c_char_p, C.c_size_t]),
     "te_tune_time_dft": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _dp]),
+    "te_tune_time_c3": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "te_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp), _ip, _vp, C.POINTER(_vp)]),
     "te_batch_destroy": (C.c_int, [_vp]),
     "te_batch_num_groups": (C.c_int, [_vp]),
@@ -408,7 +409,7 @@ class DftPlan:
 
 
 # the autotuner's table (te_tune_*): families
-TUNE_DFT_GENERIC, TUNE_DFT_MFMA_2D = 0, 1
+TUNE_DFT_GENERIC, TUNE_DFT_MFMA_2D, TUNE_C3_SWEEP = 0, 1, 2
 
 
 def tune_get(family, dim, n, dtype):
@@ -438,10 +439,10 @@ def tune_table_path():
 
 
 def tune_family(dim, n, use_mfma=True):
-    """the family of the DFT patch solve a level of this shape runs (None: 3D 16^3 patches,
-    whose matrix-core kernels have no variants)"""
+    """the tuned family of the DFT patch solve a level of this shape runs (3D 16^3: the
+    compact cycle's fused sweeps)"""
     if use_mfma and dim == 3 and n == 16:
-        return None
+        return TUNE_C3_SWEEP
     return TUNE_DFT_MFMA_2D if (use_mfma and dim == 2 and n in (16, 32)) else TUNE_DFT_GENERIC
 
 
