@@ -149,3 +149,28 @@ def test_compact3d_fixture_mesh_and_cg():
     x, it, _ = g.cg(b, tol=1e-10)
     assert it == itr
     assert relerr(mesh.interior(x, 3), mesh.interior(xr, 3)) <= 1e-8
+
+
+@pytest.mark.parametrize("prec", [R.F64, R.F32])
+def test_compact3d_face_columns_bit_identical(monkeypatch, prec):
+    """The face columns the compact path keeps behind its solution buffers (the x = 0 and
+    x = 15 cells of every row, so a neighbour's x face is a contiguous read) hold exact
+    copies: cycles (pre-smoothing from zero, without pre-smoothing, without post-smoothing)
+    and solves give the same bits as reading the x faces from the cells (TE_NO_XCOL=1)."""
+    _print_case("compact3d face columns", prec)
+    levels = _levels(3)
+    f = random_field(mesh.padded_shape(512, 3, NS), 97)
+    b = _rhs_sin(levels)
+    for pre, post in ((2, 2), (0, 1), (1, 0)):
+        out = []
+        for nx in ("0", "1"):
+            monkeypatch.setenv("TE_NO_XCOL", nx)
+            g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=prec, pre_sweeps=pre,
+                           post_sweeps=post)
+            v = g.vcycle(f)
+            x, its, hist = g.solve(b, tol=1e-10, precond=2)
+            out.append((v, x, its, hist))
+        (v0, x0, i0, h0), (v1, x1, i1, h1) = out
+        assert np.array_equal(mesh.interior(v0, 3), mesh.interior(v1, 3)), (pre, post)
+        assert i0 == i1 and np.array_equal(np.asarray(h0), np.asarray(h1)), (pre, post)
+        assert np.array_equal(mesh.interior(x0, 3), mesh.interior(x1, 3)), (pre, post)

@@ -260,11 +260,19 @@ int visit_fused(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_
   return TE_OK;
 }
 
+// the face columns of compact buffer v (np patches, type dt), or nullptr when the solver
+// does not keep them
+static void* xcol_of(const te_gmg* g, const void* v, long np, int dt)
+{
+  return g->xcol ? c3_xcol(v, np, dt) : nullptr;
+}
+
 // VCycle::visit (GMG/VCycle.h:46-70) on compact vectors (3D, cycle3d.h): per level one launch
 // per smoothing sweep and one for Cycle::restrict (residual and restriction); the
 // interpolation and the ghost values of every sweep are computed inside the sweeps'
 // launches.  Sweeps after the first read the solution from one buffer and write the other;
-// the finest level's last sweep writes u_ext (compact, type ext_dt).
+// the finest level's last sweep writes u_ext (compact, type ext_dt).  Every buffer a sweep
+// writes has face columns (cycle3d.h), written by the sweep and read by the launches after it.
 int visit_c3(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_dt)
 {
   GLevel& Lv = g->L[li];
@@ -282,7 +290,7 @@ int visit_c3(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_dt)
     const int dtc = mode == 2 ? g->L[li + 1].dt : TE_F64;
     TE_RC(sweep_timing(g, li, false));
     TE_RC(launch_c3_sweep(Lv.plan, Lv.dt, dtf, f, buf[cur], dto, out, mode, dtc, uc, mode == 2 ? Lv.link : nullptr,
-                          g->s));
+                          g->s, xcol_of(g, buf[cur], Lv.d->np, Lv.dt), xcol_of(g, out, Lv.d->np, dto)));
     TE_RC(sweep_timing(g, li, true));
     if (to_ext) {
       Cl.cur = u_ext;
@@ -293,7 +301,7 @@ int visit_c3(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_dt)
     return TE_OK;
   };
   auto zero_level = [&]() -> int {
-    TE_RC(launch_c3_zero(ncell, Lv.dt, buf[0], g->s));
+    TE_RC(launch_c3_zero(ncell + (long)Lv.d->np * kC3XCol, Lv.dt, buf[0], g->s));  // cells and face columns
     cur = 0;
     Cl.cur = buf[0];
     return TE_OK;
@@ -311,7 +319,8 @@ int visit_c3(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_dt)
   for (int k = 0; k < g->o.pre_sweeps; k++) TE_RC(sweep(k == 0 ? 0 : 1, false));
   if (g->o.pre_sweeps == 0) TE_RC(zero_level());
   GLevel& Cv = g->L[li + 1];
-  TE_RC(launch_c3_resid_restrict(Lv.d, Lv.link, g->o.neumann, Lv.dt, dtf, f, Cl.cur, Cv.dt, g->C3[li + 1].f, g->s));
+  TE_RC(launch_c3_resid_restrict(Lv.d, Lv.link, g->o.neumann, Lv.dt, dtf, f, Cl.cur, Cv.dt, g->C3[li + 1].f, g->s,
+                                 xcol_of(g, Cl.cur, Lv.d->np, Lv.dt)));
   TE_RC(visit_c3(g, li + 1, g->C3[li + 1].f, Cv.dt, u_ext, ext_dt));
   const int n = g->o.post_sweeps;
   if (n == 0) {
@@ -569,6 +578,8 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
       c3 = sk == 0 && c3_supported(levels[i]);
     }
     g->c3 = c3;
+    const char* nx = std::getenv("TE_NO_XCOL");
+    g->xcol = !(nx && nx[0] == '1');
   }
   for (int i = 0; i < num_levels && !rc; i++) {
     GLevel& Lv = g->L[i];
@@ -582,12 +593,14 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
     if (rc) break;
     long n = te_domain_vector_length(Lv.d, 1);
     if (g->c3) {
-      // compact level vectors only (the padded ones serve the one-launch-per-step cycle)
+      // compact level vectors only (the padded ones serve the one-launch-per-step cycle); the
+      // solutions u, u2 carry face columns
       g->C3.resize(num_levels);
-      const size_t cb = esize(Lv.dt) * (size_t)std::max<long>((long)Lv.d->np * kC3Cells, 1);
+      const size_t cf = esize(Lv.dt) * (size_t)std::max<long>((long)Lv.d->np * kC3Cells, 1);
+      const size_t cb = esize(Lv.dt) * (size_t)std::max<long>((long)Lv.d->np * (kC3Cells + kC3XCol), 1);
       te_gmg::C3Level& Cl = g->C3[i];
-      if ((e = hipMalloc(&Cl.f, cb)) != hipSuccess || (e = hipMalloc(&Cl.u, cb)) != hipSuccess ||
-          (e = hipMalloc(&Cl.u2, cb)) != hipSuccess || (e = hipMemset(Cl.f, 0, cb)) != hipSuccess ||
+      if ((e = hipMalloc(&Cl.f, cf)) != hipSuccess || (e = hipMalloc(&Cl.u, cb)) != hipSuccess ||
+          (e = hipMalloc(&Cl.u2, cb)) != hipSuccess || (e = hipMemset(Cl.f, 0, cf)) != hipSuccess ||
           (e = hipMemset(Cl.u, 0, cb)) != hipSuccess || (e = hipMemset(Cl.u2, 0, cb)) != hipSuccess) {
         set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
         rc = TE_ERR_HIP;
@@ -739,9 +752,11 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
         if (e == hipSuccess) e = hipMalloc(v, sizeof(double) * nc);
         if (e == hipSuccess) e = hipMemset(*v, 0, sizeof(double) * nc);
       }
+      // the cycle's outputs, with face columns (the outer operator reads M p and M s)
+      const size_t ncx = nc + (size_t)d0->np * kC3XCol;
       for (void** v : { &g->c_mp, &g->c_ms, &g->c_tmp }) {
-        if (e == hipSuccess) e = hipMalloc(v, em * nc);
-        if (e == hipSuccess) e = hipMemset(*v, 0, em * nc);
+        if (e == hipSuccess) e = hipMalloc(v, em * ncx);
+        if (e == hipSuccess) e = hipMemset(*v, 0, em * ncx);
       }
       if (g->L[0].dt == TE_F32) {
         for (float** v : { &g->c_p32, &g->c_s32 }) {
@@ -879,6 +894,10 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   void* c3_mp = use_precond ? g->c_mp : (void*)g->c_p;
   void* c3_ms = use_precond ? g->c_ms : (void*)g->c_s;
   const bool c3_32 = use_precond && L0.dt == TE_F32;
+  // the face columns of M p and M s: written when the cycle's last finest-level step is a sweep
+  const bool c3_xc = use_precond && (g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0);
+  const void* c3_xmp = c3_xc ? xcol_of(g, c3_mp, d->np, c3m) : nullptr;
+  const void* c3_xms = c3_xc ? xcol_of(g, c3_ms, d->np, c3m) : nullptr;
   auto iteration_c3 = [&]() -> int {
     C3Bicg a{};
     a.scal = g->scal;
@@ -894,7 +913,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.dtm = c3m;
     if (use_precond)
       TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_p32 : (const void*)g->c_p, c3_32 ? TE_F32 : TE_F64, c3_mp, c3m));
-    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_mp, g->c_ap, g->c_rhat, g->pa, nullptr, nb, s));
+    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_mp, g->c_ap, g->c_rhat, g->pa, nullptr, nb, s, c3_xmp));
     // s = r - alpha ap, alpha folded from the rhat.ap partials; s.s partials into pc
     a.w32 = c3_32 ? g->c_s32 : nullptr;
     a.pa = g->pc;
@@ -903,7 +922,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     TE_RC(launch_c3_bicg(BICG_S, d, a, nb, s));
     if (use_precond)
       TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_s32 : (const void*)g->c_s, c3_32 ? TE_F32 : TE_F64, c3_ms, c3m));
-    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_ms, g->c_as, g->c_s, g->pa, g->pb, nb, s));
+    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_ms, g->c_as, g->c_s, g->pa, g->pb, nb, s, c3_xms));
     TE_RC(launch_bicg_finalize(FIN_SOMEGA, g->pa, g->pb, nb, g->scal, s, 0.0, nullptr, g->pc));
     // x and r updates; after the s-norm test fired, x += alpha p (use_precond 2: alpha M p)
     a.w32 = nullptr;
@@ -1174,19 +1193,27 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
   hipStream_t s = g->s;
   const int nb = g->nblk;
   auto launch = [&]() -> int {
+    const long np = L0.d->np;
+    void* xu = xcol_of(g, C0.u, np, L0.dt);
+    void* xu2 = xcol_of(g, C0.u2, np, L0.dt);
     switch (which) {
       case TE_K_SWEEP_ZERO:
-        return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 0, TE_F64, nullptr, nullptr, s);
+        return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 0, TE_F64, nullptr, nullptr, s,
+                               nullptr, xu2);
       case TE_K_SWEEP:
-        return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 1, TE_F64, nullptr, nullptr, s);
+        return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 1, TE_F64, nullptr, nullptr, s, xu,
+                               xu2);
       case TE_K_SWEEP_INTERP:
         return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 2, g->L[1].dt, g->C3[1].u, L0.link,
-                               s);
+                               s, xu, xu2);
       case TE_K_RESID_RESTRICT:
         return launch_c3_resid_restrict(L0.d, L0.link, g->o.neumann, L0.dt, L0.dt, C0.f, C0.u, g->L[1].dt,
-                                        g->C3[1].f, s);
-      case TE_K_OPERATOR:
-        return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_ms, g->c_as, g->c_s, g->pa, g->pb, nb, s);
+                                        g->C3[1].f, s, xu);
+      case TE_K_OPERATOR: {
+        const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
+        return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_ms, g->c_as, g->c_s, g->pa, g->pb, nb, s,
+                                  xc ? xcol_of(g, g->c_ms, np, L0.dt) : nullptr);
+      }
       default: {
         C3Bicg a{};
         a.scal = g->scal;

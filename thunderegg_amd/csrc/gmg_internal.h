@@ -115,6 +115,9 @@ struct te_gmg {
   // 3D hierarchies of 16^3 patches whose face neighbours are all on the same level: the
   // cycle and the BiCGStab work on compact vectors (interior cells only, cycle3d.h)
   bool c3 = false;
+  // the compact path keeps face columns behind its solution buffers (cycle3d.h);
+  // TE_NO_XCOL=1 reads the x faces from the cells instead (same values)
+  bool xcol = true;
   struct C3Level {
     void* f = nullptr;
     void* u = nullptr;

@@ -13,6 +13,17 @@ namespace te {
 
 constexpr int kC3N = 16;
 constexpr int kC3Cells = kC3N * kC3N * kC3N;
+// Face columns of a compact vector: the cells x = 0 and x = N - 1 of every row (z, y) of
+// each patch, [patch][side][z][y], stored behind the vector's np * kC3Cells cells.  Rows
+// are contiguous in x, so a neighbour's x face read from the vector itself costs a 64-byte
+// (FP32) row per value; from the face columns, 16 values are one 64-byte read.  The
+// kernels that write a vector with face columns write them too; a reader is given them
+// only where every writer of the vector did so (nullptr: the x faces come from the cells).
+constexpr int kC3XCol = 2 * kC3N * kC3N;
+inline void* c3_xcol(const void* v, long np, int dt)
+{
+  return v ? (void*)((char*)v + (size_t)np * kC3Cells * (dt == TE_F64 ? 8 : 4)) : nullptr;
+}
 
 // 3D, 16^3 cells, every face neighbour on the same level
 bool c3_supported(const te_domain* d);
@@ -22,19 +33,22 @@ bool c3_supported(const te_domain* d);
 // modifyRHSForInternalBoundaryConditions from uin's boundary cells and the neighbours';
 // 2: the same on uin + P uc (DirectInterpolator, GMG/DirectInterpolator.h:60-91; uc is the
 // coarser level's compact solution of type dtc, link: this level -> the coarser one).
-// out must not alias uin.  dt: arithmetic type; dtf / dto: types of f and out.
+// out must not alias uin.  dt: arithmetic type; dtf / dto: types of f and out.  xin: uin's
+// face columns (or nullptr); xout: out's face columns to write (or nullptr).
 int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const void* uin, int dto, void* out,
-                    int mode, int dtc, const void* uc, const te_level_link* link, hipStream_t s);
+                    int mode, int dtc, const void* uc, const te_level_link* link, hipStream_t s,
+                    const void* xin = nullptr, void* xout = nullptr);
 // Cycle::restrict (GMG/Cycle.h:57-65): fc = restriction (GMG/LinearRestrictor.h:101-155) of
 // f - A u, A the Poisson star operator (neumann: Neumann physical sides); fc compact, of the
-// coarser level, type dtc.
+// coarser level, type dtc.  xu: u's face columns (or nullptr).
 int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int neumann, int dt, int dtf,
-                             const void* f, const void* u, int dtc, void* fc, hipStream_t s);
+                             const void* f, const void* u, int dtc, void* fc, hipStream_t s,
+                             const void* xu = nullptr);
 // out = A u (FP64, compact; u compact of type udt, widened as read) with the per-workgroup
 // partial sums pa = sum w out and, when pb is not null, pb = sum out out; nslots partial
-// slots, those past the launch's workgroups written zero.
+// slots, those past the launch's workgroups written zero.  xu: u's face columns (or nullptr).
 int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, double* out, const double* w,
-                       double* pa, double* pb, int nslots, hipStream_t s);
+                       double* pa, double* pb, int nslots, hipStream_t s, const void* xu = nullptr);
 // BiCGStab vector updates (Iterative/BiCGStab.h:111-203) on compact vectors, the modes and
 // scalar slots of krylov.h.  b and xpad are the caller's padded FP64 vectors; x is a compact
 // FP64 copy of the caller's x that the iterations update (written back at the end).

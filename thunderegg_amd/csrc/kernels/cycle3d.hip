@@ -75,6 +75,8 @@ __device__ __forceinline__ void st4(TD* __restrict__ base, int off, const T (&r)
 }
 
 __device__ __forceinline__ int cidx(int x, int y, int z) { return x + 16 * y + 256 * z; }
+// face column entry (side 0: x = 0, side 1: x = N - 1) of row (y, z) of patch p
+__device__ __forceinline__ int xci(int p, int side, int y, int z) { return p * kC3XCol + 256 * side + 16 * z + y; }
 
 // u (+ the coarser level's correction uc, INTERP) at cells of patch q, compact vectors
 template <typename T, typename TC, bool INTERP>
@@ -121,14 +123,17 @@ struct C3U {
 
 // ------------------------------------------------------------------------------------------
 // The patch solve.  MODE 0: u = 0 (f only); 1: interface terms from uin; 2: from uin + P uc.
-// SB: slices per load batch of modes 1 / 2 (the autotuner's variants; mode 0 uses 4)
-template <typename T, typename TF, typename TO, typename TC, int MODE, int SB>
+// SB: slices per load batch of modes 1 / 2 (the autotuner's variants; mode 0 uses 4).
+// XC: uin and out have face columns (cycle3d.h): the x faces are read from uin's, and out's
+// are written
+template <typename T, typename TF, typename TO, typename TC, int MODE, int SB, bool XC>
 __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restrict__ dp,
                                                        const int* __restrict__ ptype, const T* __restrict__ mats,
                                                        const T* __restrict__ s2tab, const TF* __restrict__ f,
                                                        const T* __restrict__ uin, TO* __restrict__ out,
                                                        const TC* __restrict__ uc, const int32_t* __restrict__ parent,
-                                                       const int32_t* __restrict__ porth)
+                                                       const int32_t* __restrict__ porth, const T* __restrict__ xin,
+                                                       TO* __restrict__ xout)
 {
   __shared__ __align__(16) T slab[kSlab];
   typedef typename Mf<T>::V V;
@@ -203,6 +208,19 @@ __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restric
   const bool onx = (g == 0 && has[0]) || (g == 3 && has[1]);
   const bool ony = (c == 0 && has[2]) || (c == N - 1 && has[3]);
   const int sx = g == 0 ? 0 : 1, sy = c == 0 ? 2 : 3;
+  // with face columns (XC) an x-face lane reads two values of its row, the own boundary cell
+  // (side xs) and the neighbour's facing cell (its side 1 - xs), as one code path for both
+  // x faces: the neighbour, its parent and orthant selected per lane here
+  const int xs = g == 0 ? 0 : 1;
+  const int qx = g == 0 ? q[0] : q[1];
+  const int qxpar = g == 0 ? qpar[0] : qpar[1], qxo = g == 0 ? qo[0] : qo[1];
+  // cell x = (side ? N - 1 : 0) of row (c, z) of patch qq from uin's face columns, u + P uc
+  // when interpolating (par, o: qq's parent and orthant): the value row() gives for it
+  auto xval = [&](int qq, int side, int par, int o, int z) -> T {
+    T v = xin[xci(qq, side, c, z)];
+    if (MODE == 2) v = T(v + T(uc[par * NC + U.coarse(o, side ? N - 1 : 0, c, z)]));
+    return v;
+  };
 
   // ---- forward transforms along x and y, one z-slice at a time; the loads of a batch of
   // SB slices (f, and for the interface terms the lane's own boundary rows and the
@@ -211,14 +229,23 @@ __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restric
 #pragma unroll
   for (int i0 = 0; i0 < 4; i0 += SB) {
     T F[SB][4], ub[SB][4], gx[SB][4], gy[SB][4], gz[SB][4];
+    T xo[SB], xn[SB];  // XC: the x-face lane's own boundary value and the neighbour's
 #pragma unroll
     for (int j = 0; j < SB; j++) {
       const int z = w + 4 * (i0 + j);
       ld4(f, p * NC + cidx(4 * g, c, z), F[j]);
       if (MODE != 0) {
         const bool onz = (z == 0 && has[4]) || (z == N - 1 && has[5]);
-        if (onx || ony || onz) row(6, p, 4 * g, c, z, ub[j]);
-        if (onx) row(sx, q[sx], g == 0 ? 12 : 0, c, z, gx[j]);
+        if (XC) {
+          if (onx) {
+            xo[j] = xval(p, xs, qpar[6], qo[6], z);
+            xn[j] = xval(qx, 1 - xs, qxpar, qxo, z);
+          }
+          if (ony || onz) row(6, p, 4 * g, c, z, ub[j]);
+        } else {
+          if (onx || ony || onz) row(6, p, 4 * g, c, z, ub[j]);
+          if (onx) row(sx, q[sx], g == 0 ? 12 : 0, c, z, gx[j]);
+        }
         if (ony) row(sy, q[sy], 4 * g, c == 0 ? N - 1 : 0, z, gy[j]);
         if (onz) row(z == 0 ? 4 : 5, q[z == 0 ? 4 : 5], 4 * g, c, z == 0 ? N - 1 : 0, gz[j]);
       }
@@ -229,8 +256,8 @@ __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restric
       if (MODE != 0) {
         // modifyRHSForInternalBoundaryConditions (Poisson/StarPatchOperator.h:165-180): on
         // every side with a neighbour, f_b -= (ghost + u_b) / h^2, sides in the reference's order
-        if (g == 0 && has[0]) F[j][0] -= (gx[j][3] + ub[j][0]) * ihx;
-        if (g == 3 && has[1]) F[j][3] -= (gx[j][0] + ub[j][3]) * ihx;
+        if (g == 0 && has[0]) F[j][0] -= ((XC ? xn[j] : gx[j][3]) + (XC ? xo[j] : ub[j][0])) * ihx;
+        if (g == 3 && has[1]) F[j][3] -= ((XC ? xn[j] : gx[j][0]) + (XC ? xo[j] : ub[j][3])) * ihx;
         if (ony) {
 #pragma unroll
           for (int k = 0; k < 4; k++) F[j][k] -= (gy[j][k] + ub[j][k]) * ihy;
@@ -302,8 +329,20 @@ __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restric
     V u2 = { 0, 0, 0, 0 };  // U[y][x] = sum_ky Syi[y][ky] R[ky][x]
 #pragma unroll
     for (int s = 0; s < 4; s++) u2 = Mf<T>::mma(SyiA[s], r1[s], u2);
+    TO v[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) out[p * NC + cidx(c, Mf<T>::row(r, g), z)] = TO(u2[r] * scale);
+    for (int r = 0; r < 4; r++) {
+      v[r] = TO(u2[r] * scale);
+      out[p * NC + cidx(c, Mf<T>::row(r, g), z)] = v[r];
+    }
+    if (XC && (c == 0 || c == N - 1)) {
+      if constexpr (sizeof(T) == 4) {
+        st4(xout, xci(p, c == 0 ? 0 : 1, 4 * g, z), v);  // FP32 layout: rows 4g .. 4g + 3
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) xout[xci(p, c == 0 ? 0 : 1, Mf<T>::row(r, g), z)] = v[r];
+      }
+    }
   }
 }
 
@@ -313,9 +352,9 @@ __global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restric
 // ghosts or boundary values of the z faces already in place).  bc: the physical ghost factor
 // (-1 Dirichlet, +1 Neumann, :115-144).  q: the six face neighbours (-1: physical side).
 template <typename T, typename TU>
-__device__ __forceinline__ void star_slice(const TU* __restrict__ u, const int (&q)[6], int lane, int c, int g, int z,
-                                           T bc, T ihx, T ihy, T ihz, const T (&cur)[4], const T (&prv)[4],
-                                           const T (&nxt)[4], T (&acc)[4])
+__device__ __forceinline__ void star_slice(const TU* __restrict__ u, const TU* __restrict__ xu, const int (&q)[6],
+                                           int lane, int c, int g, int z, T bc, T ihx, T ihy, T ihz,
+                                           const T (&cur)[4], const T (&prv)[4], const T (&nxt)[4], T (&acc)[4])
 {
   T xm = __shfl(cur[3], (lane - 16) & 63, 64);
   T xp = __shfl(cur[0], (lane + 16) & 63, 64);
@@ -325,8 +364,10 @@ __device__ __forceinline__ void star_slice(const TU* __restrict__ u, const int (
     ym[k] = __shfl(cur[k], (lane - 1) & 63, 64);
     yp[k] = __shfl(cur[k], (lane + 1) & 63, 64);
   }
-  if (g == 0) xm = q[0] >= 0 ? T(u[q[0] * NC + cidx(N - 1, c, z)]) : bc * cur[0];
-  if (g == 3) xp = q[1] >= 0 ? T(u[q[1] * NC + cidx(0, c, z)]) : bc * cur[3];
+  // x faces: the neighbour's facing column, from u's face columns when it has them
+  if (g == 0)
+    xm = q[0] >= 0 ? T(xu ? xu[xci(q[0], 1, c, z)] : u[q[0] * NC + cidx(N - 1, c, z)]) : bc * cur[0];
+  if (g == 3) xp = q[1] >= 0 ? T(xu ? xu[xci(q[1], 0, c, z)] : u[q[1] * NC + cidx(0, c, z)]) : bc * cur[3];
   if (c == 0) {
     if (q[2] >= 0) ld4(u, q[2] * NC + cidx(4 * g, N - 1, z), ym);
     else
@@ -371,7 +412,8 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
                                                                 const T* __restrict__ u, const TF* __restrict__ f,
                                                                 TC* __restrict__ fc,
                                                                 const int32_t* __restrict__ parent,
-                                                                const int32_t* __restrict__ porth, int np, int neumann)
+                                                                const int32_t* __restrict__ porth, int np, int neumann,
+                                                                const T* __restrict__ xu)
 {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
   const int ro = cidx(4 * g, c, 0);
@@ -396,7 +438,7 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
     T fv[4];
     ld4(f, base + 256 * z + ro, fv);
     T acc[4];
-    star_slice(u, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
+    star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
     T r[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) r[k] = fv[k] - acc[k];
@@ -429,7 +471,7 @@ template <typename TU, bool SQ>
 __global__ void __launch_bounds__(256) c3_star_dot_kernel(const DevPatch* __restrict__ dp, const TU* __restrict__ u,
                                                           double* __restrict__ out, const double* __restrict__ w,
                                                           double* __restrict__ pa, double* __restrict__ pb, int np,
-                                                          int nslots, int neumann)
+                                                          int nslots, int neumann, const TU* __restrict__ xu)
 {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
   const int ro = cidx(4 * g, c, 0);
@@ -451,7 +493,7 @@ __global__ void __launch_bounds__(256) c3_star_dot_kernel(const DevPatch* __rest
       double wv4[4];
       ld4(w, base + 256 * z + ro, wv4);
       double acc[4];
-      star_slice(u, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
+      star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
       st4(out, base + 256 * z + ro, acc);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -608,7 +650,8 @@ bool c3_supported(const te_domain* d)
 }
 
 int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const void* uin, int dto, void* out,
-                    int mode, int dtc, const void* uc, const te_level_link* link, hipStream_t s)
+                    int mode, int dtc, const void* uc, const te_level_link* link, hipStream_t s, const void* xin,
+                    void* xout)
 {
   const te_domain* d = pl->g->d;
   if (d->np == 0) return TE_OK;
@@ -621,11 +664,23 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
   // slices per load batch of modes 1 / 2: the autotuner's choice, else 2 (FP32) / 1 (FP64)
   const int var = tune_variant(TUNE_C3_SWEEP, 3, N, dt);
   const int sb = c3_sweep_batch(var, dt);
-#define TE_C3S(T_, TF_, TO_, TC_, M_, SB_)                                                                           \
-  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_, SB_>), dim3(d->np), dim3(256), 0, s, d->d_patches,    \
+  // face columns: written when out has them; then uin (modes 1 and 2) must have them too
+  const bool xc = xout != nullptr;
+  if (xc && mode != 0 && !xin) {
+    set_error("internal: a sweep that writes face columns needs its input's");
+    return TE_ERR_ARG;
+  }
+#define TE_C3X(T_, TF_, TO_, TC_, M_, SB_, XC_)                                                                      \
+  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_, SB_, XC_>), dim3(d->np), dim3(256), 0, s, d->d_patches, \
                      pl->d_ptype, (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32), \
                      (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32),             \
-                     (const TF_*)f, (const T_*)uin, (TO_*)out, (const TC_*)uc, par, por)
+                     (const TF_*)f, (const T_*)uin, (TO_*)out, (const TC_*)uc, par, por, (const T_*)xin,   \
+                     (TO_*)xout)
+#define TE_C3S(T_, TF_, TO_, TC_, M_, SB_)     \
+  do {                                         \
+    if (xc) TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true); \
+    else TE_C3X(T_, TF_, TO_, TC_, M_, SB_, false);   \
+  } while (0)
 #define TE_C3B(T_, TF_, TO_, TC_, M_)                \
   do {                                               \
     if (sb == 1) TE_C3S(T_, TF_, TO_, TC_, M_, 1);   \
@@ -654,18 +709,20 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
 #undef TE_C3M
 #undef TE_C3B
 #undef TE_C3S
+#undef TE_C3X
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
 
 int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int neumann, int dt, int dtf,
-                             const void* f, const void* u, int dtc, void* fc, hipStream_t s)
+                             const void* f, const void* u, int dtc, void* fc, hipStream_t s, const void* xu)
 {
   if (d->np == 0) return TE_OK;
   const int grid = (d->np + 3) / 4;
 #define TE_C3R(T_, TF_, TC_)                                                                                     \
   hipLaunchKernelGGL((c3_resid_restrict_kernel<T_, TF_, TC_>), dim3(grid), dim3(256), 0, s, d->d_patches,       \
-                     (const T_*)u, (const TF_*)f, (TC_*)fc, link->d_parent, link->d_orth, d->np, neumann)
+                     (const T_*)u, (const TF_*)f, (TC_*)fc, link->d_parent, link->d_orth, d->np, neumann, \
+                     (const T_*)xu)
   if (dt == TE_F64 && dtf == TE_F64 && dtc == TE_F64) TE_C3R(double, double, double);
   else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F32) TE_C3R(float, float, float);
   else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F64) TE_C3R(float, float, double);
@@ -681,7 +738,7 @@ int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int 
 }
 
 int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, double* out, const double* w,
-                       double* pa, double* pb, int nslots, hipStream_t s)
+                       double* pa, double* pb, int nslots, hipStream_t s, const void* xu)
 {
   if (nslots < 1 || nslots > kFoldMax) {
     set_error("internal: bad partial count");
@@ -692,7 +749,7 @@ int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, 
   if (grid < 1) grid = 1;
 #define TE_C3D(TU_, Q_)                                                                                         \
   hipLaunchKernelGGL((c3_star_dot_kernel<TU_, Q_>), dim3(grid), dim3(256), 0, s, d->d_patches, (const TU_*)u, out, \
-                     w, pa, pb, d->np, nslots, neumann)
+                     w, pa, pb, d->np, nslots, neumann, (const TU_*)xu)
   if (udt == TE_F64) {
     if (pb) TE_C3D(double, true);
     else TE_C3D(double, false);
