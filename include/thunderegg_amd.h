@@ -401,7 +401,7 @@ int te_batch_bicgstab(te_batch* b, const double* const* bv, double* const* x, in
 enum te_tune_family {
   TE_TUNE_DFT_GENERIC = 0, /* VALU DFT patch solve: patches per workgroup x threads per patch */
   TE_TUNE_DFT_MFMA_2D = 1, /* 2D matrix-core DFT patch solve (n 16, 32): workgroups per launch */
-  TE_TUNE_C3_SWEEP = 2     /* 3D 16^3 fused sweeps: slices whose loads are issued together */
+  TE_TUNE_C3_SWEEP = 2     /* 3D 16^3 fused sweeps: slices whose loads are issued together, waves per SIMD */
 };
 int te_tune_num_variants(int family);
 int te_tune_load(const char* path);

@@ -29,7 +29,7 @@ DEFAULT_OUT = os.path.join(HERE, "tune_table.txt")
 
 # (dim, n, uniform level): about 4-16 thousand patches each
 SHAPES = [(2, 8, 7), (2, 16, 7), (2, 32, 6), (3, 8, 4)]
-# the 3D compact cycle (16^3 patches): its mode-1 and mode-2 sweeps on a uniform level
+# the 3D compact cycle (16^3 patches): its finest-level sweeps on a uniform level
 C3_LEVEL = 4
 
 
@@ -70,9 +70,11 @@ def tune_shape(dim, n, level, dtype, reps=20, margin=0.03, log=print):
 
 
 def tune_c3(dtype, reps=20, margin=0.03, log=print):
-    """The 3D fused sweeps' variants (slices per load batch) timed on the compact cycle of a
-    uniform level of 16^3 patches (te_tune_time_c3: mode-1 plus mode-2 sweep); each
-    variant's cycle result must have the default's bits."""
+    """The 3D fused sweeps' variants (slices per load batch, registers sized for more waves
+    per SIMD) timed on the compact cycle of a uniform level of 16^3 patches
+    (te_tune_time_c3): the finest level's sweeps of one 2 + 2 V-cycle, i.e. the sweep from
+    zero, two interface sweeps and the interpolating one; each variant's cycle result must
+    have the default's bits."""
     from . import solver
 
     levels = uniform_levels(3, C3_LEVEL, (16, 16, 16))
@@ -84,10 +86,10 @@ def tune_c3(dtype, reps=20, margin=0.03, log=print):
     times, outs = {}, {}
     for v in range(R.tune_num_variants(fam)):
         t = 0.0
-        for name in ("sweep", "sweep_interp"):
+        for name, per_cycle in (("sweep_zero", 1), ("sweep", 2), ("sweep_interp", 1)):
             ms = C.c_double()
             R.check(R.lib().te_tune_time_c3(g.h, g.KERNELS[name], v, reps, C.byref(ms)))
-            t += ms.value
+            t += per_cycle * ms.value
         times[v] = t
         R.tune_set(fam, 3, 16, dtype, v)
         outs[v] = g.vcycle(f)
@@ -99,7 +101,7 @@ def tune_c3(dtype, reps=20, margin=0.03, log=print):
     if times[best] > times[0] * (1.0 - margin):
         best = 0
     log(f"[autotune] 3D n=16 {'f64' if dtype == R.F64 else 'f32'} family {fam} ({len(levels[0])} patches, "
-        "mode-1 + mode-2 sweep): " + ", ".join(f"v{v} {t * 1e3:.1f}us" for v, t in times.items()) + f" -> v{best}")
+        "finest-level sweeps of a 2 + 2 cycle): " + ", ".join(f"v{v} {t * 1e3:.1f}us" for v, t in times.items()) + f" -> v{best}")
     return fam, best, times
 
 

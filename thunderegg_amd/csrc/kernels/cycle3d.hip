@@ -125,9 +125,9 @@ struct C3U {
 // The patch solve.  MODE 0: u = 0 (f only); 1: interface terms from uin; 2: from uin + P uc.
 // SB: slices per load batch of modes 1 / 2 (the autotuner's variants; mode 0 uses 4).
 // XC: uin and out have face columns (cycle3d.h): the x faces are read from uin's, and out's
-// are written
-template <typename T, typename TF, typename TO, typename TC, int MODE, int SB, bool XC>
-__global__ void __launch_bounds__(256) c3_sweep_kernel(const DevPatch* __restrict__ dp,
+// are written.  WPE: the minimum waves per SIMD the registers are allocated for (1: no request)
+template <typename T, typename TF, typename TO, typename TC, int MODE, int SB, bool XC, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) c3_sweep_kernel(const DevPatch* __restrict__ dp,
                                                        const int* __restrict__ ptype, const T* __restrict__ mats,
                                                        const T* __restrict__ s2tab, const TF* __restrict__ f,
                                                        const T* __restrict__ uin, TO* __restrict__ out,
@@ -664,22 +664,30 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
   // slices per load batch of modes 1 / 2: the autotuner's choice, else 2 (FP32) / 1 (FP64)
   const int var = tune_variant(TUNE_C3_SWEEP, 3, N, dt);
   const int sb = c3_sweep_batch(var, dt);
+  const int wpe = c3_sweep_waves(var);  // with face columns only (the cycle's own path)
   // face columns: written when out has them; then uin (modes 1 and 2) must have them too
   const bool xc = xout != nullptr;
   if (xc && mode != 0 && !xin) {
     set_error("internal: a sweep that writes face columns needs its input's");
     return TE_ERR_ARG;
   }
-#define TE_C3X(T_, TF_, TO_, TC_, M_, SB_, XC_)                                                                      \
-  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_, SB_, XC_>), dim3(d->np), dim3(256), 0, s, d->d_patches, \
+#define TE_C3X(T_, TF_, TO_, TC_, M_, SB_, XC_, W_)                                                                  \
+  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_, SB_, XC_, W_>), dim3(d->np), dim3(256), 0, s,          \
+                     d->d_patches,                                                                                  \
                      pl->d_ptype, (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32), \
                      (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32),             \
                      (const TF_*)f, (const T_*)uin, (TO_*)out, (const TC_*)uc, par, por, (const T_*)xin,   \
                      (TO_*)xout)
-#define TE_C3S(T_, TF_, TO_, TC_, M_, SB_)     \
-  do {                                         \
-    if (xc) TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true); \
-    else TE_C3X(T_, TF_, TO_, TC_, M_, SB_, false);   \
+#define TE_C3S(T_, TF_, TO_, TC_, M_, SB_)                    \
+  do {                                                        \
+    if constexpr (M_ == 0 || SB_ < 4) {                       \
+      if (xc && wpe == 5) {                                   \
+        TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 5);          \
+        break;                                                \
+      }                                                       \
+    }                                                         \
+    if (xc) TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 1);      \
+    else TE_C3X(T_, TF_, TO_, TC_, M_, SB_, false, 1);        \
   } while (0)
 #define TE_C3B(T_, TF_, TO_, TC_, M_)                \
   do {                                               \

@@ -129,7 +129,7 @@ int tune_num_variants(int family)
 {
   if (family == TUNE_DFT_GENERIC) return kGenericCount;
   if (family == TUNE_DFT_MFMA_2D) return MFMA_GRID_VARIANTS;
-  if (family == TUNE_C3_SWEEP) return 4;
+  if (family == TUNE_C3_SWEEP) return kC3Variants;
   return 0;
 }
 

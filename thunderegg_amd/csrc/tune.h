@@ -39,14 +39,18 @@ enum DftMfmaGrid {
   MFMA_GRID_VARIANTS
 };
 
-// 3D fused sweeps: slices per load batch of variant v (0: 2 in FP32, 1 in FP64; 1, 2, 3:
-// 1, 2, 4 slices)
+// 3D fused sweeps, variant v: slices per load batch (v 0: 2 in FP32, 1 in FP64; v 1, 2, 3:
+// 1, 2, 4 slices; v 4, 5: 1, 2 slices) and the minimum waves per SIMD the compiler is asked to
+// fit the registers for (v 4, 5: 5, which the sweep from zero and the one-slice batch reach
+// without spilling; else no request)
+constexpr int kC3Variants = 6;
 inline int c3_sweep_batch(int v, int dtype)
 {
-  if (v == 1) return 1;
-  if (v == 2) return 2;
+  if (v == 1 || v == 4) return 1;
+  if (v == 2 || v == 5) return 2;
   if (v == 3) return 4;
   return dtype == 1 ? 2 : 1;  // TE_F32 = 1
 }
+inline int c3_sweep_waves(int v) { return v >= 4 ? 5 : 1; }
 
 }  // namespace te
