@@ -141,22 +141,26 @@ def kernel_rooflines_3d(g, P, its, reps=20):
       sweep           f, u's boundary layer, out    s (2N + B)
       sweep_interp    + the coarse boundary cells   s (2N + B + Bc)
       resid_restrict  f, u, coarse f                s (2N + N/8)
-      operator        M s (FP32), s, A M s (FP64)   4N + 8N + 8N
-      update_xr       x, r read + written; M p, M s (FP32); A M p, A M s, rhat  (8 + 8) 2 + 4 2 + 8 3 per cell
+      operator        M s (FP32), s (FP64)          4N + 8N (+ 8N: A M s stored, TE_NO_XRFLY=1)
+      operator_p      M p (FP32), rhat, A M p (FP64)  4N + 8N + 8N
+      update_xr       x, r read + written; M p, M s (FP32); A M p, rhat (+ A M s, TE_NO_XRFLY=1)
+                      (8 + 8) 2 + 4 2 + 8 2 (+ 8) per cell
     Flops of the patch solve: six 16 x 16 x 16 transform products per axis pass, 12 n^4."""
     np_ = len(P.levels[0])
     N, B, Bc = 4096, 16 ** 3 - 14 ** 3, 8 ** 3 - 6 ** 3
     s = 4
     fl = np_ * 12 * 16 ** 4
     pre, post = P.pre, P.post
+    xrfly = os.environ.get("TE_NO_XRFLY") != "1"  # the x / r update computes A M s itself
     specs = {
         # name: (bytes, flops, launches per solve)
         "sweep_zero": (np_ * s * 2 * N, fl, 2 * its),
         "sweep": (np_ * s * (2 * N + B), fl, 2 * its * max(pre - 1 + post - 1, 0)),
         "sweep_interp": (np_ * s * (2 * N + B + Bc), fl, 2 * its),
         "resid_restrict": (np_ * s * (2 * N + N // 8), None, 2 * its),
-        "operator": (np_ * N * (4 + 8 + 8), None, 2 * its),
-        "update_xr": (np_ * N * (16 * 2 + 4 * 2 + 8 * 3), None, its),
+        "operator": (np_ * N * (4 + 8 + (0 if xrfly else 8)), None, its),
+        "operator_p": (np_ * N * (4 + 8 + 8), None, its),
+        "update_xr": (np_ * N * (16 * 2 + 4 * 2 + 8 * 2 + (0 if xrfly else 8)), None, its),
     }
     out = {}
     for name, (byt, flops, per_solve) in specs.items():

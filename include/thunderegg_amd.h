@@ -356,8 +356,10 @@ enum te_kernel_id {
   TE_K_SWEEP = 1,           /* sweep with the interface terms from u (later sweeps) */
   TE_K_SWEEP_INTERP = 2,    /* sweep on u + P uc (interpolation fused, first post-smoothing) */
   TE_K_RESID_RESTRICT = 3,  /* r = f - A u restricted to the coarser level */
-  TE_K_OPERATOR = 4,        /* outer FP64 A (M s) with the dots as.s and as.as */
-  TE_K_UPDATE_XR = 5        /* outer x and r updates with the dots r.rhat and r.r */
+  TE_K_OPERATOR = 4,        /* outer FP64 A (M s) with the dots as.s and as.as (A M s stored only
+                               where the x / r update reads it back: TE_NO_XRFLY=1) */
+  TE_K_UPDATE_XR = 5,       /* outer x and r updates with the dots r.rhat and r.r */
+  TE_K_OPERATOR_P = 6       /* outer FP64 A (M p), stored, with the dot rhat.ap */
 };
 int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms);
 

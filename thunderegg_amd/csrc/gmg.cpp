@@ -580,6 +580,8 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
     g->c3 = c3;
     const char* nx = std::getenv("TE_NO_XCOL");
     g->xcol = !(nx && nx[0] == '1');
+    const char* nxr = std::getenv("TE_NO_XRFLY");
+    g->xrfly = !(nxr && nxr[0] == '1');
   }
   for (int i = 0; i < num_levels && !rc; i++) {
     GLevel& Lv = g->L[i];
@@ -922,7 +924,9 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     TE_RC(launch_c3_bicg(BICG_S, d, a, nb, s));
     if (use_precond)
       TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_s32 : (const void*)g->c_s, c3_32 ? TE_F32 : TE_F64, c3_ms, c3m));
-    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_ms, g->c_as, g->c_s, g->pa, g->pb, nb, s, c3_xms));
+    // as.s and as.as (A M s itself is recomputed by the x / r update unless xrfly is off)
+    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_ms, g->xrfly ? nullptr : g->c_as, g->c_s, g->pa, g->pb, nb, s,
+                             c3_xms));
     TE_RC(launch_bicg_finalize(FIN_SOMEGA, g->pa, g->pb, nb, g->scal, s, 0.0, nullptr, g->pc));
     // x and r updates; after the s-norm test fired, x += alpha p (use_precond 2: alpha M p)
     a.w32 = nullptr;
@@ -932,7 +936,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.pb = g->pb;
     a.dir = use_precond == 2 ? (const void*)c3_mp : (const void*)g->c_p;
     a.dir_m = use_precond == 2 ? 1 : 0;
-    TE_RC(launch_c3_bicg(BICG_XR, d, a, nb, s));
+    if (g->xrfly) TE_RC(launch_c3_xr(d, g->o.neumann, a, c3_xms, nb, s));
+    else TE_RC(launch_c3_bicg(BICG_XR, d, a, nb, s));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
     a.dir = nullptr;
     a.w32 = c3_32 ? g->c_p32 : nullptr;
@@ -1175,7 +1180,7 @@ double te_gmg_last_solve_seconds(const te_gmg* g) { return g ? g->last_seconds :
 
 int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
 {
-  if (!g || !ms || reps < 1 || which < TE_K_SWEEP_ZERO || which > TE_K_UPDATE_XR) {
+  if (!g || !ms || reps < 1 || which < TE_K_SWEEP_ZERO || which > TE_K_OPERATOR_P) {
     set_error("te_gmg_time_kernel: bad argument");
     return TE_ERR_ARG;
   }
@@ -1209,10 +1214,15 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
       case TE_K_RESID_RESTRICT:
         return launch_c3_resid_restrict(L0.d, L0.link, g->o.neumann, L0.dt, L0.dt, C0.f, C0.u, g->L[1].dt,
                                         g->C3[1].f, s, xu);
+      case TE_K_OPERATOR_P: {
+        const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
+        return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_mp, g->c_ap, g->c_rhat, g->pa, nullptr, nb, s,
+                                  xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr);
+      }
       case TE_K_OPERATOR: {
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
-        return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_ms, g->c_as, g->c_s, g->pa, g->pb, nb, s,
-                                  xc ? xcol_of(g, g->c_ms, np, L0.dt) : nullptr);
+        return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_ms, g->xrfly ? nullptr : g->c_as, g->c_s, g->pa,
+                                  g->pb, nb, s, xc ? xcol_of(g, g->c_ms, np, L0.dt) : nullptr);
       }
       default: {
         C3Bicg a{};
@@ -1227,7 +1237,9 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
         a.dtm = L0.dt;
         a.pa = g->pa;
         a.pb = g->pb;
-        return launch_c3_bicg(BICG_XR, L0.d, a, nb, s);
+        if (!g->xrfly) return launch_c3_bicg(BICG_XR, L0.d, a, nb, s);
+        const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
+        return launch_c3_xr(L0.d, g->o.neumann, a, xc ? xcol_of(g, g->c_ms, np, L0.dt) : nullptr, nb, s);
       }
     }
   };

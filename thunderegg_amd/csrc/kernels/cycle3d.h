@@ -44,9 +44,10 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
 int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int neumann, int dt, int dtf,
                              const void* f, const void* u, int dtc, void* fc, hipStream_t s,
                              const void* xu = nullptr);
-// out = A u (FP64, compact; u compact of type udt, widened as read) with the per-workgroup
-// partial sums pa = sum w out and, when pb is not null, pb = sum out out; nslots partial
-// slots, those past the launch's workgroups written zero.  xu: u's face columns (or nullptr).
+// out = A u (FP64, compact; u compact of type udt, widened as read; out may be nullptr: only
+// the sums) with the per-workgroup partial sums pa = sum w out and, when pb is not null,
+// pb = sum out out; nslots partial slots, those past the launch's workgroups written zero.
+// xu: u's face columns (or nullptr).
 int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, double* out, const double* w,
                        double* pa, double* pb, int nslots, hipStream_t s, const void* xu = nullptr);
 // BiCGStab vector updates (Iterative/BiCGStab.h:111-203) on compact vectors, the modes and
@@ -81,6 +82,10 @@ struct C3Bicg {
   int nfold;
 };
 int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hipStream_t s);
+// BICG_XR with A M s computed from a.ms in the launch (a.as unused; the operator launch for
+// M s then need not store it); xms: the face columns of a.ms (or nullptr); same partial sums
+// and early-exit behaviour as BICG_XR.  One wavefront per patch.
+int launch_c3_xr(const te_domain* d, int neumann, const C3Bicg& a, const void* xms, int nslots, hipStream_t s);
 // compact <-> padded copies with element-type conversion (interior cells; the padded
 // vector's ghost cells are not written)
 int launch_c3_pack(const te_domain* d, int dtd, void* dst, int dts, const void* src_padded, hipStream_t s);

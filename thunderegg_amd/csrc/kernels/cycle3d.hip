@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(256) c3_star_dot_kernel(const DevPatch* __rest
       ld4(w, base + 256 * z + ro, wv4);
       double acc[4];
       star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
-      st4(out, base + 256 * z + ro, acc);
+      if (out) st4(out, base + 256 * z + ro, acc);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         sa += wv4[k] * acc[k];  // rhat.dot(ap) / as.dot(s): w first, as the reference
@@ -605,6 +605,88 @@ __global__ void __launch_bounds__(256) c3_bicg_kernel(C3Bicg a, long n, int nslo
   }
 }
 
+// BiCGStab's x and r updates (Iterative/BiCGStab.h:170-185) with A M s computed here from
+// M s, as the outer operator launch computes it (star_slice on the same widened values), so
+// that launch only forms the dot products and A M s never goes through memory:
+//   x += alpha M p + omega M s;  r -= alpha A M p + omega A M s;  partial sums r.rhat, r.r
+// One wavefront per patch (persistent over the patches), walking its slices like
+// c3_star_dot_kernel; xms: the face columns of M s (or nullptr).  After the s-norm test
+// fired: x += alpha dir only (dir of type TM when dir_m, else FP64).
+template <typename TM>
+__global__ void __launch_bounds__(256) c3_xr_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
+                                                    const TM* __restrict__ xms, int np, int nslots, int neumann)
+{
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const int ro = cidx(4 * g, c, 0);
+  const double* __restrict__ scal = a.scal;
+  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], st = scal[SC_STATE];
+  if (a.dir && st == ST_EARLY) {
+    for (int p = blockIdx.x * 4 + wv; p < np; p += gridDim.x * 4)
+      for (int z = 0; z < N; z++) {
+        const int o = p * NC + 256 * z + ro;
+        double dv[4], xv[4];
+        if (a.dir_m) ld4((const TM*)a.dir, o, dv);
+        else ld4((const double*)a.dir, o, dv);
+        ld4(a.x, o, xv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) xv[k] += dv[k] * alpha;
+        st4(a.x, o, xv);
+      }
+    return;
+  }
+  if (st != ST_RUN) return;
+  const TM* __restrict__ mp = (const TM*)a.mp;
+  const TM* __restrict__ ms = (const TM*)a.ms;
+  const double bc = neumann ? 1.0 : -1.0;
+  double sa = 0.0, sb = 0.0;
+  for (int p = blockIdx.x * 4 + wv; p < np; p += gridDim.x * 4) {
+    const DevPatch& P = dp[p];
+    int q[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) q[s] = P.type[20 + s] >= 0 ? P.nbr[20 + s][0] : -1;
+    const double ihx = P.inv_h2[0], ihy = P.inv_h2[1], ihz = P.inv_h2[2];
+    const int base = p * NC;
+    double cur[4], prv[4], nxt[4];
+    ld4(ms, base + ro, cur);
+    z_ghost(ms, q[4], N - 1, ro, bc, cur, prv);
+    for (int z = 0; z < N; z++) {
+      if (z < N - 1) ld4(ms, base + 256 * (z + 1) + ro, nxt);
+      else z_ghost(ms, q[5], 0, ro, bc, cur, nxt);
+      double as[4];
+      star_slice(ms, xms, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, as);
+      const int o = base + 256 * z + ro;
+      double mpv[4], xv[4], rv[4], apv[4], rh[4];
+      ld4(mp, o, mpv);
+      ld4(a.x, o, xv);
+      ld4(a.r, o, rv);
+      ld4(a.ap, o, apv);
+      ld4(a.rhat, o, rh);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        xv[k] += mpv[k] * alpha + cur[k] * omega;
+        double r = rv[k];
+        r += apv[k] * -alpha + as[k] * -omega;
+        rv[k] = r;
+        sa += r * rh[k];
+        sb += r * r;
+        prv[k] = cur[k];
+        cur[k] = nxt[k];
+      }
+      st4(a.x, o, xv);
+      st4(a.r, o, rv);
+    }
+  }
+  const double ra = block_reduce<RED_SUM>(sa);
+  if (threadIdx.x == 0) a.pa[blockIdx.x] = ra;
+  const double rb = block_reduce<RED_SUM>(sb);
+  if (threadIdx.x == 0) a.pb[blockIdx.x] = rb;
+  if (blockIdx.x == 0)
+    for (int k = (int)gridDim.x + threadIdx.x; k < nslots; k += blockDim.x) {
+      a.pa[k] = 0.0;
+      a.pb[k] = 0.0;
+    }
+}
+
 template <typename TD, typename TS, bool TO_PADDED>
 __global__ void __launch_bounds__(256) c3_copy_kernel(TD* __restrict__ dst, const TS* __restrict__ src, long n)
 {
@@ -680,7 +762,7 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
                      (TO_*)xout)
 #define TE_C3S(T_, TF_, TO_, TC_, M_, SB_)                    \
   do {                                                        \
-    if constexpr (M_ == 0 || SB_ < 4) {                       \
+    if constexpr ((M_ == 0 || SB_ < 4) && sizeof(T_) == 4) {  \
       if (xc && wpe == 5) {                                   \
         TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 5);          \
         break;                                                \
@@ -795,6 +877,25 @@ int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hi
       return TE_ERR_ARG;
   }
 #undef TE_C3B
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_xr(const te_domain* d, int neumann, const C3Bicg& a, const void* xms, int nslots, hipStream_t s)
+{
+  if (nslots < 1 || nslots > kFoldMax) {
+    set_error("internal: bad partial count");
+    return TE_ERR_ARG;
+  }
+  int grid = (d->np + 3) / 4;
+  if (grid > nslots) grid = nslots;
+  if (grid < 1) grid = 1;
+  if (a.dtm == TE_F32)
+    hipLaunchKernelGGL((c3_xr_kernel<float>), dim3(grid), dim3(256), 0, s, d->d_patches, a, (const float*)xms, d->np,
+                       nslots, neumann);
+  else
+    hipLaunchKernelGGL((c3_xr_kernel<double>), dim3(grid), dim3(256), 0, s, d->d_patches, a, (const double*)xms,
+                       d->np, nslots, neumann);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

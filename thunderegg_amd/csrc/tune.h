@@ -41,8 +41,9 @@ enum DftMfmaGrid {
 
 // 3D fused sweeps, variant v: slices per load batch (v 0: 2 in FP32, 1 in FP64; v 1, 2, 3:
 // 1, 2, 4 slices; v 4, 5: 1, 2 slices) and the minimum waves per SIMD the compiler is asked to
-// fit the registers for (v 4, 5: 5, which the sweep from zero and the one-slice batch reach
-// without spilling; else no request)
+// fit the registers for (v 4, 5: 5, which the FP32 sweep from zero and one-slice batch reach
+// without spilling; else, and in FP64, where the patch's LDS allows 3 workgroups per CU, no
+// request)
 constexpr int kC3Variants = 6;
 inline int c3_sweep_batch(int v, int dtype)
 {

@@ -192,7 +192,8 @@ class GMG:
         return R.lib().te_gmg_last_solve_seconds(self.h)
 
     # te_kernel_id: the finest level's kernels of the compact 3D cycle and the solver
-    KERNELS = {"sweep_zero": 0, "sweep": 1, "sweep_interp": 2, "resid_restrict": 3, "operator": 4, "update_xr": 5}
+    KERNELS = {"sweep_zero": 0, "sweep": 1, "sweep_interp": 2, "resid_restrict": 3, "operator": 4, "update_xr": 5,
+               "operator_p": 6}
 
     def time_kernel(self, name, reps=20):
         """Average milliseconds of `reps` standalone launches of one kernel on the solver's
