@@ -142,16 +142,19 @@ def kernel_rooflines_3d(g, P, its, reps=20):
       sweep_interp    + the coarse boundary cells   s (2N + B + Bc)
       resid_restrict  f, u, coarse f                s (2N + N/8)
       operator        M s (FP32), s (FP64)          4N + 8N (+ 8N: A M s stored, TE_NO_XRFLY=1)
-      operator_p      M p (FP32), rhat, A M p (FP64)  4N + 8N + 8N
-      update_xr       x, r read + written; M p, M s (FP32); A M p, rhat (+ A M s, TE_NO_XRFLY=1)
-                      (8 + 8) 2 + 4 2 + 8 2 (+ 8) per cell
+      operator_p      M p (FP32), rhat (FP64)       4N + 8N (+ 8N: A M p stored, TE_NO_XRFLY=1)
+      update_xr       x read + written, r written; M p, M s (FP32); s, rhat (FP64)
+                      8 2 + 8 + 4 2 + 8 2 per cell (TE_NO_XRFLY=1: r read, A M p and A M s
+                      instead of s: + 8 + 8)
+      update_s        r, M p (FP32) -> s, s32        8 + 4 + 8 + 4 (M p -> A M p, TE_NO_XRFLY=1: + 4)
+      update_p        p, M p (FP32), r -> p, p32     8 + 4 + 8 + 8 + 4 (likewise + 4)
     Flops of the patch solve: six 16 x 16 x 16 transform products per axis pass, 12 n^4."""
     np_ = len(P.levels[0])
     N, B, Bc = 4096, 16 ** 3 - 14 ** 3, 8 ** 3 - 6 ** 3
     s = 4
     fl = np_ * 12 * 16 ** 4
     pre, post = P.pre, P.post
-    xrfly = os.environ.get("TE_NO_XRFLY") != "1"  # the x / r update computes A M s itself
+    xrfly = os.environ.get("TE_NO_XRFLY") != "1"  # the updates form A M p and A M s themselves
     specs = {
         # name: (bytes, flops, launches per solve)
         "sweep_zero": (np_ * s * 2 * N, fl, 2 * its),
@@ -159,8 +162,10 @@ def kernel_rooflines_3d(g, P, its, reps=20):
         "sweep_interp": (np_ * s * (2 * N + B + Bc), fl, 2 * its),
         "resid_restrict": (np_ * s * (2 * N + N // 8), None, 2 * its),
         "operator": (np_ * N * (4 + 8 + (0 if xrfly else 8)), None, its),
-        "operator_p": (np_ * N * (4 + 8 + 8), None, its),
-        "update_xr": (np_ * N * (16 * 2 + 4 * 2 + 8 * 2 + (0 if xrfly else 8)), None, its),
+        "operator_p": (np_ * N * (4 + 8 + (0 if xrfly else 8)), None, its),
+        "update_xr": (np_ * N * (8 * 2 + 8 + 4 * 2 + 8 * 2 + (0 if xrfly else 16)), None, its),
+        "update_s": (np_ * N * (8 + 4 + 8 + 4 + (0 if xrfly else 4)), None, its),
+        "update_p": (np_ * N * (8 + 4 + 8 + 8 + 4 + (0 if xrfly else 4)), None, its),
     }
     out = {}
     for name, (byt, flops, per_solve) in specs.items():

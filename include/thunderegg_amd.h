@@ -357,9 +357,11 @@ enum te_kernel_id {
   TE_K_SWEEP_INTERP = 2,    /* sweep on u + P uc (interpolation fused, first post-smoothing) */
   TE_K_RESID_RESTRICT = 3,  /* r = f - A u restricted to the coarser level */
   TE_K_OPERATOR = 4,        /* outer FP64 A (M s) with the dots as.s and as.as (A M s stored only
-                               where the x / r update reads it back: TE_NO_XRFLY=1) */
+                               where the updates read it back: TE_NO_XRFLY=1) */
   TE_K_UPDATE_XR = 5,       /* outer x and r updates with the dots r.rhat and r.r */
-  TE_K_OPERATOR_P = 6       /* outer FP64 A (M p), stored, with the dot rhat.ap */
+  TE_K_OPERATOR_P = 6,      /* outer FP64 A (M p) with the dot rhat.ap (stored with TE_NO_XRFLY=1) */
+  TE_K_UPDATE_S = 7,        /* outer s = r - alpha A M p (+ its FP32 copy) with the dot s.s */
+  TE_K_UPDATE_P = 8         /* outer p = r + beta (p - omega A M p) (+ its FP32 copy) */
 };
 int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms);
 

@@ -178,12 +178,14 @@ def test_compact3d_face_columns_bit_identical(monkeypatch, prec):
 
 @pytest.mark.parametrize("prec", [R.F64, R.F32])
 def test_compact3d_xr_update_computes_operator(monkeypatch, prec):
-    """The x / r update that forms A M s itself from M s (the operator launch for M s then
-    stores nothing) against the update that reads the stored A M s (TE_NO_XRFLY=1): A M s has
-    the same bits either way; only the r.rhat and r.r partial sums are summed per patch
-    instead of per grid stride, so FP64 solves take the same iterations to rounding-level
-    differences, and both reach the tolerance in the true residual.  Also with the
-    unpreconditioned iteration (M s = s, FP64, no face columns) and the early exit."""
+    """The s, x / r and p updates that form A M p and A M s themselves from M p and M s (the
+    operator launches then store nothing; r = s - omega A M s) against the updates that read
+    the stored products (TE_NO_XRFLY=1, r - alpha A M p - omega A M s as the reference):
+    the products have the same bits either way; r is rounded in another order and the
+    partial sums are summed per patch instead of per grid stride, so FP64 solves take the
+    same iterations to rounding-level differences, and both reach the tolerance in the true
+    residual.  Also with the unpreconditioned iteration (M s = s, FP64, no face columns) and
+    the early exit."""
     _print_case("compact3d xr fly", prec)
     levels = _levels(3)
     b = _rhs_sin(levels)

@@ -900,6 +900,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   const bool c3_xc = use_precond && (g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0);
   const void* c3_xmp = c3_xc ? xcol_of(g, c3_mp, d->np, c3m) : nullptr;
   const void* c3_xms = c3_xc ? xcol_of(g, c3_ms, d->np, c3m) : nullptr;
+  // the updates form A M p from M p (M p is a buffer of its own when preconditioned)
+  const bool apf = g->xrfly && use_precond;
   auto iteration_c3 = [&]() -> int {
     C3Bicg a{};
     a.scal = g->scal;
@@ -915,16 +917,20 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.dtm = c3m;
     if (use_precond)
       TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_p32 : (const void*)g->c_p, c3_32 ? TE_F32 : TE_F64, c3_mp, c3m));
-    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_mp, g->c_ap, g->c_rhat, g->pa, nullptr, nb, s, c3_xmp));
+    // rhat.ap (A M p itself is formed again by the updates that use it, launch_c3_upd, where
+    // M p is apart from p; else stored)
+    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_mp, apf ? nullptr : g->c_ap, g->c_rhat, g->pa, nullptr, nb, s,
+                             c3_xmp));
     // s = r - alpha ap, alpha folded from the rhat.ap partials; s.s partials into pc
     a.w32 = c3_32 ? g->c_s32 : nullptr;
     a.pa = g->pc;
     a.pfold = g->pa;
     a.nfold = nb;
-    TE_RC(launch_c3_bicg(BICG_S, d, a, nb, s));
+    if (g->xrfly) TE_RC(launch_c3_upd(BICG_S, d, g->o.neumann, a, apf, c3_xmp, nullptr, nb, s));
+    else TE_RC(launch_c3_bicg(BICG_S, d, a, nb, s));
     if (use_precond)
       TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_s32 : (const void*)g->c_s, c3_32 ? TE_F32 : TE_F64, c3_ms, c3m));
-    // as.s and as.as (A M s itself is recomputed by the x / r update unless xrfly is off)
+    // as.s and as.as (A M s itself is formed again by the x / r update unless xrfly is off)
     TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_ms, g->xrfly ? nullptr : g->c_as, g->c_s, g->pa, g->pb, nb, s,
                              c3_xms));
     TE_RC(launch_bicg_finalize(FIN_SOMEGA, g->pa, g->pb, nb, g->scal, s, 0.0, nullptr, g->pc));
@@ -936,11 +942,12 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.pb = g->pb;
     a.dir = use_precond == 2 ? (const void*)c3_mp : (const void*)g->c_p;
     a.dir_m = use_precond == 2 ? 1 : 0;
-    if (g->xrfly) TE_RC(launch_c3_xr(d, g->o.neumann, a, c3_xms, nb, s));
+    if (g->xrfly) TE_RC(launch_c3_upd(BICG_XR, d, g->o.neumann, a, apf, c3_xmp, c3_xms, nb, s));
     else TE_RC(launch_c3_bicg(BICG_XR, d, a, nb, s));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
     a.dir = nullptr;
     a.w32 = c3_32 ? g->c_p32 : nullptr;
+    if (g->xrfly) return launch_c3_upd(BICG_P, d, g->o.neumann, a, apf, c3_xmp, nullptr, nb, s);
     return launch_c3_bicg(BICG_P, d, a, nb, s);
   };
   auto iteration = [&]() -> int {
@@ -1180,7 +1187,7 @@ double te_gmg_last_solve_seconds(const te_gmg* g) { return g ? g->last_seconds :
 
 int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
 {
-  if (!g || !ms || reps < 1 || which < TE_K_SWEEP_ZERO || which > TE_K_OPERATOR_P) {
+  if (!g || !ms || reps < 1 || which < TE_K_SWEEP_ZERO || which > TE_K_UPDATE_P) {
     set_error("te_gmg_time_kernel: bad argument");
     return TE_ERR_ARG;
   }
@@ -1216,8 +1223,8 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
                                         g->C3[1].f, s, xu);
       case TE_K_OPERATOR_P: {
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
-        return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_mp, g->c_ap, g->c_rhat, g->pa, nullptr, nb, s,
-                                  xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr);
+        return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_mp, g->xrfly ? nullptr : g->c_ap, g->c_rhat, g->pa,
+                                  nullptr, nb, s, xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr);
       }
       case TE_K_OPERATOR: {
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
@@ -1234,12 +1241,26 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
         a.as = g->c_as;
         a.mp = g->c_mp;
         a.ms = g->c_ms;
+        a.sv = g->c_s;
         a.dtm = L0.dt;
         a.pa = g->pa;
         a.pb = g->pb;
-        if (!g->xrfly) return launch_c3_bicg(BICG_XR, L0.d, a, nb, s);
+        // the s and p updates as in the mixed-precision iteration (with the FP32 copies)
+        int mode = BICG_XR;
+        if (which == TE_K_UPDATE_S) {
+          mode = BICG_S;
+          a.sv = g->c_s;
+          a.w32 = L0.dt == TE_F32 ? g->c_s32 : nullptr;
+          a.pa = g->pc;
+        } else if (which == TE_K_UPDATE_P) {
+          mode = BICG_P;
+          a.p = g->c_p;
+          a.w32 = L0.dt == TE_F32 ? g->c_p32 : nullptr;
+        }
+        if (!g->xrfly) return launch_c3_bicg(mode, L0.d, a, nb, s);
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
-        return launch_c3_xr(L0.d, g->o.neumann, a, xc ? xcol_of(g, g->c_ms, np, L0.dt) : nullptr, nb, s);
+        return launch_c3_upd(mode, L0.d, g->o.neumann, a, true, xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr,
+                             xc ? xcol_of(g, g->c_ms, np, L0.dt) : nullptr, nb, s);
       }
     }
   };

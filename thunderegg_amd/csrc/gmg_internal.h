@@ -118,8 +118,9 @@ struct te_gmg {
   // the compact path keeps face columns behind its solution buffers (cycle3d.h);
   // TE_NO_XCOL=1 reads the x faces from the cells instead (same values)
   bool xcol = true;
-  // the compact path's x / r update computes A M s itself (launch_c3_xr), so the operator
-  // launch for M s stores nothing; TE_NO_XRFLY=1 stores A M s and reads it back
+  // the compact path's s, x / r and p updates form A M p and A M s themselves
+  // (launch_c3_upd), so the operator launches store nothing; TE_NO_XRFLY=1 stores them and
+  // reads them back
   bool xrfly = true;
   struct C3Level {
     void* f = nullptr;

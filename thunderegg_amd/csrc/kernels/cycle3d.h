@@ -82,10 +82,15 @@ struct C3Bicg {
   int nfold;
 };
 int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hipStream_t s);
-// BICG_XR with A M s computed from a.ms in the launch (a.as unused; the operator launch for
-// M s then need not store it); xms: the face columns of a.ms (or nullptr); same partial sums
-// and early-exit behaviour as BICG_XR.  One wavefront per patch.
-int launch_c3_xr(const te_domain* d, int neumann, const C3Bicg& a, const void* xms, int nslots, hipStream_t s);
+// BICG_S, BICG_XR and BICG_P with the operator products formed in the launch, one wavefront
+// per patch: A M s (XR) from a.ms, and with apf A M p (S, P) from a.mp (then a.ap is unused:
+// the operator launches need not store either; apf needs a.mp apart from a.p, as in the
+// preconditioned iteration).  XR forms r = s - omega A M s from a.sv (the same vector as
+// launch_c3_bicg's r - alpha ap - omega as, rounded in another order).  xmp, xms: the face
+// columns of a.mp and a.ms (or nullptr).  The early exit and partial-sum slots of
+// launch_c3_bicg.
+int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bool apf, const void* xmp,
+                  const void* xms, int nslots, hipStream_t s);
 // compact <-> padded copies with element-type conversion (interior cells; the padded
 // vector's ghost cells are not written)
 int launch_c3_pack(const te_domain* d, int dtd, void* dst, int dts, const void* src_padded, hipStream_t s);

@@ -605,22 +605,66 @@ __global__ void __launch_bounds__(256) c3_bicg_kernel(C3Bicg a, long n, int nslo
   }
 }
 
-// BiCGStab's x and r updates (Iterative/BiCGStab.h:170-185) with A M s computed here from
-// M s, as the outer operator launch computes it (star_slice on the same widened values), so
-// that launch only forms the dot products and A M s never goes through memory:
-//   x += alpha M p + omega M s;  r -= alpha A M p + omega A M s;  partial sums r.rhat, r.r
-// One wavefront per patch (persistent over the patches), walking its slices like
-// c3_star_dot_kernel; xms: the face columns of M s (or nullptr).  After the s-norm test
-// fired: x += alpha dir only (dir of type TM when dir_m, else FP64).
-template <typename TM>
-__global__ void __launch_bounds__(256) c3_xr_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
-                                                    const TM* __restrict__ xms, int np, int nslots, int neumann)
+// The operator A (the Poisson star, star_slice) applied to a compact vector one slice at a
+// time by one wavefront, as c3_star_dot_kernel applies it: slices z - 1, z, z + 1 of the
+// lane's row in registers (widened to FP64), the face ghosts from the neighbours (x faces
+// from the face columns xu when given), so A u has the operator launch's bits.  init(p)
+// before slice 0; step(z, acc) forms A u on slice z; next() moves on; cur holds slice z.
+template <typename TU>
+struct StarWalk {
+  const TU* __restrict__ u;
+  const TU* __restrict__ xu;
+  int base, ro, lane, c, g;
+  double bc;
+  double cur[4], prv[4], nxt[4];
+  __device__ __forceinline__ void init(int p, const int (&q)[6])
+  {
+    base = p * NC;
+    ld4(u, base + ro, cur);
+    z_ghost(u, q[4], N - 1, ro, bc, cur, prv);
+  }
+  __device__ __forceinline__ void step(int z, const int (&q)[6], double ihx, double ihy, double ihz, double (&acc)[4])
+  {
+    if (z < N - 1) ld4(u, base + 256 * (z + 1) + ro, nxt);
+    else z_ghost(u, q[5], 0, ro, bc, cur, nxt);
+    star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
+  }
+  __device__ __forceinline__ void next()
+  {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      prv[k] = cur[k];
+      cur[k] = nxt[k];
+    }
+  }
+};
+
+// BiCGStab's s, x / r and p updates (Iterative/BiCGStab.h:150-203) on compact vectors with the
+// operator products formed in the launch from M p and M s, as the operator launches form them
+// (StarWalk: the same stencil code on the same widened values, so the same bits), so that
+// A M p and A M s never go through memory and the operator launches only form dot products:
+//   BICG_S:  s = r - alpha A M p (+ s32), sum s.s (alpha folded from pfold when given)
+//   BICG_XR: x += alpha M p + omega M s; r = s - omega A M s; sums r.rhat, r.r.  The
+//            reference forms r - alpha A M p - omega A M s (:181); with s = r - alpha A M p
+//            at hand this is the same vector with one product instead of two, rounded in
+//            another order.  After the s-norm test fired, x += alpha dir only (dir of type
+//            TM when dir_m).
+//   BICG_P:  p = r + beta (p - omega A M p) (+ p32)
+// APF (s and p updates): A M p from M p here (needs M p apart from p: the preconditioned
+// iteration); else read from a.ap.  One wavefront per patch, persistent over the patches;
+// xmp, xms: the face columns of M p and M s (or nullptr).
+template <int MODE, typename TM, bool APF>
+__global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
+                                                     const TM* __restrict__ xmp, const TM* __restrict__ xms, int np,
+                                                     int nslots, int neumann)
 {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
   const int ro = cidx(4 * g, c, 0);
-  const double* __restrict__ scal = a.scal;
-  const double alpha = scal[SC_ALPHA], omega = scal[SC_OMEGA], st = scal[SC_STATE];
-  if (a.dir && st == ST_EARLY) {
+  const bool fold = MODE == BICG_S && a.pfold != nullptr;
+  double* __restrict__ scal = a.scal;
+  double alpha = fold ? 0.0 : scal[SC_ALPHA];
+  const double omega = scal[SC_OMEGA], beta = scal[SC_BETA], rho = scal[SC_RHO], st = scal[SC_STATE];
+  if (MODE == BICG_XR && a.dir && st == ST_EARLY) {
     for (int p = blockIdx.x * 4 + wv; p < np; p += gridDim.x * 4)
       for (int z = 0; z < N; z++) {
         const int o = p * NC + 256 * z + ro;
@@ -635,9 +679,16 @@ __global__ void __launch_bounds__(256) c3_xr_kernel(const DevPatch* __restrict__
     return;
   }
   if (st != ST_RUN) return;
-  const TM* __restrict__ mp = (const TM*)a.mp;
-  const TM* __restrict__ ms = (const TM*)a.ms;
+  if (fold) {  // FIN_ALPHA: alpha = rho / rhat.dot(ap)
+    const double* const parts[3] = { a.pfold, nullptr, nullptr };
+    double r[3] = { 0.0, 0.0, 0.0 };
+    fold_n<1>(parts, a.nfold, r);
+    alpha = rho / r[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[SC_ALPHA] = alpha;
+  }
   const double bc = neumann ? 1.0 : -1.0;
+  StarWalk<TM> wp{ (const TM*)a.mp, xmp, 0, ro, lane, c, g, bc };
+  StarWalk<TM> wm{ (const TM*)a.ms, xms, 0, ro, lane, c, g, bc };
   double sa = 0.0, sb = 0.0;
   for (int p = blockIdx.x * 4 + wv; p < np; p += gridDim.x * 4) {
     const DevPatch& P = dp[p];
@@ -645,45 +696,75 @@ __global__ void __launch_bounds__(256) c3_xr_kernel(const DevPatch* __restrict__
 #pragma unroll
     for (int s = 0; s < 6; s++) q[s] = P.type[20 + s] >= 0 ? P.nbr[20 + s][0] : -1;
     const double ihx = P.inv_h2[0], ihy = P.inv_h2[1], ihz = P.inv_h2[2];
-    const int base = p * NC;
-    double cur[4], prv[4], nxt[4];
-    ld4(ms, base + ro, cur);
-    z_ghost(ms, q[4], N - 1, ro, bc, cur, prv);
+    constexpr bool WP = APF && MODE != BICG_XR;
+    if (WP) wp.init(p, q);
+    if (MODE == BICG_XR) wm.init(p, q);
     for (int z = 0; z < N; z++) {
-      if (z < N - 1) ld4(ms, base + 256 * (z + 1) + ro, nxt);
-      else z_ghost(ms, q[5], 0, ro, bc, cur, nxt);
-      double as[4];
-      star_slice(ms, xms, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, as);
-      const int o = base + 256 * z + ro;
-      double mpv[4], xv[4], rv[4], apv[4], rh[4];
-      ld4(mp, o, mpv);
-      ld4(a.x, o, xv);
-      ld4(a.r, o, rv);
-      ld4(a.ap, o, apv);
-      ld4(a.rhat, o, rh);
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        xv[k] += mpv[k] * alpha + cur[k] * omega;
-        double r = rv[k];
-        r += apv[k] * -alpha + as[k] * -omega;
-        rv[k] = r;
-        sa += r * rh[k];
-        sb += r * r;
-        prv[k] = cur[k];
-        cur[k] = nxt[k];
+      const int o = p * NC + 256 * z + ro;
+      double apv[4];
+      if (MODE != BICG_XR) {
+        if (WP) wp.step(z, q, ihx, ihy, ihz, apv);
+        else ld4(a.ap, o, apv);
       }
-      st4(a.x, o, xv);
-      st4(a.r, o, rv);
+      if (MODE == BICG_S) {
+        double rv[4], sv[4];
+        ld4(a.r, o, rv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          double sk = rv[k];
+          sk += apv[k] * -alpha;
+          sv[k] = sk;
+          sa += sk * sk;
+        }
+        st4(a.sv, o, sv);
+        if (a.w32) st4(a.w32, o, sv);
+      } else if (MODE == BICG_XR) {
+        double asv[4], mpv[4], xv[4], rv[4], rh[4];
+        wm.step(z, q, ihx, ihy, ihz, asv);
+        ld4((const TM*)a.mp, o, mpv);
+        ld4(a.x, o, xv);
+        ld4(a.sv, o, rv);
+        ld4(a.rhat, o, rh);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          xv[k] += mpv[k] * alpha + wm.cur[k] * omega;
+          double r = rv[k];
+          r += asv[k] * -omega;
+          rv[k] = r;
+          sa += r * rh[k];
+          sb += r * r;
+        }
+        st4(a.x, o, xv);
+        st4(a.r, o, rv);
+        wm.next();
+      } else {  // BICG_P
+        double pv[4], rv[4];
+        ld4(a.p, o, pv);
+        ld4(a.r, o, rv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          double v = pv[k];
+          v += apv[k] * -omega;
+          pv[k] = v * beta + rv[k];
+        }
+        st4(a.p, o, pv);
+        if (a.w32) st4(a.w32, o, pv);
+      }
+      if (WP) wp.next();
     }
   }
-  const double ra = block_reduce<RED_SUM>(sa);
-  if (threadIdx.x == 0) a.pa[blockIdx.x] = ra;
-  const double rb = block_reduce<RED_SUM>(sb);
-  if (threadIdx.x == 0) a.pb[blockIdx.x] = rb;
-  if (blockIdx.x == 0)
+  if (MODE == BICG_S || MODE == BICG_XR) {
+    const double ra = block_reduce<RED_SUM>(sa);
+    if (threadIdx.x == 0) a.pa[blockIdx.x] = ra;
+  }
+  if (MODE == BICG_XR) {
+    const double rb = block_reduce<RED_SUM>(sb);
+    if (threadIdx.x == 0) a.pb[blockIdx.x] = rb;
+  }
+  if ((MODE == BICG_S || MODE == BICG_XR) && blockIdx.x == 0)
     for (int k = (int)gridDim.x + threadIdx.x; k < nslots; k += blockDim.x) {
       a.pa[k] = 0.0;
-      a.pb[k] = 0.0;
+      if (MODE == BICG_XR) a.pb[k] = 0.0;
     }
 }
 
@@ -881,21 +962,40 @@ int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hi
   return TE_OK;
 }
 
-int launch_c3_xr(const te_domain* d, int neumann, const C3Bicg& a, const void* xms, int nslots, hipStream_t s)
+int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bool apf, const void* xmp,
+                  const void* xms, int nslots, hipStream_t s)
 {
-  if (nslots < 1 || nslots > kFoldMax) {
-    set_error("internal: bad partial count");
+  if (nslots < 1 || nslots > kFoldMax || (a.pfold && (mode != BICG_S || a.nfold < 1 || a.nfold > kFoldMax)) ||
+      (mode != BICG_S && mode != BICG_XR && mode != BICG_P)) {
+    set_error("internal: bad arguments for the 3D BiCGStab updates");
     return TE_ERR_ARG;
   }
   int grid = (d->np + 3) / 4;
   if (grid > nslots) grid = nslots;
   if (grid < 1) grid = 1;
-  if (a.dtm == TE_F32)
-    hipLaunchKernelGGL((c3_xr_kernel<float>), dim3(grid), dim3(256), 0, s, d->d_patches, a, (const float*)xms, d->np,
-                       nslots, neumann);
-  else
-    hipLaunchKernelGGL((c3_xr_kernel<double>), dim3(grid), dim3(256), 0, s, d->d_patches, a, (const double*)xms,
-                       d->np, nslots, neumann);
+#define TE_C3U(M_, TM_, APF_)                                                                                    \
+  hipLaunchKernelGGL((c3_upd_kernel<M_, TM_, APF_>), dim3(grid), dim3(256), 0, s, d->d_patches, a,                \
+                     (const TM_*)xmp, (const TM_*)xms, d->np, nslots, neumann)
+#define TE_C3UM(M_)                                      \
+  do {                                                   \
+    if (a.dtm == TE_F32) {                               \
+      if (apf) TE_C3U(M_, float, true);                  \
+      else TE_C3U(M_, float, false);                     \
+    } else {                                             \
+      if (apf) TE_C3U(M_, double, true);                 \
+      else TE_C3U(M_, double, false);                    \
+    }                                                    \
+  } while (0)
+  if (mode == BICG_S) {
+    TE_C3UM(BICG_S);
+  } else if (mode == BICG_XR) {  // forms A M s only
+    if (a.dtm == TE_F32) TE_C3U(BICG_XR, float, false);
+    else TE_C3U(BICG_XR, double, false);
+  } else {
+    TE_C3UM(BICG_P);
+  }
+#undef TE_C3UM
+#undef TE_C3U
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
