@@ -360,10 +360,12 @@ def run_many_problems(args, nprob=None, emit=True):
     log(f"many problems: batched mixed {args.steps} rounds in {Tb32:.3f}s")
     Tb64, itb64, _, worst64 = batched(R.F64)
     log(f"many problems: batched fp64 {args.steps} rounds in {Tb64:.3f}s")
-    Tc64, itc64 = concurrent(R.F64)
-    log(f"many problems: per-problem fp64 (concurrent streams) {args.steps} rounds in {Tc64:.3f}s")
-    Tc32, itc32 = concurrent(R.F32)
-    log(f"many problems: per-problem mixed (concurrent streams) {args.steps} rounds in {Tc32:.3f}s")
+    Tc64 = Tc32 = itc64 = None
+    if not args.batched_only:
+        Tc64, itc64 = concurrent(R.F64)
+        log(f"many problems: per-problem fp64 (concurrent streams) {args.steps} rounds in {Tc64:.3f}s")
+        Tc32, _ = concurrent(R.F32)
+        log(f"many problems: per-problem mixed (concurrent streams) {args.steps} rounds in {Tc32:.3f}s")
     out = {
         "metric": METRIC,
         "value": dof * args.steps / Tb32,
@@ -383,12 +385,13 @@ def run_many_problems(args, nprob=None, emit=True):
                    "patches_min_max": [min(len(P.levels[0]) for P in probs), max(len(P.levels[0]) for P in probs)]},
         "fp64_dof_per_s": dof * args.steps / Tb64,
         "mixed_over_fp64": Tb64 / Tb32,
-        "per_problem_fp64_dof_per_s": dof * args.steps / Tc64,
-        "per_problem_mixed_dof_per_s": dof * args.steps / Tc32,
-        "batched_over_per_problem_fp64": Tc64 / Tb32,
+        "per_problem_fp64_dof_per_s": dof * args.steps / Tc64 if Tc64 else None,
+        "per_problem_mixed_dof_per_s": dof * args.steps / Tc32 if Tc32 else None,
+        "batched_over_per_problem_fp64": Tc64 / Tb64 if Tc64 else None,
+        "batched_over_per_problem_mixed": Tc32 / Tb32 if Tc32 else None,
         "concurrent_threads": args.threads,
         "batch_setup_seconds": setup32,
-        "fp64_iterations_equal_per_problem": itb64 == itc64,
+        "fp64_iterations_equal_per_problem": itb64 == itc64 if itc64 else None,
         "tune": tune_report(R, [(2, P.ns[0]) for P in probs]),
         "worst_true_residual_mixed": worst32,
         "worst_true_residual_fp64": worst64,
@@ -483,6 +486,8 @@ def main():
                     help="only the standalone kernel timings (for counter runs under rocprofv3)")
     ap.add_argument("--problems", type=int, default=512, help="many problems: number of problems")
     ap.add_argument("--threads", type=int, default=8, help="many problems: concurrent solves")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="many problems: the batched solves only (no per-problem comparison)")
     ap.add_argument("--timer", metavar="FILE",
                     help="after the timed solves, one more mixed-precision solve with a Timer attached: its "
                          "timing tree (Iteration, per-level Total Patch Smooth) saved as JSON to FILE and "
@@ -589,9 +594,9 @@ def main():
         del g2m, g2d
         try:
             r5 = run_many_problems(argparse.Namespace(steps=args.steps, warmup=args.warmup, problems=512,
-                                                      threads=args.threads), emit=False)
+                                                      threads=args.threads, batched_only=False), emit=False)
             keep = ("value", "fp64_dof_per_s", "mixed_over_fp64", "per_problem_fp64_dof_per_s",
-                    "per_problem_mixed_dof_per_s", "batched_over_per_problem_fp64",
+                    "per_problem_mixed_dof_per_s", "batched_over_per_problem_fp64", "batched_over_per_problem_mixed",
                     "fp64_iterations_equal_per_problem", "ms_per_step", "worst_true_residual_mixed",
                     "worst_true_residual_fp64", "tune")
             out["batched_small_problems"] = {"workload": r5["config"]["workload"], "dof": r5["config"]["dof"],
