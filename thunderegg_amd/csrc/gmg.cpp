@@ -1005,9 +1005,13 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   auto t0 = std::chrono::steady_clock::now();
   int rc = TE_OK;
   int num_its = 0;
-  // resid = b - A x ; r0_norm = ||b|| ; rhat = p = resid ; rho = rhat.resid
-  TE_RC(launch_ghost_fill(gf, TE_F64, x, 1, s));
-  TE_RC(launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, b, x, g->resid, 1, s));
+  // resid = b - A x ; r0_norm = ||b|| ; rhat = p = resid ; rho = rhat.resid (on the compact
+  // path in one launch from the padded b and x, launch_c3_init)
+  const bool c3_init = g->c3 && g->xrfly;
+  if (!c3_init) {
+    TE_RC(launch_ghost_fill(gf, TE_F64, x, 1, s));
+    TE_RC(launch_star(gf, TE_F64, TE_F64, TE_F64, g->o.op_kind, g->o.neumann, cf, b, x, g->resid, 1, s));
+  }
   if (g->c3) {
     C3Bicg a{};
     a.scal = g->scal;
@@ -1022,7 +1026,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.pa = g->pa;
     a.pb = g->pb;
     a.dtm = TE_F64;
-    TE_RC(launch_c3_bicg(BICG_INIT, d, a, nb, s));
+    if (c3_init) TE_RC(launch_c3_init(d, g->o.neumann, a, nb, s));
+    else TE_RC(launch_c3_bicg(BICG_INIT, d, a, nb, s));
   } else {
     TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s,
                       TE_F64, m32 ? g->p32 : nullptr));

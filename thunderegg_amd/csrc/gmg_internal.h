@@ -119,8 +119,9 @@ struct te_gmg {
   // TE_NO_XCOL=1 reads the x faces from the cells instead (same values)
   bool xcol = true;
   // the compact path's s, x / r and p updates form A M p and A M s themselves
-  // (launch_c3_upd), so the operator launches store nothing; TE_NO_XRFLY=1 stores them and
-  // reads them back
+  // (launch_c3_upd), so the operator launches store nothing, and the setup forms b - A x
+  // from the padded b and x in one launch (launch_c3_init); TE_NO_XRFLY=1 stores the
+  // products and reads them back, and sets up through the padded residual
   bool xrfly = true;
   struct C3Level {
     void* f = nullptr;

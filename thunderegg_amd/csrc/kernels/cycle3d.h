@@ -82,6 +82,10 @@ struct C3Bicg {
   int nfold;
 };
 int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hipStream_t s);
+// BICG_INIT from the caller's padded b and x alone: r = b - A x formed in the launch (the face
+// ghosts from the same-level neighbours' cells of x or the physical condition; a.rpad unused),
+// then x, r, rhat, p (+ a.w32) compact and the partial sums b.b, r.r.  One wavefront per patch.
+int launch_c3_init(const te_domain* d, int neumann, const C3Bicg& a, int nslots, hipStream_t s);
 // BICG_S, BICG_XR and BICG_P with the operator products formed in the launch, one wavefront
 // per patch: A M s (XR) from a.ms, and with apf A M p (S, P) from a.mp (then a.ap is unused:
 // the operator launches need not store either; apf needs a.mp apart from a.p, as in the

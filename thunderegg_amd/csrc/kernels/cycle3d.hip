@@ -768,6 +768,106 @@ __global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict_
     }
 }
 
+// BiCGStab's setup (Iterative/BiCGStab.h:118-140) from the caller's padded b and x: r = b - A x
+// (the star operator with the face ghosts taken from the same-level neighbours' cells of x or
+// the physical condition, star_slice's arithmetic), then the compact x, r, rhat = r, p = r
+// (+ p32) and the partial sums b.b and r.r: the ghost fill, the residual on padded vectors
+// and the packing in one pass.  One wavefront per patch, persistent over the patches.
+__global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict__ dp, C3Bicg a, int np,
+                                                      int nslots, int neumann)
+{
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const double bc = neumann ? 1.0 : -1.0;
+  const double* __restrict__ xp = a.xpad;
+  // cell (x, y, z) of patch q in the padded layout (18^3 values per patch)
+  auto pidx = [](int q, int x, int y, int z) { return q * (18 * 18 * 18) + (x + 1) + 18 * (y + 1) + 324 * (z + 1); };
+  auto prow = [&](const double* __restrict__ v, int q, int y, int z, double (&r)[4]) {
+    const int o = pidx(q, 4 * g, y, z);
+#pragma unroll
+    for (int k = 0; k < 4; k++) r[k] = v[o + k];
+  };
+  double sa = 0.0, sb = 0.0;
+  for (int p = blockIdx.x * 4 + wv; p < np; p += gridDim.x * 4) {
+    const DevPatch& P = dp[p];
+    int q[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) q[s] = P.type[20 + s] >= 0 ? P.nbr[20 + s][0] : -1;
+    const double ihx = P.inv_h2[0], ihy = P.inv_h2[1], ihz = P.inv_h2[2];
+    double cur[4], prv[4], nxt[4];
+    prow(xp, p, c, 0, cur);
+    if (q[4] >= 0) prow(xp, q[4], c, N - 1, prv);
+    else
+#pragma unroll
+      for (int k = 0; k < 4; k++) prv[k] = bc * cur[k];
+    for (int z = 0; z < N; z++) {
+      if (z < N - 1) prow(xp, p, c, z + 1, nxt);
+      else if (q[5] >= 0) prow(xp, q[5], c, 0, nxt);
+      else
+#pragma unroll
+        for (int k = 0; k < 4; k++) nxt[k] = bc * cur[k];
+      // the operator on slice z (star_slice with the faces from the padded neighbours)
+      double xm = __shfl(cur[3], (lane - 16) & 63, 64);
+      double xq = __shfl(cur[0], (lane + 16) & 63, 64);
+      double ym[4], yp[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        ym[k] = __shfl(cur[k], (lane - 1) & 63, 64);
+        yp[k] = __shfl(cur[k], (lane + 1) & 63, 64);
+      }
+      if (g == 0) xm = q[0] >= 0 ? xp[pidx(q[0], N - 1, c, z)] : bc * cur[0];
+      if (g == 3) xq = q[1] >= 0 ? xp[pidx(q[1], 0, c, z)] : bc * cur[3];
+      if (c == 0) {
+        if (q[2] >= 0) prow(xp, q[2], N - 1, z, ym);
+        else
+#pragma unroll
+          for (int k = 0; k < 4; k++) ym[k] = bc * cur[k];
+      }
+      if (c == N - 1) {
+        if (q[3] >= 0) prow(xp, q[3], 0, z, yp);
+        else
+#pragma unroll
+          for (int k = 0; k < 4; k++) yp[k] = bc * cur[k];
+      }
+      double bv[4];
+      prow(a.b, p, c, z, bv);
+      double rv[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const double mid = cur[k];
+        const double lo = k == 0 ? xm : cur[k - 1];
+        const double hi = k == 3 ? xq : cur[k + 1];
+        double acc = 0.0;
+        acc += (hi - 2.0 * mid + lo) * ihx;
+        acc += (yp[k] - 2.0 * mid + ym[k]) * ihy;
+        acc += (nxt[k] - 2.0 * mid + prv[k]) * ihz;
+        rv[k] = bv[k] - acc;
+        sa += bv[k] * bv[k];
+        sb += rv[k] * rv[k];
+      }
+      const int o = p * NC + cidx(4 * g, c, z);
+      st4(a.x, o, cur);
+      st4(a.r, o, rv);
+      st4(a.rhat, o, rv);
+      st4(a.p, o, rv);
+      if (a.w32) st4(a.w32, o, rv);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        prv[k] = cur[k];
+        cur[k] = nxt[k];
+      }
+    }
+  }
+  const double ra = block_reduce<RED_SUM>(sa);
+  if (threadIdx.x == 0) a.pa[blockIdx.x] = ra;
+  const double rb = block_reduce<RED_SUM>(sb);
+  if (threadIdx.x == 0) a.pb[blockIdx.x] = rb;
+  if (blockIdx.x == 0)
+    for (int k = (int)gridDim.x + threadIdx.x; k < nslots; k += blockDim.x) {
+      a.pa[k] = 0.0;
+      a.pb[k] = 0.0;
+    }
+}
+
 template <typename TD, typename TS, bool TO_PADDED>
 __global__ void __launch_bounds__(256) c3_copy_kernel(TD* __restrict__ dst, const TS* __restrict__ src, long n)
 {
@@ -996,6 +1096,20 @@ int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bo
   }
 #undef TE_C3UM
 #undef TE_C3U
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_init(const te_domain* d, int neumann, const C3Bicg& a, int nslots, hipStream_t s)
+{
+  if (nslots < 1 || nslots > kFoldMax) {
+    set_error("internal: bad partial count");
+    return TE_ERR_ARG;
+  }
+  int grid = (d->np + 3) / 4;
+  if (grid > nslots) grid = nslots;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(c3_init_kernel, dim3(grid), dim3(256), 0, s, d->d_patches, a, d->np, nslots, neumann);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
