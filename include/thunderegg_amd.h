@@ -354,7 +354,8 @@ double te_gmg_last_solve_seconds(const te_gmg* g);
 enum te_kernel_id {
   TE_K_SWEEP_ZERO = 0,      /* patch-solve sweep from u = 0 (first pre-smoothing sweep) */
   TE_K_SWEEP = 1,           /* sweep with the interface terms from u (later sweeps) */
-  TE_K_SWEEP_INTERP = 2,    /* sweep on u + P uc (interpolation fused, first post-smoothing) */
+  TE_K_SWEEP_INTERP = 2,    /* sweep on u + P uc (first post-smoothing; on large levels the boundary
+                               layer of u + P uc, then a sweep on it) */
   TE_K_RESID_RESTRICT = 3,  /* r = f - A u restricted to the coarser level */
   TE_K_OPERATOR = 4,        /* outer FP64 A (M s) with the dots as.s and as.as (A M s stored only
                                where the updates read it back: TE_NO_XRFLY=1) */

@@ -232,3 +232,28 @@ def test_compact3d_bicgstab_initial_guess_matches_oracle(monkeypatch):
         assert relerr(mesh.interior(x, 3), mesh.interior(xref, 3)) <= 1e-10, nx
         sols.append(mesh.interior(x, 3))
     assert relerr(sols[0], sols[1]) <= 1e-11
+
+
+@pytest.mark.parametrize("prec", [R.F64, R.F32])
+def test_compact3d_boundary_interpolation_bit_identical(monkeypatch, prec):
+    """On levels of 4096 or more patches the first post-smoothing sweep runs as a launch that
+    writes the boundary layer of u + P uc (and its face columns) and a plain sweep on it: the
+    values the interpolating sweep gathers itself (TE_NO_FACEINTERP=1), so cycles and solves
+    give the same bits.  Five levels, 4096 finest patches."""
+    _print_case("compact3d boundary interpolation", prec)
+    levels = _levels(4)
+    assert len(levels[0]) == 4096
+    f = random_field(mesh.padded_shape(4096, 3, NS), 101)
+    b = _rhs_sin(levels)
+    out = []
+    for nf in ("0", "1"):
+        monkeypatch.setenv("TE_NO_FACEINTERP", nf)
+        g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=prec, pre_sweeps=2,
+                       post_sweeps=2)
+        v = g.vcycle(f)
+        x, its, hist = g.solve(b, tol=1e-10, precond=2)
+        out.append((mesh.interior(v, 3), mesh.interior(x, 3), its, np.asarray(hist)))
+    (v0, x0, i0, h0), (v1, x1, i1, h1) = out
+    assert np.array_equal(v0, v1)
+    assert i0 == i1 and np.array_equal(h0, h1)
+    assert np.array_equal(x0, x1)

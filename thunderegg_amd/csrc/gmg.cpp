@@ -288,9 +288,18 @@ int visit_c3(te_gmg* g, int li, const void* f, int dtf, void* u_ext, int ext_dt)
     const int dto = to_ext ? ext_dt : Lv.dt;
     const void* uc = mode == 2 ? g->C3[li + 1].cur : nullptr;
     const int dtc = mode == 2 ? g->L[li + 1].dt : TE_F64;
+    const long np = Lv.d->np;
     TE_RC(sweep_timing(g, li, false));
-    TE_RC(launch_c3_sweep(Lv.plan, Lv.dt, dtf, f, buf[cur], dto, out, mode, dtc, uc, mode == 2 ? Lv.link : nullptr,
-                          g->s, xcol_of(g, buf[cur], Lv.d->np, Lv.dt), xcol_of(g, out, Lv.d->np, dto)));
+    if (mode == 2 && Cl.u3 && g->finterp && g->xcol) {
+      // the boundary layer of u + P uc first, then a mode-1 sweep on it (the same values)
+      TE_RC(launch_c3_face_interp(Lv.link, Lv.dt, buf[cur], xcol_of(g, buf[cur], np, Lv.dt), dtc, uc, Cl.u3,
+                                  xcol_of(g, Cl.u3, np, Lv.dt), g->s));
+      TE_RC(launch_c3_sweep(Lv.plan, Lv.dt, dtf, f, Cl.u3, dto, out, 1, TE_F64, nullptr, nullptr, g->s,
+                            xcol_of(g, Cl.u3, np, Lv.dt), xcol_of(g, out, np, dto)));
+    } else {
+      TE_RC(launch_c3_sweep(Lv.plan, Lv.dt, dtf, f, buf[cur], dto, out, mode, dtc, uc, mode == 2 ? Lv.link : nullptr,
+                            g->s, xcol_of(g, buf[cur], np, Lv.dt), xcol_of(g, out, np, dto)));
+    }
     TE_RC(sweep_timing(g, li, true));
     if (to_ext) {
       Cl.cur = u_ext;
@@ -497,6 +506,7 @@ void free_gmg(te_gmg* g)
     (void)hipFree(c.f);
     (void)hipFree(c.u);
     (void)hipFree(c.u2);
+    (void)hipFree(c.u3);
   }
   for (void* v : { (void*)g->c_r, (void*)g->c_rhat, (void*)g->c_p, (void*)g->c_ap, (void*)g->c_as, (void*)g->c_s,
                    (void*)g->c_x,
@@ -582,6 +592,8 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
     g->xcol = !(nx && nx[0] == '1');
     const char* nxr = std::getenv("TE_NO_XRFLY");
     g->xrfly = !(nxr && nxr[0] == '1');
+    const char* nfi = std::getenv("TE_NO_FACEINTERP");
+    g->finterp = !(nfi && nfi[0] == '1');
   }
   for (int i = 0; i < num_levels && !rc; i++) {
     GLevel& Lv = g->L[i];
@@ -603,7 +615,9 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
       te_gmg::C3Level& Cl = g->C3[i];
       if ((e = hipMalloc(&Cl.f, cf)) != hipSuccess || (e = hipMalloc(&Cl.u, cb)) != hipSuccess ||
           (e = hipMalloc(&Cl.u2, cb)) != hipSuccess || (e = hipMemset(Cl.f, 0, cf)) != hipSuccess ||
-          (e = hipMemset(Cl.u, 0, cb)) != hipSuccess || (e = hipMemset(Cl.u2, 0, cb)) != hipSuccess) {
+          (e = hipMemset(Cl.u, 0, cb)) != hipSuccess || (e = hipMemset(Cl.u2, 0, cb)) != hipSuccess ||
+          (!coarsest && Lv.d->np >= kC3InterpMin &&
+           ((e = hipMalloc(&Cl.u3, cb)) != hipSuccess || (e = hipMemset(Cl.u3, 0, cb)) != hipSuccess))) {
         set_error(std::string("te_gmg_create: ") + hipGetErrorString(e));
         rc = TE_ERR_HIP;
         break;
@@ -1221,6 +1235,12 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
         return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 1, TE_F64, nullptr, nullptr, s, xu,
                                xu2);
       case TE_K_SWEEP_INTERP:
+        if (C0.u3 && g->finterp && g->xcol) {  // as the cycle runs it: boundary interpolation + mode-1 sweep
+          void* xu3 = xcol_of(g, C0.u3, np, L0.dt);
+          TE_RC(launch_c3_face_interp(L0.link, L0.dt, C0.u, xu, g->L[1].dt, g->C3[1].u, C0.u3, xu3, s));
+          return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u3, L0.dt, C0.u2, 1, TE_F64, nullptr, nullptr, s,
+                                 xu3, xu2);
+        }
         return launch_c3_sweep(L0.plan, L0.dt, L0.dt, C0.f, C0.u, L0.dt, C0.u2, 2, g->L[1].dt, g->C3[1].u, L0.link,
                                s, xu, xu2);
       case TE_K_RESID_RESTRICT:

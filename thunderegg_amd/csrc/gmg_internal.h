@@ -127,8 +127,13 @@ struct te_gmg {
     void* f = nullptr;
     void* u = nullptr;
     void* u2 = nullptr;
+    // levels of at least kC3InterpMin patches below a coarser one: the boundary layer of
+    // u + P uc for the first post-smoothing sweep (launch_c3_face_interp), or nullptr
+    void* u3 = nullptr;
     void* cur = nullptr;  // the buffer holding this level's solution after its visit
   };
+  // TE_NO_FACEINTERP=1: the first post-smoothing sweep gathers u + P uc itself (mode 2)
+  bool finterp = true;
   std::vector<C3Level> C3;
   double *c_r = nullptr, *c_rhat = nullptr, *c_p = nullptr, *c_ap = nullptr, *c_as = nullptr, *c_s = nullptr;
   double* c_x = nullptr;  // compact copy of the caller's x during a solve

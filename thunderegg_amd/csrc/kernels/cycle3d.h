@@ -20,6 +20,9 @@ constexpr int kC3Cells = kC3N * kC3N * kC3N;
 // kernels that write a vector with face columns write them too; a reader is given them
 // only where every writer of the vector did so (nullptr: the x faces come from the cells).
 constexpr int kC3XCol = 2 * kC3N * kC3N;
+// levels of at least this many patches run the first post-smoothing sweep as a boundary
+// interpolation (launch_c3_face_interp) and a mode-1 sweep
+constexpr int kC3InterpMin = 4096;
 inline void* c3_xcol(const void* v, long np, int dt)
 {
   return v ? (void*)((char*)v + (size_t)np * kC3Cells * (dt == TE_F64 ? 8 : 4)) : nullptr;
@@ -102,6 +105,11 @@ int launch_c3_unpack(const te_domain* d, int dtd, void* dst_padded, int dts, con
 // out = u + P uc (compact, this level's type dt; uc of type dtc)
 int launch_c3_interp(const te_level_link* link, int dtc, const void* uc, int dt, const void* u, void* out,
                      hipStream_t s);
+// the boundary layer of u + P uc (this level's type dt, uc of type dtc) and its face columns
+// into out / xout: what the interpolating sweep (mode 2) reads, for a mode-1 sweep on out;
+// xu: u's face columns
+int launch_c3_face_interp(const te_level_link* link, int dt, const void* u, const void* xu, int dtc, const void* uc,
+                          void* out, void* xout, hipStream_t s);
 // x := 0 (n elements of type dt), a kernel (no memset node in captured graphs)
 int launch_c3_zero(long n, int dt, void* x, hipStream_t s);
 

@@ -868,6 +868,39 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
     }
 }
 
+// The boundary layer of u + P uc (DirectInterpolator, GMG/DirectInterpolator.h:60-91) and its
+// face columns: the values the interpolating sweep (c3_sweep_kernel mode 2) reads from u and
+// uc, written into out / xout, so that the first post-smoothing sweep can run as a mode-1
+// sweep on out (the same values, so the same bits) without gathering coarse values.  Cells
+// written: slices z = 0 and 15, rows y = 0 and 15, and the face columns of every row; the
+// rest of out is left alone (no sweep reads it).  One wavefront per patch.
+template <typename T, typename TC>
+__global__ void __launch_bounds__(256) c3_face_interp_kernel(const T* __restrict__ u, const T* __restrict__ xu,
+                                                             const TC* __restrict__ uc,
+                                                             const int32_t* __restrict__ parent,
+                                                             const int32_t* __restrict__ porth, T* __restrict__ out,
+                                                             T* __restrict__ xout, int np)
+{
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const int p = blockIdx.x * 4 + wv;
+  if (p >= np) return;
+  const C3U<T, TC, true> U{ u, uc, parent, porth };
+  const int cp = parent[p], o = porth[p];
+  const int side = g == 0 ? 0 : 1;
+  for (int z = 0; z < N; z++) {
+    if (z == 0 || z == N - 1 || c == 0 || c == N - 1) {
+      T r[4];
+      U.row4(p, 4 * g, c, z, r);
+      st4(out, p * NC + cidx(4 * g, c, z), r);
+    }
+    if (g == 0 || g == 3) {
+      T v = xu[xci(p, side, c, z)];
+      v = T(v + T(uc[cp * NC + U.coarse(o, side ? N - 1 : 0, c, z)]));
+      xout[xci(p, side, c, z)] = v;
+    }
+  }
+}
+
 template <typename TD, typename TS, bool TO_PADDED>
 __global__ void __launch_bounds__(256) c3_copy_kernel(TD* __restrict__ dst, const TS* __restrict__ src, long n)
 {
@@ -1110,6 +1143,31 @@ int launch_c3_init(const te_domain* d, int neumann, const C3Bicg& a, int nslots,
   if (grid > nslots) grid = nslots;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(c3_init_kernel, dim3(grid), dim3(256), 0, s, d->d_patches, a, d->np, nslots, neumann);
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_face_interp(const te_level_link* link, int dt, const void* u, const void* xu, int dtc, const void* uc,
+                          void* out, void* xout, hipStream_t s)
+{
+  const int np = link->fine->np;
+  if (np == 0) return TE_OK;
+  if (!u || !xu || !uc || !out || !xout) {
+    set_error("internal: the boundary interpolation needs u, uc and face columns");
+    return TE_ERR_ARG;
+  }
+  const int grid = (np + 3) / 4;
+#define TE_C3F(T_, TC_)                                                                                       \
+  hipLaunchKernelGGL((c3_face_interp_kernel<T_, TC_>), dim3(grid), dim3(256), 0, s, (const T_*)u, (const T_*)xu, \
+                     (const TC_*)uc, link->d_parent, link->d_orth, (T_*)out, (T_*)xout, np)
+  if (dt == TE_F64 && dtc == TE_F64) TE_C3F(double, double);
+  else if (dt == TE_F32 && dtc == TE_F64) TE_C3F(float, double);
+  else if (dt == TE_F32 && dtc == TE_F32) TE_C3F(float, float);
+  else {
+    set_error("unsupported dtype combination for the 3D boundary interpolation");
+    return TE_ERR_UNSUPPORTED;
+  }
+#undef TE_C3F
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
