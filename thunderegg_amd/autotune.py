@@ -7,9 +7,11 @@ of the 2D matrix-core kernel) on a uniform level of about 4-16 thousand patches,
 events (te_tune_time_dft), checks that the variant's result has the same bits as the hand
 default's, and writes the table the library loads on first use (tune_table.txt beside
 libthunderegg_amd.so).  A variant replaces the default only when it is faster by more than
-`--margin` (3 %).
+`--margin` (3 %).  The variants are timed in interleaved rounds (each round times every
+variant once, starting at a different one) and each keeps its median over the rounds, so
+that clock drift during the search favours none.
 
-Usage: python -m thunderegg_amd.autotune [--out PATH] [--reps N] [--margin F]
+Usage: python -m thunderegg_amd.autotune [--out PATH] [--reps N] [--rounds R] [--margin F]
 """
 from __future__ import annotations
 
@@ -40,7 +42,18 @@ def _level(dim, n, level):
     return dom, filler, R.DftPlan(filler)
 
 
-def tune_shape(dim, n, level, dtype, reps=20, margin=0.03, log=print):
+def interleaved(nvar, time_one, rounds):
+    """{variant: median time} over `rounds` rounds, each timing every variant once, the
+    starting variant rotating from round to round."""
+    samples = {v: [] for v in range(nvar)}
+    for r in range(rounds):
+        for k in range(nvar):
+            v = (k + r) % nvar
+            samples[v].append(time_one(v))
+    return {v: float(np.median(t)) for v, t in samples.items()}
+
+
+def tune_shape(dim, n, level, dtype, reps=20, margin=0.03, log=print, rounds=5):
     """Times every variant on one shape; returns (family, best variant, {variant: ms})."""
     dom, filler, plan = _level(dim, n, level)
     fam = R.tune_family(dim, n)
@@ -51,11 +64,12 @@ def tune_shape(dim, n, level, dtype, reps=20, margin=0.03, log=print):
     u = R.DeviceArray.from_host(rng.standard_normal(shape).astype(npdt))
     filler.fill_ghost(u)
     outs = {}
-    times = {}
-    for v in range(R.tune_num_variants(fam)):
-        out = R.DeviceArray(shape, npdt)
+    nvar = R.tune_num_variants(fam)
+    out = R.DeviceArray(shape, npdt)
+    times = interleaved(nvar, lambda v: plan.time_variant(f, u, out, v, reps), rounds)
+    for v in range(nvar):
         out.zero()
-        times[v] = plan.time_variant(f, u, out, v, reps)
+        plan.time_variant(f, u, out, v, 1)
         outs[v] = out.get()
     ref = outs[0]
     for v, o in outs.items():
@@ -69,7 +83,7 @@ def tune_shape(dim, n, level, dtype, reps=20, margin=0.03, log=print):
     return fam, best, times
 
 
-def tune_c3(dtype, reps=20, margin=0.03, log=print):
+def tune_c3(dtype, reps=20, margin=0.03, log=print, rounds=5):
     """The 3D fused sweeps' variants (slices per load batch, registers sized for more waves
     per SIMD) timed on the compact cycle of a uniform level of 16^3 patches
     (te_tune_time_c3): the finest level's sweeps of one 2 + 2 V-cycle, i.e. the sweep from
@@ -83,14 +97,18 @@ def tune_c3(dtype, reps=20, margin=0.03, log=print):
     rng = np.random.default_rng(5)
     f = rng.standard_normal(g.shape)
     fam = R.TUNE_C3_SWEEP
-    times, outs = {}, {}
-    for v in range(R.tune_num_variants(fam)):
+    def time_one(v):
         t = 0.0
         for name, per_cycle in (("sweep_zero", 1), ("sweep", 2), ("sweep_interp", 1)):
             ms = C.c_double()
             R.check(R.lib().te_tune_time_c3(g.h, g.KERNELS[name], v, reps, C.byref(ms)))
             t += per_cycle * ms.value
-        times[v] = t
+        return t
+
+    nvar = R.tune_num_variants(fam)
+    times = interleaved(nvar, time_one, rounds)
+    outs = {}
+    for v in range(nvar):
         R.tune_set(fam, 3, 16, dtype, v)
         outs[v] = g.vcycle(f)
     R.tune_clear()
@@ -110,22 +128,24 @@ def main(argv=None):
     ap.add_argument("--out", default=DEFAULT_OUT)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--margin", type=float, default=0.03)
+    ap.add_argument("--rounds", type=int, default=5)
     args = ap.parse_args(argv)
     R.lib()
     R.tune_clear()  # time against the hand defaults, not a table loaded earlier
     lines = ["# DFT patch-solve launch variants: family dim n dtype variant (thunderegg_amd/autotune.py)",
              f"# written {time.strftime('%Y-%m-%d')} by: python -m thunderegg_amd.autotune --reps {args.reps} "
-             f"--margin {args.margin}",
+             f"--rounds {args.rounds} --margin {args.margin}",
              "# timings per variant in microseconds (v0 = hand default)"]
     for dim, n, level in SHAPES:
         for dtype in (R.F32, R.F64):
             fam, best, times = tune_shape(dim, n, level, dtype, args.reps, args.margin,
-                                          log=lambda *a: print(*a, file=sys.stderr, flush=True))
+                                          log=lambda *a: print(*a, file=sys.stderr, flush=True), rounds=args.rounds)
             lines.append("# " + " ".join(f"v{v}={t * 1e3:.1f}" for v, t in times.items())
                          + f"  gain {times[0] / times[best]:.3f}x")
             lines.append(f"{fam} {dim} {n} {dtype} {best}")
     for dtype in (R.F32, R.F64):
-        fam, best, times = tune_c3(dtype, args.reps, args.margin, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+        fam, best, times = tune_c3(dtype, args.reps, args.margin, log=lambda *a: print(*a, file=sys.stderr, flush=True),
+                                   rounds=args.rounds)
         lines.append("# " + " ".join(f"v{v}={t * 1e3:.1f}" for v, t in times.items())
                      + f"  gain {times[0] / times[best]:.3f}x")
         lines.append(f"{fam} 3 16 {dtype} {best}")

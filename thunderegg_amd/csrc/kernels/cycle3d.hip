@@ -125,8 +125,10 @@ struct C3U {
 // The patch solve.  MODE 0: u = 0 (f only); 1: interface terms from uin; 2: from uin + P uc.
 // SB: slices per load batch of modes 1 / 2 (the autotuner's variants; mode 0 uses 4).
 // XC: uin and out have face columns (cycle3d.h): the x faces are read from uin's, and out's
-// are written.  WPE: the minimum waves per SIMD the registers are allocated for (1: no request)
-template <typename T, typename TF, typename TO, typename TC, int MODE, int SB, bool XC, int WPE>
+// are written.  WPE: the minimum waves per SIMD the registers are allocated for (1: no request).
+// FA (modes 1, 2): the loads of f of all four slices are issued at the start, the interface
+// loads per batch
+template <typename T, typename TF, typename TO, typename TC, int MODE, int SB, bool XC, int WPE, bool FA>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) c3_sweep_kernel(const DevPatch* __restrict__ dp,
                                                        const int* __restrict__ ptype, const T* __restrict__ mats,
                                                        const T* __restrict__ s2tab, const TF* __restrict__ f,
@@ -226,6 +228,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
   // SB slices (f, and for the interface terms the lane's own boundary rows and the
   // neighbours' facing rows) are all issued before its first product (more slices per
   // batch: more loads in flight, more registers, fewer waves per SIMD)
+  T Fa[FA ? 4 : 1][4];
+  if (FA) {
+#pragma unroll
+    for (int i = 0; i < (FA ? 4 : 1); i++) ld4(f, p * NC + cidx(4 * g, c, w + 4 * i), Fa[i]);
+  }
 #pragma unroll
   for (int i0 = 0; i0 < 4; i0 += SB) {
     T F[SB][4], ub[SB][4], gx[SB][4], gy[SB][4], gz[SB][4];
@@ -233,7 +240,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
 #pragma unroll
     for (int j = 0; j < SB; j++) {
       const int z = w + 4 * (i0 + j);
-      ld4(f, p * NC + cidx(4 * g, c, z), F[j]);
+      if (FA) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[j][k] = Fa[FA ? i0 + j : 0][k];
+      } else {
+        ld4(f, p * NC + cidx(4 * g, c, z), F[j]);
+      }
       if (MODE != 0) {
         const bool onz = (z == 0 && has[4]) || (z == N - 1 && has[5]);
         if (XC) {
@@ -961,14 +973,15 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
   const int var = tune_variant(TUNE_C3_SWEEP, 3, N, dt);
   const int sb = c3_sweep_batch(var, dt);
   const int wpe = c3_sweep_waves(var);  // with face columns only (the cycle's own path)
+  const bool fa = c3_sweep_f_ahead(var);
   // face columns: written when out has them; then uin (modes 1 and 2) must have them too
   const bool xc = xout != nullptr;
   if (xc && mode != 0 && !xin) {
     set_error("internal: a sweep that writes face columns needs its input's");
     return TE_ERR_ARG;
   }
-#define TE_C3X(T_, TF_, TO_, TC_, M_, SB_, XC_, W_)                                                                  \
-  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_, SB_, XC_, W_>), dim3(d->np), dim3(256), 0, s,          \
+#define TE_C3X(T_, TF_, TO_, TC_, M_, SB_, XC_, W_, FA_)                                                             \
+  hipLaunchKernelGGL((c3_sweep_kernel<T_, TF_, TO_, TC_, M_, SB_, XC_, W_, FA_>), dim3(d->np), dim3(256), 0, s,     \
                      d->d_patches,                                                                                  \
                      pl->d_ptype, (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_mats64 : (const void*)pl->d_mats32), \
                      (const T_*)(sizeof(T_) == 8 ? (const void*)pl->d_s2_64 : (const void*)pl->d_s2_32),             \
@@ -978,12 +991,18 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
   do {                                                        \
     if constexpr ((M_ == 0 || SB_ < 4) && sizeof(T_) == 4) {  \
       if (xc && wpe == 5) {                                   \
-        TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 5);          \
+        TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 5, false);   \
         break;                                                \
       }                                                       \
     }                                                         \
-    if (xc) TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 1);      \
-    else TE_C3X(T_, TF_, TO_, TC_, M_, SB_, false, 1);        \
+    if constexpr (M_ != 0 && SB_ < 4 && sizeof(T_) == 4) {    \
+      if (xc && fa) {                                         \
+        TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 1, true);    \
+        break;                                                \
+      }                                                       \
+    }                                                         \
+    if (xc) TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 1, false); \
+    else TE_C3X(T_, TF_, TO_, TC_, M_, SB_, false, 1, false);   \
   } while (0)
 #define TE_C3B(T_, TF_, TO_, TC_, M_)                \
   do {                                               \
