@@ -711,58 +711,86 @@ __global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict_
     constexpr bool WP = APF && MODE != BICG_XR;
     if (WP) wp.init(p, q);
     if (MODE == BICG_XR) wm.init(p, q);
+    // the vectors of slice z (c*): S: r (, ap); XR: M p, x, s, rhat; P: p, r (, ap).  S and P
+    // load them one slice ahead (n*); XR, with two stencil windows already, at the slice
+    // (ahead, its registers would cost a wave per SIMD)
+    constexpr bool PF = MODE != BICG_XR;
+    double c0[4], c1[4], c2[4], c3[4], n0[4], n1[4], n2[4], n3[4];
+    auto load = [&](int z, double(&s0)[4], double(&s1)[4], double(&s2)[4], double(&s3)[4]) {
+      const int o = p * NC + 256 * z + ro;
+      if (MODE == BICG_XR) {
+        ld4((const TM*)a.mp, o, s0);
+        ld4(a.x, o, s1);
+        ld4(a.sv, o, s2);
+        ld4(a.rhat, o, s3);
+      } else {
+        ld4(MODE == BICG_S ? a.r : a.p, o, s0);
+        if (MODE == BICG_P) ld4(a.r, o, s1);
+        if (!WP) ld4(a.ap, o, s3);
+      }
+    };
+    if (PF) load(0, c0, c1, c2, c3);
     for (int z = 0; z < N; z++) {
       const int o = p * NC + 256 * z + ro;
+      if (PF && z < N - 1) load(z + 1, n0, n1, n2, n3);
       double apv[4];
       if (MODE != BICG_XR) {
-        if (WP) wp.step(z, q, ihx, ihy, ihz, apv);
-        else ld4(a.ap, o, apv);
+        if (WP) {
+          wp.step(z, q, ihx, ihy, ihz, apv);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; k++) apv[k] = c3[k];
+        }
       }
-      if (MODE == BICG_S) {
-        double rv[4], sv[4];
-        ld4(a.r, o, rv);
+      if (MODE == BICG_S) {  // c0: r
+        double sv[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          double sk = rv[k];
+          double sk = c0[k];
           sk += apv[k] * -alpha;
           sv[k] = sk;
           sa += sk * sk;
         }
         st4(a.sv, o, sv);
         if (a.w32) st4(a.w32, o, sv);
-      } else if (MODE == BICG_XR) {
-        double asv[4], mpv[4], xv[4], rv[4], rh[4];
+      } else if (MODE == BICG_XR) {  // c0: M p, c1: x, c2: s, c3: rhat
+        double asv[4], xv[4], rv[4];
         wm.step(z, q, ihx, ihy, ihz, asv);
-        ld4((const TM*)a.mp, o, mpv);
-        ld4(a.x, o, xv);
-        ld4(a.sv, o, rv);
-        ld4(a.rhat, o, rh);
+        load(z, c0, c1, c2, c3);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          xv[k] += mpv[k] * alpha + wm.cur[k] * omega;
-          double r = rv[k];
+          xv[k] = c1[k];
+          xv[k] += c0[k] * alpha + wm.cur[k] * omega;
+          double r = c2[k];
           r += asv[k] * -omega;
           rv[k] = r;
-          sa += r * rh[k];
+          sa += r * c3[k];
           sb += r * r;
         }
         st4(a.x, o, xv);
         st4(a.r, o, rv);
         wm.next();
-      } else {  // BICG_P
-        double pv[4], rv[4];
-        ld4(a.p, o, pv);
-        ld4(a.r, o, rv);
+      } else {  // BICG_P, c0: p, c1: r
+        double pv[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          double v = pv[k];
+          double v = c0[k];
           v += apv[k] * -omega;
-          pv[k] = v * beta + rv[k];
+          pv[k] = v * beta + c1[k];
         }
         st4(a.p, o, pv);
         if (a.w32) st4(a.w32, o, pv);
       }
       if (WP) wp.next();
+      if (PF) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          c0[k] = n0[k];
+          c1[k] = n1[k];
+          c2[k] = n2[k];
+          c3[k] = n3[k];
+        }
+      }
     }
   }
   if (MODE == BICG_S || MODE == BICG_XR) {
