@@ -15,7 +15,7 @@ def clean_table():
 def test_variant_counts():
     assert R.tune_num_variants(R.TUNE_DFT_GENERIC) == 7
     assert R.tune_num_variants(R.TUNE_DFT_MFMA_2D) == 4
-    assert R.tune_num_variants(R.TUNE_C3_SWEEP) == 8
+    assert R.tune_num_variants(R.TUNE_C3_SWEEP) == 10
     assert R.tune_num_variants(7) == 0
 
 

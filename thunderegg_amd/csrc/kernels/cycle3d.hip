@@ -1025,7 +1025,8 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
     }                                                         \
     if constexpr (M_ != 0 && SB_ < 4 && sizeof(T_) == 4) {    \
       if (xc && fa) {                                         \
-        TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 1, true);    \
+        if (wpe == 6) TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 6, true); \
+        else TE_C3X(T_, TF_, TO_, TC_, M_, SB_, true, 1, true); \
         break;                                                \
       }                                                       \
     }                                                         \

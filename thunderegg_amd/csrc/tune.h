@@ -40,20 +40,21 @@ enum DftMfmaGrid {
 };
 
 // 3D fused sweeps, variant v: slices per load batch (v 0: 2 in FP32, 1 in FP64; v 1, 2, 3:
-// 1, 2, 4 slices; v 4, 5: 1, 2 slices; v 6, 7: 2, 1 slices), the minimum waves per SIMD the
-// compiler is asked to fit the registers for (v 4, 5: 5, which the FP32 sweep from zero and
-// one-slice batch reach without spilling; else, and in FP64, where the patch's LDS allows 3
-// workgroups per CU, no request), and whether the f loads of all four slices of a wave are
-// issued at its start (v 6, 7, FP32; else per batch with the interface loads)
-constexpr int kC3Variants = 8;
+// 1, 2, 4 slices; v 4, 5: 1, 2 slices; v 6, 7: 2, 1 slices; v 8, 9: 1, 2 slices), the minimum
+// waves per SIMD the compiler is asked to fit the registers for (v 4, 5: 5, which the FP32
+// sweep from zero and one-slice batch reach without spilling; v 8, 9: 6, which the FP32
+// one-slice interface sweep with f ahead reaches without spilling; else, and in FP64, where
+// the patch's LDS allows 3 workgroups per CU, no request), and whether the f loads of all four
+// slices of a wave are issued at its start (v 6 - 9, FP32 interface sweeps; else per batch)
+constexpr int kC3Variants = 10;
 inline int c3_sweep_batch(int v, int dtype)
 {
-  if (v == 1 || v == 4 || v == 7) return 1;
-  if (v == 2 || v == 5 || v == 6) return 2;
+  if (v == 1 || v == 4 || v == 7 || v == 8) return 1;
+  if (v == 2 || v == 5 || v == 6 || v == 9) return 2;
   if (v == 3) return 4;
   return dtype == 1 ? 2 : 1;  // TE_F32 = 1
 }
-inline int c3_sweep_waves(int v) { return v == 4 || v == 5 ? 5 : 1; }
-inline bool c3_sweep_f_ahead(int v) { return v == 6 || v == 7; }
+inline int c3_sweep_waves(int v) { return v == 4 || v == 5 ? 5 : (v == 8 || v == 9 ? 6 : 1); }
+inline bool c3_sweep_f_ahead(int v) { return v >= 6 && v <= 9; }
 
 }  // namespace te
