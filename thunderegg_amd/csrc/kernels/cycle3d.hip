@@ -43,19 +43,36 @@ constexpr int kYP = 20;               // LDS pitch of a spectral row (16 + 4: 16
 constexpr int kZP = 16 * kYP + 4;     // LDS pitch of a spectral slice
 constexpr int kSlab = 16 * kZP;
 
+// Addressing: every offset here is a non-negative element index into a vector of at most
+// 2 GiB (a compact vector with face columns, 4608 values per patch, is smaller than the
+// padded vector of the same domain, whose size the domain checks), so the byte offset is
+// formed in 32 bits (at(), te_internal.h) and a load or store addresses from the kernel
+// argument's scalar base plus a 32-bit vector offset, without 64-bit vector address
+// arithmetic.
+template <typename V, typename TS>
+__device__ __forceinline__ const V& vat(const TS* base, int off)
+{
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + (unsigned)off * (unsigned)sizeof(TS));
+}
+template <typename V, typename TS>
+__device__ __forceinline__ V& vat(TS* base, int off)
+{
+  return *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + (unsigned)off * (unsigned)sizeof(TS));
+}
+
 // four consecutive values starting at an offset that is a multiple of 4, converted to T
 template <typename T, typename TS>
 __device__ __forceinline__ void ld4(const TS* __restrict__ base, int off, T (&r)[4])
 {
   if constexpr (sizeof(TS) == 4) {
-    const float4 v = *reinterpret_cast<const float4*>(base + off);
+    const float4 v = vat<float4>(base, off);
     r[0] = T(v.x);
     r[1] = T(v.y);
     r[2] = T(v.z);
     r[3] = T(v.w);
   } else {
-    const double2 a = *reinterpret_cast<const double2*>(base + off);
-    const double2 b = *reinterpret_cast<const double2*>(base + off + 2);
+    const double2 a = vat<double2>(base, off);
+    const double2 b = vat<double2>(base, off + 2);
     r[0] = T(a.x);
     r[1] = T(a.y);
     r[2] = T(b.x);
@@ -67,10 +84,10 @@ template <typename TD, typename T>
 __device__ __forceinline__ void st4(TD* __restrict__ base, int off, const T (&r)[4])
 {
   if constexpr (sizeof(TD) == 4) {
-    *reinterpret_cast<float4*>(base + off) = make_float4(float(r[0]), float(r[1]), float(r[2]), float(r[3]));
+    vat<float4>(base, off) = make_float4(float(r[0]), float(r[1]), float(r[2]), float(r[3]));
   } else {
-    *reinterpret_cast<double2*>(base + off) = make_double2(double(r[0]), double(r[1]));
-    *reinterpret_cast<double2*>(base + off + 2) = make_double2(double(r[2]), double(r[3]));
+    vat<double2>(base, off) = make_double2(double(r[0]), double(r[1]));
+    vat<double2>(base, off + 2) = make_double2(double(r[2]), double(r[3]));
   }
 }
 
@@ -93,8 +110,8 @@ struct C3U {
   }
   __device__ __forceinline__ T one(int q, int x, int y, int z) const
   {
-    T v = u[q * NC + cidx(x, y, z)];
-    if (INTERP) v = T(v + T(uc[parent[q] * NC + coarse(porth[q], x, y, z)]));
+    T v = at(u, q * NC + cidx(x, y, z));
+    if (INTERP) v = T(v + T(at(uc, parent[q] * NC + coarse(porth[q], x, y, z))));
     return v;
   }
   // cells x0 .. x0 + 3 of row (y, z), x0 a multiple of 4
@@ -106,7 +123,7 @@ struct C3U {
       const TC* b = uc + parent[q] * NC;
       if (o >= 0) {
         const int k = coarse(o, x0, y, z);
-        const T a0 = T(b[k]), a1 = T(b[k + 1]);
+        const T a0 = T(at(b, k)), a1 = T(at(b, k + 1));
         r[0] = T(r[0] + a0);
         r[1] = T(r[1] + a0);
         r[2] = T(r[2] + a1);
@@ -191,7 +208,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
       const TC* b = uc + qpar[s] * NC;
       if (o >= 0) {
         const int k = U.coarse(o, x0, y, z);
-        const T a0 = T(b[k]), a1 = T(b[k + 1]);
+        const T a0 = T(at(b, k)), a1 = T(at(b, k + 1));
         r[0] = T(r[0] + a0);
         r[1] = T(r[1] + a0);
         r[2] = T(r[2] + a1);
@@ -219,8 +236,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
   // cell x = (side ? N - 1 : 0) of row (c, z) of patch qq from uin's face columns, u + P uc
   // when interpolating (par, o: qq's parent and orthant): the value row() gives for it
   auto xval = [&](int qq, int side, int par, int o, int z) -> T {
-    T v = xin[xci(qq, side, c, z)];
-    if (MODE == 2) v = T(v + T(uc[par * NC + U.coarse(o, side ? N - 1 : 0, c, z)]));
+    T v = at(xin, xci(qq, side, c, z));
+    if (MODE == 2) v = T(v + T(at(uc, par * NC + U.coarse(o, side ? N - 1 : 0, c, z))));
     return v;
   };
 
@@ -345,14 +362,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       v[r] = TO(u2[r] * scale);
-      out[p * NC + cidx(c, Mf<T>::row(r, g), z)] = v[r];
+      at(out, p * NC + cidx(c, Mf<T>::row(r, g), z)) = v[r];
     }
     if (XC && (c == 0 || c == N - 1)) {
       if constexpr (sizeof(T) == 4) {
         st4(xout, xci(p, c == 0 ? 0 : 1, 4 * g, z), v);  // FP32 layout: rows 4g .. 4g + 3
       } else {
 #pragma unroll
-        for (int r = 0; r < 4; r++) xout[xci(p, c == 0 ? 0 : 1, Mf<T>::row(r, g), z)] = v[r];
+        for (int r = 0; r < 4; r++) at(xout, xci(p, c == 0 ? 0 : 1, Mf<T>::row(r, g), z)) = v[r];
       }
     }
   }
@@ -378,8 +395,8 @@ __device__ __forceinline__ void star_slice(const TU* __restrict__ u, const TU* _
   }
   // x faces: the neighbour's facing column, from u's face columns when it has them
   if (g == 0)
-    xm = q[0] >= 0 ? T(xu ? xu[xci(q[0], 1, c, z)] : u[q[0] * NC + cidx(N - 1, c, z)]) : bc * cur[0];
-  if (g == 3) xp = q[1] >= 0 ? T(xu ? xu[xci(q[1], 0, c, z)] : u[q[1] * NC + cidx(0, c, z)]) : bc * cur[3];
+    xm = q[0] >= 0 ? T(xu ? at(xu, xci(q[0], 1, c, z)) : at(u, q[0] * NC + cidx(N - 1, c, z))) : bc * cur[0];
+  if (g == 3) xp = q[1] >= 0 ? T(xu ? at(xu, xci(q[1], 0, c, z)) : at(u, q[1] * NC + cidx(0, c, z))) : bc * cur[3];
   if (c == 0) {
     if (q[2] >= 0) ld4(u, q[2] * NC + cidx(4 * g, N - 1, z), ym);
     else
@@ -934,9 +951,9 @@ __global__ void __launch_bounds__(256) c3_face_interp_kernel(const T* __restrict
       st4(out, p * NC + cidx(4 * g, c, z), r);
     }
     if (g == 0 || g == 3) {
-      T v = xu[xci(p, side, c, z)];
-      v = T(v + T(uc[cp * NC + U.coarse(o, side ? N - 1 : 0, c, z)]));
-      xout[xci(p, side, c, z)] = v;
+      T v = at(xu, xci(p, side, c, z));
+      v = T(v + T(at(uc, cp * NC + U.coarse(o, side ? N - 1 : 0, c, z))));
+      at(xout, xci(p, side, c, z)) = v;
     }
   }
 }
