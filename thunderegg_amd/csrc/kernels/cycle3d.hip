@@ -60,11 +60,27 @@ __device__ __forceinline__ V& vat(TS* base, int off)
   return *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + (unsigned)off * (unsigned)sizeof(TS));
 }
 
+// Streaming (nontemporal) access: a load or store marked nontemporal goes to HBM without
+// keeping the line in the caches (measured on this part: 7.0 TB/s for a read-only stream
+// against 6.3 TB/s with plain loads, 6.2 against 5.7 TB/s for a copy).  For values that no
+// other wave of the launch reads again: the own cells of a vector that is read once, never
+// the cells that neighbours read as face values.  FP32 only: a lane's four FP64 values are
+// two 16-byte accesses, each instruction covering every other 16 bytes of a row, and in
+// that form the streaming hint measured slower (the FP64 forms below are plain accesses).
+typedef float c3f4 __attribute__((ext_vector_type(4)));
+
 // four consecutive values starting at an offset that is a multiple of 4, converted to T
-template <typename T, typename TS>
+// (NT: a streaming load of FP32 values)
+template <bool NT = false, typename T, typename TS>
 __device__ __forceinline__ void ld4(const TS* __restrict__ base, int off, T (&r)[4])
 {
-  if constexpr (sizeof(TS) == 4) {
+  if constexpr (NT && sizeof(TS) == 4) {
+    const c3f4 v = __builtin_nontemporal_load(&vat<c3f4>(base, off));
+    r[0] = T(v.x);
+    r[1] = T(v.y);
+    r[2] = T(v.z);
+    r[3] = T(v.w);
+  } else if constexpr (sizeof(TS) == 4) {
     const float4 v = vat<float4>(base, off);
     r[0] = T(v.x);
     r[1] = T(v.y);
@@ -80,10 +96,13 @@ __device__ __forceinline__ void ld4(const TS* __restrict__ base, int off, T (&r)
   }
 }
 
-template <typename TD, typename T>
+// (NT: a streaming store of FP32 values)
+template <bool NT = false, typename TD, typename T>
 __device__ __forceinline__ void st4(TD* __restrict__ base, int off, const T (&r)[4])
 {
-  if constexpr (sizeof(TD) == 4) {
+  if constexpr (NT && sizeof(TD) == 4) {
+    __builtin_nontemporal_store(c3f4{ float(r[0]), float(r[1]), float(r[2]), float(r[3]) }, &vat<c3f4>(base, off));
+  } else if constexpr (sizeof(TD) == 4) {
     vat<float4>(base, off) = make_float4(float(r[0]), float(r[1]), float(r[2]), float(r[3]));
   } else {
     vat<double2>(base, off) = make_double2(double(r[0]), double(r[1]));
@@ -248,7 +267,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
   T Fa[FA ? 4 : 1][4];
   if (FA) {
 #pragma unroll
-    for (int i = 0; i < (FA ? 4 : 1); i++) ld4(f, p * NC + cidx(4 * g, c, w + 4 * i), Fa[i]);
+    for (int i = 0; i < (FA ? 4 : 1); i++) ld4<true>(f, p * NC + cidx(4 * g, c, w + 4 * i), Fa[i]);
   }
 #pragma unroll
   for (int i0 = 0; i0 < 4; i0 += SB) {
@@ -261,7 +280,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
 #pragma unroll
         for (int k = 0; k < 4; k++) F[j][k] = Fa[FA ? i0 + j : 0][k];
       } else {
-        ld4(f, p * NC + cidx(4 * g, c, z), F[j]);
+        ld4<true>(f, p * NC + cidx(4 * g, c, z), F[j]);
       }
       if (MODE != 0) {
         const bool onz = (z == 0 && has[4]) || (z == N - 1 && has[5]);
@@ -362,7 +381,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       v[r] = TO(u2[r] * scale);
-      at(out, p * NC + cidx(c, Mf<T>::row(r, g), z)) = v[r];
+      if constexpr (sizeof(TO) == 4) __builtin_nontemporal_store(v[r], &at(out, p * NC + cidx(c, Mf<T>::row(r, g), z)));
+      else at(out, p * NC + cidx(c, Mf<T>::row(r, g), z)) = v[r];
     }
     if (XC && (c == 0 || c == N - 1)) {
       if constexpr (sizeof(T) == 4) {
@@ -465,14 +485,14 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
     if (z < N - 1) ld4(u, base + 256 * (z + 1) + ro, nxt);
     else z_ghost(u, q[5], 0, ro, bc, cur, nxt);
     T fv[4];
-    ld4(f, base + 256 * z + ro, fv);
+    ld4<true>(f, base + 256 * z + ro, fv);
     T acc[4];
     star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
     T r[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) r[k] = fv[k] - acc[k];
     if (o < 0) {  // copyToParent (GMG/LinearRestrictor.h:138-155)
-      st4(fc, cp * NC + 256 * z + ro, r);
+      st4<true>(fc, cp * NC + 256 * z + ro, r);
     } else {      // restrictToCoarserParent (:101-128): coarse[(i + s) / 2] += fine[i] / 2^D
       const T sx0 = r[0] * wgt + r[1] * wgt, sx1 = r[2] * wgt + r[3] * wgt;
       const T sy0 = sx0 + __shfl(sx0, (lane + 1) & 63, 64);
@@ -736,7 +756,7 @@ __global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict_
     auto load = [&](int z, double(&s0)[4], double(&s1)[4], double(&s2)[4], double(&s3)[4]) {
       const int o = p * NC + 256 * z + ro;
       if (MODE == BICG_XR) {
-        ld4((const TM*)a.mp, o, s0);
+        ld4<true>((const TM*)a.mp, o, s0);
         ld4(a.x, o, s1);
         ld4(a.sv, o, s2);
         ld4(a.rhat, o, s3);
@@ -769,7 +789,7 @@ __global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict_
           sa += sk * sk;
         }
         st4(a.sv, o, sv);
-        if (a.w32) st4(a.w32, o, sv);
+        if (a.w32) st4<true>(a.w32, o, sv);
       } else if (MODE == BICG_XR) {  // c0: M p, c1: x, c2: s, c3: rhat
         double asv[4], xv[4], rv[4];
         wm.step(z, q, ihx, ihy, ihz, asv);
@@ -796,7 +816,7 @@ __global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict_
           pv[k] = v * beta + c1[k];
         }
         st4(a.p, o, pv);
-        if (a.w32) st4(a.w32, o, pv);
+        if (a.w32) st4<true>(a.w32, o, pv);
       }
       if (WP) wp.next();
       if (PF) {
@@ -906,7 +926,7 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
       st4(a.r, o, rv);
       st4(a.rhat, o, rv);
       st4(a.p, o, rv);
-      if (a.w32) st4(a.w32, o, rv);
+      if (a.w32) st4<true>(a.w32, o, rv);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         prv[k] = cur[k];
