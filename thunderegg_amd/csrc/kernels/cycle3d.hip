@@ -133,10 +133,11 @@ struct C3U {
     if (INTERP) v = T(v + T(at(uc, parent[q] * NC + coarse(porth[q], x, y, z))));
     return v;
   }
-  // cells x0 .. x0 + 3 of row (y, z), x0 a multiple of 4
+  // cells x0 .. x0 + 3 of row (y, z), x0 a multiple of 4 (NT: u's cells by streaming loads)
+  template <bool NT = false>
   __device__ __forceinline__ void row4(int q, int x0, int y, int z, T (&r)[4]) const
   {
-    ld4(u, q * NC + cidx(x0, y, z), r);
+    ld4<NT>(u, q * NC + cidx(x0, y, z), r);
     if (INTERP) {
       const int o = porth[q];
       const TC* b = uc + parent[q] * NC;
@@ -967,8 +968,8 @@ __global__ void __launch_bounds__(256) c3_face_interp_kernel(const T* __restrict
   for (int z = 0; z < N; z++) {
     if (z == 0 || z == N - 1 || c == 0 || c == N - 1) {
       T r[4];
-      U.row4(p, 4 * g, c, z, r);
-      st4(out, p * NC + cidx(4 * g, c, z), r);
+      U.template row4<true>(p, 4 * g, c, z, r);
+      st4<true>(out, p * NC + cidx(4 * g, c, z), r);
     }
     if (g == 0 || g == 3) {
       T v = at(xu, xci(p, side, c, z));
