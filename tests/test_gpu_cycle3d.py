@@ -209,6 +209,34 @@ def test_compact3d_xr_update_computes_operator(monkeypatch, prec):
         assert np.sqrt((r * r).sum() / (mesh.interior(b, 3) ** 2).sum()) <= 1.5e-10
 
 
+def test_compact3d_fp32_shadow_residual(monkeypatch):
+    """Mixed precision keeps BiCGStab's shadow residual rhat as an FP32 copy of r0 (rho_0 and
+    every rhat.ap, rhat.r formed from the same widened values): the same iteration with the
+    fixed vector fl32(r0) in place of r0.  Against the FP64 rhat (TE_NO_RHAT32=1): equal
+    iteration counts, solutions to 1e-8, both at the tolerance in the true residual; with a
+    zero and a random initial guess, both early-exit variants."""
+    _print_case("compact3d fp32 rhat")
+    levels = _levels(3)
+    b = _rhs_sin(levels)
+    od = O.Domain(3, NS, levels[0])
+    x0 = random_field(mesh.padded_shape(len(levels[0]), 3, NS), 7)
+    out = []
+    for nr in ("0", "1"):
+        monkeypatch.setenv("TE_NO_RHAT32", nr)
+        g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=R.F32, pre_sweeps=2,
+                       post_sweeps=2)
+        out.append([g.solve(b, tol=1e-10, precond=pc) for pc in (2, True)] +
+                   [g.solve(b, x0=x0, tol=1e-10, precond=2)])
+    for k, ((xa, ia, ha), (xb, ib, hb)) in enumerate(zip(out[0], out[1])):
+        assert ia == ib, k
+        assert np.allclose(ha[:ia + 1], hb[:ib + 1], rtol=1e-3, atol=0), k
+        assert relerr(mesh.interior(xa, 3), mesh.interior(xb, 3)) <= 1e-8, k
+    for x, _, _ in (out[0][0], out[0][2], out[1][0]):
+        ax = O.poisson_apply(od, x.copy(), kind=R.TRILINEAR)
+        r = mesh.interior(b - ax, 3)
+        assert np.sqrt((r * r).sum() / (mesh.interior(b, 3) ** 2).sum()) <= 1.5e-10
+
+
 def test_compact3d_bicgstab_initial_guess_matches_oracle(monkeypatch):
     """A nonzero initial guess: the setup forms r = b - A x from the caller's padded b and x
     in one launch (face ghosts from the neighbours' cells of x or the physical condition);

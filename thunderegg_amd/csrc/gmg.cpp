@@ -510,7 +510,7 @@ void free_gmg(te_gmg* g)
   }
   for (void* v : { (void*)g->c_r, (void*)g->c_rhat, (void*)g->c_p, (void*)g->c_ap, (void*)g->c_as, (void*)g->c_s,
                    (void*)g->c_x,
-                   g->c_mp, g->c_ms, (void*)g->c_p32, (void*)g->c_s32, g->c_tmp })
+                   g->c_mp, g->c_ms, (void*)g->c_p32, (void*)g->c_s32, (void*)g->c_rhat32, g->c_tmp })
     (void)hipFree(v);
   if (g->host_scal) (void)hipHostFree(g->host_scal);
   if (g->mirror_h) (void)hipHostFree(g->mirror_h);
@@ -592,6 +592,8 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
     g->xcol = !(nx && nx[0] == '1');
     const char* nxr = std::getenv("TE_NO_XRFLY");
     g->xrfly = !(nxr && nxr[0] == '1');
+    const char* nr32 = std::getenv("TE_NO_RHAT32");
+    g->rhat32 = !(nr32 && nr32[0] == '1');
     const char* nfi = std::getenv("TE_NO_FACEINTERP");
     g->finterp = !(nfi && nfi[0] == '1');
   }
@@ -775,7 +777,7 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
         if (e == hipSuccess) e = hipMemset(*v, 0, em * ncx);
       }
       if (g->L[0].dt == TE_F32) {
-        for (float** v : { &g->c_p32, &g->c_s32 }) {
+        for (float** v : { &g->c_p32, &g->c_s32, &g->c_rhat32 }) {
           if (e == hipSuccess) e = hipMalloc(v, sizeof(float) * nc);
           if (e == hipSuccess) e = hipMemset(*v, 0, sizeof(float) * nc);
         }
@@ -916,6 +918,11 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   const void* c3_xms = c3_xc ? xcol_of(g, c3_ms, d->np, c3m) : nullptr;
   // the updates form A M p from M p (M p is a buffer of its own when preconditioned)
   const bool apf = g->xrfly && use_precond;
+  // mixed precision: rhat as an FP32 copy of r0.  BiCGStab uses rhat only in the inner
+  // products rhat.ap and rhat.r, both formed from the same stored (widened) values, so this
+  // is the same iteration with the shadow residual fl32(r0) in place of r0 (BiCGStab.h:132-133
+  // sets rhat = r0; any fixed rhat with rhat.r0 != 0 serves); the recurrences stay FP64.
+  const bool c3_r32 = c3_32 && g->xrfly && g->rhat32 && g->c_rhat32;
   auto iteration_c3 = [&]() -> int {
     C3Bicg a{};
     a.scal = g->scal;
@@ -934,7 +941,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     // rhat.ap (A M p itself is formed again by the updates that use it, launch_c3_upd, where
     // M p is apart from p; else stored)
     TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_mp, apf ? nullptr : g->c_ap, g->c_rhat, g->pa, nullptr, nb, s,
-                             c3_xmp));
+                             c3_xmp, c3_r32 ? g->c_rhat32 : nullptr));
     // s = r - alpha ap, alpha folded from the rhat.ap partials; s.s partials into pc
     a.w32 = c3_32 ? g->c_s32 : nullptr;
     a.pa = g->pc;
@@ -956,10 +963,12 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.pb = g->pb;
     a.dir = use_precond == 2 ? (const void*)c3_mp : (const void*)g->c_p;
     a.dir_m = use_precond == 2 ? 1 : 0;
+    a.rh32 = c3_r32 ? g->c_rhat32 : nullptr;
     if (g->xrfly) TE_RC(launch_c3_upd(BICG_XR, d, g->o.neumann, a, apf, c3_xmp, c3_xms, nb, s));
     else TE_RC(launch_c3_bicg(BICG_XR, d, a, nb, s));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
     a.dir = nullptr;
+    a.rh32 = nullptr;
     a.w32 = c3_32 ? g->c_p32 : nullptr;
     if (g->xrfly) return launch_c3_upd(BICG_P, d, g->o.neumann, a, apf, c3_xmp, nullptr, nb, s);
     return launch_c3_bicg(BICG_P, d, a, nb, s);
@@ -1037,6 +1046,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.rhat = g->c_rhat;
     a.p = g->c_p;
     a.w32 = c3_32 ? g->c_p32 : nullptr;
+    a.rh32 = c3_r32 ? g->c_rhat32 : nullptr;  // c3_r32 implies c3_init
+    a.pr = g->pc;
     a.pa = g->pa;
     a.pb = g->pb;
     a.dtm = TE_F64;
@@ -1046,7 +1057,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     TE_RC(launch_bicg(BICG_INIT, d, g->scal, g->rhat, g->p, g->resid, b, nullptr, nullptr, nullptr, g->pa, g->pb, s,
                       TE_F64, m32 ? g->p32 : nullptr));
   }
-  TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s, tol));
+  TE_RC(launch_bicg_finalize(FIN_INIT, g->pa, g->pb, nb, g->scal, s, tol, nullptr, c3_r32 ? g->pc : nullptr));
   // the first V-cycle's zero-guess sweep on the finest level (the later ones run in the p
   // updates)
   if (l0f) TE_RC(launch_dft_mfma_fused(L0.plan, TE_F32, TE_F32, TE_F32, g->p32, u0, u0, true, 0, TE_F64, nullptr,
@@ -1248,8 +1259,10 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
                                         g->C3[1].f, s, xu);
       case TE_K_OPERATOR_P: {
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
+        const bool r32 = L0.dt == TE_F32 && g->xrfly && g->rhat32;  // as the mixed-precision iteration
         return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_mp, g->xrfly ? nullptr : g->c_ap, g->c_rhat, g->pa,
-                                  nullptr, nb, s, xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr);
+                                  nullptr, nb, s, xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr,
+                                  r32 ? g->c_rhat32 : nullptr);
       }
       case TE_K_OPERATOR: {
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
@@ -1283,6 +1296,7 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
           a.w32 = L0.dt == TE_F32 ? g->c_p32 : nullptr;
         }
         if (!g->xrfly) return launch_c3_bicg(mode, L0.d, a, nb, s);
+        if (mode == BICG_XR && L0.dt == TE_F32 && g->rhat32) a.rh32 = g->c_rhat32;
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
         return launch_c3_upd(mode, L0.d, g->o.neumann, a, true, xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr,
                              xc ? xcol_of(g, g->c_ms, np, L0.dt) : nullptr, nb, s);

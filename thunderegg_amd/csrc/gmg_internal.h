@@ -123,6 +123,9 @@ struct te_gmg {
   // from the padded b and x in one launch (launch_c3_init); TE_NO_XRFLY=1 stores the
   // products and reads them back, and sets up through the padded residual
   bool xrfly = true;
+  // mixed precision on the compact path (FP32 cycle, xrfly): BiCGStab's shadow residual rhat
+  // is kept as an FP32 copy of r0 (c_rhat32); TE_NO_RHAT32=1 keeps it in FP64
+  bool rhat32 = true;
   struct C3Level {
     void* f = nullptr;
     void* u = nullptr;
@@ -141,6 +144,7 @@ struct te_gmg {
   void* c_ms = nullptr;
   float* c_p32 = nullptr;  // FP32 copies of p and s (mixed precision: the cycle's input)
   float* c_s32 = nullptr;
+  float* c_rhat32 = nullptr;  // rhat as FP32 (rhat32)
   void* c_tmp = nullptr;   // the cycle's output for padded callers
   // Batch mode (te_batch): the levels are unions of many independent problems' levels,
   // aligned at their coarsest level, so a problem's finest level can be any of them.  On

@@ -50,9 +50,10 @@ int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int 
 // out = A u (FP64, compact; u compact of type udt, widened as read; out may be nullptr: only
 // the sums) with the per-workgroup partial sums pa = sum w out and, when pb is not null,
 // pb = sum out out; nslots partial slots, those past the launch's workgroups written zero.
-// xu: u's face columns (or nullptr).
+// xu: u's face columns (or nullptr).  w32: an FP32 w used in place of w (u FP32, pb null).
 int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, double* out, const double* w,
-                       double* pa, double* pb, int nslots, hipStream_t s, const void* xu = nullptr);
+                       double* pa, double* pb, int nslots, hipStream_t s, const void* xu = nullptr,
+                       const float* w32 = nullptr);
 // BiCGStab vector updates (Iterative/BiCGStab.h:111-203) on compact vectors, the modes and
 // scalar slots of krylov.h.  b and xpad are the caller's padded FP64 vectors; x is a compact
 // FP64 copy of the caller's x that the iterations update (written back at the end).
@@ -83,11 +84,15 @@ struct C3Bicg {
   double* pb;
   const double* pfold;
   int nfold;
+  float* rh32;       // FP32 rhat in place of rhat, or null: INIT (launch_c3_init) writes it and
+                     // the partial sums rhat.r into pr; XR (launch_c3_upd, FP32 M p and M s) reads it
+  double* pr;
 };
 int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hipStream_t s);
 // BICG_INIT from the caller's padded b and x alone: r = b - A x formed in the launch (the face
 // ghosts from the same-level neighbours' cells of x or the physical condition; a.rpad unused),
-// then x, r, rhat, p (+ a.w32) compact and the partial sums b.b, r.r.  One wavefront per patch.
+// then x, r, rhat (or a.rh32), p (+ a.w32) compact and the partial sums b.b, r.r (+ rhat.r into
+// a.pr with a.rh32).  One wavefront per patch.
 int launch_c3_init(const te_domain* d, int neumann, const C3Bicg& a, int nslots, hipStream_t s);
 // BICG_S, BICG_XR and BICG_P with the operator products formed in the launch, one wavefront
 // per patch: A M s (XR) from a.ms, and with apf A M p (S, P) from a.mp (then a.ap is unused:

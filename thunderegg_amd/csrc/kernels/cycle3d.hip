@@ -517,9 +517,9 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
 }
 
 // out = A u with the partial sums w.out (and out.out); persistent wavefronts over patches
-template <typename TU, bool SQ>
+template <typename TU, bool SQ, typename TW>
 __global__ void __launch_bounds__(256) c3_star_dot_kernel(const DevPatch* __restrict__ dp, const TU* __restrict__ u,
-                                                          double* __restrict__ out, const double* __restrict__ w,
+                                                          double* __restrict__ out, const TW* __restrict__ w,
                                                           double* __restrict__ pa, double* __restrict__ pb, int np,
                                                           int nslots, int neumann, const TU* __restrict__ xu)
 {
@@ -540,8 +540,8 @@ __global__ void __launch_bounds__(256) c3_star_dot_kernel(const DevPatch* __rest
     for (int z = 0; z < N; z++) {
       if (z < N - 1) ld4(u, base + 256 * (z + 1) + ro, nxt);
       else z_ghost(u, q[5], 0, ro, bc, cur, nxt);
-      double wv4[4];
-      ld4(w, base + 256 * z + ro, wv4);
+      double wv4[4];  // w: FP64, or FP32 widened as read (a streaming load)
+      ld4<sizeof(TW) == 4>(w, base + 256 * z + ro, wv4);
       double acc[4];
       star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
       if (out) st4(out, base + 256 * z + ro, acc);
@@ -703,7 +703,8 @@ struct StarWalk {
 // APF (s and p updates): A M p from M p here (needs M p apart from p: the preconditioned
 // iteration); else read from a.ap.  One wavefront per patch, persistent over the patches;
 // xmp, xms: the face columns of M p and M s (or nullptr).
-template <int MODE, typename TM, bool APF>
+// R32 (x / r update): rhat from a.rh32, its FP32 copy, widened as read.
+template <int MODE, typename TM, bool APF, bool R32>
 __global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
                                                      const TM* __restrict__ xmp, const TM* __restrict__ xms, int np,
                                                      int nslots, int neumann)
@@ -760,7 +761,8 @@ __global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict_
         ld4<true>((const TM*)a.mp, o, s0);
         ld4(a.x, o, s1);
         ld4(a.sv, o, s2);
-        ld4(a.rhat, o, s3);
+        if (R32) ld4<true>(a.rh32, o, s3);
+        else ld4(a.rhat, o, s3);
       } else {
         ld4(MODE == BICG_S ? a.r : a.p, o, s0);
         if (MODE == BICG_P) ld4(a.r, o, s1);
@@ -864,7 +866,7 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
 #pragma unroll
     for (int k = 0; k < 4; k++) r[k] = v[o + k];
   };
-  double sa = 0.0, sb = 0.0;
+  double sa = 0.0, sb = 0.0, sr = 0.0;
   for (int p = blockIdx.x * 4 + wv; p < np; p += gridDim.x * 4) {
     const DevPatch& P = dp[p];
     int q[6];
@@ -921,11 +923,13 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
         rv[k] = bv[k] - acc;
         sa += bv[k] * bv[k];
         sb += rv[k] * rv[k];
+        if (a.rh32) sr += double(float(rv[k])) * rv[k];  // rhat.r with the FP32 rhat
       }
       const int o = p * NC + cidx(4 * g, c, z);
       st4(a.x, o, cur);
       st4(a.r, o, rv);
-      st4(a.rhat, o, rv);
+      if (a.rh32) st4<true>(a.rh32, o, rv);
+      else st4(a.rhat, o, rv);
       st4(a.p, o, rv);
       if (a.w32) st4<true>(a.w32, o, rv);
 #pragma unroll
@@ -939,10 +943,15 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
   if (threadIdx.x == 0) a.pa[blockIdx.x] = ra;
   const double rb = block_reduce<RED_SUM>(sb);
   if (threadIdx.x == 0) a.pb[blockIdx.x] = rb;
+  if (a.rh32) {
+    const double rr = block_reduce<RED_SUM>(sr);
+    if (threadIdx.x == 0) a.pr[blockIdx.x] = rr;
+  }
   if (blockIdx.x == 0)
     for (int k = (int)gridDim.x + threadIdx.x; k < nslots; k += blockDim.x) {
       a.pa[k] = 0.0;
       a.pb[k] = 0.0;
+      if (a.rh32) a.pr[k] = 0.0;
     }
 }
 
@@ -1128,7 +1137,7 @@ int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int 
 }
 
 int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, double* out, const double* w,
-                       double* pa, double* pb, int nslots, hipStream_t s, const void* xu)
+                       double* pa, double* pb, int nslots, hipStream_t s, const void* xu, const float* w32)
 {
   if (nslots < 1 || nslots > kFoldMax) {
     set_error("internal: bad partial count");
@@ -1137,15 +1146,21 @@ int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, 
   int grid = (d->np + 3) / 4;
   if (grid > nslots) grid = nslots;
   if (grid < 1) grid = 1;
-#define TE_C3D(TU_, Q_)                                                                                         \
-  hipLaunchKernelGGL((c3_star_dot_kernel<TU_, Q_>), dim3(grid), dim3(256), 0, s, d->d_patches, (const TU_*)u, out, \
-                     w, pa, pb, d->np, nslots, neumann, (const TU_*)xu)
+#define TE_C3D(TU_, Q_, TW_, W_)                                                                                \
+  hipLaunchKernelGGL((c3_star_dot_kernel<TU_, Q_, TW_>), dim3(grid), dim3(256), 0, s, d->d_patches, (const TU_*)u, \
+                     out, W_, pa, pb, d->np, nslots, neumann, (const TU_*)xu)
+  if (w32 && (udt != TE_F32 || pb)) {
+    set_error("internal: an FP32 weight vector goes with FP32 M p and one sum");
+    return TE_ERR_ARG;
+  }
   if (udt == TE_F64) {
-    if (pb) TE_C3D(double, true);
-    else TE_C3D(double, false);
+    if (pb) TE_C3D(double, true, double, w);
+    else TE_C3D(double, false, double, w);
+  } else if (w32) {
+    TE_C3D(float, false, float, w32);
   } else {
-    if (pb) TE_C3D(float, true);
-    else TE_C3D(float, false);
+    if (pb) TE_C3D(float, true, double, w);
+    else TE_C3D(float, false, double, w);
   }
 #undef TE_C3D
   TE_HIP(hipGetLastError());
@@ -1192,9 +1207,10 @@ int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bo
   int grid = (d->np + 3) / 4;
   if (grid > nslots) grid = nslots;
   if (grid < 1) grid = 1;
-#define TE_C3U(M_, TM_, APF_)                                                                                    \
-  hipLaunchKernelGGL((c3_upd_kernel<M_, TM_, APF_>), dim3(grid), dim3(256), 0, s, d->d_patches, a,                \
+#define TE_C3U4(M_, TM_, APF_, R32_)                                                                             \
+  hipLaunchKernelGGL((c3_upd_kernel<M_, TM_, APF_, R32_>), dim3(grid), dim3(256), 0, s, d->d_patches, a,          \
                      (const TM_*)xmp, (const TM_*)xms, d->np, nslots, neumann)
+#define TE_C3U(M_, TM_, APF_) TE_C3U4(M_, TM_, APF_, false)
 #define TE_C3UM(M_)                                      \
   do {                                                   \
     if (a.dtm == TE_F32) {                               \
@@ -1208,13 +1224,19 @@ int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bo
   if (mode == BICG_S) {
     TE_C3UM(BICG_S);
   } else if (mode == BICG_XR) {  // forms A M s only
-    if (a.dtm == TE_F32) TE_C3U(BICG_XR, float, false);
+    if (a.rh32 && a.dtm != TE_F32) {
+      set_error("internal: the FP32 rhat goes with FP32 M p and M s");
+      return TE_ERR_ARG;
+    }
+    if (a.rh32) TE_C3U4(BICG_XR, float, false, true);
+    else if (a.dtm == TE_F32) TE_C3U(BICG_XR, float, false);
     else TE_C3U(BICG_XR, double, false);
   } else {
     TE_C3UM(BICG_P);
   }
 #undef TE_C3UM
 #undef TE_C3U
+#undef TE_C3U4
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

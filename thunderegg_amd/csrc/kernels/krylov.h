@@ -83,6 +83,8 @@ int launch_bicg(int mode, const te_domain* d, double* scal, double* v0, double* 
                 hipStream_t s, int dtm = TE_F64, float* w32 = nullptr, const void* xp_dir = nullptr,
                 int xp_dir_m = 0, const double* pfold = nullptr, int nfold = 0, const BatchMap* bm = nullptr);
 // tol: the relative tolerance, stored by FIN_INIT for the s-norm test of FIN_SNORM.
+// pc: FIN_SOMEGA's s.s partials; for FIN_INIT, when given, the rhat.r partials (rhat not r
+// itself: rho from them instead of r.r).
 // mirror (FIN_RHO): device pointer to host-coherent memory of SC_COUNT doubles; FIN_RHO
 // copies the scalars there and then bumps SC_SEQ, so the host can read an iteration's
 // outcome while the rest of the iteration is still running.

@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double
   const double state = scal[SC_STATE];
   const double* const parts[3] = { pa, pb, pc };
   double r[3] = { 0.0, 0.0, 0.0 };
-  if (mode == FIN_SOMEGA) fold_n<3>(parts, n, r);
+  if (mode == FIN_SOMEGA || (mode == FIN_INIT && pc)) fold_n<3>(parts, n, r);
   else if (mode == FIN_ALPHA || mode == FIN_SNORM) fold_n<1>(parts, n, r);
   else fold_n<2>(parts, n, r);
   const bool run = mode == FIN_INIT || state == ST_RUN;
@@ -269,9 +269,9 @@ __global__ void __launch_bounds__(kThreads) bicg_finalize(int mode, const double
   const double A = r[0], B = r[1], C = r[2];
   if (threadIdx.x != 0) return;
   switch (mode) {
-    case FIN_INIT:
+    case FIN_INIT:  // pc: the rhat.r partials when rhat is not r itself (else rho = r.r)
       scal[SC_R0] = sqrt(A);
-      scal[SC_RHO] = B;
+      scal[SC_RHO] = pc ? C : B;
       scal[SC_RNORM] = sqrt(B);
       scal[SC_TOL] = tol;
       scal[SC_STATE] = ST_RUN;
