@@ -142,10 +142,11 @@ def kernel_rooflines_3d(g, P, its, reps=20):
       sweep_interp    + the coarse boundary cells   s (2N + B + Bc)
       resid_restrict  f, u, coarse f                s (2N + N/8)
       operator        M s (FP32), s (FP64)          4N + 8N (+ 8N: A M s stored, TE_NO_XRFLY=1)
-      operator_p      M p (FP32), rhat (FP64)       4N + 8N (+ 8N: A M p stored, TE_NO_XRFLY=1)
-      update_xr       x read + written, r written; M p, M s (FP32); s, rhat (FP64)
-                      8 2 + 8 + 4 2 + 8 2 per cell (TE_NO_XRFLY=1: r read, A M p and A M s
-                      instead of s: + 8 + 8)
+      operator_p      M p (FP32), rhat (FP32)       4N + 4N (TE_NO_RHAT32=1: rhat FP64, + 4N;
+                                                    TE_NO_XRFLY=1: that, and A M p stored, + 8N)
+      update_xr       x read + written, r written; M p, M s (FP32); s (FP64), rhat (FP32)
+                      8 2 + 8 + 4 2 + 8 + 4 per cell (rhat FP64: + 4; TE_NO_XRFLY=1: that, and
+                      r read, A M p and A M s instead of s: + 8 + 8)
       update_s        r, M p (FP32) -> s, s32        8 + 4 + 8 + 4 (M p -> A M p, TE_NO_XRFLY=1: + 4)
       update_p        p, M p (FP32), r -> p, p32     8 + 4 + 8 + 8 + 4 (likewise + 4)
     Flops of the patch solve: six 16 x 16 x 16 transform products per axis pass, 12 n^4."""
@@ -155,6 +156,7 @@ def kernel_rooflines_3d(g, P, its, reps=20):
     fl = np_ * 12 * 16 ** 4
     pre, post = P.pre, P.post
     xrfly = os.environ.get("TE_NO_XRFLY") != "1"  # the updates form A M p and A M s themselves
+    rh = 8 if (not xrfly or os.environ.get("TE_NO_RHAT32") == "1") else 4  # bytes per rhat value
     specs = {
         # name: (bytes, flops, launches per solve)
         "sweep_zero": (np_ * s * 2 * N, fl, 2 * its),
@@ -162,8 +164,8 @@ def kernel_rooflines_3d(g, P, its, reps=20):
         "sweep_interp": (np_ * s * (2 * N + B + Bc), fl, 2 * its),
         "resid_restrict": (np_ * s * (2 * N + N // 8), None, 2 * its),
         "operator": (np_ * N * (4 + 8 + (0 if xrfly else 8)), None, its),
-        "operator_p": (np_ * N * (4 + 8 + (0 if xrfly else 8)), None, its),
-        "update_xr": (np_ * N * (8 * 2 + 8 + 4 * 2 + 8 * 2 + (0 if xrfly else 16)), None, its),
+        "operator_p": (np_ * N * (4 + rh + (0 if xrfly else 8)), None, its),
+        "update_xr": (np_ * N * (8 * 2 + 8 + 4 * 2 + 8 + rh + (0 if xrfly else 16)), None, its),
         "update_s": (np_ * N * (8 + 4 + 8 + 4 + (0 if xrfly else 4)), None, its),
         "update_p": (np_ * N * (8 + 4 + 8 + 8 + 4 + (0 if xrfly else 4)), None, its),
     }
