@@ -443,6 +443,56 @@ __device__ __forceinline__ void star_slice(const TU* __restrict__ u, const TU* _
   }
 }
 
+// The face values star_slice reads from the neighbours for slice z, loaded ahead of their use
+// (raw, of u's type): xr the facing x-face cell of lanes g = 0 (q[0]) and g = 3 (q[1]), yr the
+// facing y-face row of lanes c = 0 (q[2]) and c = N - 1 (q[3]); lanes that take no face value
+// (or face a physical side) load nothing.
+template <typename TU>
+__device__ __forceinline__ void star_faces(const TU* __restrict__ u, const TU* __restrict__ xu, const int (&q)[6],
+                                           int c, int g, int z, TU& xr, TU (&yr)[4])
+{
+  if (g == 0 && q[0] >= 0) xr = xu ? at(xu, xci(q[0], 1, c, z)) : at(u, q[0] * NC + cidx(N - 1, c, z));
+  if (g == 3 && q[1] >= 0) xr = xu ? at(xu, xci(q[1], 0, c, z)) : at(u, q[1] * NC + cidx(0, c, z));
+  if (c == 0 && q[2] >= 0) ld4(u, q[2] * NC + cidx(4 * g, N - 1, z), yr);
+  if (c == N - 1 && q[3] >= 0) ld4(u, q[3] * NC + cidx(4 * g, 0, z), yr);
+}
+// star_slice with the face values from star_faces (the same arithmetic, so the same bits)
+template <typename T, typename TU>
+__device__ __forceinline__ void star_slice_f(const int (&q)[6], int lane, int c, int g, T bc, T ihx, T ihy, T ihz,
+                                             TU xr, const TU (&yr)[4], const T (&cur)[4], const T (&prv)[4],
+                                             const T (&nxt)[4], T (&acc)[4])
+{
+  T xm = __shfl(cur[3], (lane - 16) & 63, 64);
+  T xp = __shfl(cur[0], (lane + 16) & 63, 64);
+  T ym[4], yp[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    ym[k] = __shfl(cur[k], (lane - 1) & 63, 64);
+    yp[k] = __shfl(cur[k], (lane + 1) & 63, 64);
+  }
+  if (g == 0) xm = q[0] >= 0 ? T(xr) : bc * cur[0];
+  if (g == 3) xp = q[1] >= 0 ? T(xr) : bc * cur[3];
+  if (c == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) ym[k] = q[2] >= 0 ? T(yr[k]) : bc * cur[k];
+  }
+  if (c == N - 1) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) yp[k] = q[3] >= 0 ? T(yr[k]) : bc * cur[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const T mid = cur[k];
+    const T lo = k == 0 ? xm : cur[k - 1];
+    const T hi = k == 3 ? xp : cur[k + 1];
+    T a = T(0);
+    a += (hi - T(2) * mid + lo) * ihx;
+    a += (yp[k] - T(2) * mid + ym[k]) * ihy;
+    a += (nxt[k] - T(2) * mid + prv[k]) * ihz;
+    acc[k] = a;
+  }
+}
+
 // the lane's row of slice z - 1 at z = 0 (bottom ghost) and of slice z + 1 at z = 15 (top)
 template <typename T, typename TU>
 __device__ __forceinline__ void z_ghost(const TU* __restrict__ u, int qz, int zsrc, int ro, T bc, const T (&cur)[4],
@@ -518,7 +568,7 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
 
 // out = A u with the partial sums w.out (and out.out); persistent wavefronts over patches
 template <typename TU, bool SQ, typename TW>
-__global__ void __launch_bounds__(256) c3_star_dot_kernel(const DevPatch* __restrict__ dp, const TU* __restrict__ u,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) c3_star_dot_kernel(const DevPatch* __restrict__ dp, const TU* __restrict__ u,
                                                           double* __restrict__ out, const TW* __restrict__ w,
                                                           double* __restrict__ pa, double* __restrict__ pb, int np,
                                                           int nslots, int neumann, const TU* __restrict__ xu)
@@ -537,21 +587,37 @@ __global__ void __launch_bounds__(256) c3_star_dot_kernel(const DevPatch* __rest
     double cur[4], prv[4], nxt[4];
     ld4(u, base + ro, cur);
     z_ghost(u, q[4], N - 1, ro, bc, cur, prv);
+    // raw loads one slice ahead of their use: u1 (slice z + 1, at z = N - 1 the neighbour's
+    // slice 0), w0 (w of slice z: FP64, or FP32 by a streaming load), x0 / y0 (slice z's
+    // faces); each iteration issues those of the next slice before its arithmetic
+    TU u1[4], u2[4], x0 = TU(0), x1 = TU(0), y0[4], y1[4];
+    TW w0[4], w1[4];
+    ld4(u, base + 256 + ro, u1);
+    ld4<sizeof(TW) == 4>(w, base + ro, w0);
+    star_faces(u, xu, q, c, g, 0, x0, y0);
     for (int z = 0; z < N; z++) {
-      if (z < N - 1) ld4(u, base + 256 * (z + 1) + ro, nxt);
-      else z_ghost(u, q[5], 0, ro, bc, cur, nxt);
-      double wv4[4];  // w: FP64, or FP32 widened as read (a streaming load)
-      ld4<sizeof(TW) == 4>(w, base + 256 * z + ro, wv4);
+      if (z + 2 < N) ld4(u, base + 256 * (z + 2) + ro, u2);
+      else if (z + 2 == N && q[5] >= 0) ld4(u, q[5] * NC + ro, u2);
+      if (z + 1 < N) {
+        ld4<sizeof(TW) == 4>(w, base + 256 * (z + 1) + ro, w1);
+        star_faces(u, xu, q, c, g, z + 1, x1, y1);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) nxt[k] = (z < N - 1 || q[5] >= 0) ? double(u1[k]) : bc * cur[k];
       double acc[4];
-      star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
+      star_slice_f(q, lane, c, g, bc, ihx, ihy, ihz, x0, y0, cur, prv, nxt, acc);
       if (out) st4(out, base + 256 * z + ro, acc);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        sa += wv4[k] * acc[k];  // rhat.dot(ap) / as.dot(s): w first, as the reference
+        sa += double(w0[k]) * acc[k];  // rhat.dot(ap) / as.dot(s): w first, as the reference
         if (SQ) sq += acc[k] * acc[k];
         prv[k] = cur[k];
         cur[k] = nxt[k];
+        u1[k] = u2[k];
+        w0[k] = w1[k];
+        y0[k] = y1[k];
       }
+      x0 = x1;
     }
   }
   const double ra = block_reduce<RED_SUM>(sa);
