@@ -568,7 +568,7 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
 
 // out = A u with the partial sums w.out (and out.out); persistent wavefronts over patches
 template <typename TU, bool SQ, typename TW>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) c3_star_dot_kernel(const DevPatch* __restrict__ dp, const TU* __restrict__ u,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TU) == 4 ? 4 : 1, 8))) c3_star_dot_kernel(const DevPatch* __restrict__ dp, const TU* __restrict__ u,
                                                           double* __restrict__ out, const TW* __restrict__ w,
                                                           double* __restrict__ pa, double* __restrict__ pb, int np,
                                                           int nslots, int neumann, const TU* __restrict__ xu)
@@ -733,17 +733,28 @@ struct StarWalk {
   int base, ro, lane, c, g;
   double bc;
   double cur[4], prv[4], nxt[4];
+  // raw loads one slice ahead of their use (as in c3_star_dot_kernel): u1 slice z + 1 (at
+  // z = N - 1 the neighbour's slice 0), u2 the one after, x0 / y0 and x1 / y1 the faces of
+  // slices z and z + 1
+  TU u1[4], u2[4], x0, x1, y0[4], y1[4];
   __device__ __forceinline__ void init(int p, const int (&q)[6])
   {
     base = p * NC;
     ld4(u, base + ro, cur);
     z_ghost(u, q[4], N - 1, ro, bc, cur, prv);
+    ld4(u, base + 256 + ro, u1);
+    x0 = TU(0);
+    x1 = TU(0);
+    star_faces(u, xu, q, c, g, 0, x0, y0);
   }
   __device__ __forceinline__ void step(int z, const int (&q)[6], double ihx, double ihy, double ihz, double (&acc)[4])
   {
-    if (z < N - 1) ld4(u, base + 256 * (z + 1) + ro, nxt);
-    else z_ghost(u, q[5], 0, ro, bc, cur, nxt);
-    star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
+    if (z + 2 < N) ld4(u, base + 256 * (z + 2) + ro, u2);
+    else if (z + 2 == N && q[5] >= 0) ld4(u, q[5] * NC + ro, u2);
+    if (z + 1 < N) star_faces(u, xu, q, c, g, z + 1, x1, y1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) nxt[k] = (z < N - 1 || q[5] >= 0) ? double(u1[k]) : bc * cur[k];
+    star_slice_f(q, lane, c, g, bc, ihx, ihy, ihz, x0, y0, cur, prv, nxt, acc);
   }
   __device__ __forceinline__ void next()
   {
@@ -751,7 +762,10 @@ struct StarWalk {
     for (int k = 0; k < 4; k++) {
       prv[k] = cur[k];
       cur[k] = nxt[k];
+      u1[k] = u2[k];
+      y0[k] = y1[k];
     }
+    x0 = x1;
   }
 };
 
@@ -771,7 +785,7 @@ struct StarWalk {
 // xmp, xms: the face columns of M p and M s (or nullptr).
 // R32 (x / r update): rhat from a.rh32, its FP32 copy, widened as read.
 template <int MODE, typename TM, bool APF, bool R32>
-__global__ void __launch_bounds__(256) c3_upd_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TM) == 4 ? 4 : 1, 8))) c3_upd_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
                                                      const TM* __restrict__ xmp, const TM* __restrict__ xms, int np,
                                                      int nslots, int neumann)
 {
