@@ -531,14 +531,28 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
   T cur[4], prv[4], nxt[4];
   ld4(u, base + ro, cur);
   z_ghost(u, q[4], N - 1, ro, bc, cur, prv);
+  // loads one slice ahead of their use (as in c3_star_dot_kernel): u1 slice z + 1 (at
+  // z = N - 1 the neighbour's slice 0), u2 the one after, f0 / f1 and the faces x0, y0 /
+  // x1, y1 of slices z and z + 1
+  T u1[4], u2[4], f0[4], f1[4], x0 = T(0), x1 = T(0), y0[4], y1[4];
+  ld4(u, base + 256 + ro, u1);
+  ld4<true>(f, base + ro, f0);
+  star_faces(u, xu, q, c, g, 0, x0, y0);
   T hold0 = T(0), hold1 = T(0);
   for (int z = 0; z < N; z++) {
-    if (z < N - 1) ld4(u, base + 256 * (z + 1) + ro, nxt);
-    else z_ghost(u, q[5], 0, ro, bc, cur, nxt);
-    T fv[4];
-    ld4<true>(f, base + 256 * z + ro, fv);
+    if (z + 2 < N) ld4(u, base + 256 * (z + 2) + ro, u2);
+    else if (z + 2 == N && q[5] >= 0) ld4(u, q[5] * NC + ro, u2);
+    if (z + 1 < N) {
+      ld4<true>(f, base + 256 * (z + 1) + ro, f1);
+      star_faces(u, xu, q, c, g, z + 1, x1, y1);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) nxt[k] = (z < N - 1 || q[5] >= 0) ? u1[k] : bc * cur[k];
     T acc[4];
-    star_slice(u, xu, q, lane, c, g, z, bc, ihx, ihy, ihz, cur, prv, nxt, acc);
+    star_slice_f(q, lane, c, g, bc, ihx, ihy, ihz, x0, y0, cur, prv, nxt, acc);
+    T fv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) fv[k] = f0[k];
     T r[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) r[k] = fv[k] - acc[k];
@@ -562,7 +576,11 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
     for (int k = 0; k < 4; k++) {
       prv[k] = cur[k];
       cur[k] = nxt[k];
+      u1[k] = u2[k];
+      f0[k] = f1[k];
+      y0[k] = y1[k];
     }
+    x0 = x1;
   }
 }
 
@@ -959,12 +977,29 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
     else
 #pragma unroll
       for (int k = 0; k < 4; k++) prv[k] = bc * cur[k];
+    // loads one slice ahead of their use (as in c3_star_dot_kernel): u1 slice z + 1 of x (at
+    // z = N - 1 the neighbour's slice 0), u2 the one after, b0 / b1 and the faces xf0, yf0 /
+    // xf1, yf1 of slices z and z + 1 (lanes g = 0, 3: the facing x-face cell; c = 0, N - 1:
+    // the facing y-face row)
+    auto faces = [&](int z, double& xf, double (&yf)[4]) {
+      if (g == 0 && q[0] >= 0) xf = xp[pidx(q[0], N - 1, c, z)];
+      if (g == 3 && q[1] >= 0) xf = xp[pidx(q[1], 0, c, z)];
+      if (c == 0 && q[2] >= 0) prow(xp, q[2], N - 1, z, yf);
+      if (c == N - 1 && q[3] >= 0) prow(xp, q[3], 0, z, yf);
+    };
+    double u1[4], u2[4], b0[4], b1[4], xf0 = 0.0, xf1 = 0.0, yf0[4], yf1[4];
+    prow(xp, p, c, 1, u1);
+    prow(a.b, p, c, 0, b0);
+    faces(0, xf0, yf0);
     for (int z = 0; z < N; z++) {
-      if (z < N - 1) prow(xp, p, c, z + 1, nxt);
-      else if (q[5] >= 0) prow(xp, q[5], c, 0, nxt);
-      else
+      if (z + 2 < N) prow(xp, p, c, z + 2, u2);
+      else if (z + 2 == N && q[5] >= 0) prow(xp, q[5], c, 0, u2);
+      if (z + 1 < N) {
+        prow(a.b, p, c, z + 1, b1);
+        faces(z + 1, xf1, yf1);
+      }
 #pragma unroll
-        for (int k = 0; k < 4; k++) nxt[k] = bc * cur[k];
+      for (int k = 0; k < 4; k++) nxt[k] = (z < N - 1 || q[5] >= 0) ? u1[k] : bc * cur[k];
       // the operator on slice z (star_slice with the faces from the padded neighbours)
       double xm = __shfl(cur[3], (lane - 16) & 63, 64);
       double xq = __shfl(cur[0], (lane + 16) & 63, 64);
@@ -974,22 +1009,19 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
         ym[k] = __shfl(cur[k], (lane - 1) & 63, 64);
         yp[k] = __shfl(cur[k], (lane + 1) & 63, 64);
       }
-      if (g == 0) xm = q[0] >= 0 ? xp[pidx(q[0], N - 1, c, z)] : bc * cur[0];
-      if (g == 3) xq = q[1] >= 0 ? xp[pidx(q[1], 0, c, z)] : bc * cur[3];
+      if (g == 0) xm = q[0] >= 0 ? xf0 : bc * cur[0];
+      if (g == 3) xq = q[1] >= 0 ? xf0 : bc * cur[3];
       if (c == 0) {
-        if (q[2] >= 0) prow(xp, q[2], N - 1, z, ym);
-        else
 #pragma unroll
-          for (int k = 0; k < 4; k++) ym[k] = bc * cur[k];
+        for (int k = 0; k < 4; k++) ym[k] = q[2] >= 0 ? yf0[k] : bc * cur[k];
       }
       if (c == N - 1) {
-        if (q[3] >= 0) prow(xp, q[3], 0, z, yp);
-        else
 #pragma unroll
-          for (int k = 0; k < 4; k++) yp[k] = bc * cur[k];
+        for (int k = 0; k < 4; k++) yp[k] = q[3] >= 0 ? yf0[k] : bc * cur[k];
       }
       double bv[4];
-      prow(a.b, p, c, z, bv);
+#pragma unroll
+      for (int k = 0; k < 4; k++) bv[k] = b0[k];
       double rv[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -1016,7 +1048,11 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
       for (int k = 0; k < 4; k++) {
         prv[k] = cur[k];
         cur[k] = nxt[k];
+        u1[k] = u2[k];
+        b0[k] = b1[k];
+        yf0[k] = yf1[k];
       }
+      xf0 = xf1;
     }
   }
   const double ra = block_reduce<RED_SUM>(sa);
