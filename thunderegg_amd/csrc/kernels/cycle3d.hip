@@ -803,7 +803,7 @@ struct StarWalk {
 // xmp, xms: the face columns of M p and M s (or nullptr).
 // R32 (x / r update): rhat from a.rh32, its FP32 copy, widened as read.
 template <int MODE, typename TM, bool APF, bool R32>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TM) == 4 ? 4 : 1, 8))) c3_upd_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TM) == 4 ? (MODE == BICG_XR ? 3 : 4) : 1, 8))) c3_upd_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
                                                      const TM* __restrict__ xmp, const TM* __restrict__ xms, int np,
                                                      int nslots, int neumann)
 {
@@ -847,38 +847,75 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     const double ihx = P.inv_h2[0], ihy = P.inv_h2[1], ihz = P.inv_h2[2];
     constexpr bool WP = APF && MODE != BICG_XR;
     if (WP) wp.init(p, q);
-    if (MODE == BICG_XR) wm.init(p, q);
-    // the vectors of slice z (c*): S: r (, ap); XR: M p, x, s, rhat; P: p, r (, ap).  S and P
-    // load them one slice ahead (n*); XR, with two stencil windows already, at the slice
-    // (ahead, its registers would cost a wave per SIMD)
-    constexpr bool PF = MODE != BICG_XR;
+    if constexpr (MODE == BICG_XR) {
+      // x / r: M p, x, s, rhat of slice z + 1 loaded (in their stored types) while slice z
+      // is formed, widened at their use (XPF; FP64 M p and M s: loaded at the slice, the
+      // registers of a second set would cost a wave per SIMD)
+      constexpr bool XPF = sizeof(TM) == 4;
+      typedef typename std::conditional<R32, float, double>::type TH;
+      wm.init(p, q);
+      TM m0[4], m1[4];
+      TH h0[4], h1[4];
+      double x0[4], x1[4], s0[4], s1[4];
+      auto ldx = [&](int z, TM(&m)[4], double(&xx)[4], double(&ss)[4], TH(&hh)[4]) {
+        const int o = p * NC + 256 * z + ro;
+        ld4<true>((const TM*)a.mp, o, m);
+        ld4(a.x, o, xx);
+        ld4(a.sv, o, ss);
+        if constexpr (R32) ld4<true>(a.rh32, o, hh);
+        else ld4(a.rhat, o, hh);
+      };
+      if (XPF) ldx(0, m0, x0, s0, h0);
+      for (int z = 0; z < N; z++) {
+        const int o = p * NC + 256 * z + ro;
+        if (XPF && z + 1 < N) ldx(z + 1, m1, x1, s1, h1);
+        double asv[4], xv[4], rv[4];
+        wm.step(z, q, ihx, ihy, ihz, asv);
+        if (!XPF) ldx(z, m0, x0, s0, h0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          xv[k] = x0[k];
+          xv[k] += double(m0[k]) * alpha + wm.cur[k] * omega;
+          double r = s0[k];
+          r += asv[k] * -omega;
+          rv[k] = r;
+          sa += r * double(h0[k]);
+          sb += r * r;
+        }
+        st4(a.x, o, xv);
+        st4(a.r, o, rv);
+        wm.next();
+        if (XPF) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            m0[k] = m1[k];
+            h0[k] = h1[k];
+            x0[k] = x1[k];
+            s0[k] = s1[k];
+          }
+        }
+      }
+      continue;
+    }
+    // S and P: the vectors of slice z (c*: S r (, ap); P p, r (, ap)), loaded one slice ahead (n*)
     double c0[4], c1[4], c2[4], c3[4], n0[4], n1[4], n2[4], n3[4];
     auto load = [&](int z, double(&s0)[4], double(&s1)[4], double(&s2)[4], double(&s3)[4]) {
       const int o = p * NC + 256 * z + ro;
-      if (MODE == BICG_XR) {
-        ld4<true>((const TM*)a.mp, o, s0);
-        ld4(a.x, o, s1);
-        ld4(a.sv, o, s2);
-        if (R32) ld4<true>(a.rh32, o, s3);
-        else ld4(a.rhat, o, s3);
-      } else {
-        ld4(MODE == BICG_S ? a.r : a.p, o, s0);
-        if (MODE == BICG_P) ld4(a.r, o, s1);
-        if (!WP) ld4(a.ap, o, s3);
-      }
+      ld4(MODE == BICG_S ? a.r : a.p, o, s0);
+      if (MODE == BICG_P) ld4(a.r, o, s1);
+      if (!WP) ld4(a.ap, o, s3);
+      (void)s2;
     };
-    if (PF) load(0, c0, c1, c2, c3);
+    load(0, c0, c1, c2, c3);
     for (int z = 0; z < N; z++) {
       const int o = p * NC + 256 * z + ro;
-      if (PF && z < N - 1) load(z + 1, n0, n1, n2, n3);
+      if (z < N - 1) load(z + 1, n0, n1, n2, n3);
       double apv[4];
-      if (MODE != BICG_XR) {
-        if (WP) {
-          wp.step(z, q, ihx, ihy, ihz, apv);
-        } else {
+      if (WP) {
+        wp.step(z, q, ihx, ihy, ihz, apv);
+      } else {
 #pragma unroll
-          for (int k = 0; k < 4; k++) apv[k] = c3[k];
-        }
+        for (int k = 0; k < 4; k++) apv[k] = c3[k];
       }
       if (MODE == BICG_S) {  // c0: r
         double sv[4];
@@ -891,23 +928,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         }
         st4(a.sv, o, sv);
         if (a.w32) st4<true>(a.w32, o, sv);
-      } else if (MODE == BICG_XR) {  // c0: M p, c1: x, c2: s, c3: rhat
-        double asv[4], xv[4], rv[4];
-        wm.step(z, q, ihx, ihy, ihz, asv);
-        load(z, c0, c1, c2, c3);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          xv[k] = c1[k];
-          xv[k] += c0[k] * alpha + wm.cur[k] * omega;
-          double r = c2[k];
-          r += asv[k] * -omega;
-          rv[k] = r;
-          sa += r * c3[k];
-          sb += r * r;
-        }
-        st4(a.x, o, xv);
-        st4(a.r, o, rv);
-        wm.next();
       } else {  // BICG_P, c0: p, c1: r
         double pv[4];
 #pragma unroll
@@ -920,14 +940,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         if (a.w32) st4<true>(a.w32, o, pv);
       }
       if (WP) wp.next();
-      if (PF) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          c0[k] = n0[k];
-          c1[k] = n1[k];
-          c2[k] = n2[k];
-          c3[k] = n3[k];
-        }
+      for (int k = 0; k < 4; k++) {
+        c0[k] = n0[k];
+        c1[k] = n1[k];
+        c2[k] = n2[k];
+        c3[k] = n3[k];
       }
     }
   }
