@@ -507,7 +507,7 @@ __device__ __forceinline__ void z_ghost(const TU* __restrict__ u, int qz, int zs
 }
 
 // Cycle::restrict: fc = restriction of f - A u (one wavefront per fine patch)
-template <typename T, typename TF, typename TC>
+template <typename T, typename TF, typename TC, int ZS>
 __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* __restrict__ dp,
                                                                 const T* __restrict__ u, const TF* __restrict__ f,
                                                                 TC* __restrict__ fc,
@@ -517,7 +517,10 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
 {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
   const int ro = cidx(4 * g, c, 0);
-  const int p = blockIdx.x * 4 + wv;
+  // a wavefront per ZS slices z0 .. z0 + ZS - 1 of a patch (ZS < N on small levels, where the
+  // launch is as long as one wave's chain of slice loads)
+  const int unit = blockIdx.x * 4 + wv;
+  const int p = unit / (N / ZS), z0 = (unit % (N / ZS)) * ZS;
   if (p >= np) return;
   const DevPatch& P = dp[p];
   int q[6];
@@ -529,20 +532,21 @@ __global__ void __launch_bounds__(256) c3_resid_restrict_kernel(const DevPatch* 
   const int cp = parent[p], o = porth[p];
   const T wgt = T(1.0 / 8);
   T cur[4], prv[4], nxt[4];
-  ld4(u, base + ro, cur);
-  z_ghost(u, q[4], N - 1, ro, bc, cur, prv);
+  ld4(u, base + 256 * z0 + ro, cur);
+  if (z0 == 0) z_ghost(u, q[4], N - 1, ro, bc, cur, prv);
+  else ld4(u, base + 256 * (z0 - 1) + ro, prv);
   // loads one slice ahead of their use (as in c3_star_dot_kernel): u1 slice z + 1 (at
   // z = N - 1 the neighbour's slice 0), u2 the one after, f0 / f1 and the faces x0, y0 /
   // x1, y1 of slices z and z + 1
   T u1[4], u2[4], f0[4], f1[4], x0 = T(0), x1 = T(0), y0[4], y1[4];
-  ld4(u, base + 256 + ro, u1);
-  ld4<true>(f, base + ro, f0);
-  star_faces(u, xu, q, c, g, 0, x0, y0);
+  ld4(u, base + 256 * (z0 + 1) + ro, u1);
+  ld4<true>(f, base + 256 * z0 + ro, f0);
+  star_faces(u, xu, q, c, g, z0, x0, y0);
   T hold0 = T(0), hold1 = T(0);
-  for (int z = 0; z < N; z++) {
-    if (z + 2 < N) ld4(u, base + 256 * (z + 2) + ro, u2);
+  for (int z = z0; z < z0 + ZS; z++) {
+    if (z + 2 < N && z + 2 <= z0 + ZS) ld4(u, base + 256 * (z + 2) + ro, u2);
     else if (z + 2 == N && q[5] >= 0) ld4(u, q[5] * NC + ro, u2);
-    if (z + 1 < N) {
+    if (z + 1 < z0 + ZS) {
       ld4<true>(f, base + 256 * (z + 1) + ro, f1);
       star_faces(u, xu, q, c, g, z + 1, x1, y1);
     }
@@ -1251,11 +1255,18 @@ int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int 
                              const void* f, const void* u, int dtc, void* fc, hipStream_t s, const void* xu)
 {
   if (d->np == 0) return TE_OK;
-  const int grid = (d->np + 3) / 4;
-#define TE_C3R(T_, TF_, TC_)                                                                                     \
-  hipLaunchKernelGGL((c3_resid_restrict_kernel<T_, TF_, TC_>), dim3(grid), dim3(256), 0, s, d->d_patches,       \
-                     (const T_*)u, (const TF_*)f, (TC_*)fc, link->d_parent, link->d_orth, d->np, neumann, \
+  // levels of fewer than kC3InterpMin patches: four waves per patch, four slices each
+  const bool split = d->np < kC3InterpMin;
+  const int grid = split ? d->np : (d->np + 3) / 4;
+#define TE_C3RZ(T_, TF_, TC_, ZS_)                                                                               \
+  hipLaunchKernelGGL((c3_resid_restrict_kernel<T_, TF_, TC_, ZS_>), dim3(grid), dim3(256), 0, s, d->d_patches,  \
+                     (const T_*)u, (const TF_*)f, (TC_*)fc, link->d_parent, link->d_orth, d->np, neumann,    \
                      (const T_*)xu)
+#define TE_C3R(T_, TF_, TC_)               \
+  do {                                     \
+    if (split) TE_C3RZ(T_, TF_, TC_, 4);   \
+    else TE_C3RZ(T_, TF_, TC_, N);         \
+  } while (0)
   if (dt == TE_F64 && dtf == TE_F64 && dtc == TE_F64) TE_C3R(double, double, double);
   else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F32) TE_C3R(float, float, float);
   else if (dt == TE_F32 && dtf == TE_F32 && dtc == TE_F64) TE_C3R(float, float, double);
@@ -1266,6 +1277,7 @@ int launch_c3_resid_restrict(const te_domain* d, const te_level_link* link, int 
     return TE_ERR_UNSUPPORTED;
   }
 #undef TE_C3R
+#undef TE_C3RZ
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
