@@ -1184,7 +1184,9 @@ int launch_c3_sweep(const te_dft_plan* pl, int dt, int dtf, const void* f, const
   const int32_t* por = link ? link->d_orth : nullptr;
   // slices per load batch of modes 1 / 2: the autotuner's choice, else 2 (FP32) / 1 (FP64)
   const int var = tune_variant(TUNE_C3_SWEEP, 3, N, dt);
-  const int sb = c3_sweep_batch(var, dt);
+  // levels below kC3InterpMin patches fill a fraction of the chip, and a sweep's time is its
+  // wavefronts' chain of loads: there all four slices' loads go in one batch
+  const int sb = d->np < kC3InterpMin ? 4 : c3_sweep_batch(var, dt);
   const int wpe = c3_sweep_waves(var);  // with face columns only (the cycle's own path)
   const bool fa = c3_sweep_f_ahead(var);
   // face columns: written when out has them; then uin (modes 1 and 2) must have them too
