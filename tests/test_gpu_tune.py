@@ -78,11 +78,13 @@ def test_mfma_variants_in_the_fused_cycle(n, clean_table):
 @pytest.mark.parametrize("prec", [R.F32, R.F64])
 def test_c3_sweep_variants_identical(prec, clean_table):
     """the 3D compact cycle's sweeps (modes 1 and 2) under every load-batch variant give the
-    hand default's bits"""
+    hand default's bits; 4096 finest patches (smaller levels issue all their loads in one batch
+    whatever the variant)"""
     from thunderegg_amd import solver
 
     _print_case("tune c3", prec)
-    levels = uniform_levels(3, 2, (16, 16, 16))
+    levels = uniform_levels(3, 4, (16, 16, 16))
+    assert len(levels[0]) == 4096
     rng = np.random.default_rng(11)
     f = rng.standard_normal((len(levels[0]), 1, 18, 18, 18))
     res = []
