@@ -920,7 +920,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   const bool apf = g->xrfly && use_precond;
   // mixed precision: rhat as an FP32 copy of r0.  BiCGStab uses rhat only in the inner
   // products rhat.ap and rhat.r, both formed from the same stored (widened) values, so this
-  // is the same iteration with the shadow residual fl32(r0) in place of r0 (BiCGStab.h:132-133
+  // is the same iteration with the shadow residual fl32(r0) in place of r0 (BiCGStab.h:126,132
   // sets rhat = r0; any fixed rhat with rhat.r0 != 0 serves); the recurrences stay FP64.
   const bool c3_r32 = c3_32 && g->xrfly && g->rhat32 && g->c_rhat32;
   auto iteration_c3 = [&]() -> int {
