@@ -144,9 +144,12 @@ def kernel_rooflines_3d(g, P, its, reps=20):
       operator        M s (FP32), s (FP64)          4N + 8N (+ 8N: A M s stored, TE_NO_XRFLY=1)
       operator_p      M p (FP32), rhat (FP32)       4N + 4N (TE_NO_RHAT32=1: rhat FP64, + 4N;
                                                     TE_NO_XRFLY=1: that, and A M p stored, + 8N)
-      update_xr       x read + written, r written; M p, M s (FP32); s (FP64), rhat (FP32)
-                      8 2 + 8 + 4 2 + 8 + 4 per cell (rhat FP64: + 4; TE_NO_XRFLY=1: that, and
-                      r read, A M p and A M s instead of s: + 8 + 8)
+      update_xr       r written; M s (FP32); s (FP64), rhat (FP32): 8 + 4 + 8 + 4 per cell, x
+                      deferred to x_combine (TE_NO_XDEFER=1: x read + written and M p read,
+                      8 2 + 8 + 4 2 + 8 + 4; rhat FP64: + 4; TE_NO_XRFLY=1: that, and r read,
+                      A M p and A M s instead of s: + 8 + 8)
+      x_combine       x read + written, M p and M s of K iterations (FP32): 8 2 + 4 2 K, once
+                      per K iterations (K = 8 slots)
       update_s        r, M p (FP32) -> s, s32        8 + 4 + 8 + 4 (M p -> A M p, TE_NO_XRFLY=1: + 4)
       update_p        p, M p (FP32), r -> p, p32     8 + 4 + 8 + 8 + 4 (likewise + 4)
     Flops of the patch solve: six 16 x 16 x 16 transform products per axis pass, 12 n^4."""
@@ -157,6 +160,8 @@ def kernel_rooflines_3d(g, P, its, reps=20):
     pre, post = P.pre, P.post
     xrfly = os.environ.get("TE_NO_XRFLY") != "1"  # the updates form A M p and A M s themselves
     rh = 8 if (not xrfly or os.environ.get("TE_NO_RHAT32") == "1") else 4  # bytes per rhat value
+    xdefer = rh == 4 and os.environ.get("TE_NO_XDEFER") != "1"  # x updated by x_combine
+    ks = max(1, min(8, int(os.environ.get("TE_XDEFER_SLOTS", "8"))))
     specs = {
         # name: (bytes, flops, launches per solve)
         "sweep_zero": (np_ * s * 2 * N, fl, 2 * its),
@@ -165,10 +170,13 @@ def kernel_rooflines_3d(g, P, its, reps=20):
         "resid_restrict": (np_ * s * (2 * N + N // 8), None, 2 * its),
         "operator": (np_ * N * (4 + 8 + (0 if xrfly else 8)), None, its),
         "operator_p": (np_ * N * (4 + rh + (0 if xrfly else 8)), None, its),
-        "update_xr": (np_ * N * (8 * 2 + 8 + 4 * 2 + 8 + rh + (0 if xrfly else 16)), None, its),
+        "update_xr": (np_ * N * ((8 + 4 + 8 + rh) if xdefer else (8 * 2 + 8 + 4 * 2 + 8 + rh + (0 if xrfly else 16))),
+                      None, its),
         "update_s": (np_ * N * (8 + 4 + 8 + 4 + (0 if xrfly else 4)), None, its),
         "update_p": (np_ * N * (8 + 4 + 8 + 8 + 4 + (0 if xrfly else 4)), None, its),
     }
+    if xdefer:
+        specs["x_combine"] = (np_ * N * (8 * 2 + 4 * 2 * ks), None, -(-its // ks))
     out = {}
     for name, (byt, flops, per_solve) in specs.items():
         ms = g.time_kernel(name, reps)

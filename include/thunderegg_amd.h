@@ -362,7 +362,9 @@ enum te_kernel_id {
   TE_K_UPDATE_XR = 5,       /* outer x and r updates with the dots r.rhat and r.r */
   TE_K_OPERATOR_P = 6,      /* outer FP64 A (M p) with the dot rhat.ap (stored with TE_NO_XRFLY=1) */
   TE_K_UPDATE_S = 7,        /* outer s = r - alpha A M p (+ its FP32 copy) with the dot s.s */
-  TE_K_UPDATE_P = 8         /* outer p = r + beta (p - omega A M p) (+ its FP32 copy) */
+  TE_K_UPDATE_P = 8,        /* outer p = r + beta (p - omega A M p) (+ its FP32 copy) */
+  TE_K_X_COMBINE = 9        /* mixed precision: the deferred x updates of 8 iterations,
+                               x += alpha_j M p_j + omega_j M s_j (TE_NO_XDEFER=1: none) */
 };
 int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms);
 

@@ -237,6 +237,41 @@ def test_compact3d_fp32_shadow_residual(monkeypatch):
         assert np.sqrt((r * r).sum() / (mesh.interior(b, 3) ** 2).sum()) <= 1.5e-10
 
 
+def test_compact3d_deferred_x_update(monkeypatch):
+    """Mixed precision defers the x update: the x / r update keeps alpha and omega, M p and M s
+    rotate through K slots, and x += alpha M p + omega M s is added slot by slot every K
+    iterations and at the end of the solve.  Against the x update in the x / r update
+    (TE_NO_XDEFER=1): the same iterations and residual history (the r recurrence is
+    untouched) and x to 1e-13, with K = 8, 3 and 1, both early-exit variants, a random
+    initial guess, and an iteration limit that ends the solve inside a group of slots."""
+    _print_case("compact3d deferred x")
+    levels = _levels(3)
+    b = _rhs_sin(levels)
+    od = O.Domain(3, NS, levels[0])
+    x0 = random_field(mesh.padded_shape(len(levels[0]), 3, NS), 5)
+
+    def runs():
+        g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=R.F32, pre_sweeps=2,
+                       post_sweeps=2)
+        return [g.solve(b, tol=1e-10, precond=2), g.solve(b, tol=1e-10, precond=True),
+                g.solve(b, x0=x0, tol=1e-10, precond=2), g.solve(b, tol=1e-10, precond=2, max_it=5)]
+
+    monkeypatch.setenv("TE_NO_XDEFER", "1")
+    ref = runs()
+    monkeypatch.delenv("TE_NO_XDEFER")
+    for k in ("8", "3", "1"):
+        monkeypatch.setenv("TE_XDEFER_SLOTS", k)
+        for j, ((xa, ia, ha), (xb, ib, hb)) in enumerate(zip(runs(), ref)):
+            assert ia == ib, (k, j)
+            assert np.allclose(ha, hb, rtol=1e-13, atol=0), (k, j)
+            assert relerr(mesh.interior(xa, 3), mesh.interior(xb, 3)) <= 1e-13, (k, j)
+            if j in (0, 2):  # precond True's early exit adds alpha p, as the reference's
+                ax = O.poisson_apply(od, xa.copy(), kind=R.TRILINEAR)
+                r = mesh.interior(b - ax, 3)
+                assert np.sqrt((r * r).sum() / (mesh.interior(b, 3) ** 2).sum()) <= 1.5e-10, (k, j)
+    assert ref[3][1] == 5
+
+
 def test_compact3d_bicgstab_initial_guess_matches_oracle(monkeypatch):
     """A nonzero initial guess: the setup forms r = b - A x from the caller's padded b and x
     in one launch (face ghosts from the neighbours' cells of x or the physical condition);

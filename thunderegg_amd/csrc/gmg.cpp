@@ -15,6 +15,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -512,6 +513,9 @@ void free_gmg(te_gmg* g)
                    (void*)g->c_x,
                    g->c_mp, g->c_ms, (void*)g->c_p32, (void*)g->c_s32, (void*)g->c_rhat32, g->c_tmp })
     (void)hipFree(v);
+  for (void* v : g->c_mph) (void)hipFree(v);
+  for (void* v : g->c_msh) (void)hipFree(v);
+  (void)hipFree(g->c_hist);
   if (g->host_scal) (void)hipHostFree(g->host_scal);
   if (g->mirror_h) (void)hipHostFree(g->mirror_h);
   if (g->ev_in) (void)hipEventDestroy(g->ev_in);
@@ -594,6 +598,10 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
     g->xrfly = !(nxr && nxr[0] == '1');
     const char* nr32 = std::getenv("TE_NO_RHAT32");
     g->rhat32 = !(nr32 && nr32[0] == '1');
+    const char* nxd = std::getenv("TE_NO_XDEFER");
+    g->xdefer = !(nxd && nxd[0] == '1');
+    const char* nxs = std::getenv("TE_XDEFER_SLOTS");
+    if (nxs) g->xslots = std::max(1, std::min(kC3XSlots, std::atoi(nxs)));
     const char* nfi = std::getenv("TE_NO_FACEINTERP");
     g->finterp = !(nfi && nfi[0] == '1');
   }
@@ -781,6 +789,20 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
           if (e == hipSuccess) e = hipMalloc(v, sizeof(float) * nc);
           if (e == hipSuccess) e = hipMemset(*v, 0, sizeof(float) * nc);
         }
+        // the deferred x updates' slots 1 .. xslots - 1 (with face columns, as c_mp / c_ms)
+        if (g->xdefer && g->xrfly && g->rhat32) {
+          for (int j = 1; j < g->xslots && e == hipSuccess; j++)
+            for (std::vector<void*>* vv : { &g->c_mph, &g->c_msh }) {
+              void* v = nullptr;
+              if (e == hipSuccess) e = hipMalloc(&v, em * ncx);
+              if (e == hipSuccess) {
+                vv->push_back(v);
+                e = hipMemset(v, 0, em * ncx);
+              }
+            }
+          if (e == hipSuccess) e = hipMalloc(&g->c_hist, sizeof(double) * 2 * kC3XSlots);
+          if (e == hipSuccess) e = hipMemset(g->c_hist, 0, sizeof(double) * 2 * kC3XSlots);
+        }
       }
     }
     if (e == hipSuccess) e = hipMalloc(&g->scal, sizeof(double) * 16);
@@ -923,7 +945,34 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // is the same iteration with the shadow residual fl32(r0) in place of r0 (BiCGStab.h:126,132
   // sets rhat = r0; any fixed rhat with rhat.r0 != 0 serves); the recurrences stay FP64.
   const bool c3_r32 = c3_32 && g->xrfly && g->rhat32 && g->c_rhat32;
-  auto iteration_c3 = [&]() -> int {
+  // mixed precision: the x update deferred.  x is read only at the end of a solve, so the
+  // x / r update leaves it alone and keeps alpha and omega; iteration i writes M p and M s
+  // into slot i % K, and launch_c3_xcomb adds x += alpha M p + omega M s for the slots in
+  // iteration order (each term as the x / r update would add it) at the end of slot K - 1's
+  // iteration and after the last iteration.  The r recurrence is untouched.
+  const bool xdf = c3_r32 && g->xdefer && g->c_hist != nullptr;
+  const int K = xdf ? g->xslots : 1;
+  auto slot_mp = [&](int j) -> void* { return j > 0 ? g->c_mph[j - 1] : c3_mp; };
+  auto slot_ms = [&](int j) -> void* { return j > 0 ? g->c_msh[j - 1] : c3_ms; };
+  auto xcomb = [&](int count, int drop_early) -> int {
+    C3XComb c{};
+    c.x = g->c_x;
+    for (int j = 0; j < count; j++) {
+      c.mp[j] = (const float*)slot_mp(j);
+      c.ms[j] = (const float*)slot_ms(j);
+    }
+    c.hist = g->c_hist;
+    c.scal = g->scal;
+    c.count = count;
+    c.drop_early = drop_early;
+    return launch_c3_xcomb(d, c, s);
+  };
+  auto iteration_c3 = [&](int slot) -> int {
+    // this iteration's M p and M s (slot 0: c3_mp, c3_ms) and their face columns
+    void* smp = xdf ? slot_mp(slot) : c3_mp;
+    void* sms = xdf ? slot_ms(slot) : c3_ms;
+    const void* sxmp = c3_xc ? xcol_of(g, smp, d->np, c3m) : nullptr;
+    const void* sxms = c3_xc ? xcol_of(g, sms, d->np, c3m) : nullptr;
     C3Bicg a{};
     a.scal = g->scal;
     a.x = g->c_x;
@@ -933,27 +982,27 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.sv = g->c_s;
     a.ap = g->c_ap;
     a.as = g->c_as;
-    a.mp = c3_mp;
-    a.ms = c3_ms;
+    a.mp = smp;
+    a.ms = sms;
     a.dtm = c3m;
     if (use_precond)
-      TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_p32 : (const void*)g->c_p, c3_32 ? TE_F32 : TE_F64, c3_mp, c3m));
+      TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_p32 : (const void*)g->c_p, c3_32 ? TE_F32 : TE_F64, smp, c3m));
     // rhat.ap (A M p itself is formed again by the updates that use it, launch_c3_upd, where
     // M p is apart from p; else stored)
-    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_mp, apf ? nullptr : g->c_ap, g->c_rhat, g->pa, nullptr, nb, s,
-                             c3_xmp, c3_r32 ? g->c_rhat32 : nullptr));
+    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, smp, apf ? nullptr : g->c_ap, g->c_rhat, g->pa, nullptr, nb, s,
+                             sxmp, c3_r32 ? g->c_rhat32 : nullptr));
     // s = r - alpha ap, alpha folded from the rhat.ap partials; s.s partials into pc
     a.w32 = c3_32 ? g->c_s32 : nullptr;
     a.pa = g->pc;
     a.pfold = g->pa;
     a.nfold = nb;
-    if (g->xrfly) TE_RC(launch_c3_upd(BICG_S, d, g->o.neumann, a, apf, c3_xmp, nullptr, nb, s));
+    if (g->xrfly) TE_RC(launch_c3_upd(BICG_S, d, g->o.neumann, a, apf, sxmp, nullptr, nb, s));
     else TE_RC(launch_c3_bicg(BICG_S, d, a, nb, s));
     if (use_precond)
-      TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_s32 : (const void*)g->c_s, c3_32 ? TE_F32 : TE_F64, c3_ms, c3m));
+      TE_RC(visit_c3(g, 0, c3_32 ? (const void*)g->c_s32 : (const void*)g->c_s, c3_32 ? TE_F32 : TE_F64, sms, c3m));
     // as.s and as.as (A M s itself is formed again by the x / r update unless xrfly is off)
-    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, c3_ms, g->xrfly ? nullptr : g->c_as, g->c_s, g->pa, g->pb, nb, s,
-                             c3_xms));
+    TE_RC(launch_c3_star_dot(d, g->o.neumann, c3m, sms, g->xrfly ? nullptr : g->c_as, g->c_s, g->pa, g->pb, nb, s,
+                             sxms));
     TE_RC(launch_bicg_finalize(FIN_SOMEGA, g->pa, g->pb, nb, g->scal, s, 0.0, nullptr, g->pc));
     // x and r updates; after the s-norm test fired, x += alpha p (use_precond 2: alpha M p)
     a.w32 = nullptr;
@@ -961,20 +1010,26 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.nfold = 0;
     a.pa = g->pa;
     a.pb = g->pb;
-    a.dir = use_precond == 2 ? (const void*)c3_mp : (const void*)g->c_p;
+    a.dir = use_precond == 2 ? (const void*)smp : (const void*)g->c_p;
     a.dir_m = use_precond == 2 ? 1 : 0;
     a.rh32 = c3_r32 ? g->c_rhat32 : nullptr;
-    if (g->xrfly) TE_RC(launch_c3_upd(BICG_XR, d, g->o.neumann, a, apf, c3_xmp, c3_xms, nb, s));
+    a.hist = xdf ? g->c_hist + 2 * slot : nullptr;
+    if (g->xrfly) TE_RC(launch_c3_upd(BICG_XR, d, g->o.neumann, a, apf, sxmp, sxms, nb, s));
     else TE_RC(launch_c3_bicg(BICG_XR, d, a, nb, s));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
     a.dir = nullptr;
     a.rh32 = nullptr;
+    a.hist = nullptr;
     a.w32 = c3_32 ? g->c_p32 : nullptr;
-    if (g->xrfly) return launch_c3_upd(BICG_P, d, g->o.neumann, a, apf, c3_xmp, nullptr, nb, s);
-    return launch_c3_bicg(BICG_P, d, a, nb, s);
+    if (g->xrfly) TE_RC(launch_c3_upd(BICG_P, d, g->o.neumann, a, apf, sxmp, nullptr, nb, s));
+    else TE_RC(launch_c3_bicg(BICG_P, d, a, nb, s));
+    // the last slot: the deferred x updates of the K iterations (less this one's when its
+    // s-norm test fired: it added alpha dir to x itself)
+    if (xdf && slot == K - 1) return xcomb(K, 1);
+    return TE_OK;
   };
-  auto iteration = [&]() -> int {
-    if (g->c3) return iteration_c3();
+  auto iteration = [&](int slot) -> int {
+    if (g->c3) return iteration_c3(slot);
     if (use_precond) TE_RC(vcycle_enqueue(g, m32 ? (const void*)g->p32 : g->p, mp, dtm, dtm, l0f));
     if (!g->fly) TE_RC(launch_ghost_fill(gf, TE_F64, mp, 1, s));
     TE_RC(launch_star_dot(gf, g->o.op_kind, g->o.neumann, cf, mp, dtm, g->ap, g->rhat, g->pa, nullptr, g->fly, s));
@@ -999,14 +1054,15 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     return launch_bicg(BICG_P, d, g->scal, g->p, nullptr, g->ap, g->resid, nullptr, nullptr, nullptr, nullptr,
                        nullptr, s, TE_F64, m32 ? g->p32 : nullptr);
   };
-  hipGraphExec_t gx = nullptr;
-  if (g->o.use_graph && !g->timer) {
-    const GraphCache::Key key{ (uintptr_t)x, (uintptr_t)use_precond, 1 };
+  hipGraphExec_t gxs[kC3XSlots] = {};  // one iteration graph per x-update slot
+  for (int slot = 0; slot < K && g->o.use_graph && !g->timer; slot++) {
+    const GraphCache::Key key{ (uintptr_t)x, (uintptr_t)use_precond + 16 * (uintptr_t)slot, 1 };
+    hipGraphExec_t& gx = gxs[slot];
     gx = g->bgraphs.find(key);
     if (!gx) {
       hipGraph_t graph;
       TE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int rc = iteration();
+      int rc = iteration(slot);
       hipError_t e = hipStreamEndCapture(s, &graph);
       if (rc) return rc;
       if (e != hipSuccess) {
@@ -1028,6 +1084,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   auto t0 = std::chrono::steady_clock::now();
   int rc = TE_OK;
   int num_its = 0;
+  bool early = false;
   // resid = b - A x ; r0_norm = ||b|| ; rhat = p = resid ; rho = rhat.resid (on the compact
   // path in one launch from the padded b and x, launch_c3_init)
   const bool c3_init = g->c3 && g->xrfly;
@@ -1098,17 +1155,26 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
         break;
       }
       if (time_its) TE_RC(timer_start_stream(g->timer, "Iteration", TE_TIMER_NONE, s));
-      if (gx) TE_HIP(hipGraphLaunch(gx, s));
-      else TE_RC(iteration());
+      const int slot = num_its % K;
+      if (gxs[slot]) TE_HIP(hipGraphLaunch(gxs[slot], s));
+      else TE_RC(iteration(slot));
       if (time_its) TE_RC(timer_stop_stream(g->timer, "Iteration", TE_TIMER_NONE, s));
       seq += 1.0;
       TE_RC(wait_published(seq));
-      if (h[SC_STATE] == ST_EARLY) break;  // x was updated on the device
+      if (h[SC_STATE] == ST_EARLY) {  // x was updated on the device
+        early = true;
+        break;
+      }
       num_its++;
       rho = h[SC_RHO];
       residual = h[SC_RNORM] / r0;
       if (history && num_its < history_cap) history[num_its] = residual;
     }
+  }
+  if (xdf) {  // the deferred x updates not yet added (slot K - 1's iteration adds its group's)
+    int pend = num_its % K;
+    if (early && pend == K - 1) pend = 0;
+    if (pend) TE_RC(xcomb(pend, 0));
   }
   if (g->c3) TE_RC(launch_c3_unpack(d, TE_F64, x, TE_F64, g->c_x, s));  // the compact x back
   TE_HIP(hipStreamSynchronize(s));
@@ -1217,7 +1283,7 @@ double te_gmg_last_solve_seconds(const te_gmg* g) { return g ? g->last_seconds :
 
 int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
 {
-  if (!g || !ms || reps < 1 || which < TE_K_SWEEP_ZERO || which > TE_K_UPDATE_P) {
+  if (!g || !ms || reps < 1 || which < TE_K_SWEEP_ZERO || which > TE_K_X_COMBINE) {
     set_error("te_gmg_time_kernel: bad argument");
     return TE_ERR_ARG;
   }
@@ -1264,6 +1330,22 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
                                   nullptr, nb, s, xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr,
                                   r32 ? g->c_rhat32 : nullptr);
       }
+      case TE_K_X_COMBINE: {  // the deferred x updates of xslots iterations
+        if (!g->c_hist) {
+          set_error("te_gmg_time_kernel: this solver does not defer the x update");
+          return TE_ERR_UNSUPPORTED;
+        }
+        C3XComb c{};
+        c.x = g->c_x;
+        for (int j = 0; j < g->xslots; j++) {
+          c.mp[j] = (const float*)(j ? g->c_mph[j - 1] : g->c_mp);
+          c.ms[j] = (const float*)(j ? g->c_msh[j - 1] : g->c_ms);
+        }
+        c.hist = g->c_hist;
+        c.scal = g->scal;
+        c.count = g->xslots;
+        return launch_c3_xcomb(L0.d, c, s);
+      }
       case TE_K_OPERATOR: {
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
         return launch_c3_star_dot(L0.d, g->o.neumann, L0.dt, g->c_ms, g->xrfly ? nullptr : g->c_as, g->c_s, g->pa,
@@ -1297,6 +1379,7 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
         }
         if (!g->xrfly) return launch_c3_bicg(mode, L0.d, a, nb, s);
         if (mode == BICG_XR && L0.dt == TE_F32 && g->rhat32) a.rh32 = g->c_rhat32;
+        if (mode == BICG_XR && a.rh32 && g->c_hist) a.hist = g->c_hist;  // x deferred, as the solve runs it
         const bool xc = g->L.size() == 1 ? g->o.coarse_sweeps > 0 : g->o.post_sweeps > 0;
         return launch_c3_upd(mode, L0.d, g->o.neumann, a, true, xc ? xcol_of(g, g->c_mp, np, L0.dt) : nullptr,
                              xc ? xcol_of(g, g->c_ms, np, L0.dt) : nullptr, nb, s);

@@ -38,7 +38,7 @@ inline size_t esize(int dt) { return dt == TE_F64 ? 8 : 4; }
 // first; at most kCap are kept (callers that pass fresh buffers every call would otherwise
 // accumulate one per call).
 struct GraphCache {
-  static constexpr size_t kCap = 4;
+  static constexpr size_t kCap = 24;  // a solve's iteration graphs: one per x-update slot
   using Key = std::array<uintptr_t, 3>;
   std::vector<std::pair<Key, hipGraphExec_t>> e;
   hipGraphExec_t find(const Key& k)
@@ -126,6 +126,15 @@ struct te_gmg {
   // mixed precision on the compact path (FP32 cycle, xrfly): BiCGStab's shadow residual rhat
   // is kept as an FP32 copy of r0 (c_rhat32); TE_NO_RHAT32=1 keeps it in FP64
   bool rhat32 = true;
+  // mixed precision on the compact path (with rhat32): the x / r update leaves x alone and
+  // keeps alpha and omega; M p and M s of kC3XSlots iterations rotate through slots (slot 0:
+  // c_mp / c_ms, the others c_mph / c_msh), and launch_c3_xcomb adds the slots' terms to x
+  // every xslots iterations and at the end of a solve.  TE_NO_XDEFER=1: x updated in the
+  // x / r update; TE_XDEFER_SLOTS=k (1..8): k slots
+  bool xdefer = true;
+  int xslots = 8;
+  std::vector<void*> c_mph, c_msh;
+  double* c_hist = nullptr;  // alpha, omega per slot
   struct C3Level {
     void* f = nullptr;
     void* u = nullptr;

@@ -87,6 +87,8 @@ struct C3Bicg {
   float* rh32;       // FP32 rhat in place of rhat, or null: INIT (launch_c3_init) writes it and
                      // the partial sums rhat.r into pr; XR (launch_c3_upd, FP32 M p and M s) reads it
   double* pr;
+  double* hist;      // XR (launch_c3_upd, with rh32): x is not updated; alpha and omega go to
+                     // hist[0], hist[1] for launch_c3_xcomb (null: x updated in the launch)
 };
 int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hipStream_t s);
 // BICG_INIT from the caller's padded b and x alone: r = b - A x formed in the launch (the face
@@ -103,6 +105,21 @@ int launch_c3_init(const te_domain* d, int neumann, const C3Bicg& a, int nslots,
 // launch_c3_bicg.
 int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bool apf, const void* xmp,
                   const void* xms, int nslots, hipStream_t s);
+// The deferred x updates: x += alpha_j M p_j + omega_j M s_j for j < count, in that order,
+// each term as the x / r update adds it (alpha_j, omega_j = hist[2 j], hist[2 j + 1]; M p_j,
+// M s_j FP32 compact).  drop_early: the last slot is left out when the state word in scal
+// says the s-norm test fired (that iteration added alpha dir to x itself).
+constexpr int kC3XSlots = 8;
+struct C3XComb {
+  double* x;
+  const float* mp[kC3XSlots];
+  const float* ms[kC3XSlots];
+  const double* hist;
+  const double* scal;
+  int count;
+  int drop_early;
+};
+int launch_c3_xcomb(const te_domain* d, const C3XComb& c, hipStream_t s);
 // compact <-> padded copies with element-type conversion (interior cells; the padded
 // vector's ghost cells are not written)
 int launch_c3_pack(const te_domain* d, int dtd, void* dst, int dts, const void* src_padded, hipStream_t s);

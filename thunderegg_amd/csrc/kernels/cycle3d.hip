@@ -24,6 +24,7 @@
 // wavefront walks the 16 slices keeping slices z - 1, z, z + 1 in registers; x and y
 // neighbours come from other lanes (shuffles), the patch's face ghosts from the same-level
 // neighbour's boundary slice or the physical boundary condition.
+#include <algorithm>
 #include <type_traits>
 
 #include "../te_internal.h"
@@ -806,8 +807,8 @@ struct StarWalk {
 // iteration); else read from a.ap.  One wavefront per patch, persistent over the patches;
 // xmp, xms: the face columns of M p and M s (or nullptr).
 // R32 (x / r update): rhat from a.rh32, its FP32 copy, widened as read.
-template <int MODE, typename TM, bool APF, bool R32>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TM) == 4 ? (MODE == BICG_XR ? 3 : 4) : 1, 8))) c3_upd_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
+template <int MODE, typename TM, bool APF, bool R32, bool XD>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TM) == 4 ? (MODE == BICG_XR && !XD ? 3 : 4) : 1, 8))) c3_upd_kernel(const DevPatch* __restrict__ dp, C3Bicg a,
                                                      const TM* __restrict__ xmp, const TM* __restrict__ xms, int np,
                                                      int nslots, int neumann)
 {
@@ -839,6 +840,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     alpha = rho / r[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) scal[SC_ALPHA] = alpha;
   }
+  // XD: the x update deferred (launch_c3_xcomb adds alpha M p + omega M s later): this
+  // iteration's alpha and omega into its history slot
+  if (XD && blockIdx.x == 0 && threadIdx.x == 0) {
+    a.hist[0] = alpha;
+    a.hist[1] = omega;
+  }
   const double bc = neumann ? 1.0 : -1.0;
   StarWalk<TM> wp{ (const TM*)a.mp, xmp, 0, ro, lane, c, g, bc };
   StarWalk<TM> wm{ (const TM*)a.ms, xms, 0, ro, lane, c, g, bc };
@@ -854,7 +861,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     if constexpr (MODE == BICG_XR) {
       // x / r: M p, x, s, rhat of slice z + 1 loaded (in their stored types) while slice z
       // is formed, widened at their use (XPF; FP64 M p and M s: loaded at the slice, the
-      // registers of a second set would cost a wave per SIMD)
+      // registers of a second set would cost a wave per SIMD).  XD: r only (x, M p unread)
       constexpr bool XPF = sizeof(TM) == 4;
       typedef typename std::conditional<R32, float, double>::type TH;
       wm.init(p, q);
@@ -863,8 +870,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
       double x0[4], x1[4], s0[4], s1[4];
       auto ldx = [&](int z, TM(&m)[4], double(&xx)[4], double(&ss)[4], TH(&hh)[4]) {
         const int o = p * NC + 256 * z + ro;
-        ld4<true>((const TM*)a.mp, o, m);
-        ld4(a.x, o, xx);
+        if (!XD) {
+          ld4<true>((const TM*)a.mp, o, m);
+          ld4(a.x, o, xx);
+        }
         ld4(a.sv, o, ss);
         if constexpr (R32) ld4<true>(a.rh32, o, hh);
         else ld4(a.rhat, o, hh);
@@ -878,15 +887,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         if (!XPF) ldx(z, m0, x0, s0, h0);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          xv[k] = x0[k];
-          xv[k] += double(m0[k]) * alpha + wm.cur[k] * omega;
+          if (!XD) {
+            xv[k] = x0[k];
+            xv[k] += double(m0[k]) * alpha + wm.cur[k] * omega;
+          }
           double r = s0[k];
           r += asv[k] * -omega;
           rv[k] = r;
           sa += r * double(h0[k]);
           sb += r * r;
         }
-        st4(a.x, o, xv);
+        if (!XD) st4(a.x, o, xv);
         st4(a.r, o, rv);
         wm.next();
         if (XPF) {
@@ -1126,6 +1137,27 @@ __global__ void __launch_bounds__(256) c3_face_interp_kernel(const T* __restrict
   }
 }
 
+// launch_c3_xcomb: four cells per thread, grid-stride
+__global__ void __launch_bounds__(256) c3_xcomb_kernel(C3XComb c, long n4)
+{
+  int cnt = c.count;
+  if (c.drop_early && c.scal[SC_STATE] == ST_EARLY) cnt--;
+  for (long i4 = (long)blockIdx.x * 256 + threadIdx.x; i4 < n4; i4 += (long)gridDim.x * 256) {
+    const int o = (int)(4 * i4);
+    double xv[4];
+    ld4(c.x, o, xv);
+    for (int j = 0; j < cnt; j++) {
+      double m[4], w[4];
+      ld4<true>(c.mp[j], o, m);
+      ld4<true>(c.ms[j], o, w);
+      const double al = c.hist[2 * j], om = c.hist[2 * j + 1];
+#pragma unroll
+      for (int k = 0; k < 4; k++) xv[k] += m[k] * al + w[k] * om;
+    }
+    st4(c.x, o, xv);
+  }
+}
+
 template <typename TD, typename TS, bool TO_PADDED>
 __global__ void __launch_bounds__(256) c3_copy_kernel(TD* __restrict__ dst, const TS* __restrict__ src, long n)
 {
@@ -1355,10 +1387,10 @@ int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bo
   int grid = (d->np + 3) / 4;
   if (grid > nslots) grid = nslots;
   if (grid < 1) grid = 1;
-#define TE_C3U4(M_, TM_, APF_, R32_)                                                                             \
-  hipLaunchKernelGGL((c3_upd_kernel<M_, TM_, APF_, R32_>), dim3(grid), dim3(256), 0, s, d->d_patches, a,          \
+#define TE_C3U5(M_, TM_, APF_, R32_, XD_)                                                                        \
+  hipLaunchKernelGGL((c3_upd_kernel<M_, TM_, APF_, R32_, XD_>), dim3(grid), dim3(256), 0, s, d->d_patches, a,     \
                      (const TM_*)xmp, (const TM_*)xms, d->np, nslots, neumann)
-#define TE_C3U(M_, TM_, APF_) TE_C3U4(M_, TM_, APF_, false)
+#define TE_C3U(M_, TM_, APF_) TE_C3U5(M_, TM_, APF_, false, false)
 #define TE_C3UM(M_)                                      \
   do {                                                   \
     if (a.dtm == TE_F32) {                               \
@@ -1372,11 +1404,12 @@ int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bo
   if (mode == BICG_S) {
     TE_C3UM(BICG_S);
   } else if (mode == BICG_XR) {  // forms A M s only
-    if (a.rh32 && a.dtm != TE_F32) {
-      set_error("internal: the FP32 rhat goes with FP32 M p and M s");
+    if ((a.rh32 || a.hist) && (a.dtm != TE_F32 || !a.rh32)) {
+      set_error("internal: the FP32 rhat and the deferred x update go with FP32 M p and M s");
       return TE_ERR_ARG;
     }
-    if (a.rh32) TE_C3U4(BICG_XR, float, false, true);
+    if (a.hist) TE_C3U5(BICG_XR, float, false, true, true);
+    else if (a.rh32) TE_C3U5(BICG_XR, float, false, true, false);
     else if (a.dtm == TE_F32) TE_C3U(BICG_XR, float, false);
     else TE_C3U(BICG_XR, double, false);
   } else {
@@ -1384,7 +1417,7 @@ int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bo
   }
 #undef TE_C3UM
 #undef TE_C3U
-#undef TE_C3U4
+#undef TE_C3U5
   TE_HIP(hipGetLastError());
   return TE_OK;
 }
@@ -1441,6 +1474,20 @@ static int copy_t(const te_domain* d, int dtd, void* dst, int dts, const void* s
   else if (dtd == TE_F32 && dts == TE_F64) TE_C3C(float, double);
   else TE_C3C(float, float);
 #undef TE_C3C
+  TE_HIP(hipGetLastError());
+  return TE_OK;
+}
+
+int launch_c3_xcomb(const te_domain* d, const C3XComb& c, hipStream_t s)
+{
+  if (c.count < 0 || c.count > kC3XSlots) {
+    set_error("internal: bad count of deferred x updates");
+    return TE_ERR_ARG;
+  }
+  const long n4 = (long)d->np * NC / 4;
+  if (n4 == 0 || c.count == 0) return TE_OK;
+  const int grid = (int)std::min<long>((n4 + 255) / 256, 8192);
+  hipLaunchKernelGGL(c3_xcomb_kernel, dim3(grid), dim3(256), 0, s, c, n4);
   TE_HIP(hipGetLastError());
   return TE_OK;
 }

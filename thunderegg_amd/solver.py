@@ -193,7 +193,7 @@ class GMG:
 
     # te_kernel_id: the finest level's kernels of the compact 3D cycle and the solver
     KERNELS = {"sweep_zero": 0, "sweep": 1, "sweep_interp": 2, "resid_restrict": 3, "operator": 4, "update_xr": 5,
-               "operator_p": 6, "update_s": 7, "update_p": 8}
+               "operator_p": 6, "update_s": 7, "update_p": 8, "x_combine": 9}
 
     def time_kernel(self, name, reps=20):
         """Average milliseconds of `reps` standalone launches of one kernel on the solver's
