@@ -151,7 +151,8 @@ def kernel_rooflines_3d(g, P, its, reps=20):
       x_combine       x read + written, M p and M s of K iterations (FP32): 8 2 + 4 2 K, once
                       per K iterations (K = 8 slots)
       update_s        r, M p (FP32) -> s, s32        8 + 4 + 8 + 4 (M p -> A M p, TE_NO_XRFLY=1: + 4)
-      update_p        p, M p (FP32), r -> p, p32     8 + 4 + 8 + 8 + 4 (likewise + 4)
+      update_p        p32, M p (FP32), r -> p32      4 + 4 + 8 + 4 (TE_NO_P32ONLY=1: p FP64 read and
+                      written as well, 8 + 4 + 8 + 8 + 4; TE_NO_XRFLY=1: + 4)
     Flops of the patch solve: six 16 x 16 x 16 transform products per axis pass, 12 n^4."""
     np_ = len(P.levels[0])
     N, B, Bc = 4096, 16 ** 3 - 14 ** 3, 8 ** 3 - 6 ** 3
@@ -162,6 +163,7 @@ def kernel_rooflines_3d(g, P, its, reps=20):
     rh = 8 if (not xrfly or os.environ.get("TE_NO_RHAT32") == "1") else 4  # bytes per rhat value
     xdefer = rh == 4 and os.environ.get("TE_NO_XDEFER") != "1"  # x updated by x_combine
     ks = max(1, min(8, int(os.environ.get("TE_XDEFER_SLOTS", "8"))))
+    p32only = rh == 4 and os.environ.get("TE_NO_P32ONLY") != "1"  # p only as its FP32 copy
     specs = {
         # name: (bytes, flops, launches per solve)
         "sweep_zero": (np_ * s * 2 * N, fl, 2 * its),
@@ -173,7 +175,7 @@ def kernel_rooflines_3d(g, P, its, reps=20):
         "update_xr": (np_ * N * ((8 + 4 + 8 + rh) if xdefer else (8 * 2 + 8 + 4 * 2 + 8 + rh + (0 if xrfly else 16))),
                       None, its),
         "update_s": (np_ * N * (8 + 4 + 8 + 4 + (0 if xrfly else 4)), None, its),
-        "update_p": (np_ * N * (8 + 4 + 8 + 8 + 4 + (0 if xrfly else 4)), None, its),
+        "update_p": (np_ * N * ((4 + 4 + 8 + 4) if p32only else (8 + 4 + 8 + 8 + 4 + (0 if xrfly else 4))), None, its),
     }
     if xdefer:
         specs["x_combine"] = (np_ * N * (8 * 2 + 4 * 2 * ks), None, -(-its // ks))

@@ -237,6 +237,33 @@ def test_compact3d_fp32_shadow_residual(monkeypatch):
         assert np.sqrt((r * r).sum() / (mesh.interior(b, 3) ** 2).sum()) <= 1.5e-10
 
 
+def test_compact3d_fp32_direction(monkeypatch):
+    """Mixed precision keeps the direction p only as its FP32 copy (the cycle's input, fl32(p),
+    which the FP64 p was rounded to anyway): the p recurrence then rounds to FP32 each
+    iteration, x, r and s stay FP64.  Against an FP64 p (TE_NO_P32ONLY=1): iteration counts
+    within one, solutions to 1e-8, both at the tolerance in the true residual; zero and
+    random initial guesses, both early-exit variants."""
+    _print_case("compact3d fp32 direction")
+    levels = _levels(3)
+    b = _rhs_sin(levels)
+    od = O.Domain(3, NS, levels[0])
+    x0 = random_field(mesh.padded_shape(len(levels[0]), 3, NS), 9)
+    out = []
+    for nv in ("0", "1"):
+        monkeypatch.setenv("TE_NO_P32ONLY", nv)
+        g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=R.F32, pre_sweeps=2,
+                       post_sweeps=2)
+        out.append([g.solve(b, tol=1e-10, precond=2), g.solve(b, tol=1e-10, precond=True),
+                    g.solve(b, x0=x0, tol=1e-10, precond=2)])
+    for j, ((xa, ia, _), (xb, ib, _)) in enumerate(zip(out[0], out[1])):
+        assert abs(ia - ib) <= 1, j
+        assert relerr(mesh.interior(xa, 3), mesh.interior(xb, 3)) <= 1e-8, j
+    for x, _, _ in (out[0][0], out[0][2], out[1][0], out[1][2]):
+        ax = O.poisson_apply(od, x.copy(), kind=R.TRILINEAR)
+        r = mesh.interior(b - ax, 3)
+        assert np.sqrt((r * r).sum() / (mesh.interior(b, 3) ** 2).sum()) <= 1.5e-10
+
+
 def test_compact3d_deferred_x_update(monkeypatch):
     """Mixed precision defers the x update: the x / r update keeps alpha and omega, M p and M s
     rotate through K slots, and x += alpha M p + omega M s is added slot by slot every K

@@ -598,6 +598,8 @@ int te::gmg_create(const te_domain* const* levels, int num_levels, const te_gmg_
     g->xrfly = !(nxr && nxr[0] == '1');
     const char* nr32 = std::getenv("TE_NO_RHAT32");
     g->rhat32 = !(nr32 && nr32[0] == '1');
+    const char* np32 = std::getenv("TE_NO_P32ONLY");
+    g->p32only = !(np32 && np32[0] == '1');
     const char* nxd = std::getenv("TE_NO_XDEFER");
     g->xdefer = !(nxd && nxd[0] == '1');
     const char* nxs = std::getenv("TE_XDEFER_SLOTS");
@@ -951,6 +953,11 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   // iteration order (each term as the x / r update would add it) at the end of slot K - 1's
   // iteration and after the last iteration.  The r recurrence is untouched.
   const bool xdf = c3_r32 && g->xdefer && g->c_hist != nullptr;
+  // mixed precision: p kept only as its FP32 copy (the cycle's input).  p enters the
+  // iteration only through M p (already applied to fl32(p)) and its own recurrence, which
+  // then rounds to FP32 each iteration: a perturbation of the size the cycle's input
+  // rounding already has; x, r and s stay FP64.
+  const bool p32 = c3_r32 && g->p32only;
   const int K = xdf ? g->xslots : 1;
   auto slot_mp = [&](int j) -> void* { return j > 0 ? g->c_mph[j - 1] : c3_mp; };
   auto slot_ms = [&](int j) -> void* { return j > 0 ? g->c_msh[j - 1] : c3_ms; };
@@ -1010,8 +1017,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.nfold = 0;
     a.pa = g->pa;
     a.pb = g->pb;
-    a.dir = use_precond == 2 ? (const void*)smp : (const void*)g->c_p;
-    a.dir_m = use_precond == 2 ? 1 : 0;
+    a.dir = use_precond == 2 ? (const void*)smp : (p32 ? (const void*)g->c_p32 : (const void*)g->c_p);
+    a.dir_m = use_precond == 2 || p32 ? 1 : 0;
     a.rh32 = c3_r32 ? g->c_rhat32 : nullptr;
     a.hist = xdf ? g->c_hist + 2 * slot : nullptr;
     if (g->xrfly) TE_RC(launch_c3_upd(BICG_XR, d, g->o.neumann, a, apf, sxmp, sxms, nb, s));
@@ -1020,6 +1027,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.dir = nullptr;
     a.rh32 = nullptr;
     a.hist = nullptr;
+    if (p32) a.p = nullptr;  // p from and to c_p32 only
     a.w32 = c3_32 ? g->c_p32 : nullptr;
     if (g->xrfly) TE_RC(launch_c3_upd(BICG_P, d, g->o.neumann, a, apf, sxmp, nullptr, nb, s));
     else TE_RC(launch_c3_bicg(BICG_P, d, a, nb, s));
@@ -1104,6 +1112,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.p = g->c_p;
     a.w32 = c3_32 ? g->c_p32 : nullptr;
     a.rh32 = c3_r32 ? g->c_rhat32 : nullptr;  // c3_r32 implies c3_init
+    if (p32) a.p = nullptr;
     a.pr = g->pc;
     a.pa = g->pa;
     a.pb = g->pb;
@@ -1376,6 +1385,7 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
           mode = BICG_P;
           a.p = g->c_p;
           a.w32 = L0.dt == TE_F32 ? g->c_p32 : nullptr;
+          if (a.w32 && g->xrfly && g->rhat32 && g->p32only) a.p = nullptr;  // as the mixed-precision solve
         }
         if (!g->xrfly) return launch_c3_bicg(mode, L0.d, a, nb, s);
         if (mode == BICG_XR && L0.dt == TE_F32 && g->rhat32) a.rh32 = g->c_rhat32;

@@ -131,6 +131,9 @@ struct te_gmg {
   // c_mp / c_ms, the others c_mph / c_msh), and launch_c3_xcomb adds the slots' terms to x
   // every xslots iterations and at the end of a solve.  TE_NO_XDEFER=1: x updated in the
   // x / r update; TE_XDEFER_SLOTS=k (1..8): k slots
+  // mixed precision on the compact path (with rhat32): the direction p is kept only as its
+  // FP32 copy c_p32, which the cycle reads anyway; TE_NO_P32ONLY=1 keeps an FP64 p as well
+  bool p32only = true;
   bool xdefer = true;
   int xslots = 8;
   std::vector<void*> c_mph, c_msh;

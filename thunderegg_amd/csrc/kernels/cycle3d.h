@@ -61,7 +61,8 @@ int launch_c3_star_dot(const te_domain* d, int neumann, int udt, const void* u, 
 //   BICG_S:    s = r - alpha ap (+ s32), sum s.s (alpha folded from pfold when given)
 //   BICG_XR:   x += alpha mp + omega ms; r -= alpha ap + omega as; sums r.rhat, r.r; after
 //              the s-norm test fired: x += alpha dir (dir of type dtm when dir_m, else FP64)
-//   BICG_P:    p = r + beta (p - omega ap) (+ p32)
+//   BICG_P:    p = r + beta (p - omega ap) (+ p32); launch_c3_upd and launch_c3_init with a.p
+//              null: p kept only as its FP32 copy a.w32 (read and written in place)
 struct C3Bicg {
   double* scal;
   double* x;         // compact FP64 (INIT writes it from xpad, XR updates it)

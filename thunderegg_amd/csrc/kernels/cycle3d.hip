@@ -916,7 +916,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     double c0[4], c1[4], c2[4], c3[4], n0[4], n1[4], n2[4], n3[4];
     auto load = [&](int z, double(&s0)[4], double(&s1)[4], double(&s2)[4], double(&s3)[4]) {
       const int o = p * NC + 256 * z + ro;
-      ld4(MODE == BICG_S ? a.r : a.p, o, s0);
+      if (MODE == BICG_P && !a.p) ld4<true>(a.w32, o, s0);  // p kept only as its FP32 copy
+      else ld4(MODE == BICG_S ? a.r : a.p, o, s0);
       if (MODE == BICG_P) ld4(a.r, o, s1);
       if (!WP) ld4(a.ap, o, s3);
       (void)s2;
@@ -951,7 +952,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
           v += apv[k] * -omega;
           pv[k] = v * beta + c1[k];
         }
-        st4(a.p, o, pv);
+        if (a.p) st4(a.p, o, pv);
         if (a.w32) st4<true>(a.w32, o, pv);
       }
       if (WP) wp.next();
@@ -1075,7 +1076,7 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
       st4(a.r, o, rv);
       if (a.rh32) st4<true>(a.rh32, o, rv);
       else st4(a.rhat, o, rv);
-      st4(a.p, o, rv);
+      if (a.p) st4(a.p, o, rv);
       if (a.w32) st4<true>(a.w32, o, rv);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -1380,7 +1381,7 @@ int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bo
                   const void* xms, int nslots, hipStream_t s)
 {
   if (nslots < 1 || nslots > kFoldMax || (a.pfold && (mode != BICG_S || a.nfold < 1 || a.nfold > kFoldMax)) ||
-      (mode != BICG_S && mode != BICG_XR && mode != BICG_P)) {
+      (mode != BICG_S && mode != BICG_XR && mode != BICG_P) || (mode == BICG_P && !a.p && !a.w32)) {
     set_error("internal: bad arguments for the 3D BiCGStab updates");
     return TE_ERR_ARG;
   }
