@@ -963,7 +963,8 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
   auto slot_ms = [&](int j) -> void* { return j > 0 ? g->c_msh[j - 1] : c3_ms; };
   auto xcomb = [&](int count, int drop_early) -> int {
     C3XComb c{};
-    c.x = g->c_x;
+    c.x = x;  // the caller's padded x: no compact copy of x on this path
+    c.padded = 1;
     for (int j = 0; j < count; j++) {
       c.mp[j] = (const float*)slot_mp(j);
       c.ms[j] = (const float*)slot_ms(j);
@@ -1021,6 +1022,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.dir_m = use_precond == 2 || p32 ? 1 : 0;
     a.rh32 = c3_r32 ? g->c_rhat32 : nullptr;
     a.hist = xdf ? g->c_hist + 2 * slot : nullptr;
+    a.xpo = xdf ? x : nullptr;
     if (g->xrfly) TE_RC(launch_c3_upd(BICG_XR, d, g->o.neumann, a, apf, sxmp, sxms, nb, s));
     else TE_RC(launch_c3_bicg(BICG_XR, d, a, nb, s));
     TE_RC(launch_bicg_finalize(FIN_RHO, g->pa, g->pb, nb, g->scal, s, 0.0, g->mirror_d));
@@ -1113,6 +1115,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     a.w32 = c3_32 ? g->c_p32 : nullptr;
     a.rh32 = c3_r32 ? g->c_rhat32 : nullptr;  // c3_r32 implies c3_init
     if (p32) a.p = nullptr;
+    if (xdf) a.x = nullptr;  // x deferred: the updates go to the caller's x itself
     a.pr = g->pc;
     a.pa = g->pa;
     a.pb = g->pb;
@@ -1185,7 +1188,7 @@ int te_gmg_bicgstab(te_gmg* g, const double* b, double* x, int use_precond, int 
     if (early && pend == K - 1) pend = 0;
     if (pend) TE_RC(xcomb(pend, 0));
   }
-  if (g->c3) TE_RC(launch_c3_unpack(d, TE_F64, x, TE_F64, g->c_x, s));  // the compact x back
+  if (g->c3 && !xdf) TE_RC(launch_c3_unpack(d, TE_F64, x, TE_F64, g->c_x, s));  // the compact x back
   TE_HIP(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
   g->last_seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1345,7 +1348,8 @@ int te_gmg_time_kernel(te_gmg* g, int which, int reps, double* ms)
           return TE_ERR_UNSUPPORTED;
         }
         C3XComb c{};
-        c.x = g->c_x;
+        c.x = g->resid;  // a padded FP64 vector, as the solve's caller x
+        c.padded = 1;
         for (int j = 0; j < g->xslots; j++) {
           c.mp[j] = (const float*)(j ? g->c_mph[j - 1] : g->c_mp);
           c.ms[j] = (const float*)(j ? g->c_msh[j - 1] : g->c_ms);

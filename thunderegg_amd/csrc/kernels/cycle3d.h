@@ -90,6 +90,8 @@ struct C3Bicg {
   double* pr;
   double* hist;      // XR (launch_c3_upd, with rh32): x is not updated; alpha and omega go to
                      // hist[0], hist[1] for launch_c3_xcomb (null: x updated in the launch)
+  double* xpo;       // XR after the s-norm test fired: x += alpha dir into this padded x instead
+                     // of the compact x (the deferred path keeps no compact x; INIT: a.x null)
 };
 int launch_c3_bicg(int mode, const te_domain* d, const C3Bicg& a, int nslots, hipStream_t s);
 // BICG_INIT from the caller's padded b and x alone: r = b - A x formed in the launch (the face
@@ -112,13 +114,14 @@ int launch_c3_upd(int mode, const te_domain* d, int neumann, const C3Bicg& a, bo
 // says the s-norm test fired (that iteration added alpha dir to x itself).
 constexpr int kC3XSlots = 8;
 struct C3XComb {
-  double* x;
+  double* x;  // compact, or the padded layout with padded
   const float* mp[kC3XSlots];
   const float* ms[kC3XSlots];
   const double* hist;
   const double* scal;
   int count;
   int drop_early;
+  int padded;
 };
 int launch_c3_xcomb(const te_domain* d, const C3XComb& c, hipStream_t s);
 // compact <-> padded copies with element-type conversion (interior cells; the padded

@@ -825,10 +825,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         double dv[4], xv[4];
         if (a.dir_m) ld4((const TM*)a.dir, o, dv);
         else ld4((const double*)a.dir, o, dv);
-        ld4(a.x, o, xv);
+        if (a.xpo) {  // x deferred: the caller's padded x (pad_of: the row's 4 cells are contiguous)
+          const int q = pad_of(o);
 #pragma unroll
-        for (int k = 0; k < 4; k++) xv[k] += dv[k] * alpha;
-        st4(a.x, o, xv);
+          for (int k = 0; k < 4; k++) a.xpo[q + k] += dv[k] * alpha;
+        } else {
+          ld4(a.x, o, xv);
+#pragma unroll
+          for (int k = 0; k < 4; k++) xv[k] += dv[k] * alpha;
+          st4(a.x, o, xv);
+        }
       }
     return;
   }
@@ -1072,7 +1078,7 @@ __global__ void __launch_bounds__(256) c3_init_kernel(const DevPatch* __restrict
         if (a.rh32) sr += double(float(rv[k])) * rv[k];  // rhat.r with the FP32 rhat
       }
       const int o = p * NC + cidx(4 * g, c, z);
-      st4(a.x, o, cur);
+      if (a.x) st4(a.x, o, cur);
       st4(a.r, o, rv);
       if (a.rh32) st4<true>(a.rh32, o, rv);
       else st4(a.rhat, o, rv);
@@ -1145,8 +1151,10 @@ __global__ void __launch_bounds__(256) c3_xcomb_kernel(C3XComb c, long n4)
   if (c.drop_early && c.scal[SC_STATE] == ST_EARLY) cnt--;
   for (long i4 = (long)blockIdx.x * 256 + threadIdx.x; i4 < n4; i4 += (long)gridDim.x * 256) {
     const int o = (int)(4 * i4);
+    const int q = c.padded ? pad_of(o) : o;  // a row's 4 cells are contiguous either way
     double xv[4];
-    ld4(c.x, o, xv);
+#pragma unroll
+    for (int k = 0; k < 4; k++) xv[k] = c.x[q + k];
     for (int j = 0; j < cnt; j++) {
       double m[4], w[4];
       ld4<true>(c.mp[j], o, m);
@@ -1155,7 +1163,8 @@ __global__ void __launch_bounds__(256) c3_xcomb_kernel(C3XComb c, long n4)
 #pragma unroll
       for (int k = 0; k < 4; k++) xv[k] += m[k] * al + w[k] * om;
     }
-    st4(c.x, o, xv);
+#pragma unroll
+    for (int k = 0; k < 4; k++) c.x[q + k] = xv[k];
   }
 }
 
