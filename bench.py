@@ -554,8 +554,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        # the compact 3D path keeps the shadow residual rhat (a fixed vector) as an FP32 copy
-        "dtype": "f32 V-cycle / f64 BiCGStab" + (" (f32 shadow residual)" if args.config == 3 else ""),
+        # the compact 3D path keeps the shadow residual rhat (a fixed vector) and the direction p
+        # (the V-cycle's input) as FP32 copies; x, r and s are FP64
+        "dtype": "f32 V-cycle / f64 BiCGStab" + (" (f32 shadow residual and direction)" if args.config == 3 else ""),
         "data": "synthetic (seeded mesh and right-hand side)",
         "config": {
             "workload": P.name,
