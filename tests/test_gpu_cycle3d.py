@@ -299,6 +299,26 @@ def test_compact3d_deferred_x_update(monkeypatch):
     assert ref[3][1] == 5
 
 
+def test_compact3d_mixed_solve_without_graphs(monkeypatch):
+    """The mixed-precision iteration (FP32 rhat and p, deferred x updates) enqueued launch by
+    launch (use_graph=False: each slot's launches directly, the x combines at the last slot
+    and after the loop) gives the bits of the captured graphs (one per slot), with 3 and 8
+    slots and an iteration limit inside a group of slots."""
+    _print_case("compact3d mixed without graphs")
+    levels = _levels(3)
+    b = _rhs_sin(levels)
+    for k in ("3", "8"):
+        monkeypatch.setenv("TE_XDEFER_SLOTS", k)
+        out = []
+        for graph in (True, False):
+            g = solver.GMG(3, NS, levels, filler_kind=R.TRILINEAR, smoother=0, precision=R.F32, pre_sweeps=2,
+                           post_sweeps=2, use_graph=graph)
+            out.append([g.solve(b, tol=1e-10, precond=2), g.solve(b, tol=1e-10, precond=2, max_it=4)])
+        for (xa, ia, ha), (xb, ib, hb) in zip(out[0], out[1]):
+            assert ia == ib and np.array_equal(ha, hb), k
+            assert np.array_equal(mesh.interior(xa, 3), mesh.interior(xb, 3)), k
+
+
 def test_compact3d_bicgstab_initial_guess_matches_oracle(monkeypatch):
     """A nonzero initial guess: the setup forms r = b - A x from the caller's padded b and x
     in one launch (face ghosts from the neighbours' cells of x or the physical condition);
